@@ -1,0 +1,176 @@
+/*
+ * srmg: MI355X-native (gfx950) block-structured AMR geometric multigrid.
+ *
+ * C ABI of libsrmg.so. It replaces the reference's elliptic-solver entry
+ * points, which a Julia caller reaches through ccall with plain pointers:
+ *
+ *   vcycle!(u, b, r, u1, s, tree, conn, geometry, bc, lpldisc;
+ *           lmax=0, ndown=3, ntop=5, nup=3, omega=1.0)     src/multigrid.jl:60-106
+ *   fmg!(u, b, r, u1, s, tree, conn, geometry, bc, lpldisc;
+ *        lmax=0, ndown=3, ntop=5, nup=3, omega=1.25)        src/multigrid.jl:17-51
+ *   resnorm!(r, b, u, s, tree, conn, geometry, bc, lpldisc)  src/multigrid.jl:391-417
+ *   fill_ghost!(u, l, conn, bc)                              src/connectivity.jl:155-159
+ *
+ * The tree and connectivity stay on the host (src/tree.jl, src/connectivity.jl);
+ * srmg_set_tree receives them at every regrid as the reference's isbits records
+ * and rebuilds the device tables. Fields stay resident in HBM between calls.
+ *
+ * Conventions: every function returns 0 on success and a nonzero status on
+ * failure, with a message in srmg_last_error() (the Julia wrapper turns it into
+ * error(...), as the reference's asserts do). The host owns host buffers, the
+ * library owns device buffers; no host pointer is retained after a call.
+ * A context is single-caller and owns one HIP stream; calls are asynchronous
+ * on that stream except where a host result is returned (resnorm, download).
+ */
+#ifndef SRMG_H
+#define SRMG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct srmg_ctx srmg_ctx;
+
+/* field element types */
+enum { SRMG_F64 = 0, SRMG_F32 = 1 };
+/* geometry: CartesianGeometry, CylindricalGeometry{1}, CylindricalGeometry{2}
+ * (src/geometry.jl:14-20) */
+enum { SRMG_CARTESIAN = 0, SRMG_CYL1 = 1, SRMG_CYL2 = 2 };
+/* discretization: LaplacianDiscretization{D,2}, HelmholtzDiscretization{D,2,T}
+ * (src/laplacian.jl:14, src/helmholtz.jl:9-11) */
+enum { SRMG_LAPLACIAN = 0, SRMG_HELMHOLTZ = 1 };
+/* coarsest-level treatment: the reference's ntop Gauss-Seidel sweeps
+ * (src/multigrid.jl:89), or an exact FP64 solve of the level-1 problem
+ * (an extension with no reference counterpart, needed when level 1 is large). */
+enum { SRMG_COARSE_GS = 0, SRMG_COARSE_EXACT = 1 };
+
+typedef struct {
+    int32_t geometry;  /* SRMG_CARTESIAN / SRMG_CYL1 / SRMG_CYL2 */
+    int32_t disc;      /* SRMG_LAPLACIAN / SRMG_HELMHOLTZ */
+    double k2;         /* HelmholtzDiscretization.k2 (ignored for the Laplacian) */
+    int8_t bc[27];     /* GeneralHomogeneousBoundaryConditions{D}.a (src/boundary.jl:12-14),
+                          3^D Int8 table in Julia column-major order */
+    int32_t coarse;    /* SRMG_COARSE_GS / SRMG_COARSE_EXACT */
+} srmg_op;
+
+/* Message for the last failing call on this thread. */
+const char *srmg_last_error(void);
+
+/* Context for D in {2,3} and block side M in {8,16} (ScalarBlockField{D,M,G,T},
+ * src/blockfield.jl:16-74). Uses the current HIP device. */
+int srmg_create(srmg_ctx **ctx, int D, int M);
+int srmg_destroy(srmg_ctx *ctx);
+
+/* Regrid (after connectivity!, src/streamer.jl:548). Records are concatenated
+ * over levels; *_off has nlevels+1 entries (record offsets per level):
+ *   pairs       (coord[D], blk)                      BlockLayer.pairs, src/blocklayer.jl:23
+ *   neighbor    (from, to, face[D])                  Neighbor{D}, src/connectivity.jl:14-20
+ *   boundary    (block, face[D], bnd[D], level)      Boundary{D}, src/connectivity.jl:6-11
+ *   refboundary (fine, coarse, face[D], subblock[D]) RefBoundary{D}, src/connectivity.jl:23-32
+ *   child       (fine, coarse, subblock[D])          Child{D}, src/connectivity.jl:36-42
+ * Block ids are the reference's 1-based ids, at most nblocks. Level 1 must be
+ * complete (populate!, src/tree.jl:31-44). Existing fields are reallocated
+ * (contents lost): upload them again after a regrid. */
+int srmg_set_tree(srmg_ctx *ctx, int nlevels, int64_t nblocks,
+                  const int64_t *pairs, const int64_t *pairs_off,
+                  const int64_t *neighbor, const int64_t *neighbor_off,
+                  const int64_t *boundary, const int64_t *boundary_off,
+                  const int64_t *refboundary, const int64_t *refboundary_off,
+                  const int64_t *child, const int64_t *child_off);
+
+/* Device-resident block field of the given element type; returns its id. */
+int srmg_field_create(srmg_ctx *ctx, int dtype, int *id);
+int srmg_field_destroy(srmg_ctx *ctx, int id);
+
+/* Host <-> device copies in the reference's host layout: column-major
+ * S^D x nblocks with S = M + 2G, block blk at offset (blk-1) S^D
+ * (src/blockfield.jl:33-59). Upload reads valid cells and the depth-1 ghost
+ * layer of every coarse-fine face; download writes valid cells and the depth-1
+ * ghost layer of every face (same-level copy, boundary mirror, coarse-fine
+ * interpolation), leaving deeper ghost layers untouched. host_dtype may differ
+ * from the field's type (converted). */
+int srmg_upload(srmg_ctx *ctx, int id, const void *host, int host_dtype, int G);
+int srmg_download(srmg_ctx *ctx, int id, void *host, int host_dtype, int G);
+/* As srmg_download, and also the depth-1 ghosts of domain-boundary faces,
+ * from the boundary table bc (3^D Int8, as in srmg_op). */
+int srmg_download_bc(srmg_ctx *ctx, int id, void *host, int host_dtype, int G, const int8_t *bc);
+/* Set a field to zero (valid cells and ghost buffers). */
+int srmg_zero(srmg_ctx *ctx, int id);
+
+/* fill_ghost!(u, level, conn, bc) (src/connectivity.jl:155-159). Same-level and
+ * boundary ghosts are always derived from the current valid cells; this call
+ * refreshes the coarse-fine interpolation ghosts of `level` from level-1. */
+int srmg_fill_ghost(srmg_ctx *ctx, int id, int level, const int8_t *bc);
+/* fill_ghost! for l = 1..nlevels, the callers' pre-solve fill
+ * (src/streamer.jl:323-325, src/photo.jl:49-51). */
+int srmg_fill_ghost_all(srmg_ctx *ctx, int id, const int8_t *bc);
+
+/* vcycle! (src/multigrid.jl:60-106): L u + s b = 0. Mutates u, b (the FAS
+ * right-hand side at covered coarse cells), r and u1 (scratch). lmax = 0 means
+ * all levels. All four fields must have the same element type. */
+int srmg_vcycle(srmg_ctx *ctx, int u, int b, int r, int u1, double s, const srmg_op *op,
+                int lmax, int ndown, int ntop, int nup, double omega);
+
+/* fmg! (src/multigrid.jl:17-51). */
+int srmg_fmg(srmg_ctx *ctx, int u, int b, int r, int u1, double s, const srmg_op *op,
+             int ndown, int ntop, int nup, double omega);
+
+/* resnorm! (src/multigrid.jl:391-417): sqrt of the sum over leaf cells of
+ * r^2 2^(-D(l-1)), r = -(b + L u / s_l). Synchronizes. Does not modify r
+ * (the reference overwrites its r argument with squared residuals). */
+int srmg_resnorm(srmg_ctx *ctx, int r, int b, int u, double s, const srmg_op *op, double *out);
+
+/* Block until all work queued on the context's stream has finished. */
+int srmg_synchronize(srmg_ctx *ctx);
+
+/* ---- solve drivers (the callers' loops of src/streamer.jl:322-334) ---- */
+
+typedef struct {
+    int32_t ndown, ntop, nup;  /* (2, 2, 4) in src/simulate.jl:145 order (nup, ndown, ntop) */
+    double omega;              /* 1.0 for V-cycles */
+    double tol;                /* stop when resnorm(u) <= tol * resnorm(u = 0) */
+    int32_t max_cycles;
+    int32_t use_graph;         /* capture one cycle in a hipGraph and replay it */
+} srmg_solve_opts;
+
+/* Solve from the current u (ghosts pre-filled here) with plain V-cycles of
+ * the fields' own type until the relative residual is <= tol. hist (may be
+ * NULL, else max_cycles+1 entries) receives resnorm/resnorm0 after each cycle
+ * (hist[0] = resnorm(u_initial)/resnorm0). */
+int srmg_solve(srmg_ctx *ctx, int u, int b, int r, int u1, double s, const srmg_op *op,
+               const srmg_solve_opts *opts, int *cycles, double *hist);
+
+/* Mixed precision: u, b FP64; e, be, re, ue FP32 scratch fields. Each cycle
+ * forms the FP64 residual, runs one FP32 V-cycle on the correction equation
+ * from e = 0 and adds e to u; the coarsest-level solve is FP64. */
+int srmg_solve_mixed(srmg_ctx *ctx, int u, int b, int e, int be, int re, int ue, double s,
+                     const srmg_op *op, const srmg_solve_opts *opts, int *cycles, double *hist);
+
+/* ---- measurement ---- */
+
+enum { SRMG_KERNEL_GS_PASS = 0, SRMG_KERNEL_RESIDUAL = 1 };
+
+/* Average duration of one launch of a level kernel, timed with HIP events on
+ * the context's stream over `reps` back-to-back launches (GS passes alternate
+ * colours). Modifies u (GS) or r (residual). */
+int srmg_time_kernel(srmg_ctx *ctx, int kind, int level, int u, int b, int r, double s,
+                     const srmg_op *op, int reps, double *ms_per_launch);
+
+/* Blocks, leaf blocks and coarse-fine ghost slabs of a level (1-based). */
+int srmg_level_info(srmg_ctx *ctx, int level, int64_t *nblocks, int64_t *nleaf, int64_t *nslots);
+
+/* ---- host utilities (no device work) ---- */
+
+/* Eigendecomposition of the level-1 1D operator along one axis with n cells
+ * (radial != 0: cylindrical weights), boundary signs alow/ahigh: T = P diag(lam) Q,
+ * P and Q row-major n x n. Used by the exact coarsest-level solve. */
+int srmg_axis_decomp(int n, int radial, double alow, double ahigh,
+                     double *lam, double *P, double *Q);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SRMG_H */

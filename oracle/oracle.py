@@ -1,0 +1,198 @@
+"""ctypes binding of the CPU oracle (liboracle.so built from srmg_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, never by the library itself. Every function
+mirrors a reference routine named in srmg_oracle.c and works in place on host
+fields in the reference's layout (``ScalarBlockField.u``, Fortran order).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+GEOM = {"cartesian": 0, "cyl1": 1, "cyl2": 2}
+
+
+def build(force: bool = False) -> str:
+    path = os.path.join(_HERE, "liboracle.so")
+    if force or not os.path.exists(path):
+        subprocess.run(["make", "-s", "-B" if force else "-s", "-C", _HERE],
+                       check=True, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    return path
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        _LIB = C.CDLL(build())
+        P = C.c_void_p
+        i64, i32, dbl = C.c_int64, C.c_int, C.c_double
+        _LIB.ora_tree_create.restype = P
+        _LIB.ora_tree_create.argtypes = [i32, i32, i32, i32, i64] + [P] * 10
+        _LIB.ora_tree_destroy.argtypes = [P]
+        for name, args in {
+            "ora_fill_ghost_copy": [P, P, i32],
+            "ora_fill_ghost_bnd": [P, P, i32, P],
+            "ora_fill_ghost_interp": [P, P, i32],
+            "ora_fill_ghost": [P, P, i32, P],
+            "ora_fill_ghost_all": [P, P, P],
+            "ora_gs_level": [P, P, P, dbl, dbl, i32, P, i32],
+            "ora_gs_iter": [P, P, P, dbl, dbl, i32, i32, P],
+            "ora_residual_level": [P, P, P, P, dbl, i32, P],
+            "ora_restrict_level": [P, P, i32],
+            "ora_postrestrict_level": [P, P, P, P, dbl, i32, P],
+            "ora_copy_level": [P, P, P, i32],
+            "ora_diff_level": [P, P, P, P, i32],
+            "ora_interp_add_level": [P, P, P, i32],
+            "ora_vcycle": [P, P, P, P, P, dbl, P, i32, i32, i32, i32, dbl, i32],
+            "ora_fmg": [P, P, P, P, P, dbl, P, i32, i32, i32, dbl, i32],
+        }.items():
+            getattr(_LIB, name).argtypes = args
+            getattr(_LIB, name).restype = None
+        _LIB.ora_resnorm.argtypes = [P, P, P, P, dbl, P]
+        _LIB.ora_resnorm.restype = dbl
+        _LIB.ora_coarse_solve.argtypes = [P, P, P, dbl, P]
+        _LIB.ora_coarse_solve.restype = i32
+    return _LIB
+
+
+class _Op(C.Structure):
+    _fields_ = [("geom", C.c_int), ("helm", C.c_int), ("k2", C.c_double),
+                ("bc", C.c_int8 * 27)]
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Oracle:
+    """Oracle bound to one tree. ``tree``/``conn`` are the host structures of
+    ``srmg.tree``; fields are ``ScalarBlockField`` objects (or their ``.u``)."""
+
+    def __init__(self, tree, conn, M: int, G: int, geom: int, bc, helm: bool = False,
+                 k2: float = 0.0):
+        from_tree = _flatten(tree, conn)
+        self.D, self.M, self.G = tree.D, M, G
+        self.nlevels = len(tree)
+        self.nblocks = tree.nblocks()
+        self._keep = from_tree
+        L = lib()
+        self.h = L.ora_tree_create(self.D, M, G, self.nlevels, self.nblocks,
+                                   *[_ptr(a) for a in from_tree])
+        self.op = _Op()
+        self.op.geom = int(geom)
+        self.op.helm = int(bool(helm))
+        self.op.k2 = float(k2)
+        bc = np.asarray(bc, np.int8).ravel()
+        for i in range(len(bc)):
+            self.op.bc[i] = int(bc[i])
+        self._bc = np.ascontiguousarray(bc, dtype=np.int8)
+
+    @classmethod
+    def for_problem(cls, p, G=None):
+        return cls(p.tree, p.conn, p.M, p.b.G if G is None else G, p.geom, p.bc, p.helm, p.k2)
+
+    def __del__(self):
+        try:
+            lib().ora_tree_destroy(self.h)
+        except Exception:
+            pass
+
+    @staticmethod
+    def _a(f):
+        a = f.u if hasattr(f, "u") else f
+        assert a.dtype == np.float64 and a.flags.f_contiguous
+        return a
+
+    def _opp(self):
+        return C.byref(self.op)
+
+    # ghost fills -----------------------------------------------------------
+    def fill_ghost(self, u, level):
+        lib().ora_fill_ghost(self.h, _ptr(self._a(u)), level, _ptr(self._bc))
+
+    def fill_ghost_all(self, u):
+        lib().ora_fill_ghost_all(self.h, _ptr(self._a(u)), _ptr(self._bc))
+
+    def fill_ghost_copy(self, u, level):
+        lib().ora_fill_ghost_copy(self.h, _ptr(self._a(u)), level)
+
+    def fill_ghost_bnd(self, u, level):
+        lib().ora_fill_ghost_bnd(self.h, _ptr(self._a(u)), level, _ptr(self._bc))
+
+    def fill_ghost_interp(self, u, level):
+        lib().ora_fill_ghost_interp(self.h, _ptr(self._a(u)), level)
+
+    # level operations ------------------------------------------------------
+    def gs_iter(self, u, b, omega, s, n, level):
+        lib().ora_gs_iter(self.h, _ptr(self._a(u)), _ptr(self._a(b)), omega, s, n, level, self._opp())
+
+    def gs_level(self, u, b, omega, s, level, P):
+        lib().ora_gs_level(self.h, _ptr(self._a(u)), _ptr(self._a(b)), omega, s, level, self._opp(), P)
+
+    def residual_level(self, r, u, b, s, level):
+        lib().ora_residual_level(self.h, _ptr(self._a(r)), _ptr(self._a(u)), _ptr(self._a(b)),
+                                 s, level, self._opp())
+
+    def restrict_level(self, u, level):
+        lib().ora_restrict_level(self.h, _ptr(self._a(u)), level)
+
+    def postrestrict_level(self, b, u, r, s, level):
+        lib().ora_postrestrict_level(self.h, _ptr(self._a(b)), _ptr(self._a(u)), _ptr(self._a(r)),
+                                     s, level, self._opp())
+
+    def copy_level(self, dst, src, level):
+        lib().ora_copy_level(self.h, _ptr(self._a(dst)), _ptr(self._a(src)), level)
+
+    def diff_level(self, dst, a1, a2, level):
+        lib().ora_diff_level(self.h, _ptr(self._a(dst)), _ptr(self._a(a1)), _ptr(self._a(a2)), level)
+
+    def interp_add_level(self, u, u1, level):
+        lib().ora_interp_add_level(self.h, _ptr(self._a(u)), _ptr(self._a(u1)), level)
+
+    # solvers ---------------------------------------------------------------
+    def vcycle(self, u, b, r, u1, s, lmax=0, ndown=3, ntop=5, nup=3, omega=1.0, coarse_exact=False):
+        lib().ora_vcycle(self.h, _ptr(self._a(u)), _ptr(self._a(b)), _ptr(self._a(r)),
+                         _ptr(self._a(u1)), s, self._opp(), lmax, ndown, ntop, nup, omega,
+                         int(bool(coarse_exact)))
+
+    def fmg(self, u, b, r, u1, s, ndown=3, ntop=5, nup=3, omega=1.25, coarse_exact=False):
+        lib().ora_fmg(self.h, _ptr(self._a(u)), _ptr(self._a(b)), _ptr(self._a(r)),
+                      _ptr(self._a(u1)), s, self._opp(), ndown, ntop, nup, omega,
+                      int(bool(coarse_exact)))
+
+    def resnorm(self, r, b, u, s) -> float:
+        return lib().ora_resnorm(self.h, _ptr(self._a(r)), _ptr(self._a(b)), _ptr(self._a(u)),
+                                 s, self._opp())
+
+    def coarse_solve(self, u, b, s) -> int:
+        return lib().ora_coarse_solve(self.h, _ptr(self._a(u)), _ptr(self._a(b)), s, self._opp())
+
+
+def _flatten(tree, conn):
+    """pairs and the four record lists as flat int64 arrays + level offsets."""
+    from_pairs = []
+    off = [0]
+    for layer in tree:
+        for c, blk in layer.pairs:
+            from_pairs.append((*c, blk))
+        off.append(len(from_pairs))
+    D = tree.D
+    pairs = np.ascontiguousarray(np.array(from_pairs, np.int64).reshape(-1, D + 1))
+    out = [pairs, np.array(off, np.int64)]
+    for lists, w in ((conn.neighbor, 2 + D), (conn.boundary, 2 + 2 * D),
+                     (conn.refboundary, 2 + 2 * D), (conn.child, 2 + D)):
+        o = [0]
+        for a in lists:
+            o.append(o[-1] + len(a))
+        arr = (np.concatenate([a.reshape(-1, w) for a in lists]) if o[-1]
+               else np.zeros((1, w), np.int64))
+        out += [np.ascontiguousarray(arr.astype(np.int64)), np.array(o, np.int64)]
+    return out

@@ -1,0 +1,777 @@
+/*
+ * srmg oracle: a plain-C restatement of the reference's block-structured AMR
+ * FAS multigrid (Poisson and Helmholtz, order 2), operating on the reference's
+ * own host memory layout.
+ *
+ * TEST INFRASTRUCTURE ONLY. This file is the checker for the HIP library: only
+ * tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg load it. The
+ * product path never calls it.
+ *
+ * Layout: a field is the reference's ScalarBlockField{D,M,G,Float64} with
+ * contiguous storage (src/blockfield.jl:33-59): column-major S^D x nblk,
+ * S = M + 2G, first index fastest, block ids 1-based. All local indices below
+ * are 1-based, as in the reference.
+ *
+ * Tree and connectivity arrive as the reference's isbits records (int64):
+ *   pairs       (coord[D], blk)                      src/blocklayer.jl:23
+ *   Neighbor    (from, to, face[D])                  src/connectivity.jl:14-20
+ *   Boundary    (block, face[D], bnd[D], level)      src/connectivity.jl:6-11
+ *   RefBoundary (fine, coarse, face[D], subblock[D]) src/connectivity.jl:23-32
+ *   Child       (fine, coarse, subblock[D])          src/connectivity.jl:36-42
+ * each concatenated over levels with an (nlevels + 1) offset array.
+ *
+ * Floating point: compiled with -ffp-contract=off; fma() appears exactly where
+ * the reference writes muladd (src/multigrid.jl:176-177). Stencil terms are
+ * summed in the order the reference's @stencil macro emits them
+ * (src/laplacian.jl:25-49: column-major over the 3^D literal, zeros skipped).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef struct {
+    int D, M, G, S, nlevels;
+    int64_t SD;        /* S^D */
+    int64_t nblocks;   /* storage blocks */
+    int64_t *pairs, *pairs_off;
+    int64_t *nbr, *nbr_off;
+    int64_t *bnd, *bnd_off;
+    int64_t *ref, *ref_off;
+    int64_t *chd, *chd_off;
+    int64_t *coord_of; /* blk -> coord[D] (index (blk-1)*D) */
+    int *level_of;     /* blk -> level */
+    /* cached level-1 axis decompositions, keyed by (geom, bc) */
+    int ck_valid, ck_geom;
+    int8_t ck_bc[27];
+    double *ck_lam[3], *ck_P[3], *ck_Q[3];
+} ora_tree;
+
+typedef struct {
+    int geom;          /* 0 Cartesian; 1 CylindricalGeometry{1}; 2 CylindricalGeometry{2} */
+    int helm;          /* 0 LaplacianDiscretization{D,2}; 1 HelmholtzDiscretization{D,2} */
+    double k2;
+    int8_t bc[27];     /* GeneralHomogeneousBoundaryConditions{D}.a, column-major */
+} ora_op;
+
+static int64_t *dup64(const int64_t *src, int64_t n) {
+    int64_t *p = (int64_t *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int64_t));
+    if (n > 0) memcpy(p, src, (size_t)n * sizeof(int64_t));
+    return p;
+}
+
+ora_tree *ora_tree_create(int D, int M, int G, int nlevels, int64_t nblocks,
+                          const int64_t *pairs, const int64_t *pairs_off,
+                          const int64_t *nbr, const int64_t *nbr_off,
+                          const int64_t *bnd, const int64_t *bnd_off,
+                          const int64_t *ref, const int64_t *ref_off,
+                          const int64_t *chd, const int64_t *chd_off) {
+    ora_tree *t = (ora_tree *)calloc(1, sizeof(ora_tree));
+    t->D = D; t->M = M; t->G = G; t->S = M + 2 * G; t->nlevels = nlevels;
+    t->SD = 1;
+    for (int d = 0; d < D; d++) t->SD *= t->S;
+    t->nblocks = nblocks;
+    t->pairs_off = dup64(pairs_off, nlevels + 1);
+    t->nbr_off = dup64(nbr_off, nlevels + 1);
+    t->bnd_off = dup64(bnd_off, nlevels + 1);
+    t->ref_off = dup64(ref_off, nlevels + 1);
+    t->chd_off = dup64(chd_off, nlevels + 1);
+    t->pairs = dup64(pairs, pairs_off[nlevels] * (D + 1));
+    t->nbr = dup64(nbr, nbr_off[nlevels] * (2 + D));
+    t->bnd = dup64(bnd, bnd_off[nlevels] * (2 + 2 * D));
+    t->ref = dup64(ref, ref_off[nlevels] * (2 + 2 * D));
+    t->chd = dup64(chd, chd_off[nlevels] * (2 + D));
+    t->coord_of = (int64_t *)calloc((size_t)(nblocks * D + 1), sizeof(int64_t));
+    t->level_of = (int *)calloc((size_t)(nblocks + 1), sizeof(int));
+    for (int l = 0; l < nlevels; l++) {
+        for (int64_t k = pairs_off[l]; k < pairs_off[l + 1]; k++) {
+            const int64_t *p = t->pairs + k * (D + 1);
+            int64_t blk = p[D];
+            for (int d = 0; d < D; d++) t->coord_of[(blk - 1) * D + d] = p[d];
+            t->level_of[blk - 1] = l + 1;
+        }
+    }
+    return t;
+}
+
+void ora_tree_destroy(ora_tree *t) {
+    if (!t) return;
+    free(t->pairs); free(t->pairs_off); free(t->nbr); free(t->nbr_off);
+    free(t->bnd); free(t->bnd_off); free(t->ref); free(t->ref_off);
+    free(t->chd); free(t->chd_off); free(t->coord_of); free(t->level_of);
+    for (int d = 0; d < 3; d++) { free(t->ck_lam[d]); free(t->ck_P[d]); free(t->ck_Q[d]); }
+    free(t);
+}
+
+/* offset of local 1-based index I (length D) in block blk (1-based) */
+static inline int64_t OFF(const ora_tree *t, const int *I, int64_t blk) {
+    int64_t o = (I[0] - 1);
+    if (t->D >= 2) o += (int64_t)(I[1] - 1) * t->S;
+    if (t->D >= 3) o += (int64_t)(I[2] - 1) * t->S * t->S;
+    return o + (blk - 1) * t->SD;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Operators: src/laplacian.jl:78-128, 278-303; src/helmholtz.jl:17-100      */
+/* ---------------------------------------------------------------------- */
+
+/* Centre coefficient of the stencil / diagonal element (src/laplacian.jl:94,
+ * 127, 302; src/helmholtz.jl:34-35, 66-67, 99-100). */
+static inline double centre_coef(const ora_tree *t, const ora_op *op, double s) {
+    double c = -2.0 * t->D;
+    if (op->helm) c = c - s * op->k2;
+    return c;
+}
+
+/* applystencil(u[blk], s, I, J, ld, geom, Val(:lhs)). The @stencil macro emits
+ * v = 0; v += a[S1] * u[I + S1 - 2] for S1 in CartesianIndices(a) (column-major)
+ * with a[S1] != 0. For the 5-point 2D literal this is the offset order
+ * (0,-1), (-1,0), (0,0), (+1,0), (0,+1); for the 7-point 3D literal
+ * (0,0,-1), (0,-1,0), (-1,0,0), (0,0,0), (+1,0,0), (0,+1,0), (0,0,+1). */
+static double applystencil(const ora_tree *t, const ora_op *op, const double *u,
+                           int64_t blk, const int *I, const int64_t *J, double s) {
+    const int D = t->D;
+    double cc = centre_coef(t, op, s);
+    /* per-dimension (minus, plus) weights */
+    double wm[3] = {1.0, 1.0, 1.0}, wp[3] = {1.0, 1.0, 1.0};
+    if (op->geom == 1 || op->geom == 2) {
+        int N = op->geom - 1;
+        int64_t i = J[N];
+        wm[N] = (double)(2 * i - 2) / (double)(2 * i - 1);
+        wp[N] = (double)(2 * i) / (double)(2 * i - 1);
+    }
+    double v = 0.0;
+    int K[3];
+    /* minus offsets, highest dimension first */
+    for (int d = D - 1; d >= 0; d--) {
+        for (int e = 0; e < D; e++) K[e] = I[e];
+        K[d] -= 1;
+        v += wm[d] * u[OFF(t, K, blk)];
+    }
+    v += cc * u[OFF(t, I, blk)];
+    for (int d = 0; d < D; d++) {
+        for (int e = 0; e < D; e++) K[e] = I[e];
+        K[d] += 1;
+        v += wp[d] * u[OFF(t, K, blk)];
+    }
+    return v; /* divided by denom = 1 in the reference: exact */
+}
+
+/* J = I - G + gbl0 - 1 with gbl0 = M (b - 1) + 1 (src/multigrid.jl:170, src/mesh.jl:76-78) */
+static inline void global_index(const ora_tree *t, int64_t blk, const int *I, int64_t *J) {
+    for (int d = 0; d < t->D; d++)
+        J[d] = I[d] - t->G + t->M * (t->coord_of[(blk - 1) * t->D + d] - 1);
+}
+
+/* ---------------------------------------------------------------------- */
+/* Ghost filling: src/connectivity.jl:155-258, src/boundary.jl:45-47,       */
+/* src/blockfield.jl:416-541                                                */
+/* ---------------------------------------------------------------------- */
+
+void ora_fill_ghost_copy(const ora_tree *t, double *u, int level) {
+    const int D = t->D, M = t->M, G = t->G;
+    const int W = 2 + D;
+    int64_t lo = t->nbr_off[level - 1], hi = t->nbr_off[level];
+#pragma omp parallel for schedule(static)
+    for (int64_t k = lo; k < hi; k++) {
+        const int64_t *e = t->nbr + k * W;
+        int64_t from = e[0], to = e[1];
+        const int64_t *f = e + 2;
+        int n[3] = {1, 1, 1};
+        for (int d = 0; d < D; d++) n[d] = (f[d] == 0) ? M : G;
+        int i[3];
+        for (i[2] = 1; i[2] <= n[2]; i[2]++)
+        for (i[1] = 1; i[1] <= n[1]; i[1]++)
+        for (i[0] = 1; i[0] <= n[0]; i[0]++) {
+            int If[3], It[3];
+            for (int d = 0; d < D; d++) {
+                If[d] = (f[d] == 0) ? G + i[d] : (f[d] == -1 ? i[d] + G : i[d] + M);
+                It[d] = (f[d] == 0) ? G + i[d] : (f[d] == 1 ? i[d] : i[d] + M + G);
+            }
+            u[OFF(t, It, to)] = u[OFF(t, If, from)];
+        }
+    }
+}
+
+static inline int bc_index(int D, const int64_t *bnd) {
+    int idx = 0, m = 1;
+    for (int d = 0; d < D; d++) { idx += (int)(bnd[d] + 1) * m; m *= 3; }
+    return idx;
+}
+
+static inline int mirror_index(int G, int M, int i, int64_t dir) {
+    if (dir == -1) return 2 * G - i + 1;
+    if (dir == 1) return 2 * (G + M) - i + 1;
+    return i;
+}
+
+void ora_fill_ghost_bnd(const ora_tree *t, double *u, int level, const int8_t *bc) {
+    const int D = t->D, M = t->M, G = t->G;
+    const int W = 2 + 2 * D;
+    int64_t lo = t->bnd_off[level - 1], hi = t->bnd_off[level];
+#pragma omp parallel for schedule(static)
+    for (int64_t k = lo; k < hi; k++) {
+        const int64_t *e = t->bnd + k * W;
+        int64_t blk = e[0];
+        const int64_t *face = e + 1, *bnd = e + 1 + D;
+        double a = (double)bc[bc_index(D, bnd)];
+        int first[3] = {1, 1, 1}, last[3] = {1, 1, 1};
+        for (int d = 0; d < D; d++) {
+            if (face[d] == 0) { first[d] = G + 1; last[d] = G + M; }
+            else if (face[d] == -1) { first[d] = 1; last[d] = G; }
+            else { first[d] = M + G + 1; last[d] = M + 2 * G; }
+        }
+        int g[3];
+        for (g[2] = first[2]; g[2] <= last[2]; g[2]++)
+        for (g[1] = first[1]; g[1] <= last[1]; g[1]++)
+        for (g[0] = first[0]; g[0] <= last[0]; g[0]++) {
+            int v[3];
+            for (int d = 0; d < D; d++) v[d] = mirror_index(G, M, g[d], bnd[d]);
+            u[OFF(t, g, blk)] = a * u[OFF(t, v, blk)];
+        }
+    }
+}
+
+/* interp!(::InterpLinear, dest, idest, src, isrc, func) (src/interp.jl:7-11, 102-141).
+ * first_d/len_d describe the destination and source rectangles (actual 1-based
+ * indices); add != 0 selects func = + (src/interp.jl:185-193). */
+static void interp_linear(const ora_tree *t, double *dst_f, int64_t dblk, const int *dfirst,
+                          const double *src_f, int64_t sblk, const int *sfirst, const int *slen,
+                          int add) {
+    const int D = t->D;
+    const double w0 = (D == 2) ? 0.5 : 0.25;
+    const double w1 = 0.25;
+    int is[3] = {0, 0, 0};
+    int sl[3] = {1, 1, 1};
+    for (int d = 0; d < D; d++) sl[d] = slen[d];
+    for (is[2] = 0; is[2] < sl[2]; is[2]++)
+    for (is[1] = 0; is[1] < sl[1]; is[1]++)
+    for (is[0] = 0; is[0] < sl[0]; is[0]++) {
+        int Js[3], Jd0[3];
+        for (int d = 0; d < D; d++) {
+            Js[d] = sfirst[d] + is[d];
+            Jd0[d] = dfirst[d] + 2 * is[d];
+        }
+        int nc = 1 << D;
+        for (int c = 0; c < nc; c++) {
+            int ii[3], Jd[3];
+            for (int d = 0; d < D; d++) { ii[d] = (c >> d) & 1; Jd[d] = Jd0[d] + ii[d]; }
+            double f = w0 * src_f[OFF(t, Js, sblk)];
+            for (int j = 0; j < D; j++) {
+                int K[3];
+                for (int d = 0; d < D; d++) K[d] = Js[d];
+                K[j] += 2 * ii[j] - 1;
+                f += w1 * src_f[OFF(t, K, sblk)];
+            }
+            int64_t o = OFF(t, Jd, dblk);
+            dst_f[o] = add ? (f + dst_f[o]) : f;
+        }
+    }
+}
+
+void ora_fill_ghost_interp(const ora_tree *t, double *u, int level) {
+    const int D = t->D, M = t->M, G = t->G;
+    const int W = 2 + 2 * D;
+    const int mh = M / 2, gh = G / 2;
+    int64_t lo = t->ref_off[level - 1], hi = t->ref_off[level];
+#pragma omp parallel for schedule(static)
+    for (int64_t k = lo; k < hi; k++) {
+        const int64_t *e = t->ref + k * W;
+        int64_t fine = e[0], coarse = e[1];
+        const int64_t *face = e + 2, *sb = e + 2 + D;
+        int dfirst[3] = {1, 1, 1}, sfirst[3] = {1, 1, 1}, slen[3] = {1, 1, 1};
+        for (int d = 0; d < D; d++) {
+            /* ghostindices(u, face) */
+            dfirst[d] = (face[d] == 0) ? G + 1 : (face[d] == -1 ? 1 : M + G + 1);
+            /* subblockbnd(u, sb, -face) */
+            int sbf = G + 1 + (int)(sb[d] - 1) * mh;
+            int64_t fd = -face[d];
+            if (fd == -1) { sfirst[d] = sbf; slen[d] = gh; }
+            else if (fd == 1) { sfirst[d] = sbf + mh - gh; slen[d] = gh; }
+            else { sfirst[d] = sbf; slen[d] = mh; }
+        }
+        interp_linear(t, u, fine, dfirst, u, coarse, sfirst, slen, 0);
+    }
+}
+
+void ora_fill_ghost(const ora_tree *t, double *u, int level, const int8_t *bc) {
+    ora_fill_ghost_copy(t, u, level);
+    ora_fill_ghost_bnd(t, u, level, bc);
+    ora_fill_ghost_interp(t, u, level);
+}
+
+/* ---------------------------------------------------------------------- */
+/* Smoother, residuals, transfers: src/multigrid.jl, src/restrict.jl,        */
+/* src/interp.jl, src/blockfield.jl:557-576                                  */
+/* ---------------------------------------------------------------------- */
+
+/* gauss_seidel!(..., ::RedBlack{P}) (src/multigrid.jl:153-180) over a level
+ * (gauss_seidel_level!, src/multigrid.jl:238-243). */
+void ora_gs_level(const ora_tree *t, double *u, const double *b, double omega, double s,
+                  int level, const ora_op *op, int P) {
+    const int D = t->D, M = t->M, G = t->G;
+    const double c = centre_coef(t, op, s);   /* diagelm */
+    const double nwc = -(omega / c);
+    int64_t lo = t->pairs_off[level - 1], hi = t->pairs_off[level];
+#pragma omp parallel for schedule(static)
+    for (int64_t k = lo; k < hi; k++) {
+        int64_t blk = t->pairs[k * (D + 1) + D];
+        int i[3] = {1, 1, 1};
+        int n2 = (D >= 2) ? M : 1, n3 = (D >= 3) ? M : 1;
+        for (int a3 = 0; a3 < n3; a3++)
+        for (int a2 = 0; a2 < n2; a2++)
+        for (i[0] = G + 1; i[0] <= G + M; i[0] += 2) {
+            i[1] = G + 1 + a2;
+            i[2] = G + 1 + a3;
+            int p = P;
+            for (int d = 1; d < D; d++) p ^= (i[d] & 1);
+            int I[3] = {i[0] + p, i[1], i[2]};
+            int64_t J[3];
+            global_index(t, blk, I, J);
+            double Lu = applystencil(t, op, u, blk, I, J, s);
+            int64_t o = OFF(t, I, blk);
+            double v = fma(s, b[o], Lu);
+            u[o] = fma(nwc, v, u[o]);
+        }
+    }
+}
+
+/* gauss_seidel_iter! for order-2 discretizations (src/multigrid.jl:250-265) */
+void ora_gs_iter(const ora_tree *t, double *u, const double *b, double omega, double s,
+                 int n, int level, const ora_op *op) {
+    for (int it = 0; it < n; it++) {
+        ora_gs_level(t, u, b, omega, s, level, op, 0);
+        ora_fill_ghost_copy(t, u, level);
+        ora_fill_ghost_bnd(t, u, level, op->bc);
+        ora_gs_level(t, u, b, omega, s, level, op, 1);
+        ora_fill_ghost_copy(t, u, level);
+        ora_fill_ghost_bnd(t, u, level, op->bc);
+    }
+}
+
+/* residual! / residual_level! (src/multigrid.jl:309-323, 349-354) */
+void ora_residual_level(const ora_tree *t, double *r, const double *u, const double *b,
+                        double s, int level, const ora_op *op) {
+    const int D = t->D, M = t->M, G = t->G;
+    int64_t lo = t->pairs_off[level - 1], hi = t->pairs_off[level];
+#pragma omp parallel for schedule(static)
+    for (int64_t k = lo; k < hi; k++) {
+        int64_t blk = t->pairs[k * (D + 1) + D];
+        int I[3] = {1, 1, 1};
+        int n2 = (D >= 2) ? M : 1, n3 = (D >= 3) ? M : 1;
+        for (int a3 = 0; a3 < n3; a3++)
+        for (int a2 = 0; a2 < n2; a2++)
+        for (int a1 = 0; a1 < M; a1++) {
+            I[0] = G + 1 + a1; I[1] = G + 1 + a2; I[2] = G + 1 + a3;
+            int64_t J[3];
+            global_index(t, blk, I, J);
+            double Lu = applystencil(t, op, u, blk, I, J, s);
+            int64_t o = OFF(t, I, blk);
+            r[o] = -(b[o] + Lu / s);
+        }
+    }
+}
+
+/* restrict! / restrict_level! (src/restrict.jl:5-37) over conn.child[level] */
+void ora_restrict_level(const ora_tree *t, double *u, int level) {
+    const int D = t->D, M = t->M, G = t->G, W = 2 + D;
+    const int mh = M / 2;
+    const double k = (double)(1 << D);
+    int64_t lo = t->chd_off[level - 1], hi = t->chd_off[level];
+#pragma omp parallel for schedule(static)
+    for (int64_t e = lo; e < hi; e++) {
+        const int64_t *c = t->chd + e * W;
+        int64_t fine = c[0], coarse = c[1];
+        const int64_t *sb = c + 2;
+        int id[3] = {0, 0, 0};
+        int n2 = (D >= 2) ? mh : 1, n3 = (D >= 3) ? mh : 1;
+        for (id[2] = 0; id[2] < n3; id[2]++)
+        for (id[1] = 0; id[1] < n2; id[1]++)
+        for (id[0] = 0; id[0] < mh; id[0]++) {
+            int Js0[3] = {1, 1, 1}, Jd[3] = {1, 1, 1};
+            for (int d = 0; d < D; d++) {
+                Js0[d] = G + 1 + 2 * id[d];
+                Jd[d] = G + 1 + (int)(sb[d] - 1) * mh + id[d];
+            }
+            double f = 0.0;
+            int nc = 1 << D;
+            for (int q = 0; q < nc; q++) {   /* i_1 innermost (@nloops order) */
+                int K[3] = {1, 1, 1};
+                for (int d = 0; d < D; d++) K[d] = Js0[d] + ((q >> d) & 1);
+                f += u[OFF(t, K, fine)];
+            }
+            u[OFF(t, Jd, coarse)] = f / k;
+        }
+    }
+}
+
+/* residual_postrestrict_level! -> residual_subblock! (src/multigrid.jl:112-119,
+ * 331-344), called as (b, u, r, 4 s_l, tree[l-1], conn.child[l]): writes the
+ * FAS right-hand side into b at every coarse cell covered by a level-l block. */
+void ora_postrestrict_level(const ora_tree *t, double *bout, const double *u, const double *r,
+                            double s, int level, const ora_op *op) {
+    const int D = t->D, M = t->M, G = t->G, W = 2 + D;
+    const int mh = M / 2;
+    int64_t lo = t->chd_off[level - 1], hi = t->chd_off[level];
+#pragma omp parallel for schedule(static)
+    for (int64_t e = lo; e < hi; e++) {
+        const int64_t *c = t->chd + e * W;
+        int64_t coarse = c[1];
+        const int64_t *sb = c + 2;
+        int id[3] = {0, 0, 0};
+        int n2 = (D >= 2) ? mh : 1, n3 = (D >= 3) ? mh : 1;
+        for (id[2] = 0; id[2] < n3; id[2]++)
+        for (id[1] = 0; id[1] < n2; id[1]++)
+        for (id[0] = 0; id[0] < mh; id[0]++) {
+            int I[3] = {1, 1, 1};
+            for (int d = 0; d < D; d++) I[d] = G + 1 + (int)(sb[d] - 1) * mh + id[d];
+            int64_t J[3];
+            global_index(t, coarse, I, J);
+            double Lu = applystencil(t, op, u, coarse, I, J, s);
+            int64_t o = OFF(t, I, coarse);
+            bout[o] = -(r[o] + Lu / s);
+        }
+    }
+}
+
+/* copyto!(dest, src, layer) (src/blockfield.jl:557-563): whole blocks, ghosts included */
+void ora_copy_level(const ora_tree *t, double *dst, const double *src, int level) {
+    const int D = t->D;
+    int64_t lo = t->pairs_off[level - 1], hi = t->pairs_off[level];
+#pragma omp parallel for schedule(static)
+    for (int64_t k = lo; k < hi; k++) {
+        int64_t blk = t->pairs[k * (D + 1) + D];
+        memcpy(dst + (blk - 1) * t->SD, src + (blk - 1) * t->SD, (size_t)t->SD * sizeof(double));
+    }
+}
+
+/* diffto!(dest, a1, a2, layer) (src/blockfield.jl:569-576): dest = a1 - a2 */
+void ora_diff_level(const ora_tree *t, double *dst, const double *a1, const double *a2, int level) {
+    const int D = t->D;
+    int64_t lo = t->pairs_off[level - 1], hi = t->pairs_off[level];
+#pragma omp parallel for schedule(static)
+    for (int64_t k = lo; k < hi; k++) {
+        int64_t blk = t->pairs[k * (D + 1) + D];
+        int64_t o = (blk - 1) * t->SD;
+        for (int64_t q = 0; q < t->SD; q++) dst[o + q] = a1[o + q] - a2[o + q];
+    }
+}
+
+/* interp_add_level!(u, u1, conn.child[level]) (src/interp.jl:185-193) */
+void ora_interp_add_level(const ora_tree *t, double *u, const double *u1, int level) {
+    const int D = t->D, M = t->M, G = t->G, W = 2 + D;
+    const int mh = M / 2;
+    int64_t lo = t->chd_off[level - 1], hi = t->chd_off[level];
+#pragma omp parallel for schedule(static)
+    for (int64_t e = lo; e < hi; e++) {
+        const int64_t *c = t->chd + e * W;
+        int64_t fine = c[0], coarse = c[1];
+        const int64_t *sb = c + 2;
+        int dfirst[3] = {1, 1, 1}, sfirst[3] = {1, 1, 1}, slen[3] = {1, 1, 1};
+        for (int d = 0; d < D; d++) {
+            dfirst[d] = G + 1;
+            sfirst[d] = G + 1 + (int)(sb[d] - 1) * mh;
+            slen[d] = mh;
+        }
+        interp_linear(t, u, fine, dfirst, u1, coarse, sfirst, slen, 1);
+    }
+}
+
+/* ---------------------------------------------------------------------- */
+/* Exact level-1 solve (documented extension; no reference counterpart).     */
+/* Level 1 is the complete root grid with homogeneous mirror BCs, so its    */
+/* operator is a Kronecker sum of 1D tridiagonal operators. Each 1D operator */
+/* T = W^-1 S (W = diag(2i-1) on the radial axis, identity otherwise, S      */
+/* symmetric) is diagonalised with the cyclic Jacobi method on              */
+/* A = W^-1/2 S W^-1/2 = Y L Y^T, so T = P L Q, P = W^-1/2 Y, Q = Y^T W^1/2. */
+/* ---------------------------------------------------------------------- */
+
+static void jacobi_eig(int n, double *A, double *V) {
+    /* A (n x n, row-major) symmetric, destroyed; eigenvalues on its diagonal;
+     * V receives the eigenvectors as columns. */
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) V[i * n + j] = (i == j) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double off = 0.0, tot = 0.0;
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) {
+                double a = A[i * n + j] * A[i * n + j];
+                tot += a;
+                if (i != j) off += a;
+            }
+        if (off <= 1e-30 * tot) break;
+        for (int p = 0; p < n - 1; p++)
+            for (int q = p + 1; q < n; q++) {
+                double apq = A[p * n + q];
+                double app = A[p * n + p], aqq = A[q * n + q];
+                if (fabs(apq) <= 1e-300 || fabs(apq) < 1e-18 * (fabs(app) + fabs(aqq))) continue;
+                double theta = (aqq - app) / (2.0 * apq);
+                double tt = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                double cs = 1.0 / sqrt(tt * tt + 1.0), sn = tt * cs;
+                for (int k = 0; k < n; k++) {          /* columns p, q */
+                    double akp = A[k * n + p], akq = A[k * n + q];
+                    A[k * n + p] = cs * akp - sn * akq;
+                    A[k * n + q] = sn * akp + cs * akq;
+                }
+                for (int k = 0; k < n; k++) {          /* rows p, q */
+                    double apk = A[p * n + k], aqk = A[q * n + k];
+                    A[p * n + k] = cs * apk - sn * aqk;
+                    A[q * n + k] = sn * apk + cs * aqk;
+                }
+                for (int k = 0; k < n; k++) {
+                    double vkp = V[k * n + p], vkq = V[k * n + q];
+                    V[k * n + p] = cs * vkp - sn * vkq;
+                    V[k * n + q] = sn * vkp + cs * vkq;
+                }
+            }
+    }
+}
+
+/* 1D operator of dimension d at level 1 with n cells; returns lam[n], P, Q (n x n row-major) */
+static void axis_decomp(int n, int radial, double alow, double ahigh,
+                        double *lam, double *P, double *Q) {
+    double *A = (double *)calloc((size_t)n * n, sizeof(double));
+    double *Y = (double *)calloc((size_t)n * n, sizeof(double));
+    double *w = (double *)calloc((size_t)n, sizeof(double));
+    for (int i = 0; i < n; i++) {
+        int ii = i + 1;
+        w[i] = radial ? (double)(2 * ii - 1) : 1.0;
+        double diag = -2.0;
+        if (!radial) {
+            if (i == 0) diag += alow;
+            if (i == n - 1) diag += ahigh;
+        } else {
+            /* axis weight (2i-2)/(2i-1) is 0 at i = 1 */
+            if (i == n - 1) diag += ((double)(2 * ii) / (double)(2 * ii - 1)) * ahigh;
+        }
+        A[i * n + i] = diag * w[i];           /* S_ii */
+        if (i + 1 < n) {
+            double sij = radial ? (double)(2 * ii) : 1.0;
+            A[i * n + i + 1] = sij;
+            A[(i + 1) * n + i] = sij;
+        }
+    }
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) A[i * n + j] /= sqrt(w[i] * w[j]);
+    jacobi_eig(n, A, Y);
+    for (int k = 0; k < n; k++) lam[k] = A[k * n + k];
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < n; k++) {
+            P[i * n + k] = Y[i * n + k] / sqrt(w[i]);
+            Q[k * n + i] = Y[i * n + k] * sqrt(w[i]);
+        }
+    free(A); free(Y); free(w);
+}
+
+/* apply the n x n matrix Mx along axis d of a column-major grid with sizes N[] */
+static void apply_axis(const double *Mx, int d, const int *N, int D, double *x, double *tmp) {
+    int64_t stride = 1;
+    for (int e = 0; e < d; e++) stride *= N[e];
+    int64_t total = 1;
+    for (int e = 0; e < D; e++) total *= N[e];
+    int n = N[d];
+    int64_t outer = total / (stride * n);
+    for (int64_t o = 0; o < outer; o++)
+        for (int64_t s0 = 0; s0 < stride; s0++) {
+            int64_t base = o * stride * n + s0;
+            for (int i = 0; i < n; i++) {
+                double acc = 0.0;
+                for (int k = 0; k < n; k++) acc += Mx[i * n + k] * x[base + (int64_t)k * stride];
+                tmp[base + (int64_t)i * stride] = acc;
+            }
+        }
+    memcpy(x, tmp, (size_t)total * sizeof(double));
+}
+
+/* u[level 1] = exact solution of L_1 u + s b = 0, then the copy and boundary
+ * ghost fills that end gauss_seidel_iter!. Returns 0, or -1 if singular. */
+int ora_coarse_solve(ora_tree *t, double *u, const double *b, double s, const ora_op *op) {
+    const int D = t->D, M = t->M, G = t->G;
+    int nb[3] = {1, 1, 1}, N[3] = {1, 1, 1};
+    /* level-1 domain extent from the pairs (level 1 is complete) */
+    for (int64_t k = t->pairs_off[0]; k < t->pairs_off[1]; k++)
+        for (int d = 0; d < D; d++) {
+            int c = (int)t->pairs[k * (D + 1) + d];
+            if (c > nb[d]) nb[d] = c;
+        }
+    int64_t total = 1;
+    for (int d = 0; d < D; d++) { N[d] = nb[d] * M; total *= N[d]; }
+    double *x = (double *)calloc((size_t)total, sizeof(double));
+    double *tmp = (double *)calloc((size_t)total, sizeof(double));
+    double *lam[3], *P[3], *Q[3];
+    int radialdim = (op->geom == 1) ? 0 : (op->geom == 2 ? 1 : -1);
+    int hit = t->ck_valid && t->ck_geom == op->geom && memcmp(t->ck_bc, op->bc, 27) == 0;
+    for (int d = 0; d < D; d++) {
+        if (!hit) {
+            int fl[3] = {0, 0, 0}, fh[3] = {0, 0, 0};
+            fl[d] = -1; fh[d] = 1;
+            int64_t bl[3] = {fl[0], fl[1], fl[2]}, bh[3] = {fh[0], fh[1], fh[2]};
+            double alow = (double)op->bc[bc_index(D, bl)];
+            double ahigh = (double)op->bc[bc_index(D, bh)];
+            free(t->ck_lam[d]); free(t->ck_P[d]); free(t->ck_Q[d]);
+            t->ck_lam[d] = (double *)malloc(sizeof(double) * N[d]);
+            t->ck_P[d] = (double *)malloc(sizeof(double) * N[d] * N[d]);
+            t->ck_Q[d] = (double *)malloc(sizeof(double) * N[d] * N[d]);
+            axis_decomp(N[d], d == radialdim, alow, ahigh, t->ck_lam[d], t->ck_P[d], t->ck_Q[d]);
+        }
+        lam[d] = t->ck_lam[d]; P[d] = t->ck_P[d]; Q[d] = t->ck_Q[d];
+    }
+    t->ck_valid = 1; t->ck_geom = op->geom;
+    memcpy(t->ck_bc, op->bc, 27);
+    /* gather rhs = -s b */
+    for (int64_t k = t->pairs_off[0]; k < t->pairs_off[1]; k++) {
+        const int64_t *p = t->pairs + k * (D + 1);
+        int64_t blk = p[D];
+        int a[3] = {0, 0, 0};
+        int n2 = (D >= 2) ? M : 1, n3 = (D >= 3) ? M : 1;
+        for (a[2] = 0; a[2] < n3; a[2]++)
+        for (a[1] = 0; a[1] < n2; a[1]++)
+        for (a[0] = 0; a[0] < M; a[0]++) {
+            int I[3] = {G + 1 + a[0], G + 1 + a[1], G + 1 + a[2]};
+            int64_t gi = 0, m = 1;
+            for (int d = 0; d < D; d++) { gi += (M * (p[d] - 1) + a[d]) * m; m *= N[d]; }
+            x[gi] = -s * b[OFF(t, I, blk)];
+        }
+    }
+    for (int d = 0; d < D; d++) apply_axis(Q[d], d, N, D, x, tmp);
+    double shift = op->helm ? -(s * op->k2) : 0.0;
+    int singular = 0;
+    for (int64_t gi = 0; gi < total; gi++) {
+        int64_t r = gi;
+        double ev = shift;
+        for (int d = 0; d < D; d++) { ev += lam[d][r % N[d]]; r /= N[d]; }
+        if (fabs(ev) < 1e-13) { singular = 1; ev = 1.0; }
+        x[gi] /= ev;
+    }
+    for (int d = 0; d < D; d++) apply_axis(P[d], d, N, D, x, tmp);
+    for (int64_t k = t->pairs_off[0]; k < t->pairs_off[1]; k++) {
+        const int64_t *p = t->pairs + k * (D + 1);
+        int64_t blk = p[D];
+        int a[3] = {0, 0, 0};
+        int n2 = (D >= 2) ? M : 1, n3 = (D >= 3) ? M : 1;
+        for (a[2] = 0; a[2] < n3; a[2]++)
+        for (a[1] = 0; a[1] < n2; a[1]++)
+        for (a[0] = 0; a[0] < M; a[0]++) {
+            int I[3] = {G + 1 + a[0], G + 1 + a[1], G + 1 + a[2]};
+            int64_t gi = 0, m = 1;
+            for (int d = 0; d < D; d++) { gi += (M * (p[d] - 1) + a[d]) * m; m *= N[d]; }
+            u[OFF(t, I, blk)] = x[gi];
+        }
+    }
+    free(x); free(tmp);
+    ora_fill_ghost_copy(t, u, 1);
+    ora_fill_ghost_bnd(t, u, 1, op->bc);
+    return singular ? -1 : 0;
+}
+
+/* ---------------------------------------------------------------------- */
+/* V-cycle, FMG, residual norm: src/multigrid.jl:17-106, 391-417            */
+/* ---------------------------------------------------------------------- */
+
+static inline double level_scale(double s, int l) {
+    int64_t f = (int64_t)1 << (l - 1);
+    return s / (double)(f * f);
+}
+
+static void top_solve(ora_tree *t, double *u, const double *b, double omega, double s,
+                      int ntop, const ora_op *op, int coarse_exact) {
+    if (coarse_exact) ora_coarse_solve(t, u, b, s, op);
+    else ora_gs_iter(t, u, b, omega, s, ntop, 1, op);
+}
+
+/* vcycle! (src/multigrid.jl:60-106). coarse_exact = 0 is the reference's
+ * ntop Gauss-Seidel sweeps at level 1. */
+void ora_vcycle(ora_tree *t, double *u, double *b, double *r, double *u1, double s,
+                const ora_op *op, int lmax, int ndown, int ntop, int nup, double omega,
+                int coarse_exact) {
+    if (lmax == 0) lmax = t->nlevels;
+    for (int l = lmax; l >= 2; l--) {
+        double sl = level_scale(s, l);
+        ora_gs_iter(t, u, b, omega, sl, ndown, l, op);
+        ora_restrict_level(t, u, l);
+        ora_fill_ghost(t, u, l - 1, op->bc);
+        ora_copy_level(t, u1, u, l - 1);
+        ora_residual_level(t, r, u, b, sl, l, op);
+        ora_fill_ghost(t, r, l, op->bc);
+        ora_restrict_level(t, r, l);
+        ora_fill_ghost(t, r, l - 1, op->bc);
+        ora_postrestrict_level(t, b, u, r, 4 * sl, l, op);
+    }
+    top_solve(t, u, b, omega, s, ntop, op, coarse_exact);
+    for (int l = 2; l <= lmax; l++) {
+        double sl = level_scale(s, l);
+        ora_diff_level(t, u1, u, u1, l - 1);
+        ora_fill_ghost(t, u1, l - 1, op->bc);
+        ora_interp_add_level(t, u, u1, l);
+        ora_fill_ghost(t, u, l, op->bc);
+        ora_gs_iter(t, u, b, omega, sl, nup, l, op);
+        ora_fill_ghost(t, u, l, op->bc);
+    }
+}
+
+static int finest_level(const ora_tree *t) {
+    int last = 0;
+    for (int l = 0; l < t->nlevels; l++)
+        if (t->pairs_off[l + 1] > t->pairs_off[l]) last = l + 1;
+    return last;
+}
+
+/* fmg! (src/multigrid.jl:17-51) */
+void ora_fmg(ora_tree *t, double *u, double *b, double *r, double *u1, double s,
+             const ora_op *op, int ndown, int ntop, int nup, double omega, int coarse_exact) {
+    int lmax = finest_level(t);
+    for (int l = lmax; l >= 2; l--) {
+        double sl = level_scale(s, l);
+        ora_restrict_level(t, u, l);
+        ora_fill_ghost(t, u, l - 1, op->bc);
+        ora_residual_level(t, r, u, b, sl, l, op);
+        ora_fill_ghost(t, r, l, op->bc);
+        ora_restrict_level(t, r, l);
+        ora_fill_ghost(t, r, l - 1, op->bc);
+        ora_postrestrict_level(t, b, u, r, 4 * sl, l, op);
+    }
+    for (int l = 1; l <= lmax; l++) {
+        ora_copy_level(t, u1, u, l);
+        if (l > 1) {
+            ora_diff_level(t, u1, u, u1, l - 1);
+            ora_fill_ghost(t, u1, l - 1, op->bc);
+            ora_interp_add_level(t, u, u1, l);
+        }
+        ora_fill_ghost(t, u, l, op->bc);
+        ora_vcycle(t, u, b, r, u1, s, op, l, ndown, ntop, nup, omega, coarse_exact);
+    }
+}
+
+/* resnorm! (src/multigrid.jl:391-417). Overwrites r. */
+double ora_resnorm(const ora_tree *t, double *r, const double *b, const double *u, double s,
+                   const ora_op *op) {
+    const int D = t->D, M = t->M, G = t->G;
+    int lmax = t->nlevels;
+    for (int l = lmax; l >= 1; l--) ora_residual_level(t, r, u, b, level_scale(s, l), l, op);
+    int64_t tot = t->nblocks * t->SD;
+    for (int64_t q = 0; q < tot; q++) r[q] = r[q] * r[q];
+    for (int l = lmax - 1; l >= 1; l--) ora_restrict_level(t, r, l + 1);
+    double w = 0.0;
+    for (int64_t k = t->pairs_off[0]; k < t->pairs_off[1]; k++) {
+        int64_t blk = t->pairs[k * (D + 1) + D];
+        double part = 0.0;
+        int first = 1;
+        int I[3] = {1, 1, 1};
+        int n2 = (D >= 2) ? M : 1, n3 = (D >= 3) ? M : 1;
+        for (int a3 = 0; a3 < n3; a3++)
+        for (int a2 = 0; a2 < n2; a2++)
+        for (int a1 = 0; a1 < M; a1++) {
+            I[0] = G + 1 + a1; I[1] = G + 1 + a2; I[2] = G + 1 + a3;
+            double v = r[OFF(t, I, blk)];
+            if (first) { part = v; first = 0; } else part += v;
+        }
+        w += part;
+    }
+    return sqrt(w);
+}
+
+/* Caller-side pre-solve fill: fill_ghost!(u, l) for l = 1..L (src/streamer.jl:323-325) */
+void ora_fill_ghost_all(const ora_tree *t, double *u, const int8_t *bc) {
+    for (int l = 1; l <= t->nlevels; l++) ora_fill_ghost(t, u, l, bc);
+}

@@ -1,0 +1,1261 @@
+// libsrmg: MI355X-native block-structured AMR geometric multigrid.
+//
+// Host side: context and regrid tables, host<->device layout conversion,
+// and the orchestration of vcycle!/fmg!/resnorm! (src/multigrid.jl) as kernel
+// sequences on one HIP stream. The C ABI is declared in include/srmg.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <type_traits>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/srmg.h"
+#include "kernels.h"
+#include "layout.h"
+
+using namespace srmg;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+
+static thread_local std::string g_err;
+
+struct Failure {
+    int code;
+    std::string msg;
+};
+
+[[noreturn]] static void fail(const std::string &m, int code = 1) { throw Failure{code, m}; }
+
+#define HIPCHK(expr)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            fail(std::string(#expr) + ": " + hipGetErrorString(e_), 2);                   \
+    } while (0)
+
+template <class F>
+static int guarded(F &&f) {
+    try {
+        f();
+        return 0;
+    } catch (const Failure &e) {
+        g_err = e.msg;
+        return e.code;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return 3;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void alloc(size_t n) {
+        release();
+        if (n == 0) n = 16;
+        HIPCHK(hipMalloc(&p, n));
+        bytes = n;
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+struct FieldBuf {
+    int dtype = SRMG_F64;
+    DevBuf data;  // nb * V values
+    DevBuf ib;    // nslot * F values (coarse-fine ghost slabs)
+    size_t word() const { return dtype == SRMG_F64 ? 8 : 4; }
+};
+
+struct LevelInfo {
+    int blk0 = 0, nblk = 0;     // device blocks
+    int slot0 = 0, nslot = 0;   // coarse-fine ghost slots owned by this level's blocks
+    int leaf0 = 0, nleaf = 0;   // entries in the leaf list
+    int ext[3] = {1, 1, 1};     // level domain in blocks
+    int wofs[2] = {0, 0};       // radial weight table offsets (dims 0 and 1)
+};
+
+struct CoarseCache {
+    bool valid = false;
+    int geom = -1;
+    int8_t bc[27] = {0};
+    int N[3] = {1, 1, 1};
+    DevBuf P[3], Q[3], lam[3];
+};
+
+struct srmg_ctx {
+    int D = 2, M = 16, V = 0, F = 0;
+    int nlevels = 0;
+    int64_t nhost = 0;      // host block ids 1..nhost
+    int nb = 0;             // device blocks
+    int nslot = 0;
+    std::vector<LevelInfo> lv;
+    // host copies of the tables
+    std::vector<int> h_nbr, h_coord, h_parent, h_sub, h_level, h_leaves;
+    std::vector<int> host2dev, dev2host;
+    std::vector<int4> h_slots;
+    // device tables
+    DevBuf d_nbr, d_coord, d_parent, d_sub, d_slots, d_leaves;
+    DevBuf d_w64, d_w32;    // radial weights (double / float), per level and dim
+    DevBuf d_partial, d_sum;
+    DevBuf d_grid, d_grid2; // level-1 uniform grids (coarse solve)
+    CoarseCache coarse;
+    std::vector<std::unique_ptr<FieldBuf>> fields;
+    std::unique_ptr<FieldBuf> zero[2];  // all-zero scratch fields (F64, F32), made on demand
+    hipStream_t stream = nullptr;
+    int maxpartials = 0;
+};
+
+static void alloc_field(srmg_ctx *c, FieldBuf &f);
+
+static FieldBuf &zero_field(srmg_ctx *c, int dtype) {
+    auto &z = c->zero[dtype];
+    if (!z || z->data.bytes != (size_t)c->nb * c->V * (dtype == SRMG_F64 ? 8 : 4)) {
+        z = std::make_unique<FieldBuf>();
+        z->dtype = dtype;
+        alloc_field(c, *z);
+    }
+    return *z;
+}
+
+static FieldBuf &field(srmg_ctx *c, int id) {
+    if (id < 0 || id >= (int)c->fields.size() || !c->fields[id]) fail("invalid field id " + std::to_string(id));
+    return *c->fields[id];
+}
+
+static void alloc_field(srmg_ctx *c, FieldBuf &f) {
+    f.data.alloc((size_t)c->nb * c->V * f.word());
+    f.ib.alloc((size_t)std::max(c->nslot, 1) * c->F * f.word());
+    HIPCHK(hipMemsetAsync(f.data.p, 0, f.data.bytes, c->stream));
+    HIPCHK(hipMemsetAsync(f.ib.p, 0, f.ib.bytes, c->stream));
+}
+
+// ---------------------------------------------------------------------------
+// regrid: device tables from the reference's records
+// ---------------------------------------------------------------------------
+
+static uint64_t morton(const int64_t *c, int D) {
+    uint64_t code = 0;
+    for (int bit = 0; bit < 21; bit++)
+        for (int d = 0; d < D; d++)
+            code |= (uint64_t)(((c[d] - 1) >> bit) & 1) << (bit * D + d);
+    return code;
+}
+
+static void set_tree(srmg_ctx *c, int nlevels, int64_t nhost, const int64_t *pairs, const int64_t *pairs_off,
+                     const int64_t *nbr, const int64_t *nbr_off, const int64_t *bnd, const int64_t *bnd_off,
+                     const int64_t *ref, const int64_t *ref_off, const int64_t *chd, const int64_t *chd_off) {
+    const int D = c->D, M = c->M;
+    if (nlevels < 1) fail("nlevels must be >= 1");
+    c->nlevels = nlevels;
+    c->nhost = nhost;
+    c->lv.assign(nlevels, LevelInfo());
+    c->host2dev.assign(nhost + 1, -1);
+    int nb = (int)pairs_off[nlevels];
+    c->nb = nb;
+    c->dev2host.assign(nb, 0);
+    c->h_coord.assign((size_t)nb * D, 0);
+    c->h_level.assign(nb, 0);
+    // level-1 extent (level 1 must be complete)
+    int ext1[3] = {1, 1, 1};
+    for (int64_t k = pairs_off[0]; k < pairs_off[1]; k++)
+        for (int d = 0; d < D; d++) ext1[d] = std::max(ext1[d], (int)pairs[k * (D + 1) + d]);
+    int64_t n1 = 1;
+    for (int d = 0; d < D; d++) n1 *= ext1[d];
+    if (pairs_off[1] - pairs_off[0] != n1) fail("level 1 must be complete (populate!)");
+    int dev = 0;
+    for (int l = 0; l < nlevels; l++) {
+        LevelInfo &L = c->lv[l];
+        L.blk0 = dev;
+        L.nblk = (int)(pairs_off[l + 1] - pairs_off[l]);
+        for (int d = 0; d < D; d++) L.ext[d] = ext1[d] << l;
+        std::vector<std::pair<uint64_t, int64_t>> order;
+        for (int64_t k = pairs_off[l]; k < pairs_off[l + 1]; k++)
+            order.push_back({morton(pairs + k * (D + 1), D), k});
+        std::sort(order.begin(), order.end());
+        for (auto &o : order) {
+            const int64_t *p = pairs + o.second * (D + 1);
+            int64_t hid = p[D];
+            if (hid < 1 || hid > nhost) fail("block id out of range");
+            if (c->host2dev[hid] >= 0) fail("block id appears twice");
+            c->host2dev[hid] = dev;
+            c->dev2host[dev] = (int)hid;
+            for (int d = 0; d < D; d++) c->h_coord[(size_t)dev * D + d] = (int)p[d];
+            c->h_level[dev] = l;
+            dev++;
+        }
+    }
+    auto d_of = [&](int64_t hid) {
+        if (hid < 1 || hid > nhost || c->host2dev[hid] < 0) fail("record names an unknown block");
+        return c->host2dev[hid];
+    };
+    auto face_code = [&](const int64_t *f) {
+        int code = -1;
+        for (int d = 0; d < D; d++)
+            if (f[d] != 0) {
+                if (code >= 0) fail("only star-stencil faces are supported (order 2)");
+                code = 2 * d + (f[d] > 0 ? 1 : 0);
+            }
+        if (code < 0) fail("zero face");
+        return code;
+    };
+    const int ND = 2 * D;
+    c->h_nbr.assign((size_t)nb * ND, INT32_MIN);
+    // Neighbor(from, to, f): `to`'s ghost on face -f comes from `from`
+    for (int l = 0; l < nlevels; l++)
+        for (int64_t k = nbr_off[l]; k < nbr_off[l + 1]; k++) {
+            const int64_t *e = nbr + k * (2 + D);
+            int from = d_of(e[0]), to = d_of(e[1]);
+            int f = face_code(e + 2);
+            c->h_nbr[(size_t)to * ND + (f ^ 1)] = from;
+        }
+    for (int l = 0; l < nlevels; l++)
+        for (int64_t k = bnd_off[l]; k < bnd_off[l + 1]; k++) {
+            const int64_t *e = bnd + k * (2 + 2 * D);
+            int blk = d_of(e[0]);
+            int f = face_code(e + 1);
+            c->h_nbr[(size_t)blk * ND + f] = NB_BOUNDARY;
+        }
+    // coarse-fine ghost slots, grouped by the level of the fine block
+    c->h_slots.clear();
+    for (int l = 0; l < nlevels; l++) {
+        c->lv[l].slot0 = (int)c->h_slots.size();
+        std::vector<std::pair<int64_t, int4>> tmp;
+        for (int64_t k = ref_off[l]; k < ref_off[l + 1]; k++) {
+            const int64_t *e = ref + k * (2 + 2 * D);
+            int fine = d_of(e[0]), coarse = d_of(e[1]);
+            int f = face_code(e + 2);
+            int sb = 0;
+            for (int d = 0; d < D; d++) {
+                int64_t s = e[2 + D + d];
+                if (s != 1 && s != 2) fail("bad subblock");
+                sb |= (int)(s - 1) << d;
+            }
+            if (c->h_level[fine] != l || c->h_level[coarse] != l - 1) fail("refinement boundary level mismatch");
+            tmp.push_back({(int64_t)fine * ND + f, make_int4(fine, f, coarse, sb)});
+        }
+        std::sort(tmp.begin(), tmp.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+        for (auto &t : tmp) {
+            int k = (int)c->h_slots.size();
+            c->h_nbr[(size_t)t.second.x * ND + t.second.y] = -2 - k;
+            c->h_slots.push_back(t.second);
+        }
+        c->lv[l].nslot = (int)c->h_slots.size() - c->lv[l].slot0;
+    }
+    c->nslot = (int)c->h_slots.size();
+    for (size_t i = 0; i < c->h_nbr.size(); i++)
+        if (c->h_nbr[i] == INT32_MIN)
+            fail("block " + std::to_string(c->dev2host[i / ND]) + " face " + std::to_string(i % ND) +
+                 " has no neighbour, boundary or refinement record");
+    // parents
+    c->h_parent.assign(nb, -1);
+    c->h_sub.assign(nb, 0);
+    for (int l = 0; l < nlevels; l++)
+        for (int64_t k = chd_off[l]; k < chd_off[l + 1]; k++) {
+            const int64_t *e = chd + k * (2 + D);
+            int fine = d_of(e[0]), coarse = d_of(e[1]);
+            int sb = 0;
+            for (int d = 0; d < D; d++) sb |= (int)(e[2 + d] - 1) << d;
+            c->h_parent[fine] = coarse;
+            c->h_sub[fine] = sb;
+        }
+    for (int l = 1; l < nlevels; l++)
+        for (int b = c->lv[l].blk0; b < c->lv[l].blk0 + c->lv[l].nblk; b++)
+            if (c->h_parent[b] < 0) fail("block without Child record");
+    // leaves: blocks without children
+    std::vector<char> has_child(nb, 0);
+    for (int b = 0; b < nb; b++)
+        if (c->h_parent[b] >= 0) has_child[c->h_parent[b]] = 1;
+    c->h_leaves.clear();
+    for (int l = 0; l < nlevels; l++) {
+        c->lv[l].leaf0 = (int)c->h_leaves.size();
+        for (int b = c->lv[l].blk0; b < c->lv[l].blk0 + c->lv[l].nblk; b++)
+            if (!has_child[b]) c->h_leaves.push_back(b);
+        c->lv[l].nleaf = (int)c->h_leaves.size() - c->lv[l].leaf0;
+    }
+    // radial weight tables (2J-2)/(2J-1), 2J/(2J-1) for J = 0..n (entry 0 unused)
+    std::vector<double> w64;
+    for (int l = 0; l < nlevels; l++)
+        for (int d = 0; d < std::min(D, 2); d++) {
+            c->lv[l].wofs[d] = (int)w64.size();
+            int n = M * c->lv[l].ext[d];
+            for (int J = 0; J <= n; J++) w64.push_back(J == 0 ? 0.0 : (double)(2 * J - 2) / (double)(2 * J - 1));
+            for (int J = 0; J <= n; J++) w64.push_back(J == 0 ? 0.0 : (double)(2 * J) / (double)(2 * J - 1));
+        }
+    std::vector<float> w32(w64.size());
+    for (size_t i = 0; i < w64.size(); i++) w32[i] = (float)w64[i];
+    // every index a kernel will follow stays inside its table
+    for (size_t i = 0; i < c->h_nbr.size(); i++) {
+        int v = c->h_nbr[i];
+        if (!(v == NB_BOUNDARY || (v >= 0 && v < nb) || (v <= -2 && nb_slot(v) < c->nslot)))
+            fail("internal: neighbour table entry out of range");
+        if (v >= 0 && c->h_level[v] != c->h_level[i / ND]) fail("neighbour on another level");
+    }
+    for (auto &sl : c->h_slots)
+        if (sl.x < 0 || sl.x >= nb || sl.z < 0 || sl.z >= nb || sl.y < 0 || sl.y >= ND)
+            fail("internal: ghost slot out of range");
+    for (int b = 0; b < nb; b++) {
+        int pb = c->h_parent[b];
+        if (c->h_level[b] > 0 && (pb < 0 || pb >= nb || c->h_level[pb] != c->h_level[b] - 1))
+            fail("internal: parent table entry out of range");
+        for (int d = 0; d < D; d++) {
+            int cd = c->h_coord[(size_t)b * D + d];
+            if (cd < 1 || cd > c->lv[c->h_level[b]].ext[d]) fail("block coordinate outside the level domain");
+        }
+    }
+    auto up = [&](DevBuf &b, const void *src, size_t bytes) {
+        b.alloc(bytes);
+        if (bytes) HIPCHK(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, c->stream));
+    };
+    up(c->d_nbr, c->h_nbr.data(), c->h_nbr.size() * sizeof(int));
+    up(c->d_coord, c->h_coord.data(), c->h_coord.size() * sizeof(int));
+    up(c->d_parent, c->h_parent.data(), c->h_parent.size() * sizeof(int));
+    up(c->d_sub, c->h_sub.data(), c->h_sub.size() * sizeof(int));
+    up(c->d_slots, c->h_slots.data(), c->h_slots.size() * sizeof(int4));
+    up(c->d_leaves, c->h_leaves.data(), c->h_leaves.size() * sizeof(int));
+    up(c->d_w64, w64.data(), w64.size() * sizeof(double));
+    up(c->d_w32, w32.data(), w32.size() * sizeof(float));
+    // reduction scratch: one partial per work group of the largest leaf launch
+    int maxp = 1;
+    for (int l = 0; l < nlevels; l++) maxp += (int)(((int64_t)c->lv[l].nleaf * c->V + WG - 1) / WG);
+    c->maxpartials = maxp;
+    c->d_partial.alloc((size_t)maxp * sizeof(double));
+    c->d_sum.alloc(sizeof(double) * 4);
+    c->coarse.valid = false;
+    c->zero[0].reset();
+    c->zero[1].reset();
+    size_t ng = (size_t)c->lv[0].nblk * c->V;
+    c->d_grid.alloc(ng * sizeof(double));
+    c->d_grid2.alloc(ng * sizeof(double));
+    for (auto &f : c->fields)
+        if (f) alloc_field(c, *f);
+    HIPCHK(hipStreamSynchronize(c->stream));
+}
+
+// ---------------------------------------------------------------------------
+// per-level kernel arguments
+// ---------------------------------------------------------------------------
+
+template <typename T>
+static LevelArgs<T> level_args(srmg_ctx *c, int l /*0-based*/, double s, const srmg_op *op, double omega) {
+    LevelArgs<T> a;
+    a.blk0 = c->lv[l].blk0;
+    a.nblk = c->lv[l].nblk;
+    int64_t f = (int64_t)1 << l;
+    double sl = s / (double)(f * f);
+    double cc = -2.0 * c->D;
+    if (op->disc == SRMG_HELMHOLTZ) cc = cc - sl * op->k2;
+    a.s = (T)sl;
+    a.cc = (T)cc;
+    a.nwc = (T)(-(omega / cc));
+    for (int q = 0; q < 6; q++) a.sg[q] = T(1);
+    for (int d = 0; d < c->D; d++)
+        for (int side = 0; side < 2; side++) {
+            int idx = 0, m = 1;
+            for (int e = 0; e < c->D; e++) {
+                int fv = (e == d) ? (side ? 1 : -1) : 0;
+                idx += (fv + 1) * m;
+                m *= 3;
+            }
+            a.sg[2 * d + side] = (T)op->bc[idx];
+        }
+    a.rdim = (op->geometry == SRMG_CYL1) ? 0 : (op->geometry == SRMG_CYL2 ? 1 : -1);
+    const T *wbase = std::is_same<T, double>::value ? (const T *)c->d_w64.p : (const T *)c->d_w32.p;
+    if (a.rdim >= 0) {
+        int n = c->M * c->lv[l].ext[a.rdim];
+        a.wm = wbase + c->lv[l].wofs[a.rdim];
+        a.wp = a.wm + (n + 1);
+    } else {
+        a.wm = a.wp = wbase;
+    }
+    return a;
+}
+
+static inline unsigned nwg(int64_t threads) { return (unsigned)((threads + WG - 1) / WG); }
+
+// ---------------------------------------------------------------------------
+// kernel launchers, dispatched on (D, M, T)
+// ---------------------------------------------------------------------------
+
+template <int D, int M, typename T>
+struct Ops {
+    using LY = Lay<D, M>;
+    srmg_ctx *c;
+    explicit Ops(srmg_ctx *ctx) : c(ctx) {}
+    T *X(FieldBuf &f) { return f.data.as<T>(); }
+    T *I(FieldBuf &f) { return f.ib.as<T>(); }
+    const int *nbr() { return c->d_nbr.as<int>(); }
+    const int *coord() { return c->d_coord.as<int>(); }
+
+    void gs_pass(FieldBuf &u, FieldBuf &b, const LevelArgs<T> &a, int pass) {
+        if (!a.nblk) return;
+        hipLaunchKernelGGL((k_gs_pass<D, M, T>), dim3(nwg((int64_t)a.nblk * LY::H)), dim3(WG), 0, c->stream,
+                           X(u), X(b), I(u), nbr(), coord(), a, pass);
+    }
+    // gauss_seidel_iter! (src/multigrid.jl:250-265): the copy and boundary ghost
+    // fills between colours are implicit (ghosts read from valid cells).
+    void smooth(FieldBuf &u, FieldBuf &b, const LevelArgs<T> &a, int n) {
+        for (int i = 0; i < n; i++) {
+            gs_pass(u, b, a, 0);
+            gs_pass(u, b, a, 1);
+        }
+    }
+    void residual(FieldBuf &r, FieldBuf &u, FieldBuf &b, const LevelArgs<T> &a) {
+        if (!a.nblk) return;
+        hipLaunchKernelGGL((k_residual<D, M, T>), dim3(nwg((int64_t)a.nblk * LY::V)), dim3(WG), 0, c->stream,
+                           X(r), X(u), X(b), I(u), nbr(), coord(), a);
+    }
+    void restrict_(FieldBuf &x, int l) {
+        const LevelInfo &L = c->lv[l];
+        if (!L.nblk) return;
+        hipLaunchKernelGGL((k_restrict<D, M, T>), dim3(nwg((int64_t)L.nblk * (LY::V / LY::NC))), dim3(WG), 0,
+                           c->stream, X(x), c->d_parent.as<int>(), c->d_sub.as<int>(), L.blk0, L.nblk);
+    }
+    void postrestrict(FieldBuf &b, FieldBuf &u, FieldBuf &r, int l, const LevelArgs<T> &ac) {
+        const LevelInfo &L = c->lv[l];
+        if (!L.nblk) return;
+        hipLaunchKernelGGL((k_postrestrict<D, M, T>), dim3(nwg((int64_t)L.nblk * (LY::V / LY::NC))), dim3(WG), 0,
+                           c->stream, X(b), X(u), X(r), I(u), nbr(), coord(), c->d_parent.as<int>(),
+                           c->d_sub.as<int>(), L.blk0, L.nblk, ac);
+    }
+    void copy(FieldBuf &dst, FieldBuf &src, int l) {
+        const LevelInfo &L = c->lv[l];
+        size_t n = (size_t)L.nblk * LY::V;
+        if (!n) return;
+        hipLaunchKernelGGL((k_copy<T>), dim3(nwg(n)), dim3(WG), 0, c->stream, X(dst) + (size_t)L.blk0 * LY::V,
+                           X(src) + (size_t)L.blk0 * LY::V, n);
+    }
+    void diff(FieldBuf &dst, FieldBuf &a1, FieldBuf &a2, int l) {
+        const LevelInfo &L = c->lv[l];
+        size_t n = (size_t)L.nblk * LY::V;
+        if (!n) return;
+        size_t o = (size_t)L.blk0 * LY::V;
+        hipLaunchKernelGGL((k_diff<T>), dim3(nwg(n)), dim3(WG), 0, c->stream, X(dst) + o, X(a1) + o, X(a2) + o, n);
+    }
+    void interp_add(FieldBuf &u, FieldBuf &u1, int l, const LevelArgs<T> &ac) {
+        const LevelInfo &L = c->lv[l];
+        if (!L.nblk) return;
+        hipLaunchKernelGGL((k_interp_add<D, M, T>), dim3(nwg((int64_t)L.nblk * LY::V)), dim3(WG), 0, c->stream,
+                           X(u), X(u1), I(u1), nbr(), c->d_parent.as<int>(), c->d_sub.as<int>(), L.blk0, L.nblk,
+                           ac);
+    }
+    // refresh the coarse-fine ghost slabs of level l from level l-1
+    void fill_interp(FieldBuf &x, int l, const LevelArgs<T> &ac) {
+        const LevelInfo &L = c->lv[l];
+        if (!L.nslot) return;
+        hipLaunchKernelGGL((k_fill_interp<D, M, T>), dim3(nwg((int64_t)L.nslot * LY::F)), dim3(WG), 0, c->stream,
+                           I(x), X(x), nbr(), c->d_slots.as<int4>(), L.slot0, L.nslot, ac);
+    }
+    // queue the squared-residual partial sums of all levels; returns partial count
+    int resnorm_partials(FieldBuf &u, FieldBuf &b, double s, const srmg_op *op) {
+        int np = 0;
+        for (int l = 0; l < c->nlevels; l++) {
+            const LevelInfo &L = c->lv[l];
+            if (!L.nleaf) continue;
+            LevelArgs<T> a = level_args<T>(c, l, s, op, 1.0);
+            double wgt = std::ldexp(1.0, -c->D * l);
+            unsigned g = nwg((int64_t)L.nleaf * LY::V);
+            hipLaunchKernelGGL((k_resnorm<D, M, T>), dim3(g), dim3(WG), 0, c->stream, X(u), X(b), I(u), nbr(),
+                               coord(), c->d_leaves.as<int>() + L.leaf0, L.nleaf, a, wgt,
+                               c->d_partial.as<double>() + np);
+            np += (int)g;
+        }
+        hipLaunchKernelGGL(k_sum, dim3(1), dim3(WG), 0, c->stream, c->d_partial.as<double>(), np,
+                           c->d_sum.as<double>());
+        return np;
+    }
+
+    // exact FP64 solve of L_1 u + s b = 0 on the complete level 1
+    void coarse_exact(FieldBuf &u, FieldBuf &b, double s, const srmg_op *op);
+
+    void top(FieldBuf &u, FieldBuf &b, double s, const srmg_op *op, int ntop, double omega) {
+        if (op->coarse == SRMG_COARSE_EXACT) {
+            coarse_exact(u, b, s, op);
+        } else {
+            LevelArgs<T> a = level_args<T>(c, 0, s, op, omega);
+            smooth(u, b, a, ntop);
+        }
+    }
+
+    // vcycle! (src/multigrid.jl:60-106). The residual of level l is formed
+    // right after smoothing: it reads only level-l cells and level-l ghosts, none
+    // of which the restriction to l-1 changes. The fill_ghost! calls on r are not
+    // issued (r's ghosts are never read), and the descent's fill_ghost!(u, l-1)
+    // needs no work: its interpolation ghosts come from level l-2, unchanged
+    // since they were last filled.
+    void vcycle(FieldBuf &u, FieldBuf &b, FieldBuf &r, FieldBuf &u1, double s, const srmg_op *op, int lmax,
+                int ndown, int ntop, int nup, double omega) {
+        if (lmax == 0) lmax = c->nlevels;
+        for (int l = lmax - 1; l >= 1; l--) {
+            LevelArgs<T> a = level_args<T>(c, l, s, op, omega);
+            LevelArgs<T> ac = level_args<T>(c, l - 1, s, op, omega);
+            smooth(u, b, a, ndown);
+            residual(r, u, b, a);
+            restrict_(u, l);
+            copy(u1, u, l - 1);
+            restrict_(r, l);
+            postrestrict(b, u, r, l, ac);
+        }
+        top(u, b, s, op, ntop, omega);
+        for (int l = 1; l < lmax; l++) {
+            LevelArgs<T> a = level_args<T>(c, l, s, op, omega);
+            LevelArgs<T> ac = level_args<T>(c, l - 1, s, op, omega);
+            diff(u1, u, u1, l - 1);
+            if (l - 1 >= 1) fill_interp(u1, l - 1, level_args<T>(c, l - 2, s, op, omega));
+            interp_add(u, u1, l, ac);
+            fill_interp(u, l, ac);
+            smooth(u, b, a, nup);
+        }
+    }
+
+    // fmg! (src/multigrid.jl:17-51)
+    void fmg(FieldBuf &u, FieldBuf &b, FieldBuf &r, FieldBuf &u1, double s, const srmg_op *op, int ndown, int ntop,
+             int nup, double omega) {
+        int lmax = 0;
+        for (int l = 0; l < c->nlevels; l++)
+            if (c->lv[l].nblk) lmax = l + 1;
+        for (int l = lmax - 1; l >= 1; l--) {
+            LevelArgs<T> a = level_args<T>(c, l, s, op, omega);
+            LevelArgs<T> ac = level_args<T>(c, l - 1, s, op, omega);
+            residual(r, u, b, a);
+            restrict_(u, l);
+            restrict_(r, l);
+            postrestrict(b, u, r, l, ac);
+        }
+        for (int l = 0; l < lmax; l++) {
+            copy(u1, u, l);
+            if (l > 0) {
+                LevelArgs<T> ac = level_args<T>(c, l - 1, s, op, omega);
+                diff(u1, u, u1, l - 1);
+                if (l - 1 >= 1) fill_interp(u1, l - 1, level_args<T>(c, l - 2, s, op, omega));
+                interp_add(u, u1, l, ac);
+                fill_interp(u, l, ac);
+            }
+            vcycle(u, b, r, u1, s, op, l + 1, ndown, ntop, nup, omega);
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// exact coarsest-level solve
+// ---------------------------------------------------------------------------
+
+// Symmetric tridiagonal eigenproblem by the implicit QL method with Wilkinson
+// shifts. d: diagonal (eigenvalues on return); e[i] couples i and i+1
+// (destroyed); z: n x n row-major, returns eigenvectors in columns.
+static bool tridiag_ql(int n, double *d, double *e, double *z) {
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < n; k++) z[i * n + k] = (i == k) ? 1.0 : 0.0;
+    if (n > 0) e[n - 1] = 0.0;
+    for (int l = 0; l < n; l++) {
+        int iter = 0;
+        for (;;) {
+            int m = l;
+            for (; m < n - 1; m++) {
+                double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
+                if (std::fabs(e[m]) <= 2.220446049250313e-16 * dd) break;
+            }
+            if (m == l) break;
+            if (++iter > 200) return false;
+            double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+            double r = std::hypot(g, 1.0);
+            g = d[m] - d[l] + e[l] / (g + std::copysign(r, g));
+            double s = 1.0, cc = 1.0, p = 0.0;
+            bool early = false;
+            for (int i = m - 1; i >= l; i--) {
+                double f = s * e[i], bb = cc * e[i];
+                r = std::hypot(f, g);
+                e[i + 1] = r;
+                if (r == 0.0) {
+                    d[i + 1] -= p;
+                    e[m] = 0.0;
+                    early = true;
+                    break;
+                }
+                s = f / r;
+                cc = g / r;
+                g = d[i + 1] - p;
+                r = (d[i] - g) * s + 2.0 * cc * bb;
+                p = s * r;
+                d[i + 1] = g + p;
+                g = cc * r - bb;
+                for (int k = 0; k < n; k++) {
+                    double zf = z[k * n + i + 1];
+                    z[k * n + i + 1] = s * z[k * n + i] + cc * zf;
+                    z[k * n + i] = cc * z[k * n + i] - s * zf;
+                }
+            }
+            if (early) continue;
+            d[l] -= p;
+            e[l] = g;
+            e[m] = 0.0;
+        }
+    }
+    return true;
+}
+
+// Level-1 1D operator along one axis (mirror ghosts, cylindrical weights on the
+// radial axis) as T = P diag(lam) Q, via the symmetric form W^1/2 T W^-1/2.
+static void axis_decomp(int n, bool radial, double alow, double ahigh, double *lam, double *P, double *Q) {
+    std::vector<double> d(n), e(n, 0.0), z((size_t)n * n), w(n);
+    for (int i = 0; i < n; i++) {
+        int ii = i + 1;
+        w[i] = radial ? (double)(2 * ii - 1) : 1.0;
+        double diag = -2.0;
+        if (!radial) {
+            if (i == 0) diag += alow;
+            if (i == n - 1) diag += ahigh;
+        } else if (i == n - 1) {
+            diag += ((double)(2 * ii) / (double)(2 * ii - 1)) * ahigh;
+        }
+        d[i] = diag;  // W^-1/2 (W T) W^-1/2 keeps the diagonal
+        if (i + 1 < n) {
+            double sij = radial ? (double)(2 * ii) : 1.0;
+            e[i] = sij / std::sqrt(w[i] * (radial ? (double)(2 * ii + 1) : 1.0));
+        }
+    }
+    if (!tridiag_ql(n, d.data(), e.data(), z.data())) fail("tridiagonal eigensolver did not converge");
+    for (int k = 0; k < n; k++) lam[k] = d[k];
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < n; k++) {
+            P[(size_t)i * n + k] = z[(size_t)i * n + k] / std::sqrt(w[i]);
+            Q[(size_t)k * n + i] = z[(size_t)i * n + k] * std::sqrt(w[i]);
+        }
+}
+
+template <int D, int M, typename T>
+void Ops<D, M, T>::coarse_exact(FieldBuf &u, FieldBuf &b, double s, const srmg_op *op) {
+    CoarseCache &cc = c->coarse;
+    const LevelInfo &L1 = c->lv[0];
+    int N[3] = {1, 1, 1};
+    for (int d = 0; d < D; d++) N[d] = M * L1.ext[d];
+    bool hit = cc.valid && cc.geom == op->geometry && std::memcmp(cc.bc, op->bc, 27) == 0;
+    if (!hit) {
+        int rdim = (op->geometry == SRMG_CYL1) ? 0 : (op->geometry == SRMG_CYL2 ? 1 : -1);
+        for (int d = 0; d < D; d++) {
+            int il = 0, ih = 0, m = 1;
+            for (int e = 0; e < D; e++) {
+                il += ((e == d ? -1 : 0) + 1) * m;
+                ih += ((e == d ? 1 : 0) + 1) * m;
+                m *= 3;
+            }
+            int n = N[d];
+            std::vector<double> lam(n), P((size_t)n * n), Q((size_t)n * n);
+            axis_decomp(n, d == rdim, (double)op->bc[il], (double)op->bc[ih], lam.data(), P.data(), Q.data());
+            cc.lam[d].alloc(n * sizeof(double));
+            cc.P[d].alloc((size_t)n * n * sizeof(double));
+            cc.Q[d].alloc((size_t)n * n * sizeof(double));
+            HIPCHK(hipMemcpy(cc.lam[d].p, lam.data(), n * sizeof(double), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(cc.P[d].p, P.data(), (size_t)n * n * sizeof(double), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(cc.Q[d].p, Q.data(), (size_t)n * n * sizeof(double), hipMemcpyHostToDevice));
+            cc.N[d] = n;
+        }
+        cc.valid = true;
+        cc.geom = op->geometry;
+        std::memcpy(cc.bc, op->bc, 27);
+    }
+    size_t total = (size_t)N[0] * N[1] * N[2];
+    double *g = c->d_grid.as<double>(), *g2 = c->d_grid2.as<double>();
+    hipLaunchKernelGGL((k_grid_gather<D, M, T>), dim3(nwg((int64_t)L1.nblk * LY::V)), dim3(WG), 0, c->stream, g,
+                       X(b), coord(), L1.blk0, L1.nblk, N[0], N[1], -s);
+    size_t stride = 1;
+    for (int d = 0; d < D; d++) {
+        hipLaunchKernelGGL(k_axis, dim3(nwg(total)), dim3(WG), 0, c->stream, cc.Q[d].as<double>(), N[d], stride,
+                           total, g, g2);
+        std::swap(g, g2);
+        stride *= N[d];
+    }
+    double shift = (op->disc == SRMG_HELMHOLTZ) ? -(s * op->k2) : 0.0;
+    hipLaunchKernelGGL(k_eig_scale, dim3(nwg(total)), dim3(WG), 0, c->stream, g, cc.lam[0].as<double>(),
+                       cc.lam[1].as<double>(), D == 3 ? cc.lam[2].as<double>() : cc.lam[1].as<double>(), N[0], N[1],
+                       D, shift, total);
+    stride = 1;
+    for (int d = 0; d < D; d++) {
+        hipLaunchKernelGGL(k_axis, dim3(nwg(total)), dim3(WG), 0, c->stream, cc.P[d].as<double>(), N[d], stride,
+                           total, g, g2);
+        std::swap(g, g2);
+        stride *= N[d];
+    }
+    hipLaunchKernelGGL((k_grid_scatter<D, M, T>), dim3(nwg((int64_t)L1.nblk * LY::V)), dim3(WG), 0, c->stream,
+                       X(u), g, coord(), L1.blk0, L1.nblk, N[0], N[1]);
+}
+
+// ---------------------------------------------------------------------------
+// dispatch
+// ---------------------------------------------------------------------------
+
+template <class Fn>
+static void dispatch(srmg_ctx *c, int dtype, Fn &&fn) {
+#define SRMG_CASE(DD, MM)                                                        \
+    if (c->D == DD && c->M == MM) {                                              \
+        if (dtype == SRMG_F64) { Ops<DD, MM, double> o(c); fn(o); return; }      \
+        if (dtype == SRMG_F32) { Ops<DD, MM, float> o(c); fn(o); return; }       \
+    }
+    SRMG_CASE(2, 16)
+    SRMG_CASE(2, 8)
+    SRMG_CASE(3, 8)
+    SRMG_CASE(3, 16)
+#undef SRMG_CASE
+    fail("unsupported (D, M, dtype)");
+}
+
+static void check_tree(srmg_ctx *c) {
+    if (c->nb == 0) fail("srmg_set_tree has not been called");
+}
+
+static int same_type(srmg_ctx *c, std::initializer_list<int> ids) {
+    int t = -1;
+    for (int id : ids) {
+        int d = field(c, id).dtype;
+        if (t >= 0 && d != t) fail("fields of one call must share an element type");
+        t = d;
+    }
+    return t;
+}
+
+// ---------------------------------------------------------------------------
+// host <-> device layout conversion
+// ---------------------------------------------------------------------------
+
+template <typename TH, typename TD>
+static void to_device_layout(srmg_ctx *c, const TH *host, int G, std::vector<TD> &data, std::vector<TD> &ib) {
+    const int D = c->D, M = c->M, S = M + 2 * G;
+    const size_t SD = (D == 2) ? (size_t)S * S : (size_t)S * S * S;
+    data.assign((size_t)c->nb * c->V, TD(0));
+    ib.assign((size_t)std::max(c->nslot, 1) * c->F, TD(0));
+    auto hoff = [&](int x, int y, int z) {  // 0-based valid coords (may be -1 / M)
+        return (size_t)(x + G) + (size_t)S * ((size_t)(y + G) + (D == 3 ? (size_t)S * (size_t)(z + G) : 0));
+    };
+    for (int b = 0; b < c->nb; b++) {
+        const TH *hb = host + (size_t)(c->dev2host[b] - 1) * SD;
+        TD *db = data.data() + (size_t)b * c->V;
+        int nz = (D == 3) ? M : 1;
+        for (int z = 0; z < nz; z++)
+            for (int y = 0; y < M; y++)
+                for (int x = 0; x < M; x++) {
+                    int odd = (x + y + z + D) & 1;
+                    int pos = (odd ? 0 : c->V / 2) + (x >> 1) + (M / 2) * (y + M * z);
+                    db[pos] = (TD)hb[hoff(x, y, z)];
+                }
+    }
+    // depth-1 ghost layer of coarse-fine faces
+    for (int k = 0; k < c->nslot; k++) {
+        int4 s = c->h_slots[k];
+        int dim = s.y >> 1, dir = (s.y & 1) ? 1 : -1;
+        const TH *hb = host + (size_t)(c->dev2host[s.x] - 1) * SD;
+        for (int t = 0; t < c->F; t++) {
+            int cc[3] = {0, 0, 0};
+            int ta = t % M, tb = t / M;
+            if (dim == 0) { cc[1] = ta; cc[2] = tb; }
+            else if (dim == 1) { cc[0] = ta; cc[2] = tb; }
+            else { cc[0] = ta; cc[1] = tb; }
+            cc[dim] = dir < 0 ? -1 : M;
+            ib[(size_t)k * c->F + t] = (TD)hb[hoff(cc[0], cc[1], cc[2])];
+        }
+    }
+}
+
+template <typename TD, typename TH>
+static void from_device_layout(srmg_ctx *c, const std::vector<TD> &data, const std::vector<TD> &ib, TH *host,
+                               int G, const int8_t *bcsign /* may be null: boundary ghosts untouched */) {
+    const int D = c->D, M = c->M, S = M + 2 * G, V = c->V, ND = 2 * D;
+    const size_t SD = (D == 2) ? (size_t)S * S : (size_t)S * S * S;
+    auto hoff = [&](int x, int y, int z) {
+        return (size_t)(x + G) + (size_t)S * ((size_t)(y + G) + (D == 3 ? (size_t)S * (size_t)(z + G) : 0));
+    };
+    auto dpos = [&](int x, int y, int z) {
+        int odd = (x + y + z + D) & 1;
+        return (odd ? 0 : V / 2) + (x >> 1) + (M / 2) * (y + M * z);
+    };
+    int nz = (D == 3) ? M : 1;
+    for (int b = 0; b < c->nb; b++) {
+        TH *hb = host + (size_t)(c->dev2host[b] - 1) * SD;
+        const TD *db = data.data() + (size_t)b * V;
+        for (int z = 0; z < nz; z++)
+            for (int y = 0; y < M; y++)
+                for (int x = 0; x < M; x++) hb[hoff(x, y, z)] = (TH)db[dpos(x, y, z)];
+        if (G < 1) continue;
+        // depth-1 face ghosts
+        for (int f = 0; f < ND; f++) {
+            int dim = f >> 1, dir = (f & 1) ? 1 : -1;
+            int t_ = c->h_nbr[(size_t)b * ND + f];
+            if (t_ == NB_BOUNDARY && !bcsign) continue;
+            for (int t = 0; t < c->F; t++) {
+                int cc[3] = {0, 0, 0};
+                int ta = t % M, tb = t / M;
+                if (dim == 0) { cc[1] = ta; cc[2] = tb; }
+                else if (dim == 1) { cc[0] = ta; cc[2] = tb; }
+                else { cc[0] = ta; cc[1] = tb; }
+                int g[3] = {cc[0], cc[1], cc[2]};
+                g[dim] = dir < 0 ? -1 : M;
+                TH v;
+                if (t_ >= 0) {
+                    int w[3] = {cc[0], cc[1], cc[2]};
+                    w[dim] = dir < 0 ? M - 1 : 0;
+                    v = (TH)data[(size_t)t_ * V + dpos(w[0], w[1], w[2])];
+                } else if (t_ == NB_BOUNDARY) {
+                    int w[3] = {cc[0], cc[1], cc[2]};
+                    w[dim] = dir < 0 ? 0 : M - 1;
+                    v = (TH)bcsign[f] * (TH)db[dpos(w[0], w[1], w[2])];
+                } else {
+                    v = (TH)ib[(size_t)nb_slot(t_) * c->F + t];
+                }
+                hb[hoff(g[0], g[1], g[2])] = v;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// solve drivers
+// ---------------------------------------------------------------------------
+
+struct Graph {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t x = nullptr;
+    ~Graph() {
+        if (x) (void)hipGraphExecDestroy(x);
+        if (g) (void)hipGraphDestroy(g);
+    }
+};
+
+// Run `body` (which only enqueues kernels on c->stream) either directly or via
+// a graph captured on its first use.
+template <class F>
+static void run_maybe_graph(srmg_ctx *c, Graph &gr, bool use_graph, F &&body) {
+    if (!use_graph) {
+        body();
+        return;
+    }
+    if (!gr.x) {
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        try {
+            body();
+        } catch (...) {
+            hipGraph_t tmp = nullptr;
+            (void)hipStreamEndCapture(c->stream, &tmp);
+            if (tmp) (void)hipGraphDestroy(tmp);
+            throw;
+        }
+        HIPCHK(hipStreamEndCapture(c->stream, &gr.g));
+        HIPCHK(hipGraphInstantiate(&gr.x, gr.g, nullptr, nullptr, 0));
+    }
+    HIPCHK(hipGraphLaunch(gr.x, c->stream));
+}
+
+static double read_sum(srmg_ctx *c) {
+    double v = 0.0;
+    HIPCHK(hipMemcpyAsync(&v, c->d_sum.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return std::sqrt(v);
+}
+
+static void check_opts(const srmg_solve_opts *o) {
+    if (!o) fail("null options");
+    if (o->max_cycles < 0 || o->ndown < 0 || o->nup < 0 || o->ntop < 0) fail("bad solve options");
+}
+
+static void solve_plain(srmg_ctx *c, int ui, int bi, int ri, int u1i, double s, const srmg_op *op,
+                        const srmg_solve_opts *o, int *cycles, double *hist) {
+    check_opts(o);
+    int t = same_type(c, {ui, bi, ri, u1i});
+    FieldBuf &u = field(c, ui), &b = field(c, bi), &r = field(c, ri), &u1 = field(c, u1i);
+    dispatch(c, t, [&](auto &ops) {
+        using T = std::remove_reference_t<decltype(*ops.X(u))>;
+        for (int l = 1; l < c->nlevels; l++) ops.fill_interp(u, l, level_args<T>(c, l - 1, s, op, 1.0));
+        ops.resnorm_partials(zero_field(c, t), b, s, op);
+        double r0 = read_sum(c);
+        if (!(r0 > 0.0)) r0 = 1.0;
+        ops.resnorm_partials(u, b, s, op);
+        double rel = read_sum(c) / r0;
+        if (hist) hist[0] = rel;
+        int k = 0;
+        Graph gr;
+        while (k < o->max_cycles && !(rel <= o->tol)) {
+            // the first cycle runs eagerly so lazily built state (coarse-solve
+            // matrices) exists before a graph is captured
+            run_maybe_graph(c, gr, o->use_graph && k > 0, [&] {
+                ops.vcycle(u, b, r, u1, s, op, 0, o->ndown, o->ntop, o->nup, o->omega);
+                ops.resnorm_partials(u, b, s, op);
+            });
+            rel = read_sum(c) / r0;
+            k++;
+            if (hist) hist[k] = rel;
+            if (!std::isfinite(rel)) fail("residual is not finite after cycle " + std::to_string(k));
+        }
+        if (cycles) *cycles = k;
+    });
+}
+
+// Mixed precision defect correction (FP64 residual and update, FP32 V-cycle on
+// the correction equation, FP64 coarsest-level solve).
+template <int D, int M>
+static void solve_mixed_dm(srmg_ctx *c, FieldBuf &u, FieldBuf &b, FieldBuf &e, FieldBuf &be, FieldBuf &re,
+                           FieldBuf &ue, double s, const srmg_op *op, const srmg_solve_opts *o, int *cycles,
+                           double *hist) {
+    using LY = Lay<D, M>;
+    Ops<D, M, double> o64(c);
+    Ops<D, M, float> o32(c);
+    for (int l = 1; l < c->nlevels; l++) o64.fill_interp(u, l, level_args<double>(c, l - 1, s, op, 1.0));
+    o64.resnorm_partials(zero_field(c, SRMG_F64), b, s, op);
+    double r0 = read_sum(c);
+    if (!(r0 > 0.0)) r0 = 1.0;
+    auto defect_and_norm = [&] {
+        for (int l = 0; l < c->nlevels; l++) {
+            LevelArgs<double> a = level_args<double>(c, l, s, op, 1.0);
+            if (!a.nblk) continue;
+            hipLaunchKernelGGL((k_defect<D, M>), dim3(nwg((int64_t)a.nblk * LY::V)), dim3(WG), 0, c->stream,
+                               be.data.as<float>(), u.data.as<double>(), b.data.as<double>(), u.ib.as<double>(),
+                               c->d_nbr.as<int>(), c->d_coord.as<int>(), a);
+        }
+        o64.resnorm_partials(u, b, s, op);
+    };
+    auto correct = [&] {
+        size_t n = (size_t)c->nb * LY::V, ni = e.ib.bytes / sizeof(float);
+        hipLaunchKernelGGL(k_fill<float>, dim3(nwg(n)), dim3(WG), 0, c->stream, e.data.as<float>(), 0.0f, n);
+        hipLaunchKernelGGL(k_fill<float>, dim3(nwg(ni)), dim3(WG), 0, c->stream, e.ib.as<float>(), 0.0f, ni);
+        o32.vcycle(e, be, re, ue, s, op, 0, o->ndown, o->ntop, o->nup, o->omega);
+        hipLaunchKernelGGL(k_add_correction, dim3(nwg(n)), dim3(WG), 0, c->stream, u.data.as<double>(),
+                           e.data.as<float>(), n);
+        for (int l = 1; l < c->nlevels; l++) o64.fill_interp(u, l, level_args<double>(c, l - 1, s, op, 1.0));
+        defect_and_norm();
+    };
+    defect_and_norm();
+    double rel = read_sum(c) / r0;
+    if (hist) hist[0] = rel;
+    int k = 0;
+    Graph gr;
+    while (k < o->max_cycles && !(rel <= o->tol)) {
+        run_maybe_graph(c, gr, o->use_graph && k > 0, correct);
+        rel = read_sum(c) / r0;
+        k++;
+        if (hist) hist[k] = rel;
+        if (!std::isfinite(rel)) fail("residual is not finite after cycle " + std::to_string(k));
+    }
+    if (cycles) *cycles = k;
+}
+
+static void solve_mixed(srmg_ctx *c, int ui, int bi, int ei, int bei, int rei, int uei, double s,
+                        const srmg_op *op, const srmg_solve_opts *o, int *cycles, double *hist) {
+    check_opts(o);
+    if (same_type(c, {ui, bi}) != SRMG_F64) fail("u and b must be FP64");
+    if (same_type(c, {ei, bei, rei, uei}) != SRMG_F32) fail("correction fields must be FP32");
+    FieldBuf &u = field(c, ui), &b = field(c, bi), &e = field(c, ei), &be = field(c, bei), &re = field(c, rei),
+             &ue = field(c, uei);
+#define SRMG_MCASE(DD, MM) \
+    if (c->D == DD && c->M == MM) return solve_mixed_dm<DD, MM>(c, u, b, e, be, re, ue, s, op, o, cycles, hist);
+    SRMG_MCASE(2, 16)
+    SRMG_MCASE(2, 8)
+    SRMG_MCASE(3, 8)
+    SRMG_MCASE(3, 16)
+#undef SRMG_MCASE
+    fail("unsupported (D, M)");
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+const char *srmg_last_error(void) { return g_err.c_str(); }
+
+int srmg_create(srmg_ctx **out, int D, int M) {
+    return guarded([&] {
+        if (!out) fail("null output pointer");
+        if (!((D == 2 || D == 3) && (M == 8 || M == 16))) fail("supported: D in {2,3}, M in {8,16}");
+        auto c = std::make_unique<srmg_ctx>();
+        c->D = D;
+        c->M = M;
+        c->V = (D == 2) ? M * M : M * M * M;
+        c->F = (D == 2) ? M : M * M;
+        HIPCHK(hipStreamCreate(&c->stream));
+        *out = c.release();
+    });
+}
+
+int srmg_destroy(srmg_ctx *c) {
+    return guarded([&] {
+        if (!c) return;
+        if (c->stream) (void)hipStreamSynchronize(c->stream);
+        c->fields.clear();
+        hipStream_t s = c->stream;
+        delete c;
+        if (s) (void)hipStreamDestroy(s);
+    });
+}
+
+int srmg_set_tree(srmg_ctx *c, int nlevels, int64_t nblocks, const int64_t *pairs, const int64_t *pairs_off,
+                  const int64_t *neighbor, const int64_t *neighbor_off, const int64_t *boundary,
+                  const int64_t *boundary_off, const int64_t *refboundary, const int64_t *refboundary_off,
+                  const int64_t *child, const int64_t *child_off) {
+    return guarded([&] {
+        if (!c) fail("null context");
+        set_tree(c, nlevels, nblocks, pairs, pairs_off, neighbor, neighbor_off, boundary, boundary_off, refboundary,
+                 refboundary_off, child, child_off);
+    });
+}
+
+int srmg_field_create(srmg_ctx *c, int dtype, int *id) {
+    return guarded([&] {
+        if (!c || !id) fail("null argument");
+        if (dtype != SRMG_F64 && dtype != SRMG_F32) fail("dtype must be SRMG_F64 or SRMG_F32");
+        auto f = std::make_unique<FieldBuf>();
+        f->dtype = dtype;
+        if (c->nb) alloc_field(c, *f);
+        c->fields.push_back(std::move(f));
+        *id = (int)c->fields.size() - 1;
+    });
+}
+
+int srmg_field_destroy(srmg_ctx *c, int id) {
+    return guarded([&] {
+        field(c, id);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->fields[id].reset();
+    });
+}
+
+int srmg_zero(srmg_ctx *c, int id) {
+    return guarded([&] {
+        FieldBuf &f = field(c, id);
+        check_tree(c);
+        HIPCHK(hipMemsetAsync(f.data.p, 0, f.data.bytes, c->stream));
+        HIPCHK(hipMemsetAsync(f.ib.p, 0, f.ib.bytes, c->stream));
+    });
+}
+
+int srmg_upload(srmg_ctx *c, int id, const void *host, int host_dtype, int G) {
+    return guarded([&] {
+        FieldBuf &f = field(c, id);
+        check_tree(c);
+        if (!host) fail("null host pointer");
+        if (G < 1) fail("host ghost depth G must be >= 1");
+        auto go = [&](auto th) {
+            using TH = decltype(th);
+            const TH *h = static_cast<const TH *>(host);
+            if (f.dtype == SRMG_F64) {
+                std::vector<double> data, ib;
+                to_device_layout(c, h, G, data, ib);
+                HIPCHK(hipMemcpy(f.data.p, data.data(), data.size() * 8, hipMemcpyHostToDevice));
+                HIPCHK(hipMemcpy(f.ib.p, ib.data(), ib.size() * 8, hipMemcpyHostToDevice));
+            } else {
+                std::vector<float> data, ib;
+                to_device_layout(c, h, G, data, ib);
+                HIPCHK(hipMemcpy(f.data.p, data.data(), data.size() * 4, hipMemcpyHostToDevice));
+                HIPCHK(hipMemcpy(f.ib.p, ib.data(), ib.size() * 4, hipMemcpyHostToDevice));
+            }
+        };
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (host_dtype == SRMG_F64) go(double());
+        else if (host_dtype == SRMG_F32) go(float());
+        else fail("bad host dtype");
+    });
+}
+
+static int download_impl(srmg_ctx *c, int id, void *host, int host_dtype, int G, const int8_t *bc) {
+    return guarded([&] {
+        FieldBuf &f = field(c, id);
+        check_tree(c);
+        if (!host) fail("null host pointer");
+        if (G < 0) fail("bad G");
+        HIPCHK(hipStreamSynchronize(c->stream));
+        int8_t sg[6] = {1, 1, 1, 1, 1, 1};
+        if (bc)
+            for (int d = 0; d < c->D; d++)
+                for (int side = 0; side < 2; side++) {
+                    int idx = 0, m = 1;
+                    for (int e = 0; e < c->D; e++) {
+                        idx += ((e == d ? (side ? 1 : -1) : 0) + 1) * m;
+                        m *= 3;
+                    }
+                    sg[2 * d + side] = bc[idx];
+                }
+        auto go = [&](auto td) {
+            using TD = decltype(td);
+            std::vector<TD> data(f.data.bytes / sizeof(TD)), ib(f.ib.bytes / sizeof(TD));
+            HIPCHK(hipMemcpy(data.data(), f.data.p, f.data.bytes, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(ib.data(), f.ib.p, f.ib.bytes, hipMemcpyDeviceToHost));
+            if (host_dtype == SRMG_F64)
+                from_device_layout(c, data, ib, static_cast<double *>(host), G, bc ? sg : nullptr);
+            else
+                from_device_layout(c, data, ib, static_cast<float *>(host), G, bc ? sg : nullptr);
+        };
+        if (host_dtype != SRMG_F64 && host_dtype != SRMG_F32) fail("bad host dtype");
+        if (f.dtype == SRMG_F64) go(double());
+        else go(float());
+    });
+}
+
+int srmg_download(srmg_ctx *c, int id, void *host, int host_dtype, int G) {
+    return download_impl(c, id, host, host_dtype, G, nullptr);
+}
+
+int srmg_download_bc(srmg_ctx *c, int id, void *host, int host_dtype, int G, const int8_t *bc) {
+    return download_impl(c, id, host, host_dtype, G, bc);
+}
+
+int srmg_fill_ghost(srmg_ctx *c, int id, int level, const int8_t *bc) {
+    return guarded([&] {
+        FieldBuf &f = field(c, id);
+        check_tree(c);
+        if (level < 1 || level > c->nlevels) fail("level out of range");
+        if (level == 1) return;
+        srmg_op op{};
+        std::memcpy(op.bc, bc, 27);
+        dispatch(c, f.dtype, [&](auto &o) {
+            using T = std::remove_reference_t<decltype(*o.X(f))>;
+            o.fill_interp(f, level - 1, level_args<T>(c, level - 2, 1.0, &op, 1.0));
+        });
+    });
+}
+
+int srmg_fill_ghost_all(srmg_ctx *c, int id, const int8_t *bc) {
+    return guarded([&] {
+        FieldBuf &f = field(c, id);
+        check_tree(c);
+        srmg_op op{};
+        std::memcpy(op.bc, bc, 27);
+        dispatch(c, f.dtype, [&](auto &o) {
+            using T = std::remove_reference_t<decltype(*o.X(f))>;
+            for (int l = 1; l < c->nlevels; l++) o.fill_interp(f, l, level_args<T>(c, l - 1, 1.0, &op, 1.0));
+        });
+    });
+}
+
+int srmg_vcycle(srmg_ctx *c, int u, int b, int r, int u1, double s, const srmg_op *op, int lmax, int ndown,
+                int ntop, int nup, double omega) {
+    return guarded([&] {
+        check_tree(c);
+        if (!op) fail("null op");
+        if (lmax < 0 || lmax > c->nlevels) fail("lmax out of range");
+        int t = same_type(c, {u, b, r, u1});
+        dispatch(c, t, [&](auto &o) {
+            o.vcycle(field(c, u), field(c, b), field(c, r), field(c, u1), s, op, lmax, ndown, ntop, nup, omega);
+        });
+        HIPCHK(hipGetLastError());
+    });
+}
+
+int srmg_fmg(srmg_ctx *c, int u, int b, int r, int u1, double s, const srmg_op *op, int ndown, int ntop, int nup,
+             double omega) {
+    return guarded([&] {
+        check_tree(c);
+        if (!op) fail("null op");
+        int t = same_type(c, {u, b, r, u1});
+        dispatch(c, t, [&](auto &o) {
+            o.fmg(field(c, u), field(c, b), field(c, r), field(c, u1), s, op, ndown, ntop, nup, omega);
+        });
+        HIPCHK(hipGetLastError());
+    });
+}
+
+int srmg_resnorm(srmg_ctx *c, int r, int b, int u, double s, const srmg_op *op, double *out) {
+    return guarded([&] {
+        check_tree(c);
+        if (!op || !out) fail("null argument");
+        (void)r;
+        int t = same_type(c, {u, b});
+        dispatch(c, t, [&](auto &o) { o.resnorm_partials(field(c, u), field(c, b), s, op); });
+        double v = 0.0;
+        HIPCHK(hipMemcpyAsync(&v, c->d_sum.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        *out = std::sqrt(v);
+    });
+}
+
+int srmg_synchronize(srmg_ctx *c) {
+    return guarded([&] { HIPCHK(hipStreamSynchronize(c->stream)); });
+}
+
+int srmg_solve(srmg_ctx *c, int u, int b, int r, int u1, double s, const srmg_op *op,
+               const srmg_solve_opts *opts, int *cycles, double *hist) {
+    return guarded([&] {
+        check_tree(c);
+        if (!op) fail("null op");
+        solve_plain(c, u, b, r, u1, s, op, opts, cycles, hist);
+    });
+}
+
+int srmg_solve_mixed(srmg_ctx *c, int u, int b, int e, int be, int re, int ue, double s, const srmg_op *op,
+                     const srmg_solve_opts *opts, int *cycles, double *hist) {
+    return guarded([&] {
+        check_tree(c);
+        if (!op) fail("null op");
+        solve_mixed(c, u, b, e, be, re, ue, s, op, opts, cycles, hist);
+    });
+}
+
+int srmg_time_kernel(srmg_ctx *c, int kind, int level, int u, int b, int r, double s, const srmg_op *op,
+                     int reps, double *ms_per_launch) {
+    return guarded([&] {
+        check_tree(c);
+        if (!op || !ms_per_launch || reps < 1) fail("bad arguments");
+        if (level < 1 || level > c->nlevels) fail("level out of range");
+        int t = same_type(c, {u, b, r});
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        float ms = 0.f;
+        dispatch(c, t, [&](auto &o) {
+            using T = std::remove_reference_t<decltype(*o.X(field(c, u)))>;
+            LevelArgs<T> a = level_args<T>(c, level - 1, s, op, 1.0);
+            auto launch = [&](int i) {
+                if (kind == SRMG_KERNEL_GS_PASS) o.gs_pass(field(c, u), field(c, b), a, i & 1);
+                else if (kind == SRMG_KERNEL_RESIDUAL) o.residual(field(c, r), field(c, u), field(c, b), a);
+                else fail("unknown kernel kind");
+            };
+            launch(0);
+            launch(1);
+            HIPCHK(hipEventRecord(e0, c->stream));
+            for (int i = 0; i < reps; i++) launch(i);
+            HIPCHK(hipEventRecord(e1, c->stream));
+            HIPCHK(hipEventSynchronize(e1));
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        });
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *ms_per_launch = ms / reps;
+    });
+}
+
+int srmg_level_info(srmg_ctx *c, int level, int64_t *nblocks, int64_t *nleaf, int64_t *nslots) {
+    return guarded([&] {
+        check_tree(c);
+        if (level < 1 || level > c->nlevels) fail("level out of range");
+        const LevelInfo &L = c->lv[level - 1];
+        if (nblocks) *nblocks = L.nblk;
+        if (nleaf) *nleaf = L.nleaf;
+        if (nslots) *nslots = L.nslot;
+    });
+}
+
+int srmg_axis_decomp(int n, int radial, double alow, double ahigh, double *lam, double *P, double *Q) {
+    return guarded([&] {
+        if (n < 1 || !lam || !P || !Q) fail("bad arguments");
+        axis_decomp(n, radial != 0, alow, ahigh, lam, P, Q);
+    });
+}
+
+}  // extern "C"

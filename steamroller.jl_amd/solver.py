@@ -1,0 +1,331 @@
+"""ctypes binding of libsrmg.so and a Python mirror of the reference's solver API.
+
+The reference's operator plug-in is Julia dispatch on a discretization and a
+geometry type (src/laplacian.jl:9-14, src/helmholtz.jl:9-11, src/geometry.jl:14-20);
+the same values are carried here by small descriptor classes and passed through
+the C ABI as an ``srmg_op``. Fields are device-resident (:class:`DeviceField`);
+the free functions ``vcycle``/``fmg``/``resnorm``/``fill_ghost`` keep the
+reference's argument order and meaning (src/multigrid.jl:17-106, 391-417;
+src/connectivity.jl:155-159).
+
+There is no CPU fallback: if the HIP library is missing or no GPU is present
+the calls raise.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from .tree import flatten_pairs, flatten_records
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsrmg.so")
+
+F64, F32 = 0, 1
+KERNEL_GS_PASS, KERNEL_RESIDUAL = 0, 1
+COARSE_GS, COARSE_EXACT = 0, 1
+_DT = {F64: np.float64, F32: np.float32}
+
+
+class SrmgError(RuntimeError):
+    pass
+
+
+# ---- operator descriptors (mirror the reference's types) ----
+
+@dataclass(frozen=True)
+class CartesianGeometry:
+    code: int = 0
+
+
+@dataclass(frozen=True)
+class CylindricalGeometry:
+    """``CylindricalGeometry{N}``: dimension N (1-based) is the radius."""
+    N: int = 1
+
+    @property
+    def code(self) -> int:
+        return self.N
+
+
+@dataclass(frozen=True)
+class LaplacianDiscretization:
+    D: int
+    P: int = 2
+    code: int = 0
+    k2: float = 0.0
+
+
+@dataclass(frozen=True)
+class HelmholtzDiscretization:
+    D: int
+    P: int
+    k2: float
+    code: int = 1
+
+
+def geometry_from_code(code: int):
+    return CartesianGeometry() if code == 0 else CylindricalGeometry(code)
+
+
+class _Op(C.Structure):
+    _fields_ = [("geometry", C.c_int32), ("disc", C.c_int32), ("k2", C.c_double),
+                ("bc", C.c_int8 * 27), ("coarse", C.c_int32)]
+
+
+class _SolveOpts(C.Structure):
+    _fields_ = [("ndown", C.c_int32), ("ntop", C.c_int32), ("nup", C.c_int32),
+                ("omega", C.c_double), ("tol", C.c_double), ("max_cycles", C.c_int32),
+                ("use_graph", C.c_int32)]
+
+
+def make_op(geometry, bc, disc, coarse: int = COARSE_GS) -> _Op:
+    if getattr(disc, "P", 2) != 2:
+        raise SrmgError("only order-2 discretizations are implemented on the device")
+    op = _Op()
+    op.geometry = int(geometry.code)
+    op.disc = int(disc.code)
+    op.k2 = float(disc.k2)
+    b = np.asarray(bc, np.int8).ravel()
+    for i in range(len(b)):
+        op.bc[i] = int(b[i])
+    op.coarse = int(coarse)
+    return op
+
+
+# ---- library loading ----
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise SrmgError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    P, i32, i64, dbl = C.c_void_p, C.c_int, C.c_int64, C.c_double
+    sig = {
+        "srmg_create": [C.POINTER(P), i32, i32],
+        "srmg_destroy": [P],
+        "srmg_set_tree": [P, i32, i64] + [P] * 10,
+        "srmg_field_create": [P, i32, C.POINTER(i32)],
+        "srmg_field_destroy": [P, i32],
+        "srmg_upload": [P, i32, P, i32, i32],
+        "srmg_download": [P, i32, P, i32, i32],
+        "srmg_download_bc": [P, i32, P, i32, i32, P],
+        "srmg_zero": [P, i32],
+        "srmg_fill_ghost": [P, i32, i32, P],
+        "srmg_fill_ghost_all": [P, i32, P],
+        "srmg_vcycle": [P, i32, i32, i32, i32, dbl, P, i32, i32, i32, i32, dbl],
+        "srmg_fmg": [P, i32, i32, i32, i32, dbl, P, i32, i32, i32, dbl],
+        "srmg_resnorm": [P, i32, i32, i32, dbl, P, C.POINTER(dbl)],
+        "srmg_synchronize": [P],
+        "srmg_solve": [P, i32, i32, i32, i32, dbl, P, P, C.POINTER(i32), P],
+        "srmg_solve_mixed": [P, i32, i32, i32, i32, i32, i32, dbl, P, P, C.POINTER(i32), P],
+        "srmg_axis_decomp": [i32, i32, dbl, dbl, P, P, P],
+        "srmg_time_kernel": [P, i32, i32, i32, i32, i32, dbl, P, i32, C.POINTER(dbl)],
+        "srmg_level_info": [P, i32, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64)],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    L.srmg_last_error.restype = C.c_char_p
+    L.srmg_last_error.argtypes = []
+    _lib = L
+    return L
+
+
+def _check(rc: int):
+    if rc != 0:
+        msg = lib().srmg_last_error().decode(errors="replace")
+        raise SrmgError(f"srmg error {rc}: {msg}")
+
+
+def _p(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def axis_decomp(n: int, radial: bool, alow: float, ahigh: float):
+    """Host-only: the level-1 1D operator as P diag(lam) Q (no device work)."""
+    lam = np.zeros(n)
+    P = np.zeros((n, n))
+    Q = np.zeros((n, n))
+    _check(lib().srmg_axis_decomp(n, int(radial), alow, ahigh, _p(lam), _p(P), _p(Q)))
+    return lam, P, Q
+
+
+# ---- device objects ----
+
+class Context:
+    """One device context bound to a tree (``srmg_create`` + ``srmg_set_tree``)."""
+
+    def __init__(self, tree, conn, M: int):
+        self.D, self.M = tree.D, M
+        self.tree, self.conn = tree, conn
+        h = C.c_void_p()
+        _check(lib().srmg_create(C.byref(h), tree.D, M))
+        self.h = h
+        self.regrid(tree, conn)
+
+    def regrid(self, tree, conn):
+        D = tree.D
+        pairs, poff = flatten_pairs(tree)
+        arrs = [pairs, poff]
+        for lists, w in ((conn.neighbor, 2 + D), (conn.boundary, 2 + 2 * D),
+                         (conn.refboundary, 2 + 2 * D), (conn.child, 2 + D)):
+            a, o = flatten_records(lists, w)
+            if len(a) == 0:
+                a = np.zeros((1, w), np.int64)
+            arrs += [a, o]
+        self._tables = arrs
+        _check(lib().srmg_set_tree(self.h, len(tree), tree.nblocks(), *[_p(a) for a in arrs]))
+        self.tree, self.conn = tree, conn
+        self.nblocks = tree.nblocks()
+
+    def level_info(self, level: int):
+        """(blocks, leaf blocks, coarse-fine ghost slabs) of a 1-based level."""
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        _check(lib().srmg_level_info(self.h, level, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def time_kernel(self, kind: int, level: int, u, b, r, s, geometry, bc, disc, reps=20) -> float:
+        """Average ms per launch of a level kernel (HIP events on the context stream)."""
+        op = make_op(geometry, bc, disc)
+        ms = C.c_double()
+        _check(lib().srmg_time_kernel(self.h, kind, level, u.id, b.id, r.id, float(s), C.byref(op),
+                                      int(reps), C.byref(ms)))
+        return ms.value
+
+    def field(self, dtype: int = F64) -> "DeviceField":
+        i = C.c_int()
+        _check(lib().srmg_field_create(self.h, dtype, C.byref(i)))
+        return DeviceField(self, i.value, dtype)
+
+    def synchronize(self):
+        _check(lib().srmg_synchronize(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().srmg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceField:
+    def __init__(self, ctx: Context, fid: int, dtype: int):
+        self.ctx, self.id, self.dtype = ctx, fid, dtype
+
+    def upload(self, host) -> "DeviceField":
+        a = host.u if hasattr(host, "u") else host
+        if not a.flags.f_contiguous:
+            raise SrmgError("host field must be Fortran-ordered (the reference's layout)")
+        G = (a.shape[0] - self.ctx.M) // 2
+        hd = F64 if a.dtype == np.float64 else F32
+        if a.dtype not in (np.float64, np.float32):
+            raise SrmgError("host field must be float64 or float32")
+        _check(lib().srmg_upload(self.ctx.h, self.id, _p(a), hd, G))
+        return self
+
+    def download(self, host, bc=None):
+        a = host.u if hasattr(host, "u") else host
+        if not a.flags.f_contiguous:
+            raise SrmgError("host field must be Fortran-ordered (the reference's layout)")
+        G = (a.shape[0] - self.ctx.M) // 2
+        hd = F64 if a.dtype == np.float64 else F32
+        if bc is None:
+            _check(lib().srmg_download(self.ctx.h, self.id, _p(a), hd, G))
+        else:
+            bcv = np.ascontiguousarray(np.asarray(bc, np.int8).ravel())
+            _check(lib().srmg_download_bc(self.ctx.h, self.id, _p(a), hd, G, _p(bcv)))
+        return host
+
+    def zero(self):
+        _check(lib().srmg_zero(self.ctx.h, self.id))
+        return self
+
+
+# ---- the reference's entry points ----
+
+def _bc_arr(bc):
+    return np.ascontiguousarray(np.asarray(bc, np.int8).ravel())
+
+
+def fill_ghost(u: DeviceField, l: int, conn=None, bc=None):
+    """fill_ghost!(u, l, conn, bc) (src/connectivity.jl:155-159)."""
+    b = _bc_arr(bc)
+    _check(lib().srmg_fill_ghost(u.ctx.h, u.id, l, _p(b)))
+
+
+def fill_ghost_all(u: DeviceField, bc):
+    b = _bc_arr(bc)
+    _check(lib().srmg_fill_ghost_all(u.ctx.h, u.id, _p(b)))
+
+
+def vcycle(u, b, r, u1, s, tree=None, conn=None, geometry=None, bc=None, lpldisc=None, *,
+           lmax=0, ndown=3, ntop=5, nup=3, omega=1.0, coarse=COARSE_GS):
+    """vcycle! (src/multigrid.jl:60-106) on device-resident fields."""
+    op = make_op(geometry, bc, lpldisc, coarse)
+    _check(lib().srmg_vcycle(u.ctx.h, u.id, b.id, r.id, u1.id, float(s), C.byref(op),
+                             int(lmax), int(ndown), int(ntop), int(nup), float(omega)))
+
+
+def fmg(u, b, r, u1, s, tree=None, conn=None, geometry=None, bc=None, lpldisc=None, *,
+        lmax=0, ndown=3, ntop=5, nup=3, omega=1.25, coarse=COARSE_GS):
+    """fmg! (src/multigrid.jl:17-51) on device-resident fields."""
+    op = make_op(geometry, bc, lpldisc, coarse)
+    _check(lib().srmg_fmg(u.ctx.h, u.id, b.id, r.id, u1.id, float(s), C.byref(op),
+                          int(ndown), int(ntop), int(nup), float(omega)))
+
+
+def resnorm(r, b, u, s, tree=None, conn=None, geometry=None, bc=None, lpldisc=None, *,
+            coarse=COARSE_GS) -> float:
+    """resnorm! (src/multigrid.jl:391-417)."""
+    op = make_op(geometry, bc, lpldisc, coarse)
+    out = C.c_double()
+    _check(lib().srmg_resnorm(u.ctx.h, r.id, b.id, u.id, float(s), C.byref(op), C.byref(out)))
+    return out.value
+
+
+def _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph):
+    o = _SolveOpts()
+    o.ndown, o.ntop, o.nup = int(ndown), int(ntop), int(nup)
+    o.omega, o.tol, o.max_cycles, o.use_graph = float(omega), float(tol), int(max_cycles), int(use_graph)
+    return o
+
+
+def solve(u, b, r, u1, s, geometry, bc, lpldisc, *, tol=1e-8, max_cycles=50, ndown=2, ntop=4,
+          nup=2, omega=1.0, coarse=COARSE_EXACT, use_graph=True):
+    """Pre-fill ghosts, then V-cycles until resnorm(u) <= tol * resnorm(0).
+    Returns (cycles, relative residual history)."""
+    op = make_op(geometry, bc, lpldisc, coarse)
+    o = _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph)
+    hist = np.zeros(max_cycles + 1)
+    n = C.c_int()
+    _check(lib().srmg_solve(u.ctx.h, u.id, b.id, r.id, u1.id, float(s), C.byref(op), C.byref(o),
+                            C.byref(n), _p(hist)))
+    return n.value, hist[: n.value + 1].copy()
+
+
+def solve_mixed(u, b, e, be, re, ue, s, geometry, bc, lpldisc, *, tol=1e-8, max_cycles=50,
+                ndown=2, ntop=4, nup=2, omega=1.0, coarse=COARSE_EXACT, use_graph=True):
+    """FP64 defect correction around FP32 V-cycles (u, b FP64; e, be, re, ue FP32)."""
+    op = make_op(geometry, bc, lpldisc, coarse)
+    o = _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph)
+    hist = np.zeros(max_cycles + 1)
+    n = C.c_int()
+    _check(lib().srmg_solve_mixed(u.ctx.h, u.id, b.id, e.id, be.id, re.id, ue.id, float(s),
+                                  C.byref(op), C.byref(o), C.byref(n), _p(hist)))
+    return n.value, hist[: n.value + 1].copy()
