@@ -148,8 +148,8 @@ def main():
                    "blocks": p.tree.nblocks(), "leaf_cells": n_leaf, "stored_cells": n_stored,
                    "seed": args.seed, "vcycle": "(ndown, ntop, nup) = (2, 4, 2), exact FP64 level-1 solve"},
         "forms": {
-            "fp64": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in forms["fp64"].items()},
-            "fp32_mixed": {k: (round(v, 6) if isinstance(v, float) else v) for k, v in forms["mixed"].items()},
+            "fp64": {k: (float(f"{v:.5g}") if isinstance(v, float) else v) for k, v in forms["fp64"].items()},
+            "fp32_mixed": {k: (float(f"{v:.5g}") if isinstance(v, float) else v) for k, v in forms["mixed"].items()},
             "speedup_mixed_vs_fp64": round(forms["fp64"]["ms"] / forms["mixed"]["ms"], 3),
         },
         "roofline": {

@@ -938,6 +938,9 @@ static void solve_mixed_dm(srmg_ctx *c, FieldBuf &u, FieldBuf &b, FieldBuf &e, F
         o32.vcycle(e, be, re, ue, s, op, 0, o->ndown, o->ntop, o->nup, o->omega);
         hipLaunchKernelGGL(k_add_correction, dim3(nwg(n)), dim3(WG), 0, c->stream, u.data.as<double>(),
                            e.data.as<float>(), n);
+        // covered coarse cells = restriction of their children, as at the FAS
+        // fixed point; otherwise the leaf residual would not pin them
+        for (int l = c->nlevels - 1; l >= 1; l--) o64.restrict_(u, l);
         for (int l = 1; l < c->nlevels; l++) o64.fill_interp(u, l, level_args<double>(c, l - 1, s, op, 1.0));
         defect_and_norm();
     };
