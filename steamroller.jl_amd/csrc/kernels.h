@@ -139,10 +139,11 @@ __global__ __launch_bounds__(WG) void k_residual(T *__restrict__ r, const T *__r
 
 // restrict! over the Child edges of a level (src/restrict.jl:5-37): each
 // coarse cell of the parent subblock = mean of its 2^D children, summed with
-// the first index fastest.
+// the first index fastest. xf holds the fine level, xc the coarse one.
 template <int D, int M, typename T>
-__global__ __launch_bounds__(WG) void k_restrict(T *__restrict__ x, const int *__restrict__ parent,
-                                                 const int *__restrict__ sub, int blk0, int nblk) {
+__global__ __launch_bounds__(WG) void k_restrict(T *__restrict__ xc, const T *__restrict__ xf,
+                                                 const int *__restrict__ parent, const int *__restrict__ sub,
+                                                 int blk0, int nblk) {
     using LY = Lay<D, M>;
     constexpr int NS = LY::V / LY::NC;  // coarse cells per subblock
     int gt = blockIdx.x * blockDim.x + threadIdx.x;
@@ -155,13 +156,13 @@ __global__ __launch_bounds__(WG) void k_restrict(T *__restrict__ x, const int *_
 #pragma unroll
     for (int q = 0; q < LY::NC; q++) {
         int i0 = q & 1, i1 = (q >> 1) & 1, i2 = (q >> 2) & 1;
-        f += x[(size_t)fb * LY::V + LY::sidx(2 * j0 + i0, 2 * j1 + i1, D == 3 ? 2 * j2 + i2 : 0)];
+        f += xf[(size_t)fb * LY::V + LY::sidx(2 * j0 + i0, 2 * j1 + i1, D == 3 ? 2 * j2 + i2 : 0)];
     }
     int pb = parent[fb];
     int sb = sub[fb];
     int X0 = (sb & 1) * LY::MH + j0, X1 = ((sb >> 1) & 1) * LY::MH + j1;
     int X2 = (D == 3) ? ((sb >> 2) & 1) * LY::MH + j2 : 0;
-    x[(size_t)pb * LY::V + LY::sidx(X0, X1, X2)] = f / T(LY::NC);
+    xc[(size_t)pb * LY::V + LY::sidx(X0, X1, X2)] = f / T(LY::NC);
 }
 
 // residual_postrestrict_level! (src/multigrid.jl:112-119, 331-344): FAS
@@ -288,8 +289,9 @@ __global__ __launch_bounds__(WG) void k_fill_interp(T *__restrict__ ib, const T 
 }
 
 // Squared-residual sum over the leaf blocks of a level, one partial per work
-// group (deterministic LDS tree); `wgt` = 2^(-D(l-1)). Equals what resnorm!
-// obtains by restricting squared residuals down to level 1 (src/multigrid.jl:391-417).
+// group (grid-stride loop over cells in a fixed order, then an LDS tree);
+// `wgt` = 2^(-D(l-1)). Equals what resnorm! obtains by restricting squared
+// residuals down to level 1 (src/multigrid.jl:391-417).
 template <int D, int M, typename T>
 __global__ __launch_bounds__(WG) void k_resnorm(const T *__restrict__ u, const T *__restrict__ b,
                                                 const T *__restrict__ ib, const int *__restrict__ nbr,
@@ -298,12 +300,12 @@ __global__ __launch_bounds__(WG) void k_resnorm(const T *__restrict__ u, const T
                                                 double *__restrict__ partial) {
     using LY = Lay<D, M>;
     __shared__ double red[WG];
-    int gt = blockIdx.x * blockDim.x + threadIdx.x;
-    int lb = gt / LY::V;
-    double val = 0.0;
-    if (lb < nleaf) {
+    double acc = 0.0;
+    const size_t total = (size_t)nleaf * LY::V;
+    for (size_t gt = (size_t)blockIdx.x * WG + threadIdx.x; gt < total; gt += (size_t)gridDim.x * WG) {
+        int lb = (int)(gt / LY::V);
         int blk = leaves[lb];
-        int q = gt - lb * LY::V;
+        int q = (int)(gt - (size_t)lb * LY::V);
         int c0, c1, c2;
         LY::decode(q, c0, c1, c2);
         size_t o = (size_t)blk * LY::V + q;
@@ -311,9 +313,9 @@ __global__ __launch_bounds__(WG) void k_resnorm(const T *__restrict__ u, const T
         radial_weights<D, M, T>(a, coord, blk, c0, c1, wm, wp);
         T Lu = stencil<D, M, T>(u, ib, nbr, blk, c0, c1, c2, u[o], a, wm, wp);
         double r = (double)(-(b[o] + Lu / a.s));
-        val = r * r;
+        acc += r * r;
     }
-    red[threadIdx.x] = val;
+    red[threadIdx.x] = acc;
     __syncthreads();
 #pragma unroll
     for (int w = WG / 2; w > 0; w >>= 1) {
@@ -335,6 +337,237 @@ __global__ __launch_bounds__(WG) void k_sum(const double *__restrict__ partial, 
         __syncthreads();
     }
     if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// ---- fused red-black sweep ----
+
+// Work-group -> tile mapping that keeps consecutive tiles (Morton order) on one
+// XCD: the hardware hands work group i to XCD i mod 8, so XCD x receives a
+// contiguous range of tiles and neighbouring blocks meet in its L2.
+__device__ __forceinline__ int xcd_remap(int i, int G) {
+    constexpr int NX = 8;
+    int x = i % NX, local = i / NX;
+    int per = G / NX, rem = G % NX;
+    return (x < rem) ? x * (per + 1) + local : rem * (per + 1) + (x - rem) * per + local;
+}
+
+template <int D, int M>
+struct Tile {
+    static constexpr int S = M + 2;
+    static constexpr int SZ = (D == 2) ? S * S : S * S * S;
+    __device__ static __forceinline__ int at(int x, int y, int z) {
+        return (x + 1) + S * ((y + 1) + (D == 3 ? S * (z + 1) : 0));
+    }
+    __device__ static __forceinline__ int at3(const int *c) { return at(c[0], c[1], D == 3 ? c[2] : 0); }
+    __device__ static constexpr int stride(int d) { return d == 0 ? 1 : (d == 1 ? S : S * S); }
+};
+
+// stencil of the tile cell at p (all neighbours present in the tile)
+template <int D, int M, typename T>
+__device__ __forceinline__ T tile_stencil(const T *t, int p, T own, const LevelArgs<T> &a, T wmr, T wpr) {
+    using TL = Tile<D, M>;
+    T v = T(0);
+#pragma unroll
+    for (int d = D - 1; d >= 0; d--) v += ((d == a.rdim) ? wmr : T(1)) * t[p - TL::stride(d)];
+    v += a.cc * own;
+#pragma unroll
+    for (int d = 0; d < D; d++) v += ((d == a.rdim) ? wpr : T(1)) * t[p + TL::stride(d)];
+    return v;
+}
+
+template <int D, int M, typename T>
+__device__ __forceinline__ void weights_at(const LevelArgs<T> &a, const int *__restrict__ coord, int blk,
+                                           const int *c, T &wm, T &wp) {
+    wm = T(1);
+    wp = T(1);
+    if (a.rdim >= 0) {
+        int J = M * (coord[blk * D + a.rdim] - 1) + c[a.rdim] + 1;
+        wm = a.wm[J];
+        wp = a.wp[J];
+    }
+}
+
+// One full gauss_seidel_iter! sweep (src/multigrid.jl:250-265: RedBlack{0} on
+// all blocks, copy/boundary ghost fill, RedBlack{1}, fill) in one launch.
+// Reads ui, writes uo (ping-pong, so neighbours always see the old values).
+// Each work group stages BPW blocks with a depth-1 face halo in LDS. The
+// second half-sweep needs the neighbours' face-layer cells after the first
+// half-sweep; the work group recomputes them from the same inputs with the
+// same operations as their owners, so the result is identical to the
+// two-launch form. Boundary ghosts mirror the cell they face, and coarse-fine
+// ghosts stay frozen during smoothing (src/multigrid.jl:256-263).
+template <int D, int M, typename T, int BPW, int NT>
+__global__ __launch_bounds__(NT) void k_gs_sweep(T *__restrict__ uo, const T *__restrict__ ui,
+                                                 const T *__restrict__ b, const T *__restrict__ ib,
+                                                 const int *__restrict__ nbr, const int *__restrict__ coord,
+                                                 LevelArgs<T> a) {
+    using LY = Lay<D, M>;
+    using TL = Tile<D, M>;
+    constexpr int F = LY::F, V = LY::V, H = LY::H, MH = LY::MH;
+    static_assert((BPW * H) % NT == 0, "cells per colour must be a multiple of the work-group size");
+    constexpr int CPT = BPW * H / NT;
+    __shared__ T tile[BPW * TL::SZ];
+    const int lb0 = xcd_remap(blockIdx.x, gridDim.x) * BPW;
+    const int tid = threadIdx.x;
+    const int nloc = min(BPW, a.nblk - lb0);
+
+    // interior cells
+    for (int i = tid; i < nloc * V; i += NT) {
+        int lb = i / V, q = i - lb * V;
+        int c[3] = {0, 0, 0};
+        LY::decode(q, c[0], c[1], c[2]);
+        tile[lb * TL::SZ + TL::at3(c)] = ui[(size_t)(a.blk0 + lb0 + lb) * V + q];
+    }
+    // depth-1 face halo: neighbour cell, mirror of the facing cell, or frozen slab
+    for (int i = tid; i < nloc * 2 * D * F; i += NT) {
+        int lb = i / (2 * D * F), rem = i - lb * (2 * D * F);
+        int f = rem / F, t = rem - f * F;
+        int blk = a.blk0 + lb0 + lb;
+        int dim = f >> 1, dir = (f & 1) ? 1 : -1;
+        int c[3] = {0, 0, 0};
+        int ta = t % M, tb = t / M;
+        if (dim == 0) { c[1] = ta; c[2] = tb; }
+        else if (dim == 1) { c[0] = ta; c[2] = tb; }
+        else { c[0] = ta; c[1] = tb; }
+        int e = nbr[blk * 2 * D + f];
+        T v;
+        if (e >= 0) {
+            c[dim] = dir > 0 ? 0 : M - 1;
+            v = ui[(size_t)e * V + LY::sidx(c[0], c[1], c[2])];
+        } else if (e == NB_BOUNDARY) {
+            c[dim] = dir > 0 ? M - 1 : 0;
+            v = a.sg[f] * ui[(size_t)blk * V + LY::sidx(c[0], c[1], c[2])];
+        } else {
+            v = ib[(size_t)nb_slot(e) * F + t];
+        }
+        c[dim] = dir > 0 ? M : -1;
+        tile[lb * TL::SZ + TL::at3(c)] = v;
+    }
+    T bo[CPT], be[CPT];
+#pragma unroll
+    for (int k = 0; k < CPT; k++) {
+        int i = tid + k * NT, lb = i / H, r = i - lb * H;
+        bo[k] = be[k] = T(0);
+        if (lb < nloc) {
+            size_t o = (size_t)(a.blk0 + lb0 + lb) * V + r;
+            bo[k] = b[o];
+            be[k] = b[o + H];
+        }
+    }
+    __syncthreads();
+
+    // first half-sweep: cells with odd global index sum
+    T n0[CPT];
+#pragma unroll
+    for (int k = 0; k < CPT; k++) {
+        int i = tid + k * NT, lb = i / H, r = i - lb * H;
+        n0[k] = T(0);
+        if (lb < nloc) {
+            int blk = a.blk0 + lb0 + lb;
+            int c[3] = {0, 0, 0};
+            LY::decode(r, c[0], c[1], c[2]);
+            int p = lb * TL::SZ + TL::at3(c);
+            T own = tile[p];
+            T wm, wp;
+            weights_at<D, M, T>(a, coord, blk, c, wm, wp);
+            T Lu = tile_stencil<D, M, T>(tile + 0, p, own, a, wm, wp);
+            T nv = fma_t<T>(a.nwc, fma_t<T>(a.s, bo[k], Lu), own);
+            tile[p] = nv;
+            n0[k] = nv;
+        }
+    }
+    // the same half-sweep on the neighbours' face layers (odd cells only)
+    for (int j = tid; j < nloc * 2 * D * (F / 2); j += NT) {
+        int lb = j / (2 * D * (F / 2)), rem = j - lb * (2 * D * (F / 2));
+        int f = rem / (F / 2), tq = rem - f * (F / 2);
+        int blk = a.blk0 + lb0 + lb;
+        int e = nbr[blk * 2 * D + f];
+        if (e < 0) continue;
+        int dim = f >> 1, dir = (f & 1) ? 1 : -1;
+        int w = dir > 0 ? 0 : M - 1;  // the neighbour's face layer
+        int da = dim == 0 ? 1 : 0, db = dim == 2 ? 1 : 2;  // tangential dimensions
+        int tb = (D == 3) ? tq / MH : 0;
+        int par = (1 + w + tb + D) & 1;
+        int ta = 2 * (tq % MH) + par;
+        int cB[3] = {0, 0, 0};
+        cB[dim] = w;
+        cB[da] = ta;
+        if (D == 3) cB[db] = tb;
+        int ch[3] = {cB[0], cB[1], cB[2]};
+        ch[dim] = dir > 0 ? M : -1;  // the same cell as seen from this block's halo
+        int ph = lb * TL::SZ + TL::at3(ch);
+        T xo = tile[ph];
+        T wm, wp;
+        weights_at<D, M, T>(a, coord, e, cB, wm, wp);
+        T nbm[3], nbp[3];
+#pragma unroll
+        for (int dd = 0; dd < D; dd++) {
+#pragma unroll
+            for (int side = 0; side < 2; side++) {
+                int sdir = side ? 1 : -1;
+                T v;
+                if (dd == dim) {
+                    int cc = w + sdir;
+                    if (cc < 0 || cc >= M) {  // across the shared face: this block's cell
+                        int cA[3] = {cB[0], cB[1], cB[2]};
+                        cA[dim] = dir > 0 ? M - 1 : 0;
+                        v = tile[lb * TL::SZ + TL::at3(cA)];
+                    } else {                  // one layer deeper inside the neighbour
+                        int c2[3] = {cB[0], cB[1], cB[2]};
+                        c2[dim] = cc;
+                        v = ui[(size_t)e * V + LY::sidx(c2[0], c2[1], c2[2])];
+                    }
+                } else {
+                    int cc = cB[dd] + sdir;
+                    if (cc >= 0 && cc < M) {  // along the face layer: in the halo
+                        int c2[3] = {ch[0], ch[1], ch[2]};
+                        c2[dd] = cc;
+                        v = tile[lb * TL::SZ + TL::at3(c2)];
+                    } else {                  // the neighbour's own ghost across its side face
+                        int g = nbr[e * 2 * D + 2 * dd + side];
+                        if (g >= 0) {
+                            int c2[3] = {cB[0], cB[1], cB[2]};
+                            c2[dd] = side ? 0 : M - 1;
+                            v = ui[(size_t)g * V + LY::sidx(c2[0], c2[1], c2[2])];
+                        } else if (g == NB_BOUNDARY) {
+                            v = a.sg[2 * dd + side] * xo;
+                        } else {
+                            v = ib[(size_t)nb_slot(g) * F + LY::faceidx(dd, cB[0], cB[1], cB[2])];
+                        }
+                    }
+                }
+                if (side) nbp[dd] = v; else nbm[dd] = v;
+            }
+        }
+        T Lx = T(0);
+#pragma unroll
+        for (int dd = D - 1; dd >= 0; dd--) Lx += ((dd == a.rdim) ? wm : T(1)) * nbm[dd];
+        Lx += a.cc * xo;
+#pragma unroll
+        for (int dd = 0; dd < D; dd++) Lx += ((dd == a.rdim) ? wp : T(1)) * nbp[dd];
+        T bx = b[(size_t)e * V + LY::sidx(cB[0], cB[1], cB[2])];
+        tile[ph] = fma_t<T>(a.nwc, fma_t<T>(a.s, bx, Lx), xo);
+    }
+    __syncthreads();
+
+    // second half-sweep: the remaining cells
+#pragma unroll
+    for (int k = 0; k < CPT; k++) {
+        int i = tid + k * NT, lb = i / H, r = i - lb * H;
+        if (lb < nloc) {
+            int blk = a.blk0 + lb0 + lb;
+            int c[3] = {0, 0, 0};
+            LY::decode(H + r, c[0], c[1], c[2]);
+            int p = lb * TL::SZ + TL::at3(c);
+            T own = tile[p];
+            T wm, wp;
+            weights_at<D, M, T>(a, coord, blk, c, wm, wp);
+            T Lu = tile_stencil<D, M, T>(tile + 0, p, own, a, wm, wp);
+            size_t o = (size_t)blk * V + r;
+            uo[o] = n0[k];
+            uo[o + H] = fma_t<T>(a.nwc, fma_t<T>(a.s, be[k], Lu), own);
+        }
+    }
 }
 
 // ---- exact level-1 solve (FP64) ----

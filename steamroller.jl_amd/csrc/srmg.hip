@@ -22,6 +22,10 @@
 
 using namespace srmg;
 
+// work groups of one level's residual-norm launch (fixed, so the summation
+// order does not depend on the level size)
+constexpr unsigned RESNORM_WGS = 1024;
+
 // ---------------------------------------------------------------------------
 // errors
 // ---------------------------------------------------------------------------
@@ -84,8 +88,15 @@ struct DevBuf {
 struct FieldBuf {
     int dtype = SRMG_F64;
     DevBuf data;  // nb * V values
+    DevBuf alt;   // second buffer for ping-pong smoothing (made on first sweep)
     DevBuf ib;    // nslot * F values (coarse-fine ghost slabs)
+    std::vector<uint8_t> cur;  // per level: 0 -> values in data, 1 -> in alt
     size_t word() const { return dtype == SRMG_F64 ? 8 : 4; }
+    bool settled() const {
+        for (auto v : cur)
+            if (v) return false;
+        return true;
+    }
 };
 
 struct LevelInfo {
@@ -145,6 +156,8 @@ static FieldBuf &field(srmg_ctx *c, int id) {
 }
 
 static void alloc_field(srmg_ctx *c, FieldBuf &f) {
+    f.alt.release();
+    f.cur.assign(c->nlevels, 0);
     f.data.alloc((size_t)c->nb * c->V * f.word());
     f.ib.alloc((size_t)std::max(c->nslot, 1) * c->F * f.word());
     HIPCHK(hipMemsetAsync(f.data.p, 0, f.data.bytes, c->stream));
@@ -338,7 +351,7 @@ static void set_tree(srmg_ctx *c, int nlevels, int64_t nhost, const int64_t *pai
     up(c->d_w32, w32.data(), w32.size() * sizeof(float));
     // reduction scratch: one partial per work group of the largest leaf launch
     int maxp = 1;
-    for (int l = 0; l < nlevels; l++) maxp += (int)(((int64_t)c->lv[l].nleaf * c->V + WG - 1) / WG);
+    maxp += nlevels * (int)RESNORM_WGS;
     c->maxpartials = maxp;
     c->d_partial.alloc((size_t)maxp * sizeof(double));
     c->d_sum.alloc(sizeof(double) * 4);
@@ -403,71 +416,98 @@ struct Ops {
     using LY = Lay<D, M>;
     srmg_ctx *c;
     explicit Ops(srmg_ctx *ctx) : c(ctx) {}
-    T *X(FieldBuf &f) { return f.data.as<T>(); }
+    // values of level l (ping-pong aware); blocks of every level keep their
+    // global device index in both buffers
+    T *X(FieldBuf &f, int l) { return (l >= 0 && f.cur[l]) ? f.alt.as<T>() : f.data.as<T>(); }
     T *I(FieldBuf &f) { return f.ib.as<T>(); }
     const int *nbr() { return c->d_nbr.as<int>(); }
     const int *coord() { return c->d_coord.as<int>(); }
 
-    void gs_pass(FieldBuf &u, FieldBuf &b, const LevelArgs<T> &a, int pass) {
-        if (!a.nblk) return;
-        hipLaunchKernelGGL((k_gs_pass<D, M, T>), dim3(nwg((int64_t)a.nblk * LY::H)), dim3(WG), 0, c->stream,
-                           X(u), X(b), I(u), nbr(), coord(), a, pass);
-    }
-    // gauss_seidel_iter! (src/multigrid.jl:250-265): the copy and boundary ghost
-    // fills between colours are implicit (ghosts read from valid cells).
-    void smooth(FieldBuf &u, FieldBuf &b, const LevelArgs<T> &a, int n) {
-        for (int i = 0; i < n; i++) {
-            gs_pass(u, b, a, 0);
-            gs_pass(u, b, a, 1);
+    // bring every level back to the primary buffer
+    void settle(FieldBuf &f) {
+        for (int l = 0; l < c->nlevels; l++) {
+            if (!f.cur[l]) continue;
+            const LevelInfo &L = c->lv[l];
+            size_t n = (size_t)L.nblk * LY::V, o = (size_t)L.blk0 * LY::V;
+            if (n)
+                hipLaunchKernelGGL((k_copy<T>), dim3(nwg(n)), dim3(WG), 0, c->stream, f.data.as<T>() + o,
+                                   f.alt.as<T>() + o, n);
+            f.cur[l] = 0;
         }
     }
-    void residual(FieldBuf &r, FieldBuf &u, FieldBuf &b, const LevelArgs<T> &a) {
+
+    void gs_pass(FieldBuf &u, FieldBuf &b, const LevelArgs<T> &a, int l, int pass) {
+        if (!a.nblk) return;
+        hipLaunchKernelGGL((k_gs_pass<D, M, T>), dim3(nwg((int64_t)a.nblk * LY::H)), dim3(WG), 0, c->stream,
+                           X(u, l), X(b, l), I(u), nbr(), coord(), a, pass);
+    }
+    // blocks per work group of the fused sweep: one colour of BPW blocks
+    // fills a 256-thread work group (M = 16 in 3D: 512 threads, 4 cells each)
+    static constexpr int SW_BPW = (LY::H >= 256) ? 1 : 256 / LY::H;
+    static constexpr int SW_NT = (LY::H > 256) ? 512 : 256;
+    void sweep(FieldBuf &u, FieldBuf &b, const LevelArgs<T> &a, int l) {
+        if (!a.nblk) return;
+        if (!u.alt.p) u.alt.alloc(u.data.bytes);
+        T *in = X(u, l);
+        T *out = u.cur[l] ? u.data.as<T>() : u.alt.as<T>();
+        unsigned g = (unsigned)((a.nblk + SW_BPW - 1) / SW_BPW);
+        hipLaunchKernelGGL((k_gs_sweep<D, M, T, SW_BPW, SW_NT>), dim3(g), dim3(SW_NT), 0, c->stream, out, in,
+                           X(b, l), I(u), nbr(), coord(), a);
+        u.cur[l] ^= 1;
+    }
+    // gauss_seidel_iter! (src/multigrid.jl:250-265)
+    void smooth(FieldBuf &u, FieldBuf &b, const LevelArgs<T> &a, int l, int n) {
+        for (int i = 0; i < n; i++) sweep(u, b, a, l);
+    }
+    void residual(FieldBuf &r, FieldBuf &u, FieldBuf &b, const LevelArgs<T> &a, int l) {
         if (!a.nblk) return;
         hipLaunchKernelGGL((k_residual<D, M, T>), dim3(nwg((int64_t)a.nblk * LY::V)), dim3(WG), 0, c->stream,
-                           X(r), X(u), X(b), I(u), nbr(), coord(), a);
+                           X(r, l), X(u, l), X(b, l), I(u), nbr(), coord(), a);
     }
     void restrict_(FieldBuf &x, int l) {
         const LevelInfo &L = c->lv[l];
         if (!L.nblk) return;
         hipLaunchKernelGGL((k_restrict<D, M, T>), dim3(nwg((int64_t)L.nblk * (LY::V / LY::NC))), dim3(WG), 0,
-                           c->stream, X(x), c->d_parent.as<int>(), c->d_sub.as<int>(), L.blk0, L.nblk);
+                           c->stream, X(x, l - 1), X(x, l), c->d_parent.as<int>(), c->d_sub.as<int>(), L.blk0,
+                           L.nblk);
     }
     void postrestrict(FieldBuf &b, FieldBuf &u, FieldBuf &r, int l, const LevelArgs<T> &ac) {
         const LevelInfo &L = c->lv[l];
         if (!L.nblk) return;
         hipLaunchKernelGGL((k_postrestrict<D, M, T>), dim3(nwg((int64_t)L.nblk * (LY::V / LY::NC))), dim3(WG), 0,
-                           c->stream, X(b), X(u), X(r), I(u), nbr(), coord(), c->d_parent.as<int>(),
-                           c->d_sub.as<int>(), L.blk0, L.nblk, ac);
+                           c->stream, X(b, l - 1), X(u, l - 1), X(r, l - 1), I(u), nbr(), coord(),
+                           c->d_parent.as<int>(), c->d_sub.as<int>(), L.blk0, L.nblk, ac);
     }
     void copy(FieldBuf &dst, FieldBuf &src, int l) {
         const LevelInfo &L = c->lv[l];
         size_t n = (size_t)L.nblk * LY::V;
         if (!n) return;
-        hipLaunchKernelGGL((k_copy<T>), dim3(nwg(n)), dim3(WG), 0, c->stream, X(dst) + (size_t)L.blk0 * LY::V,
-                           X(src) + (size_t)L.blk0 * LY::V, n);
+        size_t o = (size_t)L.blk0 * LY::V;
+        hipLaunchKernelGGL((k_copy<T>), dim3(nwg(n)), dim3(WG), 0, c->stream, X(dst, l) + o, X(src, l) + o, n);
     }
     void diff(FieldBuf &dst, FieldBuf &a1, FieldBuf &a2, int l) {
         const LevelInfo &L = c->lv[l];
         size_t n = (size_t)L.nblk * LY::V;
         if (!n) return;
         size_t o = (size_t)L.blk0 * LY::V;
-        hipLaunchKernelGGL((k_diff<T>), dim3(nwg(n)), dim3(WG), 0, c->stream, X(dst) + o, X(a1) + o, X(a2) + o, n);
+        hipLaunchKernelGGL((k_diff<T>), dim3(nwg(n)), dim3(WG), 0, c->stream, X(dst, l) + o, X(a1, l) + o,
+                           X(a2, l) + o, n);
     }
     void interp_add(FieldBuf &u, FieldBuf &u1, int l, const LevelArgs<T> &ac) {
         const LevelInfo &L = c->lv[l];
         if (!L.nblk) return;
         hipLaunchKernelGGL((k_interp_add<D, M, T>), dim3(nwg((int64_t)L.nblk * LY::V)), dim3(WG), 0, c->stream,
-                           X(u), X(u1), I(u1), nbr(), c->d_parent.as<int>(), c->d_sub.as<int>(), L.blk0, L.nblk,
-                           ac);
+                           X(u, l), X(u1, l - 1), I(u1), nbr(), c->d_parent.as<int>(), c->d_sub.as<int>(), L.blk0,
+                           L.nblk, ac);
     }
     // refresh the coarse-fine ghost slabs of level l from level l-1
     void fill_interp(FieldBuf &x, int l, const LevelArgs<T> &ac) {
         const LevelInfo &L = c->lv[l];
         if (!L.nslot) return;
         hipLaunchKernelGGL((k_fill_interp<D, M, T>), dim3(nwg((int64_t)L.nslot * LY::F)), dim3(WG), 0, c->stream,
-                           I(x), X(x), nbr(), c->d_slots.as<int4>(), L.slot0, L.nslot, ac);
+                           I(x), X(x, l - 1), nbr(), c->d_slots.as<int4>(), L.slot0, L.nslot, ac);
     }
-    // queue the squared-residual partial sums of all levels; returns partial count
+    // queue the squared-residual partial sums of all levels and their total
     int resnorm_partials(FieldBuf &u, FieldBuf &b, double s, const srmg_op *op) {
         int np = 0;
         for (int l = 0; l < c->nlevels; l++) {
@@ -475,9 +515,9 @@ struct Ops {
             if (!L.nleaf) continue;
             LevelArgs<T> a = level_args<T>(c, l, s, op, 1.0);
             double wgt = std::ldexp(1.0, -c->D * l);
-            unsigned g = nwg((int64_t)L.nleaf * LY::V);
-            hipLaunchKernelGGL((k_resnorm<D, M, T>), dim3(g), dim3(WG), 0, c->stream, X(u), X(b), I(u), nbr(),
-                               coord(), c->d_leaves.as<int>() + L.leaf0, L.nleaf, a, wgt,
+            unsigned g = std::min<unsigned>(nwg((int64_t)L.nleaf * LY::V), RESNORM_WGS);
+            hipLaunchKernelGGL((k_resnorm<D, M, T>), dim3(g), dim3(WG), 0, c->stream, X(u, l), X(b, l), I(u),
+                               nbr(), coord(), c->d_leaves.as<int>() + L.leaf0, L.nleaf, a, wgt,
                                c->d_partial.as<double>() + np);
             np += (int)g;
         }
@@ -494,7 +534,7 @@ struct Ops {
             coarse_exact(u, b, s, op);
         } else {
             LevelArgs<T> a = level_args<T>(c, 0, s, op, omega);
-            smooth(u, b, a, ntop);
+            smooth(u, b, a, 0, ntop);
         }
     }
 
@@ -503,15 +543,16 @@ struct Ops {
     // of which the restriction to l-1 changes. The fill_ghost! calls on r are not
     // issued (r's ghosts are never read), and the descent's fill_ghost!(u, l-1)
     // needs no work: its interpolation ghosts come from level l-2, unchanged
-    // since they were last filled.
+    // since they were last filled. Levels whose sweep count is odd are copied
+    // back at the end so that every cycle starts from the same buffers.
     void vcycle(FieldBuf &u, FieldBuf &b, FieldBuf &r, FieldBuf &u1, double s, const srmg_op *op, int lmax,
                 int ndown, int ntop, int nup, double omega) {
         if (lmax == 0) lmax = c->nlevels;
         for (int l = lmax - 1; l >= 1; l--) {
             LevelArgs<T> a = level_args<T>(c, l, s, op, omega);
             LevelArgs<T> ac = level_args<T>(c, l - 1, s, op, omega);
-            smooth(u, b, a, ndown);
-            residual(r, u, b, a);
+            smooth(u, b, a, l, ndown);
+            residual(r, u, b, a, l);
             restrict_(u, l);
             copy(u1, u, l - 1);
             restrict_(r, l);
@@ -525,8 +566,9 @@ struct Ops {
             if (l - 1 >= 1) fill_interp(u1, l - 1, level_args<T>(c, l - 2, s, op, omega));
             interp_add(u, u1, l, ac);
             fill_interp(u, l, ac);
-            smooth(u, b, a, nup);
+            smooth(u, b, a, l, nup);
         }
+        settle(u);
     }
 
     // fmg! (src/multigrid.jl:17-51)
@@ -538,7 +580,7 @@ struct Ops {
         for (int l = lmax - 1; l >= 1; l--) {
             LevelArgs<T> a = level_args<T>(c, l, s, op, omega);
             LevelArgs<T> ac = level_args<T>(c, l - 1, s, op, omega);
-            residual(r, u, b, a);
+            residual(r, u, b, a, l);
             restrict_(u, l);
             restrict_(r, l);
             postrestrict(b, u, r, l, ac);
@@ -678,7 +720,7 @@ void Ops<D, M, T>::coarse_exact(FieldBuf &u, FieldBuf &b, double s, const srmg_o
     size_t total = (size_t)N[0] * N[1] * N[2];
     double *g = c->d_grid.as<double>(), *g2 = c->d_grid2.as<double>();
     hipLaunchKernelGGL((k_grid_gather<D, M, T>), dim3(nwg((int64_t)L1.nblk * LY::V)), dim3(WG), 0, c->stream, g,
-                       X(b), coord(), L1.blk0, L1.nblk, N[0], N[1], -s);
+                       X(b, 0), coord(), L1.blk0, L1.nblk, N[0], N[1], -s);
     size_t stride = 1;
     for (int d = 0; d < D; d++) {
         hipLaunchKernelGGL(k_axis, dim3(nwg(total)), dim3(WG), 0, c->stream, cc.Q[d].as<double>(), N[d], stride,
@@ -698,7 +740,7 @@ void Ops<D, M, T>::coarse_exact(FieldBuf &u, FieldBuf &b, double s, const srmg_o
         stride *= N[d];
     }
     hipLaunchKernelGGL((k_grid_scatter<D, M, T>), dim3(nwg((int64_t)L1.nblk * LY::V)), dim3(WG), 0, c->stream,
-                       X(u), g, coord(), L1.blk0, L1.nblk, N[0], N[1]);
+                       X(u, 0), g, coord(), L1.blk0, L1.nblk, N[0], N[1]);
 }
 
 // ---------------------------------------------------------------------------
@@ -882,7 +924,7 @@ static void solve_plain(srmg_ctx *c, int ui, int bi, int ri, int u1i, double s, 
     int t = same_type(c, {ui, bi, ri, u1i});
     FieldBuf &u = field(c, ui), &b = field(c, bi), &r = field(c, ri), &u1 = field(c, u1i);
     dispatch(c, t, [&](auto &ops) {
-        using T = std::remove_reference_t<decltype(*ops.X(u))>;
+        using T = std::remove_reference_t<decltype(*ops.X(u, 0))>;
         for (int l = 1; l < c->nlevels; l++) ops.fill_interp(u, l, level_args<T>(c, l - 1, s, op, 1.0));
         ops.resnorm_partials(zero_field(c, t), b, s, op);
         double r0 = read_sum(c);
@@ -1046,6 +1088,7 @@ int srmg_zero(srmg_ctx *c, int id) {
         check_tree(c);
         HIPCHK(hipMemsetAsync(f.data.p, 0, f.data.bytes, c->stream));
         HIPCHK(hipMemsetAsync(f.ib.p, 0, f.ib.bytes, c->stream));
+        std::fill(f.cur.begin(), f.cur.end(), 0);
     });
 }
 
@@ -1071,6 +1114,7 @@ int srmg_upload(srmg_ctx *c, int id, const void *host, int host_dtype, int G) {
             }
         };
         HIPCHK(hipStreamSynchronize(c->stream));
+        std::fill(f.cur.begin(), f.cur.end(), 0);
         if (host_dtype == SRMG_F64) go(double());
         else if (host_dtype == SRMG_F32) go(float());
         else fail("bad host dtype");
@@ -1083,6 +1127,7 @@ static int download_impl(srmg_ctx *c, int id, void *host, int host_dtype, int G,
         check_tree(c);
         if (!host) fail("null host pointer");
         if (G < 0) fail("bad G");
+        if (!f.settled()) dispatch(c, f.dtype, [&](auto &o) { o.settle(f); });
         HIPCHK(hipStreamSynchronize(c->stream));
         int8_t sg[6] = {1, 1, 1, 1, 1, 1};
         if (bc)
@@ -1128,7 +1173,7 @@ int srmg_fill_ghost(srmg_ctx *c, int id, int level, const int8_t *bc) {
         srmg_op op{};
         std::memcpy(op.bc, bc, 27);
         dispatch(c, f.dtype, [&](auto &o) {
-            using T = std::remove_reference_t<decltype(*o.X(f))>;
+            using T = std::remove_reference_t<decltype(*o.X(f, 0))>;
             o.fill_interp(f, level - 1, level_args<T>(c, level - 2, 1.0, &op, 1.0));
         });
     });
@@ -1141,7 +1186,7 @@ int srmg_fill_ghost_all(srmg_ctx *c, int id, const int8_t *bc) {
         srmg_op op{};
         std::memcpy(op.bc, bc, 27);
         dispatch(c, f.dtype, [&](auto &o) {
-            using T = std::remove_reference_t<decltype(*o.X(f))>;
+            using T = std::remove_reference_t<decltype(*o.X(f, 0))>;
             for (int l = 1; l < c->nlevels; l++) o.fill_interp(f, l, level_args<T>(c, l - 1, 1.0, &op, 1.0));
         });
     });
@@ -1222,11 +1267,13 @@ int srmg_time_kernel(srmg_ctx *c, int kind, int level, int u, int b, int r, doub
         HIPCHK(hipEventCreate(&e1));
         float ms = 0.f;
         dispatch(c, t, [&](auto &o) {
-            using T = std::remove_reference_t<decltype(*o.X(field(c, u)))>;
+            using T = std::remove_reference_t<decltype(*o.X(field(c, u), 0))>;
             LevelArgs<T> a = level_args<T>(c, level - 1, s, op, 1.0);
+            int l = level - 1;
             auto launch = [&](int i) {
-                if (kind == SRMG_KERNEL_GS_PASS) o.gs_pass(field(c, u), field(c, b), a, i & 1);
-                else if (kind == SRMG_KERNEL_RESIDUAL) o.residual(field(c, r), field(c, u), field(c, b), a);
+                if (kind == SRMG_KERNEL_GS_PASS) o.gs_pass(field(c, u), field(c, b), a, l, i & 1);
+                else if (kind == SRMG_KERNEL_RESIDUAL) o.residual(field(c, r), field(c, u), field(c, b), a, l);
+                else if (kind == SRMG_KERNEL_GS_SWEEP) o.sweep(field(c, u), field(c, b), a, l);
                 else fail("unknown kernel kind");
             };
             launch(0);
@@ -1236,6 +1283,7 @@ int srmg_time_kernel(srmg_ctx *c, int kind, int level, int u, int b, int r, doub
             HIPCHK(hipEventRecord(e1, c->stream));
             HIPCHK(hipEventSynchronize(e1));
             HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            o.settle(field(c, u));
         });
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
