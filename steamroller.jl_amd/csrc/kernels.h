@@ -387,6 +387,21 @@ __device__ __forceinline__ void weights_at(const LevelArgs<T> &a, const int *__r
     }
 }
 
+// Per-block halo table used by the fused sweep: the block's own 2D face
+// entries, then for each face f the neighbour-table entries of the block across
+// f on its 2(D-1) side faces (tangential dimensions in increasing order, low
+// side first); unused when face f is not a same-level face.
+template <int D>
+struct Halo {
+    static constexpr int ND = 2 * D;
+    static constexpr int NS = 2 * (D - 1);
+    static constexpr int N = ND + ND * NS;
+    __host__ __device__ static constexpr int side_index(int dim, int dd, int side) {
+        // position of tangential face (dd, side) among the side faces of a face normal to dim
+        return 2 * ((dd < dim) ? dd : dd - 1) + side;
+    }
+};
+
 // One full gauss_seidel_iter! sweep (src/multigrid.jl:250-265: RedBlack{0} on
 // all blocks, copy/boundary ghost fill, RedBlack{1}, fill) in one launch.
 // Reads ui, writes uo (ping-pong, so neighbours always see the old values).
@@ -396,40 +411,61 @@ __device__ __forceinline__ void weights_at(const LevelArgs<T> &a, const int *__r
 // same operations as their owners, so the result is identical to the
 // two-launch form. Boundary ghosts mirror the cell they face, and coarse-fine
 // ghosts stay frozen during smoothing (src/multigrid.jl:256-263).
+// Memory round trips before compute: (interior, b, halo table), then (halo
+// cells, recompute operands).
 template <int D, int M, typename T, int BPW, int NT>
 __global__ __launch_bounds__(NT) void k_gs_sweep(T *__restrict__ uo, const T *__restrict__ ui,
                                                  const T *__restrict__ b, const T *__restrict__ ib,
-                                                 const int *__restrict__ nbr, const int *__restrict__ coord,
+                                                 const int *__restrict__ htab, const int *__restrict__ coord,
                                                  LevelArgs<T> a) {
     using LY = Lay<D, M>;
     using TL = Tile<D, M>;
-    constexpr int F = LY::F, V = LY::V, H = LY::H, MH = LY::MH;
+    using HT = Halo<D>;
+    constexpr int F = LY::F, V = LY::V, H = LY::H, MH = LY::MH, F2 = F / 2;
     static_assert((BPW * H) % NT == 0, "cells per colour must be a multiple of the work-group size");
     constexpr int CPT = BPW * H / NT;
+    constexpr int NREC = BPW * 2 * D * F2;         // neighbour face-layer cells to recompute
+    constexpr int RPT = (NREC + NT - 1) / NT;
     __shared__ T tile[BPW * TL::SZ];
+    __shared__ int tab[BPW * HT::N];
     const int lb0 = xcd_remap(blockIdx.x, gridDim.x) * BPW;
     const int tid = threadIdx.x;
     const int nloc = min(BPW, a.nblk - lb0);
+    const int blk0 = a.blk0 + lb0;
 
-    // interior cells
+    // round trip 1: interior cells, own right-hand side, halo tables
     for (int i = tid; i < nloc * V; i += NT) {
         int lb = i / V, q = i - lb * V;
         int c[3] = {0, 0, 0};
         LY::decode(q, c[0], c[1], c[2]);
-        tile[lb * TL::SZ + TL::at3(c)] = ui[(size_t)(a.blk0 + lb0 + lb) * V + q];
+        tile[lb * TL::SZ + TL::at3(c)] = ui[(size_t)(blk0 + lb) * V + q];
     }
-    // depth-1 face halo: neighbour cell, mirror of the facing cell, or frozen slab
+    T bo[CPT], be[CPT];
+#pragma unroll
+    for (int k = 0; k < CPT; k++) {
+        int i = tid + k * NT, lb = i / H, r = i - lb * H;
+        bo[k] = be[k] = T(0);
+        if (lb < nloc) {
+            size_t o = (size_t)(blk0 + lb) * V + r;
+            bo[k] = b[o];
+            be[k] = b[o + H];
+        }
+    }
+    for (int i = tid; i < nloc * HT::N; i += NT) tab[i] = htab[(size_t)blk0 * HT::N + i];
+    __syncthreads();
+
+    // round trip 2: halo cells into the tile
     for (int i = tid; i < nloc * 2 * D * F; i += NT) {
         int lb = i / (2 * D * F), rem = i - lb * (2 * D * F);
         int f = rem / F, t = rem - f * F;
-        int blk = a.blk0 + lb0 + lb;
+        int blk = blk0 + lb;
         int dim = f >> 1, dir = (f & 1) ? 1 : -1;
         int c[3] = {0, 0, 0};
         int ta = t % M, tb = t / M;
         if (dim == 0) { c[1] = ta; c[2] = tb; }
         else if (dim == 1) { c[0] = ta; c[2] = tb; }
         else { c[0] = ta; c[1] = tb; }
-        int e = nbr[blk * 2 * D + f];
+        int e = tab[lb * HT::N + f];
         T v;
         if (e >= 0) {
             c[dim] = dir > 0 ? 0 : M - 1;
@@ -443,15 +479,54 @@ __global__ __launch_bounds__(NT) void k_gs_sweep(T *__restrict__ uo, const T *__
         c[dim] = dir > 0 ? M : -1;
         tile[lb * TL::SZ + TL::at3(c)] = v;
     }
-    T bo[CPT], be[CPT];
+    // ... and the operands of the recomputed neighbour cells that are not in
+    // the tile: one layer deeper in the neighbour, its right-hand side, and its
+    // own ghosts across its side faces (side ghosts mirroring the cell itself
+    // are taken from the tile later)
+    T rd[RPT], rb[RPT], rg[RPT][2 * (D - 1)];
+    int rinfo[RPT];  // -1: nothing to do; else (lb, f, tq) packed
 #pragma unroll
-    for (int k = 0; k < CPT; k++) {
-        int i = tid + k * NT, lb = i / H, r = i - lb * H;
-        bo[k] = be[k] = T(0);
-        if (lb < nloc) {
-            size_t o = (size_t)(a.blk0 + lb0 + lb) * V + r;
-            bo[k] = b[o];
-            be[k] = b[o + H];
+    for (int k = 0; k < RPT; k++) {
+        int j = tid + k * NT;
+        rinfo[k] = -1;
+        rd[k] = rb[k] = T(0);
+#pragma unroll
+        for (int q = 0; q < 2 * (D - 1); q++) rg[k][q] = T(0);
+        if (j >= nloc * 2 * D * F2) continue;
+        int lb = j / (2 * D * F2), rem = j - lb * (2 * D * F2);
+        int f = rem / F2, tq = rem - f * F2;
+        int e = tab[lb * HT::N + f];
+        if (e < 0) continue;
+        rinfo[k] = j;
+        int dim = f >> 1, dir = (f & 1) ? 1 : -1;
+        int w = dir > 0 ? 0 : M - 1;
+        int da = dim == 0 ? 1 : 0, db = dim == 2 ? 1 : 2;
+        int tb = (D == 3) ? tq / MH : 0;
+        int ta = 2 * (tq % MH) + ((1 + w + tb + D) & 1);
+        int cB[3] = {0, 0, 0};
+        cB[dim] = w;
+        cB[da] = ta;
+        if (D == 3) cB[db] = tb;
+        int c2[3] = {cB[0], cB[1], cB[2]};
+        c2[dim] = dir > 0 ? 1 : M - 2;
+        rd[k] = ui[(size_t)e * V + LY::sidx(c2[0], c2[1], c2[2])];
+        rb[k] = b[(size_t)e * V + LY::sidx(cB[0], cB[1], cB[2])];
+#pragma unroll
+        for (int dd = 0; dd < D; dd++) {
+            if (dd == dim) continue;
+#pragma unroll
+            for (int side = 0; side < 2; side++) {
+                if (cB[dd] != (side ? M - 1 : 0)) continue;
+                int g = tab[lb * HT::N + HT::ND + f * HT::NS + HT::side_index(dim, dd, side)];
+                int q = HT::side_index(dim, dd, side);
+                if (g >= 0) {
+                    int c3[3] = {cB[0], cB[1], cB[2]};
+                    c3[dd] = side ? 0 : M - 1;
+                    rg[k][q] = ui[(size_t)g * V + LY::sidx(c3[0], c3[1], c3[2])];
+                } else if (g != NB_BOUNDARY) {
+                    rg[k][q] = ib[(size_t)nb_slot(g) * F + LY::faceidx(dd, cB[0], cB[1], cB[2])];
+                }
+            }
         }
     }
     __syncthreads();
@@ -463,7 +538,7 @@ __global__ __launch_bounds__(NT) void k_gs_sweep(T *__restrict__ uo, const T *__
         int i = tid + k * NT, lb = i / H, r = i - lb * H;
         n0[k] = T(0);
         if (lb < nloc) {
-            int blk = a.blk0 + lb0 + lb;
+            int blk = blk0 + lb;
             int c[3] = {0, 0, 0};
             LY::decode(r, c[0], c[1], c[2]);
             int p = lb * TL::SZ + TL::at3(c);
@@ -476,25 +551,25 @@ __global__ __launch_bounds__(NT) void k_gs_sweep(T *__restrict__ uo, const T *__
             n0[k] = nv;
         }
     }
-    // the same half-sweep on the neighbours' face layers (odd cells only)
-    for (int j = tid; j < nloc * 2 * D * (F / 2); j += NT) {
-        int lb = j / (2 * D * (F / 2)), rem = j - lb * (2 * D * (F / 2));
-        int f = rem / (F / 2), tq = rem - f * (F / 2);
-        int blk = a.blk0 + lb0 + lb;
-        int e = nbr[blk * 2 * D + f];
-        if (e < 0) continue;
+    // the same half-sweep on the neighbours' face-layer cells (odd cells only)
+#pragma unroll
+    for (int k = 0; k < RPT; k++) {
+        int j = rinfo[k];
+        if (j < 0) continue;
+        int lb = j / (2 * D * F2), rem = j - lb * (2 * D * F2);
+        int f = rem / F2, tq = rem - f * F2;
+        int e = tab[lb * HT::N + f];
         int dim = f >> 1, dir = (f & 1) ? 1 : -1;
-        int w = dir > 0 ? 0 : M - 1;  // the neighbour's face layer
-        int da = dim == 0 ? 1 : 0, db = dim == 2 ? 1 : 2;  // tangential dimensions
+        int w = dir > 0 ? 0 : M - 1;
+        int da = dim == 0 ? 1 : 0, db = dim == 2 ? 1 : 2;
         int tb = (D == 3) ? tq / MH : 0;
-        int par = (1 + w + tb + D) & 1;
-        int ta = 2 * (tq % MH) + par;
+        int ta = 2 * (tq % MH) + ((1 + w + tb + D) & 1);
         int cB[3] = {0, 0, 0};
         cB[dim] = w;
         cB[da] = ta;
         if (D == 3) cB[db] = tb;
         int ch[3] = {cB[0], cB[1], cB[2]};
-        ch[dim] = dir > 0 ? M : -1;  // the same cell as seen from this block's halo
+        ch[dim] = dir > 0 ? M : -1;  // the same cell in this block's halo
         int ph = lb * TL::SZ + TL::at3(ch);
         T xo = tile[ph];
         T wm, wp;
@@ -512,10 +587,8 @@ __global__ __launch_bounds__(NT) void k_gs_sweep(T *__restrict__ uo, const T *__
                         int cA[3] = {cB[0], cB[1], cB[2]};
                         cA[dim] = dir > 0 ? M - 1 : 0;
                         v = tile[lb * TL::SZ + TL::at3(cA)];
-                    } else {                  // one layer deeper inside the neighbour
-                        int c2[3] = {cB[0], cB[1], cB[2]};
-                        c2[dim] = cc;
-                        v = ui[(size_t)e * V + LY::sidx(c2[0], c2[1], c2[2])];
+                    } else {
+                        v = rd[k];
                     }
                 } else {
                     int cc = cB[dd] + sdir;
@@ -523,17 +596,9 @@ __global__ __launch_bounds__(NT) void k_gs_sweep(T *__restrict__ uo, const T *__
                         int c2[3] = {ch[0], ch[1], ch[2]};
                         c2[dd] = cc;
                         v = tile[lb * TL::SZ + TL::at3(c2)];
-                    } else {                  // the neighbour's own ghost across its side face
-                        int g = nbr[e * 2 * D + 2 * dd + side];
-                        if (g >= 0) {
-                            int c2[3] = {cB[0], cB[1], cB[2]};
-                            c2[dd] = side ? 0 : M - 1;
-                            v = ui[(size_t)g * V + LY::sidx(c2[0], c2[1], c2[2])];
-                        } else if (g == NB_BOUNDARY) {
-                            v = a.sg[2 * dd + side] * xo;
-                        } else {
-                            v = ib[(size_t)nb_slot(g) * F + LY::faceidx(dd, cB[0], cB[1], cB[2])];
-                        }
+                    } else {
+                        int g = tab[lb * HT::N + HT::ND + f * HT::NS + HT::side_index(dim, dd, side)];
+                        v = (g == NB_BOUNDARY) ? a.sg[2 * dd + side] * xo : rg[k][HT::side_index(dim, dd, side)];
                     }
                 }
                 if (side) nbp[dd] = v; else nbm[dd] = v;
@@ -545,8 +610,7 @@ __global__ __launch_bounds__(NT) void k_gs_sweep(T *__restrict__ uo, const T *__
         Lx += a.cc * xo;
 #pragma unroll
         for (int dd = 0; dd < D; dd++) Lx += ((dd == a.rdim) ? wp : T(1)) * nbp[dd];
-        T bx = b[(size_t)e * V + LY::sidx(cB[0], cB[1], cB[2])];
-        tile[ph] = fma_t<T>(a.nwc, fma_t<T>(a.s, bx, Lx), xo);
+        tile[ph] = fma_t<T>(a.nwc, fma_t<T>(a.s, rb[k], Lx), xo);
     }
     __syncthreads();
 
@@ -555,7 +619,7 @@ __global__ __launch_bounds__(NT) void k_gs_sweep(T *__restrict__ uo, const T *__
     for (int k = 0; k < CPT; k++) {
         int i = tid + k * NT, lb = i / H, r = i - lb * H;
         if (lb < nloc) {
-            int blk = a.blk0 + lb0 + lb;
+            int blk = blk0 + lb;
             int c[3] = {0, 0, 0};
             LY::decode(H + r, c[0], c[1], c[2]);
             int p = lb * TL::SZ + TL::at3(c);

@@ -24,7 +24,7 @@ using namespace srmg;
 
 // work groups of one level's residual-norm launch (fixed, so the summation
 // order does not depend on the level size)
-constexpr unsigned RESNORM_WGS = 1024;
+constexpr unsigned RESNORM_WGS = 4096;
 
 // ---------------------------------------------------------------------------
 // errors
@@ -127,7 +127,7 @@ struct srmg_ctx {
     std::vector<int> host2dev, dev2host;
     std::vector<int4> h_slots;
     // device tables
-    DevBuf d_nbr, d_coord, d_parent, d_sub, d_slots, d_leaves;
+    DevBuf d_nbr, d_coord, d_parent, d_sub, d_slots, d_leaves, d_halo;
     DevBuf d_w64, d_w32;    // radial weights (double / float), per level and dim
     DevBuf d_partial, d_sum;
     DevBuf d_grid, d_grid2; // level-1 uniform grids (coarse solve)
@@ -337,6 +337,25 @@ static void set_tree(srmg_ctx *c, int nlevels, int64_t nhost, const int64_t *pai
             if (cd < 1 || cd > c->lv[c->h_level[b]].ext[d]) fail("block coordinate outside the level domain");
         }
     }
+    // halo tables of the fused sweep (Halo<D>)
+    const int HN = ND + ND * 2 * (D - 1);
+    std::vector<int> htab((size_t)nb * HN, 0);
+    for (int b = 0; b < nb; b++) {
+        int *h = htab.data() + (size_t)b * HN;
+        for (int f = 0; f < ND; f++) h[f] = c->h_nbr[(size_t)b * ND + f];
+        for (int f = 0; f < ND; f++) {
+            int e = h[f];
+            if (e < 0) continue;
+            int dim = f >> 1;
+            for (int dd = 0; dd < D; dd++) {
+                if (dd == dim) continue;
+                for (int side = 0; side < 2; side++) {
+                    int k = 2 * ((dd < dim) ? dd : dd - 1) + side;
+                    h[ND + f * 2 * (D - 1) + k] = c->h_nbr[(size_t)e * ND + 2 * dd + side];
+                }
+            }
+        }
+    }
     auto up = [&](DevBuf &b, const void *src, size_t bytes) {
         b.alloc(bytes);
         if (bytes) HIPCHK(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, c->stream));
@@ -347,6 +366,7 @@ static void set_tree(srmg_ctx *c, int nlevels, int64_t nhost, const int64_t *pai
     up(c->d_sub, c->h_sub.data(), c->h_sub.size() * sizeof(int));
     up(c->d_slots, c->h_slots.data(), c->h_slots.size() * sizeof(int4));
     up(c->d_leaves, c->h_leaves.data(), c->h_leaves.size() * sizeof(int));
+    up(c->d_halo, htab.data(), htab.size() * sizeof(int));
     up(c->d_w64, w64.data(), w64.size() * sizeof(double));
     up(c->d_w32, w32.data(), w32.size() * sizeof(float));
     // reduction scratch: one partial per work group of the largest leaf launch
@@ -452,7 +472,7 @@ struct Ops {
         T *out = u.cur[l] ? u.data.as<T>() : u.alt.as<T>();
         unsigned g = (unsigned)((a.nblk + SW_BPW - 1) / SW_BPW);
         hipLaunchKernelGGL((k_gs_sweep<D, M, T, SW_BPW, SW_NT>), dim3(g), dim3(SW_NT), 0, c->stream, out, in,
-                           X(b, l), I(u), nbr(), coord(), a);
+                           X(b, l), I(u), c->d_halo.as<int>(), coord(), a);
         u.cur[l] ^= 1;
     }
     // gauss_seidel_iter! (src/multigrid.jl:250-265)
