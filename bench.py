@@ -74,6 +74,13 @@ def main():
     ctx.synchronize()
     log(f"[bench] uploaded ({time.time() - t0:.1f}s)")
 
+    t1 = time.time()
+    tuned64 = ctx.autotune(u, b, p.s, geom, p.bc, disc, reps=4)
+    e.zero(); be.upload(p.b)
+    tuned32 = ctx.autotune(e, be, p.s, geom, p.bc, disc, reps=4)
+    log(f"[bench] autotuned sweep tile shapes per level: fp64 {tuned64.tolist()} fp32 {tuned32.tolist()} "
+        f"({time.time() - t1:.1f}s)")
+
     def reset():
         # restore the right-hand side, start from u = 0 (outside the timed region)
         b.upload(p.b)
@@ -121,7 +128,9 @@ def main():
         ms = ctx.time_kernel(solver.KERNEL_GS_SWEEP, L, uu, bb, rr, p.s / 4 ** (L - 1), geom, p.bc, disc, reps=40)
         algo = 3.0 * w * cells  # one red-black sweep: u read, b read, u write per cell
         roof[name] = {"ms": ms, "gbs": algo / (ms * 1e-3) / 1e9, "bytes": algo}
-    log(f"[bench] smoother: {roof}")
+    ms = ctx.time_kernel(solver.KERNEL_STREAM, L, u, b, r, p.s, geom, p.bc, disc, reps=40)
+    stream_gbs = 3 * 8 * cells / (ms * 1e-3) / 1e9
+    log(f"[bench] smoother: {roof}; stream r=u+b on the same level: {stream_gbs:.0f} GB/s")
 
     cpu = None
     if not args.no_cpu:
@@ -164,6 +173,8 @@ def main():
                             "frac": round(roof["fp64"]["gbs"] / HBM_PEAK_GBS, 4),
                             "ms": round(roof["fp64"]["ms"], 5)},
             "ms_per_launch": round(roof["fp32"]["ms"], 5),
+            "measured_stream_gbs": round(stream_gbs, 1),
+            "tile_shapes": {"fp64": tuned64.tolist(), "fp32": tuned32.tolist()},
             "algorithmic_bytes_per_launch": roof["fp32"]["bytes"],
         },
         "cpu_baseline": cpu,

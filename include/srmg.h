@@ -150,14 +150,23 @@ int srmg_solve_mixed(srmg_ctx *ctx, int u, int b, int e, int be, int re, int ue,
 
 /* ---- measurement ---- */
 
-enum { SRMG_KERNEL_GS_PASS = 0, SRMG_KERNEL_RESIDUAL = 1, SRMG_KERNEL_GS_SWEEP = 2 };
+enum { SRMG_KERNEL_GS_PASS = 0, SRMG_KERNEL_RESIDUAL = 1, SRMG_KERNEL_GS_SWEEP = 2, SRMG_KERNEL_STREAM = 3 };
 
 /* Average duration of one launch of a level kernel, timed with HIP events on
  * the context's stream over `reps` back-to-back launches: one colour of the
- * Gauss-Seidel sweep (alternating), the residual, or one fused red-black sweep.
- * Modifies u (smoothers) or r (residual). */
+ * Gauss-Seidel sweep (alternating), the residual, one fused red-black sweep,
+ * or the stream kernel r = u + b over the level (bandwidth calibration).
+ * Modifies u (smoothers) or r (residual, stream). */
 int srmg_time_kernel(srmg_ctx *ctx, int kind, int level, int u, int b, int r, double s,
                      const srmg_op *op, int reps, double *ms_per_launch);
+
+/* Autotuner: times every compiled tile shape of the fused sweep (blocks per
+ * work group x work-group size) on every level with `reps` sweeps of u, keeps
+ * the fastest per level for later solves of fields of u's element type, and
+ * writes the choices (0..3) to chosen[nlevels] if non-NULL. u is restored. */
+int srmg_autotune(srmg_ctx *ctx, int u, int b, double s, const srmg_op *op, int reps, int *chosen);
+/* Force the tile shape (0..3) of the fused sweep for one element type and level (1-based). */
+int srmg_set_sweep_config(srmg_ctx *ctx, int dtype, int level, int cfg);
 
 /* Blocks, leaf blocks and coarse-fine ghost slabs of a level (1-based). */
 int srmg_level_info(srmg_ctx *ctx, int level, int64_t *nblocks, int64_t *nleaf, int64_t *nslots);

@@ -207,6 +207,14 @@ __global__ __launch_bounds__(WG) void k_diff(T *__restrict__ dst, const T *__res
     if (i < n) dst[i] = a[i] - b[i];
 }
 
+// r = a + b: a stream kernel of the solver's own size, for bandwidth calibration
+template <typename T>
+__global__ __launch_bounds__(WG) void k_triad(T *__restrict__ r, const T *__restrict__ a, const T *__restrict__ b,
+                                              size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) r[i] = a[i] + b[i];
+}
+
 template <typename T>
 __global__ __launch_bounds__(WG) void k_fill(T *__restrict__ dst, T v, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;

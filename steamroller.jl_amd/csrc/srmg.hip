@@ -136,6 +136,7 @@ struct srmg_ctx {
     std::unique_ptr<FieldBuf> zero[2];  // all-zero scratch fields (F64, F32), made on demand
     hipStream_t stream = nullptr;
     int maxpartials = 0;
+    std::vector<int> sweep_cfg;  // per (dtype, level): tile shape of the fused sweep (see Ops::sweep)
 };
 
 static void alloc_field(srmg_ctx *c, FieldBuf &f);
@@ -376,6 +377,7 @@ static void set_tree(srmg_ctx *c, int nlevels, int64_t nhost, const int64_t *pai
     c->d_partial.alloc((size_t)maxp * sizeof(double));
     c->d_sum.alloc(sizeof(double) * 4);
     c->coarse.valid = false;
+    c->sweep_cfg.assign(2 * nlevels, 0);
     c->zero[0].reset();
     c->zero[1].reset();
     size_t ng = (size_t)c->lv[0].nblk * c->V;
@@ -461,18 +463,36 @@ struct Ops {
         hipLaunchKernelGGL((k_gs_pass<D, M, T>), dim3(nwg((int64_t)a.nblk * LY::H)), dim3(WG), 0, c->stream,
                            X(u, l), X(b, l), I(u), nbr(), coord(), a, pass);
     }
-    // blocks per work group of the fused sweep: one colour of BPW blocks
-    // fills a 256-thread work group (M = 16 in 3D: 512 threads, 4 cells each)
-    static constexpr int SW_BPW = (LY::H >= 256) ? 1 : 256 / LY::H;
-    static constexpr int SW_NT = (LY::H > 256) ? 512 : 256;
-    void sweep(FieldBuf &u, FieldBuf &b, const LevelArgs<T> &a, int l) {
+    // Tile shapes of the fused sweep, k = 0..3: blocks per work group x
+    // work-group size. Small blocks (one colour <= 256 cells) group 1, 2, 2, 4
+    // times the blocks that fill 256 threads on 256, 256, 512, 512 threads;
+    // large blocks (3D, M = 16) use one block on 256, 512, 1024 or 512 threads.
+    static constexpr bool SMALL = LY::H <= 256;
+    static constexpr int BPW0 = SMALL ? 256 / LY::H : 1;
+    static constexpr int NCFG = 4;
+    template <int K> static constexpr int cfg_bpw() { return SMALL ? BPW0 * (K == 0 ? 1 : (K == 3 ? 4 : 2)) : 1; }
+    template <int K> static constexpr int cfg_nt() {
+        return SMALL ? (K < 2 ? 256 : 512) : (K == 0 ? 256 : (K == 2 ? 1024 : 512));
+    }
+    template <int K>
+    void sweep_launch(T *out, T *in, FieldBuf &u, FieldBuf &b, const LevelArgs<T> &a, int l) {
+        constexpr int BPW = cfg_bpw<K>(), NT = cfg_nt<K>();
+        unsigned g = (unsigned)((a.nblk + BPW - 1) / BPW);
+        hipLaunchKernelGGL((k_gs_sweep<D, M, T, BPW, NT>), dim3(g), dim3(NT), 0, c->stream, out, in, X(b, l), I(u),
+                           c->d_halo.as<int>(), coord(), a);
+    }
+    void sweep(FieldBuf &u, FieldBuf &b, const LevelArgs<T> &a, int l, int cfg = -1) {
         if (!a.nblk) return;
         if (!u.alt.p) u.alt.alloc(u.data.bytes);
         T *in = X(u, l);
         T *out = u.cur[l] ? u.data.as<T>() : u.alt.as<T>();
-        unsigned g = (unsigned)((a.nblk + SW_BPW - 1) / SW_BPW);
-        hipLaunchKernelGGL((k_gs_sweep<D, M, T, SW_BPW, SW_NT>), dim3(g), dim3(SW_NT), 0, c->stream, out, in,
-                           X(b, l), I(u), c->d_halo.as<int>(), coord(), a);
+        if (cfg < 0) cfg = c->sweep_cfg[(std::is_same<T, double>::value ? 0 : c->nlevels) + l];
+        switch (cfg) {
+            case 1: sweep_launch<1>(out, in, u, b, a, l); break;
+            case 2: sweep_launch<2>(out, in, u, b, a, l); break;
+            case 3: sweep_launch<3>(out, in, u, b, a, l); break;
+            default: sweep_launch<0>(out, in, u, b, a, l); break;
+        }
         u.cur[l] ^= 1;
     }
     // gauss_seidel_iter! (src/multigrid.jl:250-265)
@@ -1294,7 +1314,12 @@ int srmg_time_kernel(srmg_ctx *c, int kind, int level, int u, int b, int r, doub
                 if (kind == SRMG_KERNEL_GS_PASS) o.gs_pass(field(c, u), field(c, b), a, l, i & 1);
                 else if (kind == SRMG_KERNEL_RESIDUAL) o.residual(field(c, r), field(c, u), field(c, b), a, l);
                 else if (kind == SRMG_KERNEL_GS_SWEEP) o.sweep(field(c, u), field(c, b), a, l);
-                else fail("unknown kernel kind");
+                else if (kind == SRMG_KERNEL_STREAM) {
+                    // r = u + b over the level: reads 2, writes 1 value per cell
+                    size_t n = (size_t)c->lv[l].nblk * c->V, off = (size_t)c->lv[l].blk0 * c->V;
+                    hipLaunchKernelGGL((k_triad<T>), dim3(nwg(n)), dim3(WG), 0, c->stream, o.X(field(c, r), l) + off,
+                                       o.X(field(c, u), l) + off, o.X(field(c, b), l) + off, n);
+                } else fail("unknown kernel kind");
             };
             launch(0);
             launch(1);
@@ -1308,6 +1333,62 @@ int srmg_time_kernel(srmg_ctx *c, int kind, int level, int u, int b, int r, doub
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         *ms_per_launch = ms / reps;
+    });
+}
+
+int srmg_autotune(srmg_ctx *c, int u, int b, double s, const srmg_op *op, int reps, int *chosen) {
+    return guarded([&] {
+        check_tree(c);
+        if (!op || reps < 1) fail("bad arguments");
+        int t = same_type(c, {u, b});
+        FieldBuf &fu = field(c, u);
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        dispatch(c, t, [&](auto &o) {
+            using T = std::remove_reference_t<decltype(*o.X(fu, 0))>;
+            o.settle(fu);
+            // the trial sweeps run on a copy of u
+            DevBuf keep;
+            keep.alloc(fu.data.bytes);
+            HIPCHK(hipMemcpyAsync(keep.p, fu.data.p, fu.data.bytes, hipMemcpyDeviceToDevice, c->stream));
+            for (int l = 0; l < c->nlevels; l++) {
+                LevelArgs<T> a = level_args<T>(c, l, s, op, 1.0);
+                if (!a.nblk) continue;
+                float best = 1e30f;
+                int bestk = 0;
+                for (int k = 0; k < o.NCFG; k++) {
+                    o.sweep(fu, field(c, b), a, l, k);
+                    HIPCHK(hipEventRecord(e0, c->stream));
+                    for (int i = 0; i < reps; i++) o.sweep(fu, field(c, b), a, l, k);
+                    HIPCHK(hipEventRecord(e1, c->stream));
+                    HIPCHK(hipEventSynchronize(e1));
+                    float ms = 0.f;
+                    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                    if (ms < best) {
+                        best = ms;
+                        bestk = k;
+                    }
+                }
+                c->sweep_cfg[(t == SRMG_F64 ? 0 : c->nlevels) + l] = bestk;
+                if (chosen) chosen[l] = bestk;
+            }
+            std::fill(fu.cur.begin(), fu.cur.end(), 0);
+            HIPCHK(hipMemcpyAsync(fu.data.p, keep.p, fu.data.bytes, hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+        });
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    });
+}
+
+int srmg_set_sweep_config(srmg_ctx *c, int dtype, int level, int cfg) {
+    return guarded([&] {
+        check_tree(c);
+        if (level < 1 || level > c->nlevels) fail("level out of range");
+        if (dtype != SRMG_F64 && dtype != SRMG_F32) fail("bad dtype");
+        if (cfg < 0 || cfg > 3) fail("sweep configuration must be 0..3");
+        c->sweep_cfg[(dtype == SRMG_F64 ? 0 : c->nlevels) + level - 1] = cfg;
     });
 }
 

@@ -27,7 +27,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libsrmg.so")
 
 F64, F32 = 0, 1
-KERNEL_GS_PASS, KERNEL_RESIDUAL, KERNEL_GS_SWEEP = 0, 1, 2
+KERNEL_GS_PASS, KERNEL_RESIDUAL, KERNEL_GS_SWEEP, KERNEL_STREAM = 0, 1, 2, 3
 COARSE_GS, COARSE_EXACT = 0, 1
 _DT = {F64: np.float64, F32: np.float32}
 
@@ -132,6 +132,8 @@ def lib():
         "srmg_axis_decomp": [i32, i32, dbl, dbl, P, P, P],
         "srmg_time_kernel": [P, i32, i32, i32, i32, i32, dbl, P, i32, C.POINTER(dbl)],
         "srmg_level_info": [P, i32, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64)],
+        "srmg_autotune": [P, i32, i32, dbl, P, i32, P],
+        "srmg_set_sweep_config": [P, i32, i32, i32],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -203,6 +205,16 @@ class Context:
         _check(lib().srmg_time_kernel(self.h, kind, level, u.id, b.id, r.id, float(s), C.byref(op),
                                       int(reps), C.byref(ms)))
         return ms.value
+
+    def autotune(self, u, b, s, geometry, bc, disc, reps=5):
+        """Pick the fastest fused-sweep tile shape per level (times the sweeps on u)."""
+        op = make_op(geometry, bc, disc)
+        out = np.zeros(len(self.tree), np.int32)
+        _check(lib().srmg_autotune(self.h, u.id, b.id, float(s), C.byref(op), int(reps), _p(out)))
+        return out
+
+    def set_sweep_config(self, dtype: int, level: int, cfg: int):
+        _check(lib().srmg_set_sweep_config(self.h, dtype, level, cfg))
 
     def field(self, dtype: int = F64) -> "DeviceField":
         i = C.c_int()

@@ -235,3 +235,25 @@ def test_mixed_precision_matches_fp64(name):
     err = np.max(np.abs(_valid(c, p) - ref)) / np.max(np.abs(ref))
     print(f"{name}: fp64 {n64} cycles, mixed {n32} cycles, potential rel err {err:.2e}")
     assert err <= 1e-7
+
+
+@pytest.mark.parametrize("name", ["cfg1", "cyl2-m8", "cart3-m8", "cart3-m16"])
+@pytest.mark.parametrize("dtype", [F64, F32])
+def test_every_sweep_tile_shape_gives_identical_results(name, dtype):
+    p = CASES[name]()
+    print(f"case {name} dtype {dtype}", flush=True)
+    geom, disc = solver.geometry_from_code(p.geom), _disc(p)
+    outs = []
+    for cfg in range(4):
+        ctx, (u, b, r, u1) = _device(p, dtype)
+        for l in range(1, len(p.tree) + 1):
+            ctx.set_sweep_config(dtype, l, cfg)
+        b.upload(p.b)
+        solver.fill_ghost_all(u, p.bc)
+        for _ in range(2):
+            solver.vcycle(u, b, r, u1, p.s, p.tree, p.conn, geom, p.bc, disc, ndown=2, ntop=4, nup=2)
+        host = p.b.like()
+        u.download(host)
+        outs.append(_valid(host, p).copy())
+    for o in outs[1:]:
+        np.testing.assert_array_equal(o, outs[0])
