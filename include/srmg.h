@@ -150,12 +150,14 @@ int srmg_solve_mixed(srmg_ctx *ctx, int u, int b, int e, int be, int re, int ue,
 
 /* ---- measurement ---- */
 
-enum { SRMG_KERNEL_GS_PASS = 0, SRMG_KERNEL_RESIDUAL = 1, SRMG_KERNEL_GS_SWEEP = 2, SRMG_KERNEL_STREAM = 3 };
+enum { SRMG_KERNEL_GS_PASS = 0, SRMG_KERNEL_RESIDUAL = 1, SRMG_KERNEL_GS_SWEEP = 2, SRMG_KERNEL_STREAM = 3,
+       SRMG_KERNEL_RESIDUAL_GATHER = 4 };
 
 /* Average duration of one launch of a level kernel, timed with HIP events on
  * the context's stream over `reps` back-to-back launches: one colour of the
- * Gauss-Seidel sweep (alternating), the residual, one fused red-black sweep,
- * or the stream kernel r = u + b over the level (bandwidth calibration).
+ * Gauss-Seidel sweep (alternating), the residual (LDS-tiled, or the per-cell
+ * gather form), one fused red-black sweep, or the stream kernel r = u + b
+ * over the level (bandwidth calibration).
  * Modifies u (smoothers) or r (residual, stream). */
 int srmg_time_kernel(srmg_ctx *ctx, int kind, int level, int u, int b, int r, double s,
                      const srmg_op *op, int reps, double *ms_per_launch);

@@ -333,6 +333,23 @@ __global__ __launch_bounds__(WG) void k_resnorm(const T *__restrict__ u, const T
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0] * wgt;
 }
 
+// First stage of a fixed-order sum: work group g sums the contiguous chunk
+// [g * chunk, (g + 1) * chunk) of the n partials.
+__global__ __launch_bounds__(WG) void k_sum_chunks(const double *__restrict__ partial, int n, int chunk,
+                                                   double *__restrict__ out) {
+    __shared__ double red[WG];
+    int lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
+    double acc = 0.0;
+    for (int i = lo + threadIdx.x; i < hi; i += WG) acc += partial[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = WG / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
 // Sum n partials in a fixed order (single work group).
 __global__ __launch_bounds__(WG) void k_sum(const double *__restrict__ partial, int n, double *__restrict__ out) {
     __shared__ double red[WG];
@@ -639,6 +656,114 @@ __global__ __launch_bounds__(NT) void k_gs_sweep(T *__restrict__ uo, const T *__
             uo[o] = n0[k];
             uo[o + H] = fma_t<T>(a.nwc, fma_t<T>(a.s, be[k], Lu), own);
         }
+    }
+}
+
+// ---- tiled residual kernels ----
+
+// Stage BPW blocks (valid cells + depth-1 face halo) of x in the LDS tile.
+// list != nullptr selects the blocks through an index list (leaf blocks).
+// Two barriers: after (interior, halo table) and after the halo cells.
+template <int D, int M, typename T, int BPW, int NT>
+__device__ __forceinline__ void load_tile(T *tile, int *tab, int *blks, const T *__restrict__ x,
+                                          const T *__restrict__ ib, const int *__restrict__ htab,
+                                          const int *__restrict__ list, int first, int nloc, const T *sg) {
+    using LY = Lay<D, M>;
+    using TL = Tile<D, M>;
+    using HT = Halo<D>;
+    constexpr int F = LY::F, V = LY::V;
+    const int tid = threadIdx.x;
+    if (tid < BPW) blks[tid] = (tid < nloc) ? (list ? list[first + tid] : first + tid) : 0;
+    __syncthreads();
+    for (int i = tid; i < nloc * V; i += NT) {
+        int lb = i / V, q = i - lb * V;
+        int c[3] = {0, 0, 0};
+        LY::decode(q, c[0], c[1], c[2]);
+        tile[lb * TL::SZ + TL::at3(c)] = x[(size_t)blks[lb] * V + q];
+    }
+    for (int i = tid; i < nloc * HT::ND; i += NT) {
+        int lb = i / HT::ND;
+        tab[i] = htab[(size_t)blks[lb] * HT::N + (i - lb * HT::ND)];
+    }
+    __syncthreads();
+    for (int i = tid; i < nloc * 2 * D * F; i += NT) {
+        int lb = i / (2 * D * F), rem = i - lb * (2 * D * F);
+        int f = rem / F, t = rem - f * F;
+        int blk = blks[lb];
+        int dim = f >> 1, dir = (f & 1) ? 1 : -1;
+        int c[3] = {0, 0, 0};
+        int ta = t % M, tb = t / M;
+        if (dim == 0) { c[1] = ta; c[2] = tb; }
+        else if (dim == 1) { c[0] = ta; c[2] = tb; }
+        else { c[0] = ta; c[1] = tb; }
+        int e = tab[lb * HT::ND + f];
+        T v;
+        if (e >= 0) {
+            c[dim] = dir > 0 ? 0 : M - 1;
+            v = x[(size_t)e * V + LY::sidx(c[0], c[1], c[2])];
+        } else if (e == NB_BOUNDARY) {
+            c[dim] = dir > 0 ? M - 1 : 0;
+            v = sg[f] * x[(size_t)blk * V + LY::sidx(c[0], c[1], c[2])];
+        } else {
+            v = ib[(size_t)nb_slot(e) * F + t];
+        }
+        c[dim] = dir > 0 ? M : -1;
+        tile[lb * TL::SZ + TL::at3(c)] = v;
+    }
+    __syncthreads();
+}
+
+enum { RES_WRITE = 0, RES_NORM = 1, RES_DEFECT = 2 };
+
+// Residual of a level from LDS tiles (src/multigrid.jl:309-323).
+//   RES_WRITE:  r = -(b + L u / s) on the blocks [a.blk0, a.blk0 + a.nblk)
+//   RES_NORM:   leaf blocks `list[0..n)`: one partial sum of r^2 (times wgt) per work group
+//   RES_DEFECT: out (float) = b + L u / s = -r on the level's blocks, and the
+//               partial sums of r^2 over its leaf blocks (isleaf[blk])
+// Work group partials are summed in a fixed order (LDS tree).
+template <int D, int M, typename T, typename TO, int BPW, int NT, int MODE>
+__global__ __launch_bounds__(NT) void k_resid_tile(TO *__restrict__ out, const T *__restrict__ u,
+                                                   const T *__restrict__ b, const T *__restrict__ ib,
+                                                   const int *__restrict__ htab, const int *__restrict__ coord,
+                                                   const int *__restrict__ list, const int *__restrict__ isleaf,
+                                                   int n, LevelArgs<T> a, double wgt, double *__restrict__ partial) {
+    using LY = Lay<D, M>;
+    using TL = Tile<D, M>;
+    using HT = Halo<D>;
+    constexpr int V = LY::V;
+    __shared__ T tile[BPW * TL::SZ];
+    __shared__ int tab[BPW * HT::ND];
+    __shared__ int blks[BPW];
+    __shared__ double red[NT];
+    const int first = xcd_remap(blockIdx.x, gridDim.x) * BPW;
+    const int nloc = min(BPW, n - first);
+    const int *lst = (MODE == RES_NORM) ? list : nullptr;
+    const int base = (MODE == RES_NORM) ? first : a.blk0 + first;
+    load_tile<D, M, T, BPW, NT>(tile, tab, blks, u, ib, htab, lst, base, nloc, a.sg);
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < nloc * V; i += NT) {
+        int lb = i / V, q = i - lb * V;
+        int blk = blks[lb];
+        int c[3] = {0, 0, 0};
+        LY::decode(q, c[0], c[1], c[2]);
+        int p = lb * TL::SZ + TL::at3(c);
+        T wm, wp;
+        weights_at<D, M, T>(a, coord, blk, c, wm, wp);
+        T Lu = tile_stencil<D, M, T>(tile + 0, p, tile[p], a, wm, wp);
+        size_t o = (size_t)blk * V + q;
+        T rr = -(b[o] + Lu / a.s);
+        if (MODE == RES_WRITE) out[o] = (TO)rr;
+        if (MODE == RES_DEFECT) out[o] = (TO)(b[o] + Lu / a.s);
+        if (MODE == RES_NORM || (MODE == RES_DEFECT && isleaf[blk])) acc += (double)rr * (double)rr;
+    }
+    if (MODE != RES_WRITE) {
+        red[threadIdx.x] = acc;
+        __syncthreads();
+        for (int w = NT / 2; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) partial[blockIdx.x] = red[0] * wgt;
     }
 }
 

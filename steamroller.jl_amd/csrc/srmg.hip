@@ -25,6 +25,8 @@ using namespace srmg;
 // work groups of one level's residual-norm launch (fixed, so the summation
 // order does not depend on the level size)
 constexpr unsigned RESNORM_WGS = 4096;
+// first-stage work groups of the fixed-order sum of partials
+constexpr int SUM_CHUNKS = 256;
 
 // ---------------------------------------------------------------------------
 // errors
@@ -127,7 +129,7 @@ struct srmg_ctx {
     std::vector<int> host2dev, dev2host;
     std::vector<int4> h_slots;
     // device tables
-    DevBuf d_nbr, d_coord, d_parent, d_sub, d_slots, d_leaves, d_halo;
+    DevBuf d_nbr, d_coord, d_parent, d_sub, d_slots, d_leaves, d_halo, d_isleaf;
     DevBuf d_w64, d_w32;    // radial weights (double / float), per level and dim
     DevBuf d_partial, d_sum;
     DevBuf d_grid, d_grid2; // level-1 uniform grids (coarse solve)
@@ -368,14 +370,17 @@ static void set_tree(srmg_ctx *c, int nlevels, int64_t nhost, const int64_t *pai
     up(c->d_slots, c->h_slots.data(), c->h_slots.size() * sizeof(int4));
     up(c->d_leaves, c->h_leaves.data(), c->h_leaves.size() * sizeof(int));
     up(c->d_halo, htab.data(), htab.size() * sizeof(int));
+    std::vector<int> isleaf(nb);
+    for (int b = 0; b < nb; b++) isleaf[b] = has_child[b] ? 0 : 1;
+    up(c->d_isleaf, isleaf.data(), isleaf.size() * sizeof(int));
     up(c->d_w64, w64.data(), w64.size() * sizeof(double));
     up(c->d_w32, w32.data(), w32.size() * sizeof(float));
     // reduction scratch: one partial per work group of the largest leaf launch
     int maxp = 1;
-    maxp += nlevels * (int)RESNORM_WGS;
+    maxp += nlevels * (int)RESNORM_WGS + nb;
     c->maxpartials = maxp;
     c->d_partial.alloc((size_t)maxp * sizeof(double));
-    c->d_sum.alloc(sizeof(double) * 4);
+    c->d_sum.alloc(sizeof(double) * (SUM_CHUNKS + 4));
     c->coarse.valid = false;
     c->sweep_cfg.assign(2 * nlevels, 0);
     c->zero[0].reset();
@@ -499,10 +504,29 @@ struct Ops {
     void smooth(FieldBuf &u, FieldBuf &b, const LevelArgs<T> &a, int l, int n) {
         for (int i = 0; i < n; i++) sweep(u, b, a, l);
     }
+    // residual kernels from LDS tiles (base tile shape)
+    static constexpr int RB = SMALL ? 256 / LY::H : 1;
+    static constexpr int RNT = SMALL ? 256 : 512;
     void residual(FieldBuf &r, FieldBuf &u, FieldBuf &b, const LevelArgs<T> &a, int l) {
+        if (!a.nblk) return;
+        unsigned g = (unsigned)((a.nblk + RB - 1) / RB);
+        hipLaunchKernelGGL((k_resid_tile<D, M, T, T, RB, RNT, RES_WRITE>), dim3(g), dim3(RNT), 0, c->stream,
+                           X(r, l), X(u, l), X(b, l), I(u), c->d_halo.as<int>(), coord(), (const int *)nullptr,
+                           (const int *)nullptr, a.nblk, a, 0.0, (double *)nullptr);
+    }
+    // the per-cell gather form of the residual (kept for timing comparisons)
+    void residual_gather(FieldBuf &r, FieldBuf &u, FieldBuf &b, const LevelArgs<T> &a, int l) {
         if (!a.nblk) return;
         hipLaunchKernelGGL((k_residual<D, M, T>), dim3(nwg((int64_t)a.nblk * LY::V)), dim3(WG), 0, c->stream,
                            X(r, l), X(u, l), X(b, l), I(u), nbr(), coord(), a);
+    }
+    // fixed-order total of the first np partials into d_sum[0]
+    void sum_partials(int np) {
+        int chunk = (np + SUM_CHUNKS - 1) / SUM_CHUNKS;
+        double *p2 = c->d_sum.as<double>() + 4;
+        hipLaunchKernelGGL(k_sum_chunks, dim3(SUM_CHUNKS), dim3(WG), 0, c->stream, c->d_partial.as<double>(), np,
+                           chunk, p2);
+        hipLaunchKernelGGL(k_sum, dim3(1), dim3(WG), 0, c->stream, p2, SUM_CHUNKS, c->d_sum.as<double>());
     }
     void restrict_(FieldBuf &x, int l) {
         const LevelInfo &L = c->lv[l];
@@ -555,15 +579,32 @@ struct Ops {
             if (!L.nleaf) continue;
             LevelArgs<T> a = level_args<T>(c, l, s, op, 1.0);
             double wgt = std::ldexp(1.0, -c->D * l);
-            unsigned g = std::min<unsigned>(nwg((int64_t)L.nleaf * LY::V), RESNORM_WGS);
-            hipLaunchKernelGGL((k_resnorm<D, M, T>), dim3(g), dim3(WG), 0, c->stream, X(u, l), X(b, l), I(u),
-                               nbr(), coord(), c->d_leaves.as<int>() + L.leaf0, L.nleaf, a, wgt,
+            unsigned g = (unsigned)((L.nleaf + RB - 1) / RB);
+            hipLaunchKernelGGL((k_resid_tile<D, M, T, T, RB, RNT, RES_NORM>), dim3(g), dim3(RNT), 0, c->stream,
+                               (T *)nullptr, X(u, l), X(b, l), I(u), c->d_halo.as<int>(), coord(),
+                               c->d_leaves.as<int>() + L.leaf0, (const int *)nullptr, L.nleaf, a, wgt,
                                c->d_partial.as<double>() + np);
             np += (int)g;
         }
-        hipLaunchKernelGGL(k_sum, dim3(1), dim3(WG), 0, c->stream, c->d_partial.as<double>(), np,
-                           c->d_sum.as<double>());
+        sum_partials(np);
         return np;
+    }
+    // mixed precision: be = b + L u / s (FP32) on every level, plus the leaf
+    // residual norm partials, in one pass (T = double)
+    void defect_and_norm(float *be, FieldBuf &u, FieldBuf &b, double s, const srmg_op *op) {
+        int np = 0;
+        for (int l = 0; l < c->nlevels; l++) {
+            LevelArgs<T> a = level_args<T>(c, l, s, op, 1.0);
+            if (!a.nblk) continue;
+            double wgt = std::ldexp(1.0, -c->D * l);
+            unsigned g = (unsigned)((a.nblk + RB - 1) / RB);
+            hipLaunchKernelGGL((k_resid_tile<D, M, T, float, RB, RNT, RES_DEFECT>), dim3(g), dim3(RNT), 0,
+                               c->stream, be, X(u, l), X(b, l), I(u), c->d_halo.as<int>(), coord(),
+                               (const int *)nullptr, c->d_isleaf.as<int>(), a.nblk, a, wgt,
+                               c->d_partial.as<double>() + np);
+            np += (int)g;
+        }
+        sum_partials(np);
     }
 
     // exact FP64 solve of L_1 u + s b = 0 on the complete level 1
@@ -1003,16 +1044,7 @@ static void solve_mixed_dm(srmg_ctx *c, FieldBuf &u, FieldBuf &b, FieldBuf &e, F
     o64.resnorm_partials(zero_field(c, SRMG_F64), b, s, op);
     double r0 = read_sum(c);
     if (!(r0 > 0.0)) r0 = 1.0;
-    auto defect_and_norm = [&] {
-        for (int l = 0; l < c->nlevels; l++) {
-            LevelArgs<double> a = level_args<double>(c, l, s, op, 1.0);
-            if (!a.nblk) continue;
-            hipLaunchKernelGGL((k_defect<D, M>), dim3(nwg((int64_t)a.nblk * LY::V)), dim3(WG), 0, c->stream,
-                               be.data.as<float>(), u.data.as<double>(), b.data.as<double>(), u.ib.as<double>(),
-                               c->d_nbr.as<int>(), c->d_coord.as<int>(), a);
-        }
-        o64.resnorm_partials(u, b, s, op);
-    };
+    auto defect_and_norm = [&] { o64.defect_and_norm(be.data.as<float>(), u, b, s, op); };
     auto correct = [&] {
         size_t n = (size_t)c->nb * LY::V, ni = e.ib.bytes / sizeof(float);
         hipLaunchKernelGGL(k_fill<float>, dim3(nwg(n)), dim3(WG), 0, c->stream, e.data.as<float>(), 0.0f, n);
@@ -1313,6 +1345,8 @@ int srmg_time_kernel(srmg_ctx *c, int kind, int level, int u, int b, int r, doub
             auto launch = [&](int i) {
                 if (kind == SRMG_KERNEL_GS_PASS) o.gs_pass(field(c, u), field(c, b), a, l, i & 1);
                 else if (kind == SRMG_KERNEL_RESIDUAL) o.residual(field(c, r), field(c, u), field(c, b), a, l);
+                else if (kind == SRMG_KERNEL_RESIDUAL_GATHER)
+                    o.residual_gather(field(c, r), field(c, u), field(c, b), a, l);
                 else if (kind == SRMG_KERNEL_GS_SWEEP) o.sweep(field(c, u), field(c, b), a, l);
                 else if (kind == SRMG_KERNEL_STREAM) {
                     // r = u + b over the level: reads 2, writes 1 value per cell

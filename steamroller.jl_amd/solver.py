@@ -27,7 +27,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libsrmg.so")
 
 F64, F32 = 0, 1
-KERNEL_GS_PASS, KERNEL_RESIDUAL, KERNEL_GS_SWEEP, KERNEL_STREAM = 0, 1, 2, 3
+KERNEL_GS_PASS, KERNEL_RESIDUAL, KERNEL_GS_SWEEP, KERNEL_STREAM, KERNEL_RESIDUAL_GATHER = 0, 1, 2, 3, 4
 COARSE_GS, COARSE_EXACT = 0, 1
 _DT = {F64: np.float64, F32: np.float32}
 
