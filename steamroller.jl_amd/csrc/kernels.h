@@ -387,32 +387,7 @@ struct Tile {
     __device__ static constexpr int stride(int d) { return d == 0 ? 1 : (d == 1 ? S : S * S); }
 };
 
-// stencil of the tile cell at p (all neighbours present in the tile)
-template <int D, int M, typename T>
-__device__ __forceinline__ T tile_stencil(const T *t, int p, T own, const LevelArgs<T> &a, T wmr, T wpr) {
-    using TL = Tile<D, M>;
-    T v = T(0);
-#pragma unroll
-    for (int d = D - 1; d >= 0; d--) v += ((d == a.rdim) ? wmr : T(1)) * t[p - TL::stride(d)];
-    v += a.cc * own;
-#pragma unroll
-    for (int d = 0; d < D; d++) v += ((d == a.rdim) ? wpr : T(1)) * t[p + TL::stride(d)];
-    return v;
-}
-
-template <int D, int M, typename T>
-__device__ __forceinline__ void weights_at(const LevelArgs<T> &a, const int *__restrict__ coord, int blk,
-                                           const int *c, T &wm, T &wp) {
-    wm = T(1);
-    wp = T(1);
-    if (a.rdim >= 0) {
-        int J = M * (coord[blk * D + a.rdim] - 1) + c[a.rdim] + 1;
-        wm = a.wm[J];
-        wp = a.wp[J];
-    }
-}
-
-// Per-block halo table used by the fused sweep: the block's own 2D face
+// Per-block halo table used by the tiled kernels: the block's own 2D face
 // entries, then for each face f the neighbour-table entries of the block across
 // f on its 2(D-1) side faces (tangential dimensions in increasing order, low
 // side first); unused when face f is not a same-level face.
@@ -421,9 +396,180 @@ struct Halo {
     static constexpr int ND = 2 * D;
     static constexpr int NS = 2 * (D - 1);
     static constexpr int N = ND + ND * NS;
-    __host__ __device__ static constexpr int side_index(int dim, int dd, int side) {
-        // position of tangential face (dd, side) among the side faces of a face normal to dim
-        return 2 * ((dd < dim) ? dd : dd - 1) + side;
+};
+
+// Stencil of the tile cell at p, all neighbours in the tile. RD is the radial
+// dimension (cylindrical geometry) or -1; terms in the reference's order.
+template <int D, int M, typename T, int RD>
+__device__ __forceinline__ T tstencil(const T *t, int p, T own, T cc, T wm, T wp) {
+    using TL = Tile<D, M>;
+    T v = T(0);
+#pragma unroll
+    for (int d = D - 1; d >= 0; d--) {
+        if (d == RD) v += wm * t[p - TL::stride(d)];
+        else v += t[p - TL::stride(d)];
+    }
+    v += cc * own;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        if (d == RD) v += wp * t[p + TL::stride(d)];
+        else v += t[p + TL::stride(d)];
+    }
+    return v;
+}
+
+template <int D, int M, typename T, int RD>
+__device__ __forceinline__ void rweights(const LevelArgs<T> &a, const int *__restrict__ coord, int blk, int cr,
+                                         T &wm, T &wp) {
+    wm = T(1);
+    wp = T(1);
+    if (RD >= 0) {
+        int J = M * (coord[blk * D + RD] - 1) + cr + 1;
+        wm = a.wm[J];
+        wp = a.wp[J];
+    }
+}
+
+// Load the depth-1 halo cell number t of face F (a compile-time face) of block
+// blk into the tile: neighbour cell, mirror of the facing cell, or frozen slab.
+template <int D, int M, typename T, int F>
+__device__ __forceinline__ void halo_cell(T *tile, int e, int blk, int t, const T *__restrict__ x,
+                                          const T *__restrict__ ib, T sg) {
+    using LY = Lay<D, M>;
+    using TL = Tile<D, M>;
+    constexpr int DIM = F >> 1, DIR = (F & 1) ? 1 : -1;
+    int c[3] = {0, 0, 0};
+    int ta = t % M, tb = t / M;
+    if (DIM == 0) { c[1] = ta; c[2] = tb; }
+    else if (DIM == 1) { c[0] = ta; c[2] = tb; }
+    else { c[0] = ta; c[1] = tb; }
+    T v;
+    if (e >= 0) {
+        c[DIM] = DIR > 0 ? 0 : M - 1;
+        v = x[(size_t)e * LY::V + LY::sidx(c[0], c[1], c[2])];
+    } else if (e == NB_BOUNDARY) {
+        c[DIM] = DIR > 0 ? M - 1 : 0;
+        v = sg * x[(size_t)blk * LY::V + LY::sidx(c[0], c[1], c[2])];
+    } else {
+        v = ib[(size_t)nb_slot(e) * LY::F + t];
+    }
+    c[DIM] = DIR > 0 ? M : -1;
+    tile[TL::at(c[0], c[1], c[2])] = v;
+}
+
+// halo_cell for a face known only at run time (uniform per wave where the
+// work layout allows it)
+template <int D, int M, typename T>
+__device__ __forceinline__ void halo_switch(T *tl, int e, int f, int blk, int t, const T *__restrict__ x,
+                                            const T *__restrict__ ib, const T *sg) {
+    switch (f) {
+        case 0: halo_cell<D, M, T, 0>(tl, e, blk, t, x, ib, sg[0]); break;
+        case 1: halo_cell<D, M, T, 1>(tl, e, blk, t, x, ib, sg[1]); break;
+        case 2: halo_cell<D, M, T, 2>(tl, e, blk, t, x, ib, sg[2]); break;
+        case 3: halo_cell<D, M, T, 3>(tl, e, blk, t, x, ib, sg[3]); break;
+        case 4: halo_cell<D, M, T, (D == 3 ? 4 : 3)>(tl, e, blk, t, x, ib, sg[4]); break;
+        default: halo_cell<D, M, T, (D == 3 ? 5 : 3)>(tl, e, blk, t, x, ib, sg[5]); break;
+    }
+}
+
+// Neighbour face-layer cells recomputed by the fused sweep: for the face
+// normal to DIM on side dir, the neighbour's first-half-sweep cells of its
+// layer facing this block, numbered tq in [0, F/2).
+template <int D, int M, typename T, int DIM>
+struct RecFace {
+    using LY = Lay<D, M>;
+    using TL = Tile<D, M>;
+    using HT = Halo<D>;
+    static constexpr int DA = DIM == 0 ? 1 : 0;
+    static constexpr int DB = DIM == 2 ? 1 : 2;
+    static constexpr int NS = 2 * (D - 1);
+
+    __device__ static __forceinline__ void cell(int dir, int tq, int *cB) {
+        constexpr int MH = LY::MH;
+        int w = dir > 0 ? 0 : M - 1;
+        int tb = (D == 3) ? tq / MH : 0;
+        cB[0] = cB[1] = cB[2] = 0;
+        cB[DIM] = w;
+        cB[DA] = 2 * (tq % MH) + ((1 + w + tb + D) & 1);
+        if (D == 3) cB[DB] = tb;
+    }
+
+    // operands outside the tile: one layer deeper in the neighbour, its
+    // right-hand side, and its ghosts across its side faces
+    __device__ static __forceinline__ void prefetch(int dir, int tq, int e, const int *tb_own, const T *__restrict__ x,
+                                                    const T *__restrict__ b, const T *__restrict__ ib, T &rd, T &rb,
+                                                    T *rg) {
+        int cB[3];
+        cell(dir, tq, cB);
+        int c2[3] = {cB[0], cB[1], cB[2]};
+        c2[DIM] = dir > 0 ? 1 : M - 2;
+        rd = x[(size_t)e * LY::V + LY::sidx(c2[0], c2[1], c2[2])];
+        rb = b[(size_t)e * LY::V + LY::sidx(cB[0], cB[1], cB[2])];
+        const int f = 2 * DIM + (dir > 0 ? 1 : 0);
+#pragma unroll
+        for (int k = 0; k < NS; k++) {
+            const int dd = (k < 2) ? DA : DB;
+            const int side = k & 1;
+            rg[k] = T(0);
+            if (cB[dd] != (side ? M - 1 : 0)) continue;
+            int g = tb_own[HT::ND + f * HT::NS + k];
+            if (g >= 0) {
+                int c3[3] = {cB[0], cB[1], cB[2]};
+                c3[dd] = side ? 0 : M - 1;
+                rg[k] = x[(size_t)g * LY::V + LY::sidx(c3[0], c3[1], c3[2])];
+            } else if (g != NB_BOUNDARY) {
+                rg[k] = ib[(size_t)nb_slot(g) * LY::F + LY::faceidx(dd, cB[0], cB[1], cB[2])];
+            }
+        }
+    }
+
+    // first-half-sweep value of the neighbour cell, written into this block's halo
+    template <int RD>
+    __device__ static __forceinline__ void compute(int dir, int tq, int e, const int *tb_own, T *tile,
+                                                   const int *__restrict__ coord, const LevelArgs<T> &a, T rd,
+                                                   T rb, const T *rg) {
+        int cB[3];
+        cell(dir, tq, cB);
+        const int f = 2 * DIM + (dir > 0 ? 1 : 0);
+        int ch[3] = {cB[0], cB[1], cB[2]};
+        ch[DIM] = dir > 0 ? M : -1;
+        const int ph = TL::at(ch[0], ch[1], ch[2]);
+        const T xo = tile[ph];
+        T wm, wp;
+        rweights<D, M, T, RD>(a, coord, e, RD >= 0 ? cB[RD < 0 ? 0 : RD] : 0, wm, wp);
+        T nbm[3], nbp[3];
+        // across the shared face: this block's cell; one layer deeper: rd
+        const int pa = TL::at(DIM == 0 ? (dir > 0 ? M - 1 : 0) : cB[0], DIM == 1 ? (dir > 0 ? M - 1 : 0) : cB[1],
+                              DIM == 2 ? (dir > 0 ? M - 1 : 0) : cB[2]);
+        if (dir > 0) { nbm[DIM] = tile[pa]; nbp[DIM] = rd; }
+        else { nbm[DIM] = rd; nbp[DIM] = tile[pa]; }
+#pragma unroll
+        for (int k = 0; k < NS; k++) {
+            const int dd = (k < 2) ? DA : DB;
+            const int side = k & 1;
+            const int cc = cB[dd] + (side ? 1 : -1);
+            T v;
+            if (cc >= 0 && cc < M) {
+                v = tile[ph + (side ? 1 : -1) * TL::stride(dd)];  // along the face layer: in the halo
+            } else {
+                int g = tb_own[HT::ND + f * HT::NS + k];
+                v = (g == NB_BOUNDARY) ? a.sg[2 * dd + side] * xo : rg[k];
+            }
+            if (side) nbp[dd] = v; else nbm[dd] = v;
+        }
+        T Lx = T(0);
+#pragma unroll
+        for (int dd = D - 1; dd >= 0; dd--) {
+            if (dd == RD) Lx += wm * nbm[dd];
+            else Lx += nbm[dd];
+        }
+        Lx += a.cc * xo;
+#pragma unroll
+        for (int dd = 0; dd < D; dd++) {
+            if (dd == RD) Lx += wp * nbp[dd];
+            else Lx += nbp[dd];
+        }
+        tile[ph] = fma_t<T>(a.nwc, fma_t<T>(a.s, rb, Lx), xo);
     }
 };
 
@@ -437,8 +583,9 @@ struct Halo {
 // two-launch form. Boundary ghosts mirror the cell they face, and coarse-fine
 // ghosts stay frozen during smoothing (src/multigrid.jl:256-263).
 // Memory round trips before compute: (interior, b, halo table), then (halo
-// cells, recompute operands).
-template <int D, int M, typename T, int BPW, int NT>
+// cells, recompute operands). Faces are compile-time constants in unrolled
+// loops, so work that is uniform per face carries no per-lane selects.
+template <int D, int M, typename T, int BPW, int NT, int RD>
 __global__ __launch_bounds__(NT) void k_gs_sweep(T *__restrict__ uo, const T *__restrict__ ui,
                                                  const T *__restrict__ b, const T *__restrict__ ib,
                                                  const int *__restrict__ htab, const int *__restrict__ coord,
@@ -446,11 +593,12 @@ __global__ __launch_bounds__(NT) void k_gs_sweep(T *__restrict__ uo, const T *__
     using LY = Lay<D, M>;
     using TL = Tile<D, M>;
     using HT = Halo<D>;
-    constexpr int F = LY::F, V = LY::V, H = LY::H, MH = LY::MH, F2 = F / 2;
+    constexpr int F = LY::F, V = LY::V, H = LY::H, F2 = F / 2, ND = 2 * D, NS = 2 * (D - 1);
     static_assert((BPW * H) % NT == 0, "cells per colour must be a multiple of the work-group size");
     constexpr int CPT = BPW * H / NT;
-    constexpr int NREC = BPW * 2 * D * F2;         // neighbour face-layer cells to recompute
+    constexpr int NREC = BPW * ND * F2;
     constexpr int RPT = (NREC + NT - 1) / NT;
+    constexpr int NHALO = BPW * ND * F;
     __shared__ T tile[BPW * TL::SZ];
     __shared__ int tab[BPW * HT::N];
     const int lb0 = xcd_remap(blockIdx.x, gridDim.x) * BPW;
@@ -459,183 +607,92 @@ __global__ __launch_bounds__(NT) void k_gs_sweep(T *__restrict__ uo, const T *__
     const int blk0 = a.blk0 + lb0;
 
     // round trip 1: interior cells, own right-hand side, halo tables
-    for (int i = tid; i < nloc * V; i += NT) {
-        int lb = i / V, q = i - lb * V;
-        int c[3] = {0, 0, 0};
-        LY::decode(q, c[0], c[1], c[2]);
-        tile[lb * TL::SZ + TL::at3(c)] = ui[(size_t)(blk0 + lb) * V + q];
-    }
-    T bo[CPT], be[CPT];
+    T ux[CPT][2], bo[CPT], be[CPT];
 #pragma unroll
     for (int k = 0; k < CPT; k++) {
         int i = tid + k * NT, lb = i / H, r = i - lb * H;
-        bo[k] = be[k] = T(0);
+        ux[k][0] = ux[k][1] = bo[k] = be[k] = T(0);
         if (lb < nloc) {
             size_t o = (size_t)(blk0 + lb) * V + r;
+            ux[k][0] = ui[o];
+            ux[k][1] = ui[o + H];
             bo[k] = b[o];
             be[k] = b[o + H];
         }
     }
     for (int i = tid; i < nloc * HT::N; i += NT) tab[i] = htab[(size_t)blk0 * HT::N + i];
+#pragma unroll
+    for (int k = 0; k < CPT; k++) {
+        int i = tid + k * NT, lb = i / H, r = i - lb * H;
+        if (lb < nloc) {
+            int x, y, z;
+            LY::decode(r, x, y, z);
+            tile[lb * TL::SZ + TL::at(x, y, z)] = ux[k][0];
+            LY::decode(H + r, x, y, z);
+            tile[lb * TL::SZ + TL::at(x, y, z)] = ux[k][1];
+        }
+    }
     __syncthreads();
 
-    // round trip 2: halo cells into the tile
-    for (int i = tid; i < nloc * 2 * D * F; i += NT) {
-        int lb = i / (2 * D * F), rem = i - lb * (2 * D * F);
+    // round trip 2: halo cells into the tile ...
+    for (int i = tid; i < NHALO; i += NT) {
+        int lb = i / (ND * F), rem = i - lb * (ND * F);
         int f = rem / F, t = rem - f * F;
-        int blk = blk0 + lb;
-        int dim = f >> 1, dir = (f & 1) ? 1 : -1;
-        int c[3] = {0, 0, 0};
-        int ta = t % M, tb = t / M;
-        if (dim == 0) { c[1] = ta; c[2] = tb; }
-        else if (dim == 1) { c[0] = ta; c[2] = tb; }
-        else { c[0] = ta; c[1] = tb; }
-        int e = tab[lb * HT::N + f];
-        T v;
-        if (e >= 0) {
-            c[dim] = dir > 0 ? 0 : M - 1;
-            v = ui[(size_t)e * V + LY::sidx(c[0], c[1], c[2])];
-        } else if (e == NB_BOUNDARY) {
-            c[dim] = dir > 0 ? M - 1 : 0;
-            v = a.sg[f] * ui[(size_t)blk * V + LY::sidx(c[0], c[1], c[2])];
-        } else {
-            v = ib[(size_t)nb_slot(e) * F + t];
-        }
-        c[dim] = dir > 0 ? M : -1;
-        tile[lb * TL::SZ + TL::at3(c)] = v;
+        if (lb >= nloc) continue;
+        halo_switch<D, M, T>(tile + lb * TL::SZ, tab[lb * HT::N + f], f, blk0 + lb, t, ui, ib, a.sg);
     }
-    // ... and the operands of the recomputed neighbour cells that are not in
-    // the tile: one layer deeper in the neighbour, its right-hand side, and its
-    // own ghosts across its side faces (side ghosts mirroring the cell itself
-    // are taken from the tile later)
-    T rd[RPT], rb[RPT], rg[RPT][2 * (D - 1)];
-    int rinfo[RPT];  // -1: nothing to do; else (lb, f, tq) packed
+    // ... and the operands of the recomputed neighbour cells
+    T rd[RPT], rb[RPT], rg[RPT][NS];
+    int rj[RPT];
 #pragma unroll
     for (int k = 0; k < RPT; k++) {
         int j = tid + k * NT;
-        rinfo[k] = -1;
+        rj[k] = -1;
         rd[k] = rb[k] = T(0);
-#pragma unroll
-        for (int q = 0; q < 2 * (D - 1); q++) rg[k][q] = T(0);
-        if (j >= nloc * 2 * D * F2) continue;
-        int lb = j / (2 * D * F2), rem = j - lb * (2 * D * F2);
+        if (j >= NREC) continue;
+        int lb = j / (ND * F2), rem = j - lb * (ND * F2);
         int f = rem / F2, tq = rem - f * F2;
-        int e = tab[lb * HT::N + f];
+        if (lb >= nloc) continue;
+        const int *tb = tab + lb * HT::N;
+        int e = tb[f];
         if (e < 0) continue;
-        rinfo[k] = j;
+        rj[k] = j;
         int dim = f >> 1, dir = (f & 1) ? 1 : -1;
-        int w = dir > 0 ? 0 : M - 1;
-        int da = dim == 0 ? 1 : 0, db = dim == 2 ? 1 : 2;
-        int tb = (D == 3) ? tq / MH : 0;
-        int ta = 2 * (tq % MH) + ((1 + w + tb + D) & 1);
-        int cB[3] = {0, 0, 0};
-        cB[dim] = w;
-        cB[da] = ta;
-        if (D == 3) cB[db] = tb;
-        int c2[3] = {cB[0], cB[1], cB[2]};
-        c2[dim] = dir > 0 ? 1 : M - 2;
-        rd[k] = ui[(size_t)e * V + LY::sidx(c2[0], c2[1], c2[2])];
-        rb[k] = b[(size_t)e * V + LY::sidx(cB[0], cB[1], cB[2])];
-#pragma unroll
-        for (int dd = 0; dd < D; dd++) {
-            if (dd == dim) continue;
-#pragma unroll
-            for (int side = 0; side < 2; side++) {
-                if (cB[dd] != (side ? M - 1 : 0)) continue;
-                int g = tab[lb * HT::N + HT::ND + f * HT::NS + HT::side_index(dim, dd, side)];
-                int q = HT::side_index(dim, dd, side);
-                if (g >= 0) {
-                    int c3[3] = {cB[0], cB[1], cB[2]};
-                    c3[dd] = side ? 0 : M - 1;
-                    rg[k][q] = ui[(size_t)g * V + LY::sidx(c3[0], c3[1], c3[2])];
-                } else if (g != NB_BOUNDARY) {
-                    rg[k][q] = ib[(size_t)nb_slot(g) * F + LY::faceidx(dd, cB[0], cB[1], cB[2])];
-                }
-            }
-        }
+        if (dim == 0) RecFace<D, M, T, 0>::prefetch(dir, tq, e, tb, ui, b, ib, rd[k], rb[k], rg[k]);
+        else if (dim == 1) RecFace<D, M, T, 1>::prefetch(dir, tq, e, tb, ui, b, ib, rd[k], rb[k], rg[k]);
+        else RecFace<D, M, T, (D == 3 ? 2 : 1)>::prefetch(dir, tq, e, tb, ui, b, ib, rd[k], rb[k], rg[k]);
     }
     __syncthreads();
 
     // first half-sweep: cells with odd global index sum
-    T n0[CPT];
 #pragma unroll
     for (int k = 0; k < CPT; k++) {
         int i = tid + k * NT, lb = i / H, r = i - lb * H;
-        n0[k] = T(0);
         if (lb < nloc) {
-            int blk = blk0 + lb;
-            int c[3] = {0, 0, 0};
-            LY::decode(r, c[0], c[1], c[2]);
-            int p = lb * TL::SZ + TL::at3(c);
-            T own = tile[p];
+            int x, y, z;
+            LY::decode(r, x, y, z);
+            int p = lb * TL::SZ + TL::at(x, y, z);
             T wm, wp;
-            weights_at<D, M, T>(a, coord, blk, c, wm, wp);
-            T Lu = tile_stencil<D, M, T>(tile + 0, p, own, a, wm, wp);
-            T nv = fma_t<T>(a.nwc, fma_t<T>(a.s, bo[k], Lu), own);
-            tile[p] = nv;
-            n0[k] = nv;
+            rweights<D, M, T, RD>(a, coord, blk0 + lb, RD == 0 ? x : y, wm, wp);
+            T Lu = tstencil<D, M, T, RD>(tile, p, ux[k][0], a.cc, wm, wp);
+            ux[k][0] = fma_t<T>(a.nwc, fma_t<T>(a.s, bo[k], Lu), ux[k][0]);
+            tile[p] = ux[k][0];
         }
     }
-    // the same half-sweep on the neighbours' face-layer cells (odd cells only)
+    // the same half-sweep on the neighbours' face-layer cells
 #pragma unroll
     for (int k = 0; k < RPT; k++) {
-        int j = rinfo[k];
+        int j = rj[k];
         if (j < 0) continue;
-        int lb = j / (2 * D * F2), rem = j - lb * (2 * D * F2);
+        int lb = j / (ND * F2), rem = j - lb * (ND * F2);
         int f = rem / F2, tq = rem - f * F2;
-        int e = tab[lb * HT::N + f];
+        const int *tb = tab + lb * HT::N;
+        T *tl = tile + lb * TL::SZ;
+        int e = tb[f];
         int dim = f >> 1, dir = (f & 1) ? 1 : -1;
-        int w = dir > 0 ? 0 : M - 1;
-        int da = dim == 0 ? 1 : 0, db = dim == 2 ? 1 : 2;
-        int tb = (D == 3) ? tq / MH : 0;
-        int ta = 2 * (tq % MH) + ((1 + w + tb + D) & 1);
-        int cB[3] = {0, 0, 0};
-        cB[dim] = w;
-        cB[da] = ta;
-        if (D == 3) cB[db] = tb;
-        int ch[3] = {cB[0], cB[1], cB[2]};
-        ch[dim] = dir > 0 ? M : -1;  // the same cell in this block's halo
-        int ph = lb * TL::SZ + TL::at3(ch);
-        T xo = tile[ph];
-        T wm, wp;
-        weights_at<D, M, T>(a, coord, e, cB, wm, wp);
-        T nbm[3], nbp[3];
-#pragma unroll
-        for (int dd = 0; dd < D; dd++) {
-#pragma unroll
-            for (int side = 0; side < 2; side++) {
-                int sdir = side ? 1 : -1;
-                T v;
-                if (dd == dim) {
-                    int cc = w + sdir;
-                    if (cc < 0 || cc >= M) {  // across the shared face: this block's cell
-                        int cA[3] = {cB[0], cB[1], cB[2]};
-                        cA[dim] = dir > 0 ? M - 1 : 0;
-                        v = tile[lb * TL::SZ + TL::at3(cA)];
-                    } else {
-                        v = rd[k];
-                    }
-                } else {
-                    int cc = cB[dd] + sdir;
-                    if (cc >= 0 && cc < M) {  // along the face layer: in the halo
-                        int c2[3] = {ch[0], ch[1], ch[2]};
-                        c2[dd] = cc;
-                        v = tile[lb * TL::SZ + TL::at3(c2)];
-                    } else {
-                        int g = tab[lb * HT::N + HT::ND + f * HT::NS + HT::side_index(dim, dd, side)];
-                        v = (g == NB_BOUNDARY) ? a.sg[2 * dd + side] * xo : rg[k][HT::side_index(dim, dd, side)];
-                    }
-                }
-                if (side) nbp[dd] = v; else nbm[dd] = v;
-            }
-        }
-        T Lx = T(0);
-#pragma unroll
-        for (int dd = D - 1; dd >= 0; dd--) Lx += ((dd == a.rdim) ? wm : T(1)) * nbm[dd];
-        Lx += a.cc * xo;
-#pragma unroll
-        for (int dd = 0; dd < D; dd++) Lx += ((dd == a.rdim) ? wp : T(1)) * nbp[dd];
-        tile[ph] = fma_t<T>(a.nwc, fma_t<T>(a.s, rb[k], Lx), xo);
+        if (dim == 0) RecFace<D, M, T, 0>::template compute<RD>(dir, tq, e, tb, tl, coord, a, rd[k], rb[k], rg[k]);
+        else if (dim == 1) RecFace<D, M, T, 1>::template compute<RD>(dir, tq, e, tb, tl, coord, a, rd[k], rb[k], rg[k]);
+        else RecFace<D, M, T, (D == 3 ? 2 : 1)>::template compute<RD>(dir, tq, e, tb, tl, coord, a, rd[k], rb[k], rg[k]);
     }
     __syncthreads();
 
@@ -644,17 +701,15 @@ __global__ __launch_bounds__(NT) void k_gs_sweep(T *__restrict__ uo, const T *__
     for (int k = 0; k < CPT; k++) {
         int i = tid + k * NT, lb = i / H, r = i - lb * H;
         if (lb < nloc) {
-            int blk = blk0 + lb;
-            int c[3] = {0, 0, 0};
-            LY::decode(H + r, c[0], c[1], c[2]);
-            int p = lb * TL::SZ + TL::at3(c);
-            T own = tile[p];
+            int x, y, z;
+            LY::decode(H + r, x, y, z);
+            int p = lb * TL::SZ + TL::at(x, y, z);
             T wm, wp;
-            weights_at<D, M, T>(a, coord, blk, c, wm, wp);
-            T Lu = tile_stencil<D, M, T>(tile + 0, p, own, a, wm, wp);
-            size_t o = (size_t)blk * V + r;
-            uo[o] = n0[k];
-            uo[o + H] = fma_t<T>(a.nwc, fma_t<T>(a.s, be[k], Lu), own);
+            rweights<D, M, T, RD>(a, coord, blk0 + lb, RD == 0 ? x : y, wm, wp);
+            T Lu = tstencil<D, M, T, RD>(tile, p, ux[k][1], a.cc, wm, wp);
+            size_t o = (size_t)(blk0 + lb) * V + r;
+            uo[o] = ux[k][0];
+            uo[o + H] = fma_t<T>(a.nwc, fma_t<T>(a.s, be[k], Lu), ux[k][1]);
         }
     }
 }
@@ -671,44 +726,25 @@ __device__ __forceinline__ void load_tile(T *tile, int *tab, int *blks, const T 
     using LY = Lay<D, M>;
     using TL = Tile<D, M>;
     using HT = Halo<D>;
-    constexpr int F = LY::F, V = LY::V;
+    constexpr int F = LY::F, V = LY::V, ND = 2 * D;
     const int tid = threadIdx.x;
     if (tid < BPW) blks[tid] = (tid < nloc) ? (list ? list[first + tid] : first + tid) : 0;
     __syncthreads();
     for (int i = tid; i < nloc * V; i += NT) {
         int lb = i / V, q = i - lb * V;
-        int c[3] = {0, 0, 0};
-        LY::decode(q, c[0], c[1], c[2]);
-        tile[lb * TL::SZ + TL::at3(c)] = x[(size_t)blks[lb] * V + q];
+        int cx, cy, cz;
+        LY::decode(q, cx, cy, cz);
+        tile[lb * TL::SZ + TL::at(cx, cy, cz)] = x[(size_t)blks[lb] * V + q];
     }
-    for (int i = tid; i < nloc * HT::ND; i += NT) {
-        int lb = i / HT::ND;
-        tab[i] = htab[(size_t)blks[lb] * HT::N + (i - lb * HT::ND)];
+    for (int i = tid; i < nloc * ND; i += NT) {
+        int lb = i / ND;
+        tab[i] = htab[(size_t)blks[lb] * HT::N + (i - lb * ND)];
     }
     __syncthreads();
-    for (int i = tid; i < nloc * 2 * D * F; i += NT) {
-        int lb = i / (2 * D * F), rem = i - lb * (2 * D * F);
+    for (int i = tid; i < nloc * ND * F; i += NT) {
+        int lb = i / (ND * F), rem = i - lb * (ND * F);
         int f = rem / F, t = rem - f * F;
-        int blk = blks[lb];
-        int dim = f >> 1, dir = (f & 1) ? 1 : -1;
-        int c[3] = {0, 0, 0};
-        int ta = t % M, tb = t / M;
-        if (dim == 0) { c[1] = ta; c[2] = tb; }
-        else if (dim == 1) { c[0] = ta; c[2] = tb; }
-        else { c[0] = ta; c[1] = tb; }
-        int e = tab[lb * HT::ND + f];
-        T v;
-        if (e >= 0) {
-            c[dim] = dir > 0 ? 0 : M - 1;
-            v = x[(size_t)e * V + LY::sidx(c[0], c[1], c[2])];
-        } else if (e == NB_BOUNDARY) {
-            c[dim] = dir > 0 ? M - 1 : 0;
-            v = sg[f] * x[(size_t)blk * V + LY::sidx(c[0], c[1], c[2])];
-        } else {
-            v = ib[(size_t)nb_slot(e) * F + t];
-        }
-        c[dim] = dir > 0 ? M : -1;
-        tile[lb * TL::SZ + TL::at3(c)] = v;
+        halo_switch<D, M, T>(tile + lb * TL::SZ, tab[lb * ND + f], f, blks[lb], t, x, ib, sg);
     }
     __syncthreads();
 }
@@ -721,7 +757,7 @@ enum { RES_WRITE = 0, RES_NORM = 1, RES_DEFECT = 2 };
 //   RES_DEFECT: out (float) = b + L u / s = -r on the level's blocks, and the
 //               partial sums of r^2 over its leaf blocks (isleaf[blk])
 // Work group partials are summed in a fixed order (LDS tree).
-template <int D, int M, typename T, typename TO, int BPW, int NT, int MODE>
+template <int D, int M, typename T, typename TO, int BPW, int NT, int MODE, int RD>
 __global__ __launch_bounds__(NT) void k_resid_tile(TO *__restrict__ out, const T *__restrict__ u,
                                                    const T *__restrict__ b, const T *__restrict__ ib,
                                                    const int *__restrict__ htab, const int *__restrict__ coord,
@@ -729,10 +765,9 @@ __global__ __launch_bounds__(NT) void k_resid_tile(TO *__restrict__ out, const T
                                                    int n, LevelArgs<T> a, double wgt, double *__restrict__ partial) {
     using LY = Lay<D, M>;
     using TL = Tile<D, M>;
-    using HT = Halo<D>;
     constexpr int V = LY::V;
     __shared__ T tile[BPW * TL::SZ];
-    __shared__ int tab[BPW * HT::ND];
+    __shared__ int tab[BPW * 2 * D];
     __shared__ int blks[BPW];
     __shared__ double red[NT];
     const int first = xcd_remap(blockIdx.x, gridDim.x) * BPW;
@@ -744,12 +779,12 @@ __global__ __launch_bounds__(NT) void k_resid_tile(TO *__restrict__ out, const T
     for (int i = threadIdx.x; i < nloc * V; i += NT) {
         int lb = i / V, q = i - lb * V;
         int blk = blks[lb];
-        int c[3] = {0, 0, 0};
-        LY::decode(q, c[0], c[1], c[2]);
-        int p = lb * TL::SZ + TL::at3(c);
+        int x, y, z;
+        LY::decode(q, x, y, z);
+        int p = lb * TL::SZ + TL::at(x, y, z);
         T wm, wp;
-        weights_at<D, M, T>(a, coord, blk, c, wm, wp);
-        T Lu = tile_stencil<D, M, T>(tile + 0, p, tile[p], a, wm, wp);
+        rweights<D, M, T, RD>(a, coord, blk, RD == 0 ? x : y, wm, wp);
+        T Lu = tstencil<D, M, T, RD>(tile, p, tile[p], a.cc, wm, wp);
         size_t o = (size_t)blk * V + q;
         T rr = -(b[o] + Lu / a.s);
         if (MODE == RES_WRITE) out[o] = (TO)rr;

@@ -450,6 +450,26 @@ struct Ops {
     const int *nbr() { return c->d_nbr.as<int>(); }
     const int *coord() { return c->d_coord.as<int>(); }
 
+    // run fn with the radial dimension as a compile-time constant
+    template <class Fn>
+    void with_rd(int rdim, Fn &&fn) {
+        if (rdim < 0) {
+            fn(std::integral_constant<int, -1>());
+            return;
+        }
+        if constexpr (D == 2) {
+            if (rdim == 0) {
+                fn(std::integral_constant<int, 0>());
+                return;
+            }
+            if (rdim == 1) {
+                fn(std::integral_constant<int, 1>());
+                return;
+            }
+        }
+        fail("cylindrical geometry is two-dimensional");
+    }
+
     // bring every level back to the primary buffer
     void settle(FieldBuf &f) {
         for (int l = 0; l < c->nlevels; l++) {
@@ -483,8 +503,11 @@ struct Ops {
     void sweep_launch(T *out, T *in, FieldBuf &u, FieldBuf &b, const LevelArgs<T> &a, int l) {
         constexpr int BPW = cfg_bpw<K>(), NT = cfg_nt<K>();
         unsigned g = (unsigned)((a.nblk + BPW - 1) / BPW);
-        hipLaunchKernelGGL((k_gs_sweep<D, M, T, BPW, NT>), dim3(g), dim3(NT), 0, c->stream, out, in, X(b, l), I(u),
-                           c->d_halo.as<int>(), coord(), a);
+        with_rd(a.rdim, [&](auto rd) {
+            constexpr int RD = decltype(rd)::value;
+            hipLaunchKernelGGL((k_gs_sweep<D, M, T, BPW, NT, RD>), dim3(g), dim3(NT), 0, c->stream, out, in, X(b, l),
+                               I(u), c->d_halo.as<int>(), coord(), a);
+        });
     }
     void sweep(FieldBuf &u, FieldBuf &b, const LevelArgs<T> &a, int l, int cfg = -1) {
         if (!a.nblk) return;
@@ -510,9 +533,12 @@ struct Ops {
     void residual(FieldBuf &r, FieldBuf &u, FieldBuf &b, const LevelArgs<T> &a, int l) {
         if (!a.nblk) return;
         unsigned g = (unsigned)((a.nblk + RB - 1) / RB);
-        hipLaunchKernelGGL((k_resid_tile<D, M, T, T, RB, RNT, RES_WRITE>), dim3(g), dim3(RNT), 0, c->stream,
-                           X(r, l), X(u, l), X(b, l), I(u), c->d_halo.as<int>(), coord(), (const int *)nullptr,
-                           (const int *)nullptr, a.nblk, a, 0.0, (double *)nullptr);
+        with_rd(a.rdim, [&](auto rd) {
+            constexpr int RD = decltype(rd)::value;
+            hipLaunchKernelGGL((k_resid_tile<D, M, T, T, RB, RNT, RES_WRITE, RD>), dim3(g), dim3(RNT), 0, c->stream,
+                               X(r, l), X(u, l), X(b, l), I(u), c->d_halo.as<int>(), coord(), (const int *)nullptr,
+                               (const int *)nullptr, a.nblk, a, 0.0, (double *)nullptr);
+        });
     }
     // the per-cell gather form of the residual (kept for timing comparisons)
     void residual_gather(FieldBuf &r, FieldBuf &u, FieldBuf &b, const LevelArgs<T> &a, int l) {
@@ -580,10 +606,13 @@ struct Ops {
             LevelArgs<T> a = level_args<T>(c, l, s, op, 1.0);
             double wgt = std::ldexp(1.0, -c->D * l);
             unsigned g = (unsigned)((L.nleaf + RB - 1) / RB);
-            hipLaunchKernelGGL((k_resid_tile<D, M, T, T, RB, RNT, RES_NORM>), dim3(g), dim3(RNT), 0, c->stream,
-                               (T *)nullptr, X(u, l), X(b, l), I(u), c->d_halo.as<int>(), coord(),
-                               c->d_leaves.as<int>() + L.leaf0, (const int *)nullptr, L.nleaf, a, wgt,
-                               c->d_partial.as<double>() + np);
+            with_rd(a.rdim, [&](auto rd) {
+                constexpr int RD = decltype(rd)::value;
+                hipLaunchKernelGGL((k_resid_tile<D, M, T, T, RB, RNT, RES_NORM, RD>), dim3(g), dim3(RNT), 0,
+                                   c->stream, (T *)nullptr, X(u, l), X(b, l), I(u), c->d_halo.as<int>(), coord(),
+                                   c->d_leaves.as<int>() + L.leaf0, (const int *)nullptr, L.nleaf, a, wgt,
+                                   c->d_partial.as<double>() + np);
+            });
             np += (int)g;
         }
         sum_partials(np);
@@ -598,10 +627,13 @@ struct Ops {
             if (!a.nblk) continue;
             double wgt = std::ldexp(1.0, -c->D * l);
             unsigned g = (unsigned)((a.nblk + RB - 1) / RB);
-            hipLaunchKernelGGL((k_resid_tile<D, M, T, float, RB, RNT, RES_DEFECT>), dim3(g), dim3(RNT), 0,
-                               c->stream, be, X(u, l), X(b, l), I(u), c->d_halo.as<int>(), coord(),
-                               (const int *)nullptr, c->d_isleaf.as<int>(), a.nblk, a, wgt,
-                               c->d_partial.as<double>() + np);
+            with_rd(a.rdim, [&](auto rd) {
+                constexpr int RD = decltype(rd)::value;
+                hipLaunchKernelGGL((k_resid_tile<D, M, T, float, RB, RNT, RES_DEFECT, RD>), dim3(g), dim3(RNT), 0,
+                                   c->stream, be, X(u, l), X(b, l), I(u), c->d_halo.as<int>(), coord(),
+                                   (const int *)nullptr, c->d_isleaf.as<int>(), a.nblk, a, wgt,
+                                   c->d_partial.as<double>() + np);
+            });
             np += (int)g;
         }
         sum_partials(np);
