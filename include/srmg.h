@@ -133,6 +133,7 @@ typedef struct {
     double tol;                /* stop when resnorm(u) <= tol * resnorm(u = 0) */
     int32_t max_cycles;
     int32_t use_graph;         /* capture one cycle in a hipGraph and replay it */
+    int32_t inner_cycles;      /* mixed precision: FP32 V-cycles per FP64 correction (0 means 1) */
 } srmg_solve_opts;
 
 /* Solve from the current u (ghosts pre-filled here) with plain V-cycles of
@@ -142,9 +143,11 @@ typedef struct {
 int srmg_solve(srmg_ctx *ctx, int u, int b, int r, int u1, double s, const srmg_op *op,
                const srmg_solve_opts *opts, int *cycles, double *hist);
 
-/* Mixed precision: u, b FP64; e, be, re, ue FP32 scratch fields. Each cycle
- * forms the FP64 residual, runs one FP32 V-cycle on the correction equation
- * from e = 0 and adds e to u; the coarsest-level solve is FP64. */
+/* Mixed precision: u, b FP64; e, be, re, ue FP32 scratch fields. Each outer
+ * step forms the FP64 residual, runs inner_cycles FP32 V-cycles on the
+ * correction equation from e = 0, adds e to u and restricts u so covered
+ * cells equal the mean of their children; the coarsest-level solve is FP64.
+ * cycles counts outer steps; hist has one entry per outer step. */
 int srmg_solve_mixed(srmg_ctx *ctx, int u, int b, int e, int be, int re, int ue, double s,
                      const srmg_op *op, const srmg_solve_opts *opts, int *cycles, double *hist);
 

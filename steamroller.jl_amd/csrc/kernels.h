@@ -856,6 +856,74 @@ __global__ __launch_bounds__(WG) void k_axis(const double *__restrict__ Mx, int 
     out[gi] = acc;
 }
 
+// Y = A X along one axis of a column-major grid: A is n x n (row-major), the
+// grid is viewed as [outer][n][inner] (inner = stride of the axis) and every
+// (outer, inner) pair is a column of X. Register-tiled FP64 GEMM: a work
+// group computes a 64 x 64 tile of Y (rows i, columns) with 256 threads, each
+// a 4 x 4 sub-tile, over k-chunks of 16 staged in LDS (zero-padded past n).
+// Each output is one dot product summed in k order.
+__global__ __launch_bounds__(256) void k_axis_gemm(const double *__restrict__ A, int n, int inner, int ncol,
+                                                   const double *__restrict__ in, double *__restrict__ out) {
+    constexpr int TI = 64, TC = 64, TK = 16;
+    __shared__ double As[TK][TI + 1];
+    __shared__ double Xs[TK][TC + 1];
+    const int tid = threadIdx.x;
+    const int i0 = blockIdx.y * TI, c0 = blockIdx.x * TC;
+    const int ti = (tid / 16) * 4, tc = (tid % 16) * 4;
+    double acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) acc[a][q] = 0.0;
+    for (int k0 = 0; k0 < n; k0 += TK) {
+        // A tile: rows i0..i0+63, columns k0..k0+15
+        for (int e = tid; e < TI * TK; e += 256) {
+            int ii = e / TK, kk = e - ii * TK;
+            int gi = i0 + ii;
+            As[kk][ii] = (gi < n && k0 + kk < n) ? A[(size_t)gi * n + k0 + kk] : 0.0;
+        }
+        // X tile: rows k0..k0+15, columns c0..c0+63
+        for (int e = tid; e < TK * TC; e += 256) {
+            int kk, cc;
+            if (inner == 1) { cc = e / TK; kk = e - cc * TK; }   // k contiguous per column
+            else { kk = e / TC; cc = e - kk * TC; }              // columns contiguous
+            int col = c0 + cc;
+            double v = 0.0;
+            if (col < ncol && k0 + kk < n) {
+                size_t o = (size_t)(col / inner), j = (size_t)(col % inner);
+                v = in[o * (size_t)n * inner + (size_t)(k0 + kk) * inner + j];
+            }
+            Xs[kk][cc] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < TK; kk++) {
+            double av[4], xv[4];
+#pragma unroll
+            for (int a = 0; a < 4; a++) av[a] = As[kk][ti + a];
+#pragma unroll
+            for (int q = 0; q < 4; q++) xv[q] = Xs[kk][tc + q];
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) acc[a][q] += av[a] * xv[q];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+        int gi = i0 + ti + a;
+        if (gi >= n) continue;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            int col = c0 + tc + q;
+            if (col >= ncol) continue;
+            size_t o = (size_t)(col / inner), j = (size_t)(col % inner);
+            out[o * (size_t)n * inner + (size_t)gi * inner + j] = acc[a][q];
+        }
+    }
+}
+
 __global__ __launch_bounds__(WG) void k_eig_scale(double *__restrict__ g, const double *__restrict__ l0,
                                                   const double *__restrict__ l1, const double *__restrict__ l2,
                                                   int N0, int N1, int D, double shift, size_t total) {

@@ -641,6 +641,11 @@ struct Ops {
 
     // exact FP64 solve of L_1 u + s b = 0 on the complete level 1
     void coarse_exact(FieldBuf &u, FieldBuf &b, double s, const srmg_op *op);
+    // out = A in along an axis of extent n and stride `inner` (ncol columns)
+    void axis(const double *A, int n, int inner, int ncol, const double *in, double *out) {
+        dim3 grid((unsigned)((ncol + 63) / 64), (unsigned)((n + 63) / 64));
+        hipLaunchKernelGGL(k_axis_gemm, grid, dim3(256), 0, c->stream, A, n, inner, ncol, in, out);
+    }
 
     void top(FieldBuf &u, FieldBuf &b, double s, const srmg_op *op, int ntop, double omega) {
         if (op->coarse == SRMG_COARSE_EXACT) {
@@ -834,10 +839,9 @@ void Ops<D, M, T>::coarse_exact(FieldBuf &u, FieldBuf &b, double s, const srmg_o
     double *g = c->d_grid.as<double>(), *g2 = c->d_grid2.as<double>();
     hipLaunchKernelGGL((k_grid_gather<D, M, T>), dim3(nwg((int64_t)L1.nblk * LY::V)), dim3(WG), 0, c->stream, g,
                        X(b, 0), coord(), L1.blk0, L1.nblk, N[0], N[1], -s);
-    size_t stride = 1;
+    int stride = 1;
     for (int d = 0; d < D; d++) {
-        hipLaunchKernelGGL(k_axis, dim3(nwg(total)), dim3(WG), 0, c->stream, cc.Q[d].as<double>(), N[d], stride,
-                           total, g, g2);
+        axis(cc.Q[d].as<double>(), N[d], stride, (int)(total / N[d]), g, g2);
         std::swap(g, g2);
         stride *= N[d];
     }
@@ -847,8 +851,7 @@ void Ops<D, M, T>::coarse_exact(FieldBuf &u, FieldBuf &b, double s, const srmg_o
                        D, shift, total);
     stride = 1;
     for (int d = 0; d < D; d++) {
-        hipLaunchKernelGGL(k_axis, dim3(nwg(total)), dim3(WG), 0, c->stream, cc.P[d].as<double>(), N[d], stride,
-                           total, g, g2);
+        axis(cc.P[d].as<double>(), N[d], stride, (int)(total / N[d]), g, g2);
         std::swap(g, g2);
         stride *= N[d];
     }
@@ -1081,7 +1084,8 @@ static void solve_mixed_dm(srmg_ctx *c, FieldBuf &u, FieldBuf &b, FieldBuf &e, F
         size_t n = (size_t)c->nb * LY::V, ni = e.ib.bytes / sizeof(float);
         hipLaunchKernelGGL(k_fill<float>, dim3(nwg(n)), dim3(WG), 0, c->stream, e.data.as<float>(), 0.0f, n);
         hipLaunchKernelGGL(k_fill<float>, dim3(nwg(ni)), dim3(WG), 0, c->stream, e.ib.as<float>(), 0.0f, ni);
-        o32.vcycle(e, be, re, ue, s, op, 0, o->ndown, o->ntop, o->nup, o->omega);
+        for (int k = 0; k < std::max(1, (int)o->inner_cycles); k++)
+            o32.vcycle(e, be, re, ue, s, op, 0, o->ndown, o->ntop, o->nup, o->omega);
         hipLaunchKernelGGL(k_add_correction, dim3(nwg(n)), dim3(WG), 0, c->stream, u.data.as<double>(),
                            e.data.as<float>(), n);
         // covered coarse cells = restriction of their children, as at the FAS

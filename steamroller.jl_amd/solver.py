@@ -81,7 +81,7 @@ class _Op(C.Structure):
 class _SolveOpts(C.Structure):
     _fields_ = [("ndown", C.c_int32), ("ntop", C.c_int32), ("nup", C.c_int32),
                 ("omega", C.c_double), ("tol", C.c_double), ("max_cycles", C.c_int32),
-                ("use_graph", C.c_int32)]
+                ("use_graph", C.c_int32), ("inner_cycles", C.c_int32)]
 
 
 def make_op(geometry, bc, disc, coarse: int = COARSE_GS) -> _Op:
@@ -311,10 +311,11 @@ def resnorm(r, b, u, s, tree=None, conn=None, geometry=None, bc=None, lpldisc=No
     return out.value
 
 
-def _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph):
+def _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph, inner_cycles=1):
     o = _SolveOpts()
     o.ndown, o.ntop, o.nup = int(ndown), int(ntop), int(nup)
     o.omega, o.tol, o.max_cycles, o.use_graph = float(omega), float(tol), int(max_cycles), int(use_graph)
+    o.inner_cycles = int(inner_cycles)
     return o
 
 
@@ -332,10 +333,11 @@ def solve(u, b, r, u1, s, geometry, bc, lpldisc, *, tol=1e-8, max_cycles=50, ndo
 
 
 def solve_mixed(u, b, e, be, re, ue, s, geometry, bc, lpldisc, *, tol=1e-8, max_cycles=50,
-                ndown=2, ntop=4, nup=2, omega=1.0, coarse=COARSE_EXACT, use_graph=True):
-    """FP64 defect correction around FP32 V-cycles (u, b FP64; e, be, re, ue FP32)."""
+                ndown=2, ntop=4, nup=2, omega=1.0, coarse=COARSE_EXACT, use_graph=True, inner_cycles=1):
+    """FP64 defect correction around FP32 V-cycles (u, b FP64; e, be, re, ue FP32).
+    Returns (outer steps, relative residual history per outer step)."""
     op = make_op(geometry, bc, lpldisc, coarse)
-    o = _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph)
+    o = _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph, inner_cycles)
     hist = np.zeros(max_cycles + 1)
     n = C.c_int()
     _check(lib().srmg_solve_mixed(u.ctx.h, u.id, b.id, e.id, be.id, re.id, ue.id, float(s),
