@@ -80,6 +80,22 @@ int srmg_set_tree(srmg_ctx *ctx, int nlevels, int64_t nblocks,
                   const int64_t *refboundary, const int64_t *refboundary_off,
                   const int64_t *child, const int64_t *child_off);
 
+/* Several independent trees in one context (a forest): each tree is an
+ * independent problem (own tree, connectivity and later its own s and
+ * operator), and every level-wide launch covers the blocks of all trees, so
+ * one launch per smoother step serves them all. Arrays have one entry per
+ * tree; the records are as in srmg_set_tree, with each tree's own 1-based
+ * block ids. Host fields of a forest are the trees' host arrays concatenated
+ * in tree order. Level 1 of every tree must be complete. */
+int srmg_set_forest(srmg_ctx *ctx, int ntrees, const int *nlevels, const int64_t *nblocks,
+                    const int64_t *const *pairs, const int64_t *const *pairs_off,
+                    const int64_t *const *neighbor, const int64_t *const *neighbor_off,
+                    const int64_t *const *boundary, const int64_t *const *boundary_off,
+                    const int64_t *const *refboundary, const int64_t *const *refboundary_off,
+                    const int64_t *const *child, const int64_t *const *child_off);
+/* Number of trees of the context. */
+int srmg_tree_count(srmg_ctx *ctx, int *ntrees);
+
 /* Device-resident block field of the given element type; returns its id. */
 int srmg_field_create(srmg_ctx *ctx, int dtype, int *id);
 int srmg_field_destroy(srmg_ctx *ctx, int id);
@@ -113,6 +129,13 @@ int srmg_fill_ghost_all(srmg_ctx *ctx, int id, const int8_t *bc);
 int srmg_vcycle(srmg_ctx *ctx, int u, int b, int r, int u1, double s, const srmg_op *op,
                 int lmax, int ndown, int ntop, int nup, double omega);
 
+/* vcycle! on every tree of a forest at once, tree t with scale s[t] and
+ * operator ops[t] (geometry, boundary table and coarse treatment must agree;
+ * k2 and the discretization may differ, e.g. the terms of a photoionization
+ * model). Each tree gets exactly its own vcycle!. */
+int srmg_vcycle_batched(srmg_ctx *ctx, int u, int b, int r, int u1, const double *s, const srmg_op *ops,
+                        int lmax, int ndown, int ntop, int nup, double omega);
+
 /* fmg! (src/multigrid.jl:17-51). */
 int srmg_fmg(srmg_ctx *ctx, int u, int b, int r, int u1, double s, const srmg_op *op,
              int ndown, int ntop, int nup, double omega);
@@ -121,6 +144,8 @@ int srmg_fmg(srmg_ctx *ctx, int u, int b, int r, int u1, double s, const srmg_op
  * r^2 2^(-D(l-1)), r = -(b + L u / s_l). Synchronizes. Does not modify r
  * (the reference overwrites its r argument with squared residuals). */
 int srmg_resnorm(srmg_ctx *ctx, int r, int b, int u, double s, const srmg_op *op, double *out);
+/* resnorm! of every tree of a forest: out[t]. */
+int srmg_resnorm_batched(srmg_ctx *ctx, int r, int b, int u, const double *s, const srmg_op *ops, double *out);
 
 /* Block until all work queued on the context's stream has finished. */
 int srmg_synchronize(srmg_ctx *ctx);
@@ -143,6 +168,11 @@ typedef struct {
 int srmg_solve(srmg_ctx *ctx, int u, int b, int r, int u1, double s, const srmg_op *op,
                const srmg_solve_opts *opts, int *cycles, double *hist);
 
+/* srmg_solve on every tree of a forest (per-tree s and operator) until all
+ * trees reach tol; hist (may be NULL) has ntrees x (max_cycles+1) entries. */
+int srmg_solve_batched(srmg_ctx *ctx, int u, int b, int r, int u1, const double *s, const srmg_op *ops,
+                       const srmg_solve_opts *opts, int *cycles, double *hist);
+
 /* Mixed precision: u, b FP64; e, be, re, ue FP32 scratch fields. Each outer
  * step forms the FP64 residual, runs inner_cycles FP32 V-cycles on the
  * correction equation from e = 0, adds e to u and restricts u so covered
@@ -150,6 +180,9 @@ int srmg_solve(srmg_ctx *ctx, int u, int b, int r, int u1, double s, const srmg_
  * cycles counts outer steps; hist has one entry per outer step. */
 int srmg_solve_mixed(srmg_ctx *ctx, int u, int b, int e, int be, int re, int ue, double s,
                      const srmg_op *op, const srmg_solve_opts *opts, int *cycles, double *hist);
+/* srmg_solve_mixed on every tree of a forest; hist as in srmg_solve_batched. */
+int srmg_solve_mixed_batched(srmg_ctx *ctx, int u, int b, int e, int be, int re, int ue, const double *s,
+                             const srmg_op *ops, const srmg_solve_opts *opts, int *cycles, double *hist);
 
 /* ---- measurement ---- */
 

@@ -27,7 +27,25 @@ struct LevelArgs {
     int rdim;      // radial dimension (cylindrical) or -1
     const T *wm;   // (2J-2)/(2J-1) by 1-based global radial index J at this level
     const T *wp;   // 2J/(2J-1)
+    // several problems in one launch: (s, cc, nwc) of block blk are
+    // pp[3 pix[blk] + 0..2]; nullptr when the context holds one problem
+    const T *pp;
+    const int *pix;
 };
+
+template <typename T>
+struct BP {
+    T s, cc, nwc;
+};
+
+template <typename T>
+__device__ __forceinline__ BP<T> bparams(const LevelArgs<T> &a, int blk) {
+    if (a.pp) {
+        const T *q = a.pp + 3 * a.pix[blk];
+        return {q[0], q[1], q[2]};
+    }
+    return {a.s, a.cc, a.nwc};
+}
 
 template <typename T> __device__ __forceinline__ T fma_t(T a, T b, T c);
 template <> __device__ __forceinline__ double fma_t<double>(double a, double b, double c) { return fma(a, b, c); }
@@ -63,14 +81,14 @@ __device__ __forceinline__ T nbv(const T *__restrict__ x, const T *__restrict__ 
 template <int D, int M, typename T>
 __device__ __forceinline__ T stencil(const T *__restrict__ x, const T *__restrict__ ib,
                                      const int *__restrict__ nbr, int blk, int c0, int c1, int c2,
-                                     T own, const LevelArgs<T> &a, T wmr, T wpr) {
+                                     T own, const LevelArgs<T> &a, T cc, T wmr, T wpr) {
     T v = T(0);
 #pragma unroll
     for (int d = D - 1; d >= 0; d--) {
         T w = (d == a.rdim) ? wmr : T(1);
         v += w * nbv<D, M, T>(x, ib, nbr, blk, c0, c1, c2, d, -1, a.sg);
     }
-    v += a.cc * own;
+    v += cc * own;
 #pragma unroll
     for (int d = 0; d < D; d++) {
         T w = (d == a.rdim) ? wpr : T(1);
@@ -111,9 +129,10 @@ __global__ __launch_bounds__(WG) void k_gs_pass(T *__restrict__ u, const T *__re
     T own = u[o];
     T wm, wp;
     radial_weights<D, M, T>(a, coord, blk, c0, c1, wm, wp);
-    T Lu = stencil<D, M, T>(u, ib, nbr, blk, c0, c1, c2, own, a, wm, wp);
-    T v = fma_t<T>(a.s, b[o], Lu);
-    u[o] = fma_t<T>(a.nwc, v, own);
+    BP<T> bp = bparams(a, blk);
+    T Lu = stencil<D, M, T>(u, ib, nbr, blk, c0, c1, c2, own, a, bp.cc, wm, wp);
+    T v = fma_t<T>(bp.s, b[o], Lu);
+    u[o] = fma_t<T>(bp.nwc, v, own);
 }
 
 // residual! over a level (src/multigrid.jl:309-323): r = -(b + L u / s)
@@ -133,8 +152,9 @@ __global__ __launch_bounds__(WG) void k_residual(T *__restrict__ r, const T *__r
     size_t o = (size_t)blk * LY::V + q;
     T wm, wp;
     radial_weights<D, M, T>(a, coord, blk, c0, c1, wm, wp);
-    T Lu = stencil<D, M, T>(u, ib, nbr, blk, c0, c1, c2, u[o], a, wm, wp);
-    r[o] = -(b[o] + Lu / a.s);
+    BP<T> bp = bparams(a, blk);
+    T Lu = stencil<D, M, T>(u, ib, nbr, blk, c0, c1, c2, u[o], a, bp.cc, wm, wp);
+    r[o] = -(b[o] + Lu / bp.s);
 }
 
 // restrict! over the Child edges of a level (src/restrict.jl:5-37): each
@@ -189,8 +209,9 @@ __global__ __launch_bounds__(WG) void k_postrestrict(T *__restrict__ bout, const
     size_t o = (size_t)pb * LY::V + LY::sidx(X0, X1, X2);
     T wm, wp;
     radial_weights<D, M, T>(a, coord, pb, X0, X1, wm, wp);
-    T Lu = stencil<D, M, T>(u, ib, nbr, pb, X0, X1, X2, u[o], a, wm, wp);
-    bout[o] = -(r[o] + Lu / a.s);
+    BP<T> bp = bparams(a, pb);
+    T Lu = stencil<D, M, T>(u, ib, nbr, pb, X0, X1, X2, u[o], a, bp.cc, wm, wp);
+    bout[o] = -(r[o] + Lu / bp.s);
 }
 
 // copyto! / diffto! over the valid cells of a block range (src/blockfield.jl:557-576)
@@ -296,43 +317,6 @@ __global__ __launch_bounds__(WG) void k_fill_interp(T *__restrict__ ib, const T 
     ib[(size_t)k * LY::F + t] = f;
 }
 
-// Squared-residual sum over the leaf blocks of a level, one partial per work
-// group (grid-stride loop over cells in a fixed order, then an LDS tree);
-// `wgt` = 2^(-D(l-1)). Equals what resnorm! obtains by restricting squared
-// residuals down to level 1 (src/multigrid.jl:391-417).
-template <int D, int M, typename T>
-__global__ __launch_bounds__(WG) void k_resnorm(const T *__restrict__ u, const T *__restrict__ b,
-                                                const T *__restrict__ ib, const int *__restrict__ nbr,
-                                                const int *__restrict__ coord, const int *__restrict__ leaves,
-                                                int nleaf, LevelArgs<T> a, double wgt,
-                                                double *__restrict__ partial) {
-    using LY = Lay<D, M>;
-    __shared__ double red[WG];
-    double acc = 0.0;
-    const size_t total = (size_t)nleaf * LY::V;
-    for (size_t gt = (size_t)blockIdx.x * WG + threadIdx.x; gt < total; gt += (size_t)gridDim.x * WG) {
-        int lb = (int)(gt / LY::V);
-        int blk = leaves[lb];
-        int q = (int)(gt - (size_t)lb * LY::V);
-        int c0, c1, c2;
-        LY::decode(q, c0, c1, c2);
-        size_t o = (size_t)blk * LY::V + q;
-        T wm, wp;
-        radial_weights<D, M, T>(a, coord, blk, c0, c1, wm, wp);
-        T Lu = stencil<D, M, T>(u, ib, nbr, blk, c0, c1, c2, u[o], a, wm, wp);
-        double r = (double)(-(b[o] + Lu / a.s));
-        acc += r * r;
-    }
-    red[threadIdx.x] = acc;
-    __syncthreads();
-#pragma unroll
-    for (int w = WG / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) partial[blockIdx.x] = red[0] * wgt;
-}
-
 // First stage of a fixed-order sum: work group g sums the contiguous chunk
 // [g * chunk, (g + 1) * chunk) of the n partials.
 __global__ __launch_bounds__(WG) void k_sum_chunks(const double *__restrict__ partial, int n, int chunk,
@@ -341,6 +325,26 @@ __global__ __launch_bounds__(WG) void k_sum_chunks(const double *__restrict__ pa
     int lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
     double acc = 0.0;
     for (int i = lo + threadIdx.x; i < hi; i += WG) acc += partial[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = WG / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+// Work group g sums the ranges rng[g * nr .. (g + 1) * nr) of the partials
+// (int2: first, count) in order into out[g]: the per-problem totals when one
+// launch holds several problems.
+__global__ __launch_bounds__(WG) void k_sum_ranges(const double *__restrict__ partial, const int2 *__restrict__ rng,
+                                                   int nr, double *__restrict__ out) {
+    __shared__ double red[WG];
+    double acc = 0.0;
+    for (int k = 0; k < nr; k++) {
+        int2 r = rng[blockIdx.x * nr + k];
+        for (int i = r.x + threadIdx.x; i < r.x + r.y; i += WG) acc += partial[i];
+    }
     red[threadIdx.x] = acc;
     __syncthreads();
     for (int w = WG / 2; w > 0; w >>= 1) {
@@ -526,8 +530,8 @@ struct RecFace {
     // first-half-sweep value of the neighbour cell, written into this block's halo
     template <int RD>
     __device__ static __forceinline__ void compute(int dir, int tq, int e, const int *tb_own, T *tile,
-                                                   const int *__restrict__ coord, const LevelArgs<T> &a, T rd,
-                                                   T rb, const T *rg) {
+                                                   const int *__restrict__ coord, const LevelArgs<T> &a,
+                                                   const BP<T> &bp, T rd, T rb, const T *rg) {
         int cB[3];
         cell(dir, tq, cB);
         const int f = 2 * DIM + (dir > 0 ? 1 : 0);
@@ -563,13 +567,13 @@ struct RecFace {
             if (dd == RD) Lx += wm * nbm[dd];
             else Lx += nbm[dd];
         }
-        Lx += a.cc * xo;
+        Lx += bp.cc * xo;
 #pragma unroll
         for (int dd = 0; dd < D; dd++) {
             if (dd == RD) Lx += wp * nbp[dd];
             else Lx += nbp[dd];
         }
-        tile[ph] = fma_t<T>(a.nwc, fma_t<T>(a.s, rb, Lx), xo);
+        tile[ph] = fma_t<T>(bp.nwc, fma_t<T>(bp.s, rb, Lx), xo);
     }
 };
 
@@ -601,12 +605,13 @@ __global__ __launch_bounds__(NT) void k_gs_sweep(T *__restrict__ uo, const T *__
     constexpr int NHALO = BPW * ND * F;
     __shared__ T tile[BPW * TL::SZ];
     __shared__ int tab[BPW * HT::N];
+    __shared__ BP<T> prm[BPW];
     const int lb0 = xcd_remap(blockIdx.x, gridDim.x) * BPW;
     const int tid = threadIdx.x;
     const int nloc = min(BPW, a.nblk - lb0);
     const int blk0 = a.blk0 + lb0;
 
-    // round trip 1: interior cells, own right-hand side, halo tables
+    // round trip 1: interior cells, own right-hand side, halo tables, parameters
     T ux[CPT][2], bo[CPT], be[CPT];
 #pragma unroll
     for (int k = 0; k < CPT; k++) {
@@ -621,6 +626,7 @@ __global__ __launch_bounds__(NT) void k_gs_sweep(T *__restrict__ uo, const T *__
         }
     }
     for (int i = tid; i < nloc * HT::N; i += NT) tab[i] = htab[(size_t)blk0 * HT::N + i];
+    if (tid < nloc) prm[tid] = bparams(a, blk0 + tid);
 #pragma unroll
     for (int k = 0; k < CPT; k++) {
         int i = tid + k * NT, lb = i / H, r = i - lb * H;
@@ -674,8 +680,9 @@ __global__ __launch_bounds__(NT) void k_gs_sweep(T *__restrict__ uo, const T *__
             int p = lb * TL::SZ + TL::at(x, y, z);
             T wm, wp;
             rweights<D, M, T, RD>(a, coord, blk0 + lb, RD == 0 ? x : y, wm, wp);
-            T Lu = tstencil<D, M, T, RD>(tile, p, ux[k][0], a.cc, wm, wp);
-            ux[k][0] = fma_t<T>(a.nwc, fma_t<T>(a.s, bo[k], Lu), ux[k][0]);
+            const BP<T> bp = prm[lb];
+            T Lu = tstencil<D, M, T, RD>(tile, p, ux[k][0], bp.cc, wm, wp);
+            ux[k][0] = fma_t<T>(bp.nwc, fma_t<T>(bp.s, bo[k], Lu), ux[k][0]);
             tile[p] = ux[k][0];
         }
     }
@@ -690,9 +697,14 @@ __global__ __launch_bounds__(NT) void k_gs_sweep(T *__restrict__ uo, const T *__
         T *tl = tile + lb * TL::SZ;
         int e = tb[f];
         int dim = f >> 1, dir = (f & 1) ? 1 : -1;
-        if (dim == 0) RecFace<D, M, T, 0>::template compute<RD>(dir, tq, e, tb, tl, coord, a, rd[k], rb[k], rg[k]);
-        else if (dim == 1) RecFace<D, M, T, 1>::template compute<RD>(dir, tq, e, tb, tl, coord, a, rd[k], rb[k], rg[k]);
-        else RecFace<D, M, T, (D == 3 ? 2 : 1)>::template compute<RD>(dir, tq, e, tb, tl, coord, a, rd[k], rb[k], rg[k]);
+        const BP<T> bp = prm[lb];
+        if (dim == 0)
+            RecFace<D, M, T, 0>::template compute<RD>(dir, tq, e, tb, tl, coord, a, bp, rd[k], rb[k], rg[k]);
+        else if (dim == 1)
+            RecFace<D, M, T, 1>::template compute<RD>(dir, tq, e, tb, tl, coord, a, bp, rd[k], rb[k], rg[k]);
+        else
+            RecFace<D, M, T, (D == 3 ? 2 : 1)>::template compute<RD>(dir, tq, e, tb, tl, coord, a, bp, rd[k], rb[k],
+                                                                     rg[k]);
     }
     __syncthreads();
 
@@ -706,10 +718,11 @@ __global__ __launch_bounds__(NT) void k_gs_sweep(T *__restrict__ uo, const T *__
             int p = lb * TL::SZ + TL::at(x, y, z);
             T wm, wp;
             rweights<D, M, T, RD>(a, coord, blk0 + lb, RD == 0 ? x : y, wm, wp);
-            T Lu = tstencil<D, M, T, RD>(tile, p, ux[k][1], a.cc, wm, wp);
+            const BP<T> bp = prm[lb];
+            T Lu = tstencil<D, M, T, RD>(tile, p, ux[k][1], bp.cc, wm, wp);
             size_t o = (size_t)(blk0 + lb) * V + r;
             uo[o] = ux[k][0];
-            uo[o + H] = fma_t<T>(a.nwc, fma_t<T>(a.s, be[k], Lu), ux[k][1]);
+            uo[o + H] = fma_t<T>(bp.nwc, fma_t<T>(bp.s, be[k], Lu), ux[k][1]);
         }
     }
 }
@@ -753,10 +766,10 @@ enum { RES_WRITE = 0, RES_NORM = 1, RES_DEFECT = 2 };
 
 // Residual of a level from LDS tiles (src/multigrid.jl:309-323).
 //   RES_WRITE:  r = -(b + L u / s) on the blocks [a.blk0, a.blk0 + a.nblk)
-//   RES_NORM:   leaf blocks `list[0..n)`: one partial sum of r^2 (times wgt) per work group
-//   RES_DEFECT: out (float) = b + L u / s = -r on the level's blocks, and the
-//               partial sums of r^2 over its leaf blocks (isleaf[blk])
-// Work group partials are summed in a fixed order (LDS tree).
+//   RES_NORM:   leaf blocks list[0..n): partial[k] = wgt * sum of r^2 over block list[k]
+//   RES_DEFECT: out (float) = b + L u / s = -r on the level's blocks, and
+//               partial[k] = wgt * sum of r^2 over block k if it is a leaf (else 0)
+// Each block's partial is an LDS tree sum in a fixed order.
 template <int D, int M, typename T, typename TO, int BPW, int NT, int MODE, int RD>
 __global__ __launch_bounds__(NT) void k_resid_tile(TO *__restrict__ out, const T *__restrict__ u,
                                                    const T *__restrict__ b, const T *__restrict__ ib,
@@ -769,13 +782,18 @@ __global__ __launch_bounds__(NT) void k_resid_tile(TO *__restrict__ out, const T
     __shared__ T tile[BPW * TL::SZ];
     __shared__ int tab[BPW * 2 * D];
     __shared__ int blks[BPW];
+    __shared__ BP<T> prm[BPW];
     __shared__ double red[NT];
     const int first = xcd_remap(blockIdx.x, gridDim.x) * BPW;
     const int nloc = min(BPW, n - first);
     const int *lst = (MODE == RES_NORM) ? list : nullptr;
     const int base = (MODE == RES_NORM) ? first : a.blk0 + first;
     load_tile<D, M, T, BPW, NT>(tile, tab, blks, u, ib, htab, lst, base, nloc, a.sg);
-    double acc = 0.0;
+    if ((int)threadIdx.x < nloc) prm[threadIdx.x] = bparams(a, blks[threadIdx.x]);
+    __syncthreads();
+    double acc[BPW];
+#pragma unroll
+    for (int lb = 0; lb < BPW; lb++) acc[lb] = 0.0;
     for (int i = threadIdx.x; i < nloc * V; i += NT) {
         int lb = i / V, q = i - lb * V;
         int blk = blks[lb];
@@ -784,21 +802,29 @@ __global__ __launch_bounds__(NT) void k_resid_tile(TO *__restrict__ out, const T
         int p = lb * TL::SZ + TL::at(x, y, z);
         T wm, wp;
         rweights<D, M, T, RD>(a, coord, blk, RD == 0 ? x : y, wm, wp);
-        T Lu = tstencil<D, M, T, RD>(tile, p, tile[p], a.cc, wm, wp);
+        const BP<T> bp = prm[lb];
+        T Lu = tstencil<D, M, T, RD>(tile, p, tile[p], bp.cc, wm, wp);
         size_t o = (size_t)blk * V + q;
-        T rr = -(b[o] + Lu / a.s);
+        T rr = -(b[o] + Lu / bp.s);
         if (MODE == RES_WRITE) out[o] = (TO)rr;
-        if (MODE == RES_DEFECT) out[o] = (TO)(b[o] + Lu / a.s);
-        if (MODE == RES_NORM || (MODE == RES_DEFECT && isleaf[blk])) acc += (double)rr * (double)rr;
+        if (MODE == RES_DEFECT) out[o] = (TO)(b[o] + Lu / bp.s);
+        if (MODE == RES_NORM || (MODE == RES_DEFECT && isleaf[blk])) {
+#pragma unroll
+            for (int k = 0; k < BPW; k++)
+                if (k == lb) acc[k] += (double)rr * (double)rr;
+        }
     }
     if (MODE != RES_WRITE) {
-        red[threadIdx.x] = acc;
-        __syncthreads();
-        for (int w = NT / 2; w > 0; w >>= 1) {
-            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        for (int lb = 0; lb < nloc; lb++) {
+            red[threadIdx.x] = acc[lb];
+            __syncthreads();
+            for (int w = NT / 2; w > 0; w >>= 1) {
+                if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+                __syncthreads();
+            }
+            if (threadIdx.x == 0) partial[first + lb] = red[0] * wgt;
             __syncthreads();
         }
-        if (threadIdx.x == 0) partial[blockIdx.x] = red[0] * wgt;
     }
 }
 
@@ -808,7 +834,8 @@ __global__ __launch_bounds__(NT) void k_resid_tile(TO *__restrict__ out, const T
 template <int D, int M, typename T>
 __global__ __launch_bounds__(WG) void k_grid_gather(double *__restrict__ g, const T *__restrict__ x,
                                                     const int *__restrict__ coord, int blk0, int nblk,
-                                                    int N0, int N1, double c) {
+                                                    int N0, int N1, int nper, size_t total,
+                                                    const double *__restrict__ cs) {
     using LY = Lay<D, M>;
     int gt = blockIdx.x * blockDim.x + threadIdx.x;
     int lb = gt / LY::V;
@@ -817,16 +844,17 @@ __global__ __launch_bounds__(WG) void k_grid_gather(double *__restrict__ g, cons
     int q = gt - lb * LY::V;
     int c0, c1, c2;
     LY::decode(q, c0, c1, c2);
-    size_t gi = (size_t)(M * (coord[blk * D] - 1) + c0) +
+    int grid = lb / nper;
+    size_t gi = (size_t)grid * total + (size_t)(M * (coord[blk * D] - 1) + c0) +
                 (size_t)N0 * ((size_t)(M * (coord[blk * D + 1] - 1) + c1) +
                               (D == 3 ? (size_t)N1 * (size_t)(M * (coord[blk * D + 2] - 1) + c2) : 0));
-    g[gi] = c * (double)x[(size_t)blk * LY::V + q];
+    g[gi] = cs[grid] * (double)x[(size_t)blk * LY::V + q];
 }
 
 template <int D, int M, typename T>
 __global__ __launch_bounds__(WG) void k_grid_scatter(T *__restrict__ x, const double *__restrict__ g,
                                                      const int *__restrict__ coord, int blk0, int nblk,
-                                                     int N0, int N1) {
+                                                     int N0, int N1, int nper, size_t total) {
     using LY = Lay<D, M>;
     int gt = blockIdx.x * blockDim.x + threadIdx.x;
     int lb = gt / LY::V;
@@ -835,25 +863,11 @@ __global__ __launch_bounds__(WG) void k_grid_scatter(T *__restrict__ x, const do
     int q = gt - lb * LY::V;
     int c0, c1, c2;
     LY::decode(q, c0, c1, c2);
-    size_t gi = (size_t)(M * (coord[blk * D] - 1) + c0) +
+    int grid = lb / nper;
+    size_t gi = (size_t)grid * total + (size_t)(M * (coord[blk * D] - 1) + c0) +
                 (size_t)N0 * ((size_t)(M * (coord[blk * D + 1] - 1) + c1) +
                               (D == 3 ? (size_t)N1 * (size_t)(M * (coord[blk * D + 2] - 1) + c2) : 0));
     x[(size_t)blk * LY::V + q] = (T)g[gi];
-}
-
-// out = Mx applied along one axis (extent n, stride `stride`) of a grid of `total` values
-__global__ __launch_bounds__(WG) void k_axis(const double *__restrict__ Mx, int n, size_t stride, size_t total,
-                                             const double *__restrict__ in, double *__restrict__ out) {
-    size_t gi = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gi >= total) return;
-    size_t inner = gi % stride;
-    size_t i = (gi / stride) % n;
-    size_t outer = gi / (stride * n);
-    const double *src = in + outer * stride * n + inner;
-    const double *row = Mx + i * n;
-    double acc = 0.0;
-    for (int k = 0; k < n; k++) acc += row[k] * src[(size_t)k * stride];
-    out[gi] = acc;
 }
 
 // Y = A X along one axis of a column-major grid: A is n x n (row-major), the
@@ -924,13 +938,18 @@ __global__ __launch_bounds__(256) void k_axis_gemm(const double *__restrict__ A,
     }
 }
 
+// divide by the eigenvalues of the Kronecker-sum operator; the grid holds
+// `ngrid` stacked level-1 grids of `total` points, grid g with shift[g]
+// (Helmholtz: -s k^2; Laplacian: 0)
 __global__ __launch_bounds__(WG) void k_eig_scale(double *__restrict__ g, const double *__restrict__ l0,
                                                   const double *__restrict__ l1, const double *__restrict__ l2,
-                                                  int N0, int N1, int D, double shift, size_t total) {
+                                                  int N0, int N1, int D, const double *__restrict__ shift,
+                                                  size_t total, int ngrid) {
     size_t gi = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gi >= total) return;
-    size_t r = gi;
-    double ev = shift;
+    if (gi >= total * ngrid) return;
+    size_t gg = gi / total;
+    size_t r = gi - gg * total;
+    double ev = shift[gg];
     ev += l0[r % N0]; r /= N0;
     ev += l1[r % N1]; r /= N1;
     if (D == 3) ev += l2[r];
@@ -938,28 +957,6 @@ __global__ __launch_bounds__(WG) void k_eig_scale(double *__restrict__ g, const 
 }
 
 // ---- mixed precision helpers ----
-
-// FP64 residual of level l written as the FP32 right-hand side of the
-// correction equation (b' = -r = b + L u / s), plus leaf squared sums.
-template <int D, int M>
-__global__ __launch_bounds__(WG) void k_defect(float *__restrict__ be, const double *__restrict__ u,
-                                               const double *__restrict__ b, const double *__restrict__ ib,
-                                               const int *__restrict__ nbr, const int *__restrict__ coord,
-                                               LevelArgs<double> a) {
-    using LY = Lay<D, M>;
-    int gt = blockIdx.x * blockDim.x + threadIdx.x;
-    int lb = gt / LY::V;
-    if (lb >= a.nblk) return;
-    int blk = a.blk0 + lb;
-    int q = gt - lb * LY::V;
-    int c0, c1, c2;
-    LY::decode(q, c0, c1, c2);
-    size_t o = (size_t)blk * LY::V + q;
-    double wm, wp;
-    radial_weights<D, M, double>(a, coord, blk, c0, c1, wm, wp);
-    double Lu = stencil<D, M, double>(u, ib, nbr, blk, c0, c1, c2, u[o], a, wm, wp);
-    be[o] = (float)(b[o] + Lu / a.s);
-}
 
 // u += e over all cells and interpolation-ghost slots (FP64 += FP32)
 __global__ __launch_bounds__(WG) void k_add_correction(double *__restrict__ u, const float *__restrict__ e, size_t n) {

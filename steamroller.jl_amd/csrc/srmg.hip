@@ -27,6 +27,8 @@ using namespace srmg;
 constexpr unsigned RESNORM_WGS = 4096;
 // first-stage work groups of the fixed-order sum of partials
 constexpr int SUM_CHUNKS = 256;
+// per-tree totals start here in d_sum
+constexpr int TSUM = SUM_CHUNKS + 4;
 
 // ---------------------------------------------------------------------------
 // errors
@@ -72,6 +74,20 @@ struct DevBuf {
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) {
+        o.p = nullptr;
+        o.bytes = 0;
+    }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p;
+            bytes = o.bytes;
+            o.p = nullptr;
+            o.bytes = 0;
+        }
+        return *this;
+    }
     ~DevBuf() { release(); }
     void release() {
         if (p) (void)hipFree(p);
@@ -105,40 +121,63 @@ struct LevelInfo {
     int blk0 = 0, nblk = 0;     // device blocks
     int slot0 = 0, nslot = 0;   // coarse-fine ghost slots owned by this level's blocks
     int leaf0 = 0, nleaf = 0;   // entries in the leaf list
-    int ext[3] = {1, 1, 1};     // level domain in blocks
+    int ext[3] = {1, 1, 1};     // largest level domain in blocks over the trees
     int wofs[2] = {0, 0};       // radial weight table offsets (dims 0 and 1)
 };
 
-struct CoarseCache {
+// One tree of a forest: an independent problem whose blocks share the
+// level-by-level launches with the other trees.
+struct TreeInfo {
+    int64_t host_off = 0;       // its host block ids are host_off + (1 .. nblocks)
+    int64_t nhost = 0;
+    int ext1[3] = {1, 1, 1};    // level-1 domain in blocks
+    int group = 0, gslot = 0;   // coarse-solve group and position in it
+    std::vector<int> blk0, nblk, leafk0, nleaf;  // per level: block range, position in the level's leaf list
+};
+
+// Trees whose level-1 grids have the same shape share one eigendecomposition
+// and one batched exact coarse solve.
+struct CoarseGroup {
+    int N[3] = {1, 1, 1};
+    int ntree = 0;
+    int l1_blk0 = 0, nper = 0;  // level-1 device blocks: contiguous, nper per tree
     bool valid = false;
     int geom = -1;
     int8_t bc[27] = {0};
-    int N[3] = {1, 1, 1};
     DevBuf P[3], Q[3], lam[3];
+    DevBuf cs, shift;           // per tree: -s (gather factor) and the Helmholtz shift -s k2
 };
 
 struct srmg_ctx {
     int D = 2, M = 16, V = 0, F = 0;
     int nlevels = 0;
-    int64_t nhost = 0;      // host block ids 1..nhost
+    int64_t nhost = 0;      // host block ids 1..nhost (all trees)
     int nb = 0;             // device blocks
     int nslot = 0;
     std::vector<LevelInfo> lv;
+    std::vector<TreeInfo> trees;
+    std::vector<CoarseGroup> groups;
     // host copies of the tables
-    std::vector<int> h_nbr, h_coord, h_parent, h_sub, h_level, h_leaves;
+    std::vector<int> h_nbr, h_coord, h_parent, h_sub, h_level, h_leaves, h_tree;
     std::vector<int> host2dev, dev2host;
     std::vector<int4> h_slots;
     // device tables
-    DevBuf d_nbr, d_coord, d_parent, d_sub, d_slots, d_leaves, d_halo, d_isleaf;
+    DevBuf d_nbr, d_coord, d_parent, d_sub, d_slots, d_leaves, d_halo, d_isleaf, d_pix;
     DevBuf d_w64, d_w32;    // radial weights (double / float), per level and dim
     DevBuf d_partial, d_sum;
-    DevBuf d_grid, d_grid2; // level-1 uniform grids (coarse solve)
-    CoarseCache coarse;
+    DevBuf d_rng_leaf, d_rng_blk;  // per (tree, level): ranges of the leaf / block partials
+    DevBuf d_grid, d_grid2; // stacked level-1 uniform grids (coarse solve)
     std::vector<std::unique_ptr<FieldBuf>> fields;
     std::unique_ptr<FieldBuf> zero[2];  // all-zero scratch fields (F64, F32), made on demand
     hipStream_t stream = nullptr;
     int maxpartials = 0;
     std::vector<int> sweep_cfg;  // per (dtype, level): tile shape of the fused sweep (see Ops::sweep)
+    // the current problem of every tree (prepare()): scale s and operator
+    std::vector<double> ps;
+    std::vector<srmg_op> pops;
+    double pomega = 1.0;
+    std::vector<double> pp_host;   // 3 x ntrees x nlevels: (s_l, cc, nwc)
+    DevBuf d_pp64, d_pp32;
 };
 
 static void alloc_field(srmg_ctx *c, FieldBuf &f);
@@ -179,53 +218,118 @@ static uint64_t morton(const int64_t *c, int D) {
     return code;
 }
 
-static void set_tree(srmg_ctx *c, int nlevels, int64_t nhost, const int64_t *pairs, const int64_t *pairs_off,
-                     const int64_t *nbr, const int64_t *nbr_off, const int64_t *bnd, const int64_t *bnd_off,
-                     const int64_t *ref, const int64_t *ref_off, const int64_t *chd, const int64_t *chd_off) {
-    const int D = c->D, M = c->M;
-    if (nlevels < 1) fail("nlevels must be >= 1");
+// One tree's records as the reference lays them out (src/connectivity.jl:6-42),
+// concatenated over its levels with nlevels+1 offsets.
+struct TreeRecords {
+    int nlevels;
+    int64_t nblocks;
+    const int64_t *pairs, *pairs_off, *nbr, *nbr_off, *bnd, *bnd_off, *ref, *ref_off, *chd, *chd_off;
+};
+
+static void set_forest(srmg_ctx *c, const std::vector<TreeRecords> &tr) {
+    const int D = c->D, M = c->M, ND = 2 * D;
+    const int nt = (int)tr.size();
+    if (nt < 1) fail("a forest needs at least one tree");
+    int nlevels = 0;
+    int64_t nhost = 0;
+    for (auto &t : tr) {
+        if (t.nlevels < 1) fail("nlevels must be >= 1");
+        nlevels = std::max(nlevels, t.nlevels);
+        nhost += t.nblocks;
+    }
     c->nlevels = nlevels;
     c->nhost = nhost;
+    c->trees.assign(nt, TreeInfo());
     c->lv.assign(nlevels, LevelInfo());
-    c->host2dev.assign(nhost + 1, -1);
-    int nb = (int)pairs_off[nlevels];
+    // level-1 extents and coarse groups (same level-1 shape)
+    c->groups.clear();
+    int64_t off = 0;
+    for (int t = 0; t < nt; t++) {
+        TreeInfo &T = c->trees[t];
+        T.host_off = off;
+        T.nhost = tr[t].nblocks;
+        off += tr[t].nblocks;
+        for (int64_t k = tr[t].pairs_off[0]; k < tr[t].pairs_off[1]; k++)
+            for (int d = 0; d < D; d++) T.ext1[d] = std::max(T.ext1[d], (int)tr[t].pairs[k * (D + 1) + d]);
+        int64_t n1 = 1;
+        for (int d = 0; d < D; d++) n1 *= T.ext1[d];
+        if (tr[t].pairs_off[1] - tr[t].pairs_off[0] != n1) fail("level 1 must be complete (populate!)");
+        int g = -1;
+        for (int k = 0; k < (int)c->groups.size(); k++)
+            if (std::equal(T.ext1, T.ext1 + D, c->groups[k].N)) g = k;
+        if (g < 0) {
+            c->groups.emplace_back();
+            g = (int)c->groups.size() - 1;
+            for (int d = 0; d < 3; d++) c->groups[g].N[d] = T.ext1[d];  // in blocks for now
+        }
+        T.group = g;
+        T.gslot = c->groups[g].ntree++;
+    }
+    // device order: by level, then coarse group, then tree, then Morton rank
+    std::vector<int> torder(nt);
+    for (int t = 0; t < nt; t++) torder[t] = t;
+    std::stable_sort(torder.begin(), torder.end(),
+                     [&](int a, int b) { return c->trees[a].group < c->trees[b].group; });
+    int nb = 0;
+    for (auto &t : tr) nb += (int)t.pairs_off[t.nlevels];
     c->nb = nb;
+    c->host2dev.assign(nhost + 1, -1);
     c->dev2host.assign(nb, 0);
     c->h_coord.assign((size_t)nb * D, 0);
     c->h_level.assign(nb, 0);
-    // level-1 extent (level 1 must be complete)
-    int ext1[3] = {1, 1, 1};
-    for (int64_t k = pairs_off[0]; k < pairs_off[1]; k++)
-        for (int d = 0; d < D; d++) ext1[d] = std::max(ext1[d], (int)pairs[k * (D + 1) + d]);
-    int64_t n1 = 1;
-    for (int d = 0; d < D; d++) n1 *= ext1[d];
-    if (pairs_off[1] - pairs_off[0] != n1) fail("level 1 must be complete (populate!)");
+    c->h_tree.assign(nb, 0);
+    for (auto &T : c->trees) {
+        T.blk0.assign(nlevels, 0);
+        T.nblk.assign(nlevels, 0);
+        T.leafk0.assign(nlevels, 0);
+        T.nleaf.assign(nlevels, 0);
+    }
     int dev = 0;
     for (int l = 0; l < nlevels; l++) {
         LevelInfo &L = c->lv[l];
         L.blk0 = dev;
-        L.nblk = (int)(pairs_off[l + 1] - pairs_off[l]);
-        for (int d = 0; d < D; d++) L.ext[d] = ext1[d] << l;
-        std::vector<std::pair<uint64_t, int64_t>> order;
-        for (int64_t k = pairs_off[l]; k < pairs_off[l + 1]; k++)
-            order.push_back({morton(pairs + k * (D + 1), D), k});
-        std::sort(order.begin(), order.end());
-        for (auto &o : order) {
-            const int64_t *p = pairs + o.second * (D + 1);
-            int64_t hid = p[D];
-            if (hid < 1 || hid > nhost) fail("block id out of range");
-            if (c->host2dev[hid] >= 0) fail("block id appears twice");
-            c->host2dev[hid] = dev;
-            c->dev2host[dev] = (int)hid;
-            for (int d = 0; d < D; d++) c->h_coord[(size_t)dev * D + d] = (int)p[d];
-            c->h_level[dev] = l;
-            dev++;
+        for (int t : torder) {
+            TreeInfo &T = c->trees[t];
+            for (int d = 0; d < D; d++) L.ext[d] = std::max(L.ext[d], T.ext1[d] << l);
+            T.blk0[l] = dev;
+            if (l >= tr[t].nlevels) continue;
+            std::vector<std::pair<uint64_t, int64_t>> order;
+            for (int64_t k = tr[t].pairs_off[l]; k < tr[t].pairs_off[l + 1]; k++)
+                order.push_back({morton(tr[t].pairs + k * (D + 1), D), k});
+            std::sort(order.begin(), order.end());
+            for (auto &o : order) {
+                const int64_t *p = tr[t].pairs + o.second * (D + 1);
+                int64_t lid = p[D];
+                if (lid < 1 || lid > T.nhost) fail("block id out of range");
+                int64_t hid = T.host_off + lid;
+                if (c->host2dev[hid] >= 0) fail("block id appears twice");
+                c->host2dev[hid] = dev;
+                c->dev2host[dev] = (int)hid;
+                for (int d = 0; d < D; d++) {
+                    int cd = (int)p[d];
+                    if (cd < 1 || cd > (T.ext1[d] << l)) fail("block coordinate outside the level domain");
+                    c->h_coord[(size_t)dev * D + d] = cd;
+                }
+                c->h_level[dev] = l;
+                c->h_tree[dev] = t;
+                dev++;
+            }
+            T.nblk[l] = dev - T.blk0[l];
         }
+        L.nblk = dev - L.blk0;
     }
-    auto d_of = [&](int64_t hid) {
-        if (hid < 1 || hid > nhost || c->host2dev[hid] < 0) fail("record names an unknown block");
-        return c->host2dev[hid];
-    };
+    for (auto &G : c->groups) {
+        int n1 = 1;
+        for (int d = 0; d < D; d++) n1 *= G.N[d];
+        G.nper = n1;
+    }
+    for (int t : torder) {
+        CoarseGroup &G = c->groups[c->trees[t].group];
+        if (c->trees[t].gslot == 0) G.l1_blk0 = c->trees[t].blk0[0];
+    }
+    for (auto &G : c->groups)
+        for (int d = 0; d < D; d++) G.N[d] *= M;  // now in cells
+
     auto face_code = [&](const int64_t *f) {
         int code = -1;
         for (int d = 0; d < D; d++)
@@ -236,41 +340,57 @@ static void set_tree(srmg_ctx *c, int nlevels, int64_t nhost, const int64_t *pai
         if (code < 0) fail("zero face");
         return code;
     };
-    const int ND = 2 * D;
     c->h_nbr.assign((size_t)nb * ND, INT32_MIN);
-    // Neighbor(from, to, f): `to`'s ghost on face -f comes from `from`
-    for (int l = 0; l < nlevels; l++)
-        for (int64_t k = nbr_off[l]; k < nbr_off[l + 1]; k++) {
-            const int64_t *e = nbr + k * (2 + D);
-            int from = d_of(e[0]), to = d_of(e[1]);
-            int f = face_code(e + 2);
-            c->h_nbr[(size_t)to * ND + (f ^ 1)] = from;
+    c->h_parent.assign(nb, -1);
+    c->h_sub.assign(nb, 0);
+    std::vector<std::vector<std::pair<int64_t, int4>>> slots_by_level(nlevels);
+    for (int t = 0; t < nt; t++) {
+        const TreeRecords &R = tr[t];
+        const int64_t hoff = c->trees[t].host_off;
+        auto d_of = [&](int64_t lid) {
+            if (lid < 1 || lid > c->trees[t].nhost || c->host2dev[hoff + lid] < 0)
+                fail("record names an unknown block");
+            return c->host2dev[hoff + lid];
+        };
+        for (int l = 0; l < R.nlevels; l++) {
+            // Neighbor(from, to, f): `to`'s ghost on face -f comes from `from`
+            for (int64_t k = R.nbr_off[l]; k < R.nbr_off[l + 1]; k++) {
+                const int64_t *e = R.nbr + k * (2 + D);
+                int from = d_of(e[0]), to = d_of(e[1]);
+                c->h_nbr[(size_t)to * ND + (face_code(e + 2) ^ 1)] = from;
+            }
+            for (int64_t k = R.bnd_off[l]; k < R.bnd_off[l + 1]; k++) {
+                const int64_t *e = R.bnd + k * (2 + 2 * D);
+                c->h_nbr[(size_t)d_of(e[0]) * ND + face_code(e + 1)] = NB_BOUNDARY;
+            }
+            for (int64_t k = R.ref_off[l]; k < R.ref_off[l + 1]; k++) {
+                const int64_t *e = R.ref + k * (2 + 2 * D);
+                int fine = d_of(e[0]), coarse = d_of(e[1]);
+                int f = face_code(e + 2);
+                int sb = 0;
+                for (int d = 0; d < D; d++) {
+                    int64_t sv = e[2 + D + d];
+                    if (sv != 1 && sv != 2) fail("bad subblock");
+                    sb |= (int)(sv - 1) << d;
+                }
+                if (c->h_level[fine] != l || c->h_level[coarse] != l - 1) fail("refinement boundary level mismatch");
+                slots_by_level[l].push_back({(int64_t)fine * ND + f, make_int4(fine, f, coarse, sb)});
+            }
+            for (int64_t k = R.chd_off[l]; k < R.chd_off[l + 1]; k++) {
+                const int64_t *e = R.chd + k * (2 + D);
+                int fine = d_of(e[0]), coarse = d_of(e[1]);
+                int sb = 0;
+                for (int d = 0; d < D; d++) sb |= (int)(e[2 + d] - 1) << d;
+                c->h_parent[fine] = coarse;
+                c->h_sub[fine] = sb;
+            }
         }
-    for (int l = 0; l < nlevels; l++)
-        for (int64_t k = bnd_off[l]; k < bnd_off[l + 1]; k++) {
-            const int64_t *e = bnd + k * (2 + 2 * D);
-            int blk = d_of(e[0]);
-            int f = face_code(e + 1);
-            c->h_nbr[(size_t)blk * ND + f] = NB_BOUNDARY;
-        }
+    }
     // coarse-fine ghost slots, grouped by the level of the fine block
     c->h_slots.clear();
     for (int l = 0; l < nlevels; l++) {
         c->lv[l].slot0 = (int)c->h_slots.size();
-        std::vector<std::pair<int64_t, int4>> tmp;
-        for (int64_t k = ref_off[l]; k < ref_off[l + 1]; k++) {
-            const int64_t *e = ref + k * (2 + 2 * D);
-            int fine = d_of(e[0]), coarse = d_of(e[1]);
-            int f = face_code(e + 2);
-            int sb = 0;
-            for (int d = 0; d < D; d++) {
-                int64_t s = e[2 + D + d];
-                if (s != 1 && s != 2) fail("bad subblock");
-                sb |= (int)(s - 1) << d;
-            }
-            if (c->h_level[fine] != l || c->h_level[coarse] != l - 1) fail("refinement boundary level mismatch");
-            tmp.push_back({(int64_t)fine * ND + f, make_int4(fine, f, coarse, sb)});
-        }
+        auto &tmp = slots_by_level[l];
         std::sort(tmp.begin(), tmp.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
         for (auto &t : tmp) {
             int k = (int)c->h_slots.size();
@@ -284,32 +404,43 @@ static void set_tree(srmg_ctx *c, int nlevels, int64_t nhost, const int64_t *pai
         if (c->h_nbr[i] == INT32_MIN)
             fail("block " + std::to_string(c->dev2host[i / ND]) + " face " + std::to_string(i % ND) +
                  " has no neighbour, boundary or refinement record");
-    // parents
-    c->h_parent.assign(nb, -1);
-    c->h_sub.assign(nb, 0);
-    for (int l = 0; l < nlevels; l++)
-        for (int64_t k = chd_off[l]; k < chd_off[l + 1]; k++) {
-            const int64_t *e = chd + k * (2 + D);
-            int fine = d_of(e[0]), coarse = d_of(e[1]);
-            int sb = 0;
-            for (int d = 0; d < D; d++) sb |= (int)(e[2 + d] - 1) << d;
-            c->h_parent[fine] = coarse;
-            c->h_sub[fine] = sb;
-        }
     for (int l = 1; l < nlevels; l++)
         for (int b = c->lv[l].blk0; b < c->lv[l].blk0 + c->lv[l].nblk; b++)
             if (c->h_parent[b] < 0) fail("block without Child record");
-    // leaves: blocks without children
+    // leaves: blocks without children; per level ordered like the blocks (tree by tree)
     std::vector<char> has_child(nb, 0);
     for (int b = 0; b < nb; b++)
         if (c->h_parent[b] >= 0) has_child[c->h_parent[b]] = 1;
     c->h_leaves.clear();
     for (int l = 0; l < nlevels; l++) {
         c->lv[l].leaf0 = (int)c->h_leaves.size();
-        for (int b = c->lv[l].blk0; b < c->lv[l].blk0 + c->lv[l].nblk; b++)
-            if (!has_child[b]) c->h_leaves.push_back(b);
+        for (int t : torder) {
+            TreeInfo &T = c->trees[t];
+            T.leafk0[l] = (int)c->h_leaves.size() - c->lv[l].leaf0;
+            for (int b = T.blk0[l]; b < T.blk0[l] + T.nblk[l]; b++)
+                if (!has_child[b]) c->h_leaves.push_back(b);
+            T.nleaf[l] = (int)c->h_leaves.size() - c->lv[l].leaf0 - T.leafk0[l];
+        }
         c->lv[l].nleaf = (int)c->h_leaves.size() - c->lv[l].leaf0;
     }
+    // every index a kernel will follow stays inside its table
+    for (size_t i = 0; i < c->h_nbr.size(); i++) {
+        int v = c->h_nbr[i];
+        if (!(v == NB_BOUNDARY || (v >= 0 && v < nb) || (v <= -2 && nb_slot(v) < c->nslot)))
+            fail("internal: neighbour table entry out of range");
+        if (v >= 0 && (c->h_level[v] != c->h_level[i / ND] || c->h_tree[v] != c->h_tree[i / ND]))
+            fail("neighbour on another level or tree");
+    }
+    for (auto &sl : c->h_slots)
+        if (sl.x < 0 || sl.x >= nb || sl.z < 0 || sl.z >= nb || sl.y < 0 || sl.y >= ND)
+            fail("internal: ghost slot out of range");
+    for (int b = 0; b < nb; b++) {
+        int pb = c->h_parent[b];
+        if (c->h_level[b] > 0 &&
+            (pb < 0 || pb >= nb || c->h_level[pb] != c->h_level[b] - 1 || c->h_tree[pb] != c->h_tree[b]))
+            fail("internal: parent table entry out of range");
+    }
+    if ((int64_t)nb * c->V >= ((int64_t)1 << 31)) fail("field too large for 32-bit cell indices");
     // radial weight tables (2J-2)/(2J-1), 2J/(2J-1) for J = 0..n (entry 0 unused)
     std::vector<double> w64;
     for (int l = 0; l < nlevels; l++)
@@ -321,26 +452,7 @@ static void set_tree(srmg_ctx *c, int nlevels, int64_t nhost, const int64_t *pai
         }
     std::vector<float> w32(w64.size());
     for (size_t i = 0; i < w64.size(); i++) w32[i] = (float)w64[i];
-    // every index a kernel will follow stays inside its table
-    for (size_t i = 0; i < c->h_nbr.size(); i++) {
-        int v = c->h_nbr[i];
-        if (!(v == NB_BOUNDARY || (v >= 0 && v < nb) || (v <= -2 && nb_slot(v) < c->nslot)))
-            fail("internal: neighbour table entry out of range");
-        if (v >= 0 && c->h_level[v] != c->h_level[i / ND]) fail("neighbour on another level");
-    }
-    for (auto &sl : c->h_slots)
-        if (sl.x < 0 || sl.x >= nb || sl.z < 0 || sl.z >= nb || sl.y < 0 || sl.y >= ND)
-            fail("internal: ghost slot out of range");
-    for (int b = 0; b < nb; b++) {
-        int pb = c->h_parent[b];
-        if (c->h_level[b] > 0 && (pb < 0 || pb >= nb || c->h_level[pb] != c->h_level[b] - 1))
-            fail("internal: parent table entry out of range");
-        for (int d = 0; d < D; d++) {
-            int cd = c->h_coord[(size_t)b * D + d];
-            if (cd < 1 || cd > c->lv[c->h_level[b]].ext[d]) fail("block coordinate outside the level domain");
-        }
-    }
-    // halo tables of the fused sweep (Halo<D>)
+    // halo tables of the tiled kernels (Halo<D>)
     const int HN = ND + ND * 2 * (D - 1);
     std::vector<int> htab((size_t)nb * HN, 0);
     for (int b = 0; b < nb; b++) {
@@ -359,6 +471,22 @@ static void set_tree(srmg_ctx *c, int nlevels, int64_t nhost, const int64_t *pai
             }
         }
     }
+    std::vector<int> isleaf(nb), pix(nb);
+    for (int b = 0; b < nb; b++) {
+        isleaf[b] = has_child[b] ? 0 : 1;
+        pix[b] = c->h_tree[b] * nlevels + c->h_level[b];
+    }
+    // partial-sum ranges per (tree, level): residual norms per tree
+    std::vector<int2> rl((size_t)nt * nlevels), rb((size_t)nt * nlevels);
+    int pl = 0, pbk = 0;
+    for (int l = 0; l < nlevels; l++) {
+        for (int t = 0; t < nt; t++) {
+            rl[(size_t)t * nlevels + l] = make_int2(pl + c->trees[t].leafk0[l], c->trees[t].nleaf[l]);
+            rb[(size_t)t * nlevels + l] = make_int2(pbk + c->trees[t].blk0[l] - c->lv[l].blk0, c->trees[t].nblk[l]);
+        }
+        pl += c->lv[l].nleaf;
+        pbk += c->lv[l].nblk;
+    }
     auto up = [&](DevBuf &b, const void *src, size_t bytes) {
         b.alloc(bytes);
         if (bytes) HIPCHK(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, c->stream));
@@ -370,24 +498,32 @@ static void set_tree(srmg_ctx *c, int nlevels, int64_t nhost, const int64_t *pai
     up(c->d_slots, c->h_slots.data(), c->h_slots.size() * sizeof(int4));
     up(c->d_leaves, c->h_leaves.data(), c->h_leaves.size() * sizeof(int));
     up(c->d_halo, htab.data(), htab.size() * sizeof(int));
-    std::vector<int> isleaf(nb);
-    for (int b = 0; b < nb; b++) isleaf[b] = has_child[b] ? 0 : 1;
     up(c->d_isleaf, isleaf.data(), isleaf.size() * sizeof(int));
+    up(c->d_pix, pix.data(), pix.size() * sizeof(int));
     up(c->d_w64, w64.data(), w64.size() * sizeof(double));
     up(c->d_w32, w32.data(), w32.size() * sizeof(float));
-    // reduction scratch: one partial per work group of the largest leaf launch
-    int maxp = 1;
-    maxp += nlevels * (int)RESNORM_WGS + nb;
-    c->maxpartials = maxp;
-    c->d_partial.alloc((size_t)maxp * sizeof(double));
-    c->d_sum.alloc(sizeof(double) * (SUM_CHUNKS + 4));
-    c->coarse.valid = false;
+    up(c->d_rng_leaf, rl.data(), rl.size() * sizeof(int2));
+    up(c->d_rng_blk, rb.data(), rb.size() * sizeof(int2));
+    // per-block partials: at most one per block of every level
+    c->maxpartials = nb + 16;
+    c->d_partial.alloc((size_t)c->maxpartials * sizeof(double));
+    c->d_sum.alloc(sizeof(double) * (SUM_CHUNKS + 4 + nt));
+    size_t ng = 0;
+    for (auto &G : c->groups) ng = std::max(ng, (size_t)G.ntree * G.N[0] * G.N[1] * (D == 3 ? G.N[2] : 1));
+    c->d_grid.alloc(ng * sizeof(double));
+    c->d_grid2.alloc(ng * sizeof(double));
+    for (auto &G : c->groups) {
+        G.cs.alloc(G.ntree * sizeof(double));
+        G.shift.alloc(G.ntree * sizeof(double));
+    }
     c->sweep_cfg.assign(2 * nlevels, 0);
     c->zero[0].reset();
     c->zero[1].reset();
-    size_t ng = (size_t)c->lv[0].nblk * c->V;
-    c->d_grid.alloc(ng * sizeof(double));
-    c->d_grid2.alloc(ng * sizeof(double));
+    c->ps.clear();
+    c->pops.clear();
+    c->pp_host.clear();
+    c->d_pp64.alloc((size_t)3 * nt * nlevels * sizeof(double));
+    c->d_pp32.alloc((size_t)3 * nt * nlevels * sizeof(float));
     for (auto &f : c->fields)
         if (f) alloc_field(c, *f);
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -397,18 +533,78 @@ static void set_tree(srmg_ctx *c, int nlevels, int64_t nhost, const int64_t *pai
 // per-level kernel arguments
 // ---------------------------------------------------------------------------
 
+// s_l, stencil centre (diagelm) and -(omega / diagelm) of one problem at level l
+static void level_scalars(int D, int l, double s, const srmg_op *op, double omega, double &sl, double &cc,
+                          double &nwc) {
+    int64_t f = (int64_t)1 << l;
+    sl = s / (double)(f * f);
+    cc = -2.0 * D;
+    if (op->disc == SRMG_HELMHOLTZ) cc = cc - sl * op->k2;
+    nwc = -(omega / cc);
+}
+
+// Install the problems of the next calls: scale s[t] and operator ops[t] of
+// every tree (one value each when `same`). Geometry, boundary table and
+// coarsest-level treatment must agree across the trees. Uploads the
+// per-(tree, level) parameter tables and the coarse-solve factors when they
+// changed; must not run while a graph is being captured with new values.
+static void prepare(srmg_ctx *c, const double *s, const srmg_op *ops, double omega, bool same) {
+    const int nt = (int)c->trees.size();
+    std::vector<double> ps(nt);
+    std::vector<srmg_op> pops(nt);
+    for (int t = 0; t < nt; t++) {
+        ps[t] = same ? s[0] : s[t];
+        pops[t] = same ? ops[0] : ops[t];
+        if (pops[t].geometry != pops[0].geometry || std::memcmp(pops[t].bc, pops[0].bc, 27) != 0 ||
+            pops[t].coarse != pops[0].coarse)
+            fail("the trees of one call must share geometry, boundary conditions and coarse treatment");
+        if (pops[t].disc != SRMG_LAPLACIAN && pops[t].disc != SRMG_HELMHOLTZ) fail("unknown discretization");
+        if (pops[t].geometry < SRMG_CARTESIAN || pops[t].geometry > SRMG_CYL2) fail("unknown geometry");
+        if (c->D == 3 && pops[t].geometry != SRMG_CARTESIAN) fail("cylindrical geometry is two-dimensional");
+    }
+    std::vector<double> pp((size_t)3 * nt * c->nlevels);
+    for (int t = 0; t < nt; t++)
+        for (int l = 0; l < c->nlevels; l++) {
+            double *q = pp.data() + 3 * ((size_t)t * c->nlevels + l);
+            level_scalars(c->D, l, ps[t], &pops[t], omega, q[0], q[1], q[2]);
+        }
+    bool changed = pp != c->pp_host || ps != c->ps;
+    if (changed) {
+        hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+        HIPCHK(hipStreamIsCapturing(c->stream, &st));
+        if (st != hipStreamCaptureStatusNone) fail("problem parameters changed during graph capture");
+        HIPCHK(hipStreamSynchronize(c->stream));
+        std::vector<float> pf(pp.size());
+        for (size_t i = 0; i < pp.size(); i++) pf[i] = (float)pp[i];
+        HIPCHK(hipMemcpy(c->d_pp64.p, pp.data(), pp.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->d_pp32.p, pf.data(), pf.size() * sizeof(float), hipMemcpyHostToDevice));
+        for (auto &G : c->groups) {
+            std::vector<double> cs(G.ntree), sh(G.ntree);
+            for (int t = 0; t < nt; t++) {
+                if (&c->groups[c->trees[t].group] != &G) continue;
+                cs[c->trees[t].gslot] = -ps[t];
+                sh[c->trees[t].gslot] = (pops[t].disc == SRMG_HELMHOLTZ) ? -(ps[t] * pops[t].k2) : 0.0;
+            }
+            HIPCHK(hipMemcpy(G.cs.p, cs.data(), cs.size() * sizeof(double), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(G.shift.p, sh.data(), sh.size() * sizeof(double), hipMemcpyHostToDevice));
+        }
+        c->pp_host = pp;
+    }
+    c->ps = ps;
+    c->pops = pops;
+    c->pomega = omega;
+}
+
 template <typename T>
 static LevelArgs<T> level_args(srmg_ctx *c, int l /*0-based*/, double s, const srmg_op *op, double omega) {
     LevelArgs<T> a;
     a.blk0 = c->lv[l].blk0;
     a.nblk = c->lv[l].nblk;
-    int64_t f = (int64_t)1 << l;
-    double sl = s / (double)(f * f);
-    double cc = -2.0 * c->D;
-    if (op->disc == SRMG_HELMHOLTZ) cc = cc - sl * op->k2;
+    double sl, cc, nwc;
+    level_scalars(c->D, l, s, op, omega, sl, cc, nwc);
     a.s = (T)sl;
     a.cc = (T)cc;
-    a.nwc = (T)(-(omega / cc));
+    a.nwc = (T)nwc;
     for (int q = 0; q < 6; q++) a.sg[q] = T(1);
     for (int d = 0; d < c->D; d++)
         for (int side = 0; side < 2; side++) {
@@ -428,6 +624,15 @@ static LevelArgs<T> level_args(srmg_ctx *c, int l /*0-based*/, double s, const s
         a.wp = a.wm + (n + 1);
     } else {
         a.wm = a.wp = wbase;
+    }
+    // several trees: per-block values from the tables installed by prepare()
+    // (their -(omega/diagelm) is for c->pomega; residual kernels do not use it)
+    if (c->trees.size() > 1) {
+        a.pp = std::is_same<T, double>::value ? (const T *)c->d_pp64.p : (const T *)c->d_pp32.p;
+        a.pix = c->d_pix.as<int>();
+    } else {
+        a.pp = nullptr;
+        a.pix = nullptr;
     }
     return a;
 }
@@ -546,13 +751,21 @@ struct Ops {
         hipLaunchKernelGGL((k_residual<D, M, T>), dim3(nwg((int64_t)a.nblk * LY::V)), dim3(WG), 0, c->stream,
                            X(r, l), X(u, l), X(b, l), I(u), nbr(), coord(), a);
     }
-    // fixed-order total of the first np partials into d_sum[0]
-    void sum_partials(int np) {
-        int chunk = (np + SUM_CHUNKS - 1) / SUM_CHUNKS;
-        double *p2 = c->d_sum.as<double>() + 4;
-        hipLaunchKernelGGL(k_sum_chunks, dim3(SUM_CHUNKS), dim3(WG), 0, c->stream, c->d_partial.as<double>(), np,
-                           chunk, p2);
-        hipLaunchKernelGGL(k_sum, dim3(1), dim3(WG), 0, c->stream, p2, SUM_CHUNKS, c->d_sum.as<double>());
+    // fixed-order totals of the partials: one tree -> d_sum[0] (chunked
+    // two-stage sum of the first np partials); several trees -> one total per
+    // tree at d_sum[TSUM + t] over its (tree, level) ranges
+    void sum_partials(int np, const DevBuf &ranges) {
+        if (c->trees.size() == 1) {
+            int chunk = (np + SUM_CHUNKS - 1) / SUM_CHUNKS;
+            double *p2 = c->d_sum.as<double>() + 4;
+            hipLaunchKernelGGL(k_sum_chunks, dim3(SUM_CHUNKS), dim3(WG), 0, c->stream, c->d_partial.as<double>(), np,
+                               chunk, p2);
+            hipLaunchKernelGGL(k_sum, dim3(1), dim3(WG), 0, c->stream, p2, SUM_CHUNKS, c->d_sum.as<double>());
+        } else {
+            hipLaunchKernelGGL(k_sum_ranges, dim3((unsigned)c->trees.size()), dim3(WG), 0, c->stream,
+                               c->d_partial.as<double>(), ranges.as<int2>(), c->nlevels,
+                               c->d_sum.as<double>() + TSUM);
+        }
     }
     void restrict_(FieldBuf &x, int l) {
         const LevelInfo &L = c->lv[l];
@@ -597,7 +810,7 @@ struct Ops {
         hipLaunchKernelGGL((k_fill_interp<D, M, T>), dim3(nwg((int64_t)L.nslot * LY::F)), dim3(WG), 0, c->stream,
                            I(x), X(x, l - 1), nbr(), c->d_slots.as<int4>(), L.slot0, L.nslot, ac);
     }
-    // queue the squared-residual partial sums of all levels and their total
+    // queue the squared-residual sums (leaf blocks, weighted by 2^(-D(l-1))) and their totals
     int resnorm_partials(FieldBuf &u, FieldBuf &b, double s, const srmg_op *op) {
         int np = 0;
         for (int l = 0; l < c->nlevels; l++) {
@@ -613,9 +826,9 @@ struct Ops {
                                    c->d_leaves.as<int>() + L.leaf0, (const int *)nullptr, L.nleaf, a, wgt,
                                    c->d_partial.as<double>() + np);
             });
-            np += (int)g;
+            np += L.nleaf;
         }
-        sum_partials(np);
+        sum_partials(np, c->d_rng_leaf);
         return np;
     }
     // mixed precision: be = b + L u / s (FP32) on every level, plus the leaf
@@ -634,9 +847,9 @@ struct Ops {
                                    (const int *)nullptr, c->d_isleaf.as<int>(), a.nblk, a, wgt,
                                    c->d_partial.as<double>() + np);
             });
-            np += (int)g;
+            np += a.nblk;
         }
-        sum_partials(np);
+        sum_partials(np, c->d_rng_blk);
     }
 
     // exact FP64 solve of L_1 u + s b = 0 on the complete level 1
@@ -806,57 +1019,60 @@ static void axis_decomp(int n, bool radial, double alow, double ahigh, double *l
 
 template <int D, int M, typename T>
 void Ops<D, M, T>::coarse_exact(FieldBuf &u, FieldBuf &b, double s, const srmg_op *op) {
-    CoarseCache &cc = c->coarse;
-    const LevelInfo &L1 = c->lv[0];
-    int N[3] = {1, 1, 1};
-    for (int d = 0; d < D; d++) N[d] = M * L1.ext[d];
-    bool hit = cc.valid && cc.geom == op->geometry && std::memcmp(cc.bc, op->bc, 27) == 0;
-    if (!hit) {
-        int rdim = (op->geometry == SRMG_CYL1) ? 0 : (op->geometry == SRMG_CYL2 ? 1 : -1);
-        for (int d = 0; d < D; d++) {
-            int il = 0, ih = 0, m = 1;
-            for (int e = 0; e < D; e++) {
-                il += ((e == d ? -1 : 0) + 1) * m;
-                ih += ((e == d ? 1 : 0) + 1) * m;
-                m *= 3;
+    (void)s;  // each tree's -s and shift come from its group's tables (prepare())
+    for (CoarseGroup &G : c->groups) {
+        const int *N = G.N;
+        bool hit = G.valid && G.geom == op->geometry && std::memcmp(G.bc, op->bc, 27) == 0;
+        if (!hit) {
+            hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+            HIPCHK(hipStreamIsCapturing(c->stream, &st));
+            if (st != hipStreamCaptureStatusNone) fail("coarse-solve setup inside graph capture");
+            int rdim = (op->geometry == SRMG_CYL1) ? 0 : (op->geometry == SRMG_CYL2 ? 1 : -1);
+            for (int d = 0; d < D; d++) {
+                int il = 0, ih = 0, m = 1;
+                for (int e = 0; e < D; e++) {
+                    il += ((e == d ? -1 : 0) + 1) * m;
+                    ih += ((e == d ? 1 : 0) + 1) * m;
+                    m *= 3;
+                }
+                int n = N[d];
+                std::vector<double> lam(n), P((size_t)n * n), Q((size_t)n * n);
+                axis_decomp(n, d == rdim, (double)op->bc[il], (double)op->bc[ih], lam.data(), P.data(), Q.data());
+                G.lam[d].alloc(n * sizeof(double));
+                G.P[d].alloc((size_t)n * n * sizeof(double));
+                G.Q[d].alloc((size_t)n * n * sizeof(double));
+                HIPCHK(hipMemcpy(G.lam[d].p, lam.data(), n * sizeof(double), hipMemcpyHostToDevice));
+                HIPCHK(hipMemcpy(G.P[d].p, P.data(), (size_t)n * n * sizeof(double), hipMemcpyHostToDevice));
+                HIPCHK(hipMemcpy(G.Q[d].p, Q.data(), (size_t)n * n * sizeof(double), hipMemcpyHostToDevice));
             }
-            int n = N[d];
-            std::vector<double> lam(n), P((size_t)n * n), Q((size_t)n * n);
-            axis_decomp(n, d == rdim, (double)op->bc[il], (double)op->bc[ih], lam.data(), P.data(), Q.data());
-            cc.lam[d].alloc(n * sizeof(double));
-            cc.P[d].alloc((size_t)n * n * sizeof(double));
-            cc.Q[d].alloc((size_t)n * n * sizeof(double));
-            HIPCHK(hipMemcpy(cc.lam[d].p, lam.data(), n * sizeof(double), hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(cc.P[d].p, P.data(), (size_t)n * n * sizeof(double), hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(cc.Q[d].p, Q.data(), (size_t)n * n * sizeof(double), hipMemcpyHostToDevice));
-            cc.N[d] = n;
+            G.valid = true;
+            G.geom = op->geometry;
+            std::memcpy(G.bc, op->bc, 27);
         }
-        cc.valid = true;
-        cc.geom = op->geometry;
-        std::memcpy(cc.bc, op->bc, 27);
+        const size_t total = (size_t)N[0] * N[1] * (D == 3 ? N[2] : 1);
+        const int nblk = G.ntree * G.nper;
+        double *g = c->d_grid.as<double>(), *g2 = c->d_grid2.as<double>();
+        hipLaunchKernelGGL((k_grid_gather<D, M, T>), dim3(nwg((int64_t)nblk * LY::V)), dim3(WG), 0, c->stream, g,
+                           X(b, 0), coord(), G.l1_blk0, nblk, N[0], N[1], G.nper, total, G.cs.as<double>());
+        int stride = 1;
+        for (int d = 0; d < D; d++) {
+            axis(G.Q[d].as<double>(), N[d], stride, (int)(G.ntree * total / N[d]), g, g2);
+            std::swap(g, g2);
+            stride *= N[d];
+        }
+        hipLaunchKernelGGL(k_eig_scale, dim3(nwg(total * G.ntree)), dim3(WG), 0, c->stream, g,
+                           G.lam[0].as<double>(), G.lam[1].as<double>(),
+                           D == 3 ? G.lam[2].as<double>() : G.lam[1].as<double>(), N[0], N[1], D,
+                           G.shift.as<double>(), total, G.ntree);
+        stride = 1;
+        for (int d = 0; d < D; d++) {
+            axis(G.P[d].as<double>(), N[d], stride, (int)(G.ntree * total / N[d]), g, g2);
+            std::swap(g, g2);
+            stride *= N[d];
+        }
+        hipLaunchKernelGGL((k_grid_scatter<D, M, T>), dim3(nwg((int64_t)nblk * LY::V)), dim3(WG), 0, c->stream,
+                           X(u, 0), g, coord(), G.l1_blk0, nblk, N[0], N[1], G.nper, total);
     }
-    size_t total = (size_t)N[0] * N[1] * N[2];
-    double *g = c->d_grid.as<double>(), *g2 = c->d_grid2.as<double>();
-    hipLaunchKernelGGL((k_grid_gather<D, M, T>), dim3(nwg((int64_t)L1.nblk * LY::V)), dim3(WG), 0, c->stream, g,
-                       X(b, 0), coord(), L1.blk0, L1.nblk, N[0], N[1], -s);
-    int stride = 1;
-    for (int d = 0; d < D; d++) {
-        axis(cc.Q[d].as<double>(), N[d], stride, (int)(total / N[d]), g, g2);
-        std::swap(g, g2);
-        stride *= N[d];
-    }
-    double shift = (op->disc == SRMG_HELMHOLTZ) ? -(s * op->k2) : 0.0;
-    hipLaunchKernelGGL(k_eig_scale, dim3(nwg(total)), dim3(WG), 0, c->stream, g, cc.lam[0].as<double>(),
-                       cc.lam[1].as<double>(), D == 3 ? cc.lam[2].as<double>() : cc.lam[1].as<double>(), N[0], N[1],
-                       D, shift, total);
-    stride = 1;
-    for (int d = 0; d < D; d++) {
-        axis(cc.P[d].as<double>(), N[d], stride, (int)(total / N[d]), g, g2);
-        std::swap(g, g2);
-        stride *= N[d];
-    }
-    hipLaunchKernelGGL((k_grid_scatter<D, M, T>), dim3(nwg((int64_t)L1.nblk * LY::V)), dim3(WG), 0, c->stream,
-                       X(u, 0), g, coord(), L1.blk0, L1.nblk, N[0], N[1]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1022,11 +1238,18 @@ static void run_maybe_graph(srmg_ctx *c, Graph &gr, bool use_graph, F &&body) {
     HIPCHK(hipGraphLaunch(gr.x, c->stream));
 }
 
-static double read_sum(srmg_ctx *c) {
-    double v = 0.0;
-    HIPCHK(hipMemcpyAsync(&v, c->d_sum.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+// the residual norms queued last (one per tree)
+static std::vector<double> read_norms(srmg_ctx *c) {
+    const int nt = (int)c->trees.size();
+    std::vector<double> v(nt);
+    if (nt == 1)
+        HIPCHK(hipMemcpyAsync(v.data(), c->d_sum.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    else
+        HIPCHK(hipMemcpyAsync(v.data(), c->d_sum.as<double>() + TSUM, nt * sizeof(double), hipMemcpyDeviceToHost,
+                              c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    return std::sqrt(v);
+    for (auto &x : v) x = std::sqrt(x);
+    return v;
 }
 
 static void check_opts(const srmg_solve_opts *o) {
@@ -1034,51 +1257,68 @@ static void check_opts(const srmg_solve_opts *o) {
     if (o->max_cycles < 0 || o->ndown < 0 || o->nup < 0 || o->ntop < 0) fail("bad solve options");
 }
 
-static void solve_plain(srmg_ctx *c, int ui, int bi, int ri, int u1i, double s, const srmg_op *op,
-                        const srmg_solve_opts *o, int *cycles, double *hist) {
+// relative residuals; true when every tree is at or below tol
+static bool record(const std::vector<double> &r, const std::vector<double> &r0, double tol, int k, int max_cycles,
+                   double *hist) {
+    bool done = true;
+    for (size_t t = 0; t < r.size(); t++) {
+        double rel = r[t] / r0[t];
+        if (hist) hist[t * (size_t)(max_cycles + 1) + k] = rel;
+        if (!std::isfinite(rel)) fail("residual is not finite after cycle " + std::to_string(k));
+        if (!(rel <= tol)) done = false;
+    }
+    return done;
+}
+
+static std::vector<double> safe_r0(std::vector<double> r0) {
+    for (auto &x : r0)
+        if (!(x > 0.0)) x = 1.0;
+    return r0;
+}
+
+static void solve_plain(srmg_ctx *c, int ui, int bi, int ri, int u1i, const srmg_solve_opts *o, int *cycles,
+                        double *hist) {
     check_opts(o);
+    const double s = c->ps[0];
+    const srmg_op *op = &c->pops[0];
     int t = same_type(c, {ui, bi, ri, u1i});
     FieldBuf &u = field(c, ui), &b = field(c, bi), &r = field(c, ri), &u1 = field(c, u1i);
     dispatch(c, t, [&](auto &ops) {
         using T = std::remove_reference_t<decltype(*ops.X(u, 0))>;
         for (int l = 1; l < c->nlevels; l++) ops.fill_interp(u, l, level_args<T>(c, l - 1, s, op, 1.0));
         ops.resnorm_partials(zero_field(c, t), b, s, op);
-        double r0 = read_sum(c);
-        if (!(r0 > 0.0)) r0 = 1.0;
+        std::vector<double> r0 = safe_r0(read_norms(c));
         ops.resnorm_partials(u, b, s, op);
-        double rel = read_sum(c) / r0;
-        if (hist) hist[0] = rel;
+        bool done = record(read_norms(c), r0, o->tol, 0, o->max_cycles, hist);
         int k = 0;
         Graph gr;
-        while (k < o->max_cycles && !(rel <= o->tol)) {
+        while (k < o->max_cycles && !done) {
             // the first cycle runs eagerly so lazily built state (coarse-solve
             // matrices) exists before a graph is captured
             run_maybe_graph(c, gr, o->use_graph && k > 0, [&] {
                 ops.vcycle(u, b, r, u1, s, op, 0, o->ndown, o->ntop, o->nup, o->omega);
                 ops.resnorm_partials(u, b, s, op);
             });
-            rel = read_sum(c) / r0;
             k++;
-            if (hist) hist[k] = rel;
-            if (!std::isfinite(rel)) fail("residual is not finite after cycle " + std::to_string(k));
+            done = record(read_norms(c), r0, o->tol, k, o->max_cycles, hist);
         }
         if (cycles) *cycles = k;
     });
 }
 
-// Mixed precision defect correction (FP64 residual and update, FP32 V-cycle on
-// the correction equation, FP64 coarsest-level solve).
+// Mixed precision defect correction (FP64 residual and update, FP32 V-cycles
+// on the correction equation, FP64 coarsest-level solve).
 template <int D, int M>
 static void solve_mixed_dm(srmg_ctx *c, FieldBuf &u, FieldBuf &b, FieldBuf &e, FieldBuf &be, FieldBuf &re,
-                           FieldBuf &ue, double s, const srmg_op *op, const srmg_solve_opts *o, int *cycles,
-                           double *hist) {
+                           FieldBuf &ue, const srmg_solve_opts *o, int *cycles, double *hist) {
     using LY = Lay<D, M>;
+    const double s = c->ps[0];
+    const srmg_op *op = &c->pops[0];
     Ops<D, M, double> o64(c);
     Ops<D, M, float> o32(c);
     for (int l = 1; l < c->nlevels; l++) o64.fill_interp(u, l, level_args<double>(c, l - 1, s, op, 1.0));
     o64.resnorm_partials(zero_field(c, SRMG_F64), b, s, op);
-    double r0 = read_sum(c);
-    if (!(r0 > 0.0)) r0 = 1.0;
+    std::vector<double> r0 = safe_r0(read_norms(c));
     auto defect_and_norm = [&] { o64.defect_and_norm(be.data.as<float>(), u, b, s, op); };
     auto correct = [&] {
         size_t n = (size_t)c->nb * LY::V, ni = e.ib.bytes / sizeof(float);
@@ -1095,29 +1335,26 @@ static void solve_mixed_dm(srmg_ctx *c, FieldBuf &u, FieldBuf &b, FieldBuf &e, F
         defect_and_norm();
     };
     defect_and_norm();
-    double rel = read_sum(c) / r0;
-    if (hist) hist[0] = rel;
+    bool done = record(read_norms(c), r0, o->tol, 0, o->max_cycles, hist);
     int k = 0;
     Graph gr;
-    while (k < o->max_cycles && !(rel <= o->tol)) {
+    while (k < o->max_cycles && !done) {
         run_maybe_graph(c, gr, o->use_graph && k > 0, correct);
-        rel = read_sum(c) / r0;
         k++;
-        if (hist) hist[k] = rel;
-        if (!std::isfinite(rel)) fail("residual is not finite after cycle " + std::to_string(k));
+        done = record(read_norms(c), r0, o->tol, k, o->max_cycles, hist);
     }
     if (cycles) *cycles = k;
 }
 
-static void solve_mixed(srmg_ctx *c, int ui, int bi, int ei, int bei, int rei, int uei, double s,
-                        const srmg_op *op, const srmg_solve_opts *o, int *cycles, double *hist) {
+static void solve_mixed(srmg_ctx *c, int ui, int bi, int ei, int bei, int rei, int uei, const srmg_solve_opts *o,
+                        int *cycles, double *hist) {
     check_opts(o);
     if (same_type(c, {ui, bi}) != SRMG_F64) fail("u and b must be FP64");
     if (same_type(c, {ei, bei, rei, uei}) != SRMG_F32) fail("correction fields must be FP32");
     FieldBuf &u = field(c, ui), &b = field(c, bi), &e = field(c, ei), &be = field(c, bei), &re = field(c, rei),
              &ue = field(c, uei);
 #define SRMG_MCASE(DD, MM) \
-    if (c->D == DD && c->M == MM) return solve_mixed_dm<DD, MM>(c, u, b, e, be, re, ue, s, op, o, cycles, hist);
+    if (c->D == DD && c->M == MM) return solve_mixed_dm<DD, MM>(c, u, b, e, be, re, ue, o, cycles, hist);
     SRMG_MCASE(2, 16)
     SRMG_MCASE(2, 8)
     SRMG_MCASE(3, 8)
@@ -1165,8 +1402,26 @@ int srmg_set_tree(srmg_ctx *c, int nlevels, int64_t nblocks, const int64_t *pair
                   const int64_t *child, const int64_t *child_off) {
     return guarded([&] {
         if (!c) fail("null context");
-        set_tree(c, nlevels, nblocks, pairs, pairs_off, neighbor, neighbor_off, boundary, boundary_off, refboundary,
-                 refboundary_off, child, child_off);
+        set_forest(c, {TreeRecords{nlevels, nblocks, pairs, pairs_off, neighbor, neighbor_off, boundary, boundary_off,
+                                   refboundary, refboundary_off, child, child_off}});
+    });
+}
+
+int srmg_set_forest(srmg_ctx *c, int ntrees, const int *nlevels, const int64_t *nblocks,
+                    const int64_t *const *pairs, const int64_t *const *pairs_off, const int64_t *const *neighbor,
+                    const int64_t *const *neighbor_off, const int64_t *const *boundary,
+                    const int64_t *const *boundary_off, const int64_t *const *refboundary,
+                    const int64_t *const *refboundary_off, const int64_t *const *child,
+                    const int64_t *const *child_off) {
+    return guarded([&] {
+        if (!c) fail("null context");
+        if (ntrees < 1) fail("ntrees must be >= 1");
+        std::vector<TreeRecords> tr;
+        for (int t = 0; t < ntrees; t++)
+            tr.push_back(TreeRecords{nlevels[t], nblocks[t], pairs[t], pairs_off[t], neighbor[t], neighbor_off[t],
+                                     boundary[t], boundary_off[t], refboundary[t], refboundary_off[t], child[t],
+                                     child_off[t]});
+        set_forest(c, tr);
     });
 }
 
@@ -1300,17 +1555,34 @@ int srmg_fill_ghost_all(srmg_ctx *c, int id, const int8_t *bc) {
     });
 }
 
+static void do_vcycle(srmg_ctx *c, int u, int b, int r, int u1, int lmax, int ndown, int ntop, int nup,
+                      double omega) {
+    if (lmax < 0 || lmax > c->nlevels) fail("lmax out of range");
+    int t = same_type(c, {u, b, r, u1});
+    dispatch(c, t, [&](auto &o) {
+        o.vcycle(field(c, u), field(c, b), field(c, r), field(c, u1), c->ps[0], &c->pops[0], lmax, ndown, ntop, nup,
+                 omega);
+    });
+    HIPCHK(hipGetLastError());
+}
+
 int srmg_vcycle(srmg_ctx *c, int u, int b, int r, int u1, double s, const srmg_op *op, int lmax, int ndown,
                 int ntop, int nup, double omega) {
     return guarded([&] {
         check_tree(c);
         if (!op) fail("null op");
-        if (lmax < 0 || lmax > c->nlevels) fail("lmax out of range");
-        int t = same_type(c, {u, b, r, u1});
-        dispatch(c, t, [&](auto &o) {
-            o.vcycle(field(c, u), field(c, b), field(c, r), field(c, u1), s, op, lmax, ndown, ntop, nup, omega);
-        });
-        HIPCHK(hipGetLastError());
+        prepare(c, &s, op, omega, true);
+        do_vcycle(c, u, b, r, u1, lmax, ndown, ntop, nup, omega);
+    });
+}
+
+int srmg_vcycle_batched(srmg_ctx *c, int u, int b, int r, int u1, const double *s, const srmg_op *ops, int lmax,
+                        int ndown, int ntop, int nup, double omega) {
+    return guarded([&] {
+        check_tree(c);
+        if (!ops || !s) fail("null argument");
+        prepare(c, s, ops, omega, false);
+        do_vcycle(c, u, b, r, u1, lmax, ndown, ntop, nup, omega);
     });
 }
 
@@ -1319,6 +1591,7 @@ int srmg_fmg(srmg_ctx *c, int u, int b, int r, int u1, double s, const srmg_op *
     return guarded([&] {
         check_tree(c);
         if (!op) fail("null op");
+        prepare(c, &s, op, omega, true);
         int t = same_type(c, {u, b, r, u1});
         dispatch(c, t, [&](auto &o) {
             o.fmg(field(c, u), field(c, b), field(c, r), field(c, u1), s, op, ndown, ntop, nup, omega);
@@ -1327,17 +1600,31 @@ int srmg_fmg(srmg_ctx *c, int u, int b, int r, int u1, double s, const srmg_op *
     });
 }
 
+static void do_resnorm(srmg_ctx *c, int b, int u, double *out) {
+    int t = same_type(c, {u, b});
+    dispatch(c, t, [&](auto &o) { o.resnorm_partials(field(c, u), field(c, b), c->ps[0], &c->pops[0]); });
+    std::vector<double> v = read_norms(c);
+    for (size_t k = 0; k < v.size(); k++) out[k] = v[k];
+}
+
 int srmg_resnorm(srmg_ctx *c, int r, int b, int u, double s, const srmg_op *op, double *out) {
     return guarded([&] {
         check_tree(c);
         if (!op || !out) fail("null argument");
         (void)r;
-        int t = same_type(c, {u, b});
-        dispatch(c, t, [&](auto &o) { o.resnorm_partials(field(c, u), field(c, b), s, op); });
-        double v = 0.0;
-        HIPCHK(hipMemcpyAsync(&v, c->d_sum.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
-        *out = std::sqrt(v);
+        if (c->trees.size() != 1) fail("several trees: use srmg_resnorm_batched");
+        prepare(c, &s, op, c->pomega, true);
+        do_resnorm(c, b, u, out);
+    });
+}
+
+int srmg_resnorm_batched(srmg_ctx *c, int r, int b, int u, const double *s, const srmg_op *ops, double *out) {
+    return guarded([&] {
+        check_tree(c);
+        if (!ops || !s || !out) fail("null argument");
+        (void)r;
+        prepare(c, s, ops, c->pomega, false);
+        do_resnorm(c, b, u, out);
     });
 }
 
@@ -1350,7 +1637,20 @@ int srmg_solve(srmg_ctx *c, int u, int b, int r, int u1, double s, const srmg_op
     return guarded([&] {
         check_tree(c);
         if (!op) fail("null op");
-        solve_plain(c, u, b, r, u1, s, op, opts, cycles, hist);
+        check_opts(opts);
+        prepare(c, &s, op, opts->omega, true);
+        solve_plain(c, u, b, r, u1, opts, cycles, hist);
+    });
+}
+
+int srmg_solve_batched(srmg_ctx *c, int u, int b, int r, int u1, const double *s, const srmg_op *ops,
+                       const srmg_solve_opts *opts, int *cycles, double *hist) {
+    return guarded([&] {
+        check_tree(c);
+        if (!ops || !s) fail("null argument");
+        check_opts(opts);
+        prepare(c, s, ops, opts->omega, false);
+        solve_plain(c, u, b, r, u1, opts, cycles, hist);
     });
 }
 
@@ -1359,7 +1659,28 @@ int srmg_solve_mixed(srmg_ctx *c, int u, int b, int e, int be, int re, int ue, d
     return guarded([&] {
         check_tree(c);
         if (!op) fail("null op");
-        solve_mixed(c, u, b, e, be, re, ue, s, op, opts, cycles, hist);
+        check_opts(opts);
+        prepare(c, &s, op, opts->omega, true);
+        solve_mixed(c, u, b, e, be, re, ue, opts, cycles, hist);
+    });
+}
+
+int srmg_solve_mixed_batched(srmg_ctx *c, int u, int b, int e, int be, int re, int ue, const double *s,
+                             const srmg_op *ops, const srmg_solve_opts *opts, int *cycles, double *hist) {
+    return guarded([&] {
+        check_tree(c);
+        if (!ops || !s) fail("null argument");
+        check_opts(opts);
+        prepare(c, s, ops, opts->omega, false);
+        solve_mixed(c, u, b, e, be, re, ue, opts, cycles, hist);
+    });
+}
+
+int srmg_tree_count(srmg_ctx *c, int *ntrees) {
+    return guarded([&] {
+        check_tree(c);
+        if (!ntrees) fail("null argument");
+        *ntrees = (int)c->trees.size();
     });
 }
 
@@ -1369,6 +1690,7 @@ int srmg_time_kernel(srmg_ctx *c, int kind, int level, int u, int b, int r, doub
         check_tree(c);
         if (!op || !ms_per_launch || reps < 1) fail("bad arguments");
         if (level < 1 || level > c->nlevels) fail("level out of range");
+        prepare(c, &s, op, 1.0, true);
         int t = same_type(c, {u, b, r});
         hipEvent_t e0, e1;
         HIPCHK(hipEventCreate(&e0));
@@ -1410,6 +1732,7 @@ int srmg_autotune(srmg_ctx *c, int u, int b, double s, const srmg_op *op, int re
     return guarded([&] {
         check_tree(c);
         if (!op || reps < 1) fail("bad arguments");
+        prepare(c, &s, op, 1.0, true);
         int t = same_type(c, {u, b});
         FieldBuf &fu = field(c, u);
         hipEvent_t e0, e1;

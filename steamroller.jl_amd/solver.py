@@ -131,6 +131,12 @@ def lib():
         "srmg_solve_mixed": [P, i32, i32, i32, i32, i32, i32, dbl, P, P, C.POINTER(i32), P],
         "srmg_axis_decomp": [i32, i32, dbl, dbl, P, P, P],
         "srmg_time_kernel": [P, i32, i32, i32, i32, i32, dbl, P, i32, C.POINTER(dbl)],
+        "srmg_set_forest": [P, i32, P, P] + [P] * 10,
+        "srmg_tree_count": [P, C.POINTER(i32)],
+        "srmg_vcycle_batched": [P, i32, i32, i32, i32, P, P, i32, i32, i32, i32, dbl],
+        "srmg_resnorm_batched": [P, i32, i32, i32, P, P, P],
+        "srmg_solve_batched": [P, i32, i32, i32, i32, P, P, P, C.POINTER(i32), P],
+        "srmg_solve_mixed_batched": [P, i32, i32, i32, i32, i32, i32, P, P, P, C.POINTER(i32), P],
         "srmg_level_info": [P, i32, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64)],
         "srmg_autotune": [P, i32, i32, dbl, P, i32, P],
         "srmg_set_sweep_config": [P, i32, i32, i32],
@@ -169,13 +175,52 @@ def axis_decomp(n: int, radial: bool, alow: float, ahigh: float):
 class Context:
     """One device context bound to a tree (``srmg_create`` + ``srmg_set_tree``)."""
 
-    def __init__(self, tree, conn, M: int):
+    def __init__(self, tree, conn, M: int, forest=None):
         self.D, self.M = tree.D, M
         self.tree, self.conn = tree, conn
         h = C.c_void_p()
         _check(lib().srmg_create(C.byref(h), tree.D, M))
         self.h = h
-        self.regrid(tree, conn)
+        if forest is None:
+            self.ntrees = 1
+            self.regrid(tree, conn)
+        else:
+            self.set_forest(forest)
+
+    @classmethod
+    def forest(cls, trees_conns, M: int) -> "Context":
+        """One context for several independent trees (``srmg_set_forest``);
+        host fields are the trees' host arrays concatenated in order
+        (see :func:`concat_fields`)."""
+        t0, c0 = trees_conns[0]
+        return cls(t0, c0, M, forest=list(trees_conns))
+
+    def set_forest(self, trees_conns):
+        D = trees_conns[0][0].D
+        keep, per = [], [[] for _ in range(10)]
+        nl, nbk = [], []
+        for tree, conn in trees_conns:
+            pairs, poff = flatten_pairs(tree)
+            arrs = [pairs, poff]
+            for lists, w in ((conn.neighbor, 2 + D), (conn.boundary, 2 + 2 * D),
+                             (conn.refboundary, 2 + 2 * D), (conn.child, 2 + D)):
+                a, o = flatten_records(lists, w)
+                if len(a) == 0:
+                    a = np.zeros((1, w), np.int64)
+                arrs += [a, o]
+            keep.append(arrs)
+            for k in range(10):
+                per[k].append(arrs[k].ctypes.data)
+            nl.append(len(tree))
+            nbk.append(tree.nblocks())
+        ptrs = [(C.c_void_p * len(v))(*v) for v in per]
+        nl_a = np.array(nl, np.int32)
+        nb_a = np.array(nbk, np.int64)
+        self._tables = (keep, ptrs, nl_a, nb_a)
+        _check(lib().srmg_set_forest(self.h, len(trees_conns), _p(nl_a), _p(nb_a), *ptrs))
+        self.ntrees = len(trees_conns)
+        self.trees = [t for t, _ in trees_conns]
+        self.nblocks = int(nb_a.sum())
 
     def regrid(self, tree, conn):
         D = tree.D
@@ -190,6 +235,7 @@ class Context:
         self._tables = arrs
         _check(lib().srmg_set_tree(self.h, len(tree), tree.nblocks(), *[_p(a) for a in arrs]))
         self.tree, self.conn = tree, conn
+        self.ntrees = 1
         self.nblocks = tree.nblocks()
 
     def level_info(self, level: int):
@@ -343,3 +389,70 @@ def solve_mixed(u, b, e, be, re, ue, s, geometry, bc, lpldisc, *, tol=1e-8, max_
     _check(lib().srmg_solve_mixed(u.ctx.h, u.id, b.id, e.id, be.id, re.id, ue.id, float(s),
                                   C.byref(op), C.byref(o), C.byref(n), _p(hist)))
     return n.value, hist[: n.value + 1].copy()
+
+
+# ---- forests: several independent problems per launch ----
+
+def concat_fields(fields):
+    """Host fields of several trees concatenated along the block axis (the
+    host layout of a forest)."""
+    return np.asfortranarray(np.concatenate([f.u if hasattr(f, "u") else f for f in fields], axis=-1))
+
+
+def split_field(arr, nblocks):
+    """Inverse of :func:`concat_fields`: per-tree views."""
+    out, o = [], 0
+    for n in nblocks:
+        out.append(arr[..., o:o + n])
+        o += n
+    return out
+
+
+def _ops(geometry, bc, discs, coarse):
+    arr = (_Op * len(discs))()
+    for i, d in enumerate(discs):
+        arr[i] = make_op(geometry, bc, d, coarse)
+    return arr
+
+
+def vcycle_batched(u, b, r, u1, s, geometry, bc, discs, *, lmax=0, ndown=3, ntop=5, nup=3, omega=1.0,
+                   coarse=COARSE_GS):
+    """vcycle! on every tree of a forest: tree t with scale s[t] and discretization discs[t]."""
+    sv = np.ascontiguousarray(np.asarray(s, np.float64))
+    ops = _ops(geometry, bc, discs, coarse)
+    _check(lib().srmg_vcycle_batched(u.ctx.h, u.id, b.id, r.id, u1.id, _p(sv), ops, int(lmax), int(ndown),
+                                     int(ntop), int(nup), float(omega)))
+
+
+def resnorm_batched(r, b, u, s, geometry, bc, discs, *, coarse=COARSE_GS):
+    sv = np.ascontiguousarray(np.asarray(s, np.float64))
+    ops = _ops(geometry, bc, discs, coarse)
+    out = np.zeros(len(discs))
+    _check(lib().srmg_resnorm_batched(u.ctx.h, r.id, b.id, u.id, _p(sv), ops, _p(out)))
+    return out
+
+
+def solve_batched(u, b, r, u1, s, geometry, bc, discs, *, tol=1e-8, max_cycles=50, ndown=2, ntop=4, nup=2,
+                  omega=1.0, coarse=COARSE_EXACT, use_graph=True):
+    """Solve every tree of a forest to tol; returns (cycles, history[ntrees, cycles + 1])."""
+    sv = np.ascontiguousarray(np.asarray(s, np.float64))
+    ops = _ops(geometry, bc, discs, coarse)
+    o = _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph)
+    hist = np.zeros((len(discs), max_cycles + 1))
+    n = C.c_int()
+    _check(lib().srmg_solve_batched(u.ctx.h, u.id, b.id, r.id, u1.id, _p(sv), ops, C.byref(o), C.byref(n),
+                                    _p(hist)))
+    return n.value, hist[:, : n.value + 1].copy()
+
+
+def solve_mixed_batched(u, b, e, be, re, ue, s, geometry, bc, discs, *, tol=1e-8, max_cycles=50, ndown=2,
+                        ntop=4, nup=2, omega=1.0, coarse=COARSE_EXACT, use_graph=True, inner_cycles=1):
+    """FP64 defect correction around FP32 V-cycles on every tree of a forest."""
+    sv = np.ascontiguousarray(np.asarray(s, np.float64))
+    ops = _ops(geometry, bc, discs, coarse)
+    o = _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph, inner_cycles)
+    hist = np.zeros((len(discs), max_cycles + 1))
+    n = C.c_int()
+    _check(lib().srmg_solve_mixed_batched(u.ctx.h, u.id, b.id, e.id, be.id, re.id, ue.id, _p(sv), ops,
+                                          C.byref(o), C.byref(n), _p(hist)))
+    return n.value, hist[:, : n.value + 1].copy()

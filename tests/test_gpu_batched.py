@@ -1,0 +1,139 @@
+"""Batched launches (several independent problems per launch) against the same
+problems solved one by one.
+
+A forest holds the three photoionization Helmholtz terms on the cfg1 tree
+(src/photo.jl:45-67: same source, own k^2 each, s = h^2) and uneven random
+trees. Every tree must get exactly its own V-cycle: the batched results equal
+the single-problem results bit for bit, with the reference's coarsest-level
+treatment and with the exact coarsest-level solve alike.
+"""
+
+import numpy as np
+import pytest
+
+from srmg import configs, solver
+from srmg.solver import F32, F64
+
+pytestmark = pytest.mark.gpu
+
+
+def _problems(n_random=4, max_cells=4e4):
+    probs = list(configs.make_helmholtz_cfg1(0))
+    _, k2s = configs.bourdon3_terms()
+    for i in range(n_random):
+        probs.append(configs.make_random_tree_problem(100 + i, min_cells=1e4, max_cells=max_cells,
+                                                      helm_k2=float(k2s[i % 3])))
+    return probs
+
+
+def _disc(p):
+    return solver.HelmholtzDiscretization(p.D, 2, p.k2) if p.helm else solver.LaplacianDiscretization(p.D, 2)
+
+
+def _valid(a, p):
+    G, M = p.b.G, p.M
+    return a[(slice(G, G + M),) * p.D]
+
+
+def _forest(probs, dtype=F64):
+    ctx = solver.Context.forest([(p.tree, p.conn) for p in probs], probs[0].M)
+    fields = [ctx.field(dtype) for _ in range(4)]
+    fields[1].upload(solver.concat_fields([p.b for p in probs]))
+    return ctx, fields
+
+
+def _download(f, probs):
+    host = solver.concat_fields([p.b.like() for p in probs])
+    f.download(host)
+    return solver.split_field(host, [p.tree.nblocks() for p in probs])
+
+
+@pytest.mark.parametrize("coarse", [solver.COARSE_GS, solver.COARSE_EXACT])
+def test_forest_vcycles_equal_single_problem_vcycles(coarse):
+    probs = _problems()
+    print(f"{len(probs)} trees, coarse={coarse}", flush=True)
+    geom = solver.geometry_from_code(probs[0].geom)
+    ctx, (u, b, r, u1) = _forest(probs)
+    solver.fill_ghost_all(u, probs[0].bc)
+    for _ in range(3):
+        solver.vcycle_batched(u, b, r, u1, [p.s for p in probs], geom, probs[0].bc,
+                              [_disc(p) for p in probs], ndown=2, ntop=4, nup=2, coarse=coarse)
+    got = _download(u, probs)
+    gotb = _download(b, probs)
+    for k, p in enumerate(probs):
+        c1 = solver.Context(p.tree, p.conn, p.M)
+        v, bb, rr, vv = (c1.field(F64) for _ in range(4))
+        bb.upload(p.b)
+        solver.fill_ghost_all(v, p.bc)
+        for _ in range(3):
+            solver.vcycle(v, bb, rr, vv, p.s, p.tree, p.conn, geom, p.bc, _disc(p), ndown=2, ntop=4, nup=2,
+                          coarse=coarse)
+        ref, refb = p.b.like(), p.b.like()
+        v.download(ref)
+        bb.download(refb)
+        np.testing.assert_array_equal(_valid(got[k], p), _valid(ref.u, p))
+        np.testing.assert_array_equal(_valid(gotb[k], p), _valid(refb.u, p))
+
+
+def test_forest_solve_matches_single_solves():
+    probs = _problems()
+    geom = solver.geometry_from_code(probs[0].geom)
+    ctx, (u, b, r, u1) = _forest(probs)
+    n, hist = solver.solve_batched(u, b, r, u1, [p.s for p in probs], geom, probs[0].bc,
+                                   [_disc(p) for p in probs], tol=1e-9)
+    assert np.all(hist[:, -1] <= 1e-9)
+    got = _download(u, probs)
+    for k, p in enumerate(probs):
+        c1 = solver.Context(p.tree, p.conn, p.M)
+        v, bb, rr, vv = (c1.field(F64) for _ in range(4))
+        bb.upload(p.b)
+        n1, h1 = solver.solve(v, bb, rr, vv, p.s, geom, p.bc, _disc(p), tol=1e-9, max_cycles=n)
+        # the batch runs until its slowest tree converges: compare the first n1
+        # cycles (the per-tree norm is summed in another fixed order than a
+        # lone tree's, so the histories agree to rounding)
+        np.testing.assert_allclose(hist[k, : n1 + 1], h1, rtol=1e-12, atol=0)
+        if n1 == n:
+            ref = p.b.like()
+            v.download(ref)
+            np.testing.assert_array_equal(_valid(got[k], p), _valid(ref.u, p))
+
+
+def test_forest_mixed_precision_matches_fp64():
+    probs = _problems(n_random=3)
+    geom = solver.geometry_from_code(probs[0].geom)
+    discs = [_disc(p) for p in probs]
+    ctx, (u, b, r, u1) = _forest(probs)
+    solver.solve_batched(u, b, r, u1, [p.s for p in probs], geom, probs[0].bc, discs, tol=1e-8)
+    ref = _download(u, probs)
+    ctx2, fs = _forest(probs)
+    v, b2 = fs[0], fs[1]
+    e, be, re, ue = (ctx2.field(F32) for _ in range(4))
+    n, hist = solver.solve_mixed_batched(v, b2, e, be, re, ue, [p.s for p in probs], geom, probs[0].bc, discs,
+                                         tol=1e-8)
+    assert np.all(hist[:, -1] <= 1e-8)
+    got = _download(v, probs)
+    for k, p in enumerate(probs):
+        a, e_ = _valid(got[k], p), _valid(ref[k], p)
+        err = np.max(np.abs(a - e_)) / np.max(np.abs(e_))
+        print(f"tree {k}: mixed vs fp64 potential rel err {err:.2e}")
+        assert err <= 1e-7
+
+
+def test_single_problem_calls_on_a_forest_apply_to_every_tree():
+    probs = [configs.make_random_tree_problem(200 + i, min_cells=1e4, max_cells=3e4) for i in range(3)]
+    geom = solver.geometry_from_code(probs[0].geom)
+    ctx, (u, b, r, u1) = _forest(probs)
+    solver.fill_ghost_all(u, probs[0].bc)
+    solver.vcycle(u, b, r, u1, 1.0e-3, None, None, geom, probs[0].bc, solver.LaplacianDiscretization(2, 2),
+                  ndown=1, ntop=2, nup=1)
+    got = _download(u, probs)
+    for k, p in enumerate(probs):
+        c1 = solver.Context(p.tree, p.conn, p.M)
+        v, bb, rr, vv = (c1.field(F64) for _ in range(4))
+        bb.upload(p.b)
+        solver.fill_ghost_all(v, p.bc)
+        solver.vcycle(v, bb, rr, vv, 1.0e-3, None, None, geom, p.bc, solver.LaplacianDiscretization(2, 2),
+                      ndown=1, ntop=2, nup=1)
+        ref = p.b.like()
+        v.download(ref)
+        np.testing.assert_array_equal(_valid(got[k], p), _valid(ref.u, p))
