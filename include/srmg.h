@@ -159,7 +159,18 @@ typedef struct {
     int32_t max_cycles;
     int32_t use_graph;         /* capture one cycle in a hipGraph and replay it */
     int32_t inner_cycles;      /* mixed precision: FP32 V-cycles per FP64 correction (0 means 1) */
+    int32_t guess;             /* first iterate: SRMG_GUESS_* */
 } srmg_solve_opts;
+
+/* First iterate of a solve. NONE: u as given. LEARNED: the network attached
+ * with srmg_set_guess predicts the level-1 solution from the level-1 right-hand
+ * side; it is interpolated to every finer level. COARSE: the same with the
+ * exact level-1 solve in place of the network. A learned or coarse first
+ * iterate replaces u; a tree whose first-iterate residual is not finite or
+ * exceeds resnorm(u = 0) starts from u = 0 instead (counted by
+ * srmg_guess_fallbacks). No counterpart in the reference (its solves start
+ * from the previous u). */
+enum { SRMG_GUESS_NONE = 0, SRMG_GUESS_LEARNED = 1, SRMG_GUESS_COARSE = 2 };
 
 /* Solve from the current u (ghosts pre-filled here) with plain V-cycles of
  * the fields' own type until the relative residual is <= tol. hist (may be
@@ -183,6 +194,49 @@ int srmg_solve_mixed(srmg_ctx *ctx, int u, int b, int e, int be, int re, int ue,
 /* srmg_solve_mixed on every tree of a forest; hist as in srmg_solve_batched. */
 int srmg_solve_mixed_batched(srmg_ctx *ctx, int u, int b, int e, int be, int re, int ue, const double *s,
                              const srmg_op *ops, const srmg_solve_opts *opts, int *cycles, double *hist);
+
+/* ---- learned first iterate ---- */
+
+/* Attach (net != NULL) or detach the network used by SRMG_GUESS_LEARNED. The
+ * context does not own it; it must outlive its use. */
+typedef struct srmg_unet srmg_unet;
+int srmg_set_guess(srmg_ctx *ctx, srmg_unet *net);
+/* Trees of the last solve whose learned/coarse first iterate was rejected. */
+int srmg_guess_fallbacks(srmg_ctx *ctx, int *n);
+
+/* Exact solve of L_1 u = f on ngrid stacked uniform level-1 grids (x fastest,
+ * n[0] x n[1] (x n[2]) cells, the level-1 stencil of op with scale s: the
+ * Helmholtz shift is -s k2), FP64 device pointers. Training targets for the
+ * network; f = -s b for the problem of srmg_solve. */
+int srmg_grid_solve(srmg_ctx *ctx, int D, const int *n, double s, const srmg_op *op, const double *f, double *u,
+                    int ngrid);
+
+/* U-Net of 3^D convolutions on a D-dimensional grid n[0] x n[1] (x n[2]):
+ * `levels` resolution levels with widths[l] channels (powers of two in 8..64;
+ * extents divisible by 2^(levels-1)), BF16 matrix-core kernels with FP32
+ * accumulation, FP32 master parameters initialized from `seed`. */
+int srmg_unet_create(srmg_unet **net, int D, const int *n, int levels, const int *widths, uint64_t seed);
+int srmg_unet_destroy(srmg_unet *net);
+int srmg_unet_param_count(const srmg_unet *net, int64_t *count);
+/* FP32 parameters: per convolution W[cout][tap][cin] then b[cout], taps
+ * t = (dx+1) + 3 (dy+1) + 9 (dz+1); convolutions in the order encoder levels
+ * 0..L-1 (two each), decoder levels L-2..0 (coarse linear, then ReLU), output. */
+int srmg_unet_get_params(srmg_unet *net, float *host);
+int srmg_unet_set_params(srmg_unet *net, const float *host);  /* also resets the optimizer */
+/* Output scale c_out: the network predicts u / (max|f| c_out). */
+int srmg_unet_set_output_scale(srmg_unet *net, double c_out);
+int srmg_unet_get_output_scale(const srmg_unet *net, double *c_out);
+/* The level-1 operator the network is trained for (geometry, disc, k2, bc);
+ * a learned first iterate is refused for any other. */
+int srmg_unet_set_problem(srmg_unet *net, const srmg_op *op);
+/* u = prediction for `batch` stacked grids f (FP32 device pointers). */
+int srmg_unet_forward(srmg_unet *net, const float *f, float *u, int batch);
+/* Mean squared error of the normalized prediction against the targets u and
+ * its gradient (host, param_count entries; either output may be NULL). */
+int srmg_unet_loss_grad(srmg_unet *net, const float *f, const float *u, int batch, double *loss, float *grad);
+/* One Adam step (beta 0.9, 0.999) on the batch; loss (may be NULL) before the step. */
+int srmg_unet_train_step(srmg_unet *net, const float *f, const float *u, int batch, float lr, double *loss);
+int srmg_unet_synchronize(srmg_unet *net);
 
 /* ---- measurement ---- */
 

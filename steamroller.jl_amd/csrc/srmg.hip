@@ -17,8 +17,10 @@
 #include <vector>
 
 #include "../../include/srmg.h"
+#include "common.h"
 #include "kernels.h"
 #include "layout.h"
+#include "unet.h"
 
 using namespace srmg;
 
@@ -31,77 +33,8 @@ constexpr int SUM_CHUNKS = 256;
 constexpr int TSUM = SUM_CHUNKS + 4;
 
 // ---------------------------------------------------------------------------
-// errors
-// ---------------------------------------------------------------------------
-
-static thread_local std::string g_err;
-
-struct Failure {
-    int code;
-    std::string msg;
-};
-
-[[noreturn]] static void fail(const std::string &m, int code = 1) { throw Failure{code, m}; }
-
-#define HIPCHK(expr)                                                                      \
-    do {                                                                                  \
-        hipError_t e_ = (expr);                                                           \
-        if (e_ != hipSuccess)                                                             \
-            fail(std::string(#expr) + ": " + hipGetErrorString(e_), 2);                   \
-    } while (0)
-
-template <class F>
-static int guarded(F &&f) {
-    try {
-        f();
-        return 0;
-    } catch (const Failure &e) {
-        g_err = e.msg;
-        return e.code;
-    } catch (const std::exception &e) {
-        g_err = e.what();
-        return 3;
-    }
-}
-
-// ---------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------
-
-struct DevBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf &) = delete;
-    DevBuf &operator=(const DevBuf &) = delete;
-    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) {
-        o.p = nullptr;
-        o.bytes = 0;
-    }
-    DevBuf &operator=(DevBuf &&o) noexcept {
-        if (this != &o) {
-            release();
-            p = o.p;
-            bytes = o.bytes;
-            o.p = nullptr;
-            o.bytes = 0;
-        }
-        return *this;
-    }
-    ~DevBuf() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    void alloc(size_t n) {
-        release();
-        if (n == 0) n = 16;
-        HIPCHK(hipMalloc(&p, n));
-        bytes = n;
-    }
-    template <class T> T *as() const { return static_cast<T *>(p); }
-};
 
 struct FieldBuf {
     int dtype = SRMG_F64;
@@ -169,6 +102,8 @@ struct srmg_ctx {
     DevBuf d_grid, d_grid2; // stacked level-1 uniform grids (coarse solve)
     std::vector<std::unique_ptr<FieldBuf>> fields;
     std::unique_ptr<FieldBuf> zero[2];  // all-zero scratch fields (F64, F32), made on demand
+    srmg_unet *guess_net = nullptr;     // network of the learned first iterate (not owned)
+    int guess_fallbacks = 0;            // trees of the last solve that started from zero instead
     hipStream_t stream = nullptr;
     int maxpartials = 0;
     std::vector<int> sweep_cfg;  // per (dtype, level): tile shape of the fused sweep (see Ops::sweep)
@@ -643,6 +578,8 @@ static inline unsigned nwg(int64_t threads) { return (unsigned)((threads + WG - 
 // kernel launchers, dispatched on (D, M, T)
 // ---------------------------------------------------------------------------
 
+static std::vector<double> read_norms(srmg_ctx *c);
+
 template <int D, int M, typename T>
 struct Ops {
     using LY = Lay<D, M>;
@@ -902,6 +839,9 @@ struct Ops {
         settle(u);
     }
 
+    // first iterate of a solve from the level-1 problem (SRMG_GUESS_*)
+    void first_iterate(FieldBuf &u, FieldBuf &b, int mode, const std::vector<double> &r0);
+
     // fmg! (src/multigrid.jl:17-51)
     void fmg(FieldBuf &u, FieldBuf &b, FieldBuf &r, FieldBuf &u1, double s, const srmg_op *op, int ndown, int ntop,
              int nup, double omega) {
@@ -929,6 +869,86 @@ struct Ops {
         }
     }
 };
+
+// ---------------------------------------------------------------------------
+// first iterate from the level-1 problem
+// ---------------------------------------------------------------------------
+
+// The charge is averaged down to level 1 (b copied into u, then restricted
+// level by level), the level-1 solution of L_1 u = -s b is predicted by the
+// network (LEARNED) or solved exactly (COARSE), interpolated to every finer
+// level and restricted back so covered cells are the mean of their children.
+// Trees whose residual is then not finite or larger than with u = 0 restart
+// from zero.
+template <int D, int M, typename T>
+void Ops<D, M, T>::first_iterate(FieldBuf &u, FieldBuf &b, int mode, const std::vector<double> &r0) {
+    const double s = c->ps[0];
+    const srmg_op *op = &c->pops[0];
+    const int nt = (int)c->trees.size();
+    if (mode == SRMG_GUESS_LEARNED) {
+        if (!c->guess_net) fail("learned first iterate requested but no network is attached");
+        int nd = 0, ne[3];
+        srmg_guess::extents(c->guess_net, &nd, ne);
+        for (const CoarseGroup &G : c->groups)
+            if (nd != D || ne[0] != G.N[0] || ne[1] != G.N[1] || (D == 3 && ne[2] != G.N[2]))
+                fail("the attached network was built for another level-1 grid");
+        for (int t = 0; t < nt; t++) {
+            const srmg_op &o = c->pops[t];
+            if (!srmg_guess::matches(c->guess_net, o.geometry, o.disc, o.disc ? o.k2 : 0.0, o.bc))
+                fail("the attached network was trained for another level-1 operator");
+        }
+    }
+    settle(u);
+    const size_t nall = (size_t)c->nb * LY::V;
+    hipLaunchKernelGGL((k_copy<T>), dim3(nwg(nall)), dim3(WG), 0, c->stream, u.data.as<T>(), X(b, 0), nall);
+    for (int l = c->nlevels - 1; l >= 1; l--) restrict_(u, l);
+    // levels >= 2 and the ghost slabs start at zero
+    const size_t o1 = (size_t)c->lv[0].nblk * LY::V, ni = u.ib.bytes / sizeof(T);
+    if (nall > o1)
+        hipLaunchKernelGGL(k_fill<T>, dim3(nwg(nall - o1)), dim3(WG), 0, c->stream, u.data.as<T>() + o1, (T)0,
+                           nall - o1);
+    if (ni) hipLaunchKernelGGL(k_fill<T>, dim3(nwg(ni)), dim3(WG), 0, c->stream, u.ib.as<T>(), (T)0, ni);
+    if (mode == SRMG_GUESS_COARSE) {
+        srmg_op oe = *op;
+        oe.coarse = SRMG_COARSE_EXACT;
+        coarse_exact(u, u, s, &oe);
+    } else {
+        for (CoarseGroup &G : c->groups) {
+            const size_t total = (size_t)G.N[0] * G.N[1] * (D == 3 ? G.N[2] : 1);
+            const int nblk = G.ntree * G.nper;
+            double *g = c->d_grid.as<double>(), *g2 = c->d_grid2.as<double>();
+            hipLaunchKernelGGL((k_grid_gather<D, M, T>), dim3(nwg((int64_t)nblk * LY::V)), dim3(WG), 0, c->stream,
+                               g, X(u, 0), coord(), G.l1_blk0, nblk, G.N[0], G.N[1], G.nper, total,
+                               G.cs.as<double>());
+            srmg_guess::infer(c->guess_net, g, g2, G.ntree, c->stream);
+            hipLaunchKernelGGL((k_grid_scatter<D, M, T>), dim3(nwg((int64_t)nblk * LY::V)), dim3(WG), 0,
+                               c->stream, X(u, 0), g2, coord(), G.l1_blk0, nblk, G.N[0], G.N[1], G.nper, total);
+        }
+    }
+    for (int l = 1; l < c->nlevels; l++) {
+        LevelArgs<T> ac = level_args<T>(c, l - 1, s, op, 1.0);
+        interp_add(u, u, l, ac);
+        fill_interp(u, l, ac);
+    }
+    for (int l = c->nlevels - 1; l >= 1; l--) restrict_(u, l);
+    for (int l = 1; l < c->nlevels; l++) fill_interp(u, l, level_args<T>(c, l - 1, s, op, 1.0));
+    resnorm_partials(u, b, s, op);
+    std::vector<double> rg = read_norms(c);
+    c->guess_fallbacks = 0;
+    for (int t = 0; t < nt; t++) {
+        if (std::isfinite(rg[t]) && rg[t] <= r0[t]) continue;
+        c->guess_fallbacks++;
+        const TreeInfo &ti = c->trees[t];
+        for (int l = 0; l < c->nlevels; l++) {
+            size_t n = (size_t)ti.nblk[l] * LY::V;
+            if (n)
+                hipLaunchKernelGGL(k_fill<T>, dim3(nwg(n)), dim3(WG), 0, c->stream,
+                                   u.data.as<T>() + (size_t)ti.blk0[l] * LY::V, (T)0, n);
+        }
+    }
+    if (c->guess_fallbacks)
+        for (int l = 1; l < c->nlevels; l++) fill_interp(u, l, level_args<T>(c, l - 1, s, op, 1.0));
+}
 
 // ---------------------------------------------------------------------------
 // exact coarsest-level solve
@@ -1255,6 +1275,7 @@ static std::vector<double> read_norms(srmg_ctx *c) {
 static void check_opts(const srmg_solve_opts *o) {
     if (!o) fail("null options");
     if (o->max_cycles < 0 || o->ndown < 0 || o->nup < 0 || o->ntop < 0) fail("bad solve options");
+    if (o->guess < SRMG_GUESS_NONE || o->guess > SRMG_GUESS_COARSE) fail("bad first-iterate mode");
 }
 
 // relative residuals; true when every tree is at or below tol
@@ -1288,6 +1309,7 @@ static void solve_plain(srmg_ctx *c, int ui, int bi, int ri, int u1i, const srmg
         for (int l = 1; l < c->nlevels; l++) ops.fill_interp(u, l, level_args<T>(c, l - 1, s, op, 1.0));
         ops.resnorm_partials(zero_field(c, t), b, s, op);
         std::vector<double> r0 = safe_r0(read_norms(c));
+        if (o->guess) ops.first_iterate(u, b, o->guess, r0);
         ops.resnorm_partials(u, b, s, op);
         bool done = record(read_norms(c), r0, o->tol, 0, o->max_cycles, hist);
         int k = 0;
@@ -1319,6 +1341,7 @@ static void solve_mixed_dm(srmg_ctx *c, FieldBuf &u, FieldBuf &b, FieldBuf &e, F
     for (int l = 1; l < c->nlevels; l++) o64.fill_interp(u, l, level_args<double>(c, l - 1, s, op, 1.0));
     o64.resnorm_partials(zero_field(c, SRMG_F64), b, s, op);
     std::vector<double> r0 = safe_r0(read_norms(c));
+    if (o->guess) o64.first_iterate(u, b, o->guess, r0);
     auto defect_and_norm = [&] { o64.defect_and_norm(be.data.as<float>(), u, b, s, op); };
     auto correct = [&] {
         size_t n = (size_t)c->nb * LY::V, ni = e.ib.bytes / sizeof(float);
@@ -1793,6 +1816,88 @@ int srmg_level_info(srmg_ctx *c, int level, int64_t *nblocks, int64_t *nleaf, in
         if (nblocks) *nblocks = L.nblk;
         if (nleaf) *nleaf = L.nleaf;
         if (nslots) *nslots = L.nslot;
+    });
+}
+
+int srmg_set_guess(srmg_ctx *c, srmg_unet *net) {
+    return guarded([&] {
+        if (!c) fail("null context");
+        c->guess_net = net;
+    });
+}
+
+int srmg_guess_fallbacks(srmg_ctx *c, int *n) {
+    return guarded([&] {
+        if (!c || !n) fail("null argument");
+        *n = c->guess_fallbacks;
+    });
+}
+
+int srmg_grid_solve(srmg_ctx *c, int D, const int *n, double s, const srmg_op *op, const double *f, double *u,
+                    int ngrid) {
+    return guarded([&] {
+        if (!c || !n || !op || !f || !u) fail("null argument");
+        if (D != 2 && D != 3) fail("D must be 2 or 3");
+        if (ngrid < 1) fail("ngrid must be >= 1");
+        int N[3] = {n[0], n[1], D == 3 ? n[2] : 1};
+        for (int d = 0; d < D; d++)
+            if (N[d] < 1) fail("bad grid extents");
+        if (op->geometry != SRMG_CARTESIAN && D != 2) fail("cylindrical geometry is two-dimensional");
+        const int rdim = (op->geometry == SRMG_CYL1) ? 0 : (op->geometry == SRMG_CYL2 ? 1 : -1);
+        DevBuf P[3], Q[3], lam[3];
+        for (int d = 0; d < D; d++) {
+            int il = 0, ih = 0, m = 1;
+            for (int e = 0; e < D; e++) {
+                il += ((e == d ? -1 : 0) + 1) * m;
+                ih += ((e == d ? 1 : 0) + 1) * m;
+                m *= 3;
+            }
+            const int nd = N[d];
+            std::vector<double> hl(nd), hP((size_t)nd * nd), hQ((size_t)nd * nd);
+            axis_decomp(nd, d == rdim, (double)op->bc[il], (double)op->bc[ih], hl.data(), hP.data(), hQ.data());
+            lam[d].alloc(nd * sizeof(double));
+            P[d].alloc((size_t)nd * nd * sizeof(double));
+            Q[d].alloc((size_t)nd * nd * sizeof(double));
+            HIPCHK(hipMemcpy(lam[d].p, hl.data(), nd * sizeof(double), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(P[d].p, hP.data(), (size_t)nd * nd * sizeof(double), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(Q[d].p, hQ.data(), (size_t)nd * nd * sizeof(double), hipMemcpyHostToDevice));
+        }
+        const size_t total = (size_t)N[0] * N[1] * N[2];
+        std::vector<double> hs(ngrid, op->disc == SRMG_HELMHOLTZ ? -s * op->k2 : 0.0);
+        DevBuf shift, g1, g2;
+        shift.alloc(ngrid * sizeof(double));
+        g1.alloc(total * ngrid * sizeof(double));
+        g2.alloc(total * ngrid * sizeof(double));
+        HIPCHK(hipMemcpy(shift.p, hs.data(), ngrid * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        const double *in = f;
+        double *a = g1.as<double>(), *b2 = g2.as<double>();
+        auto axis = [&](const DevBuf &A, int d, int stride, const double *x, double *y) {
+            dim3 grid((unsigned)((ngrid * total / N[d] + 63) / 64), (unsigned)((N[d] + 63) / 64));
+            hipLaunchKernelGGL(k_axis_gemm, grid, dim3(256), 0, c->stream, A.as<double>(), N[d], stride,
+                               (int)(ngrid * total / N[d]), x, y);
+        };
+        int stride = 1;
+        for (int d = 0; d < D; d++) {
+            axis(Q[d], d, stride, in, a);
+            in = a;
+            std::swap(a, b2);
+            stride *= N[d];
+        }
+        double *cur = const_cast<double *>(in);
+        hipLaunchKernelGGL(k_eig_scale, dim3(nwg(total * ngrid)), dim3(WG), 0, c->stream, cur, lam[0].as<double>(),
+                           lam[1].as<double>(), D == 3 ? lam[2].as<double>() : lam[1].as<double>(), N[0], N[1], D,
+                           shift.as<double>(), total, ngrid);
+        stride = 1;
+        for (int d = 0; d < D; d++) {
+            double *dst = (d == D - 1) ? u : a;
+            axis(P[d], d, stride, cur, dst);
+            cur = dst;
+            std::swap(a, b2);
+            stride *= N[d];
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
     });
 }
 

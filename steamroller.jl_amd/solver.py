@@ -29,6 +29,7 @@ LIB_PATH = os.path.join(_HERE, "libsrmg.so")
 F64, F32 = 0, 1
 KERNEL_GS_PASS, KERNEL_RESIDUAL, KERNEL_GS_SWEEP, KERNEL_STREAM, KERNEL_RESIDUAL_GATHER = 0, 1, 2, 3, 4
 COARSE_GS, COARSE_EXACT = 0, 1
+GUESS_NONE, GUESS_LEARNED, GUESS_COARSE = 0, 1, 2
 _DT = {F64: np.float64, F32: np.float32}
 
 
@@ -81,7 +82,7 @@ class _Op(C.Structure):
 class _SolveOpts(C.Structure):
     _fields_ = [("ndown", C.c_int32), ("ntop", C.c_int32), ("nup", C.c_int32),
                 ("omega", C.c_double), ("tol", C.c_double), ("max_cycles", C.c_int32),
-                ("use_graph", C.c_int32), ("inner_cycles", C.c_int32)]
+                ("use_graph", C.c_int32), ("inner_cycles", C.c_int32), ("guess", C.c_int32)]
 
 
 def make_op(geometry, bc, disc, coarse: int = COARSE_GS) -> _Op:
@@ -140,6 +141,21 @@ def lib():
         "srmg_level_info": [P, i32, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64)],
         "srmg_autotune": [P, i32, i32, dbl, P, i32, P],
         "srmg_set_sweep_config": [P, i32, i32, i32],
+        "srmg_set_guess": [P, P],
+        "srmg_guess_fallbacks": [P, C.POINTER(i32)],
+        "srmg_grid_solve": [P, i32, P, dbl, P, P, P, i32],
+        "srmg_unet_create": [C.POINTER(P), i32, P, i32, P, C.c_uint64],
+        "srmg_unet_destroy": [P],
+        "srmg_unet_param_count": [P, C.POINTER(i64)],
+        "srmg_unet_get_params": [P, P],
+        "srmg_unet_set_params": [P, P],
+        "srmg_unet_set_output_scale": [P, dbl],
+        "srmg_unet_get_output_scale": [P, C.POINTER(dbl)],
+        "srmg_unet_set_problem": [P, P],
+        "srmg_unet_forward": [P, P, P, i32],
+        "srmg_unet_loss_grad": [P, P, P, i32, P, P],
+        "srmg_unet_train_step": [P, P, P, i32, C.c_float, P],
+        "srmg_unet_synchronize": [P],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -357,20 +373,21 @@ def resnorm(r, b, u, s, tree=None, conn=None, geometry=None, bc=None, lpldisc=No
     return out.value
 
 
-def _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph, inner_cycles=1):
+def _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph, inner_cycles=1, guess=0):
     o = _SolveOpts()
     o.ndown, o.ntop, o.nup = int(ndown), int(ntop), int(nup)
     o.omega, o.tol, o.max_cycles, o.use_graph = float(omega), float(tol), int(max_cycles), int(use_graph)
     o.inner_cycles = int(inner_cycles)
+    o.guess = int(guess)
     return o
 
 
 def solve(u, b, r, u1, s, geometry, bc, lpldisc, *, tol=1e-8, max_cycles=50, ndown=2, ntop=4,
-          nup=2, omega=1.0, coarse=COARSE_EXACT, use_graph=True):
+          nup=2, omega=1.0, coarse=COARSE_EXACT, use_graph=True, guess=GUESS_NONE):
     """Pre-fill ghosts, then V-cycles until resnorm(u) <= tol * resnorm(0).
     Returns (cycles, relative residual history)."""
     op = make_op(geometry, bc, lpldisc, coarse)
-    o = _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph)
+    o = _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph, 1, guess)
     hist = np.zeros(max_cycles + 1)
     n = C.c_int()
     _check(lib().srmg_solve(u.ctx.h, u.id, b.id, r.id, u1.id, float(s), C.byref(op), C.byref(o),
@@ -379,11 +396,12 @@ def solve(u, b, r, u1, s, geometry, bc, lpldisc, *, tol=1e-8, max_cycles=50, ndo
 
 
 def solve_mixed(u, b, e, be, re, ue, s, geometry, bc, lpldisc, *, tol=1e-8, max_cycles=50,
-                ndown=2, ntop=4, nup=2, omega=1.0, coarse=COARSE_EXACT, use_graph=True, inner_cycles=1):
+                ndown=2, ntop=4, nup=2, omega=1.0, coarse=COARSE_EXACT, use_graph=True, inner_cycles=1,
+                guess=GUESS_NONE):
     """FP64 defect correction around FP32 V-cycles (u, b FP64; e, be, re, ue FP32).
     Returns (outer steps, relative residual history per outer step)."""
     op = make_op(geometry, bc, lpldisc, coarse)
-    o = _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph, inner_cycles)
+    o = _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph, inner_cycles, guess)
     hist = np.zeros(max_cycles + 1)
     n = C.c_int()
     _check(lib().srmg_solve_mixed(u.ctx.h, u.id, b.id, e.id, be.id, re.id, ue.id, float(s),
@@ -433,11 +451,11 @@ def resnorm_batched(r, b, u, s, geometry, bc, discs, *, coarse=COARSE_GS):
 
 
 def solve_batched(u, b, r, u1, s, geometry, bc, discs, *, tol=1e-8, max_cycles=50, ndown=2, ntop=4, nup=2,
-                  omega=1.0, coarse=COARSE_EXACT, use_graph=True):
+                  omega=1.0, coarse=COARSE_EXACT, use_graph=True, guess=GUESS_NONE):
     """Solve every tree of a forest to tol; returns (cycles, history[ntrees, cycles + 1])."""
     sv = np.ascontiguousarray(np.asarray(s, np.float64))
     ops = _ops(geometry, bc, discs, coarse)
-    o = _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph)
+    o = _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph, 1, guess)
     hist = np.zeros((len(discs), max_cycles + 1))
     n = C.c_int()
     _check(lib().srmg_solve_batched(u.ctx.h, u.id, b.id, r.id, u1.id, _p(sv), ops, C.byref(o), C.byref(n),
@@ -446,13 +464,118 @@ def solve_batched(u, b, r, u1, s, geometry, bc, discs, *, tol=1e-8, max_cycles=5
 
 
 def solve_mixed_batched(u, b, e, be, re, ue, s, geometry, bc, discs, *, tol=1e-8, max_cycles=50, ndown=2,
-                        ntop=4, nup=2, omega=1.0, coarse=COARSE_EXACT, use_graph=True, inner_cycles=1):
+                        ntop=4, nup=2, omega=1.0, coarse=COARSE_EXACT, use_graph=True, inner_cycles=1,
+                        guess=GUESS_NONE):
     """FP64 defect correction around FP32 V-cycles on every tree of a forest."""
     sv = np.ascontiguousarray(np.asarray(s, np.float64))
     ops = _ops(geometry, bc, discs, coarse)
-    o = _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph, inner_cycles)
+    o = _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph, inner_cycles, guess)
     hist = np.zeros((len(discs), max_cycles + 1))
     n = C.c_int()
     _check(lib().srmg_solve_mixed_batched(u.ctx.h, u.id, b.id, e.id, be.id, re.id, ue.id, _p(sv), ops,
                                           C.byref(o), C.byref(n), _p(hist)))
     return n.value, hist[:, : n.value + 1].copy()
+
+
+# ---- learned first iterate ----
+
+def _ptr(x) -> int:
+    """Device address of a torch tensor or an integer address."""
+    return int(x.data_ptr()) if hasattr(x, "data_ptr") else int(x)
+
+
+def grid_solve(ctx: Context, D: int, n, s: float, geometry, bc, disc, f, u, ngrid: int):
+    """Exact FP64 solve of L_1 u = f on ngrid stacked uniform level-1 grids
+    (device buffers, x fastest)."""
+    op = make_op(geometry, bc, disc, COARSE_EXACT)
+    nn = (C.c_int * 3)(*(list(n) + [1] * (3 - len(n))))
+    _check(lib().srmg_grid_solve(ctx.h, int(D), nn, float(s), C.byref(op), C.c_void_p(_ptr(f)),
+                                 C.c_void_p(_ptr(u)), int(ngrid)))
+
+
+def set_guess(ctx: Context, net: Optional["UNet"]):
+    _check(lib().srmg_set_guess(ctx.h, None if net is None else net.h))
+    ctx._guess = net  # keep the network alive while attached
+
+
+def guess_fallbacks(ctx: Context) -> int:
+    n = C.c_int()
+    _check(lib().srmg_guess_fallbacks(ctx.h, C.byref(n)))
+    return n.value
+
+
+class UNet:
+    """The network of the learned first iterate (srmg_unet_* in the C ABI).
+
+    Data arguments are device buffers (torch tensors or addresses) of FP32
+    stacked grids, x fastest."""
+
+    def __init__(self, D: int, n, levels: int = 4, widths=(8, 16, 32, 64), seed: int = 0):
+        self.D, self.n, self.levels, self.widths = int(D), tuple(int(x) for x in n), int(levels), tuple(widths)
+        h = C.c_void_p()
+        nn = (C.c_int * 3)(*(list(self.n) + [1] * (3 - len(self.n))))
+        ww = (C.c_int * len(self.widths))(*self.widths)
+        _check(lib().srmg_unet_create(C.byref(h), self.D, nn, self.levels, ww, C.c_uint64(seed)))
+        self.h = h
+
+    @property
+    def nparams(self) -> int:
+        n = C.c_int64()
+        _check(lib().srmg_unet_param_count(self.h, C.byref(n)))
+        return n.value
+
+    def get_params(self) -> np.ndarray:
+        out = np.zeros(self.nparams, np.float32)
+        _check(lib().srmg_unet_get_params(self.h, _p(out)))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(np.asarray(p, np.float32))
+        if p.size != self.nparams:
+            raise SrmgError(f"expected {self.nparams} parameters, got {p.size}")
+        _check(lib().srmg_unet_set_params(self.h, _p(p)))
+
+    @property
+    def output_scale(self) -> float:
+        c = C.c_double()
+        _check(lib().srmg_unet_get_output_scale(self.h, C.byref(c)))
+        return c.value
+
+    @output_scale.setter
+    def output_scale(self, c: float):
+        _check(lib().srmg_unet_set_output_scale(self.h, float(c)))
+
+    def set_problem(self, geometry, bc, disc):
+        op = make_op(geometry, bc, disc, COARSE_EXACT)
+        _check(lib().srmg_unet_set_problem(self.h, C.byref(op)))
+        self.problem = (int(op.geometry), np.asarray(bc, np.int8).ravel().copy(), int(op.disc), float(op.k2))
+
+    def forward(self, f, u, batch: int):
+        _check(lib().srmg_unet_forward(self.h, C.c_void_p(_ptr(f)), C.c_void_p(_ptr(u)), int(batch)))
+
+    def loss_grad(self, f, u, batch: int):
+        loss = C.c_double()
+        g = np.zeros(self.nparams, np.float32)
+        _check(lib().srmg_unet_loss_grad(self.h, C.c_void_p(_ptr(f)), C.c_void_p(_ptr(u)), int(batch),
+                                         C.byref(loss), _p(g)))
+        return loss.value, g
+
+    def train_step(self, f, u, batch: int, lr: float, want_loss: bool = False):
+        loss = C.c_double()
+        _check(lib().srmg_unet_train_step(self.h, C.c_void_p(_ptr(f)), C.c_void_p(_ptr(u)), int(batch),
+                                          C.c_float(lr), C.byref(loss) if want_loss else None))
+        return loss.value if want_loss else None
+
+    def synchronize(self):
+        _check(lib().srmg_unet_synchronize(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().srmg_unet_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,189 @@
+"""The learned first iterate on the device: BF16 matrix-core network kernels
+against the CPU restatement (oracle/unet_ref.py), the exact level-1 grid solve
+against a sparse direct solve, first iterates inside the solvers (gain over
+the zero iterate, fallback to zero on a bad first iterate).
+
+Tolerances: the network runs in BF16 (8-bit mantissa) with FP32 accumulation
+through up to 15 convolutions, so its outputs are compared with the FP32/FP64
+restatement at a few per cent in the max norm (stated per test); the FP64
+grid solve at 1e-10 relative."""
+
+import os
+import sys
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+from srmg import configs, solver
+from srmg.solver import F32, F64
+
+sys.path.insert(0, os.path.join(os.path.dirname(__file__), "..", "oracle"))
+sys.path.insert(0, os.path.dirname(__file__))
+import unet_ref as U  # noqa: E402
+from assemble import grid_operator  # noqa: E402
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+CASES = [(2, (32, 16), (8, 16, 32)), (3, (16, 16, 8), (8, 16, 32)), (3, (16, 16, 16), (8, 16, 32, 64))]
+
+
+def _dev(a):
+    return torch.tensor(np.ascontiguousarray(a), dtype=torch.float32, device="cuda")
+
+
+def _grid_shape(n):
+    return tuple(reversed(n))  # numpy C order: x (n[0]) fastest
+
+
+@pytest.mark.parametrize("D,n,widths", CASES)
+def test_forward_matches_cpu_restatement(D, n, widths):
+    print(f"forward D={D} n={n} widths={widths}", flush=True)
+    net = solver.UNet(D, n, len(widths), widths, seed=5)
+    rng = np.random.default_rng(7)
+    p = U.he_params(D, widths, rng, np.float32)
+    net.set_params(p)
+    net.output_scale = 3.0
+    B = 3
+    f = U.smooth_field(rng, (B,) + _grid_shape(n))
+    fd = _dev(f)
+    ud = torch.zeros_like(fd)
+    net.forward(fd, ud, B)
+    got = ud.cpu().numpy().astype(np.float64)
+    y, ca = U.forward(p.astype(np.float64), f, D, widths)
+    ref = y * ca["scale"].reshape((B,) + (1,) * D) * 3.0
+    err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+    print(f"  max-norm relative error {err:.3e}")
+    assert err < 3e-2
+
+
+@pytest.mark.parametrize("D,n,widths", CASES)
+def test_loss_and_gradient_match_cpu_restatement(D, n, widths):
+    print(f"gradient D={D} n={n} widths={widths}", flush=True)
+    net = solver.UNet(D, n, len(widths), widths, seed=5)
+    rng = np.random.default_rng(11)
+    p = U.he_params(D, widths, rng, np.float32)
+    net.set_params(p)
+    net.output_scale = 20.0
+    B = 2
+    f = U.smooth_field(rng, (B,) + _grid_shape(n))
+    u = U.smooth_field(rng, (B,) + _grid_shape(n)) * 30.0
+    loss, g = net.loss_grad(_dev(f), _dev(u), B)
+    rloss, rg = U.loss_and_grad(p.astype(np.float64), f, u, 20.0, D, widths)
+    print(f"  loss {loss:.6e} vs {rloss:.6e}")
+    assert abs(loss - rloss) <= 2e-2 * rloss
+    o = 0
+    ntap = 3 ** D
+    for k, (_, ci, co, _) in enumerate(U.layer_specs(widths)):
+        m = co * ntap * ci + co
+        a, b = g[o:o + m].astype(np.float64), rg[o:o + m]
+        rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+        print(f"  conv {k:2d} ({ci:2d}->{co:2d}) gradient relative error {rel:.3e}")
+        assert rel < 6e-2, k
+        o += m
+
+
+def test_training_reduces_the_loss():
+    D, n, widths = 2, (32, 16), (8, 16, 32)
+    net = solver.UNet(D, n, 3, widths, seed=1)
+    rng = np.random.default_rng(0)
+    B = 8
+    f = U.smooth_field(rng, (B,) + _grid_shape(n))
+    u = np.cumsum(f, axis=-1) * 0.5 + 3.0 * f   # some smooth target map
+    net.output_scale = float(np.max(np.abs(u)) / np.max(np.abs(f)))
+    fd, ud = _dev(f), _dev(u)
+    first = net.train_step(fd, ud, B, 2e-3, want_loss=True)
+    last = first
+    for _ in range(150):
+        last = net.train_step(fd, ud, B, 2e-3, want_loss=True)
+    print(f"loss {first:.4e} -> {last:.4e}")
+    assert last < 0.2 * first
+
+
+@pytest.mark.parametrize("D,n,geom,bcdesc,helm", [
+    (3, (16, 8, 8), solver.CartesianGeometry(), ((1, 1), (1, 1), (-1, -1)), 0.0),
+    (2, (16, 32), solver.CylindricalGeometry(1), ((1, -1), (-1, -1)), 0.0),
+    (2, (16, 32), solver.CylindricalGeometry(1), ((1, 1), (-1, -1)), 3.0e5),
+])
+def test_grid_solve_matches_sparse_direct_solve(D, n, geom, bcdesc, helm):
+    from srmg import tree as T
+    bc = T.boundaryconditions(bcdesc)
+    p = configs.uniform_problem(D, tuple(x // 8 for x in n), 8, 1, geom.code, bcdesc)
+    ctx = solver.Context(p.tree, p.conn, p.M)
+    rng = np.random.default_rng(4)
+    ng = 3
+    f = rng.normal(size=(ng, int(np.prod(n))))
+    s = 2.5e-6
+    disc = solver.HelmholtzDiscretization(D, 2, helm) if helm else solver.LaplacianDiscretization(D, 2)
+    fd = torch.tensor(f, dtype=torch.float64, device="cuda")
+    ud = torch.zeros_like(fd)
+    solver.grid_solve(ctx, D, n, s, geom, bc, disc, fd, ud, ng)
+    torch.cuda.synchronize()
+    A = grid_operator(n, geom.code, bc, -s * helm if helm else 0.0)
+    for g in range(ng):
+        ref = spla.spsolve(A, f[g])
+        got = ud[g].cpu().numpy()
+        assert np.max(np.abs(got - ref)) <= 1e-10 * np.max(np.abs(ref))
+
+
+def _small_cfg4_like(levels=3):
+    """A 3D Cartesian tree with the cfg4 level-1 grid (64^3 in 8^3 blocks) and
+    two refined levels around seeded filaments."""
+    return configs.make_cfg4(0, scale=0.35, levels=levels)
+
+
+def _solve(p, mode, net=None, mixed=False, fallback_scale=None):
+    ctx = solver.Context(p.tree, p.conn, p.M)
+    geom = solver.geometry_from_code(p.geom)
+    disc = solver.LaplacianDiscretization(p.D, 2)
+    if net is not None:
+        solver.set_guess(ctx, net)
+    u, b = ctx.field(F64), ctx.field(F64)
+    b.upload(p.b)
+    if mixed:
+        e, be, re, ue = (ctx.field(F32) for _ in range(4))
+        n, hist = solver.solve_mixed(u, b, e, be, re, ue, p.s, geom, p.bc, disc, tol=1e-8, guess=mode)
+    else:
+        r, u1 = ctx.field(F64), ctx.field(F64)
+        n, hist = solver.solve(u, b, r, u1, p.s, geom, p.bc, disc, tol=1e-8, guess=mode)
+    out = p.b.like()
+    u.download(out)
+    return n, hist, out.u, solver.guess_fallbacks(ctx)
+
+
+def test_coarse_first_iterate_cuts_cycles():
+    p = _small_cfg4_like()
+    print(f"tree {[len(l) for l in p.tree]} blocks", flush=True)
+    n0, h0, u0, _ = _solve(p, solver.GUESS_NONE)
+    n1, h1, u1, fb = _solve(p, solver.GUESS_COARSE)
+    print(f"cycles: zero {n0}, coarse {n1}; first residual {h1[0]:.3e}")
+    assert fb == 0 and h1[0] < 1.0 and n1 < n0 and h1[-1] <= 1e-8
+    G, M = p.b.G, p.M
+    v = (slice(G, G + M),) * 3
+    err = np.max(np.abs(u1[v] - u0[v])) / np.max(np.abs(u0[v]))
+    assert err < 1e-6
+
+
+def test_bad_learned_first_iterate_falls_back_to_zero():
+    p = _small_cfg4_like()
+    net = solver.UNet(3, (64, 64, 64), 4, (8, 16, 32, 64), seed=3)
+    net.set_problem(solver.geometry_from_code(p.geom), p.bc, solver.LaplacianDiscretization(3, 2))
+    net.output_scale = 1e9   # a first iterate far larger than the solution
+    n0, h0, u0, _ = _solve(p, solver.GUESS_NONE)
+    n1, h1, u1, fb = _solve(p, solver.GUESS_LEARNED, net)
+    assert fb == 1
+    np.testing.assert_array_equal(h1, h0)
+    np.testing.assert_array_equal(u1, u0)
+
+
+def test_learned_first_iterate_refuses_another_operator():
+    p = _small_cfg4_like()
+    net = solver.UNet(3, (64, 64, 64), 4, (8, 16, 32, 64), seed=3)
+    from srmg import tree as T
+    other = T.boundaryconditions(((1, 1), (1, 1), (1, 1)))
+    net.set_problem(solver.geometry_from_code(p.geom), other, solver.LaplacianDiscretization(3, 2))
+    with pytest.raises(solver.SrmgError, match="another level-1 operator"):
+        _solve(p, solver.GUESS_LEARNED, net)
+    with pytest.raises(solver.SrmgError, match="no network"):
+        _solve(p, solver.GUESS_LEARNED, None)
