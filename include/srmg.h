@@ -160,6 +160,7 @@ typedef struct {
     int32_t use_graph;         /* capture one cycle in a hipGraph and replay it */
     int32_t inner_cycles;      /* mixed precision: FP32 V-cycles per FP64 correction (0 means 1) */
     int32_t guess;             /* first iterate: SRMG_GUESS_* */
+    int32_t guess_smooth;      /* Gauss-Seidel sweeps on level 1 after a learned first iterate */
 } srmg_solve_opts;
 
 /* First iterate of a solve. NONE: u as given. LEARNED: the network attached
@@ -252,13 +253,18 @@ enum { SRMG_KERNEL_GS_PASS = 0, SRMG_KERNEL_RESIDUAL = 1, SRMG_KERNEL_GS_SWEEP =
 int srmg_time_kernel(srmg_ctx *ctx, int kind, int level, int u, int b, int r, double s,
                      const srmg_op *op, int reps, double *ms_per_launch);
 
-/* Autotuner: times every compiled tile shape of the fused sweep (blocks per
- * work group x work-group size) on every level with `reps` sweeps of u, keeps
- * the fastest per level for later solves of fields of u's element type, and
- * writes the choices (0..3) to chosen[nlevels] if non-NULL. u is restored. */
+/* Autotuner: times every compiled tile shape of the fused sweep on every
+ * level with `reps` sweeps of u, keeps the fastest per level for later solves
+ * of fields of u's element type, and writes the choices to chosen[nlevels] if
+ * non-NULL. u is restored. Shapes: 0..3 blocks per work group x work-group
+ * size; 4, 5, 6 (3D M=8 and 2D M=16) one sibling group of 2^D blocks as one
+ * tile on 256 / 512 / 1024 threads (levels without complete sibling groups,
+ * and thread counts that do not divide a group's cells per colour, run 0). */
 int srmg_autotune(srmg_ctx *ctx, int u, int b, double s, const srmg_op *op, int reps, int *chosen);
-/* Force the tile shape (0..3) of the fused sweep for one element type and level (1-based). */
+/* Force the tile shape (0..6) of the fused sweep for one element type and level (1-based). */
 int srmg_set_sweep_config(srmg_ctx *ctx, int dtype, int level, int cfg);
+/* 1 when the level's blocks form complete sibling groups (tile shapes 4..6 apply). */
+int srmg_level_grouped(srmg_ctx *ctx, int level, int *grouped);
 
 /* Blocks, leaf blocks and coarse-fine ghost slabs of a level (1-based). */
 int srmg_level_info(srmg_ctx *ctx, int level, int64_t *nblocks, int64_t *nleaf, int64_t *nslots);

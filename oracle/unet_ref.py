@@ -29,6 +29,18 @@ import numpy as np
 FEAT = 8
 
 
+def to_bf16(x):
+    """Round to the nearest BF16 value (ties to even), via FP32."""
+    a = np.asarray(x, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16 << 16
+    return (u.astype(np.uint32)).view(np.float32).astype(np.float64)
+
+
+def _ident(x):
+    return x
+
+
 def layer_specs(widths) -> List[Tuple[int, int, int, bool]]:
     """(level, cin, cout, relu) of every convolution, in parameter order."""
     L = len(widths)
@@ -154,42 +166,58 @@ def features(f):
     return x, scale
 
 
-def forward(params, f, D: int, widths):
-    """Normalized prediction y [B, *grid] and the activations for backward."""
-    P = unpack(params, D, widths)
+def _weights(params, D, widths, q):
+    return [(q(W), b) for W, b in unpack(params, D, widths)]
+
+
+def forward(params, f, D: int, widths, bf16: bool = False):
+    """Normalized prediction y [B, *grid] and the activations for backward.
+    bf16: round weights, input features and every stored activation to BF16
+    where the device kernels store them (accumulation stays exact here)."""
+    q = to_bf16 if bf16 else _ident
+    P = _weights(params, D, widths, q)
     L = len(widths)
     x, scale = features(f)
+    x = q(x)
     cache = {"x": x}
     A, E, Pp, Cc, S, Dd = {}, {}, {}, {}, {}, {}
     k = 0
     for l in range(L):
-        inp = x if l == 0 else Pp.setdefault(l, pool(E[l - 1]))
+        inp = x if l == 0 else Pp.setdefault(l, q(pool(E[l - 1])))
         W, b = P[k]; k += 1
-        A[l] = np.maximum(conv(inp, W, b), 0)
+        A[l] = q(np.maximum(conv(inp, W, b), 0))
         W, b = P[k]; k += 1
-        E[l] = np.maximum(conv(A[l], W, b), 0)
+        E[l] = q(np.maximum(conv(A[l], W, b), 0))
     for l in range(L - 2, -1, -1):
         src = E[L - 1] if l == L - 2 else Dd[l + 1]
         W, b = P[k]; k += 1
-        Cc[l] = conv(src, W, b)
-        S[l] = upsample(Cc[l]) + E[l]
+        Cc[l] = q(conv(src, W, b))
+        S[l] = q(upsample(Cc[l]) + E[l])
         W, b = P[k]; k += 1
-        Dd[l] = np.maximum(conv(S[l], W, b), 0)
+        Dd[l] = q(np.maximum(conv(S[l], W, b), 0))
     W, b = P[k]
     y = conv(Dd[0], W, b)[..., 0]
+    if bf16:
+        y = y.astype(np.float32).astype(np.float64)
     cache.update(A=A, E=E, P=Pp, C=Cc, S=S, Dd=Dd, scale=scale)
     return y, cache
 
 
-def loss_and_grad(params, f, u, c_out: float, D: int, widths):
-    """MSE of the normalized prediction and its gradient (parameter order)."""
-    y, ca = forward(params, f, D, widths)
+def loss_and_grad(params, f, u, c_out: float, D: int, widths, bf16: bool = False):
+    """MSE of the normalized prediction and its gradient (parameter order).
+    bf16: also round every stored gradient tensor to BF16 where the device
+    kernels store them."""
+    q = to_bf16 if bf16 else _ident
+    y, ca = forward(params, f, D, widths, bf16)
     B = f.shape[0]
     t = u / (ca["scale"].reshape((B,) + (1,) * D) * c_out)
     d = y - t
     loss = float(np.mean(d * d))
-    dy = (2.0 / d.size) * d
-    P = unpack(params, D, widths)
+    if bf16:
+        dy = q(np.float32(2.0 / d.size) * d.astype(np.float32))
+    else:
+        dy = (2.0 / d.size) * d
+    P = _weights(params, D, widths, q)
     L = len(widths)
     grads = [None] * len(P)
     A, E, Pp, Cc, S, Dd = ca["A"], ca["E"], ca["P"], ca["C"], ca["S"], ca["Dd"]
@@ -206,32 +234,32 @@ def loss_and_grad(params, f, u, c_out: float, D: int, widths):
     W, _ = P[idx_y]
     dx, dW, db = conv_backward(Dd[0], W, dy[..., None])
     grads[idx_y] = (dW, db)
-    gDd[0] = dx * (Dd[0] > 0)
+    gDd[0] = q(dx * (Dd[0] > 0))
     for l in range(0, L - 1):
         W, _ = P[idx_d[l]]
         dx, dW, db = conv_backward(S[l], W, gDd[l])
         grads[idx_d[l]] = (dW, db)
-        gS[l] = dx
-        gC = upsample_backward(gS[l])
+        gS[l] = q(dx)
+        gC = q(upsample_backward(gS[l]))
         src = E[L - 1] if l == L - 2 else Dd[l + 1]
         W, _ = P[idx_c[l]]
         dx, dW, db = conv_backward(src, W, gC)
         grads[idx_c[l]] = (dW, db)
         if l == L - 2:
-            gE[L - 1] = dx * (src > 0)
+            gE[L - 1] = q(dx * (src > 0))
         else:
-            gDd[l + 1] = dx * (src > 0)
+            gDd[l + 1] = q(dx * (src > 0))
     for l in range(L - 1, -1, -1):
         W, _ = P[idx_e[l]]
         dx, dW, db = conv_backward(A[l], W, gE[l])
         grads[idx_e[l]] = (dW, db)
-        gA = dx * (A[l] > 0)
+        gA = q(dx * (A[l] > 0))
         inp = ca["x"] if l == 0 else Pp[l]
         W, _ = P[idx_a[l]]
         dx, dW, db = conv_backward(inp, W, gA)
         grads[idx_a[l]] = (dW, db)
         if l > 0:
-            gE[l - 1] = (gS[l - 1] + pool_backward(dx, E[l - 1])) * (E[l - 1] > 0)
+            gE[l - 1] = q((gS[l - 1] + pool_backward(q(dx), E[l - 1])) * (E[l - 1] > 0))
     flat = np.concatenate([np.concatenate([gw.ravel(), gb.ravel()]) for gw, gb in grads])
     return loss, flat
 

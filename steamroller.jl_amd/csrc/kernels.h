@@ -380,15 +380,22 @@ __device__ __forceinline__ int xcd_remap(int i, int G) {
     return (x < rem) ? x * (per + 1) + local : rem * (per + 1) + (x - rem) * per + local;
 }
 
-template <int D, int M>
+// LDS tile of G^D blocks of M^D cells with a depth-1 halo: cells -1 .. G M.
+// Helpers that work on one block of a tile take a pointer offset to that
+// block's origin (the tile index is linear in the coordinates).
+template <int D, int M, int G = 1>
 struct Tile {
-    static constexpr int S = M + 2;
+    static constexpr int S = G * M + 2;
     static constexpr int SZ = (D == 2) ? S * S : S * S * S;
     __device__ static __forceinline__ int at(int x, int y, int z) {
         return (x + 1) + S * ((y + 1) + (D == 3 ? S * (z + 1) : 0));
     }
     __device__ static __forceinline__ int at3(const int *c) { return at(c[0], c[1], D == 3 ? c[2] : 0); }
     __device__ static constexpr int stride(int d) { return d == 0 ? 1 : (d == 1 ? S : S * S); }
+    // offset of block k of a G = 2 group (k's bit d = upper half along d)
+    __device__ static __forceinline__ int origin(int k) {
+        return M * ((k & 1) + S * (((k >> 1) & 1) + (D == 3 ? S * ((k >> 2) & 1) : 0)));
+    }
 };
 
 // Per-block halo table used by the tiled kernels: the block's own 2D face
@@ -404,9 +411,8 @@ struct Halo {
 
 // Stencil of the tile cell at p, all neighbours in the tile. RD is the radial
 // dimension (cylindrical geometry) or -1; terms in the reference's order.
-template <int D, int M, typename T, int RD>
+template <int D, int M, typename T, int RD, class TL = Tile<D, M>>
 __device__ __forceinline__ T tstencil(const T *t, int p, T own, T cc, T wm, T wp) {
-    using TL = Tile<D, M>;
     T v = T(0);
 #pragma unroll
     for (int d = D - 1; d >= 0; d--) {
@@ -436,11 +442,10 @@ __device__ __forceinline__ void rweights(const LevelArgs<T> &a, const int *__res
 
 // Load the depth-1 halo cell number t of face F (a compile-time face) of block
 // blk into the tile: neighbour cell, mirror of the facing cell, or frozen slab.
-template <int D, int M, typename T, int F>
+template <int D, int M, typename T, int F, class TL = Tile<D, M>>
 __device__ __forceinline__ void halo_cell(T *tile, int e, int blk, int t, const T *__restrict__ x,
                                           const T *__restrict__ ib, T sg) {
     using LY = Lay<D, M>;
-    using TL = Tile<D, M>;
     constexpr int DIM = F >> 1, DIR = (F & 1) ? 1 : -1;
     int c[3] = {0, 0, 0};
     int ta = t % M, tb = t / M;
@@ -463,26 +468,25 @@ __device__ __forceinline__ void halo_cell(T *tile, int e, int blk, int t, const 
 
 // halo_cell for a face known only at run time (uniform per wave where the
 // work layout allows it)
-template <int D, int M, typename T>
+template <int D, int M, typename T, class TL = Tile<D, M>>
 __device__ __forceinline__ void halo_switch(T *tl, int e, int f, int blk, int t, const T *__restrict__ x,
                                             const T *__restrict__ ib, const T *sg) {
     switch (f) {
-        case 0: halo_cell<D, M, T, 0>(tl, e, blk, t, x, ib, sg[0]); break;
-        case 1: halo_cell<D, M, T, 1>(tl, e, blk, t, x, ib, sg[1]); break;
-        case 2: halo_cell<D, M, T, 2>(tl, e, blk, t, x, ib, sg[2]); break;
-        case 3: halo_cell<D, M, T, 3>(tl, e, blk, t, x, ib, sg[3]); break;
-        case 4: halo_cell<D, M, T, (D == 3 ? 4 : 3)>(tl, e, blk, t, x, ib, sg[4]); break;
-        default: halo_cell<D, M, T, (D == 3 ? 5 : 3)>(tl, e, blk, t, x, ib, sg[5]); break;
+        case 0: halo_cell<D, M, T, 0, TL>(tl, e, blk, t, x, ib, sg[0]); break;
+        case 1: halo_cell<D, M, T, 1, TL>(tl, e, blk, t, x, ib, sg[1]); break;
+        case 2: halo_cell<D, M, T, 2, TL>(tl, e, blk, t, x, ib, sg[2]); break;
+        case 3: halo_cell<D, M, T, 3, TL>(tl, e, blk, t, x, ib, sg[3]); break;
+        case 4: halo_cell<D, M, T, (D == 3 ? 4 : 3), TL>(tl, e, blk, t, x, ib, sg[4]); break;
+        default: halo_cell<D, M, T, (D == 3 ? 5 : 3), TL>(tl, e, blk, t, x, ib, sg[5]); break;
     }
 }
 
 // Neighbour face-layer cells recomputed by the fused sweep: for the face
 // normal to DIM on side dir, the neighbour's first-half-sweep cells of its
 // layer facing this block, numbered tq in [0, F/2).
-template <int D, int M, typename T, int DIM>
+template <int D, int M, typename T, int DIM, class TL = Tile<D, M>>
 struct RecFace {
     using LY = Lay<D, M>;
-    using TL = Tile<D, M>;
     using HT = Halo<D>;
     static constexpr int DA = DIM == 0 ? 1 : 0;
     static constexpr int DB = DIM == 2 ? 1 : 2;
@@ -724,6 +728,138 @@ __global__ __launch_bounds__(NT) void k_gs_sweep(T *__restrict__ uo, const T *__
             uo[o] = ux[k][0];
             uo[o + H] = fma_t<T>(bp.nwc, fma_t<T>(bp.s, be[k], Lu), ux[k][1]);
         }
+    }
+}
+
+// k_gs_sweep on sibling groups: the 2^D children of one parent (consecutive
+// in the Morton order of a level, always created together) form one
+// (2M)^D tile. The faces between siblings need no halo and no recompute; the
+// group's exterior faces are handled per block exactly as in k_gs_sweep (own
+// halo-table entries, neighbour recompute from the same inputs), so the result
+// is bitwise the same. One group per work group.
+template <int D, int M, typename T, int NT, int RD>
+__global__ __launch_bounds__(NT) void k_gs_sweep_group(T *__restrict__ uo, const T *__restrict__ ui,
+                                                       const T *__restrict__ b, const T *__restrict__ ib,
+                                                       const int *__restrict__ htab, const int *__restrict__ coord,
+                                                       LevelArgs<T> a) {
+    using LY = Lay<D, M>;
+    using TL = Tile<D, M, 2>;
+    using HT = Halo<D>;
+    constexpr int NB = 1 << D;
+    constexpr int F = LY::F, V = LY::V, H = LY::H, F2 = F / 2, NS = 2 * (D - 1);
+    static_assert((NB * H) % NT == 0, "cells per colour must be a multiple of the work-group size");
+    constexpr int CPT = NB * H / NT;
+    constexpr int NREC = NB * D * F2;  // exterior faces: D per block
+    constexpr int RPT = (NREC + NT - 1) / NT;
+    constexpr int NHALO = NB * D * F;
+    __shared__ T tile[TL::SZ];
+    __shared__ int tab[NB * HT::N];
+    __shared__ BP<T> prm[NB];
+    const int blk0 = a.blk0 + xcd_remap(blockIdx.x, gridDim.x) * NB;
+    const int tid = threadIdx.x;
+
+    T ux[CPT][2], bo[CPT], be[CPT];
+#pragma unroll
+    for (int k = 0; k < CPT; k++) {
+        const int i = tid + k * NT, lb = i / H, r = i - lb * H;
+        const size_t o = (size_t)(blk0 + lb) * V + r;
+        ux[k][0] = ui[o];
+        ux[k][1] = ui[o + H];
+        bo[k] = b[o];
+        be[k] = b[o + H];
+    }
+    for (int i = tid; i < NB * HT::N; i += NT) tab[i] = htab[(size_t)blk0 * HT::N + i];
+    if (tid < NB) prm[tid] = bparams(a, blk0 + tid);
+#pragma unroll
+    for (int k = 0; k < CPT; k++) {
+        const int i = tid + k * NT, lb = i / H, r = i - lb * H;
+        int x, y, z;
+        LY::decode(r, x, y, z);
+        tile[TL::origin(lb) + TL::at(x, y, z)] = ux[k][0];
+        LY::decode(H + r, x, y, z);
+        tile[TL::origin(lb) + TL::at(x, y, z)] = ux[k][1];
+    }
+    __syncthreads();
+
+    // halo of the exterior faces: block lb, dimension d, face side = its bit d
+    for (int i = tid; i < NHALO; i += NT) {
+        const int lb = i / (D * F), rem = i - lb * (D * F);
+        const int d = rem / F, t = rem - d * F;
+        const int f = 2 * d + ((lb >> d) & 1);
+        halo_switch<D, M, T, TL>(tile + TL::origin(lb), tab[lb * HT::N + f], f, blk0 + lb, t, ui, ib, a.sg);
+    }
+    T rd[RPT], rb[RPT], rg[RPT][NS];
+    int rj[RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; k++) {
+        const int j = tid + k * NT;
+        rj[k] = -1;
+        rd[k] = rb[k] = T(0);
+        if (j >= NREC) continue;
+        const int lb = j / (D * F2), rem = j - lb * (D * F2);
+        const int d = rem / F2, tq = rem - d * F2;
+        const int f = 2 * d + ((lb >> d) & 1);
+        const int *tb = tab + lb * HT::N;
+        const int e = tb[f];
+        if (e < 0) continue;
+        rj[k] = j;
+        const int dir = (f & 1) ? 1 : -1;
+        if (d == 0) RecFace<D, M, T, 0, TL>::prefetch(dir, tq, e, tb, ui, b, ib, rd[k], rb[k], rg[k]);
+        else if (d == 1) RecFace<D, M, T, 1, TL>::prefetch(dir, tq, e, tb, ui, b, ib, rd[k], rb[k], rg[k]);
+        else RecFace<D, M, T, (D == 3 ? 2 : 1), TL>::prefetch(dir, tq, e, tb, ui, b, ib, rd[k], rb[k], rg[k]);
+    }
+    __syncthreads();
+
+    // first half-sweep
+#pragma unroll
+    for (int k = 0; k < CPT; k++) {
+        const int i = tid + k * NT, lb = i / H, r = i - lb * H;
+        int x, y, z;
+        LY::decode(r, x, y, z);
+        const int p = TL::origin(lb) + TL::at(x, y, z);
+        T wm, wp;
+        rweights<D, M, T, RD>(a, coord, blk0 + lb, RD == 0 ? x : y, wm, wp);
+        const BP<T> bp = prm[lb];
+        T Lu = tstencil<D, M, T, RD, TL>(tile, p, ux[k][0], bp.cc, wm, wp);
+        ux[k][0] = fma_t<T>(bp.nwc, fma_t<T>(bp.s, bo[k], Lu), ux[k][0]);
+        tile[p] = ux[k][0];
+    }
+#pragma unroll
+    for (int k = 0; k < RPT; k++) {
+        const int j = rj[k];
+        if (j < 0) continue;
+        const int lb = j / (D * F2), rem = j - lb * (D * F2);
+        const int d = rem / F2, tq = rem - d * F2;
+        const int f = 2 * d + ((lb >> d) & 1);
+        const int *tb = tab + lb * HT::N;
+        T *tl = tile + TL::origin(lb);
+        const int e = tb[f];
+        const int dir = (f & 1) ? 1 : -1;
+        const BP<T> bp = prm[lb];
+        if (d == 0)
+            RecFace<D, M, T, 0, TL>::template compute<RD>(dir, tq, e, tb, tl, coord, a, bp, rd[k], rb[k], rg[k]);
+        else if (d == 1)
+            RecFace<D, M, T, 1, TL>::template compute<RD>(dir, tq, e, tb, tl, coord, a, bp, rd[k], rb[k], rg[k]);
+        else
+            RecFace<D, M, T, (D == 3 ? 2 : 1), TL>::template compute<RD>(dir, tq, e, tb, tl, coord, a, bp, rd[k],
+                                                                         rb[k], rg[k]);
+    }
+    __syncthreads();
+
+    // second half-sweep
+#pragma unroll
+    for (int k = 0; k < CPT; k++) {
+        const int i = tid + k * NT, lb = i / H, r = i - lb * H;
+        int x, y, z;
+        LY::decode(H + r, x, y, z);
+        const int p = TL::origin(lb) + TL::at(x, y, z);
+        T wm, wp;
+        rweights<D, M, T, RD>(a, coord, blk0 + lb, RD == 0 ? x : y, wm, wp);
+        const BP<T> bp = prm[lb];
+        T Lu = tstencil<D, M, T, RD, TL>(tile, p, ux[k][1], bp.cc, wm, wp);
+        const size_t o = (size_t)(blk0 + lb) * V + r;
+        uo[o] = ux[k][0];
+        uo[o + H] = fma_t<T>(bp.nwc, fma_t<T>(bp.s, be[k], Lu), ux[k][1]);
     }
 }
 

@@ -56,6 +56,7 @@ struct LevelInfo {
     int leaf0 = 0, nleaf = 0;   // entries in the leaf list
     int ext[3] = {1, 1, 1};     // largest level domain in blocks over the trees
     int wofs[2] = {0, 0};       // radial weight table offsets (dims 0 and 1)
+    bool grouped = false;       // blocks form complete sibling groups of 2^D (k_gs_sweep_group)
 };
 
 // One tree of a forest: an independent problem whose blocks share the
@@ -252,6 +253,25 @@ static void set_forest(srmg_ctx *c, const std::vector<TreeRecords> &tr) {
             T.nblk[l] = dev - T.blk0[l];
         }
         L.nblk = dev - L.blk0;
+    }
+    // sibling groups: 2^D consecutive blocks of one tree whose coordinates are
+    // 2P + (bit d of the position in the group)
+    const int NBg = 1 << D;
+    for (int l = 0; l < nlevels; l++) {
+        LevelInfo &L = c->lv[l];
+        bool ok = L.nblk > 0 && L.nblk % NBg == 0;
+        for (int g = 0; ok && g < L.nblk / NBg; g++) {
+            const int b0 = L.blk0 + g * NBg;
+            for (int k = 0; ok && k < NBg; k++) {
+                const int bk = b0 + k;
+                if (c->h_tree[bk] != c->h_tree[b0]) ok = false;
+                for (int d = 0; ok && d < D; d++) {
+                    const int c0 = c->h_coord[(size_t)b0 * D + d] - 1, ck = c->h_coord[(size_t)bk * D + d] - 1;
+                    if ((c0 & 1) != 0 || ck != c0 + ((k >> d) & 1)) ok = false;
+                }
+            }
+        }
+        L.grouped = ok;
     }
     for (auto &G : c->groups) {
         int n1 = 1;
@@ -636,7 +656,9 @@ struct Ops {
     // large blocks (3D, M = 16) use one block on 256, 512, 1024 or 512 threads.
     static constexpr bool SMALL = LY::H <= 256;
     static constexpr int BPW0 = SMALL ? 256 / LY::H : 1;
-    static constexpr int NCFG = 4;
+    // sibling-group tiles (k_gs_sweep_group) where a group's tile fits in LDS
+    static constexpr bool GROUP_OK = (D == 3 && M == 8) || (D == 2 && M == 16);
+    static constexpr int NCFG = GROUP_OK ? 7 : 4;
     template <int K> static constexpr int cfg_bpw() { return SMALL ? BPW0 * (K == 0 ? 1 : (K == 3 ? 4 : 2)) : 1; }
     template <int K> static constexpr int cfg_nt() {
         return SMALL ? (K < 2 ? 256 : 512) : (K == 0 ? 256 : (K == 2 ? 1024 : 512));
@@ -651,16 +673,36 @@ struct Ops {
                                I(u), c->d_halo.as<int>(), coord(), a);
         });
     }
+    template <int NTG>
+    void sweep_group_launch(T *out, T *in, FieldBuf &u, FieldBuf &b, const LevelArgs<T> &a, int l) {
+        if constexpr (GROUP_OK && (((1 << D) * LY::H) % NTG == 0)) {
+            unsigned g = (unsigned)(a.nblk >> D);
+            with_rd(a.rdim, [&](auto rd) {
+                constexpr int RD = decltype(rd)::value;
+                hipLaunchKernelGGL((k_gs_sweep_group<D, M, T, NTG, RD>), dim3(g), dim3(NTG), 0, c->stream, out, in,
+                                   X(b, l), I(u), c->d_halo.as<int>(), coord(), a);
+            });
+        } else {
+            sweep_launch<0>(out, in, u, b, a, l);
+        }
+    }
+    // tile shapes 0..3: k_gs_sweep (cfg_bpw / cfg_nt); 4, 5, 6: sibling groups
+    // on 256 / 512 / 1024 threads (levels that are not grouped, and group sizes
+    // the thread count does not divide, use shape 0)
     void sweep(FieldBuf &u, FieldBuf &b, const LevelArgs<T> &a, int l, int cfg = -1) {
         if (!a.nblk) return;
         if (!u.alt.p) u.alt.alloc(u.data.bytes);
         T *in = X(u, l);
         T *out = u.cur[l] ? u.data.as<T>() : u.alt.as<T>();
         if (cfg < 0) cfg = c->sweep_cfg[(std::is_same<T, double>::value ? 0 : c->nlevels) + l];
+        if (cfg >= 4 && !(GROUP_OK && c->lv[l].grouped)) cfg = 0;
         switch (cfg) {
             case 1: sweep_launch<1>(out, in, u, b, a, l); break;
             case 2: sweep_launch<2>(out, in, u, b, a, l); break;
             case 3: sweep_launch<3>(out, in, u, b, a, l); break;
+            case 4: sweep_group_launch<256>(out, in, u, b, a, l); break;
+            case 5: sweep_group_launch<512>(out, in, u, b, a, l); break;
+            case 6: sweep_group_launch<1024>(out, in, u, b, a, l); break;
             default: sweep_launch<0>(out, in, u, b, a, l); break;
         }
         u.cur[l] ^= 1;
@@ -840,7 +882,7 @@ struct Ops {
     }
 
     // first iterate of a solve from the level-1 problem (SRMG_GUESS_*)
-    void first_iterate(FieldBuf &u, FieldBuf &b, int mode, const std::vector<double> &r0);
+    void first_iterate(FieldBuf &u, FieldBuf &b, int mode, int nsmooth, const std::vector<double> &r0);
 
     // fmg! (src/multigrid.jl:17-51)
     void fmg(FieldBuf &u, FieldBuf &b, FieldBuf &r, FieldBuf &u1, double s, const srmg_op *op, int ndown, int ntop,
@@ -881,7 +923,7 @@ struct Ops {
 // Trees whose residual is then not finite or larger than with u = 0 restart
 // from zero.
 template <int D, int M, typename T>
-void Ops<D, M, T>::first_iterate(FieldBuf &u, FieldBuf &b, int mode, const std::vector<double> &r0) {
+void Ops<D, M, T>::first_iterate(FieldBuf &u, FieldBuf &b, int mode, int nsmooth, const std::vector<double> &r0) {
     const double s = c->ps[0];
     const srmg_op *op = &c->pops[0];
     const int nt = (int)c->trees.size();
@@ -923,6 +965,12 @@ void Ops<D, M, T>::first_iterate(FieldBuf &u, FieldBuf &b, int mode, const std::
             srmg_guess::infer(c->guess_net, g, g2, G.ntree, c->stream);
             hipLaunchKernelGGL((k_grid_scatter<D, M, T>), dim3(nwg((int64_t)nblk * LY::V)), dim3(WG), 0,
                                c->stream, X(u, 0), g2, coord(), G.l1_blk0, nblk, G.N[0], G.N[1], G.nper, total);
+        }
+        // the prediction's grid-scale error is what the Laplacian amplifies:
+        // a few level-1 sweeps remove it
+        if (nsmooth > 0) {
+            smooth(u, b, level_args<T>(c, 0, s, op, 1.0), 0, nsmooth);
+            settle(u);
         }
     }
     for (int l = 1; l < c->nlevels; l++) {
@@ -1276,6 +1324,7 @@ static void check_opts(const srmg_solve_opts *o) {
     if (!o) fail("null options");
     if (o->max_cycles < 0 || o->ndown < 0 || o->nup < 0 || o->ntop < 0) fail("bad solve options");
     if (o->guess < SRMG_GUESS_NONE || o->guess > SRMG_GUESS_COARSE) fail("bad first-iterate mode");
+    if (o->guess_smooth < 0) fail("bad first-iterate smoothing count");
 }
 
 // relative residuals; true when every tree is at or below tol
@@ -1309,7 +1358,7 @@ static void solve_plain(srmg_ctx *c, int ui, int bi, int ri, int u1i, const srmg
         for (int l = 1; l < c->nlevels; l++) ops.fill_interp(u, l, level_args<T>(c, l - 1, s, op, 1.0));
         ops.resnorm_partials(zero_field(c, t), b, s, op);
         std::vector<double> r0 = safe_r0(read_norms(c));
-        if (o->guess) ops.first_iterate(u, b, o->guess, r0);
+        if (o->guess) ops.first_iterate(u, b, o->guess, o->guess_smooth, r0);
         ops.resnorm_partials(u, b, s, op);
         bool done = record(read_norms(c), r0, o->tol, 0, o->max_cycles, hist);
         int k = 0;
@@ -1341,7 +1390,7 @@ static void solve_mixed_dm(srmg_ctx *c, FieldBuf &u, FieldBuf &b, FieldBuf &e, F
     for (int l = 1; l < c->nlevels; l++) o64.fill_interp(u, l, level_args<double>(c, l - 1, s, op, 1.0));
     o64.resnorm_partials(zero_field(c, SRMG_F64), b, s, op);
     std::vector<double> r0 = safe_r0(read_norms(c));
-    if (o->guess) o64.first_iterate(u, b, o->guess, r0);
+    if (o->guess) o64.first_iterate(u, b, o->guess, o->guess_smooth, r0);
     auto defect_and_norm = [&] { o64.defect_and_norm(be.data.as<float>(), u, b, s, op); };
     auto correct = [&] {
         size_t n = (size_t)c->nb * LY::V, ni = e.ib.bytes / sizeof(float);
@@ -1803,8 +1852,16 @@ int srmg_set_sweep_config(srmg_ctx *c, int dtype, int level, int cfg) {
         check_tree(c);
         if (level < 1 || level > c->nlevels) fail("level out of range");
         if (dtype != SRMG_F64 && dtype != SRMG_F32) fail("bad dtype");
-        if (cfg < 0 || cfg > 3) fail("sweep configuration must be 0..3");
+        if (cfg < 0 || cfg > 6) fail("sweep configuration must be 0..6");
         c->sweep_cfg[(dtype == SRMG_F64 ? 0 : c->nlevels) + level - 1] = cfg;
+    });
+}
+
+int srmg_level_grouped(srmg_ctx *c, int level, int *grouped) {
+    return guarded([&] {
+        check_tree(c);
+        if (level < 1 || level > c->nlevels || !grouped) fail("bad arguments");
+        *grouped = c->lv[level - 1].grouped ? 1 : 0;
     });
 }
 

@@ -82,7 +82,8 @@ class _Op(C.Structure):
 class _SolveOpts(C.Structure):
     _fields_ = [("ndown", C.c_int32), ("ntop", C.c_int32), ("nup", C.c_int32),
                 ("omega", C.c_double), ("tol", C.c_double), ("max_cycles", C.c_int32),
-                ("use_graph", C.c_int32), ("inner_cycles", C.c_int32), ("guess", C.c_int32)]
+                ("use_graph", C.c_int32), ("inner_cycles", C.c_int32), ("guess", C.c_int32),
+                ("guess_smooth", C.c_int32)]
 
 
 def make_op(geometry, bc, disc, coarse: int = COARSE_GS) -> _Op:
@@ -141,6 +142,7 @@ def lib():
         "srmg_level_info": [P, i32, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64)],
         "srmg_autotune": [P, i32, i32, dbl, P, i32, P],
         "srmg_set_sweep_config": [P, i32, i32, i32],
+        "srmg_level_grouped": [P, i32, C.POINTER(i32)],
         "srmg_set_guess": [P, P],
         "srmg_guess_fallbacks": [P, C.POINTER(i32)],
         "srmg_grid_solve": [P, i32, P, dbl, P, P, P, i32],
@@ -275,6 +277,11 @@ class Context:
         _check(lib().srmg_autotune(self.h, u.id, b.id, float(s), C.byref(op), int(reps), _p(out)))
         return out
 
+    def level_grouped(self, level: int) -> bool:
+        g = C.c_int()
+        _check(lib().srmg_level_grouped(self.h, int(level), C.byref(g)))
+        return bool(g.value)
+
     def set_sweep_config(self, dtype: int, level: int, cfg: int):
         _check(lib().srmg_set_sweep_config(self.h, dtype, level, cfg))
 
@@ -373,21 +380,25 @@ def resnorm(r, b, u, s, tree=None, conn=None, geometry=None, bc=None, lpldisc=No
     return out.value
 
 
-def _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph, inner_cycles=1, guess=0):
+GUESS_SMOOTH = 4  # level-1 sweeps after a learned first iterate (default)
+
+
+def _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph, inner_cycles=1, guess=0, guess_smooth=GUESS_SMOOTH):
     o = _SolveOpts()
     o.ndown, o.ntop, o.nup = int(ndown), int(ntop), int(nup)
     o.omega, o.tol, o.max_cycles, o.use_graph = float(omega), float(tol), int(max_cycles), int(use_graph)
     o.inner_cycles = int(inner_cycles)
     o.guess = int(guess)
+    o.guess_smooth = int(guess_smooth)
     return o
 
 
 def solve(u, b, r, u1, s, geometry, bc, lpldisc, *, tol=1e-8, max_cycles=50, ndown=2, ntop=4,
-          nup=2, omega=1.0, coarse=COARSE_EXACT, use_graph=True, guess=GUESS_NONE):
+          nup=2, omega=1.0, coarse=COARSE_EXACT, use_graph=True, guess=GUESS_NONE, guess_smooth=GUESS_SMOOTH):
     """Pre-fill ghosts, then V-cycles until resnorm(u) <= tol * resnorm(0).
     Returns (cycles, relative residual history)."""
     op = make_op(geometry, bc, lpldisc, coarse)
-    o = _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph, 1, guess)
+    o = _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph, 1, guess, guess_smooth)
     hist = np.zeros(max_cycles + 1)
     n = C.c_int()
     _check(lib().srmg_solve(u.ctx.h, u.id, b.id, r.id, u1.id, float(s), C.byref(op), C.byref(o),
@@ -397,11 +408,11 @@ def solve(u, b, r, u1, s, geometry, bc, lpldisc, *, tol=1e-8, max_cycles=50, ndo
 
 def solve_mixed(u, b, e, be, re, ue, s, geometry, bc, lpldisc, *, tol=1e-8, max_cycles=50,
                 ndown=2, ntop=4, nup=2, omega=1.0, coarse=COARSE_EXACT, use_graph=True, inner_cycles=1,
-                guess=GUESS_NONE):
+                guess=GUESS_NONE, guess_smooth=GUESS_SMOOTH):
     """FP64 defect correction around FP32 V-cycles (u, b FP64; e, be, re, ue FP32).
     Returns (outer steps, relative residual history per outer step)."""
     op = make_op(geometry, bc, lpldisc, coarse)
-    o = _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph, inner_cycles, guess)
+    o = _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph, inner_cycles, guess, guess_smooth)
     hist = np.zeros(max_cycles + 1)
     n = C.c_int()
     _check(lib().srmg_solve_mixed(u.ctx.h, u.id, b.id, e.id, be.id, re.id, ue.id, float(s),
@@ -451,11 +462,11 @@ def resnorm_batched(r, b, u, s, geometry, bc, discs, *, coarse=COARSE_GS):
 
 
 def solve_batched(u, b, r, u1, s, geometry, bc, discs, *, tol=1e-8, max_cycles=50, ndown=2, ntop=4, nup=2,
-                  omega=1.0, coarse=COARSE_EXACT, use_graph=True, guess=GUESS_NONE):
+                  omega=1.0, coarse=COARSE_EXACT, use_graph=True, guess=GUESS_NONE, guess_smooth=GUESS_SMOOTH):
     """Solve every tree of a forest to tol; returns (cycles, history[ntrees, cycles + 1])."""
     sv = np.ascontiguousarray(np.asarray(s, np.float64))
     ops = _ops(geometry, bc, discs, coarse)
-    o = _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph, 1, guess)
+    o = _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph, 1, guess, guess_smooth)
     hist = np.zeros((len(discs), max_cycles + 1))
     n = C.c_int()
     _check(lib().srmg_solve_batched(u.ctx.h, u.id, b.id, r.id, u1.id, _p(sv), ops, C.byref(o), C.byref(n),
@@ -465,11 +476,11 @@ def solve_batched(u, b, r, u1, s, geometry, bc, discs, *, tol=1e-8, max_cycles=5
 
 def solve_mixed_batched(u, b, e, be, re, ue, s, geometry, bc, discs, *, tol=1e-8, max_cycles=50, ndown=2,
                         ntop=4, nup=2, omega=1.0, coarse=COARSE_EXACT, use_graph=True, inner_cycles=1,
-                        guess=GUESS_NONE):
+                        guess=GUESS_NONE, guess_smooth=GUESS_SMOOTH):
     """FP64 defect correction around FP32 V-cycles on every tree of a forest."""
     sv = np.ascontiguousarray(np.asarray(s, np.float64))
     ops = _ops(geometry, bc, discs, coarse)
-    o = _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph, inner_cycles, guess)
+    o = _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph, inner_cycles, guess, guess_smooth)
     hist = np.zeros((len(discs), max_cycles + 1))
     n = C.c_int()
     _check(lib().srmg_solve_mixed_batched(u.ctx.h, u.id, b.id, e.id, be.id, re.id, ue.id, _p(sv), ops,

@@ -137,3 +137,23 @@ def test_single_problem_calls_on_a_forest_apply_to_every_tree():
         ref = p.b.like()
         v.download(ref)
         np.testing.assert_array_equal(_valid(got[k], p), _valid(ref.u, p))
+
+
+@pytest.mark.parametrize("cfg", [4, 5, 6])
+def test_forest_sibling_group_sweeps_equal_block_sweeps(cfg):
+    probs = _problems(n_random=3)
+    geom = solver.geometry_from_code(probs[0].geom)
+    outs = []
+    for c in (0, cfg):
+        ctx, (u, b, r, u1) = _forest(probs)
+        nl = max(len(p.tree) for p in probs)
+        for l in range(1, nl + 1):
+            ctx.set_sweep_config(F64, l, c)
+        if c:
+            print(f"cfg {cfg}: grouped levels {[ctx.level_grouped(l) for l in range(1, nl + 1)]}", flush=True)
+        solver.fill_ghost_all(u, probs[0].bc)
+        for _ in range(2):
+            solver.vcycle_batched(u, b, r, u1, [p.s for p in probs], geom, probs[0].bc, [_disc(p) for p in probs],
+                                  ndown=2, ntop=4, nup=2, coarse=solver.COARSE_EXACT)
+        outs.append(np.asarray(_download(u, probs)[0]).copy())
+    np.testing.assert_array_equal(outs[1], outs[0])

@@ -244,8 +244,12 @@ def test_every_sweep_tile_shape_gives_identical_results(name, dtype):
     print(f"case {name} dtype {dtype}", flush=True)
     geom, disc = solver.geometry_from_code(p.geom), _disc(p)
     outs = []
-    for cfg in range(4):
+    grouped = []
+    for cfg in range(7):
         ctx, (u, b, r, u1) = _device(p, dtype)
+        if cfg == 0:
+            grouped = [ctx.level_grouped(l) for l in range(1, len(p.tree) + 1)]
+            print(f"  levels with sibling groups: {grouped}", flush=True)
         for l in range(1, len(p.tree) + 1):
             ctx.set_sweep_config(dtype, l, cfg)
         b.upload(p.b)

@@ -91,3 +91,22 @@ def test_features_layout():
 def test_param_count_of_the_cfg4_network():
     # 4 levels, widths 8..64, 3D: about 3.3e5 parameters
     assert U.param_count(3, (8, 16, 32, 64)) == 330264 + sum(co for _, _, co, _ in U.layer_specs((8, 16, 32, 64)))
+
+
+def test_bf16_rounding_known_values():
+    vals = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -9, 1.0 + 2 ** -7 + 2 ** -8, -3.0e-3, 65504.0, 0.0])
+    # halfway cases round to the even neighbour
+    want = np.array([1.0, 1.0, 1.0 + 2 ** -7, 1.0 + 2 ** -6, -0.0030059814453125, 65536.0, 0.0])
+    np.testing.assert_array_equal(U.to_bf16(vals), want)
+
+
+def test_bf16_emulation_stays_close_to_exact():
+    rng = np.random.default_rng(5)
+    widths = (8, 16)
+    p = U.he_params(2, widths, rng)
+    f = U.smooth_field(rng, (2, 8, 16))
+    u = U.smooth_field(rng, (2, 8, 16)) * 10
+    l0, g0 = U.loss_and_grad(p, f, u, 10.0, 2, widths)
+    l1, g1 = U.loss_and_grad(p, f, u, 10.0, 2, widths, bf16=True)
+    assert abs(l1 - l0) < 1e-2 * l0 and l1 != l0
+    assert np.linalg.norm(g1 - g0) < 5e-2 * np.linalg.norm(g0)
