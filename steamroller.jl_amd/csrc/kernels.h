@@ -898,6 +898,44 @@ __device__ __forceinline__ void load_tile(T *tile, int *tab, int *blks, const T 
     __syncthreads();
 }
 
+// k_interp_add on a level of complete sibling groups: the 2^D children of a
+// parent are one work group, which stages the parent (with its depth-1 face
+// halo: neighbour cells, mirrors, coarse-fine slabs of level l-1) in LDS and
+// interpolates every child cell from there. Same operations and order as
+// k_interp_add, so the result is bitwise the same.
+template <int D, int M, typename T, int NT>
+__global__ __launch_bounds__(NT) void k_interp_add_group(T *__restrict__ u, const T *__restrict__ u1,
+                                                         const T *__restrict__ ib1, const int *__restrict__ htab,
+                                                         const int *__restrict__ parent, int blk0,
+                                                         LevelArgs<T> ac) {
+    using LY = Lay<D, M>;
+    using TL = Tile<D, M>;
+    using HT = Halo<D>;
+    constexpr int NB = 1 << D, V = LY::V, MH = LY::MH;
+    static_assert((NB * V) % NT == 0, "cells of a group must be a multiple of the work-group size");
+    __shared__ T tile[TL::SZ];
+    __shared__ int tab[2 * D];
+    __shared__ int blks[1];
+    const int g0 = blk0 + blockIdx.x * NB;
+    load_tile<D, M, T, 1, NT>(tile, tab, blks, u1, ib1, htab, parent, g0, 1, ac.sg);
+#pragma unroll
+    for (int k = 0; k < NB * V / NT; k++) {
+        const int i = threadIdx.x + k * NT, lb = i / V, q = i - lb * V;
+        int x0, x1, x2;
+        LY::decode(q, x0, x1, x2);
+        const int J0 = (lb & 1) * MH + (x0 >> 1);
+        const int J1 = ((lb >> 1) & 1) * MH + (x1 >> 1);
+        const int J2 = (D == 3) ? ((lb >> 2) & 1) * MH + (x2 >> 1) : 0;
+        const int p = TL::at(J0, J1, J2);
+        T f = w0<D, T>() * tile[p];
+        f += w1<T>() * tile[p + (2 * (x0 & 1) - 1)];
+        f += w1<T>() * tile[p + (2 * (x1 & 1) - 1) * TL::stride(1)];
+        if (D == 3) f += w1<T>() * tile[p + (2 * (x2 & 1) - 1) * TL::stride(2)];
+        const size_t o = (size_t)(g0 + lb) * V + q;
+        u[o] = f + u[o];
+    }
+}
+
 enum { RES_WRITE = 0, RES_NORM = 1, RES_DEFECT = 2 };
 
 // Residual of a level from LDS tiles (src/multigrid.jl:309-323).
@@ -1009,20 +1047,22 @@ __global__ __launch_bounds__(WG) void k_grid_scatter(T *__restrict__ x, const do
 // Y = A X along one axis of a column-major grid: A is n x n (row-major), the
 // grid is viewed as [outer][n][inner] (inner = stride of the axis) and every
 // (outer, inner) pair is a column of X. Register-tiled FP64 GEMM: a work
-// group computes a 64 x 64 tile of Y (rows i, columns) with 256 threads, each
-// a 4 x 4 sub-tile, over k-chunks of 16 staged in LDS (zero-padded past n).
-// Each output is one dot product summed in k order.
+// group computes a 64 x TC tile of Y (rows i, columns) with 256 threads, each
+// an RA x 4 sub-tile (RA = TC / 16), over k-chunks of 16 staged in LDS
+// (zero-padded past n). Each output is one dot product summed in k order, so
+// every TC gives the same bits; narrow tiles give small grids more work groups.
+template <int TC>
 __global__ __launch_bounds__(256) void k_axis_gemm(const double *__restrict__ A, int n, int inner, int ncol,
                                                    const double *__restrict__ in, double *__restrict__ out) {
-    constexpr int TI = 64, TC = 64, TK = 16;
+    constexpr int TI = 64, TK = 16, RA = TC / 16, TCQ = TC / 4;
     __shared__ double As[TK][TI + 1];
     __shared__ double Xs[TK][TC + 1];
     const int tid = threadIdx.x;
     const int i0 = blockIdx.y * TI, c0 = blockIdx.x * TC;
-    const int ti = (tid / 16) * 4, tc = (tid % 16) * 4;
-    double acc[4][4];
+    const int ti = (tid / TCQ) * RA, tc = (tid % TCQ) * 4;
+    double acc[RA][4];
 #pragma unroll
-    for (int a = 0; a < 4; a++)
+    for (int a = 0; a < RA; a++)
 #pragma unroll
         for (int q = 0; q < 4; q++) acc[a][q] = 0.0;
     for (int k0 = 0; k0 < n; k0 += TK) {
@@ -1048,20 +1088,20 @@ __global__ __launch_bounds__(256) void k_axis_gemm(const double *__restrict__ A,
         __syncthreads();
 #pragma unroll
         for (int kk = 0; kk < TK; kk++) {
-            double av[4], xv[4];
+            double av[RA], xv[4];
 #pragma unroll
-            for (int a = 0; a < 4; a++) av[a] = As[kk][ti + a];
+            for (int a = 0; a < RA; a++) av[a] = As[kk][ti + a];
 #pragma unroll
             for (int q = 0; q < 4; q++) xv[q] = Xs[kk][tc + q];
 #pragma unroll
-            for (int a = 0; a < 4; a++)
+            for (int a = 0; a < RA; a++)
 #pragma unroll
                 for (int q = 0; q < 4; q++) acc[a][q] += av[a] * xv[q];
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int a = 0; a < 4; a++) {
+    for (int a = 0; a < RA; a++) {
         int gi = i0 + ti + a;
         if (gi >= n) continue;
 #pragma unroll

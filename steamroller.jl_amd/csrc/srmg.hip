@@ -600,6 +600,23 @@ static inline unsigned nwg(int64_t threads) { return (unsigned)((threads + WG - 
 
 static std::vector<double> read_norms(srmg_ctx *c);
 
+// one axis of the level-1 transforms; the column tile is narrowed until the
+// launch has at least 512 work groups (64-wide tiles for large batches)
+static void launch_axis_gemm(hipStream_t st, const double *A, int n, int inner, int ncol, const double *in,
+                             double *out) {
+    const unsigned gy = (unsigned)((n + 63) / 64);
+    auto groups = [&](int tc) { return (int64_t)((ncol + tc - 1) / tc) * gy; };
+    if (groups(64) >= 512)
+        hipLaunchKernelGGL(k_axis_gemm<64>, dim3((unsigned)((ncol + 63) / 64), gy), dim3(256), 0, st, A, n, inner,
+                           ncol, in, out);
+    else if (groups(32) >= 512)
+        hipLaunchKernelGGL(k_axis_gemm<32>, dim3((unsigned)((ncol + 31) / 32), gy), dim3(256), 0, st, A, n, inner,
+                           ncol, in, out);
+    else
+        hipLaunchKernelGGL(k_axis_gemm<16>, dim3((unsigned)((ncol + 15) / 16), gy), dim3(256), 0, st, A, n, inner,
+                           ncol, in, out);
+}
+
 template <int D, int M, typename T>
 struct Ops {
     using LY = Lay<D, M>;
@@ -778,6 +795,14 @@ struct Ops {
     void interp_add(FieldBuf &u, FieldBuf &u1, int l, const LevelArgs<T> &ac) {
         const LevelInfo &L = c->lv[l];
         if (!L.nblk) return;
+        if constexpr ((LY::V << D) % 256 == 0 && (LY::V << D) / 256 <= 16) {
+            if (L.grouped) {
+                hipLaunchKernelGGL((k_interp_add_group<D, M, T, 256>), dim3((unsigned)(L.nblk >> D)), dim3(256), 0,
+                                   c->stream, X(u, l), X(u1, l - 1), I(u1), c->d_halo.as<int>(),
+                                   c->d_parent.as<int>(), L.blk0, ac);
+                return;
+            }
+        }
         hipLaunchKernelGGL((k_interp_add<D, M, T>), dim3(nwg((int64_t)L.nblk * LY::V)), dim3(WG), 0, c->stream,
                            X(u, l), X(u1, l - 1), I(u1), nbr(), c->d_parent.as<int>(), c->d_sub.as<int>(), L.blk0,
                            L.nblk, ac);
@@ -835,8 +860,7 @@ struct Ops {
     void coarse_exact(FieldBuf &u, FieldBuf &b, double s, const srmg_op *op);
     // out = A in along an axis of extent n and stride `inner` (ncol columns)
     void axis(const double *A, int n, int inner, int ncol, const double *in, double *out) {
-        dim3 grid((unsigned)((ncol + 63) / 64), (unsigned)((n + 63) / 64));
-        hipLaunchKernelGGL(k_axis_gemm, grid, dim3(256), 0, c->stream, A, n, inner, ncol, in, out);
+        launch_axis_gemm(c->stream, A, n, inner, ncol, in, out);
     }
 
     void top(FieldBuf &u, FieldBuf &b, double s, const srmg_op *op, int ntop, double omega) {
@@ -1930,9 +1954,7 @@ int srmg_grid_solve(srmg_ctx *c, int D, const int *n, double s, const srmg_op *o
         const double *in = f;
         double *a = g1.as<double>(), *b2 = g2.as<double>();
         auto axis = [&](const DevBuf &A, int d, int stride, const double *x, double *y) {
-            dim3 grid((unsigned)((ngrid * total / N[d] + 63) / 64), (unsigned)((N[d] + 63) / 64));
-            hipLaunchKernelGGL(k_axis_gemm, grid, dim3(256), 0, c->stream, A.as<double>(), N[d], stride,
-                               (int)(ngrid * total / N[d]), x, y);
+            launch_axis_gemm(c->stream, A.as<double>(), N[d], stride, (int)(ngrid * total / N[d]), x, y);
         };
         int stride = 1;
         for (int d = 0; d < D; d++) {
