@@ -265,6 +265,9 @@ int srmg_autotune(srmg_ctx *ctx, int u, int b, double s, const srmg_op *op, int 
 int srmg_set_sweep_config(srmg_ctx *ctx, int dtype, int level, int cfg);
 /* 1 when the level's blocks form complete sibling groups (tile shapes 4..6 apply). */
 int srmg_level_grouped(srmg_ctx *ctx, int level, int *grouped);
+/* Use (default) or bypass the sibling-group forms of the sweep, residual and
+ * prolongation kernels on grouped levels; both give the same bits. */
+int srmg_set_group_tiles(srmg_ctx *ctx, int enable);
 
 /* Blocks, leaf blocks and coarse-fine ghost slabs of a level (1-based). */
 int srmg_level_info(srmg_ctx *ctx, int level, int64_t *nblocks, int64_t *nleaf, int64_t *nslots);

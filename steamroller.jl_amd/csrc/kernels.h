@@ -1002,6 +1002,84 @@ __global__ __launch_bounds__(NT) void k_resid_tile(TO *__restrict__ out, const T
     }
 }
 
+// k_resid_tile (RES_WRITE, RES_DEFECT) on a level of complete sibling groups:
+// one group of 2^D blocks per work group of 1024 threads, staged as one
+// (2M)^D tile with the halo of the group's exterior faces only. Teams of RT
+// threads take one block at a time and visit its cells in the order of
+// k_resid_tile<.., 1, RT, ..> (cells t, t + RT, ...; partials as the same
+// RT-wide tree sum), so outputs and partials are bitwise the same.
+template <int D, int M, typename T, typename TO, int RT, int MODE, int RD>
+__global__ __launch_bounds__(1024) void k_resid_group(TO *__restrict__ out, const T *__restrict__ u,
+                                                      const T *__restrict__ b, const T *__restrict__ ib,
+                                                      const int *__restrict__ htab, const int *__restrict__ coord,
+                                                      const int *__restrict__ isleaf, LevelArgs<T> a, double wgt,
+                                                      double *__restrict__ partial) {
+    using LY = Lay<D, M>;
+    using TL = Tile<D, M, 2>;
+    using HT = Halo<D>;
+    constexpr int NT = 1024, NB = 1 << D, V = LY::V, F = LY::F;
+    constexpr int TEAMS = NT / RT;
+    static_assert(NB % TEAMS == 0 && RT >= 64, "team layout");
+    __shared__ T tile[TL::SZ];
+    __shared__ int tab[NB * 2 * D];
+    __shared__ BP<T> prm[NB];
+    __shared__ double red[MODE == RES_DEFECT ? NT : 1];
+    const int gl = xcd_remap(blockIdx.x, gridDim.x);
+    const int g0 = a.blk0 + gl * NB;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < NB * V; i += NT) {
+        const int lb = i / V, q = i - lb * V;
+        int x, y, z;
+        LY::decode(q, x, y, z);
+        tile[TL::origin(lb) + TL::at(x, y, z)] = u[(size_t)(g0 + lb) * V + q];
+    }
+    for (int i = tid; i < NB * 2 * D; i += NT) {
+        const int lb = i / (2 * D);
+        tab[i] = htab[(size_t)(g0 + lb) * HT::N + (i - lb * 2 * D)];
+    }
+    if (tid < NB) prm[tid] = bparams(a, g0 + tid);
+    __syncthreads();
+    for (int i = tid; i < NB * D * F; i += NT) {
+        const int lb = i / (D * F), rem = i - lb * (D * F);
+        const int d = rem / F, t = rem - d * F;
+        const int f = 2 * d + ((lb >> d) & 1);
+        halo_switch<D, M, T, TL>(tile + TL::origin(lb), tab[lb * 2 * D + f], f, g0 + lb, t, u, ib, a.sg);
+    }
+    __syncthreads();
+    const int team = tid / RT, t = tid - team * RT;
+    for (int round = 0; round < NB / TEAMS; round++) {
+        const int lb = team + round * TEAMS, blk = g0 + lb;
+        const BP<T> bp = prm[lb];
+        const bool leaf = (MODE == RES_DEFECT) && isleaf[blk];
+        double acc = 0.0;
+        for (int q = t; q < V; q += RT) {
+            int x, y, z;
+            LY::decode(q, x, y, z);
+            const int p = TL::origin(lb) + TL::at(x, y, z);
+            T wm, wp;
+            rweights<D, M, T, RD>(a, coord, blk, RD == 0 ? x : y, wm, wp);
+            T Lu = tstencil<D, M, T, RD, TL>(tile, p, tile[p], bp.cc, wm, wp);
+            const size_t o = (size_t)blk * V + q;
+            T rr = -(b[o] + Lu / bp.s);
+            if (MODE == RES_WRITE) out[o] = (TO)rr;
+            if (MODE == RES_DEFECT) {
+                out[o] = (TO)(b[o] + Lu / bp.s);
+                if (leaf) acc += (double)rr * (double)rr;
+            }
+        }
+        if (MODE == RES_DEFECT) {
+            red[tid] = acc;
+            __syncthreads();
+            for (int w = RT / 2; w > 0; w >>= 1) {
+                if (t < w) red[tid] += red[tid + w];
+                __syncthreads();
+            }
+            if (t == 0) partial[(blk - a.blk0)] = red[tid] * wgt;
+            __syncthreads();
+        }
+    }
+}
+
 // ---- exact level-1 solve (FP64) ----
 
 // level-1 valid cells -> uniform grid (column-major, dimension 1 fastest), times c

@@ -104,6 +104,7 @@ struct srmg_ctx {
     std::vector<std::unique_ptr<FieldBuf>> fields;
     std::unique_ptr<FieldBuf> zero[2];  // all-zero scratch fields (F64, F32), made on demand
     srmg_unet *guess_net = nullptr;     // network of the learned first iterate (not owned)
+    bool group_tiles = true;            // sibling-group kernels on grouped levels
     int guess_fallbacks = 0;            // trees of the last solve that started from zero instead
     hipStream_t stream = nullptr;
     int maxpartials = 0;
@@ -676,6 +677,7 @@ struct Ops {
     // sibling-group tiles (k_gs_sweep_group) where a group's tile fits in LDS
     static constexpr bool GROUP_OK = (D == 3 && M == 8) || (D == 2 && M == 16);
     static constexpr int NCFG = GROUP_OK ? 7 : 4;
+    bool grouped(int l) const { return c->group_tiles && c->lv[l].grouped; }
     template <int K> static constexpr int cfg_bpw() { return SMALL ? BPW0 * (K == 0 ? 1 : (K == 3 ? 4 : 2)) : 1; }
     template <int K> static constexpr int cfg_nt() {
         return SMALL ? (K < 2 ? 256 : 512) : (K == 0 ? 256 : (K == 2 ? 1024 : 512));
@@ -712,7 +714,7 @@ struct Ops {
         T *in = X(u, l);
         T *out = u.cur[l] ? u.data.as<T>() : u.alt.as<T>();
         if (cfg < 0) cfg = c->sweep_cfg[(std::is_same<T, double>::value ? 0 : c->nlevels) + l];
-        if (cfg >= 4 && !(GROUP_OK && c->lv[l].grouped)) cfg = 0;
+        if (cfg >= 4 && !(GROUP_OK && grouped(l))) cfg = 0;
         switch (cfg) {
             case 1: sweep_launch<1>(out, in, u, b, a, l); break;
             case 2: sweep_launch<2>(out, in, u, b, a, l); break;
@@ -733,6 +735,18 @@ struct Ops {
     static constexpr int RNT = SMALL ? 256 : 512;
     void residual(FieldBuf &r, FieldBuf &u, FieldBuf &b, const LevelArgs<T> &a, int l) {
         if (!a.nblk) return;
+        if constexpr (GROUP_OK) {
+            if (grouped(l)) {
+                with_rd(a.rdim, [&](auto rd) {
+                    constexpr int RD = decltype(rd)::value;
+                    hipLaunchKernelGGL((k_resid_group<D, M, T, T, RNT, RES_WRITE, RD>), dim3((unsigned)(a.nblk >> D)),
+                                       dim3(1024), 0, c->stream, X(r, l), X(u, l), X(b, l), I(u),
+                                       c->d_halo.as<int>(), coord(), (const int *)nullptr, a, 0.0,
+                                       (double *)nullptr);
+                });
+                return;
+            }
+        }
         unsigned g = (unsigned)((a.nblk + RB - 1) / RB);
         with_rd(a.rdim, [&](auto rd) {
             constexpr int RD = decltype(rd)::value;
@@ -796,7 +810,7 @@ struct Ops {
         const LevelInfo &L = c->lv[l];
         if (!L.nblk) return;
         if constexpr ((LY::V << D) % 256 == 0 && (LY::V << D) / 256 <= 16) {
-            if (L.grouped) {
+            if (grouped(l)) {
                 hipLaunchKernelGGL((k_interp_add_group<D, M, T, 256>), dim3((unsigned)(L.nblk >> D)), dim3(256), 0,
                                    c->stream, X(u, l), X(u1, l - 1), I(u1), c->d_halo.as<int>(),
                                    c->d_parent.as<int>(), L.blk0, ac);
@@ -844,8 +858,19 @@ struct Ops {
             if (!a.nblk) continue;
             double wgt = std::ldexp(1.0, -c->D * l);
             unsigned g = (unsigned)((a.nblk + RB - 1) / RB);
+            bool grouped = false;
+            if constexpr (GROUP_OK) grouped = this->grouped(l);
             with_rd(a.rdim, [&](auto rd) {
                 constexpr int RD = decltype(rd)::value;
+                if constexpr (GROUP_OK) {
+                    if (grouped) {
+                        hipLaunchKernelGGL((k_resid_group<D, M, T, float, RNT, RES_DEFECT, RD>),
+                                           dim3((unsigned)(a.nblk >> D)), dim3(1024), 0, c->stream, be, X(u, l),
+                                           X(b, l), I(u), c->d_halo.as<int>(), coord(), c->d_isleaf.as<int>(), a,
+                                           wgt, c->d_partial.as<double>() + np);
+                        return;
+                    }
+                }
                 hipLaunchKernelGGL((k_resid_tile<D, M, T, float, RB, RNT, RES_DEFECT, RD>), dim3(g), dim3(RNT), 0,
                                    c->stream, be, X(u, l), X(b, l), I(u), c->d_halo.as<int>(), coord(),
                                    (const int *)nullptr, c->d_isleaf.as<int>(), a.nblk, a, wgt,
@@ -1878,6 +1903,13 @@ int srmg_set_sweep_config(srmg_ctx *c, int dtype, int level, int cfg) {
         if (dtype != SRMG_F64 && dtype != SRMG_F32) fail("bad dtype");
         if (cfg < 0 || cfg > 6) fail("sweep configuration must be 0..6");
         c->sweep_cfg[(dtype == SRMG_F64 ? 0 : c->nlevels) + level - 1] = cfg;
+    });
+}
+
+int srmg_set_group_tiles(srmg_ctx *c, int enable) {
+    return guarded([&] {
+        if (!c) fail("null context");
+        c->group_tiles = enable != 0;
     });
 }
 

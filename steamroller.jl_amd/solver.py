@@ -143,6 +143,7 @@ def lib():
         "srmg_autotune": [P, i32, i32, dbl, P, i32, P],
         "srmg_set_sweep_config": [P, i32, i32, i32],
         "srmg_level_grouped": [P, i32, C.POINTER(i32)],
+        "srmg_set_group_tiles": [P, i32],
         "srmg_set_guess": [P, P],
         "srmg_guess_fallbacks": [P, C.POINTER(i32)],
         "srmg_grid_solve": [P, i32, P, dbl, P, P, P, i32],
@@ -281,6 +282,9 @@ class Context:
         g = C.c_int()
         _check(lib().srmg_level_grouped(self.h, int(level), C.byref(g)))
         return bool(g.value)
+
+    def set_group_tiles(self, enable: bool):
+        _check(lib().srmg_set_group_tiles(self.h, int(bool(enable))))
 
     def set_sweep_config(self, dtype: int, level: int, cfg: int):
         _check(lib().srmg_set_sweep_config(self.h, dtype, level, cfg))

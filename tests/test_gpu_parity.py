@@ -261,3 +261,36 @@ def test_every_sweep_tile_shape_gives_identical_results(name, dtype):
         outs.append(_valid(host, p).copy())
     for o in outs[1:]:
         np.testing.assert_array_equal(o, outs[0])
+
+
+@pytest.mark.parametrize("name", ["cfg1", "cart3-m8", "cfg4-small"])
+def test_sibling_group_kernels_give_identical_solves(name):
+    p = configs.make_cfg4(0, scale=0.35, levels=3) if name == "cfg4-small" else CASES[name]()
+    geom, disc = solver.geometry_from_code(p.geom), _disc(p)
+    res = []
+    for on in (False, True):
+        ctx = solver.Context(p.tree, p.conn, p.M)
+        ctx.set_group_tiles(on)
+        if on:
+            print(f"{name}: grouped levels {[ctx.level_grouped(l) for l in range(1, len(p.tree) + 1)]}", flush=True)
+            for l in range(1, len(p.tree) + 1):
+                ctx.set_sweep_config(F64, l, 5)
+                ctx.set_sweep_config(F32, l, 6)
+        u, b, r, u1 = (ctx.field(F64) for _ in range(4))
+        b.upload(p.b)
+        n, h = solver.solve(u, b, r, u1, p.s, geom, p.bc, disc, tol=1e-9, max_cycles=12)
+        v, b2 = ctx.field(F64), ctx.field(F64)
+        b2.upload(p.b)
+        e, be, re, ue = (ctx.field(F32) for _ in range(4))
+        nm, hm = solver.solve_mixed(v, b2, e, be, re, ue, p.s, geom, p.bc, disc, tol=1e-9, max_cycles=12,
+                                    inner_cycles=2)
+        outs = []
+        for f in (u, v):
+            host = p.b.like()
+            f.download(host)
+            outs.append(_valid(host, p).copy())
+        res.append((h, hm, outs))
+    np.testing.assert_array_equal(res[1][0], res[0][0])
+    np.testing.assert_array_equal(res[1][1], res[0][1])
+    np.testing.assert_array_equal(res[1][2][0], res[0][2][0])
+    np.testing.assert_array_equal(res[1][2][1], res[0][2][1])
