@@ -6,10 +6,14 @@ blocks, 5 levels, about 31.7M stored / 27.8M leaf cells, three seeded branching
 Gaussian filaments) from u = 0 to resnorm(u) <= 1e-8 resnorm(0), with the
 right-hand side already resident in HBM. Two forms are timed: plain FP64
 V-cycles, and FP64 defect correction around FP32 V-cycles (FP64 residual,
-update and coarsest-level solve). The smoother kernel's average launch time is
-measured with HIP events on the solver's stream for the roofline figure. The
-CPU baseline is the oracle (a C restatement of the reference algorithm) on a
-bounded sample of the same workload.
+update and coarsest-level solve) with 1, 2 or 3 FP32 cycles per correction,
+from u = 0 and from the learned level-1 first iterate (the package's trained
+network). The headline is the fastest FP32 form. The smoother kernel's average
+launch time is measured with HIP events on the solver's stream for the
+roofline figure; its HBM traffic per launch comes from the committed counter
+summary when that summary is for the same kernel. The CPU baseline is the
+oracle (a C restatement of the reference algorithm) on a bounded sample of the
+same workload.
 
 Prints one JSON line.
 """
@@ -86,18 +90,30 @@ def main():
         b.upload(p.b)
         u.zero()
 
+    # the learned level-1 first iterate (trained for this workload's level 1)
+    net = None
+    if args.config == "cfg4":
+        from srmg import guess
+        if os.path.exists(guess.cfg4_weights_path()):
+            net = guess.load(guess.cfg4_weights_path())
+            solver.set_guess(ctx, net)
+
+    specs = {"fp64": (False, 1, solver.GUESS_NONE), "mixed": (True, 1, solver.GUESS_NONE),
+             "mixed2": (True, 2, solver.GUESS_NONE), "mixed3": (True, 3, solver.GUESS_NONE)}
+    if net is not None:
+        specs["mixed2_learned"] = (True, 2, solver.GUESS_LEARNED)
     forms = {}
-    for name in ("fp64", "mixed", "mixed2"):
+    for name, (mixed, inner, gmode) in specs.items():
         times, cycles = [], []
         for it in range(args.warmup + args.steps):
             reset()
             ctx.synchronize()
             ts = time.perf_counter()
-            if name == "fp64":
-                n, hist = solver.solve(u, b, r, u1, p.s, geom, p.bc, disc, tol=1e-8, max_cycles=60)
+            if not mixed:
+                n, hist = solver.solve(u, b, r, u1, p.s, geom, p.bc, disc, tol=1e-8, max_cycles=60, guess=gmode)
             else:
                 n, hist = solver.solve_mixed(u, b, e, be, re, ue, p.s, geom, p.bc, disc, tol=1e-8,
-                                             max_cycles=60, inner_cycles=2 if name == "mixed2" else 1)
+                                             max_cycles=60, inner_cycles=inner, guess=gmode)
             ctx.synchronize()
             dt = time.perf_counter() - ts
             if it >= args.warmup:
@@ -106,12 +122,15 @@ def main():
             if hist[-1] > 1e-8:
                 raise RuntimeError(f"{name} did not converge: {hist}")
         forms[name] = {"ms": 1e3 * float(np.mean(times)), "cycles": int(cycles[-1]),
-                       "mcells_s": n_leaf / float(np.mean(times)) / 1e6, "final_rel_residual": float(hist[-1])}
+                       "mcells_s": n_leaf / float(np.mean(times)) / 1e6, "final_rel_residual": float(hist[-1]),
+                       "first_rel_residual": float(hist[0])}
+        if gmode != solver.GUESS_NONE:
+            forms[name]["guess_fallbacks"] = solver.guess_fallbacks(ctx)
         log(f"[bench] {name}: {forms[name]}")
         if name == "fp64":
             ref = p.b.like()
             u.download(ref)
-        if name.startswith("mixed"):
+        else:
             # potential error of the mixed form against the FP64 solution
             got = p.b.like()
             u.download(got)
@@ -119,11 +138,13 @@ def main():
             vs = (slice(G, G + M),) * p.D
             forms[name]["potential_rel_err_vs_fp64"] = float(
                 np.max(np.abs(got.u[vs] - ref.u[vs])) / np.max(np.abs(ref.u[vs])))
-    best = min(("mixed", "mixed2"), key=lambda k: forms[k]["ms"])
+    best = min((k for k in forms if k != "fp64"), key=lambda k: forms[k]["ms"])
 
     # roofline: the smoother pass on the finest level, FP64 and FP32
     L = len(p.tree)
     nblk, _, _ = ctx.level_info(L)
+    kname = {t: sweep_kernel_name(p.D, p.M, t, int(tuned[L - 1]), ctx.level_grouped(L))
+             for t, tuned in (("double", tuned64), ("float", tuned32))}
     cells = nblk * p.M ** p.D
     roof = {}
     for name, (uu, bb, rr, w) in {"fp64": (u, b, r, 8), "fp32": (e, be, re, 4)}.items():
@@ -139,7 +160,12 @@ def main():
         cpu = cpu_baseline(p, forms["fp64"]["cycles"], n_leaf, args.cpu_threads)
         log(f"[bench] cpu: {cpu}")
 
+    traffic = committed_traffic(kname["float"], args.config)
     ms = forms[best]["ms"]
+
+    def rounded(d):
+        return {k: (float(f"{v:.5g}") if isinstance(v, float) else v) for k, v in d.items()}
+
     out = {
         "metric": "Poisson solve to 1e-8 residual: Mcells/s, plain FP64 vs FP32+learned; HBM %",
         "value": round(forms[best]["mcells_s"], 3),
@@ -157,23 +183,20 @@ def main():
                                f"{n_stored} stored / {n_leaf} leaf cells, to 1e-8 relative residual",
                    "blocks": p.tree.nblocks(), "leaf_cells": n_leaf, "stored_cells": n_stored,
                    "seed": args.seed, "vcycle": "(ndown, ntop, nup) = (2, 4, 2), exact FP64 level-1 solve"},
-        "forms": {
-            "fp64": {k: (float(f"{v:.5g}") if isinstance(v, float) else v) for k, v in forms["fp64"].items()},
-            "fp32_mixed": {k: (float(f"{v:.5g}") if isinstance(v, float) else v) for k, v in forms["mixed"].items()},
-            "fp32_mixed_2inner": {k: (float(f"{v:.5g}") if isinstance(v, float) else v)
-                                  for k, v in forms["mixed2"].items()},
-            "headline_form": best,
-            "speedup_vs_fp64": round(forms["fp64"]["ms"] / forms[best]["ms"], 3),
-        },
+        "forms": dict({k: rounded(v) for k, v in forms.items()},
+                      headline_form=best,
+                      speedup_vs_fp64=round(forms["fp64"]["ms"] / forms[best]["ms"], 3),
+                      note="mixedK: FP64 defect correction around K FP32 V-cycles; _learned: first iterate "
+                           "from the network's level-1 prediction (level-1 smoothing, zero fallback)"),
         "roofline": {
             "bound": "hbm",
-            "kernel": f"k_gs_sweep<3,8,float> (level {L}, {nblk} blocks)",
+            "kernel": f"{kname['float']} (level {L}, {nblk} blocks)",
             "achieved": round(roof["fp32"]["gbs"], 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(roof["fp32"]["gbs"] / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "fp64_kernel": {"achieved": round(roof["fp64"]["gbs"], 1),
+            "traffic": traffic,
+            "fp64_kernel": {"kernel": kname["double"], "achieved": round(roof["fp64"]["gbs"], 1),
                             "frac": round(roof["fp64"]["gbs"] / HBM_PEAK_GBS, 4),
                             "ms": round(roof["fp64"]["ms"], 5)},
             "ms_per_launch": round(roof["fp32"]["ms"], 5),
@@ -184,6 +207,32 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def sweep_kernel_name(D, M, t, cfg, grouped):
+    """Name of the sweep kernel a tile shape launches (as the profiler lists it)."""
+    if cfg >= 4 and grouped and ((D, M) in ((3, 8), (2, 16))):
+        nt = {4: 256, 5: 512, 6: 1024}[cfg]
+        if ((1 << D) * M ** D // 2) % nt == 0:
+            return f"k_gs_sweep_group<{D}, {M}, {t}, {nt}, -1>"
+    small = (M ** D // 2) <= 256
+    bpw0 = 256 // (M ** D // 2) if small else 1
+    k = cfg if cfg < 4 else 0
+    bpw = bpw0 * (1 if k == 0 else (4 if k == 3 else 2)) if small else 1
+    nt = (256 if k < 2 else 512) if small else (256 if k == 0 else (1024 if k == 2 else 512))
+    return f"k_gs_sweep<{D}, {M}, {t}, {bpw}, {nt}, -1>"
+
+
+def committed_traffic(kernel, config):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (profiles/), or None when the summary is for another kernel or workload."""
+    path = os.path.join(ROOT, "profiles", "sweep_traffic_cfg4.json")
+    if config != "cfg4" or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    k = d.get("kernels", {}).get(kernel)
+    return None if k is None else k.get("traffic_bytes")
 
 
 def cpu_baseline(p, cycles, n_leaf, threads):
