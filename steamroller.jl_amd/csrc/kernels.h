@@ -1192,6 +1192,65 @@ __global__ __launch_bounds__(256) void k_axis_gemm(const double *__restrict__ A,
     }
 }
 
+// k_axis_gemm for n <= 64 in one load phase: all of A (n x n) and the n x TC
+// column tile of X in LDS behind one barrier, then every thread sums its
+// 4 outputs over k = 0 .. n-1 in order (the same bits as k_axis_gemm).
+template <int TC>
+__global__ __launch_bounds__(256) void k_axis_gemm_small(const double *__restrict__ A, int n, int inner, int ncol,
+                                                         const double *__restrict__ in, double *__restrict__ out) {
+    constexpr int NMAX = 64, RA = TC / 16, TCQ = TC / 4;
+    __shared__ double As[NMAX][NMAX + 1];   // As[k][i] = A[i][k]
+    __shared__ double Xs[NMAX][TC + 1];
+    const int tid = threadIdx.x;
+    const int c0 = blockIdx.x * TC;
+    for (int e = tid; e < n * n; e += 256) {
+        const int i = e / n, k = e - i * n;
+        As[k][i] = A[e];
+    }
+    for (int e = tid; e < n * TC; e += 256) {
+        int kk, cc;
+        if (inner == 1) { cc = e / n; kk = e - cc * n; }   // k contiguous per column
+        else { kk = e / TC; cc = e - kk * TC; }            // columns contiguous
+        const int col = c0 + cc;
+        double v = 0.0;
+        if (col < ncol) {
+            const size_t o = (size_t)(col / inner), j = (size_t)(col % inner);
+            v = in[o * (size_t)n * inner + (size_t)kk * inner + j];
+        }
+        Xs[kk][cc] = v;
+    }
+    __syncthreads();
+    const int ti = (tid / TCQ) * RA, tc = (tid % TCQ) * 4;
+    double acc[RA][4];
+#pragma unroll
+    for (int a = 0; a < RA; a++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) acc[a][q] = 0.0;
+    for (int k = 0; k < n; k++) {
+        double av[RA], xv[4];
+#pragma unroll
+        for (int a = 0; a < RA; a++) av[a] = As[k][ti + a];
+#pragma unroll
+        for (int q = 0; q < 4; q++) xv[q] = Xs[k][tc + q];
+#pragma unroll
+        for (int a = 0; a < RA; a++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) acc[a][q] += av[a] * xv[q];
+    }
+#pragma unroll
+    for (int a = 0; a < RA; a++) {
+        const int gi = ti + a;
+        if (gi >= n) continue;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int col = c0 + tc + q;
+            if (col >= ncol) continue;
+            const size_t o = (size_t)(col / inner), j = (size_t)(col % inner);
+            out[o * (size_t)n * inner + (size_t)gi * inner + j] = acc[a][q];
+        }
+    }
+}
+
 // divide by the eigenvalues of the Kronecker-sum operator; the grid holds
 // `ngrid` stacked level-1 grids of `total` points, grid g with shift[g]
 // (Helmholtz: -s k^2; Laplacian: 0)

@@ -605,6 +605,12 @@ static std::vector<double> read_norms(srmg_ctx *c);
 // launch has at least 512 work groups (64-wide tiles for large batches)
 static void launch_axis_gemm(hipStream_t st, const double *A, int n, int inner, int ncol, const double *in,
                              double *out) {
+    if (n <= 64) {
+        // one load phase: A and the column tile in LDS (16 columns per work group)
+        hipLaunchKernelGGL(k_axis_gemm_small<16>, dim3((unsigned)((ncol + 15) / 16)), dim3(256), 0, st, A, n, inner,
+                           ncol, in, out);
+        return;
+    }
     const unsigned gy = (unsigned)((n + 63) / 64);
     auto groups = [&](int tc) { return (int64_t)((ncol + tc - 1) / tc) * gy; };
     if (groups(64) >= 512)
