@@ -196,6 +196,19 @@ int srmg_solve_mixed(srmg_ctx *ctx, int u, int b, int e, int be, int re, int ue,
 int srmg_solve_mixed_batched(srmg_ctx *ctx, int u, int b, int e, int be, int re, int ue, const double *s,
                              const srmg_op *ops, const srmg_solve_opts *opts, int *cycles, double *hist);
 
+/* ---- electric field (src/multigrid.jl:421-451, caller src/streamer.jl:340-351) ---- */
+
+/* E = -grad u + e0 on the cell faces and |E| at the cell centres of every block
+ * of every level, for a constant external field e0[D] and root cell size h
+ * (level l uses h / 2^(l-1)). u (FP64) must hold current coarse-fine ghost
+ * slabs (true after any solve or upload); bc gives its boundary ghosts (the
+ * Poisson bc). |E| goes to the FP64 field eabs (valid cells). If e_host is not
+ * NULL, E is written to it in the reference's VectorBlockField{D,M,G} layout
+ * ((M+2G+1)^D x D x nblocks, column-major; only the valid faces
+ * G+1..G+M+1 along the component's dimension are written). */
+int srmg_electric_field(srmg_ctx *ctx, int u, int eabs, double h, const double *e0, const int8_t *bc,
+                        double *e_host, int G);
+
 /* ---- learned first iterate ---- */
 
 /* Attach (net != NULL) or detach the network used by SRMG_GUESS_LEARNED. The

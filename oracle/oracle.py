@@ -60,6 +60,8 @@ def lib():
         _LIB.ora_resnorm.restype = dbl
         _LIB.ora_coarse_solve.argtypes = [P, P, P, dbl, P]
         _LIB.ora_coarse_solve.restype = i32
+        _LIB.ora_electric_field.argtypes = [P, P, P, P, dbl, P]
+        _LIB.ora_electric_field.restype = None
     return _LIB
 
 
@@ -171,6 +173,18 @@ class Oracle:
     def resnorm(self, r, b, u, s) -> float:
         return lib().ora_resnorm(self.h, _ptr(self._a(r)), _ptr(self._a(b)), _ptr(self._a(u)),
                                  s, self._opp())
+
+    def electric_field(self, u, h, e0):
+        """electric_field! for a constant external field: (E in the
+        VectorBlockField layout, |E| as a host field array). u's ghosts must be
+        filled (fill_ghost_all)."""
+        a = self._a(u)
+        S = self.M + 2 * self.G + 1
+        f = np.zeros((S,) * self.D + (self.D, self.nblocks), order="F")
+        fabs = np.zeros_like(a)
+        e = np.ascontiguousarray(np.asarray(e0, np.float64))
+        lib().ora_electric_field(self.h, _ptr(f), _ptr(fabs), _ptr(a), float(h), _ptr(e))
+        return f, fabs
 
     def coarse_solve(self, u, b, s) -> int:
         return lib().ora_coarse_solve(self.h, _ptr(self._a(u)), _ptr(self._a(b)), s, self._opp())

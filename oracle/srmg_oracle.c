@@ -775,3 +775,71 @@ double ora_resnorm(const ora_tree *t, double *r, const double *b, const double *
 void ora_fill_ghost_all(const ora_tree *t, double *u, const int8_t *bc) {
     for (int l = 1; l <= t->nlevels; l++) ora_fill_ghost(t, u, l, bc);
 }
+
+/* ---------------------------------------------------------------------- */
+/* electric_field! (src/multigrid.jl:421-449) with a constant external     */
+/* field e0, the method of src/multigrid.jl:451                             */
+/* ---------------------------------------------------------------------- */
+
+/* For every block of every level (the @bkernel loop), with h_l = h / 2^(l-1):
+ *   f[I, d, blk] = (u[I - e_d, blk] - u[I, blk]) / h_l + e0[d]
+ * for I in validindices(f, d) (src/blockfield.jl:403-405: G+1..G+M+1 along d,
+ * G+1..G+M elsewhere), then for I in validindices(fabs)
+ *   fabs[I, blk] = sqrt(sum_d (f[I, d, blk] + f[I + e_d, d, blk])^2 / 4).
+ * f is a VectorBlockField{D,M,G} with contiguous storage
+ * (src/blockfield.jl:88-95): (M+2G+1)^D x D x nblocks, column-major. u must have
+ * its ghosts filled as at src/streamer.jl:340-349. */
+void ora_electric_field(const ora_tree *t, double *f, double *fabs, const double *u, double h,
+                        const double *e0) {
+    const int D = t->D, M = t->M, G = t->G;
+    const int64_t Sv = M + 2 * G + 1;
+    int64_t SvD = 1;
+    for (int d = 0; d < D; d++) SvD *= Sv;
+    for (int l = 1; l <= t->nlevels; l++) {
+        const double hl = h / (double)(1 << (l - 1));
+        for (int64_t k = t->pairs_off[l - 1]; k < t->pairs_off[l]; k++) {
+            const int64_t blk = t->pairs[k * (D + 1) + D];
+            double *fb = f + (blk - 1) * D * SvD;
+            for (int d = 0; d < D; d++) {
+                int lo[3] = {1, 1, 1}, hi[3] = {1, 1, 1};
+                for (int e = 0; e < D; e++) {
+                    lo[e] = G + 1;
+                    hi[e] = (e == d) ? G + M + 1 : G + M;
+                }
+                int I[3];
+                for (I[2] = lo[2]; I[2] <= hi[2]; I[2]++)
+                    for (I[1] = lo[1]; I[1] <= hi[1]; I[1]++)
+                        for (I[0] = lo[0]; I[0] <= hi[0]; I[0]++) {
+                            int I1[3] = {I[0], I[1], I[2]};
+                            I1[d] -= 1;
+                            int64_t fo = 0, st = 1;
+                            for (int e = 0; e < D; e++) {
+                                fo += (int64_t)(I[e] - 1) * st;
+                                st *= Sv;
+                            }
+                            fb[fo + d * SvD] = (u[OFF(t, I1, blk)] - u[OFF(t, I, blk)]) / hl + e0[d];
+                        }
+            }
+            int I[3] = {1, 1, 1};
+            int lo = G + 1, hiv = G + M;
+            for (I[2] = (D == 3 ? lo : 1); I[2] <= (D == 3 ? hiv : 1); I[2]++)
+                for (I[1] = lo; I[1] <= hiv; I[1]++)
+                    for (I[0] = lo; I[0] <= hiv; I[0]++) {
+                        double f2 = 0.0;
+                        for (int d = 0; d < D; d++) {
+                            int64_t fo = 0, fo1, st = 1;
+                            for (int e = 0; e < D; e++) {
+                                fo += (int64_t)(I[e] - 1) * st;
+                                st *= Sv;
+                            }
+                            st = 1;
+                            for (int e = 0; e < d; e++) st *= Sv;
+                            fo1 = fo + st;
+                            const double a = fb[fo + d * SvD] + fb[fo1 + d * SvD];
+                            f2 += a * a / 4.0;
+                        }
+                        fabs[OFF(t, I, blk)] = sqrt(f2);
+                    }
+        }
+    }
+}

@@ -1080,6 +1080,64 @@ __global__ __launch_bounds__(1024) void k_resid_group(TO *__restrict__ out, cons
     }
 }
 
+// ---- electric field ----
+
+template <typename T>
+struct EFieldArgs {
+    int blk0;    // first device block of the level
+    T hl;        // cell size of the level: h / 2^(l-1)
+    T e0[3];     // constant external field
+    T sg[6];     // boundary ghost signs of u
+};
+
+// electric_field! (src/multigrid.jl:421-449) with a constant external field
+// (src/multigrid.jl:451) on one block per work group: u with its depth-1 halo
+// in LDS (the ghosts of the post-solve fills, src/streamer.jl:340-349);
+// E_d on the M+1 faces of every line along d, (u[c - e_d] - u[c]) / h_l + e0_d
+// (face i is the lower face of cell i), stored as ef[blk][d][i + (M+1) o]
+// with o the other coordinates in increasing dimension order; |E| at the cell
+// centres, sqrt(sum_d (E_d(lower) + E_d(upper))^2 / 4), in the block layout.
+template <int D, int M, typename T>
+__global__ __launch_bounds__(256) void k_efield(T *__restrict__ eabs, T *__restrict__ ef, const T *__restrict__ u,
+                                                const T *__restrict__ ib, const int *__restrict__ htab,
+                                                EFieldArgs<T> ea) {
+    using LY = Lay<D, M>;
+    using TL = Tile<D, M>;
+    constexpr int V = LY::V, NF = (M + 1) * (D == 2 ? M : M * M);
+    __shared__ T tile[TL::SZ];
+    __shared__ int tab[2 * D];
+    __shared__ int blks[1];
+    const int blk = ea.blk0 + blockIdx.x;
+    load_tile<D, M, T, 1, 256>(tile, tab, blks, u, ib, htab, nullptr, blk, 1, ea.sg);
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        const int da = d == 0 ? 1 : 0, db = d == 2 ? 1 : 2;
+        for (int i = threadIdx.x; i < NF; i += 256) {
+            const int fi = i % (M + 1), o = i / (M + 1);
+            int c[3] = {0, 0, 0};
+            c[d] = fi;
+            c[da] = o % M;
+            if (D == 3) c[db] = o / M;
+            const int p = TL::at(c[0], c[1], c[2]);
+            ef[((size_t)blk * D + d) * NF + i] = (tile[p - TL::stride(d)] - tile[p]) / ea.hl + ea.e0[d];
+        }
+    }
+    for (int q = threadIdx.x; q < V; q += 256) {
+        int x, y, z;
+        LY::decode(q, x, y, z);
+        const int p = TL::at(x, y, z);
+        T f2 = T(0);
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            const T lo = (tile[p - TL::stride(d)] - tile[p]) / ea.hl + ea.e0[d];
+            const T hi = (tile[p] - tile[p + TL::stride(d)]) / ea.hl + ea.e0[d];
+            const T t = lo + hi;
+            f2 += t * t / T(4);
+        }
+        eabs[(size_t)blk * V + q] = sqrt(f2);
+    }
+}
+
 // ---- exact level-1 solve (FP64) ----
 
 // level-1 valid cells -> uniform grid (column-major, dimension 1 fastest), times c

@@ -936,6 +936,11 @@ struct Ops {
         settle(u);
     }
 
+    static void launch_efield(srmg_ctx *c, T *eabs, T *ef, const T *u, const T *ib, const EFieldArgs<T> &args,
+                              int nblk) {
+        hipLaunchKernelGGL((k_efield<D, M, T>), dim3((unsigned)nblk), dim3(256), 0, c->stream, eabs, ef, u, ib,
+                           c->d_halo.as<int>(), args);
+    }
     // first iterate of a solve from the level-1 problem (SRMG_GUESS_*)
     void first_iterate(FieldBuf &u, FieldBuf &b, int mode, int nsmooth, const std::vector<double> &r0);
 
@@ -1935,6 +1940,74 @@ int srmg_level_info(srmg_ctx *c, int level, int64_t *nblocks, int64_t *nleaf, in
         if (nblocks) *nblocks = L.nblk;
         if (nleaf) *nleaf = L.nleaf;
         if (nslots) *nslots = L.nslot;
+    });
+}
+
+int srmg_electric_field(srmg_ctx *c, int ui, int eabsi, double h, const double *e0, const int8_t *bc,
+                        double *e_host, int G) {
+    return guarded([&] {
+        check_tree(c);
+        if (!e0 || !bc) fail("null argument");
+        if (h <= 0.0) fail("h must be positive");
+        if (e_host && G < 1) fail("G must be >= 1");
+        FieldBuf &u = field(c, ui), &ea = field(c, eabsi);
+        if (u.dtype != SRMG_F64 || ea.dtype != SRMG_F64) fail("u and eabs must be FP64");
+        const int D = c->D, M = c->M;
+        const int64_t NF = (int64_t)(M + 1) * (D == 2 ? M : M * M);
+        DevBuf ef;
+        ef.alloc((size_t)c->nb * D * NF * sizeof(double));
+        srmg_op op{};
+        op.geometry = SRMG_CARTESIAN;
+        std::memcpy(op.bc, bc, 27);
+        dispatch(c, SRMG_F64, [&](auto &o) {
+            using OT = std::remove_reference_t<decltype(o)>;
+            using T = std::remove_reference_t<decltype(*o.X(u, 0))>;
+            if constexpr (!std::is_same<T, double>::value) return;
+            else {
+            o.settle(u);
+            o.settle(ea);
+            for (int l = 0; l < c->nlevels; l++) {
+                if (!c->lv[l].nblk) continue;
+                LevelArgs<double> a = level_args<double>(c, l, 1.0, &op, 1.0);
+                EFieldArgs<double> args;
+                args.blk0 = c->lv[l].blk0;
+                args.hl = h / (double)(1 << l);
+                for (int d = 0; d < 3; d++) args.e0[d] = d < D ? e0[d] : 0.0;
+                for (int q = 0; q < 6; q++) args.sg[q] = a.sg[q];
+                OT::launch_efield(c, ea.data.as<double>(), ef.as<double>(), u.data.as<double>(), u.ib.as<double>(),
+                                  args, c->lv[l].nblk);
+            }
+            }
+        });
+        HIPCHK(hipGetLastError());
+        if (e_host) {
+            std::vector<double> h_ef((size_t)c->nb * D * NF);
+            HIPCHK(hipMemcpyAsync(h_ef.data(), ef.p, h_ef.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                  c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            const int64_t Sv = M + 2 * G + 1;
+            const int64_t SvD = D == 2 ? Sv * Sv : Sv * Sv * Sv;
+            for (int b = 0; b < c->nb; b++) {
+                const int64_t hid = c->dev2host[b];  // 1-based host block id
+                for (int d = 0; d < D; d++) {
+                    const int da = d == 0 ? 1 : 0, db = d == 2 ? 1 : 2;
+                    for (int64_t i = 0; i < NF; i++) {
+                        int cc[3] = {0, 0, 0};
+                        cc[d] = (int)(i % (M + 1));
+                        cc[da] = (int)((i / (M + 1)) % M);
+                        if (D == 3) cc[db] = (int)((i / (M + 1)) / M);
+                        int64_t off = 0, st = 1;
+                        for (int e = 0; e < D; e++) {
+                            off += (int64_t)(cc[e] + G) * st;
+                            st *= Sv;
+                        }
+                        e_host[off + (int64_t)d * SvD + (hid - 1) * D * SvD] = h_ef[((size_t)b * D + d) * NF + i];
+                    }
+                }
+            }
+        } else {
+            HIPCHK(hipStreamSynchronize(c->stream));
+        }
     });
 }
 

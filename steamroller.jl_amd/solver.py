@@ -144,6 +144,7 @@ def lib():
         "srmg_set_sweep_config": [P, i32, i32, i32],
         "srmg_level_grouped": [P, i32, C.POINTER(i32)],
         "srmg_set_group_tiles": [P, i32],
+        "srmg_electric_field": [P, i32, i32, dbl, P, P, P, i32],
         "srmg_set_guess": [P, P],
         "srmg_guess_fallbacks": [P, C.POINTER(i32)],
         "srmg_grid_solve": [P, i32, P, dbl, P, P, P, i32],
@@ -490,6 +491,24 @@ def solve_mixed_batched(u, b, e, be, re, ue, s, geometry, bc, discs, *, tol=1e-8
     _check(lib().srmg_solve_mixed_batched(u.ctx.h, u.id, b.id, e.id, be.id, re.id, ue.id, _p(sv), ops,
                                           C.byref(o), C.byref(n), _p(hist)))
     return n.value, hist[:, : n.value + 1].copy()
+
+
+# ---- electric field ----
+
+def electric_field(u: DeviceField, eabs: DeviceField, h: float, e0, bc, G: int = 2, with_faces: bool = True):
+    """electric_field!(tree, e, eabs, u, h, t, eb) (src/multigrid.jl:421-451)
+    for a constant external field e0: |E| into the device field eabs; returns
+    E in the reference's VectorBlockField layout (or None)."""
+    c = u.ctx
+    e = np.ascontiguousarray(np.asarray(e0, np.float64))
+    b = _bc_arr(bc)
+    out = None
+    if with_faces:
+        S = c.M + 2 * G + 1
+        out = np.zeros((S,) * c.D + (c.D, c.nblocks), order="F")
+    _check(lib().srmg_electric_field(c.h, u.id, eabs.id, float(h), _p(e), _p(b),
+                                     None if out is None else out.ctypes.data_as(C.c_void_p), int(G)))
+    return out
 
 
 # ---- learned first iterate ----

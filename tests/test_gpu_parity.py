@@ -294,3 +294,29 @@ def test_sibling_group_kernels_give_identical_solves(name):
     np.testing.assert_array_equal(res[1][1], res[0][1])
     np.testing.assert_array_equal(res[1][2][0], res[0][2][0])
     np.testing.assert_array_equal(res[1][2][1], res[0][2][1])
+
+
+@pytest.mark.parametrize("name", ["cfg1", "cyl2-m8", "cart3-m8", "cart3-m16"])
+def test_electric_field_matches_oracle(name):
+    # electric_field! after two V-cycles: E on the faces and |E| at the centres,
+    # bit for bit against the oracle on the same potential
+    p = CASES[name]()
+    print(f"case {name}", flush=True)
+    geom, disc = solver.geometry_from_code(p.geom), _disc(p)
+    ctx, (u, b, r, u1) = _device(p)
+    b.upload(p.b)
+    solver.fill_ghost_all(u, p.bc)
+    for _ in range(2):
+        solver.vcycle(u, b, r, u1, p.s, p.tree, p.conn, geom, p.bc, disc, ndown=2, ntop=4, nup=2)
+    e0 = [0.0, -2.0e6, 1.0e5][:p.D]
+    eabs = ctx.field(F64)
+    E = solver.electric_field(u, eabs, p.h, e0, p.bc, G=p.b.G)
+    host = p.b.like()
+    u.download(host)
+    o = Oracle.for_problem(p)
+    o.fill_ghost_all(host)
+    Eo, fabs = o.electric_field(host, p.h, e0)
+    np.testing.assert_array_equal(E, Eo)
+    got = p.b.like()
+    eabs.download(got)
+    np.testing.assert_array_equal(_valid(got, p), fabs[(slice(p.b.G, p.b.G + p.M),) * p.D])

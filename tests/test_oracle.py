@@ -328,3 +328,48 @@ def test_vcycle_converges_on_cfg1_with_exact_coarse_solve():
         rel.append(o.resnorm(r, b, u, p.s) / r0)
     assert rel[-1] < 1e-8
     assert all(b2 < a2 for a2, b2 in zip(rel[:-1], rel[1:]))
+
+
+@pytest.mark.parametrize("D,M,rootsize", [(2, 8, (4, 4)), (3, 4, (4, 4, 4))])
+def test_electric_field_of_a_linear_potential(D, M, rootsize):
+    # u = g . x with dyadic h and integer g: every difference and quotient is
+    # exact, so on faces away from the domain boundary E = -g + e0 exactly
+    # (coarse-fine ghosts by linear interpolation are exact for linear data),
+    # and |E| = sqrt(sum (2E)^2 / 4) at cells whose faces are all interior
+    tree, conn = _two_level_patch(D, M, rootsize)
+    G = 2
+    h = 0.25
+    g = np.array([3.0, -5.0, 2.0][:D])
+    e0 = np.array([0.5, -2.0, 1.0][:D])
+    u = ScalarBlockField(D, M, G, tree.nblocks())
+    for level in (1, 2):
+        hl = h / (1 << (level - 1))
+        for c, blk in tree[level - 1].pairs:
+            ax = [hl * (M * (c[d] - 1) + np.arange(-G, M + G) + 0.5) for d in range(D)]
+            grids = np.meshgrid(*ax, indexing="ij")
+            u.block(blk)[...] = sum(g[d] * grids[d] for d in range(D))
+    o = Oracle(tree, conn, M, G, 0, boundaryconditions(((1, 1),) * D))
+    o.fill_ghost_all(u)
+    f, fabs = o.electric_field(u, h, e0)
+    S = M + 2 * G + 1
+    want_abs = np.sqrt(sum((2 * (e0[d] - g[d])) ** 2 / 4 for d in range(D)))
+    nface = ncell = 0
+    for level in (1, 2):
+        ext = [r << (level - 1) for r in rootsize]
+        for c, blk in tree[level - 1].pairs:
+            fb = f[..., blk - 1]
+            for d in range(D):
+                sl = [slice(G, G + M)] * D
+                lo = G + (1 if c[d] == 1 else 0)
+                hi = G + M + (0 if c[d] == ext[d] else 1)
+                sl[d] = slice(lo, hi)
+                vals = fb[tuple(sl) + (d,)]
+                assert np.all(vals == e0[d] - g[d]), (level, c, d)
+                nface += vals.size
+            inner = [slice(G + (1 if c[d] == 1 else 0), G + M - (1 if c[d] == ext[d] else 0)) for d in range(D)]
+            cells = fabs[tuple(inner) + (blk - 1,)]
+            assert np.all(cells == want_abs)
+            ncell += cells.size
+    assert nface > 0 and ncell > 0
+    # faces outside the valid range are not written
+    assert np.all(f[(slice(0, G),) + (slice(None),) * D] == 0)
