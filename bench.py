@@ -43,6 +43,7 @@ def parse():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched photoionization measurement")
     return ap.parse_args()
 
 
@@ -160,6 +161,7 @@ def main():
         cpu = cpu_baseline(p, forms["fp64"]["cycles"], n_leaf, args.cpu_threads)
         log(f"[bench] cpu: {cpu}")
 
+    batched = None if args.no_batched else batched_cfg3(log)
     traffic = committed_traffic(kname["float"], args.config)
     ms = forms[best]["ms"]
 
@@ -205,8 +207,84 @@ def main():
             "algorithmic_bytes_per_launch": roof["fp32"]["bytes"],
         },
         "cpu_baseline": cpu,
+        "batched_photoionization": batched,
     }
     print(json.dumps(out), flush=True)
+
+
+def batched_cfg3(log, ntrees=64):
+    """Batched Helmholtz solves (cfg3): the three photoionization terms on the
+    cfg1 tree plus `ntrees` uneven random trees (seeds 0..ntrees-1, 1e4-1e6
+    cells, k^2 cycling over the three terms), solved to 1e-8 in one forest
+    (every launch covers every tree) and one by one in their own contexts;
+    FP64 and FP32 with FP64 residual. Each timing is the second of two runs."""
+    import numpy as np
+    from srmg import configs, solver
+    from srmg.solver import F32, F64
+    t0 = time.time()
+    probs = list(configs.make_helmholtz_cfg1(0))
+    _, k2s = configs.bourdon3_terms()
+    probs += [configs.make_random_tree_problem(s, min_cells=1e4, max_cells=1e6, helm_k2=float(k2s[s % 3]))
+              for s in range(ntrees)]
+    leaf = sum(p.n_leaf for p in probs)
+    geom = solver.geometry_from_code(probs[0].geom)
+    bc = probs[0].bc
+    discs = [solver.HelmholtzDiscretization(p.D, 2, p.k2) for p in probs]
+    ss = [p.s for p in probs]
+    log(f"[bench] batched: {len(probs)} problems, {leaf} leaf cells (built in {time.time() - t0:.1f}s)")
+
+    res = {}
+    for mixed in (False, True):
+        # one forest
+        ctx = solver.Context.forest([(p.tree, p.conn) for p in probs], probs[0].M)
+        u, b, r, u1 = (ctx.field(F64) for _ in range(4))
+        e, be, re, ue = (ctx.field(F32) for _ in range(4))
+        bh = solver.concat_fields([p.b for p in probs])
+
+        def run_forest():
+            b.upload(bh)
+            u.zero()
+            ctx.synchronize()
+            t = time.perf_counter()
+            if mixed:
+                n, h = solver.solve_mixed_batched(u, b, e, be, re, ue, ss, geom, bc, discs, tol=1e-8,
+                                                  max_cycles=60, inner_cycles=2)
+            else:
+                n, h = solver.solve_batched(u, b, r, u1, ss, geom, bc, discs, tol=1e-8, max_cycles=60)
+            ctx.synchronize()
+            return time.perf_counter() - t, n, float(h[:, -1].max())
+        run_forest()
+        tf, nf, worst = run_forest()
+        del ctx, u, b, r, u1, e, be, re, ue
+        # one by one
+        tseq = 0.0
+        for p, d in zip(probs, discs):
+            c1 = solver.Context(p.tree, p.conn, p.M)
+            v, bb, rr, vv = (c1.field(F64) for _ in range(4))
+            ee, bee, ree, uee = (c1.field(F32) for _ in range(4))
+
+            def run_one():
+                bb.upload(p.b)
+                v.zero()
+                c1.synchronize()
+                t = time.perf_counter()
+                if mixed:
+                    solver.solve_mixed(v, bb, ee, bee, ree, uee, p.s, geom, bc, d, tol=1e-8, max_cycles=60,
+                                       inner_cycles=2)
+                else:
+                    solver.solve(v, bb, rr, vv, p.s, geom, bc, d, tol=1e-8, max_cycles=60)
+                c1.synchronize()
+                return time.perf_counter() - t
+            run_one()
+            tseq += run_one()
+        key = "fp32_mixed2" if mixed else "fp64"
+        res[key] = {"forest_ms": round(1e3 * tf, 3), "one_by_one_ms": round(1e3 * tseq, 3),
+                    "speedup": round(tseq / tf, 2), "forest_mcells_s": round(leaf / tf / 1e6, 1),
+                    "cycles": int(nf), "worst_final_rel_residual": float(f"{worst:.4g}")}
+        log(f"[bench] batched {key}: {res[key]}")
+    res["workload"] = (f"cfg3: 3 photoionization Helmholtz terms on the cfg1 tree + {ntrees} random trees, "
+                       f"{len(probs)} problems, {leaf} leaf cells, to 1e-8 each")
+    return res
 
 
 def sweep_kernel_name(D, M, t, cfg, grouped):
