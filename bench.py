@@ -99,8 +99,9 @@ def main():
             net = guess.load(guess.cfg4_weights_path())
             solver.set_guess(ctx, net)
 
-    specs = {"fp64": (False, 1, solver.GUESS_NONE), "mixed": (True, 1, solver.GUESS_NONE),
-             "mixed2": (True, 2, solver.GUESS_NONE), "mixed3": (True, 3, solver.GUESS_NONE)}
+    # every form starts from u = 0 (GUESS_ZERO: the initial norm and defect from b alone)
+    Z = solver.GUESS_ZERO
+    specs = {"fp64": (False, 1, Z), "mixed": (True, 1, Z), "mixed2": (True, 2, Z), "mixed3": (True, 3, Z)}
     if net is not None:
         specs["mixed2_learned"] = (True, 2, solver.GUESS_LEARNED)
     forms = {}
@@ -125,7 +126,7 @@ def main():
         forms[name] = {"ms": 1e3 * float(np.mean(times)), "cycles": int(cycles[-1]),
                        "mcells_s": n_leaf / float(np.mean(times)) / 1e6, "final_rel_residual": float(hist[-1]),
                        "first_rel_residual": float(hist[0])}
-        if gmode != solver.GUESS_NONE:
+        if gmode == solver.GUESS_LEARNED:
             forms[name]["guess_fallbacks"] = solver.guess_fallbacks(ctx)
         log(f"[bench] {name}: {forms[name]}")
         if name == "fp64":
@@ -248,9 +249,10 @@ def batched_cfg3(log, ntrees=64):
             t = time.perf_counter()
             if mixed:
                 n, h = solver.solve_mixed_batched(u, b, e, be, re, ue, ss, geom, bc, discs, tol=1e-8,
-                                                  max_cycles=60, inner_cycles=2)
+                                                  max_cycles=60, inner_cycles=2, guess=solver.GUESS_ZERO)
             else:
-                n, h = solver.solve_batched(u, b, r, u1, ss, geom, bc, discs, tol=1e-8, max_cycles=60)
+                n, h = solver.solve_batched(u, b, r, u1, ss, geom, bc, discs, tol=1e-8, max_cycles=60,
+                                            guess=solver.GUESS_ZERO)
             ctx.synchronize()
             return time.perf_counter() - t, n, float(h[:, -1].max())
         run_forest()
@@ -270,9 +272,9 @@ def batched_cfg3(log, ntrees=64):
                 t = time.perf_counter()
                 if mixed:
                     solver.solve_mixed(v, bb, ee, bee, ree, uee, p.s, geom, bc, d, tol=1e-8, max_cycles=60,
-                                       inner_cycles=2)
+                                       inner_cycles=2, guess=solver.GUESS_ZERO)
                 else:
-                    solver.solve(v, bb, rr, vv, p.s, geom, bc, d, tol=1e-8, max_cycles=60)
+                    solver.solve(v, bb, rr, vv, p.s, geom, bc, d, tol=1e-8, max_cycles=60, guess=solver.GUESS_ZERO)
                 c1.synchronize()
                 return time.perf_counter() - t
             run_one()

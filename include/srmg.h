@@ -163,7 +163,9 @@ typedef struct {
     int32_t guess_smooth;      /* Gauss-Seidel sweeps on level 1 after a learned first iterate */
 } srmg_solve_opts;
 
-/* First iterate of a solve. NONE: u as given. LEARNED: the network attached
+/* First iterate of a solve. NONE: u as given. ZERO: u = 0 whatever u holds
+ * (its residual is -b, so the initial norm and FP64 defect come from b alone).
+ * LEARNED: the network attached
  * with srmg_set_guess predicts the level-1 solution from the level-1 right-hand
  * side; it is interpolated to every finer level. COARSE: the same with the
  * exact level-1 solve in place of the network. A learned or coarse first
@@ -171,7 +173,7 @@ typedef struct {
  * exceeds resnorm(u = 0) starts from u = 0 instead (counted by
  * srmg_guess_fallbacks). No counterpart in the reference (its solves start
  * from the previous u). */
-enum { SRMG_GUESS_NONE = 0, SRMG_GUESS_LEARNED = 1, SRMG_GUESS_COARSE = 2 };
+enum { SRMG_GUESS_NONE = 0, SRMG_GUESS_LEARNED = 1, SRMG_GUESS_COARSE = 2, SRMG_GUESS_ZERO = 3 };
 
 /* Solve from the current u (ghosts pre-filled here) with plain V-cycles of
  * the fields' own type until the relative residual is <= tol. hist (may be

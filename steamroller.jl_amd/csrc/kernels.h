@@ -1138,6 +1138,47 @@ __global__ __launch_bounds__(256) void k_efield(T *__restrict__ eabs, T *__restr
     }
 }
 
+// resnorm partials of u = 0 (the residual is -b): the same blocks, visiting
+// order and tree sums as k_resid_tile<RES_NORM>, so the partials are bitwise
+// those of a residual norm of a zero field, without reading one.
+template <int D, int M, typename T, int BPW, int NT>
+__global__ __launch_bounds__(NT) void k_bnorm(const T *__restrict__ b, const int *__restrict__ list, int n,
+                                              double wgt, double *__restrict__ partial) {
+    constexpr int V = Lay<D, M>::V;
+    __shared__ int blks[BPW];
+    __shared__ double red[NT];
+    const int first = xcd_remap(blockIdx.x, gridDim.x) * BPW;
+    const int nloc = min(BPW, n - first);
+    if ((int)threadIdx.x < BPW) blks[threadIdx.x] = ((int)threadIdx.x < nloc) ? list[first + threadIdx.x] : 0;
+    __syncthreads();
+    double acc[BPW];
+#pragma unroll
+    for (int lb = 0; lb < BPW; lb++) acc[lb] = 0.0;
+    for (int i = threadIdx.x; i < nloc * V; i += NT) {
+        const int lb = i / V, q = i - lb * V;
+        const T rr = -b[(size_t)blks[lb] * V + q];
+#pragma unroll
+        for (int k = 0; k < BPW; k++)
+            if (k == lb) acc[k] += (double)rr * (double)rr;
+    }
+    for (int lb = 0; lb < nloc; lb++) {
+        red[threadIdx.x] = acc[lb];
+        __syncthreads();
+        for (int w = NT / 2; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) partial[first + lb] = red[0] * wgt;
+        __syncthreads();
+    }
+}
+
+// be = (float) b over n values (the FP64 defect of u = 0)
+__global__ __launch_bounds__(WG) void k_to_float(float *__restrict__ out, const double *__restrict__ in, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (float)in[i];
+}
+
 // ---- exact level-1 solve (FP64) ----
 
 // level-1 valid cells -> uniform grid (column-major, dimension 1 fastest), times c

@@ -855,6 +855,29 @@ struct Ops {
         sum_partials(np, c->d_rng_leaf);
         return np;
     }
+    // resnorm partials of u = 0 from b alone (bitwise those of resnorm_partials
+    // on a zero field)
+    int bnorm_partials(FieldBuf &b) {
+        int np = 0;
+        for (int l = 0; l < c->nlevels; l++) {
+            const LevelInfo &L = c->lv[l];
+            if (!L.nleaf) continue;
+            double wgt = std::ldexp(1.0, -c->D * l);
+            unsigned g = (unsigned)((L.nleaf + RB - 1) / RB);
+            hipLaunchKernelGGL((k_bnorm<D, M, T, RB, RNT>), dim3(g), dim3(RNT), 0, c->stream, X(b, l),
+                               c->d_leaves.as<int>() + L.leaf0, L.nleaf, wgt, c->d_partial.as<double>() + np);
+            np += L.nleaf;
+        }
+        sum_partials(np, c->d_rng_leaf);
+        return np;
+    }
+    // u = 0 on every level and slab (values in the primary buffer)
+    void zero_all(FieldBuf &u) {
+        const size_t n = (size_t)c->nb * LY::V, ni = u.ib.bytes / sizeof(T);
+        hipLaunchKernelGGL(k_fill<T>, dim3(nwg(n)), dim3(WG), 0, c->stream, u.data.as<T>(), (T)0, n);
+        if (ni) hipLaunchKernelGGL(k_fill<T>, dim3(nwg(ni)), dim3(WG), 0, c->stream, u.ib.as<T>(), (T)0, ni);
+        std::fill(u.cur.begin(), u.cur.end(), 0);
+    }
     // mixed precision: be = b + L u / s (FP32) on every level, plus the leaf
     // residual norm partials, in one pass (T = double)
     void defect_and_norm(float *be, FieldBuf &u, FieldBuf &b, double s, const srmg_op *op) {
@@ -1383,7 +1406,7 @@ static std::vector<double> read_norms(srmg_ctx *c) {
 static void check_opts(const srmg_solve_opts *o) {
     if (!o) fail("null options");
     if (o->max_cycles < 0 || o->ndown < 0 || o->nup < 0 || o->ntop < 0) fail("bad solve options");
-    if (o->guess < SRMG_GUESS_NONE || o->guess > SRMG_GUESS_COARSE) fail("bad first-iterate mode");
+    if (o->guess < SRMG_GUESS_NONE || o->guess > SRMG_GUESS_ZERO) fail("bad first-iterate mode");
     if (o->guess_smooth < 0) fail("bad first-iterate smoothing count");
 }
 
@@ -1415,12 +1438,22 @@ static void solve_plain(srmg_ctx *c, int ui, int bi, int ri, int u1i, const srmg
     FieldBuf &u = field(c, ui), &b = field(c, bi), &r = field(c, ri), &u1 = field(c, u1i);
     dispatch(c, t, [&](auto &ops) {
         using T = std::remove_reference_t<decltype(*ops.X(u, 0))>;
-        for (int l = 1; l < c->nlevels; l++) ops.fill_interp(u, l, level_args<T>(c, l - 1, s, op, 1.0));
-        ops.resnorm_partials(zero_field(c, t), b, s, op);
-        std::vector<double> r0 = safe_r0(read_norms(c));
-        if (o->guess) ops.first_iterate(u, b, o->guess, o->guess_smooth, r0);
-        ops.resnorm_partials(u, b, s, op);
-        bool done = record(read_norms(c), r0, o->tol, 0, o->max_cycles, hist);
+        bool done;
+        std::vector<double> r0;
+        if (o->guess == SRMG_GUESS_ZERO) {
+            // u = 0: its residual is -b, so resnorm(u) = resnorm(0) from b alone
+            ops.zero_all(u);
+            ops.bnorm_partials(b);
+            r0 = safe_r0(read_norms(c));
+            done = record(r0, r0, o->tol, 0, o->max_cycles, hist);
+        } else {
+            for (int l = 1; l < c->nlevels; l++) ops.fill_interp(u, l, level_args<T>(c, l - 1, s, op, 1.0));
+            ops.resnorm_partials(zero_field(c, t), b, s, op);
+            r0 = safe_r0(read_norms(c));
+            if (o->guess) ops.first_iterate(u, b, o->guess, o->guess_smooth, r0);
+            ops.resnorm_partials(u, b, s, op);
+            done = record(read_norms(c), r0, o->tol, 0, o->max_cycles, hist);
+        }
         int k = 0;
         Graph gr;
         while (k < o->max_cycles && !done) {
@@ -1447,10 +1480,18 @@ static void solve_mixed_dm(srmg_ctx *c, FieldBuf &u, FieldBuf &b, FieldBuf &e, F
     const srmg_op *op = &c->pops[0];
     Ops<D, M, double> o64(c);
     Ops<D, M, float> o32(c);
-    for (int l = 1; l < c->nlevels; l++) o64.fill_interp(u, l, level_args<double>(c, l - 1, s, op, 1.0));
-    o64.resnorm_partials(zero_field(c, SRMG_F64), b, s, op);
-    std::vector<double> r0 = safe_r0(read_norms(c));
-    if (o->guess) o64.first_iterate(u, b, o->guess, o->guess_smooth, r0);
+    std::vector<double> r0;
+    const bool zero_start = o->guess == SRMG_GUESS_ZERO;
+    if (zero_start) {
+        o64.zero_all(u);
+        o64.bnorm_partials(b);
+        r0 = safe_r0(read_norms(c));
+    } else {
+        for (int l = 1; l < c->nlevels; l++) o64.fill_interp(u, l, level_args<double>(c, l - 1, s, op, 1.0));
+        o64.resnorm_partials(zero_field(c, SRMG_F64), b, s, op);
+        r0 = safe_r0(read_norms(c));
+        if (o->guess) o64.first_iterate(u, b, o->guess, o->guess_smooth, r0);
+    }
     auto defect_and_norm = [&] { o64.defect_and_norm(be.data.as<float>(), u, b, s, op); };
     auto correct = [&] {
         size_t n = (size_t)c->nb * LY::V, ni = e.ib.bytes / sizeof(float);
@@ -1466,8 +1507,17 @@ static void solve_mixed_dm(srmg_ctx *c, FieldBuf &u, FieldBuf &b, FieldBuf &e, F
         for (int l = 1; l < c->nlevels; l++) o64.fill_interp(u, l, level_args<double>(c, l - 1, s, op, 1.0));
         defect_and_norm();
     };
-    defect_and_norm();
-    bool done = record(read_norms(c), r0, o->tol, 0, o->max_cycles, hist);
+    bool done;
+    if (zero_start) {
+        // the FP64 defect of u = 0 is b itself (b + L 0 / s rounds to b)
+        const size_t n = (size_t)c->nb * LY::V;
+        hipLaunchKernelGGL(k_to_float, dim3(nwg(n)), dim3(WG), 0, c->stream, be.data.as<float>(), b.data.as<double>(),
+                           n);
+        done = record(r0, r0, o->tol, 0, o->max_cycles, hist);
+    } else {
+        defect_and_norm();
+        done = record(read_norms(c), r0, o->tol, 0, o->max_cycles, hist);
+    }
     int k = 0;
     Graph gr;
     while (k < o->max_cycles && !done) {

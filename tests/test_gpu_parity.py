@@ -320,3 +320,39 @@ def test_electric_field_matches_oracle(name):
     got = p.b.like()
     eabs.download(got)
     np.testing.assert_array_equal(_valid(got, p), fabs[(slice(p.b.G, p.b.G + p.M),) * p.D])
+
+
+@pytest.mark.parametrize("name", ["cfg1", "cart3-m8", "cart3-helm"])
+def test_zero_start_equals_a_solve_from_zeroed_u(name):
+    # SRMG_GUESS_ZERO takes resnorm(0) and the first FP64 defect from b alone;
+    # every later step must be bitwise that of a solve from a zeroed u
+    p = CASES[name]()
+    geom, disc = solver.geometry_from_code(p.geom), _disc(p)
+    out = {}
+    for mode in (solver.GUESS_NONE, solver.GUESS_ZERO):
+        ctx = solver.Context(p.tree, p.conn, p.M)
+        u, b, r, u1 = (ctx.field(F64) for _ in range(4))
+        b.upload(p.b)
+        if mode == solver.GUESS_ZERO:
+            u.upload(p.b)  # garbage the zero start must ignore
+        n, h = solver.solve(u, b, r, u1, p.s, geom, p.bc, disc, tol=1e-9, max_cycles=12, guess=mode)
+        v, b2 = ctx.field(F64), ctx.field(F64)
+        b2.upload(p.b)
+        if mode == solver.GUESS_ZERO:
+            v.upload(p.b)
+        e, be, re, ue = (ctx.field(F32) for _ in range(4))
+        nm, hm = solver.solve_mixed(v, b2, e, be, re, ue, p.s, geom, p.bc, disc, tol=1e-9, max_cycles=12,
+                                    inner_cycles=2, guess=mode)
+        fields = []
+        for f in (u, v):
+            host = p.b.like()
+            f.download(host)
+            fields.append(_valid(host, p).copy())
+        out[mode] = (h, hm, fields)
+    h0, hm0, f0 = out[solver.GUESS_NONE]
+    h1, hm1, f1 = out[solver.GUESS_ZERO]
+    assert h1[0] == 1.0 and hm1[0] == 1.0
+    np.testing.assert_array_equal(h1[1:], h0[1:])
+    np.testing.assert_array_equal(hm1[1:], hm0[1:])
+    np.testing.assert_array_equal(f1[0], f0[0])
+    np.testing.assert_array_equal(f1[1], f0[1])
