@@ -280,11 +280,10 @@ __global__ __launch_bounds__(WG) void k_interp_add(T *__restrict__ u, const T *_
 // to the fine block's valid cells, i.e. child offset 1 on a low face and 0 on a
 // high face in the face dimension. `a` describes the coarse level (mirror signs).
 template <int D, int M, typename T>
-__global__ __launch_bounds__(WG) void k_fill_interp(T *__restrict__ ib, const T *__restrict__ x,
-                                                    const int *__restrict__ nbr, const int4 *__restrict__ slots,
-                                                    int slot0, int nslot, LevelArgs<T> a) {
+__device__ __forceinline__ void fill_interp_one(int gt, T *__restrict__ ib, const T *__restrict__ x,
+                                                const int *__restrict__ nbr, const int4 *__restrict__ slots,
+                                                int slot0, int nslot, const LevelArgs<T> &a) {
     using LY = Lay<D, M>;
-    int gt = blockIdx.x * blockDim.x + threadIdx.x;
     int ls = gt / LY::F;
     if (ls >= nslot) return;
     int k = slot0 + ls;
@@ -315,6 +314,30 @@ __global__ __launch_bounds__(WG) void k_fill_interp(T *__restrict__ ib, const T 
     for (int j = 0; j < D; j++)
         f += w1<T>() * nbv<D, M, T>(x, ib, nbr, pb, J[0], J[1], J[2], j, 2 * i[j] - 1, a.sg);
     ib[(size_t)k * LY::F + t] = f;
+}
+
+template <int D, int M, typename T>
+__global__ __launch_bounds__(WG) void k_fill_interp(T *__restrict__ ib, const T *__restrict__ x,
+                                                    const int *__restrict__ nbr, const int4 *__restrict__ slots,
+                                                    int slot0, int nslot, LevelArgs<T> a) {
+    fill_interp_one<D, M, T>(blockIdx.x * blockDim.x + threadIdx.x, ib, x, nbr, slots, slot0, nslot, a);
+}
+
+// Two independent steps of the V-cycle ascent in one launch: the first
+// nwg_diff work groups form dst = a1 - a2 over n values (k_diff), the rest
+// refresh nslot coarse-fine ghost slabs (k_fill_interp). The slabs read
+// another level than the difference writes.
+template <int D, int M, typename T>
+__global__ __launch_bounds__(WG) void k_diff_fill(T *dst, const T *a1, const T *a2, size_t n, unsigned nwg_diff,
+                                                  T *__restrict__ ib, const T *__restrict__ x,
+                                                  const int *__restrict__ nbr, const int4 *__restrict__ slots,
+                                                  int slot0, int nslot, LevelArgs<T> a) {
+    if (blockIdx.x < nwg_diff) {
+        const size_t i = (size_t)blockIdx.x * WG + threadIdx.x;
+        if (i < n) dst[i] = a1[i] - a2[i];
+    } else {
+        fill_interp_one<D, M, T>((blockIdx.x - nwg_diff) * WG + threadIdx.x, ib, x, nbr, slots, slot0, nslot, a);
+    }
 }
 
 // First stage of a fixed-order sum: work group g sums the contiguous chunk
@@ -904,19 +927,18 @@ __device__ __forceinline__ void load_tile(T *tile, int *tab, int *blks, const T 
 // interpolates every child cell from there. Same operations and order as
 // k_interp_add, so the result is bitwise the same.
 template <int D, int M, typename T, int NT>
-__global__ __launch_bounds__(NT) void k_interp_add_group(T *__restrict__ u, const T *__restrict__ u1,
-                                                         const T *__restrict__ ib1, const int *__restrict__ htab,
-                                                         const int *__restrict__ parent, int blk0,
-                                                         LevelArgs<T> ac) {
+__device__ __forceinline__ void interp_add_group_one(int g, T *__restrict__ u, const T *__restrict__ u1,
+                                                     const T *__restrict__ ib1, const int *__restrict__ htab,
+                                                     const int *__restrict__ parent, int blk0,
+                                                     const LevelArgs<T> &ac) {
     using LY = Lay<D, M>;
     using TL = Tile<D, M>;
-    using HT = Halo<D>;
     constexpr int NB = 1 << D, V = LY::V, MH = LY::MH;
     static_assert((NB * V) % NT == 0, "cells of a group must be a multiple of the work-group size");
     __shared__ T tile[TL::SZ];
     __shared__ int tab[2 * D];
     __shared__ int blks[1];
-    const int g0 = blk0 + blockIdx.x * NB;
+    const int g0 = blk0 + g * NB;
     load_tile<D, M, T, 1, NT>(tile, tab, blks, u1, ib1, htab, parent, g0, 1, ac.sg);
 #pragma unroll
     for (int k = 0; k < NB * V / NT; k++) {
@@ -934,6 +956,31 @@ __global__ __launch_bounds__(NT) void k_interp_add_group(T *__restrict__ u, cons
         const size_t o = (size_t)(g0 + lb) * V + q;
         u[o] = f + u[o];
     }
+}
+
+template <int D, int M, typename T, int NT>
+__global__ __launch_bounds__(NT) void k_interp_add_group(T *__restrict__ u, const T *__restrict__ u1,
+                                                         const T *__restrict__ ib1, const int *__restrict__ htab,
+                                                         const int *__restrict__ parent, int blk0,
+                                                         LevelArgs<T> ac) {
+    interp_add_group_one<D, M, T, NT>(blockIdx.x, u, u1, ib1, htab, parent, blk0, ac);
+}
+
+// k_interp_add_group on the first ngroups work groups and k_fill_interp of the
+// fine level's slabs (from the coarse level of u, already final) on the rest:
+// the prolongation writes the fine cells of u, the slab fill writes u's
+// fine-level slabs; neither reads what the other writes.
+template <int D, int M, typename T, int NT>
+__global__ __launch_bounds__(NT) void k_interp_fill_group(T *__restrict__ u, const T *__restrict__ u1,
+                                                          const T *__restrict__ ib1, const int *__restrict__ htab,
+                                                          const int *__restrict__ parent, int blk0, unsigned ngroups,
+                                                          T *__restrict__ ibu, const T *__restrict__ xc,
+                                                          const int *__restrict__ nbr, const int4 *__restrict__ slots,
+                                                          int slot0, int nslot, LevelArgs<T> ac) {
+    if (blockIdx.x < ngroups)
+        interp_add_group_one<D, M, T, NT>(blockIdx.x, u, u1, ib1, htab, parent, blk0, ac);
+    else
+        fill_interp_one<D, M, T>((blockIdx.x - ngroups) * NT + threadIdx.x, ibu, xc, nbr, slots, slot0, nslot, ac);
 }
 
 enum { RES_WRITE = 0, RES_NORM = 1, RES_DEFECT = 2 };

@@ -827,6 +827,36 @@ struct Ops {
                            X(u, l), X(u1, l - 1), I(u1), nbr(), c->d_parent.as<int>(), c->d_sub.as<int>(), L.blk0,
                            L.nblk, ac);
     }
+    // diff(u1, u, u1, l) and, when acf is given, fill_interp(u1, l, *acf) in
+    // one launch (the slab fill reads level l-1, the difference writes level l)
+    void diff_fill(FieldBuf &u1, FieldBuf &u, int l, const LevelArgs<T> *acf) {
+        const LevelInfo &L = c->lv[l];
+        const size_t n = (size_t)L.nblk * LY::V;
+        const int nslot = (acf && l >= 1) ? L.nslot : 0;
+        if (!n && !nslot) return;
+        const unsigned nd = nwg(n), nf = nwg((int64_t)nslot * LY::F);
+        const size_t o = (size_t)L.blk0 * LY::V;
+        const LevelArgs<T> a = nslot ? *acf : LevelArgs<T>{};
+        hipLaunchKernelGGL((k_diff_fill<D, M, T>), dim3(nd + nf), dim3(WG), 0, c->stream, X(u1, l) + o, X(u, l) + o,
+                           X(u1, l) + o, n, nd, I(u1), X(u1, nslot ? l - 1 : l), nbr(), c->d_slots.as<int4>(),
+                           L.slot0, nslot, a);
+    }
+    // interp_add(u, u1, l, ac) and fill_interp(u, l, ac) in one launch on
+    // grouped levels (the slab fill reads level l-1 of u, final by now)
+    void interp_fill(FieldBuf &u, FieldBuf &u1, int l, const LevelArgs<T> &ac) {
+        const LevelInfo &L = c->lv[l];
+        if constexpr ((LY::V << D) % 256 == 0 && (LY::V << D) / 256 <= 16) {
+            if (L.nblk && grouped(l)) {
+                const unsigned ng = (unsigned)(L.nblk >> D), nf = nwg((int64_t)L.nslot * LY::F);
+                hipLaunchKernelGGL((k_interp_fill_group<D, M, T, 256>), dim3(ng + nf), dim3(256), 0, c->stream,
+                                   X(u, l), X(u1, l - 1), I(u1), c->d_halo.as<int>(), c->d_parent.as<int>(), L.blk0,
+                                   ng, I(u), X(u, l - 1), nbr(), c->d_slots.as<int4>(), L.slot0, L.nslot, ac);
+                return;
+            }
+        }
+        interp_add(u, u1, l, ac);
+        fill_interp(u, l, ac);
+    }
     // refresh the coarse-fine ghost slabs of level l from level l-1
     void fill_interp(FieldBuf &x, int l, const LevelArgs<T> &ac) {
         const LevelInfo &L = c->lv[l];
@@ -950,10 +980,13 @@ struct Ops {
         for (int l = 1; l < lmax; l++) {
             LevelArgs<T> a = level_args<T>(c, l, s, op, omega);
             LevelArgs<T> ac = level_args<T>(c, l - 1, s, op, omega);
-            diff(u1, u, u1, l - 1);
-            if (l - 1 >= 1) fill_interp(u1, l - 1, level_args<T>(c, l - 2, s, op, omega));
-            interp_add(u, u1, l, ac);
-            fill_interp(u, l, ac);
+            if (l - 1 >= 1) {
+                LevelArgs<T> a2 = level_args<T>(c, l - 2, s, op, omega);
+                diff_fill(u1, u, l - 1, &a2);
+            } else {
+                diff_fill(u1, u, l - 1, nullptr);
+            }
+            interp_fill(u, u1, l, ac);
             smooth(u, b, a, l, nup);
         }
         settle(u);
@@ -985,10 +1018,13 @@ struct Ops {
             copy(u1, u, l);
             if (l > 0) {
                 LevelArgs<T> ac = level_args<T>(c, l - 1, s, op, omega);
-                diff(u1, u, u1, l - 1);
-                if (l - 1 >= 1) fill_interp(u1, l - 1, level_args<T>(c, l - 2, s, op, omega));
-                interp_add(u, u1, l, ac);
-                fill_interp(u, l, ac);
+                if (l - 1 >= 1) {
+                    LevelArgs<T> a2 = level_args<T>(c, l - 2, s, op, omega);
+                    diff_fill(u1, u, l - 1, &a2);
+                } else {
+                    diff_fill(u1, u, l - 1, nullptr);
+                }
+                interp_fill(u, u1, l, ac);
             }
             vcycle(u, b, r, u1, s, op, l + 1, ndown, ntop, nup, omega);
         }
@@ -1056,11 +1092,7 @@ void Ops<D, M, T>::first_iterate(FieldBuf &u, FieldBuf &b, int mode, int nsmooth
             settle(u);
         }
     }
-    for (int l = 1; l < c->nlevels; l++) {
-        LevelArgs<T> ac = level_args<T>(c, l - 1, s, op, 1.0);
-        interp_add(u, u, l, ac);
-        fill_interp(u, l, ac);
-    }
+    for (int l = 1; l < c->nlevels; l++) interp_fill(u, u, l, level_args<T>(c, l - 1, s, op, 1.0));
     for (int l = c->nlevels - 1; l >= 1; l--) restrict_(u, l);
     for (int l = 1; l < c->nlevels; l++) fill_interp(u, l, level_args<T>(c, l - 1, s, op, 1.0));
     resnorm_partials(u, b, s, op);
