@@ -1127,6 +1127,101 @@ __global__ __launch_bounds__(1024) void k_resid_group(TO *__restrict__ out, cons
     }
 }
 
+// The V-cycle descent's residual and restrictions on a level of complete
+// sibling groups, in one launch: residual of the group (as k_resid_group,
+// kept in LDS instead of written), then on the parent's cells the restricted
+// residual rc and the restricted u (into uc and, if u1c is not null, u1c),
+// each the sum of the 2^D children in k_restrict's order divided by 2^D. Work
+// groups past ngroups copy u -> u1 on the coarse level's leaf blocks (lists
+// leaves[0..nleafc)), completing copyto!(u1, u) on that level. Same values
+// and operations as residual + restrict(u) + copy + restrict(r).
+template <int D, int M, typename T, int RT, int RD>
+__global__ __launch_bounds__(1024) void k_resid_restrict(const T *__restrict__ u, const T *__restrict__ b,
+                                                         const T *__restrict__ ib, const int *__restrict__ htab,
+                                                         const int *__restrict__ coord,
+                                                         const int *__restrict__ parent, LevelArgs<T> a,
+                                                         unsigned ngroups, T *__restrict__ rc, T *__restrict__ uc,
+                                                         T *__restrict__ u1c, const T *__restrict__ ucin,
+                                                         const int *__restrict__ leaves, int nleafc) {
+    using LY = Lay<D, M>;
+    using TL = Tile<D, M, 2>;
+    using HT = Halo<D>;
+    constexpr int NT = 1024, NB = 1 << D, V = LY::V, F = LY::F, MH = LY::MH;
+    constexpr int TEAMS = NT / RT;
+    static_assert(NB % TEAMS == 0 && RT >= 64, "team layout");
+    const int tid = threadIdx.x;
+    if (blockIdx.x >= ngroups) {
+        // u1 = u on the coarse level's leaf blocks
+        const int64_t i = (int64_t)(blockIdx.x - ngroups) * NT + tid;
+        if (i < (int64_t)nleafc * V) {
+            const size_t o = (size_t)leaves[i / V] * V + (size_t)(i % V);
+            u1c[o] = ucin[o];
+        }
+        return;
+    }
+    __shared__ T tile[TL::SZ];
+    __shared__ T rt[NB * V];
+    __shared__ int tab[NB * 2 * D];
+    __shared__ BP<T> prm[NB];
+    const int gl = xcd_remap(blockIdx.x, ngroups);
+    const int g0 = a.blk0 + gl * NB;
+    for (int i = tid; i < NB * V; i += NT) {
+        const int lb = i / V, q = i - lb * V;
+        int x, y, z;
+        LY::decode(q, x, y, z);
+        tile[TL::origin(lb) + TL::at(x, y, z)] = u[(size_t)(g0 + lb) * V + q];
+    }
+    for (int i = tid; i < NB * 2 * D; i += NT) {
+        const int lb = i / (2 * D);
+        tab[i] = htab[(size_t)(g0 + lb) * HT::N + (i - lb * 2 * D)];
+    }
+    if (tid < NB) prm[tid] = bparams(a, g0 + tid);
+    __syncthreads();
+    for (int i = tid; i < NB * D * F; i += NT) {
+        const int lb = i / (D * F), rem = i - lb * (D * F);
+        const int d = rem / F, t = rem - d * F;
+        const int f = 2 * d + ((lb >> d) & 1);
+        halo_switch<D, M, T, TL>(tile + TL::origin(lb), tab[lb * 2 * D + f], f, g0 + lb, t, u, ib, a.sg);
+    }
+    __syncthreads();
+    const int team = tid / RT, t = tid - team * RT;
+    for (int round = 0; round < NB / TEAMS; round++) {
+        const int lb = team + round * TEAMS, blk = g0 + lb;
+        const BP<T> bp = prm[lb];
+        for (int q = t; q < V; q += RT) {
+            int x, y, z;
+            LY::decode(q, x, y, z);
+            const int p = TL::origin(lb) + TL::at(x, y, z);
+            T wm, wp;
+            rweights<D, M, T, RD>(a, coord, blk, RD == 0 ? x : y, wm, wp);
+            T Lu = tstencil<D, M, T, RD, TL>(tile, p, tile[p], bp.cc, wm, wp);
+            rt[lb * V + q] = -(b[(size_t)blk * V + q] + Lu / bp.s);
+        }
+    }
+    __syncthreads();
+    // the parent's cells: coarse cell X of sub-block k = (X >= MH per dim)
+    const int pb = parent[g0];
+    for (int i = tid; i < V; i += NT) {
+        int X0, X1, X2;
+        LY::decode(i, X0, X1, X2);
+        const int k = (X0 >= MH ? 1 : 0) | (X1 >= MH ? 2 : 0) | (D == 3 && X2 >= MH ? 4 : 0);
+        const int j0 = X0 - (k & 1) * MH, j1 = X1 - ((k >> 1) & 1) * MH, j2 = (D == 3) ? X2 - ((k >> 2) & 1) * MH : 0;
+        T fr = T(0), fu = T(0);
+#pragma unroll
+        for (int q = 0; q < LY::NC; q++) {
+            const int i0 = q & 1, i1 = (q >> 1) & 1, i2 = (q >> 2) & 1;
+            const int cx = 2 * j0 + i0, cy = 2 * j1 + i1, cz = D == 3 ? 2 * j2 + i2 : 0;
+            fr += rt[k * V + LY::sidx(cx, cy, cz)];
+            fu += tile[TL::origin(k) + TL::at(cx, cy, cz)];
+        }
+        const size_t o = (size_t)pb * V + i;
+        rc[o] = fr / T(LY::NC);
+        const T uv = fu / T(LY::NC);
+        uc[o] = uv;
+        if (u1c) u1c[o] = uv;
+    }
+}
+
 // ---- electric field ----
 
 template <typename T>

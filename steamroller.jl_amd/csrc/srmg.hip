@@ -761,6 +761,27 @@ struct Ops {
                                (const int *)nullptr, a.nblk, a, 0.0, (double *)nullptr);
         });
     }
+    // residual(r, u, b, l); restrict_(u, l); [copy(u1, u, l - 1);] restrict_(r, l)
+    // in one launch on grouped levels (r on level l itself is not written);
+    // false when the level is not grouped (the caller runs the four steps)
+    bool resid_restrict(FieldBuf &r, FieldBuf &u, FieldBuf *u1, FieldBuf &b, const LevelArgs<T> &a, int l) {
+        if constexpr (GROUP_OK) {
+            if (!a.nblk || !grouped(l)) return false;
+            const LevelInfo &Lc = c->lv[l - 1];
+            const unsigned ng = (unsigned)(a.nblk >> D);
+            const int nleafc = u1 ? Lc.nleaf : 0;
+            const unsigned ncp = (unsigned)(((int64_t)nleafc * LY::V + 1023) / 1024);
+            with_rd(a.rdim, [&](auto rd) {
+                constexpr int RD = decltype(rd)::value;
+                hipLaunchKernelGGL((k_resid_restrict<D, M, T, RNT, RD>), dim3(ng + ncp), dim3(1024), 0, c->stream,
+                                   X(u, l), X(b, l), I(u), c->d_halo.as<int>(), coord(), c->d_parent.as<int>(), a,
+                                   ng, X(r, l - 1), X(u, l - 1), u1 ? X(*u1, l - 1) : (T *)nullptr, X(u, l - 1),
+                                   c->d_leaves.as<int>() + Lc.leaf0, nleafc);
+            });
+            return true;
+        }
+        return false;
+    }
     // the per-cell gather form of the residual (kept for timing comparisons)
     void residual_gather(FieldBuf &r, FieldBuf &u, FieldBuf &b, const LevelArgs<T> &a, int l) {
         if (!a.nblk) return;
@@ -970,10 +991,12 @@ struct Ops {
             LevelArgs<T> a = level_args<T>(c, l, s, op, omega);
             LevelArgs<T> ac = level_args<T>(c, l - 1, s, op, omega);
             smooth(u, b, a, l, ndown);
-            residual(r, u, b, a, l);
-            restrict_(u, l);
-            copy(u1, u, l - 1);
-            restrict_(r, l);
+            if (!resid_restrict(r, u, &u1, b, a, l)) {
+                residual(r, u, b, a, l);
+                restrict_(u, l);
+                copy(u1, u, l - 1);
+                restrict_(r, l);
+            }
             postrestrict(b, u, r, l, ac);
         }
         top(u, b, s, op, ntop, omega);
