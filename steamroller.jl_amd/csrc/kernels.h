@@ -1222,6 +1222,38 @@ __global__ __launch_bounds__(1024) void k_resid_restrict(const T *__restrict__ u
     }
 }
 
+// k_postrestrict on a level of complete sibling groups: the parent of group g
+// is fully covered, so one work group stages the parent's u (level l-1, with
+// its depth-1 halo) in LDS and forms b = -(r + L u / s) on all its cells.
+// Same operations as k_postrestrict (the tile stencil equals the gather one).
+template <int D, int M, typename T, int RD>
+__global__ __launch_bounds__(256) void k_postrestrict_group(T *__restrict__ bout, const T *__restrict__ u,
+                                                            const T *__restrict__ r, const T *__restrict__ ib,
+                                                            const int *__restrict__ htab,
+                                                            const int *__restrict__ coord,
+                                                            const int *__restrict__ parent, int fblk0,
+                                                            LevelArgs<T> ac) {
+    using LY = Lay<D, M>;
+    using TL = Tile<D, M>;
+    constexpr int NB = 1 << D, V = LY::V;
+    __shared__ T tile[TL::SZ];
+    __shared__ int tab[2 * D];
+    __shared__ int blks[1];
+    load_tile<D, M, T, 1, 256>(tile, tab, blks, u, ib, htab, parent, fblk0 + (int)blockIdx.x * NB, 1, ac.sg);
+    const int pb = blks[0];
+    const BP<T> bp = bparams(ac, pb);
+    for (int q = threadIdx.x; q < V; q += 256) {
+        int x, y, z;
+        LY::decode(q, x, y, z);
+        const int p = TL::at(x, y, z);
+        T wm, wp;
+        rweights<D, M, T, RD>(ac, coord, pb, RD == 0 ? x : y, wm, wp);
+        T Lu = tstencil<D, M, T, RD>(tile, p, tile[p], bp.cc, wm, wp);
+        const size_t o = (size_t)pb * V + q;
+        bout[o] = -(r[o] + Lu / bp.s);
+    }
+}
+
 // ---- electric field ----
 
 template <typename T>

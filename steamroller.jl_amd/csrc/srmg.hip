@@ -814,6 +814,15 @@ struct Ops {
     void postrestrict(FieldBuf &b, FieldBuf &u, FieldBuf &r, int l, const LevelArgs<T> &ac) {
         const LevelInfo &L = c->lv[l];
         if (!L.nblk) return;
+        if (grouped(l)) {
+            with_rd(ac.rdim, [&](auto rd) {
+                constexpr int RD = decltype(rd)::value;
+                hipLaunchKernelGGL((k_postrestrict_group<D, M, T, RD>), dim3((unsigned)(L.nblk >> D)), dim3(256), 0,
+                                   c->stream, X(b, l - 1), X(u, l - 1), X(r, l - 1), I(u), c->d_halo.as<int>(),
+                                   coord(), c->d_parent.as<int>(), L.blk0, ac);
+            });
+            return;
+        }
         hipLaunchKernelGGL((k_postrestrict<D, M, T>), dim3(nwg((int64_t)L.nblk * (LY::V / LY::NC))), dim3(WG), 0,
                            c->stream, X(b, l - 1), X(u, l - 1), X(r, l - 1), I(u), nbr(), coord(),
                            c->d_parent.as<int>(), c->d_sub.as<int>(), L.blk0, L.nblk, ac);
