@@ -1544,10 +1544,71 @@ __global__ __launch_bounds__(WG) void k_eig_scale(double *__restrict__ g, const 
 
 // ---- mixed precision helpers ----
 
-// u += e over all cells and interpolation-ghost slots (FP64 += FP32)
+// u += e over n cells (FP64 += FP32)
 __global__ __launch_bounds__(WG) void k_add_correction(double *__restrict__ u, const float *__restrict__ e, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) u[i] += (double)e[i];
+}
+
+// The end of a mixed-precision outer step on a grouped level, run from the
+// finest level down: on every block of sibling group g, u += e if the block is
+// a leaf (a covered block already holds the restriction of its children,
+// written by the finer level's launch, and u += e there would be overwritten
+// anyway) and e = 0; then the parent's cells get the restriction of the
+// group's u (k_restrict's order) and e = 0. Work groups past ngroups do
+// u += e, e = 0 on the leaf blocks leaves[0..nleafc) of the coarser level
+// (used for the coarsest level, which has no parent). The values equal those
+// of u += e everywhere followed by restrict! level by level.
+template <int D, int M>
+__global__ __launch_bounds__(256) void k_add_restrict(double *__restrict__ u, float *__restrict__ e,
+                                                      const int *__restrict__ isleaf, const int *__restrict__ parent,
+                                                      int blk0, unsigned ngroups, const int *__restrict__ leaves,
+                                                      int nleafc) {
+    using LY = Lay<D, M>;
+    constexpr int NB = 1 << D, V = LY::V, MH = LY::MH;
+    const int tid = threadIdx.x;
+    if (blockIdx.x >= ngroups) {
+        for (int i = (blockIdx.x - ngroups) * 256 + tid; i < (blockIdx.x - ngroups + 1) * 256 && i < nleafc * V;
+             i += 256) {
+            const size_t o = (size_t)leaves[i / V] * V + (size_t)(i % V);
+            u[o] += (double)e[o];
+            e[o] = 0.0f;
+        }
+        return;
+    }
+    __shared__ double ut[NB * V];
+    __shared__ int leaf[NB];
+    const int g0 = blk0 + (int)blockIdx.x * NB;
+    if (tid < NB) leaf[tid] = isleaf[g0 + tid];
+    __syncthreads();
+    for (int i = tid; i < NB * V; i += 256) {
+        const int k = i / V;
+        const size_t o = (size_t)g0 * V + i;
+        double v = u[o];
+        if (leaf[k]) {
+            v += (double)e[o];
+            u[o] = v;
+        }
+        e[o] = 0.0f;
+        ut[i] = v;
+    }
+    __syncthreads();
+    const int pb = parent[g0];
+    for (int i = tid; i < V; i += 256) {
+        int X0, X1, X2;
+        LY::decode(i, X0, X1, X2);
+        const int k = (X0 >= MH ? 1 : 0) | (X1 >= MH ? 2 : 0) | (D == 3 && X2 >= MH ? 4 : 0);
+        const int j0 = X0 - (k & 1) * MH, j1 = X1 - ((k >> 1) & 1) * MH, j2 = (D == 3) ? X2 - ((k >> 2) & 1) * MH : 0;
+        double f = 0.0;
+#pragma unroll
+        for (int q = 0; q < LY::NC; q++) {
+            const int i0 = q & 1, i1 = (q >> 1) & 1, i2 = (q >> 2) & 1;
+            f += ut[k * V + LY::sidx(2 * j0 + i0, 2 * j1 + i1, D == 3 ? 2 * j2 + i2 : 0)];
+        }
+        const size_t o = (size_t)pb * V + i;
+        u[o] = f / double(LY::NC);
+        e[o] = 0.0f;
+    }
 }
 
 }  // namespace srmg

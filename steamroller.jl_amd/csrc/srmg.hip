@@ -1557,17 +1557,41 @@ static void solve_mixed_dm(srmg_ctx *c, FieldBuf &u, FieldBuf &b, FieldBuf &e, F
         if (o->guess) o64.first_iterate(u, b, o->guess, o->guess_smooth, r0);
     }
     auto defect_and_norm = [&] { o64.defect_and_norm(be.data.as<float>(), u, b, s, op); };
+    // on trees whose refined levels are all grouped, the add, the restriction
+    // and the zeroing of e for the next step run as one launch per level
+    bool fused = (Ops<D, M, double>::GROUP_OK) && c->group_tiles && c->nlevels > 1;
+    for (int l = 1; l < c->nlevels && fused; l++) fused = c->lv[l].nblk > 0 && c->lv[l].grouped;
+    const size_t ncell = (size_t)c->nb * LY::V, nib = e.ib.bytes / sizeof(float);
+    if (fused)
+        hipLaunchKernelGGL(k_fill<float>, dim3(nwg(ncell)), dim3(WG), 0, c->stream, e.data.as<float>(), 0.0f, ncell);
     auto correct = [&] {
-        size_t n = (size_t)c->nb * LY::V, ni = e.ib.bytes / sizeof(float);
-        hipLaunchKernelGGL(k_fill<float>, dim3(nwg(n)), dim3(WG), 0, c->stream, e.data.as<float>(), 0.0f, n);
-        hipLaunchKernelGGL(k_fill<float>, dim3(nwg(ni)), dim3(WG), 0, c->stream, e.ib.as<float>(), 0.0f, ni);
+        if (!fused)
+            hipLaunchKernelGGL(k_fill<float>, dim3(nwg(ncell)), dim3(WG), 0, c->stream, e.data.as<float>(), 0.0f,
+                               ncell);
+        hipLaunchKernelGGL(k_fill<float>, dim3(nwg(nib)), dim3(WG), 0, c->stream, e.ib.as<float>(), 0.0f, nib);
         for (int k = 0; k < std::max(1, (int)o->inner_cycles); k++)
             o32.vcycle(e, be, re, ue, s, op, 0, o->ndown, o->ntop, o->nup, o->omega);
-        hipLaunchKernelGGL(k_add_correction, dim3(nwg(n)), dim3(WG), 0, c->stream, u.data.as<double>(),
-                           e.data.as<float>(), n);
-        // covered coarse cells = restriction of their children, as at the FAS
-        // fixed point; otherwise the leaf residual would not pin them
-        for (int l = c->nlevels - 1; l >= 1; l--) o64.restrict_(u, l);
+        if constexpr (Ops<D, M, double>::GROUP_OK) {
+            if (fused) {
+            // u += e, e = 0 and the restriction of u, finest level first
+            for (int l = c->nlevels - 1; l >= 1; l--) {
+                const LevelInfo &L = c->lv[l], &L0 = c->lv[0];
+                const unsigned ng = (unsigned)(L.nblk >> D);
+                const int nleafc = (l == 1) ? L0.nleaf : 0;
+                const unsigned nx = (unsigned)(((int64_t)nleafc * LY::V + 255) / 256);
+                hipLaunchKernelGGL((k_add_restrict<D, M>), dim3(ng + nx), dim3(256), 0, c->stream,
+                                   u.data.as<double>(), e.data.as<float>(), c->d_isleaf.as<int>(),
+                                   c->d_parent.as<int>(), L.blk0, ng, c->d_leaves.as<int>() + L0.leaf0, nleafc);
+            }
+            }
+        }
+        if (!fused) {
+            hipLaunchKernelGGL(k_add_correction, dim3(nwg(ncell)), dim3(WG), 0, c->stream, u.data.as<double>(),
+                               e.data.as<float>(), ncell);
+            // covered coarse cells = restriction of their children, as at the FAS
+            // fixed point; otherwise the leaf residual would not pin them
+            for (int l = c->nlevels - 1; l >= 1; l--) o64.restrict_(u, l);
+        }
         for (int l = 1; l < c->nlevels; l++) o64.fill_interp(u, l, level_args<double>(c, l - 1, s, op, 1.0));
         defect_and_norm();
     };
