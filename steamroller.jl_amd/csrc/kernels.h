@@ -463,11 +463,24 @@ __device__ __forceinline__ void rweights(const LevelArgs<T> &a, const int *__res
     }
 }
 
-// Load the depth-1 halo cell number t of face F (a compile-time face) of block
-// blk into the tile: neighbour cell, mirror of the facing cell, or frozen slab.
-template <int D, int M, typename T, int F, class TL = Tile<D, M>>
-__device__ __forceinline__ void halo_cell(T *tile, int e, int blk, int t, const T *__restrict__ x,
-                                          const T *__restrict__ ib, T sg) {
+// Depth-1 halo cell number t of face F (a compile-time face) of block blk:
+// its tile position, and its value (neighbour cell, mirror of the facing cell,
+// or frozen slab).
+template <int D, int M, int F, class TL = Tile<D, M>>
+__device__ __forceinline__ int halo_at(int t) {
+    constexpr int DIM = F >> 1, DIR = (F & 1) ? 1 : -1;
+    int c[3] = {0, 0, 0};
+    int ta = t % M, tb = t / M;
+    if (DIM == 0) { c[1] = ta; c[2] = tb; }
+    else if (DIM == 1) { c[0] = ta; c[2] = tb; }
+    else { c[0] = ta; c[1] = tb; }
+    c[DIM] = DIR > 0 ? M : -1;
+    return TL::at(c[0], c[1], c[2]);
+}
+
+template <int D, int M, typename T, int F>
+__device__ __forceinline__ T halo_value(int e, int blk, int t, const T *__restrict__ x, const T *__restrict__ ib,
+                                        T sg) {
     using LY = Lay<D, M>;
     constexpr int DIM = F >> 1, DIR = (F & 1) ? 1 : -1;
     int c[3] = {0, 0, 0};
@@ -475,33 +488,53 @@ __device__ __forceinline__ void halo_cell(T *tile, int e, int blk, int t, const 
     if (DIM == 0) { c[1] = ta; c[2] = tb; }
     else if (DIM == 1) { c[0] = ta; c[2] = tb; }
     else { c[0] = ta; c[1] = tb; }
-    T v;
     if (e >= 0) {
         c[DIM] = DIR > 0 ? 0 : M - 1;
-        v = x[(size_t)e * LY::V + LY::sidx(c[0], c[1], c[2])];
+        return x[(size_t)e * LY::V + LY::sidx(c[0], c[1], c[2])];
     } else if (e == NB_BOUNDARY) {
         c[DIM] = DIR > 0 ? M - 1 : 0;
-        v = sg * x[(size_t)blk * LY::V + LY::sidx(c[0], c[1], c[2])];
-    } else {
-        v = ib[(size_t)nb_slot(e) * LY::F + t];
+        return sg * x[(size_t)blk * LY::V + LY::sidx(c[0], c[1], c[2])];
     }
-    c[DIM] = DIR > 0 ? M : -1;
-    tile[TL::at(c[0], c[1], c[2])] = v;
+    return ib[(size_t)nb_slot(e) * LY::F + t];
 }
 
-// halo_cell for a face known only at run time (uniform per wave where the
-// work layout allows it)
+template <int D, int M, typename T, int F, class TL = Tile<D, M>>
+__device__ __forceinline__ void halo_cell(T *tile, int e, int blk, int t, const T *__restrict__ x,
+                                          const T *__restrict__ ib, T sg) {
+    tile[halo_at<D, M, F, TL>(t)] = halo_value<D, M, T, F>(e, blk, t, x, ib, sg);
+}
+
+// face number f known only at run time (uniform per wave where the work
+// layout allows it)
+#define SRMG_FACE_SWITCH(f, EXPR)                \
+    switch (f) {                                 \
+        case 0: { constexpr int FF = 0; EXPR; } break; \
+        case 1: { constexpr int FF = 1; EXPR; } break; \
+        case 2: { constexpr int FF = 2; EXPR; } break; \
+        case 3: { constexpr int FF = 3; EXPR; } break; \
+        case 4: { constexpr int FF = (D == 3 ? 4 : 3); EXPR; } break; \
+        default: { constexpr int FF = (D == 3 ? 5 : 3); EXPR; } break; \
+    }
+
 template <int D, int M, typename T, class TL = Tile<D, M>>
 __device__ __forceinline__ void halo_switch(T *tl, int e, int f, int blk, int t, const T *__restrict__ x,
                                             const T *__restrict__ ib, const T *sg) {
-    switch (f) {
-        case 0: halo_cell<D, M, T, 0, TL>(tl, e, blk, t, x, ib, sg[0]); break;
-        case 1: halo_cell<D, M, T, 1, TL>(tl, e, blk, t, x, ib, sg[1]); break;
-        case 2: halo_cell<D, M, T, 2, TL>(tl, e, blk, t, x, ib, sg[2]); break;
-        case 3: halo_cell<D, M, T, 3, TL>(tl, e, blk, t, x, ib, sg[3]); break;
-        case 4: halo_cell<D, M, T, (D == 3 ? 4 : 3), TL>(tl, e, blk, t, x, ib, sg[4]); break;
-        default: halo_cell<D, M, T, (D == 3 ? 5 : 3), TL>(tl, e, blk, t, x, ib, sg[5]); break;
-    }
+    SRMG_FACE_SWITCH(f, (halo_cell<D, M, T, FF, TL>(tl, e, blk, t, x, ib, sg[FF])))
+}
+
+template <int D, int M, typename T>
+__device__ __forceinline__ T halo_value_sw(int f, int e, int blk, int t, const T *__restrict__ x,
+                                           const T *__restrict__ ib, const T *sg) {
+    T v = T(0);
+    SRMG_FACE_SWITCH(f, (v = halo_value<D, M, T, FF>(e, blk, t, x, ib, sg[FF])))
+    return v;
+}
+
+template <int D, int M, class TL>
+__device__ __forceinline__ int halo_at_sw(int f, int t) {
+    int p = 0;
+    SRMG_FACE_SWITCH(f, (p = halo_at<D, M, FF, TL>(t)))
+    return p;
 }
 
 // Neighbour face-layer cells recomputed by the fused sweep: for the face
@@ -527,7 +560,7 @@ struct RecFace {
 
     // operands outside the tile: one layer deeper in the neighbour, its
     // right-hand side, and its ghosts across its side faces
-    __device__ static __forceinline__ void prefetch(int dir, int tq, int e, const int *tb_own, const T *__restrict__ x,
+    __device__ static __forceinline__ void prefetch(int dir, int tq, int e, const int *gs, const T *__restrict__ x,
                                                     const T *__restrict__ b, const T *__restrict__ ib, T &rd, T &rb,
                                                     T *rg) {
         int cB[3];
@@ -536,14 +569,13 @@ struct RecFace {
         c2[DIM] = dir > 0 ? 1 : M - 2;
         rd = x[(size_t)e * LY::V + LY::sidx(c2[0], c2[1], c2[2])];
         rb = b[(size_t)e * LY::V + LY::sidx(cB[0], cB[1], cB[2])];
-        const int f = 2 * DIM + (dir > 0 ? 1 : 0);
 #pragma unroll
         for (int k = 0; k < NS; k++) {
             const int dd = (k < 2) ? DA : DB;
             const int side = k & 1;
             rg[k] = T(0);
             if (cB[dd] != (side ? M - 1 : 0)) continue;
-            int g = tb_own[HT::ND + f * HT::NS + k];
+            int g = gs[k];
             if (g >= 0) {
                 int c3[3] = {cB[0], cB[1], cB[2]};
                 c3[dd] = side ? 0 : M - 1;
@@ -556,12 +588,11 @@ struct RecFace {
 
     // first-half-sweep value of the neighbour cell, written into this block's halo
     template <int RD>
-    __device__ static __forceinline__ void compute(int dir, int tq, int e, const int *tb_own, T *tile,
+    __device__ static __forceinline__ void compute(int dir, int tq, int e, const int *gs, T *tile,
                                                    const int *__restrict__ coord, const LevelArgs<T> &a,
                                                    const BP<T> &bp, T rd, T rb, const T *rg) {
         int cB[3];
         cell(dir, tq, cB);
-        const int f = 2 * DIM + (dir > 0 ? 1 : 0);
         int ch[3] = {cB[0], cB[1], cB[2]};
         ch[DIM] = dir > 0 ? M : -1;
         const int ph = TL::at(ch[0], ch[1], ch[2]);
@@ -583,7 +614,7 @@ struct RecFace {
             if (cc >= 0 && cc < M) {
                 v = tile[ph + (side ? 1 : -1) * TL::stride(dd)];  // along the face layer: in the halo
             } else {
-                int g = tb_own[HT::ND + f * HT::NS + k];
+                int g = gs[k];
                 v = (g == NB_BOUNDARY) ? a.sg[2 * dd + side] * xo : rg[k];
             }
             if (side) nbp[dd] = v; else nbm[dd] = v;
@@ -686,14 +717,14 @@ __global__ __launch_bounds__(NT) void k_gs_sweep(T *__restrict__ uo, const T *__
         int lb = j / (ND * F2), rem = j - lb * (ND * F2);
         int f = rem / F2, tq = rem - f * F2;
         if (lb >= nloc) continue;
-        const int *tb = tab + lb * HT::N;
+        const int *tb = tab + lb * HT::N, *gs = tb + HT::ND + f * HT::NS;
         int e = tb[f];
         if (e < 0) continue;
         rj[k] = j;
         int dim = f >> 1, dir = (f & 1) ? 1 : -1;
-        if (dim == 0) RecFace<D, M, T, 0>::prefetch(dir, tq, e, tb, ui, b, ib, rd[k], rb[k], rg[k]);
-        else if (dim == 1) RecFace<D, M, T, 1>::prefetch(dir, tq, e, tb, ui, b, ib, rd[k], rb[k], rg[k]);
-        else RecFace<D, M, T, (D == 3 ? 2 : 1)>::prefetch(dir, tq, e, tb, ui, b, ib, rd[k], rb[k], rg[k]);
+        if (dim == 0) RecFace<D, M, T, 0>::prefetch(dir, tq, e, gs, ui, b, ib, rd[k], rb[k], rg[k]);
+        else if (dim == 1) RecFace<D, M, T, 1>::prefetch(dir, tq, e, gs, ui, b, ib, rd[k], rb[k], rg[k]);
+        else RecFace<D, M, T, (D == 3 ? 2 : 1)>::prefetch(dir, tq, e, gs, ui, b, ib, rd[k], rb[k], rg[k]);
     }
     __syncthreads();
 
@@ -720,17 +751,17 @@ __global__ __launch_bounds__(NT) void k_gs_sweep(T *__restrict__ uo, const T *__
         if (j < 0) continue;
         int lb = j / (ND * F2), rem = j - lb * (ND * F2);
         int f = rem / F2, tq = rem - f * F2;
-        const int *tb = tab + lb * HT::N;
+        const int *tb = tab + lb * HT::N, *gs = tb + HT::ND + f * HT::NS;
         T *tl = tile + lb * TL::SZ;
         int e = tb[f];
         int dim = f >> 1, dir = (f & 1) ? 1 : -1;
         const BP<T> bp = prm[lb];
         if (dim == 0)
-            RecFace<D, M, T, 0>::template compute<RD>(dir, tq, e, tb, tl, coord, a, bp, rd[k], rb[k], rg[k]);
+            RecFace<D, M, T, 0>::template compute<RD>(dir, tq, e, gs, tl, coord, a, bp, rd[k], rb[k], rg[k]);
         else if (dim == 1)
-            RecFace<D, M, T, 1>::template compute<RD>(dir, tq, e, tb, tl, coord, a, bp, rd[k], rb[k], rg[k]);
+            RecFace<D, M, T, 1>::template compute<RD>(dir, tq, e, gs, tl, coord, a, bp, rd[k], rb[k], rg[k]);
         else
-            RecFace<D, M, T, (D == 3 ? 2 : 1)>::template compute<RD>(dir, tq, e, tb, tl, coord, a, bp, rd[k], rb[k],
+            RecFace<D, M, T, (D == 3 ? 2 : 1)>::template compute<RD>(dir, tq, e, gs, tl, coord, a, bp, rd[k], rb[k],
                                                                      rg[k]);
     }
     __syncthreads();
@@ -822,14 +853,14 @@ __global__ __launch_bounds__(NT) void k_gs_sweep_group(T *__restrict__ uo, const
         const int lb = j / (D * F2), rem = j - lb * (D * F2);
         const int d = rem / F2, tq = rem - d * F2;
         const int f = 2 * d + ((lb >> d) & 1);
-        const int *tb = tab + lb * HT::N;
+        const int *tb = tab + lb * HT::N, *gs = tb + HT::ND + f * HT::NS;
         const int e = tb[f];
         if (e < 0) continue;
         rj[k] = j;
         const int dir = (f & 1) ? 1 : -1;
-        if (d == 0) RecFace<D, M, T, 0, TL>::prefetch(dir, tq, e, tb, ui, b, ib, rd[k], rb[k], rg[k]);
-        else if (d == 1) RecFace<D, M, T, 1, TL>::prefetch(dir, tq, e, tb, ui, b, ib, rd[k], rb[k], rg[k]);
-        else RecFace<D, M, T, (D == 3 ? 2 : 1), TL>::prefetch(dir, tq, e, tb, ui, b, ib, rd[k], rb[k], rg[k]);
+        if (d == 0) RecFace<D, M, T, 0, TL>::prefetch(dir, tq, e, gs, ui, b, ib, rd[k], rb[k], rg[k]);
+        else if (d == 1) RecFace<D, M, T, 1, TL>::prefetch(dir, tq, e, gs, ui, b, ib, rd[k], rb[k], rg[k]);
+        else RecFace<D, M, T, (D == 3 ? 2 : 1), TL>::prefetch(dir, tq, e, gs, ui, b, ib, rd[k], rb[k], rg[k]);
     }
     __syncthreads();
 
@@ -854,17 +885,17 @@ __global__ __launch_bounds__(NT) void k_gs_sweep_group(T *__restrict__ uo, const
         const int lb = j / (D * F2), rem = j - lb * (D * F2);
         const int d = rem / F2, tq = rem - d * F2;
         const int f = 2 * d + ((lb >> d) & 1);
-        const int *tb = tab + lb * HT::N;
+        const int *tb = tab + lb * HT::N, *gs = tb + HT::ND + f * HT::NS;
         T *tl = tile + TL::origin(lb);
         const int e = tb[f];
         const int dir = (f & 1) ? 1 : -1;
         const BP<T> bp = prm[lb];
         if (d == 0)
-            RecFace<D, M, T, 0, TL>::template compute<RD>(dir, tq, e, tb, tl, coord, a, bp, rd[k], rb[k], rg[k]);
+            RecFace<D, M, T, 0, TL>::template compute<RD>(dir, tq, e, gs, tl, coord, a, bp, rd[k], rb[k], rg[k]);
         else if (d == 1)
-            RecFace<D, M, T, 1, TL>::template compute<RD>(dir, tq, e, tb, tl, coord, a, bp, rd[k], rb[k], rg[k]);
+            RecFace<D, M, T, 1, TL>::template compute<RD>(dir, tq, e, gs, tl, coord, a, bp, rd[k], rb[k], rg[k]);
         else
-            RecFace<D, M, T, (D == 3 ? 2 : 1), TL>::template compute<RD>(dir, tq, e, tb, tl, coord, a, bp, rd[k],
+            RecFace<D, M, T, (D == 3 ? 2 : 1), TL>::template compute<RD>(dir, tq, e, gs, tl, coord, a, bp, rd[k],
                                                                          rb[k], rg[k]);
     }
     __syncthreads();
@@ -884,6 +915,153 @@ __global__ __launch_bounds__(NT) void k_gs_sweep_group(T *__restrict__ uo, const
         uo[o] = ux[k][0];
         uo[o + H] = fma_t<T>(bp.nwc, fma_t<T>(bp.s, be[k], Lu), ux[k][1]);
     }
+}
+
+// k_gs_sweep_group for D = 3, M = 8 with a column mapping: 256 threads per
+// sibling group (16^3 cells), each thread owns one (X, Y) column of the group
+// tile and keeps its 16 cells and right-hand sides in registers, so that the
+// Z neighbours of a cell come from registers and only the X and Y neighbours
+// from the LDS tile. Wave w covers rows Y in [8 (w >> 1), 8 (w >> 1) + 8) and
+// the columns with X + Y = w (mod 2): the red cells of all its lanes then sit
+// at the same Z (Z = w mod 2, step 2) and the colour tests are wave-uniform.
+// Halo cells, neighbour recompute and the order of every sum as in
+// k_gs_sweep_group, so the same bits. Round trips: the group's halo-table
+// rows (issued before the interior loads, waited for alone), then the halo
+// cells and recompute operands with the interior loads still in flight.
+template <int D, int M, typename T, int RD>
+__global__ __launch_bounds__(256) void k_gs_sweep_col(T *__restrict__ uo, const T *__restrict__ ui,
+                                                      const T *__restrict__ b, const T *__restrict__ ib,
+                                                      const int *__restrict__ htab, const int *__restrict__ coord,
+                                                      LevelArgs<T> a) {
+    static_assert(D == 3 && M == 8 && RD < 0, "column sweep is written for 8^3 Cartesian blocks");
+    using LY = Lay<D, M>;
+    using TL = Tile<D, M, 2>;
+    using HT = Halo<D>;
+    constexpr int NB = 8, V = LY::V, H = LY::H, S = TL::S, S2 = S * S, NG = 2 * M;
+    __shared__ T tile[TL::SZ];
+    __shared__ int tab[NB * HT::N];
+    __shared__ BP<T> prm[NB];
+    const int blk0 = a.blk0 + xcd_remap(blockIdx.x, gridDim.x) * NB;
+    const int tid = threadIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    const int par = w & 1;
+    const int Y = 8 * (w >> 1) + (lane >> 3), X = 2 * (lane & 7) + ((Y + par) & 1);
+    const int x = X & 7, y = Y & 7, kxy = (X >> 3) | ((Y >> 3) << 1);
+
+    // halo-table rows of the group, first
+    int tv = 0;
+    if (tid < NB * HT::N) tv = htab[(size_t)blk0 * HT::N + tid];
+    // interior: cell Z of the column is red (first half-sweep, storage half
+    // 0) when Z = par (mod 2)
+    const size_t g0 = (size_t)(blk0 + kxy) * V + (size_t)((x >> 1) + LY::MH * y);
+    T uc[NG], bc[NG];
+#pragma unroll
+    for (int Z = 0; Z < NG; Z++) {
+        const size_t o = g0 + (size_t)((Z >> 3) * 4 * V + LY::MH * M * (Z & 7) + (((Z + par) & 1) ? H : 0));
+        uc[Z] = ui[o];
+        bc[Z] = b[o];
+    }
+    BP<T> bpl{};
+    if (tid < NB) bpl = bparams(a, blk0 + tid);
+    if (tid < NB * HT::N) tab[tid] = tv;
+    if (tid < NB) prm[tid] = bpl;
+    __syncthreads();
+
+    // halo cells of the six group faces: wave w takes the block face w of
+    // each (tangential bits of the block from w), lane = cell t of the face
+    T hv[6];
+#define SRMG_COL_HALO(F)                                                                                   \
+    {                                                                                                      \
+        constexpr int DIM = (F) >> 1, DA = DIM == 0 ? 1 : 0, DB = DIM == 2 ? 1 : 2;                       \
+        const int k = (((F) & 1) << DIM) | ((w & 1) << DA) | ((w >> 1) << DB);                           \
+        hv[F] = halo_value<D, M, T, (F)>(tab[k * HT::N + (F)], blk0 + k, lane, ui, ib, a.sg[F]);          \
+    }
+    SRMG_COL_HALO(0) SRMG_COL_HALO(1) SRMG_COL_HALO(2) SRMG_COL_HALO(3) SRMG_COL_HALO(4) SRMG_COL_HALO(5)
+#undef SRMG_COL_HALO
+    // recompute operands: for dimension DIM, waves 0-1 the low group face,
+    // waves 2-3 the high one; half-wave = block face, tq = lane & 31
+    const int rside = w >> 1, rbf = ((w & 1) << 1) | ((lane >> 5) & 1), rtq = lane & 31;
+    T rd[3], rb[3], rg[3][4];
+    int re[3];
+#define SRMG_COL_RPRE(DIM)                                                                                 \
+    {                                                                                                      \
+        constexpr int DA = DIM == 0 ? 1 : 0, DB = DIM == 2 ? 1 : 2;                                       \
+        const int k = (rside << DIM) | ((rbf & 1) << DA) | ((rbf >> 1) << DB), f = 2 * DIM + rside;      \
+        const int *tb = tab + k * HT::N;                                                                  \
+        re[DIM] = tb[f];                                                                                  \
+        rd[DIM] = rb[DIM] = T(0);                                                                         \
+        if (re[DIM] >= 0)                                                                                 \
+            RecFace<D, M, T, DIM, TL>::prefetch(rside ? 1 : -1, rtq, re[DIM], tb + HT::ND + f * HT::NS, ui, b, \
+                                                ib, rd[DIM], rb[DIM], rg[DIM]);                           \
+    }
+    SRMG_COL_RPRE(0) SRMG_COL_RPRE(1) SRMG_COL_RPRE(2)
+#undef SRMG_COL_RPRE
+    // the tile
+    const int pc = TL::at(X, Y, 0);
+#pragma unroll
+    for (int Z = 0; Z < NG; Z++) tile[pc + Z * S2] = uc[Z];
+#define SRMG_COL_HPUT(F)                                                                                   \
+    {                                                                                                      \
+        constexpr int DIM = (F) >> 1, DA = DIM == 0 ? 1 : 0, DB = DIM == 2 ? 1 : 2;                       \
+        const int k = (((F) & 1) << DIM) | ((w & 1) << DA) | ((w >> 1) << DB);                           \
+        tile[TL::origin(k) + halo_at<D, M, (F), TL>(lane)] = hv[F];                                       \
+    }
+    SRMG_COL_HPUT(0) SRMG_COL_HPUT(1) SRMG_COL_HPUT(2) SRMG_COL_HPUT(3) SRMG_COL_HPUT(4) SRMG_COL_HPUT(5)
+#undef SRMG_COL_HPUT
+    __syncthreads();
+
+    // one cell update of the column (tstencil's order of terms)
+    const BP<T> bq[2] = {prm[kxy], prm[kxy | 4]};
+    auto update = [&](int Z, T zm, T zp) {
+        const int p = pc + Z * S2;
+        const BP<T> &bp = bq[Z >> 3];
+        T v = T(0);
+        v += zm;
+        v += tile[p - S];
+        v += tile[p - 1];
+        v += bp.cc * uc[Z];
+        v += tile[p + 1];
+        v += tile[p + S];
+        v += zp;
+        return fma_t<T>(bp.nwc, fma_t<T>(bp.s, bc[Z], v), uc[Z]);
+    };
+    // first half-sweep: red cells
+#pragma unroll
+    for (int Z = 0; Z < NG; Z++) {
+        if (((Z + par) & 1) == 0) {
+            const T zm = Z == 0 ? tile[pc - S2] : uc[Z > 0 ? Z - 1 : 0];
+            const T zp = Z == NG - 1 ? tile[pc + NG * S2] : uc[Z < NG - 1 ? Z + 1 : 0];
+            uc[Z] = update(Z, zm, zp);
+            tile[pc + Z * S2] = uc[Z];
+        }
+    }
+    // the neighbours' face-layer cells
+#define SRMG_COL_RCMP(DIM)                                                                                 \
+    {                                                                                                      \
+        constexpr int DA = DIM == 0 ? 1 : 0, DB = DIM == 2 ? 1 : 2;                                       \
+        const int k = (rside << DIM) | ((rbf & 1) << DA) | ((rbf >> 1) << DB), f = 2 * DIM + rside;      \
+        if (re[DIM] >= 0)                                                                                 \
+            RecFace<D, M, T, DIM, TL>::template compute<RD>(rside ? 1 : -1, rtq, re[DIM],                 \
+                                                            tab + k * HT::N + HT::ND + f * HT::NS,        \
+                                                            tile + TL::origin(k), coord, a, prm[k], rd[DIM], \
+                                                            rb[DIM], rg[DIM]);                            \
+    }
+    SRMG_COL_RCMP(0) SRMG_COL_RCMP(1) SRMG_COL_RCMP(2)
+#undef SRMG_COL_RCMP
+    __syncthreads();
+
+    // second half-sweep: black cells; then the column goes out
+#pragma unroll
+    for (int Z = 0; Z < NG; Z++) {
+        if (((Z + par) & 1) == 1) {
+            const T zm = Z == 0 ? tile[pc - S2] : uc[Z > 0 ? Z - 1 : 0];
+            const T zp = Z == NG - 1 ? tile[pc + NG * S2] : uc[Z < NG - 1 ? Z + 1 : 0];
+            uc[Z] = update(Z, zm, zp);
+        }
+    }
+#pragma unroll
+    for (int Z = 0; Z < NG; Z++)
+        uo[g0 + (size_t)((Z >> 3) * 4 * V + LY::MH * M * (Z & 7) + (((Z + par) & 1) ? H : 0))] = uc[Z];
 }
 
 // ---- tiled residual kernels ----

@@ -245,7 +245,7 @@ def test_every_sweep_tile_shape_gives_identical_results(name, dtype):
     geom, disc = solver.geometry_from_code(p.geom), _disc(p)
     outs = []
     grouped = []
-    for cfg in range(7):
+    for cfg in range(solver.SWEEP_SHAPES):
         ctx, (u, b, r, u1) = _device(p, dtype)
         if cfg == 0:
             grouped = [ctx.level_grouped(l) for l in range(1, len(p.tree) + 1)]
