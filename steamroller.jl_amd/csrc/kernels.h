@@ -917,6 +917,14 @@ __global__ __launch_bounds__(NT) void k_gs_sweep_group(T *__restrict__ uo, const
     }
 }
 
+// Work-group barrier that orders LDS accesses only: global loads still in
+// flight stay in flight across it (__syncthreads would wait for them).
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // k_gs_sweep_group for D = 3, M = 8 with a column mapping: 256 threads per
 // sibling group (16^3 cells), each thread owns one (X, Y) column of the group
 // tile and keeps its 16 cells and right-hand sides in registers, so that the
@@ -929,7 +937,7 @@ __global__ __launch_bounds__(NT) void k_gs_sweep_group(T *__restrict__ uo, const
 // rows (issued before the interior loads, waited for alone), then the halo
 // cells and recompute operands with the interior loads still in flight.
 template <int D, int M, typename T, int RD>
-__global__ __launch_bounds__(256) void k_gs_sweep_col(T *__restrict__ uo, const T *__restrict__ ui,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 1))) void k_gs_sweep_col(T *__restrict__ uo, const T *__restrict__ ui,
                                                       const T *__restrict__ b, const T *__restrict__ ib,
                                                       const int *__restrict__ htab, const int *__restrict__ coord,
                                                       LevelArgs<T> a) {
@@ -948,9 +956,11 @@ __global__ __launch_bounds__(256) void k_gs_sweep_col(T *__restrict__ uo, const 
     const int Y = 8 * (w >> 1) + (lane >> 3), X = 2 * (lane & 7) + ((Y + par) & 1);
     const int x = X & 7, y = Y & 7, kxy = (X >> 3) | ((Y >> 3) << 1);
 
-    // halo-table rows of the group, first
+    // halo-table rows and block parameters of the group, first
     int tv = 0;
     if (tid < NB * HT::N) tv = htab[(size_t)blk0 * HT::N + tid];
+    BP<T> bpl{};
+    if (tid < NB) bpl = bparams(a, blk0 + tid);
     // interior: cell Z of the column is red (first half-sweep, storage half
     // 0) when Z = par (mod 2)
     const size_t g0 = (size_t)(blk0 + kxy) * V + (size_t)((x >> 1) + LY::MH * y);
@@ -961,20 +971,19 @@ __global__ __launch_bounds__(256) void k_gs_sweep_col(T *__restrict__ uo, const 
         uc[Z] = ui[o];
         bc[Z] = b[o];
     }
-    BP<T> bpl{};
-    if (tid < NB) bpl = bparams(a, blk0 + tid);
     if (tid < NB * HT::N) tab[tid] = tv;
     if (tid < NB) prm[tid] = bpl;
-    __syncthreads();
+    lds_barrier();
 
     // halo cells of the six group faces: wave w takes the block face w of
     // each (tangential bits of the block from w), lane = cell t of the face
-    T hv[6];
+    T hv[6], hm[6];  // halo value = hm * hv (hm: mirror factor of a boundary face, else 1)
 #define SRMG_COL_HALO(F)                                                                                   \
     {                                                                                                      \
         constexpr int DIM = (F) >> 1, DA = DIM == 0 ? 1 : 0, DB = DIM == 2 ? 1 : 2;                       \
         const int k = (((F) & 1) << DIM) | ((w & 1) << DA) | ((w >> 1) << DB);                           \
-        hv[F] = halo_value<D, M, T, (F)>(tab[k * HT::N + (F)], blk0 + k, lane, ui, ib, a.sg[F]);          \
+        hv[F] = halo_value<D, M, T, (F)>(tab[k * HT::N + (F)], blk0 + k, lane, ui, ib, T(1));             \
+        hm[F] = tab[k * HT::N + (F)] == NB_BOUNDARY ? a.sg[F] : T(1);                                     \
     }
     SRMG_COL_HALO(0) SRMG_COL_HALO(1) SRMG_COL_HALO(2) SRMG_COL_HALO(3) SRMG_COL_HALO(4) SRMG_COL_HALO(5)
 #undef SRMG_COL_HALO
@@ -1004,11 +1013,11 @@ __global__ __launch_bounds__(256) void k_gs_sweep_col(T *__restrict__ uo, const 
     {                                                                                                      \
         constexpr int DIM = (F) >> 1, DA = DIM == 0 ? 1 : 0, DB = DIM == 2 ? 1 : 2;                       \
         const int k = (((F) & 1) << DIM) | ((w & 1) << DA) | ((w >> 1) << DB);                           \
-        tile[TL::origin(k) + halo_at<D, M, (F), TL>(lane)] = hv[F];                                       \
+        tile[TL::origin(k) + halo_at<D, M, (F), TL>(lane)] = hm[F] * hv[F];                               \
     }
     SRMG_COL_HPUT(0) SRMG_COL_HPUT(1) SRMG_COL_HPUT(2) SRMG_COL_HPUT(3) SRMG_COL_HPUT(4) SRMG_COL_HPUT(5)
 #undef SRMG_COL_HPUT
-    __syncthreads();
+    lds_barrier();
 
     // one cell update of the column (tstencil's order of terms)
     const BP<T> bq[2] = {prm[kxy], prm[kxy | 4]};
@@ -1048,7 +1057,7 @@ __global__ __launch_bounds__(256) void k_gs_sweep_col(T *__restrict__ uo, const 
     }
     SRMG_COL_RCMP(0) SRMG_COL_RCMP(1) SRMG_COL_RCMP(2)
 #undef SRMG_COL_RCMP
-    __syncthreads();
+    lds_barrier();
 
     // second half-sweep: black cells; then the column goes out
 #pragma unroll
