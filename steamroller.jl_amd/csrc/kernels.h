@@ -51,6 +51,15 @@ template <typename T> __device__ __forceinline__ T fma_t(T a, T b, T c);
 template <> __device__ __forceinline__ double fma_t<double>(double a, double b, double c) { return fma(a, b, c); }
 template <> __device__ __forceinline__ float fma_t<float>(float a, float b, float c) { return fmaf(a, b, c); }
 
+// f(integral_constant<int, i>) for i = 0 .. N-1, unrolled at compile time
+template <int N, int I = 0, class Fn>
+__device__ __forceinline__ void static_for(Fn &&f) {
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>());
+        static_for<N, I + 1>(f);
+    }
+}
+
 // Value of cell (c0,c1,c2) + dir e_dim of block blk, crossing the block face if
 // needed: same-level neighbour cell, mirror (depth-1 ghost = sign * own cell,
 // src/boundary.jl:45-47 with src/blockfield.jl:513-521), or the coarse-fine
@@ -404,20 +413,22 @@ __device__ __forceinline__ int xcd_remap(int i, int G) {
 }
 
 // LDS tile of G^D blocks of M^D cells with a depth-1 halo: cells -1 .. G M.
+// Rows hold R >= S entries (R > S pads rows against LDS bank conflicts).
 // Helpers that work on one block of a tile take a pointer offset to that
 // block's origin (the tile index is linear in the coordinates).
-template <int D, int M, int G = 1>
+template <int D, int M, int G = 1, int R = G * M + 2>
 struct Tile {
     static constexpr int S = G * M + 2;
-    static constexpr int SZ = (D == 2) ? S * S : S * S * S;
+    static_assert(R >= S, "row pitch below the row length");
+    static constexpr int SZ = (D == 2) ? R * S : R * S * S;
     __device__ static __forceinline__ int at(int x, int y, int z) {
-        return (x + 1) + S * ((y + 1) + (D == 3 ? S * (z + 1) : 0));
+        return (x + 1) + R * ((y + 1) + (D == 3 ? S * (z + 1) : 0));
     }
     __device__ static __forceinline__ int at3(const int *c) { return at(c[0], c[1], D == 3 ? c[2] : 0); }
-    __device__ static constexpr int stride(int d) { return d == 0 ? 1 : (d == 1 ? S : S * S); }
+    __host__ __device__ static constexpr int stride(int d) { return d == 0 ? 1 : (d == 1 ? R : R * S); }
     // offset of block k of a G = 2 group (k's bit d = upper half along d)
     __device__ static __forceinline__ int origin(int k) {
-        return M * ((k & 1) + S * (((k >> 1) & 1) + (D == 3 ? S * ((k >> 2) & 1) : 0)));
+        return M * ((k & 1) + R * (((k >> 1) & 1) + (D == 3 ? S * ((k >> 2) & 1) : 0)));
     }
 };
 
@@ -569,20 +580,21 @@ struct RecFace {
         c2[DIM] = dir > 0 ? 1 : M - 2;
         rd = x[(size_t)e * LY::V + LY::sidx(c2[0], c2[1], c2[2])];
         rb = b[(size_t)e * LY::V + LY::sidx(cB[0], cB[1], cB[2])];
+        // one guarded load per side face: the neighbour's cell (same level)
+        // or the frozen slab (coarse-fine); nothing at a domain boundary or
+        // off the edge of the face
 #pragma unroll
         for (int k = 0; k < NS; k++) {
             const int dd = (k < 2) ? DA : DB;
             const int side = k & 1;
+            const int g = gs[k];
+            int c3[3] = {cB[0], cB[1], cB[2]};
+            c3[dd] = side ? 0 : M - 1;
+            const size_t o = g >= 0 ? (size_t)g * LY::V + LY::sidx(c3[0], c3[1], c3[2])
+                                    : (size_t)max(nb_slot(g), 0) * LY::F + LY::faceidx(dd, cB[0], cB[1], cB[2]);
+            const T *src = g >= 0 ? x : ib;
             rg[k] = T(0);
-            if (cB[dd] != (side ? M - 1 : 0)) continue;
-            int g = gs[k];
-            if (g >= 0) {
-                int c3[3] = {cB[0], cB[1], cB[2]};
-                c3[dd] = side ? 0 : M - 1;
-                rg[k] = x[(size_t)g * LY::V + LY::sidx(c3[0], c3[1], c3[2])];
-            } else if (g != NB_BOUNDARY) {
-                rg[k] = ib[(size_t)nb_slot(g) * LY::F + LY::faceidx(dd, cB[0], cB[1], cB[2])];
-            }
+            if (cB[dd] == (side ? M - 1 : 0) && g != NB_BOUNDARY) rg[k] = src[o];
         }
     }
 
@@ -605,18 +617,17 @@ struct RecFace {
                               DIM == 2 ? (dir > 0 ? M - 1 : 0) : cB[2]);
         if (dir > 0) { nbm[DIM] = tile[pa]; nbp[DIM] = rd; }
         else { nbm[DIM] = rd; nbp[DIM] = tile[pa]; }
+        // along the face layer: in the halo; off the edge of the face: the
+        // prefetched ghost, or the mirror at a domain boundary (the tile
+        // position read for an off-edge cell is inside the tile and unused)
 #pragma unroll
         for (int k = 0; k < NS; k++) {
             const int dd = (k < 2) ? DA : DB;
             const int side = k & 1;
             const int cc = cB[dd] + (side ? 1 : -1);
-            T v;
-            if (cc >= 0 && cc < M) {
-                v = tile[ph + (side ? 1 : -1) * TL::stride(dd)];  // along the face layer: in the halo
-            } else {
-                int g = gs[k];
-                v = (g == NB_BOUNDARY) ? a.sg[2 * dd + side] * xo : rg[k];
-            }
+            const T vt = tile[ph + (side ? 1 : -1) * TL::stride(dd)];
+            const T vg = (gs[k] == NB_BOUNDARY) ? a.sg[2 * dd + side] * xo : rg[k];
+            const T v = (cc >= 0 && cc < M) ? vt : vg;
             if (side) nbp[dd] = v; else nbm[dd] = v;
         }
         T Lx = T(0);
@@ -931,146 +942,166 @@ __device__ __forceinline__ void lds_barrier() {
 // Z neighbours of a cell come from registers and only the X and Y neighbours
 // from the LDS tile. Wave w covers rows Y in [8 (w >> 1), 8 (w >> 1) + 8) and
 // the columns with X + Y = w (mod 2): the red cells of all its lanes then sit
-// at the same Z (Z = w mod 2, step 2) and the colour tests are wave-uniform.
-// Halo cells, neighbour recompute and the order of every sum as in
-// k_gs_sweep_group, so the same bits. Round trips: the group's halo-table
-// rows (issued before the interior loads, waited for alone), then the halo
-// cells and recompute operands with the interior loads still in flight.
+// at Z = P (mod 2), P = w & 1, a compile-time constant of the code path the
+// wave runs. Tile rows are padded to 24 entries (no bank conflicts in the X
+// and Y neighbour reads of a wave). Halo cells, neighbour recompute and the
+// order of every sum as in k_gs_sweep_group, so the same bits. Round trips:
+// the group's halo-table rows (issued before the interior loads and waited
+// for alone), then the halo cells and recompute operands with the interior
+// loads still in flight.
 template <int D, int M, typename T, int RD>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 1))) void k_gs_sweep_col(T *__restrict__ uo, const T *__restrict__ ui,
-                                                      const T *__restrict__ b, const T *__restrict__ ib,
-                                                      const int *__restrict__ htab, const int *__restrict__ coord,
-                                                      LevelArgs<T> a) {
+struct ColSweep {
     static_assert(D == 3 && M == 8 && RD < 0, "column sweep is written for 8^3 Cartesian blocks");
     using LY = Lay<D, M>;
-    using TL = Tile<D, M, 2>;
+    using TL = Tile<D, M, 2, 24>;
     using HT = Halo<D>;
-    constexpr int NB = 8, V = LY::V, H = LY::H, S = TL::S, S2 = S * S, NG = 2 * M;
-    __shared__ T tile[TL::SZ];
-    __shared__ int tab[NB * HT::N];
-    __shared__ BP<T> prm[NB];
-    const int blk0 = a.blk0 + xcd_remap(blockIdx.x, gridDim.x) * NB;
+    static constexpr int NB = 8, V = LY::V, H = LY::H, NG = 2 * M;
+    static constexpr int SY = TL::stride(1), SZ = TL::stride(2);
+    struct Smem {
+        T tile[TL::SZ];
+        int tab[NB * HT::N];
+        BP<T> prm[NB];
+    };
+
+    // offset of cell Z of a column from the column's base in its block (the
+    // lower block for Z < 8, the upper one, 4 blocks on, for Z >= 8; storage
+    // half 0 holds the red cells)
+    template <int P, int Z>
+    __device__ static constexpr int zoff() {
+        return LY::MH * M * (Z & 7) + (((Z + P) & 1) ? H : 0);
+    }
+
+    template <int P>
+    __device__ static __forceinline__ void run(Smem &sm, T *__restrict__ uo, const T *__restrict__ ui,
+                                               const T *__restrict__ b, const T *__restrict__ ib,
+                                               const int *__restrict__ coord, const LevelArgs<T> &a, int blk0,
+                                               int w, int lane, int tv, const BP<T> &bpl) {
+        const int tid = w * 64 + lane;
+        const int Y = 8 * (w >> 1) + (lane >> 3), X = 2 * (lane & 7) + ((Y + P) & 1);
+        const int x = X & 7, y = Y & 7, kxy = (X >> 3) | ((Y >> 3) << 1);
+        const size_t g0 = (size_t)(blk0 + kxy) * V + (size_t)((x >> 1) + LY::MH * y);
+        const T *__restrict__ ul = ui + g0, *__restrict__ uh = ul + 4 * V;
+        const T *__restrict__ bl = b + g0, *__restrict__ bh = bl + 4 * V;
+        T uc[NG], bc[NG];
+        static_for<NG>([&](auto zc) {
+            constexpr int Z = decltype(zc)::value;
+            uc[Z] = (Z < 8 ? ul : uh)[zoff<P, Z>()];
+            bc[Z] = (Z < 8 ? bl : bh)[zoff<P, Z>()];
+        });
+        if (tid < NB * HT::N) sm.tab[tid] = tv;
+        if (tid < NB) sm.prm[tid] = bpl;
+        lds_barrier();
+
+        // halo cells of the six group faces: wave w takes the block face w of
+        // each (tangential bits of the block from w), lane = cell t of the
+        // face; a boundary face's mirror factor is applied at the tile write
+        T hv[6], hm[6];
+        static_for<6>([&](auto fc) {
+            constexpr int F = decltype(fc)::value;
+            constexpr int DIM = F >> 1, DA = DIM == 0 ? 1 : 0, DB = DIM == 2 ? 1 : 2;
+            const int k = ((F & 1) << DIM) | ((w & 1) << DA) | ((w >> 1) << DB);
+            const int e = __builtin_amdgcn_readfirstlane(sm.tab[k * HT::N + F]);
+            hv[F] = halo_value<D, M, T, F>(e, blk0 + k, lane, ui, ib, T(1));
+            hm[F] = e == NB_BOUNDARY ? a.sg[F] : T(1);
+        });
+        // recompute operands: for dimension DIM, waves 0-1 the low group face,
+        // waves 2-3 the high one; half-wave = block face, tq = lane & 31
+        const int rside = w >> 1, rbf = ((w & 1) << 1) | ((lane >> 5) & 1), rtq = lane & 31;
+        T rd[3], rb[3], rg[3][4];
+        int re[3];
+        static_for<3>([&](auto dc) {
+            constexpr int DIM = decltype(dc)::value;
+            constexpr int DA = DIM == 0 ? 1 : 0, DB = DIM == 2 ? 1 : 2;
+            const int k = (rside << DIM) | ((rbf & 1) << DA) | ((rbf >> 1) << DB), f = 2 * DIM + rside;
+            const int *tb = sm.tab + k * HT::N;
+            re[DIM] = tb[f];
+            rd[DIM] = rb[DIM] = T(0);
+            if (re[DIM] >= 0)
+                RecFace<D, M, T, DIM, TL>::prefetch(rside ? 1 : -1, rtq, re[DIM], tb + HT::ND + f * HT::NS, ui, b, ib,
+                                                    rd[DIM], rb[DIM], rg[DIM]);
+        });
+        // the tile
+        const int pc = TL::at(X, Y, 0);
+        static_for<NG>([&](auto zc) {
+            constexpr int Z = decltype(zc)::value;
+            sm.tile[pc + Z * SZ] = uc[Z];
+        });
+        static_for<6>([&](auto fc) {
+            constexpr int F = decltype(fc)::value;
+            constexpr int DIM = F >> 1, DA = DIM == 0 ? 1 : 0, DB = DIM == 2 ? 1 : 2;
+            const int k = ((F & 1) << DIM) | ((w & 1) << DA) | ((w >> 1) << DB);
+            sm.tile[TL::origin(k) + halo_at<D, M, F, TL>(lane)] = hm[F] * hv[F];
+        });
+        lds_barrier();
+
+        // one cell update of the column (tstencil's order of terms)
+        const BP<T> bq[2] = {sm.prm[kxy], sm.prm[kxy | 4]};
+        T *const t = sm.tile;
+        auto update = [&](auto zc) {
+            constexpr int Z = decltype(zc)::value;
+            const int p = pc + Z * SZ;
+            const BP<T> &bp = bq[Z >> 3];
+            const T zm = (Z == 0) ? t[pc - SZ] : uc[Z > 0 ? Z - 1 : 0];
+            const T zp = (Z == NG - 1) ? t[pc + NG * SZ] : uc[Z < NG - 1 ? Z + 1 : 0];
+            T v = T(0);
+            v += zm;
+            v += t[p - SY];
+            v += t[p - 1];
+            v += bp.cc * uc[Z];
+            v += t[p + 1];
+            v += t[p + SY];
+            v += zp;
+            uc[Z] = fma_t<T>(bp.nwc, fma_t<T>(bp.s, bc[Z], v), uc[Z]);
+        };
+        // first half-sweep: red cells
+        static_for<NG / 2>([&](auto ic) {
+            constexpr int Z = 2 * decltype(ic)::value + P;
+            update(std::integral_constant<int, Z>());
+            t[pc + Z * SZ] = uc[Z];
+        });
+        // the neighbours' face-layer cells
+        static_for<3>([&](auto dc) {
+            constexpr int DIM = decltype(dc)::value;
+            constexpr int DA = DIM == 0 ? 1 : 0, DB = DIM == 2 ? 1 : 2;
+            const int k = (rside << DIM) | ((rbf & 1) << DA) | ((rbf >> 1) << DB), f = 2 * DIM + rside;
+            if (re[DIM] >= 0)
+                RecFace<D, M, T, DIM, TL>::template compute<RD>(rside ? 1 : -1, rtq, re[DIM],
+                                                                sm.tab + k * HT::N + HT::ND + f * HT::NS,
+                                                                t + TL::origin(k), coord, a, sm.prm[k], rd[DIM],
+                                                                rb[DIM], rg[DIM]);
+        });
+        lds_barrier();
+
+        // second half-sweep: black cells; then the column goes out
+        static_for<NG / 2>([&](auto ic) {
+            constexpr int Z = 2 * decltype(ic)::value + 1 - P;
+            update(std::integral_constant<int, Z>());
+        });
+        T *__restrict__ ol = uo + g0, *__restrict__ oh = ol + 4 * V;
+        static_for<NG>([&](auto zc) {
+            constexpr int Z = decltype(zc)::value;
+            (Z < 8 ? ol : oh)[zoff<P, Z>()] = uc[Z];
+        });
+    }
+};
+
+template <int D, int M, typename T, int RD>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : 1))) void k_gs_sweep_col(
+    T *__restrict__ uo, const T *__restrict__ ui, const T *__restrict__ b, const T *__restrict__ ib,
+    const int *__restrict__ htab, const int *__restrict__ coord, LevelArgs<T> a) {
+    using CS = ColSweep<D, M, T, RD>;
+    using HT = Halo<D>;
+    __shared__ typename CS::Smem sm;
+    const int blk0 = a.blk0 + xcd_remap(blockIdx.x, gridDim.x) * CS::NB;
     const int tid = threadIdx.x;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
-    const int par = w & 1;
-    const int Y = 8 * (w >> 1) + (lane >> 3), X = 2 * (lane & 7) + ((Y + par) & 1);
-    const int x = X & 7, y = Y & 7, kxy = (X >> 3) | ((Y >> 3) << 1);
-
     // halo-table rows and block parameters of the group, first
     int tv = 0;
-    if (tid < NB * HT::N) tv = htab[(size_t)blk0 * HT::N + tid];
+    if (tid < CS::NB * HT::N) tv = htab[(size_t)blk0 * HT::N + tid];
     BP<T> bpl{};
-    if (tid < NB) bpl = bparams(a, blk0 + tid);
-    // interior: cell Z of the column is red (first half-sweep, storage half
-    // 0) when Z = par (mod 2)
-    const size_t g0 = (size_t)(blk0 + kxy) * V + (size_t)((x >> 1) + LY::MH * y);
-    T uc[NG], bc[NG];
-#pragma unroll
-    for (int Z = 0; Z < NG; Z++) {
-        const size_t o = g0 + (size_t)((Z >> 3) * 4 * V + LY::MH * M * (Z & 7) + (((Z + par) & 1) ? H : 0));
-        uc[Z] = ui[o];
-        bc[Z] = b[o];
-    }
-    if (tid < NB * HT::N) tab[tid] = tv;
-    if (tid < NB) prm[tid] = bpl;
-    lds_barrier();
-
-    // halo cells of the six group faces: wave w takes the block face w of
-    // each (tangential bits of the block from w), lane = cell t of the face
-    T hv[6], hm[6];  // halo value = hm * hv (hm: mirror factor of a boundary face, else 1)
-#define SRMG_COL_HALO(F)                                                                                   \
-    {                                                                                                      \
-        constexpr int DIM = (F) >> 1, DA = DIM == 0 ? 1 : 0, DB = DIM == 2 ? 1 : 2;                       \
-        const int k = (((F) & 1) << DIM) | ((w & 1) << DA) | ((w >> 1) << DB);                           \
-        hv[F] = halo_value<D, M, T, (F)>(tab[k * HT::N + (F)], blk0 + k, lane, ui, ib, T(1));             \
-        hm[F] = tab[k * HT::N + (F)] == NB_BOUNDARY ? a.sg[F] : T(1);                                     \
-    }
-    SRMG_COL_HALO(0) SRMG_COL_HALO(1) SRMG_COL_HALO(2) SRMG_COL_HALO(3) SRMG_COL_HALO(4) SRMG_COL_HALO(5)
-#undef SRMG_COL_HALO
-    // recompute operands: for dimension DIM, waves 0-1 the low group face,
-    // waves 2-3 the high one; half-wave = block face, tq = lane & 31
-    const int rside = w >> 1, rbf = ((w & 1) << 1) | ((lane >> 5) & 1), rtq = lane & 31;
-    T rd[3], rb[3], rg[3][4];
-    int re[3];
-#define SRMG_COL_RPRE(DIM)                                                                                 \
-    {                                                                                                      \
-        constexpr int DA = DIM == 0 ? 1 : 0, DB = DIM == 2 ? 1 : 2;                                       \
-        const int k = (rside << DIM) | ((rbf & 1) << DA) | ((rbf >> 1) << DB), f = 2 * DIM + rside;      \
-        const int *tb = tab + k * HT::N;                                                                  \
-        re[DIM] = tb[f];                                                                                  \
-        rd[DIM] = rb[DIM] = T(0);                                                                         \
-        if (re[DIM] >= 0)                                                                                 \
-            RecFace<D, M, T, DIM, TL>::prefetch(rside ? 1 : -1, rtq, re[DIM], tb + HT::ND + f * HT::NS, ui, b, \
-                                                ib, rd[DIM], rb[DIM], rg[DIM]);                           \
-    }
-    SRMG_COL_RPRE(0) SRMG_COL_RPRE(1) SRMG_COL_RPRE(2)
-#undef SRMG_COL_RPRE
-    // the tile
-    const int pc = TL::at(X, Y, 0);
-#pragma unroll
-    for (int Z = 0; Z < NG; Z++) tile[pc + Z * S2] = uc[Z];
-#define SRMG_COL_HPUT(F)                                                                                   \
-    {                                                                                                      \
-        constexpr int DIM = (F) >> 1, DA = DIM == 0 ? 1 : 0, DB = DIM == 2 ? 1 : 2;                       \
-        const int k = (((F) & 1) << DIM) | ((w & 1) << DA) | ((w >> 1) << DB);                           \
-        tile[TL::origin(k) + halo_at<D, M, (F), TL>(lane)] = hm[F] * hv[F];                               \
-    }
-    SRMG_COL_HPUT(0) SRMG_COL_HPUT(1) SRMG_COL_HPUT(2) SRMG_COL_HPUT(3) SRMG_COL_HPUT(4) SRMG_COL_HPUT(5)
-#undef SRMG_COL_HPUT
-    lds_barrier();
-
-    // one cell update of the column (tstencil's order of terms)
-    const BP<T> bq[2] = {prm[kxy], prm[kxy | 4]};
-    auto update = [&](int Z, T zm, T zp) {
-        const int p = pc + Z * S2;
-        const BP<T> &bp = bq[Z >> 3];
-        T v = T(0);
-        v += zm;
-        v += tile[p - S];
-        v += tile[p - 1];
-        v += bp.cc * uc[Z];
-        v += tile[p + 1];
-        v += tile[p + S];
-        v += zp;
-        return fma_t<T>(bp.nwc, fma_t<T>(bp.s, bc[Z], v), uc[Z]);
-    };
-    // first half-sweep: red cells
-#pragma unroll
-    for (int Z = 0; Z < NG; Z++) {
-        if (((Z + par) & 1) == 0) {
-            const T zm = Z == 0 ? tile[pc - S2] : uc[Z > 0 ? Z - 1 : 0];
-            const T zp = Z == NG - 1 ? tile[pc + NG * S2] : uc[Z < NG - 1 ? Z + 1 : 0];
-            uc[Z] = update(Z, zm, zp);
-            tile[pc + Z * S2] = uc[Z];
-        }
-    }
-    // the neighbours' face-layer cells
-#define SRMG_COL_RCMP(DIM)                                                                                 \
-    {                                                                                                      \
-        constexpr int DA = DIM == 0 ? 1 : 0, DB = DIM == 2 ? 1 : 2;                                       \
-        const int k = (rside << DIM) | ((rbf & 1) << DA) | ((rbf >> 1) << DB), f = 2 * DIM + rside;      \
-        if (re[DIM] >= 0)                                                                                 \
-            RecFace<D, M, T, DIM, TL>::template compute<RD>(rside ? 1 : -1, rtq, re[DIM],                 \
-                                                            tab + k * HT::N + HT::ND + f * HT::NS,        \
-                                                            tile + TL::origin(k), coord, a, prm[k], rd[DIM], \
-                                                            rb[DIM], rg[DIM]);                            \
-    }
-    SRMG_COL_RCMP(0) SRMG_COL_RCMP(1) SRMG_COL_RCMP(2)
-#undef SRMG_COL_RCMP
-    lds_barrier();
-
-    // second half-sweep: black cells; then the column goes out
-#pragma unroll
-    for (int Z = 0; Z < NG; Z++) {
-        if (((Z + par) & 1) == 1) {
-            const T zm = Z == 0 ? tile[pc - S2] : uc[Z > 0 ? Z - 1 : 0];
-            const T zp = Z == NG - 1 ? tile[pc + NG * S2] : uc[Z < NG - 1 ? Z + 1 : 0];
-            uc[Z] = update(Z, zm, zp);
-        }
-    }
-#pragma unroll
-    for (int Z = 0; Z < NG; Z++)
-        uo[g0 + (size_t)((Z >> 3) * 4 * V + LY::MH * M * (Z & 7) + (((Z + par) & 1) ? H : 0))] = uc[Z];
+    if (tid < CS::NB) bpl = bparams(a, blk0 + tid);
+    if (w & 1) CS::template run<1>(sm, uo, ui, b, ib, coord, a, blk0, w, lane, tv, bpl);
+    else CS::template run<0>(sm, uo, ui, b, ib, coord, a, blk0, w, lane, tv, bpl);
 }
 
 // ---- tiled residual kernels ----
