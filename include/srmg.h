@@ -273,16 +273,22 @@ int srmg_time_kernel(srmg_ctx *ctx, int kind, int level, int u, int b, int r, do
  * of fields of u's element type, and writes the choices to chosen[nlevels] if
  * non-NULL. u is restored. Shapes: 0..3 blocks per work group x work-group
  * size; 4, 5, 6 (3D M=8 and 2D M=16) one sibling group of 2^D blocks as one
- * tile on 256 / 512 / 1024 threads (levels without complete sibling groups,
- * and thread counts that do not divide a group's cells per colour, run 0). */
+ * tile on 256 / 512 / 1024 threads; 7 (3D M=8) one sibling group on 256
+ * threads, one thread per column of the group (levels without complete
+ * sibling groups, and thread counts that do not divide a group's cells per
+ * colour, run 0; 7 runs 4 where the column form does not apply). */
 int srmg_autotune(srmg_ctx *ctx, int u, int b, double s, const srmg_op *op, int reps, int *chosen);
-/* Force the tile shape (0..6) of the fused sweep for one element type and level (1-based). */
+/* Force the tile shape (0..7) of the fused sweep for one element type and level (1-based). */
 int srmg_set_sweep_config(srmg_ctx *ctx, int dtype, int level, int cfg);
-/* 1 when the level's blocks form complete sibling groups (tile shapes 4..6 apply). */
+/* 1 when the level's blocks form complete sibling groups (tile shapes 4..7 apply). */
 int srmg_level_grouped(srmg_ctx *ctx, int level, int *grouped);
 /* Use (default) or bypass the sibling-group forms of the sweep, residual and
  * prolongation kernels on grouped levels; both give the same bits. */
 int srmg_set_group_tiles(srmg_ctx *ctx, int enable);
+/* Kernels of the level-1 axis transforms (exact coarsest solve and
+ * srmg_grid_solve): 1 (default) one wave per work group without LDS where the
+ * axis length is at most 64, 0 the LDS-tiled kernels; both give the same bits. */
+int srmg_set_axis_gemm(srmg_ctx *ctx, int variant);
 
 /* Blocks, leaf blocks and coarse-fine ghost slabs of a level (1-based). */
 int srmg_level_info(srmg_ctx *ctx, int level, int64_t *nblocks, int64_t *nleaf, int64_t *nslots);

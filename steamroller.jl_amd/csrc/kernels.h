@@ -1742,6 +1742,63 @@ __global__ __launch_bounds__(256) void k_axis_gemm_small(const double *__restric
     }
 }
 
+// k_axis_gemm for n <= 64 with one wave per work group and no LDS; each
+// output is the k-ordered sum of products of k_axis_gemm (same bits).
+// Axis with stride 1 (a column is n contiguous values): lane i computes row i
+// of RB consecutive columns. Row k of At (= column k of A) is a coalesced load;
+// X[col][k] is the same for every lane (a scalar load).
+template <int RB>
+__global__ __launch_bounds__(64) void k_axis_gemm_c(const double *__restrict__ At, int n, int ncol,
+                                                    const double *__restrict__ in, double *__restrict__ out) {
+    const int i = threadIdx.x;
+    const int c0 = blockIdx.x * RB;
+    if (i >= n) return;
+    double acc[RB];
+#pragma unroll
+    for (int r = 0; r < RB; r++) acc[r] = 0.0;
+#pragma unroll 8
+    for (int k = 0; k < n; k++) {
+        const double a = At[(size_t)k * n + i];
+#pragma unroll
+        for (int r = 0; r < RB; r++) {
+            const int col = min(c0 + r, ncol - 1);
+            acc[r] += a * in[(size_t)col * n + k];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < RB; r++)
+        if (c0 + r < ncol) out[(size_t)(c0 + r) * n + i] = acc[r];
+}
+
+// Axis with a stride (inner) that is a multiple of 64: lane j is one of 64
+// consecutive columns (the same outer index), and the work group computes RB
+// rows of them. X[o][k][j] is a coalesced load; A[i][k] is the same for every
+// lane (a scalar load).
+template <int RB>
+__global__ __launch_bounds__(64) void k_axis_gemm_r(const double *__restrict__ A, int n, int inner,
+                                                    const double *__restrict__ in, double *__restrict__ out) {
+    const int col = blockIdx.x * 64 + threadIdx.x;
+    const int i0 = blockIdx.y * RB;
+    const size_t o = (size_t)(col / inner), jj = (size_t)(col % inner);
+    const double *__restrict__ x = in + o * (size_t)n * inner + jj;
+    double acc[RB];
+#pragma unroll
+    for (int r = 0; r < RB; r++) acc[r] = 0.0;
+#pragma unroll 8
+    for (int k = 0; k < n; k++) {
+        const double xv = x[(size_t)k * inner];
+#pragma unroll
+        for (int r = 0; r < RB; r++) {
+            const int i = min(i0 + r, n - 1);
+            acc[r] += A[(size_t)i * n + k] * xv;
+        }
+    }
+    double *__restrict__ y = out + o * (size_t)n * inner + jj;
+#pragma unroll
+    for (int r = 0; r < RB; r++)
+        if (i0 + r < n) y[(size_t)(i0 + r) * inner] = acc[r];
+}
+
 // divide by the eigenvalues of the Kronecker-sum operator; the grid holds
 // `ngrid` stacked level-1 grids of `total` points, grid g with shift[g]
 // (Helmholtz: -s k^2; Laplacian: 0)

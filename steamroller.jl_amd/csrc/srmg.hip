@@ -105,6 +105,7 @@ struct srmg_ctx {
     std::unique_ptr<FieldBuf> zero[2];  // all-zero scratch fields (F64, F32), made on demand
     srmg_unet *guess_net = nullptr;     // network of the learned first iterate (not owned)
     bool group_tiles = true;            // sibling-group kernels on grouped levels
+    int axis_gemm = 1;                  // level-1 transform kernels (launch_axis_gemm)
     int guess_fallbacks = 0;            // trees of the last solve that started from zero instead
     hipStream_t stream = nullptr;
     int maxpartials = 0;
@@ -601,10 +602,39 @@ static inline unsigned nwg(int64_t threads) { return (unsigned)((threads + WG - 
 
 static std::vector<double> read_norms(srmg_ctx *c);
 
-// one axis of the level-1 transforms; the column tile is narrowed until the
-// launch has at least 512 work groups (64-wide tiles for large batches)
+// n x n matrix of an axis transform on the device: A (row-major), then its
+// transpose (the coalesced operand of k_axis_gemm_c)
+static void upload_axis_matrix(DevBuf &buf, const std::vector<double> &A, int n) {
+    const size_t nn = (size_t)n * n;
+    std::vector<double> both(2 * nn);
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < n; k++) {
+            both[(size_t)i * n + k] = A[(size_t)i * n + k];
+            both[nn + (size_t)k * n + i] = A[(size_t)i * n + k];
+        }
+    buf.alloc(2 * nn * sizeof(double));
+    HIPCHK(hipMemcpy(buf.p, both.data(), 2 * nn * sizeof(double), hipMemcpyHostToDevice));
+}
+
+// one axis of the level-1 transforms (A from upload_axis_matrix). n <= 64:
+// one wave per work group and no LDS where the stride allows it (variant 1),
+// else the LDS-tiled kernels; the column tile of the tiled kernel is narrowed
+// until the launch has at least 512 work groups. Every variant gives the same
+// bits.
 static void launch_axis_gemm(hipStream_t st, const double *A, int n, int inner, int ncol, const double *in,
-                             double *out) {
+                             double *out, int variant) {
+    if (n <= 64 && variant == 1 && inner == 1) {
+        constexpr int RB = 2;
+        hipLaunchKernelGGL(k_axis_gemm_c<RB>, dim3((unsigned)((ncol + RB - 1) / RB)), dim3(64), 0, st,
+                           A + (size_t)n * n, n, ncol, in, out);
+        return;
+    }
+    if (n <= 64 && variant == 1 && inner % 64 == 0) {
+        constexpr int RB = 2;
+        hipLaunchKernelGGL(k_axis_gemm_r<RB>, dim3((unsigned)(ncol / 64), (unsigned)((n + RB - 1) / RB)), dim3(64),
+                           0, st, A, n, inner, in, out);
+        return;
+    }
     if (n <= 64) {
         // one load phase: A and the column tile in LDS (16 columns per work group)
         hipLaunchKernelGGL(k_axis_gemm_small<16>, dim3((unsigned)((ncol + 15) / 16)), dim3(256), 0, st, A, n, inner,
@@ -982,7 +1012,7 @@ struct Ops {
     void coarse_exact(FieldBuf &u, FieldBuf &b, double s, const srmg_op *op);
     // out = A in along an axis of extent n and stride `inner` (ncol columns)
     void axis(const double *A, int n, int inner, int ncol, const double *in, double *out) {
-        launch_axis_gemm(c->stream, A, n, inner, ncol, in, out);
+        launch_axis_gemm(c->stream, A, n, inner, ncol, in, out, c->axis_gemm);
     }
 
     void top(FieldBuf &u, FieldBuf &b, double s, const srmg_op *op, int ntop, double omega) {
@@ -1262,11 +1292,9 @@ void Ops<D, M, T>::coarse_exact(FieldBuf &u, FieldBuf &b, double s, const srmg_o
                 std::vector<double> lam(n), P((size_t)n * n), Q((size_t)n * n);
                 axis_decomp(n, d == rdim, (double)op->bc[il], (double)op->bc[ih], lam.data(), P.data(), Q.data());
                 G.lam[d].alloc(n * sizeof(double));
-                G.P[d].alloc((size_t)n * n * sizeof(double));
-                G.Q[d].alloc((size_t)n * n * sizeof(double));
                 HIPCHK(hipMemcpy(G.lam[d].p, lam.data(), n * sizeof(double), hipMemcpyHostToDevice));
-                HIPCHK(hipMemcpy(G.P[d].p, P.data(), (size_t)n * n * sizeof(double), hipMemcpyHostToDevice));
-                HIPCHK(hipMemcpy(G.Q[d].p, Q.data(), (size_t)n * n * sizeof(double), hipMemcpyHostToDevice));
+                upload_axis_matrix(G.P[d], P, n);
+                upload_axis_matrix(G.Q[d], Q, n);
             }
             G.valid = true;
             G.geom = op->geometry;
@@ -2070,6 +2098,14 @@ int srmg_set_group_tiles(srmg_ctx *c, int enable) {
     });
 }
 
+int srmg_set_axis_gemm(srmg_ctx *c, int variant) {
+    return guarded([&] {
+        if (!c) fail("null context");
+        if (variant < 0 || variant > 1) fail("axis transform variant must be 0 or 1");
+        c->axis_gemm = variant;
+    });
+}
+
 int srmg_level_grouped(srmg_ctx *c, int level, int *grouped) {
     return guarded([&] {
         check_tree(c);
@@ -2194,11 +2230,9 @@ int srmg_grid_solve(srmg_ctx *c, int D, const int *n, double s, const srmg_op *o
             std::vector<double> hl(nd), hP((size_t)nd * nd), hQ((size_t)nd * nd);
             axis_decomp(nd, d == rdim, (double)op->bc[il], (double)op->bc[ih], hl.data(), hP.data(), hQ.data());
             lam[d].alloc(nd * sizeof(double));
-            P[d].alloc((size_t)nd * nd * sizeof(double));
-            Q[d].alloc((size_t)nd * nd * sizeof(double));
             HIPCHK(hipMemcpy(lam[d].p, hl.data(), nd * sizeof(double), hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(P[d].p, hP.data(), (size_t)nd * nd * sizeof(double), hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(Q[d].p, hQ.data(), (size_t)nd * nd * sizeof(double), hipMemcpyHostToDevice));
+            upload_axis_matrix(P[d], hP, nd);
+            upload_axis_matrix(Q[d], hQ, nd);
         }
         const size_t total = (size_t)N[0] * N[1] * N[2];
         std::vector<double> hs(ngrid, op->disc == SRMG_HELMHOLTZ ? -s * op->k2 : 0.0);
@@ -2211,7 +2245,8 @@ int srmg_grid_solve(srmg_ctx *c, int D, const int *n, double s, const srmg_op *o
         const double *in = f;
         double *a = g1.as<double>(), *b2 = g2.as<double>();
         auto axis = [&](const DevBuf &A, int d, int stride, const double *x, double *y) {
-            launch_axis_gemm(c->stream, A.as<double>(), N[d], stride, (int)(ngrid * total / N[d]), x, y);
+            launch_axis_gemm(c->stream, A.as<double>(), N[d], stride, (int)(ngrid * total / N[d]), x, y,
+                             c->axis_gemm);
         };
         int stride = 1;
         for (int d = 0; d < D; d++) {

@@ -10,6 +10,11 @@ src/connectivity.jl:155-159).
 
 There is no CPU fallback: if the HIP library is missing or no GPU is present
 the calls raise.
+
+When torch shares the process (the learned first iterate's training data,
+``grid_solve`` and ``UNet`` on torch tensors), import torch before the first
+library call: torch ships its own HIP runtime, and it finds no GPU when
+another HIP runtime was initialised first.
 """
 
 from __future__ import annotations
@@ -145,6 +150,7 @@ def lib():
         "srmg_set_sweep_config": [P, i32, i32, i32],
         "srmg_level_grouped": [P, i32, C.POINTER(i32)],
         "srmg_set_group_tiles": [P, i32],
+        "srmg_set_axis_gemm": [P, i32],
         "srmg_electric_field": [P, i32, i32, dbl, P, P, P, i32],
         "srmg_set_guess": [P, P],
         "srmg_guess_fallbacks": [P, C.POINTER(i32)],
@@ -287,6 +293,11 @@ class Context:
 
     def set_group_tiles(self, enable: bool):
         _check(lib().srmg_set_group_tiles(self.h, int(bool(enable))))
+
+    def set_axis_gemm(self, variant: int):
+        """Kernels of the level-1 axis transforms: 1 (default) one wave per
+        work group, 0 the LDS-tiled ones (the same bits)."""
+        _check(lib().srmg_set_axis_gemm(self.h, int(variant)))
 
     def set_sweep_config(self, dtype: int, level: int, cfg: int):
         _check(lib().srmg_set_sweep_config(self.h, dtype, level, cfg))

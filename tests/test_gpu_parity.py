@@ -11,6 +11,10 @@ potential and 1e-12 (relative to resnorm(0)) in the residual history.
 import numpy as np
 import pytest
 
+# torch's HIP runtime must be the process's first: import it before the
+# library is loaded (a later first import of torch finds no GPU)
+torch = pytest.importorskip("torch")
+
 from oracle import Oracle
 from srmg import configs, solver
 from srmg.solver import F32, F64
@@ -194,6 +198,47 @@ def test_exact_coarse_solve_history(name):
     u.download(gu)
     ref = _valid(ou, p)
     assert np.max(np.abs(_valid(gu, p) - ref)) <= 1e-10 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("name", ["cfg1", "cyl2-m8", "cart3-m8", "cfg4-small"])
+def test_axis_transform_variants_give_identical_solves(name):
+    """The one-wave level-1 transform kernels and the LDS-tiled ones sum the
+    same products in the same order: exact-coarse solves agree bit for bit."""
+    p = configs.make_cfg4(0, scale=0.35, levels=3) if name == "cfg4-small" else CASES[name]()
+    print(f"case {name}", flush=True)
+    geom, disc = solver.geometry_from_code(p.geom), _disc(p)
+    res = []
+    for variant in (0, 1):
+        ctx, (u, b, r, u1) = _device(p)
+        ctx.set_axis_gemm(variant)
+        b.upload(p.b)
+        n, hist = solver.solve(u, b, r, u1, p.s, geom, p.bc, disc, tol=1e-10, max_cycles=6)
+        host = p.b.like()
+        u.download(host)
+        res.append((np.asarray(hist), _valid(host, p).copy()))
+    np.testing.assert_array_equal(res[1][0], res[0][0])
+    np.testing.assert_array_equal(res[1][1], res[0][1])
+
+
+@pytest.mark.parametrize("D,n", [(3, (64, 64, 64)), (3, (64, 32, 16)), (2, (64, 128))])
+def test_axis_transform_variants_give_identical_grid_solves(D, n):
+    p = CASES["cart3-m8" if D == 3 else "cyl2-m8"]()
+    bc = p.bc
+    geom = solver.CartesianGeometry()
+    disc = solver.LaplacianDiscretization(D, 2)
+    V = int(np.prod(n))
+    g = torch.Generator(device="cpu").manual_seed(7)
+    f = torch.randn((2, V), dtype=torch.float64, generator=g).cuda()
+    outs = []
+    for variant in (0, 1):
+        ctx = solver.Context(p.tree, p.conn, p.M)
+        ctx.set_axis_gemm(variant)
+        u = torch.empty_like(f)
+        torch.cuda.synchronize()
+        solver.grid_solve(ctx, D, n, 0.7, geom, bc, disc, f, u, 2)
+        torch.cuda.synchronize()
+        outs.append(u.cpu().numpy())
+    np.testing.assert_array_equal(outs[1], outs[0])
 
 
 @pytest.mark.parametrize("use_graph", [False, True])
