@@ -1440,6 +1440,148 @@ __global__ __launch_bounds__(1024) void k_resid_restrict(const T *__restrict__ u
     }
 }
 
+// k_resid_restrict for D = 3, M = 8 with the column mapping of k_gs_sweep_col:
+// 256 threads per sibling group, thread = (X, Y) column of the group with its
+// 16 cells and right-hand sides in registers (Z neighbours from registers).
+// The residual goes to an LDS copy of the group, from which (and from the u
+// tile) the parent's cells are restricted in k_restrict's order. Same
+// operations as k_resid_restrict, so the same bits. Work groups past ngroups
+// copy u to u1 on the coarse level's leaf blocks (256 cells each).
+template <int D, int M, typename T, int RD>
+struct ColResid {
+    static_assert(D == 3 && M == 8 && RD < 0, "column residual is written for 8^3 Cartesian blocks");
+    using LY = Lay<D, M>;
+    using TL = Tile<D, M, 2>;
+    static constexpr int NB = 8, V = LY::V, NG = 2 * M, ND = 2 * D;
+    static constexpr int SY = TL::stride(1), SZ = TL::stride(2);
+    struct Smem {
+        T tile[TL::SZ];
+        T rt[NG * NG * NG];  // residual of group cell (X, Y, Z) at X + NG (Y + NG Z)
+        int tab[NB * ND];
+        BP<T> prm[NB];
+    };
+
+    template <int P>
+    __device__ static __forceinline__ void run(Smem &sm, const T *__restrict__ u, const T *__restrict__ b,
+                                               const T *__restrict__ ib, const int *__restrict__ parent,
+                                               const LevelArgs<T> &a, int g0, int w, int lane, int tv,
+                                               const BP<T> &bpl, T *__restrict__ rc, T *__restrict__ uc,
+                                               T *__restrict__ u1c) {
+        using CS = ColSweep<D, M, T, RD>;
+        const int tid = w * 64 + lane;
+        const int Y = 8 * (w >> 1) + (lane >> 3), X = 2 * (lane & 7) + ((Y + P) & 1);
+        const int x = X & 7, y = Y & 7, kxy = (X >> 3) | ((Y >> 3) << 1);
+        const size_t gb = (size_t)(g0 + kxy) * V + (size_t)((x >> 1) + LY::MH * y);
+        const T *__restrict__ ul = u + gb, *__restrict__ uh = ul + 4 * V;
+        const T *__restrict__ bl = b + gb, *__restrict__ bh = bl + 4 * V;
+        T ux[NG], bx[NG];
+        static_for<NG>([&](auto zc) {
+            constexpr int Z = decltype(zc)::value;
+            ux[Z] = (Z < 8 ? ul : uh)[CS::template zoff<P, Z>()];
+            bx[Z] = (Z < 8 ? bl : bh)[CS::template zoff<P, Z>()];
+        });
+        if (tid < NB * ND) sm.tab[tid] = tv;
+        if (tid < NB) sm.prm[tid] = bpl;
+        lds_barrier();
+        // halo of the six group faces (wave w: block face w of each; lane: cell)
+        T hv[6], hm[6];
+        static_for<6>([&](auto fc) {
+            constexpr int F = decltype(fc)::value;
+            constexpr int DIM = F >> 1, DA = DIM == 0 ? 1 : 0, DB = DIM == 2 ? 1 : 2;
+            const int k = ((F & 1) << DIM) | ((w & 1) << DA) | ((w >> 1) << DB);
+            const int e = __builtin_amdgcn_readfirstlane(sm.tab[k * ND + F]);
+            hv[F] = halo_value<D, M, T, F>(e, g0 + k, lane, u, ib, T(1));
+            hm[F] = e == NB_BOUNDARY ? a.sg[F] : T(1);
+        });
+        const int pc = TL::at(X, Y, 0);
+        static_for<NG>([&](auto zc) {
+            constexpr int Z = decltype(zc)::value;
+            sm.tile[pc + Z * SZ] = ux[Z];
+        });
+        static_for<6>([&](auto fc) {
+            constexpr int F = decltype(fc)::value;
+            constexpr int DIM = F >> 1, DA = DIM == 0 ? 1 : 0, DB = DIM == 2 ? 1 : 2;
+            const int k = ((F & 1) << DIM) | ((w & 1) << DA) | ((w >> 1) << DB);
+            sm.tile[TL::origin(k) + halo_at<D, M, F, TL>(lane)] = hm[F] * hv[F];
+        });
+        lds_barrier();
+        // residual r = -(b + L u / s) (tstencil's order of terms)
+        const BP<T> bq[2] = {sm.prm[kxy], sm.prm[kxy | 4]};
+        const T *const t = sm.tile;
+        static_for<NG>([&](auto zc) {
+            constexpr int Z = decltype(zc)::value;
+            const int p = pc + Z * SZ;
+            const BP<T> &bp = bq[Z >> 3];
+            const T zm = (Z == 0) ? t[pc - SZ] : ux[Z > 0 ? Z - 1 : 0];
+            const T zp = (Z == NG - 1) ? t[pc + NG * SZ] : ux[Z < NG - 1 ? Z + 1 : 0];
+            T v = T(0);
+            v += zm;
+            v += t[p - SY];
+            v += t[p - 1];
+            v += bp.cc * ux[Z];
+            v += t[p + 1];
+            v += t[p + SY];
+            v += zp;
+            sm.rt[X + NG * (Y + NG * Z)] = -(bx[Z] + v / bp.s);
+        });
+        lds_barrier();
+        // the parent's cells, in storage order: coarse cell (X0, X1, X2) sums
+        // group cells (2 X0 + i0, 2 X1 + i1, 2 X2 + i2), q = i0 + 2 i1 + 4 i2
+        const int pb = parent[g0];
+#pragma unroll
+        for (int h = 0; h < V / 256; h++) {
+            const int i = tid + 256 * h;
+            int X0, X1, X2;
+            LY::decode(i, X0, X1, X2);
+            T fr = T(0), fu = T(0);
+#pragma unroll
+            for (int q = 0; q < LY::NC; q++) {
+                const int cx = 2 * X0 + (q & 1), cy = 2 * X1 + ((q >> 1) & 1), cz = 2 * X2 + ((q >> 2) & 1);
+                fr += sm.rt[cx + NG * (cy + NG * cz)];
+                fu += sm.tile[TL::at(cx, cy, cz)];
+            }
+            const size_t o = (size_t)pb * V + i;
+            rc[o] = fr / T(LY::NC);
+            const T uv = fu / T(LY::NC);
+            uc[o] = uv;
+            if (u1c) u1c[o] = uv;
+        }
+    }
+};
+
+template <int D, int M, typename T, int RD>
+__global__ __launch_bounds__(256) void k_resid_restrict_col(const T *__restrict__ u, const T *__restrict__ b,
+                                                            const T *__restrict__ ib, const int *__restrict__ htab,
+                                                            const int *__restrict__ parent, LevelArgs<T> a,
+                                                            unsigned ngroups, T *__restrict__ rc, T *__restrict__ uc,
+                                                            T *__restrict__ u1c, const T *__restrict__ ucin,
+                                                            const int *__restrict__ leaves, int nleafc) {
+    using CR = ColResid<D, M, T, RD>;
+    using HT = Halo<D>;
+    constexpr int V = CR::V, ND = CR::ND;
+    const int tid = threadIdx.x;
+    if (blockIdx.x >= ngroups) {
+        const int64_t i = (int64_t)(blockIdx.x - ngroups) * 256 + tid;
+        if (i < (int64_t)nleafc * V) {
+            const size_t o = (size_t)leaves[i / V] * V + (size_t)(i % V);
+            u1c[o] = ucin[o];
+        }
+        return;
+    }
+    __shared__ typename CR::Smem sm;
+    const int g0 = a.blk0 + xcd_remap(blockIdx.x, ngroups) * CR::NB;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    int tv = 0;
+    if (tid < CR::NB * ND) {
+        const int lb = tid / ND;
+        tv = htab[(size_t)(g0 + lb) * HT::N + (tid - lb * ND)];
+    }
+    BP<T> bpl{};
+    if (tid < CR::NB) bpl = bparams(a, g0 + tid);
+    if (w & 1) CR::template run<1>(sm, u, b, ib, parent, a, g0, w, lane, tv, bpl, rc, uc, u1c);
+    else CR::template run<0>(sm, u, b, ib, parent, a, g0, w, lane, tv, bpl, rc, uc, u1c);
+}
+
 // k_postrestrict on a level of complete sibling groups: the parent of group g
 // is fully covered, so one work group stages the parent's u (level l-1, with
 // its depth-1 halo) in LDS and forms b = -(r + L u / s) on all its cells.

@@ -808,13 +808,23 @@ struct Ops {
             const LevelInfo &Lc = c->lv[l - 1];
             const unsigned ng = (unsigned)(a.nblk >> D);
             const int nleafc = u1 ? Lc.nleaf : 0;
-            const unsigned ncp = (unsigned)(((int64_t)nleafc * LY::V + 1023) / 1024);
             with_rd(a.rdim, [&](auto rd) {
                 constexpr int RD = decltype(rd)::value;
-                hipLaunchKernelGGL((k_resid_restrict<D, M, T, RNT, RD>), dim3(ng + ncp), dim3(1024), 0, c->stream,
-                                   X(u, l), X(b, l), I(u), c->d_halo.as<int>(), coord(), c->d_parent.as<int>(), a,
-                                   ng, X(r, l - 1), X(u, l - 1), u1 ? X(*u1, l - 1) : (T *)nullptr, X(u, l - 1),
-                                   c->d_leaves.as<int>() + Lc.leaf0, nleafc);
+                // FP32 3D 8^3 Cartesian: the column form (k_resid_restrict_col)
+                if constexpr (D == 3 && M == 8 && RD < 0 && std::is_same<T, float>::value) {
+                    const unsigned ncp = (unsigned)(((int64_t)nleafc * LY::V + 255) / 256);
+                    hipLaunchKernelGGL((k_resid_restrict_col<D, M, T, RD>), dim3(ng + ncp), dim3(256), 0, c->stream,
+                                       X(u, l), X(b, l), I(u), c->d_halo.as<int>(), c->d_parent.as<int>(), a, ng,
+                                       X(r, l - 1), X(u, l - 1), u1 ? X(*u1, l - 1) : (T *)nullptr, X(u, l - 1),
+                                       c->d_leaves.as<int>() + Lc.leaf0, nleafc);
+                } else {
+                    const unsigned ncp = (unsigned)(((int64_t)nleafc * LY::V + 1023) / 1024);
+                    hipLaunchKernelGGL((k_resid_restrict<D, M, T, RNT, RD>), dim3(ng + ncp), dim3(1024), 0,
+                                       c->stream, X(u, l), X(b, l), I(u), c->d_halo.as<int>(), coord(),
+                                       c->d_parent.as<int>(), a, ng, X(r, l - 1), X(u, l - 1),
+                                       u1 ? X(*u1, l - 1) : (T *)nullptr, X(u, l - 1),
+                                       c->d_leaves.as<int>() + Lc.leaf0, nleafc);
+                }
             });
             return true;
         }
