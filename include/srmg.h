@@ -211,6 +211,33 @@ int srmg_solve_mixed_batched(srmg_ctx *ctx, int u, int b, int e, int be, int re,
 int srmg_electric_field(srmg_ctx *ctx, int u, int eabs, double h, const double *e0, const int8_t *bc,
                         double *e_host, int G);
 
+/* ---- free boundary conditions (src/freebc.jl; 2D cylindrical, caller
+ *      src/streamer.jl:320 and 340-349) ---- */
+
+typedef struct srmg_freebc srmg_freebc;
+
+/* FreeBC constructor (src/freebc.jl:32-54): dim is the radial dimension of
+ * CylindricalGeometry{dim} (1 or 2), level the level at which the boundary is
+ * resolved (refine_maxlevel), rootsize[2] the level-1 block domain, h the
+ * level-1 cell size and f[n] the boundary kernel
+ * f_k = (2/H) / (k (I1/I0 + K1/K0)(k R)), k = pi m / H, m = 1..n, with
+ * n = rootsize[other] 2^(level-1) M (computed by the caller, as the reference
+ * computes it with SpecialFunctions). */
+int srmg_freebc_create(srmg_ctx *ctx, srmg_freebc **fb, int dim, int level, const int64_t *rootsize, double h,
+                       const double *f, int64_t n);
+int srmg_freebc_destroy(srmg_freebc *fb);
+/* freebc!(q, u, h, s, tree, conn, fb) (src/freebc.jl:87-110): the outer
+ * values of u over h_l (getouter!), the two sine transforms with the kernel f,
+ * the boundary values b, and setouter! on the FP64 field q (outermost cells and
+ * the corner cells at coarse-fine faces). u is the homogeneous solution. */
+int srmg_freebc_apply(srmg_ctx *ctx, srmg_freebc *fb, int q, int u, double s);
+/* The boundary values b[n] of the last srmg_freebc_apply. */
+int srmg_freebc_boundary(srmg_freebc *fb, double *b, int64_t n);
+/* srmg_electric_field with the ghost fill of fill_ghost_free! (src/freebc.jl:218-238):
+ * the ghosts beyond the outer radial face gain 2 mean(b) before E is formed. */
+int srmg_freebc_electric_field(srmg_ctx *ctx, srmg_freebc *fb, int u, int eabs, double h, const double *e0,
+                               const int8_t *bc, double *e_host, int G);
+
 /* ---- learned first iterate ---- */
 
 /* Attach (net != NULL) or detach the network used by SRMG_GUESS_LEARNED. The

@@ -62,7 +62,23 @@ def lib():
         _LIB.ora_coarse_solve.restype = i32
         _LIB.ora_electric_field.argtypes = [P, P, P, P, dbl, P]
         _LIB.ora_electric_field.restype = None
+        for name, args in {
+            "ora_freebc_getouter": [P, P, P, i32, i32, dbl, P],
+            "ora_dst_matrix": [P, i64, i32],
+            "ora_freebc_setouter": [P, P, P, i32, i32, dbl, P],
+            "ora_freebc": [P, P, P, dbl, dbl, i32, i32, P, P, i64, P],
+            "ora_fill_ghost_free": [P, P, P, i32, i32, i32, P],
+        }.items():
+            getattr(_LIB, name).argtypes = args
+            getattr(_LIB, name).restype = None
     return _LIB
+
+
+def dst_matrix(n: int, kind: int) -> np.ndarray:
+    """The n x n matrix of FFTW's RODFT10 (kind 2) or RODFT01 (kind 3)."""
+    S = np.zeros((n, n))
+    lib().ora_dst_matrix(_ptr(S), int(n), int(kind))
+    return S
 
 
 class _Op(C.Structure):
@@ -185,6 +201,32 @@ class Oracle:
         e = np.ascontiguousarray(np.asarray(e0, np.float64))
         lib().ora_electric_field(self.h, _ptr(f), _ptr(fabs), _ptr(a), float(h), _ptr(e))
         return f, fabs
+
+    # free boundary conditions (src/freebc.jl) -----------------------------
+    def freebc_getouter(self, u, dim, level, h, rootsize, n):
+        u1 = np.zeros(n)
+        root = np.ascontiguousarray(np.asarray(rootsize, np.int64))
+        lib().ora_freebc_getouter(self.h, _ptr(u1), _ptr(self._a(u)), int(dim), int(level), float(h), _ptr(root))
+        return u1
+
+    def freebc_setouter(self, q, b, dim, level, s, rootsize):
+        root = np.ascontiguousarray(np.asarray(rootsize, np.int64))
+        bb = np.ascontiguousarray(np.asarray(b, np.float64))
+        lib().ora_freebc_setouter(self.h, _ptr(self._a(q)), _ptr(bb), int(dim), int(level), float(s), _ptr(root))
+
+    def freebc(self, q, u, h, s, dim, level, rootsize, f):
+        """freebc!(q, u, h, s, tree, conn, conf); returns the boundary values b."""
+        root = np.ascontiguousarray(np.asarray(rootsize, np.int64))
+        ff = np.ascontiguousarray(np.asarray(f, np.float64))
+        b = np.zeros(len(ff))
+        lib().ora_freebc(self.h, _ptr(self._a(q)), _ptr(self._a(u)), float(h), float(s), int(dim), int(level),
+                         _ptr(root), _ptr(ff), len(ff), _ptr(b))
+        return b
+
+    def fill_ghost_free(self, u, b, dim, level, l, rootsize):
+        root = np.ascontiguousarray(np.asarray(rootsize, np.int64))
+        bb = np.ascontiguousarray(np.asarray(b, np.float64))
+        lib().ora_fill_ghost_free(self.h, _ptr(self._a(u)), _ptr(bb), int(dim), int(level), int(l), _ptr(root))
 
     def coarse_solve(self, u, b, s) -> int:
         return lib().ora_coarse_solve(self.h, _ptr(self._a(u)), _ptr(self._a(b)), s, self._opp())

@@ -843,3 +843,161 @@ void ora_electric_field(const ora_tree *t, double *f, double *fabs, const double
         }
     }
 }
+
+/* ---- free boundary conditions (src/freebc.jl; cylindrical 2D) ----
+ * dim is the geometry's radial dimension (1-based, CylindricalGeometry{dim});
+ * rootsize[D] the level-1 block domain; n = rootsize[other] 2^(level-1) M. */
+
+/* block of level l at coordinate (kdim, iother), 0 if absent */
+static int64_t block_at(const ora_tree *t, int l, int dim, int64_t kdim, int64_t iother) {
+    const int D = t->D, od = (dim % D) + 1;
+    for (int64_t k = t->pairs_off[l - 1]; k < t->pairs_off[l]; k++) {
+        const int64_t *p = t->pairs + k * (D + 1);
+        if (p[dim - 1] == kdim && p[od - 1] == iother) return p[D];
+    }
+    return 0;
+}
+
+/* getouter!(u1, u, tree, dim, level, h) (src/freebc.jl:112-142) */
+void ora_freebc_getouter(const ora_tree *t, double *u1, const double *u, int dim, int level, double h,
+                         const int64_t *rootsize) {
+    const int D = t->D, M = t->M, G = t->G, od = (dim % D) + 1;
+    for (int l = 1; l <= level && l <= t->nlevels; l++) {
+        const int64_t kd = rootsize[dim - 1] << (l - 1), no = rootsize[od - 1] << (l - 1);
+        const double hl = h / (double)(1 << (l - 1));
+        const int64_t p = (int64_t)1 << (level - l);
+        for (int64_t iblk = 1; iblk <= no; iblk++) {
+            const int64_t blk = block_at(t, l, dim, kd, iblk);
+            if (!blk) continue;
+            const int64_t i0 = (iblk - 1) * M;
+            for (int i = 1; i <= M; i++) {
+                int I[3] = {1, 1, 1};
+                I[dim - 1] = G + M;
+                I[od - 1] = G + i;
+                const double v = u[OFF(t, I, blk)] / hl;
+                for (int64_t j = p * (i0 + i - 1); j < p * (i0 + i); j++) u1[j] = v;
+            }
+        }
+    }
+}
+
+/* FFTW's RODFT10 (DST-II), y_k = 2 sum_j x_j sin(pi (j + 1/2)(k + 1)/n), and
+ * RODFT01 (DST-III), y_k = (-1)^k x_{n-1} + 2 sum_{j<n-1} x_j sin(pi (j + 1)(k + 1/2)/n),
+ * as direct sums in j order (the FFTW plans of src/freebc.jl:45-46) */
+void ora_dst_matrix(double *S, int64_t n, int type) {
+    for (int64_t k = 0; k < n; k++)
+        for (int64_t j = 0; j < n; j++) {
+            double v;
+            if (type == 2) v = 2.0 * sin(M_PI * ((double)j + 0.5) * (double)(k + 1) / (double)n);
+            else v = (j == n - 1) ? ((k & 1) ? -1.0 : 1.0)
+                                   : 2.0 * sin(M_PI * (double)(j + 1) * ((double)k + 0.5) / (double)n);
+            S[k * n + j] = v;
+        }
+}
+
+static void matvec(const double *S, const double *x, double *y, int64_t n) {
+    for (int64_t k = 0; k < n; k++) {
+        double acc = 0.0;
+        for (int64_t j = 0; j < n; j++) acc += S[k * n + j] * x[j];
+        y[k] = acc;
+    }
+}
+
+/* sum of b[a .. a+len) (0-based) in order */
+static double bsum(const double *b, int64_t a, int64_t len) {
+    double s = 0.0;
+    for (int64_t j = a; j < a + len; j++) s += b[j];
+    return s;
+}
+
+/* factor(::CylindricalGeometry{dim}, c, delta) with delta = e_dim (src/geometry.jl:27-29) */
+static double cyl_factor(int64_t cdim) { return fabs(1.0 + 1.0 / (double)(2 * cdim - 1)); }
+
+/* setouter!(q, b, tree, conn, dim, level, s, geom) (src/freebc.jl:149-215) */
+void ora_freebc_setouter(const ora_tree *t, double *q, const double *b, int dim, int level, double s,
+                         const int64_t *rootsize) {
+    const int D = t->D, M = t->M, G = t->G, od = (dim % D) + 1;
+    const int W = 2 + 2 * D;
+    for (int l = 1; l <= level && l <= t->nlevels; l++) {
+        const double sl = s / (double)(((int64_t)1 << (l - 1)) * ((int64_t)1 << (l - 1)));
+        const int64_t p = (int64_t)1 << (level - l);
+        const int64_t kd = rootsize[dim - 1] << (l - 1);
+        for (int64_t k = t->bnd_off[l - 1]; k < t->bnd_off[l]; k++) {
+            const int64_t *e = t->bnd + k * W;
+            const int64_t *face = e + 1;
+            int is0 = 1;
+            for (int d = 0; d < D; d++) is0 &= (face[d] == (d == dim - 1 ? 1 : 0));
+            if (!is0) continue;
+            const int64_t iblk = e[0];
+            const int64_t *B = t->coord_of + (iblk - 1) * D;
+            const int64_t i0 = (B[od - 1] - 1) * M;
+            for (int i = 1; i <= M; i++) {
+                int I[3] = {1, 1, 1};
+                I[dim - 1] = G + M;
+                I[od - 1] = G + i;
+                const int64_t Jd = M * (B[dim - 1] - 1) + M;
+                const double bmean = bsum(b, p * (i0 + i - 1), p) / (double)p;
+                q[OFF(t, I, iblk)] += 2.0 * cyl_factor(Jd) * bmean / sl;
+            }
+        }
+        for (int64_t k = t->ref_off[l - 1]; k < t->ref_off[l]; k++) {
+            const int64_t *e = t->ref + k * W;
+            const int64_t iblk = e[0];
+            const int64_t *face = e + 2;
+            const int64_t *B = t->coord_of + (iblk - 1) * D;
+            if (B[dim - 1] != kd) continue;
+            const int64_t dir = face[od - 1];
+            const int i = (dir == -1) ? G + 1 : M + G;
+            int I[3] = {1, 1, 1};
+            I[dim - 1] = G + M;
+            I[od - 1] = i;
+            const int64_t j = (B[od - 1] - 1) * M + i - G;
+            const int64_t Jd = M * (B[dim - 1] - 1) + M;
+            const int64_t J1d = (Jd - 1) / 2 + 1;
+            double bmean;
+            if (dir == 1) bmean = bsum(b, p * j, 2 * p) / (double)(2 * p);
+            else bmean = bsum(b, p * (j - 3) - 1, 2 * p) / (double)(2 * p);
+            q[OFF(t, I, iblk)] += 2.0 * 0.25 * cyl_factor(J1d) * bmean / sl;
+        }
+    }
+}
+
+/* freebc!(q, u, h, s, tree, conn, conf) (src/freebc.jl:87-110): f[n] is the
+ * kernel of the FreeBC constructor (src/freebc.jl:32-44); b_out[n] receives b */
+void ora_freebc(const ora_tree *t, double *q, const double *u, double h, double s, int dim, int level,
+                const int64_t *rootsize, const double *f, int64_t n, double *b_out) {
+    double *S = (double *)malloc((size_t)(n * n) * sizeof(double));
+    double *u1 = (double *)calloc((size_t)n, sizeof(double));
+    double *am = (double *)malloc((size_t)n * sizeof(double));
+    ora_freebc_getouter(t, u1, u, dim, level, h, rootsize);
+    ora_dst_matrix(S, n, 2);
+    matvec(S, u1, am, n);
+    /* am[end] *= 3 acts on the previous contents of am, which the next
+     * line overwrites (src/freebc.jl:102-103) */
+    for (int64_t k = 0; k < n; k++) am[k] = f[k] * am[k];
+    ora_dst_matrix(S, n, 3);
+    matvec(S, am, u1, n);
+    for (int64_t k = 0; k < n; k++) b_out[k] = u1[k] * h / (double)(1 << (level - 1)) / 2.0;
+    ora_freebc_setouter(t, q, b_out, dim, level, s, rootsize);
+    free(S); free(u1); free(am);
+}
+
+/* fill_ghost_free!(u, conf, tree, l) (src/freebc.jl:218-238) */
+void ora_fill_ghost_free(const ora_tree *t, double *u, const double *b, int dim, int level, int l,
+                         const int64_t *rootsize) {
+    const int D = t->D, M = t->M, G = t->G, od = (dim % D) + 1;
+    const int64_t kd = rootsize[dim - 1] << (l - 1), no = rootsize[od - 1] << (l - 1);
+    const int64_t p = (int64_t)1 << (level - l);
+    for (int64_t iblk = 1; iblk <= no; iblk++) {
+        const int64_t blk = block_at(t, l, dim, kd, iblk);
+        if (!blk) continue;
+        const int64_t i0 = (iblk - 1) * M;
+        for (int i = 1; i <= M; i++) {
+            int I[3] = {1, 1, 1};
+            I[dim - 1] = G + M + 1;
+            I[od - 1] = G + i;
+            const double bmean = bsum(b, p * (i0 + i - 1), p) / (double)p;
+            u[OFF(t, I, blk)] += 2.0 * bmean;
+        }
+    }
+}

@@ -1622,6 +1622,12 @@ struct EFieldArgs {
     T hl;        // cell size of the level: h / 2^(l-1)
     T e0[3];     // constant external field
     T sg[6];     // boundary ghost signs of u
+    // free boundary (fill_ghost_free!, src/freebc.jl:218-238), 2D only: the
+    // ghosts on the +fdim face of the outermost blocks (coordinate fkd along
+    // fdim) gain 2 mean(fbb over fp entries); fbb == nullptr: none
+    const T *fbb;
+    const int *coord;
+    int fdim, fkd, fp;
 };
 
 // electric_field! (src/multigrid.jl:421-449) with a constant external field
@@ -1643,6 +1649,22 @@ __global__ __launch_bounds__(256) void k_efield(T *__restrict__ eabs, T *__restr
     __shared__ int blks[1];
     const int blk = ea.blk0 + blockIdx.x;
     load_tile<D, M, T, 1, 256>(tile, tab, blks, u, ib, htab, nullptr, blk, 1, ea.sg);
+    if constexpr (D == 2) {
+        if (ea.fbb != nullptr && ea.coord[blk * D + ea.fdim] == ea.fkd) {
+            const int od = 1 - ea.fdim;
+            const int i0 = (ea.coord[blk * D + od] - 1) * M;
+            for (int i = threadIdx.x; i < M; i += 256) {
+                T sum = T(0);
+                for (int j = ea.fp * (i0 + i); j < ea.fp * (i0 + i + 1); j++) sum += ea.fbb[j];
+                const T bmean = sum / T(ea.fp);
+                int c[3] = {0, 0, 0};
+                c[ea.fdim] = M;
+                c[od] = i;
+                tile[TL::at(c[0], c[1], 0)] += T(2) * bmean;
+            }
+            __syncthreads();
+        }
+    }
 #pragma unroll
     for (int d = 0; d < D; d++) {
         const int da = d == 0 ? 1 : 0, db = d == 2 ? 1 : 2;
@@ -1670,6 +1692,49 @@ __global__ __launch_bounds__(256) void k_efield(T *__restrict__ eabs, T *__restr
         }
         eabs[(size_t)blk * V + q] = sqrt(f2);
     }
+}
+
+// ---- free boundary conditions (src/freebc.jl, 2D cylindrical) ----
+
+// getouter! (src/freebc.jl:112-142) as a gather: u1[j] = u[src[j]] / hl[j],
+// where src names the outermost cell of the finest level covering entry j
+__global__ __launch_bounds__(WG) void k_fb_gather(double *__restrict__ u1, const double *__restrict__ u,
+                                                  const int64_t *__restrict__ src, const double *__restrict__ hl,
+                                                  int64_t n) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) u1[j] = u[src[j]] / hl[j];
+}
+
+// y = S x for an n x n sine matrix given transposed (ST[j n + k] = S[k][j]),
+// summed in j order; then y_k = fm[k] * y_k (fm != nullptr: am = f .* u1,
+// src/freebc.jl:103) or y_k = y_k h / 2^(level-1) / 2 (src/freebc.jl:106)
+__global__ __launch_bounds__(WG) void k_fb_dst(const double *__restrict__ ST, const double *__restrict__ x,
+                                               double *__restrict__ y, int64_t n, const double *__restrict__ fm,
+                                               double h, double lev) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    double acc = 0.0;
+    for (int64_t j = 0; j < n; j++) acc += ST[j * n + k] * x[j];
+    y[k] = fm ? fm[k] * acc : acc * h / lev / 2.0;
+}
+
+// setouter! (src/freebc.jl:149-215): cell cells[i] receives, in order, the
+// contributions r in [rs[i], rs[i+1]): q += coef[r] (sum of b over
+// [bs[r], bs[r] + bl[r]) / bd[r]) / sl[r]
+__global__ __launch_bounds__(WG) void k_fb_setouter(double *__restrict__ q, const int64_t *__restrict__ cells,
+                                                    const int *__restrict__ rs, const double *__restrict__ coef,
+                                                    const int64_t *__restrict__ bs, const int *__restrict__ bl,
+                                                    const double *__restrict__ bd, const double *__restrict__ sl,
+                                                    const double *__restrict__ b, int ncells) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ncells) return;
+    double v = q[cells[i]];
+    for (int r = rs[i]; r < rs[i + 1]; r++) {
+        double sum = 0.0;
+        for (int64_t j = bs[r]; j < bs[r] + bl[r]; j++) sum += b[j];
+        v += coef[r] * (sum / bd[r]) / sl[r];
+    }
+    q[cells[i]] = v;
 }
 
 // resnorm partials of u = 0 (the residual is -b): the same blocks, visiting

@@ -1359,6 +1359,34 @@ static void check_tree(srmg_ctx *c) {
     if (c->nb == 0) fail("srmg_set_tree has not been called");
 }
 
+// Free boundary of a 2D cylindrical Poisson problem (FreeBC,
+// src/freebc.jl:7-54): the kernel f of the constructor (computed by the
+// caller), the two sine transforms as transposed n x n matrices, and work
+// vectors; b holds the boundary values of the last srmg_freebc_apply.
+struct srmg_freebc {
+    int dim = 0, od = 1, level = 1;   // radial dimension (0-based), the other one
+    int64_t n = 0;
+    int64_t root[2] = {1, 1};
+    double h = 0.0;
+    DevBuf S2T, S3T, f, u1, am, b;
+    bool have_b = false;
+};
+
+// the sine matrices of FFTW's RODFT10 / RODFT01 (src/freebc.jl:45-46),
+// transposed: ST[j n + k] = S[k][j]
+static std::vector<double> dst_matrix_t(int64_t n, int type) {
+    std::vector<double> ST((size_t)(n * n));
+    for (int64_t k = 0; k < n; k++)
+        for (int64_t j = 0; j < n; j++) {
+            double v;
+            if (type == 2) v = 2.0 * std::sin(M_PI * ((double)j + 0.5) * (double)(k + 1) / (double)n);
+            else v = (j == n - 1) ? ((k & 1) ? -1.0 : 1.0)
+                                   : 2.0 * std::sin(M_PI * (double)(j + 1) * ((double)k + 0.5) / (double)n);
+            ST[(size_t)(j * n + k)] = v;
+        }
+    return ST;
+}
+
 static int same_type(srmg_ctx *c, std::initializer_list<int> ids) {
     int t = -1;
     for (int id : ids) {
@@ -2135,9 +2163,32 @@ int srmg_level_info(srmg_ctx *c, int level, int64_t *nblocks, int64_t *nleaf, in
     });
 }
 
+}  // extern "C"
+
+static void electric_field_impl(srmg_ctx *c, int ui, int eabsi, double h, const double *e0, const int8_t *bc,
+                                double *e_host, int G, const srmg_freebc *fb);
+
+extern "C" {
+
 int srmg_electric_field(srmg_ctx *c, int ui, int eabsi, double h, const double *e0, const int8_t *bc,
                         double *e_host, int G) {
+    return guarded([&] { electric_field_impl(c, ui, eabsi, h, e0, bc, e_host, G, nullptr); });
+}
+
+int srmg_freebc_electric_field(srmg_ctx *c, srmg_freebc *fb, int ui, int eabsi, double h, const double *e0,
+                               const int8_t *bc, double *e_host, int G) {
     return guarded([&] {
+        if (!fb) fail("null free boundary");
+        if (!fb->have_b) fail("srmg_freebc_apply has not been called");
+        electric_field_impl(c, ui, eabsi, h, e0, bc, e_host, G, fb);
+    });
+}
+
+}  // extern "C"
+
+static void electric_field_impl(srmg_ctx *c, int ui, int eabsi, double h, const double *e0, const int8_t *bc,
+                                double *e_host, int G, const srmg_freebc *fb) {
+    {
         check_tree(c);
         if (!e0 || !bc) fail("null argument");
         if (h <= 0.0) fail("h must be positive");
@@ -2166,6 +2217,17 @@ int srmg_electric_field(srmg_ctx *c, int ui, int eabsi, double h, const double *
                 args.hl = h / (double)(1 << l);
                 for (int d = 0; d < 3; d++) args.e0[d] = d < D ? e0[d] : 0.0;
                 for (int q = 0; q < 6; q++) args.sg[q] = a.sg[q];
+                args.fbb = nullptr;
+                args.coord = c->d_coord.as<int>();
+                args.fdim = 0;
+                args.fkd = 0;
+                args.fp = 1;
+                if (fb && l + 1 <= fb->level) {
+                    args.fbb = fb->b.as<double>();
+                    args.fdim = fb->dim;
+                    args.fkd = (int)(fb->root[fb->dim] << l);
+                    args.fp = 1 << (fb->level - 1 - l);
+                }
                 OT::launch_efield(c, ea.data.as<double>(), ef.as<double>(), u.data.as<double>(), u.ib.as<double>(),
                                   args, c->lv[l].nblk);
             }
@@ -2200,8 +2262,10 @@ int srmg_electric_field(srmg_ctx *c, int ui, int eabsi, double h, const double *
         } else {
             HIPCHK(hipStreamSynchronize(c->stream));
         }
-    });
+    }
 }
+
+extern "C" {
 
 int srmg_set_guess(srmg_ctx *c, srmg_unet *net) {
     return guarded([&] {
@@ -2286,6 +2350,177 @@ int srmg_axis_decomp(int n, int radial, double alow, double ahigh, double *lam, 
     return guarded([&] {
         if (n < 1 || !lam || !P || !Q) fail("bad arguments");
         axis_decomp(n, radial != 0, alow, ahigh, lam, P, Q);
+    });
+}
+
+
+int srmg_freebc_create(srmg_ctx *c, srmg_freebc **out, int dim, int level, const int64_t *rootsize, double h,
+                       const double *f, int64_t n) {
+    return guarded([&] {
+        if (!c || !out || !rootsize || !f) fail("null argument");
+        *out = nullptr;
+        if (c->D != 2) fail("free boundary conditions are two-dimensional (src/freebc.jl:90)");
+        if (dim != 1 && dim != 2) fail("dim must be 1 or 2 (the radial dimension of CylindricalGeometry{dim})");
+        if (level < 1 || level > 30) fail("bad level");
+        if (h <= 0.0) fail("h must be positive");
+        auto fb = std::make_unique<srmg_freebc>();
+        fb->dim = dim - 1;
+        fb->od = 1 - fb->dim;
+        fb->level = level;
+        fb->root[0] = rootsize[0];
+        fb->root[1] = rootsize[1];
+        if (fb->root[0] < 1 || fb->root[1] < 1) fail("bad rootsize");
+        fb->h = h;
+        fb->n = (fb->root[fb->od] << (level - 1)) * c->M;
+        if (n != fb->n) fail("f must hold rootsize[other] 2^(level-1) M values");
+        if (n > 16384) fail("free boundary longer than 16384 cells");
+        auto up = [&](DevBuf &d, const std::vector<double> &v) {
+            d.alloc(v.size() * sizeof(double));
+            HIPCHK(hipMemcpy(d.p, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+        };
+        up(fb->S2T, dst_matrix_t(n, 2));
+        up(fb->S3T, dst_matrix_t(n, 3));
+        up(fb->f, std::vector<double>(f, f + n));
+        fb->u1.alloc((size_t)n * sizeof(double));
+        fb->am.alloc((size_t)n * sizeof(double));
+        fb->b.alloc((size_t)n * sizeof(double));
+        *out = fb.release();
+    });
+}
+
+int srmg_freebc_destroy(srmg_freebc *fb) {
+    return guarded([&] { delete fb; });
+}
+
+int srmg_freebc_apply(srmg_ctx *c, srmg_freebc *fb, int qi, int ui, double s) {
+    return guarded([&] {
+        if (!c || !fb) fail("null argument");
+        check_tree(c);
+        if (c->D != 2) fail("free boundary conditions are two-dimensional");
+        if (c->trees.size() != 1) fail("free boundary conditions need a context holding one tree");
+        FieldBuf &q = field(c, qi), &u = field(c, ui);
+        if (q.dtype != SRMG_F64 || u.dtype != SRMG_F64) fail("q and u must be FP64");
+        const int M = c->M, D = 2, dim = fb->dim, od = fb->od, ND = 2 * D;
+        const int64_t n = fb->n;
+        dispatch(c, SRMG_F64, [&](auto &o) {
+            o.settle(u);
+            o.settle(q);
+        });
+        auto cell = [&](int blk, int cd, int co) {  // storage index of local (dim: cd, other: co)
+            int x[2];
+            x[dim] = cd;
+            x[od] = co;
+            return (int64_t)blk * c->V + (M == 16 ? Lay<2, 16>::sidx(x[0], x[1], 0) : Lay<2, 8>::sidx(x[0], x[1], 0));
+        };
+        if (M != 16 && M != 8) fail("unsupported M");
+        const int lmax = std::min(fb->level, c->nlevels);
+        // getouter!: the finest level's outermost cell for every entry
+        std::vector<int64_t> src((size_t)n, -1);
+        std::vector<double> hl((size_t)n, 0.0);
+        for (int l = 1; l <= lmax; l++) {
+            const LevelInfo &L = c->lv[l - 1];
+            const int64_t kd = fb->root[dim] << (l - 1);
+            const int64_t p = (int64_t)1 << (fb->level - l);
+            const double hv = fb->h / (double)(1 << (l - 1));
+            for (int bk = L.blk0; bk < L.blk0 + L.nblk; bk++) {
+                if (c->h_coord[(size_t)bk * D + dim] != kd) continue;
+                const int64_t i0 = (int64_t)(c->h_coord[(size_t)bk * D + od] - 1) * M;
+                for (int i = 1; i <= M; i++)
+                    for (int64_t j = p * (i0 + i - 1); j < p * (i0 + i); j++) {
+                        src[(size_t)j] = cell(bk, M - 1, i - 1);
+                        hl[(size_t)j] = hv;
+                    }
+            }
+        }
+        for (int64_t j = 0; j < n; j++)
+            if (src[(size_t)j] < 0) fail("the outer boundary is not covered at level 1");
+        // setouter!: contributions in the reference's order, per target cell
+        struct Contrib { int64_t cell; double coef; int64_t bs; int bl; double bd, sl; };
+        std::vector<Contrib> cs;
+        auto cyl_factor = [](int64_t cdim) { return std::fabs(1.0 + 1.0 / (double)(2 * cdim - 1)); };
+        for (int l = 1; l <= lmax; l++) {
+            const LevelInfo &L = c->lv[l - 1];
+            const double sl = s / (double)(((int64_t)1 << (l - 1)) * ((int64_t)1 << (l - 1)));
+            const int64_t p = (int64_t)1 << (fb->level - l);
+            const int64_t kd = fb->root[dim] << (l - 1);
+            for (int bk = L.blk0; bk < L.blk0 + L.nblk; bk++) {
+                if (c->h_nbr[(size_t)bk * ND + 2 * dim + 1] != NB_BOUNDARY) continue;
+                const int64_t Bd = c->h_coord[(size_t)bk * D + dim], Bo = c->h_coord[(size_t)bk * D + od];
+                const int64_t i0 = (Bo - 1) * M, Jd = (int64_t)M * (Bd - 1) + M;
+                for (int i = 1; i <= M; i++)
+                    cs.push_back({cell(bk, M - 1, i - 1), 2.0 * cyl_factor(Jd), p * (i0 + i - 1), (int)p,
+                                  (double)p, sl});
+            }
+            for (int k = L.slot0; k < L.slot0 + L.nslot; k++) {
+                const int4 sl4 = c->h_slots[(size_t)k];
+                const int bk = sl4.x, fd = sl4.y >> 1, side = sl4.y & 1;
+                const int64_t Bd = c->h_coord[(size_t)bk * D + dim], Bo = c->h_coord[(size_t)bk * D + od];
+                if (Bd != kd) continue;
+                const int dir = (fd == od) ? (side ? 1 : -1) : 0;
+                const int iv = (dir == -1) ? 1 : M;  // 1-based valid index along the other dimension
+                const int64_t j = (Bo - 1) * M + iv;
+                const int64_t Jd = (int64_t)M * (Bd - 1) + M, J1d = (Jd - 1) / 2 + 1;
+                const int64_t bs = (dir == 1) ? p * j : p * (j - 3) - 1;
+                if (bs < 0 || bs + 2 * p > n) fail("free boundary corner outside the boundary vector");
+                cs.push_back({cell(bk, M - 1, iv - 1), 2.0 * 0.25 * cyl_factor(J1d), bs, (int)(2 * p),
+                              (double)(2 * p), sl});
+            }
+        }
+        std::stable_sort(cs.begin(), cs.end(), [](const Contrib &a, const Contrib &b) { return a.cell < b.cell; });
+        std::vector<int64_t> cells, bs;
+        std::vector<int> rs, bl;
+        std::vector<double> coef, bd, slv;
+        for (size_t r = 0; r < cs.size(); r++) {
+            if (r == 0 || cs[r].cell != cs[r - 1].cell) {
+                cells.push_back(cs[r].cell);
+                rs.push_back((int)r);
+            }
+            coef.push_back(cs[r].coef);
+            bs.push_back(cs[r].bs);
+            bl.push_back(cs[r].bl);
+            bd.push_back(cs[r].bd);
+            slv.push_back(cs[r].sl);
+        }
+        rs.push_back((int)cs.size());
+        DevBuf d_src, d_hl, d_cells, d_rs, d_coef, d_bs, d_bl, d_bd, d_sl;
+        auto up = [&](DevBuf &d, const void *h, size_t bytes) {
+            d.alloc(std::max<size_t>(bytes, 8));
+            if (bytes) HIPCHK(hipMemcpyAsync(d.p, h, bytes, hipMemcpyHostToDevice, c->stream));
+        };
+        up(d_src, src.data(), src.size() * sizeof(int64_t));
+        up(d_hl, hl.data(), hl.size() * sizeof(double));
+        up(d_cells, cells.data(), cells.size() * sizeof(int64_t));
+        up(d_rs, rs.data(), rs.size() * sizeof(int));
+        up(d_coef, coef.data(), coef.size() * sizeof(double));
+        up(d_bs, bs.data(), bs.size() * sizeof(int64_t));
+        up(d_bl, bl.data(), bl.size() * sizeof(int));
+        up(d_bd, bd.data(), bd.size() * sizeof(double));
+        up(d_sl, slv.data(), slv.size() * sizeof(double));
+        const unsigned gn = (unsigned)((n + WG - 1) / WG);
+        hipLaunchKernelGGL(k_fb_gather, dim3(gn), dim3(WG), 0, c->stream, fb->u1.as<double>(), u.data.as<double>(),
+                           d_src.as<int64_t>(), d_hl.as<double>(), n);
+        hipLaunchKernelGGL(k_fb_dst, dim3(gn), dim3(WG), 0, c->stream, fb->S2T.as<double>(), fb->u1.as<double>(),
+                           fb->am.as<double>(), n, fb->f.as<double>(), 0.0, 1.0);
+        hipLaunchKernelGGL(k_fb_dst, dim3(gn), dim3(WG), 0, c->stream, fb->S3T.as<double>(), fb->am.as<double>(),
+                           fb->b.as<double>(), n, (const double *)nullptr, fb->h,
+                           (double)(1 << (fb->level - 1)));
+        if (!cells.empty())
+            hipLaunchKernelGGL(k_fb_setouter, dim3((unsigned)((cells.size() + WG - 1) / WG)), dim3(WG), 0,
+                               c->stream, q.data.as<double>(), d_cells.as<int64_t>(), d_rs.as<int>(),
+                               d_coef.as<double>(), d_bs.as<int64_t>(), d_bl.as<int>(), d_bd.as<double>(),
+                               d_sl.as<double>(), fb->b.as<double>(), (int)cells.size());
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
+        fb->have_b = true;
+    });
+}
+
+int srmg_freebc_boundary(srmg_freebc *fb, double *b, int64_t n) {
+    return guarded([&] {
+        if (!fb || !b) fail("null argument");
+        if (!fb->have_b) fail("srmg_freebc_apply has not been called");
+        if (n != fb->n) fail("wrong length");
+        HIPCHK(hipMemcpy(b, fb->b.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
     });
 }
 

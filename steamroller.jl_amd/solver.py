@@ -152,6 +152,11 @@ def lib():
         "srmg_set_group_tiles": [P, i32],
         "srmg_set_axis_gemm": [P, i32],
         "srmg_electric_field": [P, i32, i32, dbl, P, P, P, i32],
+        "srmg_freebc_create": [P, C.POINTER(P), i32, i32, P, dbl, P, i64],
+        "srmg_freebc_destroy": [P],
+        "srmg_freebc_apply": [P, P, i32, i32, dbl],
+        "srmg_freebc_boundary": [P, P, i64],
+        "srmg_freebc_electric_field": [P, P, i32, i32, dbl, P, P, P, i32],
         "srmg_set_guess": [P, P],
         "srmg_guess_fallbacks": [P, C.POINTER(i32)],
         "srmg_grid_solve": [P, i32, P, dbl, P, P, P, i32],
