@@ -393,6 +393,26 @@ def fmg(u, b, r, u1, s, tree=None, conn=None, geometry=None, bc=None, lpldisc=No
                           int(ndown), int(ntop), int(nup), float(omega)))
 
 
+def poisson_step(u, q1, r, u1, s, geometry, pbc, lpl, *, poisson_iter=1, poisson_fmg=False, nup=2, ndown=2,
+                 ntop=4, freebc=None, coarse=COARSE_GS):
+    """The Poisson block of derivs! (src/streamer.jl:320-333), with the charge
+    already restricted into q1: setfreebc! when a free boundary is given
+    (``srmg.freebc.FreeBC``), then ``poisson_iter`` times the ghost pre-fill of u
+    on every level and one fmg! (``poisson_fmg``) or vcycle!, warm-started from
+    the previous u. (nup, ndown, ntop) is the reference's poisson_level_iter."""
+    if freebc is not None:
+        freebc.setfreebc(q1, r, u1, s, pbc, lpl, poisson_iter=poisson_iter, nup=nup, ndown=ndown, ntop=ntop,
+                         poisson_fmg=poisson_fmg)
+    for _ in range(int(poisson_iter)):
+        fill_ghost_all(u, pbc)
+        if poisson_fmg:
+            fmg(u, q1, r, u1, s, geometry=geometry, bc=pbc, lpldisc=lpl, nup=nup, ndown=ndown, ntop=ntop,
+                coarse=coarse)
+        else:
+            vcycle(u, q1, r, u1, s, geometry=geometry, bc=pbc, lpldisc=lpl, nup=nup, ndown=ndown, ntop=ntop,
+                   coarse=coarse)
+
+
 def resnorm(r, b, u, s, tree=None, conn=None, geometry=None, bc=None, lpldisc=None, *,
             coarse=COARSE_GS) -> float:
     """resnorm! (src/multigrid.jl:391-417)."""

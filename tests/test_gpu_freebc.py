@@ -119,3 +119,39 @@ def test_electric_field_with_free_ghosts_matches_oracle(name):
     # the free ghosts changed the radial field at the outer boundary
     Ep = solver.electric_field(u, eabs, p.h, e0, p.bc, G=p.b.G)
     assert not np.array_equal(E, Ep)
+
+
+@pytest.mark.parametrize("poisson_fmg", [False, True])
+def test_poisson_step_with_free_boundary_matches_oracle(poisson_fmg):
+    # derivs!'s Poisson block (src/streamer.jl:320-333): setfreebc!, then
+    # poisson_iter warm-start cycles (fmg! or vcycle!) on u
+    p, root = _outer_refined()
+    level = len(p.tree)
+    geom, disc = solver.CylindricalGeometry(1), solver.LaplacianDiscretization(2, 2)
+    ctx = solver.Context(p.tree, p.conn, p.M)
+    fb = FreeBC(ctx, geom, level, root, p.h)
+    u, q1, r, u1 = (ctx.field(F64) for _ in range(4))
+    for _ in range(2):  # two steps: the second starts warm (u and the homogeneous potential)
+        q1.upload(p.b)  # the charge, as rhs_level! rebuilds q1 every step
+        solver.poisson_step(u, q1, r, u1, p.s, geom, p.bc, disc, poisson_iter=2, poisson_fmg=poisson_fmg,
+                            freebc=fb)
+    got = p.b.like()
+    u.download(got)
+    o = Oracle.for_problem(p)
+    uo, uh, ro, u1o = p.b.like(), p.b.like(), p.b.like(), p.b.like()
+    for _ in range(2):
+        qo = p.b.copy()
+        for _ in range(2):
+            o.fill_ghost_all(uh)
+            if poisson_fmg:
+                o.fmg(uh, qo, ro, u1o, p.s, ndown=2, ntop=4, nup=2)
+            else:
+                o.vcycle(uh, qo, ro, u1o, p.s, ndown=2, ntop=4, nup=2)
+        o.freebc(qo, uh, p.h, p.s, 1, level, root, fb.f)
+        for _ in range(2):
+            o.fill_ghost_all(uo)
+            if poisson_fmg:
+                o.fmg(uo, qo, ro, u1o, p.s, ndown=2, ntop=4, nup=2)
+            else:
+                o.vcycle(uo, qo, ro, u1o, p.s, ndown=2, ntop=4, nup=2)
+    np.testing.assert_array_equal(_valid(got, p), _valid(uo, p))
