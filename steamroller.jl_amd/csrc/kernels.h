@@ -1447,13 +1447,102 @@ __global__ __launch_bounds__(1024) void k_resid_restrict(const T *__restrict__ u
 // tile) the parent's cells are restricted in k_restrict's order. Same
 // operations as k_resid_restrict, so the same bits. Work groups past ngroups
 // copy u to u1 on the coarse level's leaf blocks (256 cells each).
+// Column staging shared by the column residual kernels (3D, 8^3, one sibling
+// group of 16^3 cells per 256-thread work group): thread (w, lane) owns the
+// (X, Y) column X + Y = w (mod 2) of rows 8 (w >> 1) .. 8 (w >> 1) + 7 (as in
+// k_gs_sweep_col); its 16 cells of u and b go to registers, u with the group's
+// depth-1 face halo to the LDS tile. tab holds the group's own face entries
+// (2D per block, loaded by the caller into tv), prm the block parameters.
 template <int D, int M, typename T, int RD>
-struct ColResid {
-    static_assert(D == 3 && M == 8 && RD < 0, "column residual is written for 8^3 Cartesian blocks");
+struct ColStage {
+    static_assert(D == 3 && M == 8 && RD < 0, "column kernels are written for 8^3 Cartesian blocks");
     using LY = Lay<D, M>;
     using TL = Tile<D, M, 2>;
     static constexpr int NB = 8, V = LY::V, NG = 2 * M, ND = 2 * D;
     static constexpr int SY = TL::stride(1), SZ = TL::stride(2);
+
+    struct Col {
+        int X, Y, x, y, kxy, pc;
+        size_t gb;  // the column's base offset in its lower block
+    };
+
+    template <int P>
+    __device__ static __forceinline__ Col column(int g0, int w, int lane) {
+        Col c;
+        c.Y = 8 * (w >> 1) + (lane >> 3);
+        c.X = 2 * (lane & 7) + ((c.Y + P) & 1);
+        c.x = c.X & 7;
+        c.y = c.Y & 7;
+        c.kxy = (c.X >> 3) | ((c.Y >> 3) << 1);
+        c.gb = (size_t)(g0 + c.kxy) * V + (size_t)((c.x >> 1) + LY::MH * c.y);
+        c.pc = TL::at(c.X, c.Y, 0);
+        return c;
+    }
+
+    template <int P>
+    __device__ static __forceinline__ void stage(T *tile, int *tab, BP<T> *prm, const Col &c, const T *__restrict__ u,
+                                                 const T *__restrict__ b, const T *__restrict__ ib,
+                                                 const LevelArgs<T> &a, int g0, int w, int lane, int tv,
+                                                 const BP<T> &bpl, T *ux, T *bx) {
+        using CS = ColSweep<D, M, T, RD>;
+        const int tid = w * 64 + lane;
+        const T *__restrict__ ul = u + c.gb, *__restrict__ uh = ul + 4 * V;
+        const T *__restrict__ bl = b + c.gb, *__restrict__ bh = bl + 4 * V;
+        static_for<NG>([&](auto zc) {
+            constexpr int Z = decltype(zc)::value;
+            ux[Z] = (Z < 8 ? ul : uh)[CS::template zoff<P, Z>()];
+            bx[Z] = (Z < 8 ? bl : bh)[CS::template zoff<P, Z>()];
+        });
+        if (tid < NB * ND) tab[tid] = tv;
+        if (tid < NB) prm[tid] = bpl;
+        lds_barrier();
+        // halo of the six group faces (wave w: block face w of each; lane: cell)
+        T hv[6], hm[6];
+        static_for<6>([&](auto fc) {
+            constexpr int F = decltype(fc)::value;
+            constexpr int DIM = F >> 1, DA = DIM == 0 ? 1 : 0, DB = DIM == 2 ? 1 : 2;
+            const int k = ((F & 1) << DIM) | ((w & 1) << DA) | ((w >> 1) << DB);
+            const int e = __builtin_amdgcn_readfirstlane(tab[k * ND + F]);
+            hv[F] = halo_value<D, M, T, F>(e, g0 + k, lane, u, ib, T(1));
+            hm[F] = e == NB_BOUNDARY ? a.sg[F] : T(1);
+        });
+        static_for<NG>([&](auto zc) {
+            constexpr int Z = decltype(zc)::value;
+            tile[c.pc + Z * SZ] = ux[Z];
+        });
+        static_for<6>([&](auto fc) {
+            constexpr int F = decltype(fc)::value;
+            constexpr int DIM = F >> 1, DA = DIM == 0 ? 1 : 0, DB = DIM == 2 ? 1 : 2;
+            const int k = ((F & 1) << DIM) | ((w & 1) << DA) | ((w >> 1) << DB);
+            tile[TL::origin(k) + halo_at<D, M, F, TL>(lane)] = hm[F] * hv[F];
+        });
+        lds_barrier();
+    }
+
+    // L u of cell Z of the column (tstencil's order of terms)
+    template <int Z>
+    __device__ static __forceinline__ T lu(const T *t, const Col &c, const T *ux, T cc) {
+        const int p = c.pc + Z * SZ;
+        const T zm = (Z == 0) ? t[c.pc - SZ] : ux[Z > 0 ? Z - 1 : 0];
+        const T zp = (Z == NG - 1) ? t[c.pc + NG * SZ] : ux[Z < NG - 1 ? Z + 1 : 0];
+        T v = T(0);
+        v += zm;
+        v += t[p - SY];
+        v += t[p - 1];
+        v += cc * ux[Z];
+        v += t[p + 1];
+        v += t[p + SY];
+        v += zp;
+        return v;
+    }
+};
+
+template <int D, int M, typename T, int RD>
+struct ColResid {
+    using CT = ColStage<D, M, T, RD>;
+    using LY = Lay<D, M>;
+    using TL = Tile<D, M, 2>;
+    static constexpr int NB = 8, V = LY::V, NG = 2 * M, ND = 2 * D;
     struct Smem {
         T tile[TL::SZ];
         T rt[NG * NG * NG];  // residual of group cell (X, Y, Z) at X + NG (Y + NG Z)
@@ -1467,62 +1556,17 @@ struct ColResid {
                                                const LevelArgs<T> &a, int g0, int w, int lane, int tv,
                                                const BP<T> &bpl, T *__restrict__ rc, T *__restrict__ uc,
                                                T *__restrict__ u1c) {
-        using CS = ColSweep<D, M, T, RD>;
         const int tid = w * 64 + lane;
-        const int Y = 8 * (w >> 1) + (lane >> 3), X = 2 * (lane & 7) + ((Y + P) & 1);
-        const int x = X & 7, y = Y & 7, kxy = (X >> 3) | ((Y >> 3) << 1);
-        const size_t gb = (size_t)(g0 + kxy) * V + (size_t)((x >> 1) + LY::MH * y);
-        const T *__restrict__ ul = u + gb, *__restrict__ uh = ul + 4 * V;
-        const T *__restrict__ bl = b + gb, *__restrict__ bh = bl + 4 * V;
+        const typename CT::Col c = CT::template column<P>(g0, w, lane);
         T ux[NG], bx[NG];
+        CT::template stage<P>(sm.tile, sm.tab, sm.prm, c, u, b, ib, a, g0, w, lane, tv, bpl, ux, bx);
+        // residual r = -(b + L u / s)
+        const BP<T> bq[2] = {sm.prm[c.kxy], sm.prm[c.kxy | 4]};
         static_for<NG>([&](auto zc) {
             constexpr int Z = decltype(zc)::value;
-            ux[Z] = (Z < 8 ? ul : uh)[CS::template zoff<P, Z>()];
-            bx[Z] = (Z < 8 ? bl : bh)[CS::template zoff<P, Z>()];
-        });
-        if (tid < NB * ND) sm.tab[tid] = tv;
-        if (tid < NB) sm.prm[tid] = bpl;
-        lds_barrier();
-        // halo of the six group faces (wave w: block face w of each; lane: cell)
-        T hv[6], hm[6];
-        static_for<6>([&](auto fc) {
-            constexpr int F = decltype(fc)::value;
-            constexpr int DIM = F >> 1, DA = DIM == 0 ? 1 : 0, DB = DIM == 2 ? 1 : 2;
-            const int k = ((F & 1) << DIM) | ((w & 1) << DA) | ((w >> 1) << DB);
-            const int e = __builtin_amdgcn_readfirstlane(sm.tab[k * ND + F]);
-            hv[F] = halo_value<D, M, T, F>(e, g0 + k, lane, u, ib, T(1));
-            hm[F] = e == NB_BOUNDARY ? a.sg[F] : T(1);
-        });
-        const int pc = TL::at(X, Y, 0);
-        static_for<NG>([&](auto zc) {
-            constexpr int Z = decltype(zc)::value;
-            sm.tile[pc + Z * SZ] = ux[Z];
-        });
-        static_for<6>([&](auto fc) {
-            constexpr int F = decltype(fc)::value;
-            constexpr int DIM = F >> 1, DA = DIM == 0 ? 1 : 0, DB = DIM == 2 ? 1 : 2;
-            const int k = ((F & 1) << DIM) | ((w & 1) << DA) | ((w >> 1) << DB);
-            sm.tile[TL::origin(k) + halo_at<D, M, F, TL>(lane)] = hm[F] * hv[F];
-        });
-        lds_barrier();
-        // residual r = -(b + L u / s) (tstencil's order of terms)
-        const BP<T> bq[2] = {sm.prm[kxy], sm.prm[kxy | 4]};
-        const T *const t = sm.tile;
-        static_for<NG>([&](auto zc) {
-            constexpr int Z = decltype(zc)::value;
-            const int p = pc + Z * SZ;
             const BP<T> &bp = bq[Z >> 3];
-            const T zm = (Z == 0) ? t[pc - SZ] : ux[Z > 0 ? Z - 1 : 0];
-            const T zp = (Z == NG - 1) ? t[pc + NG * SZ] : ux[Z < NG - 1 ? Z + 1 : 0];
-            T v = T(0);
-            v += zm;
-            v += t[p - SY];
-            v += t[p - 1];
-            v += bp.cc * ux[Z];
-            v += t[p + 1];
-            v += t[p + SY];
-            v += zp;
-            sm.rt[X + NG * (Y + NG * Z)] = -(bx[Z] + v / bp.s);
+            const T v = CT::template lu<Z>(sm.tile, c, ux, bp.cc);
+            sm.rt[c.X + NG * (c.Y + NG * Z)] = -(bx[Z] + v / bp.s);
         });
         lds_barrier();
         // the parent's cells, in storage order: coarse cell (X0, X1, X2) sums
@@ -1548,6 +1592,100 @@ struct ColResid {
         }
     }
 };
+
+// k_resid_group<RES_DEFECT> (FP64 in, FP32 out) with the column mapping: out =
+// b + L u / s on every cell of the group, and per block the partial
+// wgt * sum of r^2 over its cells if it is a leaf (0 otherwise), summed as
+// k_resid_group sums it (thread t of 256: cells q = t, t + 256 in that order,
+// then the halving tree over t), so the partials are the same bits.
+template <int D, int M, int RD>
+struct ColDefect {
+    using T = double;
+    using CT = ColStage<D, M, T, RD>;
+    using LY = Lay<D, M>;
+    using TL = Tile<D, M, 2>;
+    static constexpr int NB = 8, V = LY::V, NG = 2 * M, ND = 2 * D;
+    static_assert(TL::SZ >= NB * V, "the squares reuse the tile");
+    struct Smem {
+        T tile[TL::SZ];  // u with its halo; then the squares sq[k V + q] of block k, cell q
+        int tab[NB * ND];
+        BP<T> prm[NB];
+        int leaf[NB];
+    };
+
+    template <int P>
+    __device__ static __forceinline__ void run(Smem &sm, float *__restrict__ out, const T *__restrict__ u,
+                                               const T *__restrict__ b, const T *__restrict__ ib,
+                                               const LevelArgs<T> &a, int g0, int w, int lane, int tv,
+                                               const BP<T> &bpl, double wgt, double *__restrict__ partial) {
+        using CS = ColSweep<D, M, T, RD>;
+        const int tid = w * 64 + lane;
+        const typename CT::Col c = CT::template column<P>(g0, w, lane);
+        T ux[NG], bx[NG];
+        CT::template stage<P>(sm.tile, sm.tab, sm.prm, c, u, b, ib, a, g0, w, lane, tv, bpl, ux, bx);
+        const BP<T> bq[2] = {sm.prm[c.kxy], sm.prm[c.kxy | 4]};
+        const bool lf[2] = {sm.leaf[c.kxy] != 0, sm.leaf[c.kxy | 4] != 0};
+        float *__restrict__ ol = out + c.gb, *__restrict__ oh = ol + 4 * V;
+        double sq[NG];
+        static_for<NG>([&](auto zc) {
+            constexpr int Z = decltype(zc)::value;
+            const BP<T> &bp = bq[Z >> 3];
+            const T v = CT::template lu<Z>(sm.tile, c, ux, bp.cc);
+            (Z < 8 ? ol : oh)[CS::template zoff<P, Z>()] = (float)(bx[Z] + v / bp.s);
+            const T rr = -(bx[Z] + v / bp.s);
+            sq[Z] = lf[Z >> 3] ? (double)rr * (double)rr : 0.0;
+        });
+        lds_barrier();  // every tile read is done: the squares take its place
+        static_for<NG>([&](auto zc) {
+            constexpr int Z = decltype(zc)::value;
+            const int k = c.kxy | ((Z >> 3) << 2);
+            sm.tile[k * V + LY::sidx(c.x, c.y, Z & 7)] = sq[Z];
+        });
+        lds_barrier();
+        // per block: acc_t = (0 + sq[t]) + sq[t + 256], then the halving tree
+        for (int k = 0; k < NB; k++) {
+            double acc = 0.0;
+            acc += sm.tile[k * V + tid];
+            acc += sm.tile[k * V + tid + 256];
+            sm.tile[k * V + tid] = acc;
+        }
+        lds_barrier();
+        for (int wd = 128; wd > 0; wd >>= 1) {
+            for (int i = tid; i < NB * wd; i += 256) {
+                const int k = i / wd, t = i - k * wd;
+                sm.tile[k * V + t] += sm.tile[k * V + t + wd];
+            }
+            lds_barrier();
+        }
+        if (tid < NB) partial[g0 + tid - a.blk0] = sm.tile[tid * V] * wgt;
+    }
+};
+
+template <int D, int M, int RD>
+__global__ __launch_bounds__(256) void k_defect_col(float *__restrict__ out, const double *__restrict__ u,
+                                                    const double *__restrict__ b, const double *__restrict__ ib,
+                                                    const int *__restrict__ htab, const int *__restrict__ isleaf,
+                                                    LevelArgs<double> a, double wgt, double *__restrict__ partial) {
+    using CD = ColDefect<D, M, RD>;
+    using HT = Halo<D>;
+    constexpr int ND = CD::ND;
+    __shared__ typename CD::Smem sm;
+    const int tid = threadIdx.x;
+    const int g0 = a.blk0 + xcd_remap(blockIdx.x, gridDim.x) * CD::NB;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    int tv = 0;
+    if (tid < CD::NB * ND) {
+        const int lb = tid / ND;
+        tv = htab[(size_t)(g0 + lb) * HT::N + (tid - lb * ND)];
+    }
+    BP<double> bpl{};
+    if (tid < CD::NB) {
+        bpl = bparams(a, g0 + tid);
+        sm.leaf[tid] = isleaf[g0 + tid];
+    }
+    if (w & 1) CD::template run<1>(sm, out, u, b, ib, a, g0, w, lane, tv, bpl, wgt, partial);
+    else CD::template run<0>(sm, out, u, b, ib, a, g0, w, lane, tv, bpl, wgt, partial);
+}
 
 template <int D, int M, typename T, int RD>
 __global__ __launch_bounds__(256) void k_resid_restrict_col(const T *__restrict__ u, const T *__restrict__ b,

@@ -1001,10 +1001,16 @@ struct Ops {
                 constexpr int RD = decltype(rd)::value;
                 if constexpr (GROUP_OK) {
                     if (grouped) {
-                        hipLaunchKernelGGL((k_resid_group<D, M, T, float, RNT, RES_DEFECT, RD>),
-                                           dim3((unsigned)(a.nblk >> D)), dim3(1024), 0, c->stream, be, X(u, l),
-                                           X(b, l), I(u), c->d_halo.as<int>(), coord(), c->d_isleaf.as<int>(), a,
-                                           wgt, c->d_partial.as<double>() + np);
+                        // 3D 8^3 Cartesian: the column form (k_defect_col), same bits
+                        if constexpr (D == 3 && M == 8 && RD < 0 && std::is_same<T, double>::value)
+                            hipLaunchKernelGGL((k_defect_col<D, M, RD>), dim3((unsigned)(a.nblk >> D)), dim3(256),
+                                               0, c->stream, be, X(u, l), X(b, l), I(u), c->d_halo.as<int>(),
+                                               c->d_isleaf.as<int>(), a, wgt, c->d_partial.as<double>() + np);
+                        else
+                            hipLaunchKernelGGL((k_resid_group<D, M, T, float, RNT, RES_DEFECT, RD>),
+                                               dim3((unsigned)(a.nblk >> D)), dim3(1024), 0, c->stream, be,
+                                               X(u, l), X(b, l), I(u), c->d_halo.as<int>(), coord(),
+                                               c->d_isleaf.as<int>(), a, wgt, c->d_partial.as<double>() + np);
                         return;
                     }
                 }
