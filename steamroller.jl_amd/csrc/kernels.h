@@ -1597,7 +1597,10 @@ struct ColResid {
 // b + L u / s on every cell of the group, and per block the partial
 // wgt * sum of r^2 over its cells if it is a leaf (0 otherwise), summed as
 // k_resid_group sums it (thread t of 256: cells q = t, t + 256 in that order,
-// then the halving tree over t), so the partials are the same bits.
+// then the halving tree over t), so the partials are the same bits. With
+// out == nullptr only the partials are formed: on a level whose blocks are all
+// leaves, in level order, these are the partials of k_resid_tile<RES_NORM>
+// (the same per-block order).
 template <int D, int M, int RD>
 struct ColDefect {
     using T = double;
@@ -1631,7 +1634,7 @@ struct ColDefect {
             constexpr int Z = decltype(zc)::value;
             const BP<T> &bp = bq[Z >> 3];
             const T v = CT::template lu<Z>(sm.tile, c, ux, bp.cc);
-            (Z < 8 ? ol : oh)[CS::template zoff<P, Z>()] = (float)(bx[Z] + v / bp.s);
+            if (out != nullptr) (Z < 8 ? ol : oh)[CS::template zoff<P, Z>()] = (float)(bx[Z] + v / bp.s);
             const T rr = -(bx[Z] + v / bp.s);
             sq[Z] = lf[Z >> 3] ? (double)rr * (double)rr : 0.0;
         });

@@ -57,6 +57,7 @@ struct LevelInfo {
     int ext[3] = {1, 1, 1};     // largest level domain in blocks over the trees
     int wofs[2] = {0, 0};       // radial weight table offsets (dims 0 and 1)
     bool grouped = false;       // blocks form complete sibling groups of 2^D (k_gs_sweep_group)
+    bool all_leaves = false;    // every block is a leaf and the leaf list is the block order
 };
 
 // One tree of a forest: an independent problem whose blocks share the
@@ -379,6 +380,10 @@ static void set_forest(srmg_ctx *c, const std::vector<TreeRecords> &tr) {
             T.nleaf[l] = (int)c->h_leaves.size() - c->lv[l].leaf0 - T.leafk0[l];
         }
         c->lv[l].nleaf = (int)c->h_leaves.size() - c->lv[l].leaf0;
+        LevelInfo &L = c->lv[l];
+        L.all_leaves = L.nblk > 0 && L.nleaf == L.nblk;
+        for (int k = 0; k < L.nleaf && L.all_leaves; k++)
+            if (c->h_leaves[(size_t)(L.leaf0 + k)] != L.blk0 + k) L.all_leaves = false;
     }
     // every index a kernel will follow stays inside its table
     for (size_t i = 0; i < c->h_nbr.size(); i++) {
@@ -810,7 +815,8 @@ struct Ops {
             const int nleafc = u1 ? Lc.nleaf : 0;
             with_rd(a.rdim, [&](auto rd) {
                 constexpr int RD = decltype(rd)::value;
-                // FP32 3D 8^3 Cartesian: the column form (k_resid_restrict_col)
+                // FP32 3D 8^3 Cartesian: the column form (k_resid_restrict_col;
+                // in FP64 its 80 KB of LDS per group make it no faster)
                 if constexpr (D == 3 && M == 8 && RD < 0 && std::is_same<T, float>::value) {
                     const unsigned ncp = (unsigned)(((int64_t)nleafc * LY::V + 255) / 256);
                     hipLaunchKernelGGL((k_resid_restrict_col<D, M, T, RD>), dim3(ng + ncp), dim3(256), 0, c->stream,
@@ -953,6 +959,17 @@ struct Ops {
             unsigned g = (unsigned)((L.nleaf + RB - 1) / RB);
             with_rd(a.rdim, [&](auto rd) {
                 constexpr int RD = decltype(rd)::value;
+                // grouped 3D 8^3 Cartesian level of leaves in block order: the
+                // column form (the same per-block sums)
+                if constexpr (GROUP_OK && D == 3 && M == 8 && RD < 0 && std::is_same<T, double>::value) {
+                    if (grouped(l) && L.all_leaves) {
+                        hipLaunchKernelGGL((k_defect_col<D, M, RD>), dim3((unsigned)(L.nblk >> D)), dim3(256), 0,
+                                           c->stream, (float *)nullptr, X(u, l), X(b, l), I(u),
+                                           c->d_halo.as<int>(), c->d_isleaf.as<int>(), a, wgt,
+                                           c->d_partial.as<double>() + np);
+                        return;
+                    }
+                }
                 hipLaunchKernelGGL((k_resid_tile<D, M, T, T, RB, RNT, RES_NORM, RD>), dim3(g), dim3(RNT), 0,
                                    c->stream, (T *)nullptr, X(u, l), X(b, l), I(u), c->d_halo.as<int>(), coord(),
                                    c->d_leaves.as<int>() + L.leaf0, (const int *)nullptr, L.nleaf, a, wgt,
