@@ -60,6 +60,14 @@ __device__ __forceinline__ void static_for(Fn &&f) {
     }
 }
 
+// Work-group barrier that orders LDS accesses only: global loads still in
+// flight stay in flight across it (__syncthreads would wait for them).
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Value of cell (c0,c1,c2) + dir e_dim of block blk, crossing the block face if
 // needed: same-level neighbour cell, mirror (depth-1 ghost = sign * own cell,
 // src/boundary.jl:45-47 with src/blockfield.jl:513-521), or the coarse-fine
@@ -928,14 +936,6 @@ __global__ __launch_bounds__(NT) void k_gs_sweep_group(T *__restrict__ uo, const
     }
 }
 
-// Work-group barrier that orders LDS accesses only: global loads still in
-// flight stay in flight across it (__syncthreads would wait for them).
-__device__ __forceinline__ void lds_barrier() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
 // k_gs_sweep_group for D = 3, M = 8 with a column mapping: 256 threads per
 // sibling group (16^3 cells), each thread owns one (X, Y) column of the group
 // tile and keeps its 16 cells and right-hand sides in registers, so that the
@@ -1108,8 +1108,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
 
 // Stage BPW blocks (valid cells + depth-1 face halo) of x in the LDS tile.
 // list != nullptr selects the blocks through an index list (leaf blocks).
-// Two barriers: after (interior, halo table) and after the halo cells.
-template <int D, int M, typename T, int BPW, int NT>
+// Two barriers: after (interior, halo table) and after the halo cells
+// (LDSB: barriers that order LDS only, for callers with loads in flight).
+template <int D, int M, typename T, int BPW, int NT, bool LDSB = false>
 __device__ __forceinline__ void load_tile(T *tile, int *tab, int *blks, const T *__restrict__ x,
                                           const T *__restrict__ ib, const int *__restrict__ htab,
                                           const int *__restrict__ list, int first, int nloc, const T *sg) {
@@ -1119,7 +1120,7 @@ __device__ __forceinline__ void load_tile(T *tile, int *tab, int *blks, const T 
     constexpr int F = LY::F, V = LY::V, ND = 2 * D;
     const int tid = threadIdx.x;
     if (tid < BPW) blks[tid] = (tid < nloc) ? (list ? list[first + tid] : first + tid) : 0;
-    __syncthreads();
+    if (LDSB) lds_barrier(); else __syncthreads();
     for (int i = tid; i < nloc * V; i += NT) {
         int lb = i / V, q = i - lb * V;
         int cx, cy, cz;
@@ -1130,13 +1131,13 @@ __device__ __forceinline__ void load_tile(T *tile, int *tab, int *blks, const T 
         int lb = i / ND;
         tab[i] = htab[(size_t)blks[lb] * HT::N + (i - lb * ND)];
     }
-    __syncthreads();
+    if (LDSB) lds_barrier(); else __syncthreads();
     for (int i = tid; i < nloc * ND * F; i += NT) {
         int lb = i / (ND * F), rem = i - lb * (ND * F);
         int f = rem / F, t = rem - f * F;
         halo_switch<D, M, T>(tile + lb * TL::SZ, tab[lb * ND + f], f, blks[lb], t, x, ib, sg);
     }
-    __syncthreads();
+    if (LDSB) lds_barrier(); else __syncthreads();
 }
 
 // k_interp_add on a level of complete sibling groups: the 2^D children of a
@@ -1157,7 +1158,12 @@ __device__ __forceinline__ void interp_add_group_one(int g, T *__restrict__ u, c
     __shared__ int tab[2 * D];
     __shared__ int blks[1];
     const int g0 = blk0 + g * NB;
-    load_tile<D, M, T, 1, NT>(tile, tab, blks, u1, ib1, htab, parent, g0, 1, ac.sg);
+    const int pb = parent[g0];
+    // the children's cells next: their loads stay in flight while the parent is staged
+    T uo[NB * V / NT];
+#pragma unroll
+    for (int k = 0; k < NB * V / NT; k++) uo[k] = u[(size_t)g0 * V + threadIdx.x + k * NT];
+    load_tile<D, M, T, 1, NT, true>(tile, tab, blks, u1, ib1, htab, nullptr, pb, 1, ac.sg);
 #pragma unroll
     for (int k = 0; k < NB * V / NT; k++) {
         const int i = threadIdx.x + k * NT, lb = i / V, q = i - lb * V;
@@ -1172,7 +1178,7 @@ __device__ __forceinline__ void interp_add_group_one(int g, T *__restrict__ u, c
         f += w1<T>() * tile[p + (2 * (x1 & 1) - 1) * TL::stride(1)];
         if (D == 3) f += w1<T>() * tile[p + (2 * (x2 & 1) - 1) * TL::stride(2)];
         const size_t o = (size_t)(g0 + lb) * V + q;
-        u[o] = f + u[o];
+        u[o] = f + uo[k];
     }
 }
 
