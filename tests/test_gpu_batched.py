@@ -157,3 +157,64 @@ def test_forest_sibling_group_sweeps_equal_block_sweeps(cfg):
                                   ndown=2, ntop=4, nup=2, coarse=solver.COARSE_EXACT)
         outs.append(np.asarray(_download(u, probs)[0]).copy())
     np.testing.assert_array_equal(outs[1], outs[0])
+
+
+def _problems_3d():
+    """Three 3D 8^3 trees with sibling groups on every refined level: a Poisson
+    problem and two Helmholtz terms (per-tree block parameters)."""
+    def crit(l, c):
+        if l == 1:
+            return (c[:, 0] >= 1) & (c[:, 1] <= 2) & (c[:, 2] <= 2)
+        return (c[:, 0] <= 2) & (c[:, 1] <= 3)
+    out = []
+    for k, (helm, k2) in enumerate([(False, 0.0), (True, 3e4), (True, 4e5)]):
+        out.append(configs.uniform_problem(3, (2, 2, 2), 8, 3, 0, ((1, 1), (1, 1), (-1, -1)), helm=helm, k2=k2,
+                                           seed=20 + k, refine=crit))
+    return out
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_forest_3d_column_kernels_equal_single_solves(mixed):
+    # the column-mapped sweep (shape 7), residual/restriction, defect and norm
+    # kernels on forests: every tree gets its own bits
+    probs = _problems_3d()
+    geom = solver.CartesianGeometry()
+    ctx = solver.Context.forest([(p.tree, p.conn) for p in probs], 8)
+    nl = max(len(p.tree) for p in probs)
+    print(f"grouped levels {[ctx.level_grouped(l) for l in range(1, nl + 1)]}", flush=True)
+    for l in range(1, nl + 1):
+        ctx.set_sweep_config(F64, l, 7)
+        ctx.set_sweep_config(F32, l, 7)
+    u, b, r, u1 = (ctx.field(F64) for _ in range(4))
+    b.upload(solver.concat_fields([p.b for p in probs]))
+    discs = [_disc(p) for p in probs]
+    if mixed:
+        e, be, re, ue = (ctx.field(F32) for _ in range(4))
+        n, hist = solver.solve_mixed_batched(u, b, e, be, re, ue, [p.s for p in probs], geom, probs[0].bc, discs,
+                                             tol=1e-30, max_cycles=3, inner_cycles=2)
+    else:
+        n, hist = solver.solve_batched(u, b, r, u1, [p.s for p in probs], geom, probs[0].bc, discs, tol=1e-30,
+                                       max_cycles=3)
+    # a fixed number of cycles for every tree (the batch would otherwise run
+    # until its slowest tree converges)
+    assert n == 3
+    got = _download(u, probs)
+    for k, p in enumerate(probs):
+        c1 = solver.Context(p.tree, p.conn, p.M)
+        for l in range(1, len(p.tree) + 1):
+            c1.set_sweep_config(F64, l, 7)
+            c1.set_sweep_config(F32, l, 7)
+        v, bb, rr, vv = (c1.field(F64) for _ in range(4))
+        bb.upload(p.b)
+        if mixed:
+            e1, be1, re1, ue1 = (c1.field(F32) for _ in range(4))
+            n1, h1 = solver.solve_mixed(v, bb, e1, be1, re1, ue1, p.s, geom, p.bc, _disc(p), tol=1e-30,
+                                        max_cycles=3, inner_cycles=2)
+        else:
+            n1, h1 = solver.solve(v, bb, rr, vv, p.s, geom, p.bc, _disc(p), tol=1e-30, max_cycles=3)
+        assert n1 == 3
+        # per-tree norms are summed in another fixed order than a lone tree's
+        np.testing.assert_allclose(hist[k], h1, rtol=1e-12, atol=0)
+        ref = p.b.like()
+        v.download(ref)
+        np.testing.assert_array_equal(_valid(got[k], p), _valid(ref.u, p))
