@@ -181,7 +181,9 @@ def test_forest_3d_column_kernels_equal_single_solves(mixed):
     geom = solver.CartesianGeometry()
     ctx = solver.Context.forest([(p.tree, p.conn) for p in probs], 8)
     nl = max(len(p.tree) for p in probs)
-    print(f"grouped levels {[ctx.level_grouped(l) for l in range(1, nl + 1)]}", flush=True)
+    grouped = [ctx.level_grouped(l) for l in range(1, nl + 1)]
+    print(f"grouped levels {grouped}", flush=True)
+    assert all(grouped[1:]), "the column kernels run on grouped levels only"
     for l in range(1, nl + 1):
         ctx.set_sweep_config(F64, l, 7)
         ctx.set_sweep_config(F32, l, 7)
