@@ -83,6 +83,17 @@ struct CoarseGroup {
     DevBuf cs, shift;           // per tree: -s (gather factor) and the Helmholtz shift -s k2
 };
 
+// a captured solve step: the graph and its executable
+struct Graph {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t x = nullptr;
+    uint64_t gen = 0;  // srmg_ctx::graph_gen when captured
+    ~Graph() {
+        if (x) (void)hipGraphExecDestroy(x);
+        if (g) (void)hipGraphDestroy(g);
+    }
+};
+
 struct srmg_ctx {
     int D = 2, M = 16, V = 0, F = 0;
     int nlevels = 0;
@@ -117,6 +128,11 @@ struct srmg_ctx {
     double pomega = 1.0;
     std::vector<double> pp_host;   // 3 x ntrees x nlevels: (s_l, cc, nwc)
     DevBuf d_pp64, d_pp32;
+    // captured solve steps, keyed by everything their launches bake in (solve
+    // kind, fields, scales, operators, options, tile shapes, kernel switches);
+    // cleared whenever a buffer they point to may move or change its role
+    std::unordered_map<std::string, std::unique_ptr<Graph>> graphs;
+    uint64_t graph_gen = 0;  // bumped when coarse-solve matrices are rebuilt (older graphs point to freed ones)
 };
 
 static void alloc_field(srmg_ctx *c, FieldBuf &f);
@@ -166,6 +182,7 @@ struct TreeRecords {
 };
 
 static void set_forest(srmg_ctx *c, const std::vector<TreeRecords> &tr) {
+    c->graphs.clear();
     const int D = c->D, M = c->M, ND = 2 * D;
     const int nt = (int)tr.size();
     if (nt < 1) fail("a forest needs at least one tree");
@@ -1313,6 +1330,7 @@ void Ops<D, M, T>::coarse_exact(FieldBuf &u, FieldBuf &b, double s, const srmg_o
             hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
             HIPCHK(hipStreamIsCapturing(c->stream, &st));
             if (st != hipStreamCaptureStatusNone) fail("coarse-solve setup inside graph capture");
+            c->graph_gen++;  // captured steps point to the matrices replaced here
             int rdim = (op->geometry == SRMG_CYL1) ? 0 : (op->geometry == SRMG_CYL2 ? 1 : -1);
             for (int d = 0; d < D; d++) {
                 int il = 0, ih = 0, m = 1;
@@ -1517,15 +1535,6 @@ static void from_device_layout(srmg_ctx *c, const std::vector<TD> &data, const s
 // solve drivers
 // ---------------------------------------------------------------------------
 
-struct Graph {
-    hipGraph_t g = nullptr;
-    hipGraphExec_t x = nullptr;
-    ~Graph() {
-        if (x) (void)hipGraphExecDestroy(x);
-        if (g) (void)hipGraphDestroy(g);
-    }
-};
-
 // Run `body` (which only enqueues kernels on c->stream) either directly or via
 // a graph captured on its first use.
 template <class F>
@@ -1546,8 +1555,53 @@ static void run_maybe_graph(srmg_ctx *c, Graph &gr, bool use_graph, F &&body) {
         }
         HIPCHK(hipStreamEndCapture(c->stream, &gr.g));
         HIPCHK(hipGraphInstantiate(&gr.x, gr.g, nullptr, nullptr, 0));
+        gr.gen = c->graph_gen;
     }
     HIPCHK(hipGraphLaunch(gr.x, c->stream));
+}
+
+// The cached graph of a solve step: the key holds every value the step's
+// launches bake in. Field buffers keep their addresses until a regrid or a
+// field destroy, which clear the cache; coarse-solve matrices keep theirs until
+// they are rebuilt, which outdates every graph captured before (dropped at the
+// next solve's start). The per-tree parameter tables are read through
+// pointers, so their contents may change between solves. The returned graph
+// stays valid for the whole solve: nothing clears the cache while one runs.
+static Graph &solve_graph(srmg_ctx *c, int kind, std::initializer_list<const FieldBuf *> fs,
+                          const srmg_solve_opts *o) {
+    // drop graphs captured before the last coarse-solve rebuild (no solve is
+    // running here, so nothing refers to them)
+    for (auto it = c->graphs.begin(); it != c->graphs.end();) {
+        if (it->second->x && it->second->gen != c->graph_gen) it = c->graphs.erase(it);
+        else ++it;
+    }
+    std::string k;
+    auto add = [&](const auto &v) { k.append(reinterpret_cast<const char *>(&v), sizeof(v)); };
+    add(kind);
+    for (const FieldBuf *f : fs) {
+        add(f->data.p);
+        add(f->ib.p);
+    }
+    for (double v : c->ps) add(v);
+    for (const srmg_op &op : c->pops) {
+        add(op.geometry);
+        add(op.disc);
+        add(op.k2);
+        k.append(reinterpret_cast<const char *>(op.bc), 27);
+        add(op.coarse);
+    }
+    add(c->pomega);
+    add(o->ndown);
+    add(o->ntop);
+    add(o->nup);
+    add(o->omega);
+    add(o->inner_cycles);
+    for (int v : c->sweep_cfg) add(v);
+    add(c->group_tiles);
+    add(c->axis_gemm);
+    auto &slot = c->graphs[k];
+    if (!slot) slot = std::make_unique<Graph>();
+    return *slot;
 }
 
 // the residual norms queued last (one per tree)
@@ -1616,11 +1670,12 @@ static void solve_plain(srmg_ctx *c, int ui, int bi, int ri, int u1i, const srmg
             done = record(read_norms(c), r0, o->tol, 0, o->max_cycles, hist);
         }
         int k = 0;
-        Graph gr;
+        Graph &gr = solve_graph(c, 0, {&u, &b, &r, &u1}, o);
         while (k < o->max_cycles && !done) {
-            // the first cycle runs eagerly so lazily built state (coarse-solve
-            // matrices) exists before a graph is captured
-            run_maybe_graph(c, gr, o->use_graph && k > 0, [&] {
+            // a first cycle without a cached graph runs eagerly, so that lazily
+            // built state (coarse-solve matrices, ping-pong buffers) exists
+            // before a graph is captured
+            run_maybe_graph(c, gr, o->use_graph && (k > 0 || gr.x), [&] {
                 ops.vcycle(u, b, r, u1, s, op, 0, o->ndown, o->ntop, o->nup, o->omega);
                 ops.resnorm_partials(u, b, s, op);
             });
@@ -1704,9 +1759,9 @@ static void solve_mixed_dm(srmg_ctx *c, FieldBuf &u, FieldBuf &b, FieldBuf &e, F
         done = record(read_norms(c), r0, o->tol, 0, o->max_cycles, hist);
     }
     int k = 0;
-    Graph gr;
+    Graph &gr = solve_graph(c, 1, {&u, &b, &e, &be, &re, &ue}, o);
     while (k < o->max_cycles && !done) {
-        run_maybe_graph(c, gr, o->use_graph && k > 0, correct);
+        run_maybe_graph(c, gr, o->use_graph && (k > 0 || gr.x), correct);
         k++;
         done = record(read_norms(c), r0, o->tol, k, o->max_cycles, hist);
     }
@@ -1808,6 +1863,7 @@ int srmg_field_destroy(srmg_ctx *c, int id) {
     return guarded([&] {
         field(c, id);
         HIPCHK(hipStreamSynchronize(c->stream));
+        c->graphs.clear();
         c->fields[id].reset();
     });
 }

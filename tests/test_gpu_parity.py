@@ -241,6 +241,48 @@ def test_axis_transform_variants_give_identical_grid_solves(D, n):
     np.testing.assert_array_equal(outs[1], outs[0])
 
 
+def test_cached_solve_graphs_replay_exactly():
+    """A context keeps the captured step of each solve configuration; a later
+    solve with the same configuration replays it from its first cycle, and a
+    solve with another scale or shape captures its own."""
+    p = configs.make_cfg4(0, scale=0.3, levels=3)
+    geom, disc = solver.CartesianGeometry(), solver.LaplacianDiscretization(3, 2)
+
+    def run(ctx, fs, s, mixed):
+        u, b = fs[0], fs[1]
+        b.upload(p.b)
+        if mixed:
+            n, h = solver.solve_mixed(u, b, *fs[2:], s, geom, p.bc, disc, tol=1e-9, max_cycles=12, inner_cycles=2,
+                                      guess=solver.GUESS_ZERO)
+        else:
+            n, h = solver.solve(u, b, *fs[2:], s, geom, p.bc, disc, tol=1e-9, max_cycles=12,
+                                guess=solver.GUESS_ZERO)
+        host = p.b.like()
+        u.download(host)
+        return np.asarray(h).copy(), _valid(host, p).copy()
+
+    for mixed in (False, True):
+        ctx = solver.Context(p.tree, p.conn, p.M)
+        fs = [ctx.field(F64), ctx.field(F64)] + [ctx.field(F32 if mixed else F64) for _ in range(4 if mixed else 2)]
+        first = run(ctx, fs, p.s, mixed)
+        again = run(ctx, fs, p.s, mixed)
+        np.testing.assert_array_equal(again[0], first[0])
+        np.testing.assert_array_equal(again[1], first[1])
+        other = run(ctx, fs, 2.0 * p.s, mixed)
+        ctx2 = solver.Context(p.tree, p.conn, p.M)
+        fs2 = [ctx2.field(F64), ctx2.field(F64)] + [ctx2.field(F32 if mixed else F64)
+                                                    for _ in range(4 if mixed else 2)]
+        fresh = run(ctx2, fs2, 2.0 * p.s, mixed)
+        np.testing.assert_array_equal(other[0], fresh[0])
+        np.testing.assert_array_equal(other[1], fresh[1])
+        assert not np.array_equal(other[1], first[1])
+        for l in range(1, len(p.tree) + 1):
+            ctx.set_sweep_config(F32 if mixed else F64, l, 0)
+        shape0 = run(ctx, fs, p.s, mixed)
+        np.testing.assert_array_equal(shape0[0], first[0])
+        np.testing.assert_array_equal(shape0[1], first[1])
+
+
 @pytest.mark.parametrize("use_graph", [False, True])
 def test_solve_graph_replay_matches_eager(use_graph):
     p = configs.make_cfg1(1)
