@@ -359,6 +359,12 @@ class DeviceField:
         _check(lib().srmg_zero(self.ctx.h, self.id))
         return self
 
+    def close(self):
+        """Release the device buffers (the id may be reused by a later field)."""
+        if self.id is not None and getattr(self.ctx, "h", None):
+            _check(lib().srmg_field_destroy(self.ctx.h, self.id))
+        self.id = None
+
 
 # ---- the reference's entry points ----
 

@@ -283,6 +283,39 @@ def test_cached_solve_graphs_replay_exactly():
         np.testing.assert_array_equal(shape0[1], first[1])
 
 
+def test_cached_solve_graphs_after_regrid_and_operator_changes():
+    """Captured steps never outlive what they point to: after a regrid, or a
+    solve with other boundary conditions (which rebuilds the level-1 matrices),
+    every solve equals the same solve in a fresh context."""
+    pa = CASES["cart3-m8"]()
+    pb = configs.make_cfg4(1, scale=0.3, levels=3)
+    geom, disc = solver.CartesianGeometry(), solver.LaplacianDiscretization(3, 2)
+    bc2 = configs.boundaryconditions(((-1, -1), (1, 1), (-1, 1)))
+
+    def solve(ctx, p, bc):
+        u, b, r, u1 = (ctx.field(F64) for _ in range(4))
+        b.upload(p.b)
+        n, h = solver.solve(u, b, r, u1, p.s, geom, bc, disc, tol=1e-9, max_cycles=6, guess=solver.GUESS_ZERO)
+        host = p.b.like()
+        u.download(host)
+        for f in (u, b, r, u1):
+            f.close()
+        return np.asarray(h).copy(), _valid(host, p).copy()
+
+    def fresh(p, bc):
+        return solve(solver.Context(p.tree, p.conn, p.M), p, bc)
+
+    ctx = solver.Context(pa.tree, pa.conn, pa.M)
+    for step, (p, bc) in enumerate([(pa, pa.bc), (pa, pa.bc), (pa, bc2), (pa, pa.bc), (pb, pb.bc), (pa, pa.bc)]):
+        if ctx.tree is not p.tree:
+            ctx.regrid(p.tree, p.conn)
+        got = solve(ctx, p, bc)
+        ref = fresh(p, bc)
+        print(f"step {step}: cycles {len(got[0]) - 1}", flush=True)
+        np.testing.assert_array_equal(got[0], ref[0])
+        np.testing.assert_array_equal(got[1], ref[1])
+
+
 @pytest.mark.parametrize("use_graph", [False, True])
 def test_solve_graph_replay_matches_eager(use_graph):
     p = configs.make_cfg1(1)
