@@ -2481,6 +2481,14 @@ int srmg_freebc_apply(srmg_ctx *c, srmg_freebc *fb, int qi, int ui, double s) {
         if (q.dtype != SRMG_F64 || u.dtype != SRMG_F64) fail("q and u must be FP64");
         const int M = c->M, D = 2, dim = fb->dim, od = fb->od, ND = 2 * D;
         const int64_t n = fb->n;
+        if (n != (fb->root[od] << (fb->level - 1)) * M) fail("the free boundary was made for another block size");
+        int finest = 0;
+        for (int l = 0; l < c->nlevels; l++)
+            if (c->lv[l].nblk) finest = l + 1;
+        // src/freebc.jl:94-95
+        if (finest > fb->level) fail("free boundary at a level below the finest level of the tree");
+        if (c->lv[0].ext[dim] != fb->root[dim] || c->lv[0].ext[od] != fb->root[od])
+            fail("the free boundary was made for another root domain");
         dispatch(c, SRMG_F64, [&](auto &o) {
             o.settle(u);
             o.settle(q);

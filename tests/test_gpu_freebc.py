@@ -155,3 +155,19 @@ def test_poisson_step_with_free_boundary_matches_oracle(poisson_fmg):
             else:
                 o.vcycle(uo, qo, ro, u1o, p.s, ndown=2, ntop=4, nup=2)
     np.testing.assert_array_equal(_valid(got, p), _valid(uo, p))
+
+
+def test_freebc_refuses_a_mismatched_setup():
+    p, root = _outer_refined()
+    ctx = solver.Context(p.tree, p.conn, p.M)
+    u, q = ctx.field(F64), ctx.field(F64)
+    u.upload(p.b)
+    q.upload(p.b)
+    geom = solver.CylindricalGeometry(1)
+    # the tree is finer than the level of the boundary (src/freebc.jl:94-95)
+    with pytest.raises(solver.SrmgError, match="finest level"):
+        FreeBC(ctx, geom, len(p.tree) - 1, root, p.h).freebc(q, u, p.s)
+    with pytest.raises(solver.SrmgError, match="root domain"):
+        FreeBC(ctx, geom, len(p.tree), (root[0] + 1, root[1]), p.h).freebc(q, u, p.s)
+    with pytest.raises(solver.SrmgError, match="2D cylindrical"):
+        FreeBC(ctx, solver.CartesianGeometry(), len(p.tree), root, p.h)
