@@ -291,10 +291,10 @@ def batched_cfg3(log, ntrees=64):
 
 def sweep_kernel_name(D, M, t, cfg, grouped):
     """Name of the sweep kernel a tile shape launches (as the profiler lists it)."""
-    if cfg == 7 and grouped and (D, M) == (3, 8):
-        return f"k_gs_sweep_col<{D}, {M}, {t}, -1>"
+    if cfg in (7, 8) and grouped and (D, M) == (3, 8):
+        return f"k_gs_sweep_col{'z' if cfg == 8 else ''}<{D}, {M}, {t}, -1>"
     if cfg >= 4 and grouped and ((D, M) in ((3, 8), (2, 16))):
-        nt = {4: 256, 5: 512, 6: 1024, 7: 256}[cfg]
+        nt = {4: 256, 5: 512, 6: 1024, 7: 256, 8: 512}[cfg]
         if ((1 << D) * M ** D // 2) % nt == 0:
             return f"k_gs_sweep_group<{D}, {M}, {t}, {nt}, -1>"
     small = (M ** D // 2) <= 256

@@ -301,13 +301,15 @@ int srmg_time_kernel(srmg_ctx *ctx, int kind, int level, int u, int b, int r, do
  * non-NULL. u is restored. Shapes: 0..3 blocks per work group x work-group
  * size; 4, 5, 6 (3D M=8 and 2D M=16) one sibling group of 2^D blocks as one
  * tile on 256 / 512 / 1024 threads; 7 (3D M=8) one sibling group on 256
- * threads, one thread per column of the group (levels without complete
- * sibling groups, and thread counts that do not divide a group's cells per
- * colour, run 0; 7 runs 4 where the column form does not apply). */
+ * threads, one thread per column of the group; 8 (3D M=8) the same on 512
+ * threads, two per column (levels without complete sibling groups, and thread
+ * counts that do not divide a group's cells per colour, run 0; 7 and 8 run 4
+ * and 5 where the column forms do not apply). */
 int srmg_autotune(srmg_ctx *ctx, int u, int b, double s, const srmg_op *op, int reps, int *chosen);
-/* Force the tile shape (0..7) of the fused sweep for one element type and level (1-based). */
+/* Force the tile shape (0..8) of the fused sweep for one element type and level (1-based).
+ * Until autotuned or forced, 3D M=8 contexts run shape 8 and the others shape 0. */
 int srmg_set_sweep_config(srmg_ctx *ctx, int dtype, int level, int cfg);
-/* 1 when the level's blocks form complete sibling groups (tile shapes 4..7 apply). */
+/* 1 when the level's blocks form complete sibling groups (tile shapes 4..8 apply). */
 int srmg_level_grouped(srmg_ctx *ctx, int level, int *grouped);
 /* Use (default) or bypass the sibling-group forms of the sweep, residual and
  * prolongation kernels on grouped levels; both give the same bits. */

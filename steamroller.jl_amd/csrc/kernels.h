@@ -1104,6 +1104,167 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
     else CS::template run<0>(sm, uo, ui, b, ib, coord, a, blk0, w, lane, tv, bpl);
 }
 
+// k_gs_sweep_col with each column split between two threads (512 threads per
+// sibling group): wave w covers the columns of wave w & 3 of k_gs_sweep_col in
+// the block layer w >> 2 (8 cells each), so a group's work is spread over
+// twice as many waves; the Z neighbours across the layers come from the tile.
+// For the small levels, whose few groups leave most of the chip idle. The
+// same operations on the same values as k_gs_sweep_col (same bits).
+template <int D, int M, typename T, int RD>
+struct ColSweepZ {
+    using CS = ColSweep<D, M, T, RD>;
+    using LY = Lay<D, M>;
+    using TL = typename CS::TL;
+    using HT = Halo<D>;
+    static constexpr int NB = 8, V = LY::V, H = LY::H;
+    static constexpr int SY = TL::stride(1), SZ = TL::stride(2);
+    using Smem = typename CS::Smem;
+
+    template <int DIM>
+    __device__ static __forceinline__ void rec_pre(Smem &sm, int q, const T *__restrict__ ui, const T *__restrict__ b,
+                                                   const T *__restrict__ ib, int &re, T &rd, T &rb, T *rg) {
+        constexpr int DA = DIM == 0 ? 1 : 0, DB = DIM == 2 ? 1 : 2;
+        const int rside = q >> 7, rbf = (q >> 5) & 3, rtq = q & 31;
+        const int k = (rside << DIM) | ((rbf & 1) << DA) | ((rbf >> 1) << DB), f = 2 * DIM + rside;
+        const int *tb = sm.tab + k * HT::N;
+        re = tb[f];
+        rd = rb = T(0);
+        if (re >= 0)
+            RecFace<D, M, T, DIM, TL>::prefetch(rside ? 1 : -1, rtq, re, tb + HT::ND + f * HT::NS, ui, b, ib, rd, rb,
+                                                rg);
+    }
+    template <int DIM>
+    __device__ static __forceinline__ void rec_cmp(Smem &sm, int q, const int *__restrict__ coord,
+                                                   const LevelArgs<T> &a, int re, T rd, T rb, const T *rg) {
+        constexpr int DA = DIM == 0 ? 1 : 0, DB = DIM == 2 ? 1 : 2;
+        const int rside = q >> 7, rbf = (q >> 5) & 3, rtq = q & 31;
+        const int k = (rside << DIM) | ((rbf & 1) << DA) | ((rbf >> 1) << DB), f = 2 * DIM + rside;
+        if (re >= 0)
+            RecFace<D, M, T, DIM, TL>::template compute<RD>(rside ? 1 : -1, rtq, re,
+                                                            sm.tab + k * HT::N + HT::ND + f * HT::NS,
+                                                            sm.tile + TL::origin(k), coord, a, sm.prm[k], rd, rb, rg);
+    }
+
+    template <int P>
+    __device__ static __forceinline__ void run(Smem &sm, T *__restrict__ uo, const T *__restrict__ ui,
+                                               const T *__restrict__ b, const T *__restrict__ ib,
+                                               const int *__restrict__ coord, const LevelArgs<T> &a, int blk0,
+                                               int w, int lane, int tv, const BP<T> &bpl) {
+        const int tid = w * 64 + lane;
+        const int wc = w & 3, wz = w >> 2;
+        const int Y = 8 * (wc >> 1) + (lane >> 3), X = 2 * (lane & 7) + ((Y + P) & 1);
+        const int x = X & 7, y = Y & 7, kb = ((X >> 3) | ((Y >> 3) << 1)) | (wz << 2);
+        const size_t g0 = (size_t)(blk0 + kb) * V + (size_t)((x >> 1) + LY::MH * y);
+        T uc[M], bc[M];
+        static_for<M>([&](auto zc) {
+            constexpr int z = decltype(zc)::value;
+            uc[z] = ui[g0 + CS::template zoff<P, z>()];
+            bc[z] = b[g0 + CS::template zoff<P, z>()];
+        });
+        if (tid < NB * HT::N) sm.tab[tid] = tv;
+        if (tid < NB) sm.prm[tid] = bpl;
+        lds_barrier();
+
+        // halo cells of the six group faces: wave w takes the (face, block
+        // face) pairs 3 w .. 3 w + 2 of the 24 (face uniform per wave), lane =
+        // cell t of the face; a boundary face's mirror factor is applied at
+        // the tile write
+        T hv[3], hm[3];
+        int hF[3], hk[3];
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const int pr = 3 * w + j, F = pr >> 2, bf = pr & 3;
+            const int DIM = F >> 1, DA = DIM == 0 ? 1 : 0, DB = DIM == 2 ? 1 : 2;
+            const int k = ((F & 1) << DIM) | ((bf & 1) << DA) | ((bf >> 1) << DB);
+            const int e = __builtin_amdgcn_readfirstlane(sm.tab[k * HT::N + F]);
+            hF[j] = F;
+            hk[j] = k;
+            T v = T(0), m = T(1);
+            SRMG_FACE_SWITCH(F, (v = halo_value<D, M, T, FF>(e, blk0 + k, lane, ui, ib, T(1)),
+                                 m = e == NB_BOUNDARY ? a.sg[FF] : T(1)));
+            hv[j] = v;
+            hm[j] = m;
+        }
+        // recompute operands: waves 0-3 dimensions 0 and 2, waves 4-7 dimension 1
+        int re[2];
+        T rd[2], rb[2], rg[2][4];
+        if (w < 4) {
+            rec_pre<0>(sm, tid, ui, b, ib, re[0], rd[0], rb[0], rg[0]);
+            rec_pre<2>(sm, tid, ui, b, ib, re[1], rd[1], rb[1], rg[1]);
+        } else {
+            rec_pre<1>(sm, tid - 256, ui, b, ib, re[0], rd[0], rb[0], rg[0]);
+        }
+        // the tile
+        const int pc = TL::at(X, Y, 8 * wz);
+        static_for<M>([&](auto zc) {
+            constexpr int z = decltype(zc)::value;
+            sm.tile[pc + z * SZ] = uc[z];
+        });
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            sm.tile[TL::origin(hk[j]) + halo_at_sw<D, M, TL>(hF[j], lane)] = hm[j] * hv[j];
+        lds_barrier();
+
+        const BP<T> bp = sm.prm[kb];
+        T *const t = sm.tile;
+        auto update = [&](auto zc) {
+            constexpr int z = decltype(zc)::value;
+            const int p = pc + z * SZ;
+            const T zm = (z == 0) ? t[p - SZ] : uc[z > 0 ? z - 1 : 0];
+            const T zp = (z == M - 1) ? t[p + SZ] : uc[z < M - 1 ? z + 1 : 0];
+            T v = T(0);
+            v += zm;
+            v += t[p - SY];
+            v += t[p - 1];
+            v += bp.cc * uc[z];
+            v += t[p + 1];
+            v += t[p + SY];
+            v += zp;
+            uc[z] = fma_t<T>(bp.nwc, fma_t<T>(bp.s, bc[z], v), uc[z]);
+        };
+        // first half-sweep: red cells (z = P mod 2 in the layer, which starts at an even Z)
+        static_for<M / 2>([&](auto ic) {
+            constexpr int z = 2 * decltype(ic)::value + P;
+            update(std::integral_constant<int, z>());
+            t[pc + z * SZ] = uc[z];
+        });
+        if (w < 4) {
+            rec_cmp<0>(sm, tid, coord, a, re[0], rd[0], rb[0], rg[0]);
+            rec_cmp<2>(sm, tid, coord, a, re[1], rd[1], rb[1], rg[1]);
+        } else {
+            rec_cmp<1>(sm, tid - 256, coord, a, re[0], rd[0], rb[0], rg[0]);
+        }
+        lds_barrier();
+        static_for<M / 2>([&](auto ic) {
+            constexpr int z = 2 * decltype(ic)::value + 1 - P;
+            update(std::integral_constant<int, z>());
+        });
+        static_for<M>([&](auto zc) {
+            constexpr int z = decltype(zc)::value;
+            uo[g0 + CS::template zoff<P, z>()] = uc[z];
+        });
+    }
+};
+
+template <int D, int M, typename T, int RD>
+__global__ __launch_bounds__(512) void k_gs_sweep_colz(T *__restrict__ uo, const T *__restrict__ ui,
+                                                       const T *__restrict__ b, const T *__restrict__ ib,
+                                                       const int *__restrict__ htab, const int *__restrict__ coord,
+                                                       LevelArgs<T> a) {
+    using CZ = ColSweepZ<D, M, T, RD>;
+    using HT = Halo<D>;
+    __shared__ typename CZ::Smem sm;
+    const int blk0 = a.blk0 + xcd_remap(blockIdx.x, gridDim.x) * CZ::NB;
+    const int tid = threadIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    int tv = 0;
+    if (tid < CZ::NB * HT::N) tv = htab[(size_t)blk0 * HT::N + tid];
+    BP<T> bpl{};
+    if (tid < CZ::NB) bpl = bparams(a, blk0 + tid);
+    if (w & 1) CZ::template run<1>(sm, uo, ui, b, ib, coord, a, blk0, w, lane, tv, bpl);
+    else CZ::template run<0>(sm, uo, ui, b, ib, coord, a, blk0, w, lane, tv, bpl);
+}
+
 // ---- tiled residual kernels ----
 
 // Stage BPW blocks (valid cells + depth-1 face halo) of x in the LDS tile.

@@ -495,7 +495,9 @@ static void set_forest(srmg_ctx *c, const std::vector<TreeRecords> &tr) {
         G.cs.alloc(G.ntree * sizeof(double));
         G.shift.alloc(G.ntree * sizeof(double));
     }
-    c->sweep_cfg.assign(2 * nlevels, 0);
+    // until autotuned or set: the column sweep on two threads per column for
+    // 8^3 blocks (levels without complete sibling groups run shape 0)
+    c->sweep_cfg.assign(2 * nlevels, (c->D == 3 && c->M == 8) ? 8 : 0);
     c->zero[0].reset();
     c->zero[1].reset();
     c->ps.clear();
@@ -734,7 +736,7 @@ struct Ops {
     static constexpr int BPW0 = SMALL ? 256 / LY::H : 1;
     // sibling-group tiles (k_gs_sweep_group) where a group's tile fits in LDS
     static constexpr bool GROUP_OK = (D == 3 && M == 8) || (D == 2 && M == 16);
-    static constexpr int NCFG = GROUP_OK ? 8 : 4;
+    static constexpr int NCFG = GROUP_OK ? 9 : 4;
     bool grouped(int l) const { return c->group_tiles && c->lv[l].grouped; }
     template <int K> static constexpr int cfg_bpw() { return SMALL ? BPW0 * (K == 0 ? 1 : (K == 3 ? 4 : 2)) : 1; }
     template <int K> static constexpr int cfg_nt() {
@@ -750,16 +752,19 @@ struct Ops {
                                I(u), c->d_halo.as<int>(), coord(), a);
         });
     }
-    template <int NTG, bool COL = false>
+    template <int NTG, int COL = 0>
     void sweep_group_launch(T *out, T *in, FieldBuf &u, FieldBuf &b, const LevelArgs<T> &a, int l) {
         if constexpr (GROUP_OK && (((1 << D) * LY::H) % NTG == 0)) {
             unsigned g = (unsigned)(a.nblk >> D);
             with_rd(a.rdim, [&](auto rd) {
                 constexpr int RD = decltype(rd)::value;
-                // the column mapping exists for 8^3 Cartesian blocks; elsewhere
-                // shape 7 runs the group kernel on NTG threads
-                if constexpr (COL && D == 3 && M == 8 && RD < 0)
+                // the column mappings exist for 8^3 Cartesian blocks; elsewhere
+                // shapes 7 and 8 run the group kernel on NTG threads
+                if constexpr (COL == 1 && D == 3 && M == 8 && RD < 0)
                     hipLaunchKernelGGL((k_gs_sweep_col<D, M, T, RD>), dim3(g), dim3(256), 0, c->stream, out, in,
+                                       X(b, l), I(u), c->d_halo.as<int>(), coord(), a);
+                else if constexpr (COL == 2 && D == 3 && M == 8 && RD < 0)
+                    hipLaunchKernelGGL((k_gs_sweep_colz<D, M, T, RD>), dim3(g), dim3(512), 0, c->stream, out, in,
                                        X(b, l), I(u), c->d_halo.as<int>(), coord(), a);
                 else
                     hipLaunchKernelGGL((k_gs_sweep_group<D, M, T, NTG, RD>), dim3(g), dim3(NTG), 0, c->stream, out,
@@ -770,8 +775,8 @@ struct Ops {
         }
     }
     // tile shapes 0..3: k_gs_sweep (cfg_bpw / cfg_nt); 4, 5, 6: sibling groups
-    // on 256 / 512 / 1024 threads; 7: sibling groups, one thread per column
-    // (levels that are not grouped, and group sizes the thread count does not
+    // on 256 / 512 / 1024 threads; 7: sibling groups, one thread per column;
+    // 8: sibling groups, two threads per column (levels that are not grouped, and group sizes the thread count does not
     // divide, use shape 0)
     void sweep(FieldBuf &u, FieldBuf &b, const LevelArgs<T> &a, int l, int cfg = -1) {
         if (!a.nblk) return;
@@ -787,7 +792,8 @@ struct Ops {
             case 4: sweep_group_launch<256>(out, in, u, b, a, l); break;
             case 5: sweep_group_launch<512>(out, in, u, b, a, l); break;
             case 6: sweep_group_launch<1024>(out, in, u, b, a, l); break;
-            case 7: sweep_group_launch<256, true>(out, in, u, b, a, l); break;
+            case 7: sweep_group_launch<256, 1>(out, in, u, b, a, l); break;
+            case 8: sweep_group_launch<512, 2>(out, in, u, b, a, l); break;
             default: sweep_launch<0>(out, in, u, b, a, l); break;
         }
         u.cur[l] ^= 1;
@@ -2203,7 +2209,7 @@ int srmg_set_sweep_config(srmg_ctx *c, int dtype, int level, int cfg) {
         check_tree(c);
         if (level < 1 || level > c->nlevels) fail("level out of range");
         if (dtype != SRMG_F64 && dtype != SRMG_F32) fail("bad dtype");
-        if (cfg < 0 || cfg > 7) fail("sweep configuration must be 0..7");
+        if (cfg < 0 || cfg > 8) fail("sweep configuration must be 0..8");
         c->sweep_cfg[(dtype == SRMG_F64 ? 0 : c->nlevels) + level - 1] = cfg;
     });
 }

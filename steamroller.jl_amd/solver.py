@@ -33,7 +33,7 @@ LIB_PATH = os.path.join(_HERE, "libsrmg.so")
 
 F64, F32 = 0, 1
 KERNEL_GS_PASS, KERNEL_RESIDUAL, KERNEL_GS_SWEEP, KERNEL_STREAM, KERNEL_RESIDUAL_GATHER = 0, 1, 2, 3, 4
-SWEEP_SHAPES = 8  # tile shapes of the fused smoother (Context.set_sweep_config)
+SWEEP_SHAPES = 9  # tile shapes of the fused smoother (Context.set_sweep_config)
 COARSE_GS, COARSE_EXACT = 0, 1
 GUESS_NONE, GUESS_LEARNED, GUESS_COARSE, GUESS_ZERO = 0, 1, 2, 3
 _DT = {F64: np.float64, F32: np.float32}

@@ -173,10 +173,11 @@ def _problems_3d():
     return out
 
 
+@pytest.mark.parametrize("shape", [7, 8])
 @pytest.mark.parametrize("mixed", [False, True])
-def test_forest_3d_column_kernels_equal_single_solves(mixed):
-    # the column-mapped sweep (shape 7), residual/restriction, defect and norm
-    # kernels on forests: every tree gets its own bits
+def test_forest_3d_column_kernels_equal_single_solves(mixed, shape):
+    # the column-mapped sweeps (shapes 7 and 8), residual/restriction, defect
+    # and norm kernels on forests: every tree gets its own bits
     probs = _problems_3d()
     geom = solver.CartesianGeometry()
     ctx = solver.Context.forest([(p.tree, p.conn) for p in probs], 8)
@@ -185,8 +186,8 @@ def test_forest_3d_column_kernels_equal_single_solves(mixed):
     print(f"grouped levels {grouped}", flush=True)
     assert all(grouped[1:]), "the column kernels run on grouped levels only"
     for l in range(1, nl + 1):
-        ctx.set_sweep_config(F64, l, 7)
-        ctx.set_sweep_config(F32, l, 7)
+        ctx.set_sweep_config(F64, l, shape)
+        ctx.set_sweep_config(F32, l, shape)
     u, b, r, u1 = (ctx.field(F64) for _ in range(4))
     b.upload(solver.concat_fields([p.b for p in probs]))
     discs = [_disc(p) for p in probs]
@@ -204,8 +205,8 @@ def test_forest_3d_column_kernels_equal_single_solves(mixed):
     for k, p in enumerate(probs):
         c1 = solver.Context(p.tree, p.conn, p.M)
         for l in range(1, len(p.tree) + 1):
-            c1.set_sweep_config(F64, l, 7)
-            c1.set_sweep_config(F32, l, 7)
+            c1.set_sweep_config(F64, l, shape)
+            c1.set_sweep_config(F32, l, shape)
         v, bb, rr, vv = (c1.field(F64) for _ in range(4))
         bb.upload(p.b)
         if mixed:
