@@ -44,6 +44,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched photoionization measurement")
+    ap.add_argument("--no-regrid", action="store_true", help="skip the regrid measurement")
     return ap.parse_args()
 
 
@@ -163,6 +164,8 @@ def main():
         log(f"[bench] cpu: {cpu}")
 
     batched = None if args.no_batched else batched_cfg3(log)
+    # last: it changes the context's tree
+    regrid = None if args.no_regrid else regrid_step(log, p, ctx, u, b, e)
     traffic = committed_traffic(kname["float"], args.config)
     ms = forms[best]["ms"]
 
@@ -209,6 +212,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "batched_photoionization": batched,
+        "regrid": regrid,
     }
     print(json.dumps(out), flush=True)
 
@@ -286,6 +290,62 @@ def batched_cfg3(log, ntrees=64):
         log(f"[bench] batched {key}: {res[key]}")
     res["workload"] = (f"cfg3: 3 photoionization Helmholtz terms on the cfg1 tree + {ntrees} random trees, "
                        f"{len(probs)} problems, {leaf} leaf cells, to 1e-8 each")
+    return res
+
+
+def regrid_step(log, p, ctx, u, b, e):
+    """One refine! step of the workload's tree with the fields on the device:
+    every eighth sibling group of the finest level removed, every eighth
+    level-(L-1) leaf whose star neighbours all exist refined; the host edits the
+    tree (srmg.tree.applydelta, connectivity), srmg_regrid rebuilds the tables
+    and carries u and b (FP64) and e (FP32) across (interp! of new blocks,
+    restrict! of removed ones). The table build alone is timed on a second
+    context."""
+    import copy
+    from srmg import solver
+    from srmg.solver import REGRID_INTERP, REGRID_RESTRICT
+    from srmg.tree import REFINE, REMOVE, applydelta, connectivity, parentcoord, star_stencil, subblocks
+    tree, D = p.tree, p.D
+    L = len(tree)
+    t0 = time.time()
+    delta = {}
+    fin, lay = tree[L - 1], tree[L - 2]
+    for k, pc in enumerate(sorted({parentcoord(c) for c, _ in fin.pairs})):
+        if k % 8 == 0:
+            for ch in subblocks(pc):
+                delta[fin.get(ch)] = REMOVE
+    cand = []
+    for c, blk in lay.pairs:
+        if not tree.isleaf(L - 1, c):
+            continue
+        nb = [tuple(x + y for x, y in zip(c, s)) for s in star_stencil(D)]
+        if all(lay.hasblock(q) or not lay.in_domain(q) for q in nb):
+            cand.append(blk)
+    for blk in cand[::8]:
+        delta[blk] = REFINE
+    t2 = copy.deepcopy(tree)
+    nb2, actions = applydelta(t2, delta, [], tree.nblocks())
+    c2 = connectivity(t2)
+    host_s = time.time() - t0
+    nnew = sum(1 for a in actions if a[0] == "interp")
+    nrem = sum(1 for a in actions if a[0] == "restrict")
+    both = REGRID_INTERP | REGRID_RESTRICT
+    ctx.synchronize()
+    ctx.regrid(t2, c2, fields=[(u, both), (b, both), (e, both)], bc=p.bc, nblocks=nb2)
+    regrid_ms = ctx.last_regrid_ms
+    other = solver.Context(tree, p.conn, p.M)
+    other.synchronize()
+    other.regrid(t2, c2, nblocks=nb2)
+    tables_ms = other.last_regrid_ms
+    del other
+    res = {"blocks_before": tree.nblocks(), "blocks_after": t2.nblocks(), "new_blocks": nnew,
+           "removed_blocks": nrem, "fields": "u, b (FP64), e (FP32): interp! + restrict!",
+           "regrid_ms": round(regrid_ms, 2), "table_build_ms": round(tables_ms, 2),
+           "host_tree_edit_and_connectivity_s": round(host_s, 2),
+           "note": "regrid_ms: the srmg_regrid call (host table build + device transfer of the three fields); "
+                   "table_build_ms: the srmg_set_tree call for the same tree on a context without fields; the "
+                   "tree edit and connectivity run in Python here (outside both figures)"}
+    log(f"[bench] regrid: {res}")
     return res
 
 

@@ -96,6 +96,30 @@ int srmg_set_forest(srmg_ctx *ctx, int ntrees, const int *nlevels, const int64_t
 /* Number of trees of the context. */
 int srmg_tree_count(srmg_ctx *ctx, int *ntrees);
 
+/* Regrid that carries fields across on the device: the field part of
+ * applydelta!(tree, refdelta, freeblocks, fields, minlevel, maxlevel)
+ * (src/refinement.jl:339-370, with interp! and restrict! of
+ * src/streamer.jl:191-221) around srmg_set_tree. The records describe the tree
+ * after the host's tree edit and connectivity! (as for srmg_set_tree); blocks
+ * are matched to the old tree by (level, coordinate), so block ids may be
+ * reused. For each listed field: blocks that keep their (level, coordinate)
+ * keep their values; with SRMG_REGRID_INTERP a new block is interp!'d from its
+ * parent's old values and depth-1 ghosts (filled first as fill_ghost! does,
+ * with bc), otherwise it starts at zero; with SRMG_REGRID_RESTRICT a removed
+ * block is restrict!'ed into its parent's subblock. A new block's parent must
+ * be in the old tree and a removed block's parent must be kept (one level per
+ * regrid, as applydelta! does); this is checked before anything changes.
+ * Fields not listed are reallocated as in srmg_set_tree. Single-tree
+ * contexts. bc may be NULL when no field is interpolated. */
+enum { SRMG_REGRID_INTERP = 1, SRMG_REGRID_RESTRICT = 2 };
+int srmg_regrid(srmg_ctx *ctx, int nlevels, int64_t nblocks,
+                const int64_t *pairs, const int64_t *pairs_off,
+                const int64_t *neighbor, const int64_t *neighbor_off,
+                const int64_t *boundary, const int64_t *boundary_off,
+                const int64_t *refboundary, const int64_t *refboundary_off,
+                const int64_t *child, const int64_t *child_off,
+                const int *fields, const int *modes, int nfields, const int8_t *bc);
+
 /* Device-resident block field of the given element type; returns its id. */
 int srmg_field_create(srmg_ctx *ctx, int dtype, int *id);
 int srmg_field_destroy(srmg_ctx *ctx, int id);

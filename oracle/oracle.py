@@ -81,6 +81,58 @@ def dst_matrix(n: int, kind: int) -> np.ndarray:
     return S
 
 
+# ---- applydelta!'s field part (numpy restatement) ----------------------------
+
+def interp_block(dest: np.ndarray, src: np.ndarray, sub, M: int, G: int):
+    """interp!(dest, validindices, src, subblockindices(sub)) with InterpLinear
+    and func = _first (src/interp.jl:7-11, 23-60, 88-91): the valid cells of
+    block array ``dest`` ((M+2G)^D) from the subblock ``sub`` (1-based) of
+    block array ``src``, whose depth-1 ghosts must be filled. Each fine cell is
+    w0 src[Js] + sum over dimensions j (in order) of w1 src[Js +- e_j], the
+    sign being that of the fine cell's offset."""
+    D = src.ndim
+    h = M // 2
+    w0, w1 = (0.5, 0.25) if D == 2 else (0.25, 0.25)
+    for par in np.ndindex(*(2,) * D):
+        js = tuple(slice(G + (sub[d] - 1) * h, G + (sub[d] - 1) * h + h) for d in range(D))
+        f = w0 * src[js]
+        for j in range(D):
+            sh = 2 * par[j] - 1
+            jj = tuple(slice(js[d].start + (sh if d == j else 0), js[d].stop + (sh if d == j else 0))
+                       for d in range(D))
+            f = f + w1 * src[jj]
+        dest[tuple(slice(G + par[d], G + M, 2) for d in range(D))] = f
+
+
+def restrict_block(dest: np.ndarray, sub, src: np.ndarray, M: int, G: int):
+    """restrict!(dest, subblockindices(sub), src, validindices) (src/restrict.jl:5-26):
+    each cell of the subblock ``sub`` of ``dest`` = the sum of its 2^D children
+    of ``src`` (first index fastest, from zero) divided by 2^D."""
+    D = src.ndim
+    h = M // 2
+    f = np.zeros((h,) * D, dtype=src.dtype)
+    for par in [tuple(reversed(t)) for t in np.ndindex(*(2,) * D)]:
+        f = f + src[tuple(slice(G + par[d], G + M, 2) for d in range(D))]
+    dest[tuple(slice(G + (sub[d] - 1) * h, G + (sub[d] - 1) * h + h) for d in range(D))] = f / src.dtype.type(1 << D)
+
+
+def applydelta_fields(u: np.ndarray, actions, nblocks: int, M: int, G: int, interp: bool = True,
+                      restrict: bool = True) -> np.ndarray:
+    """The field part of applydelta! (src/refinement.jl:339-370; interp! and
+    restrict! of src/streamer.jl:191-221) on a host field ``u`` whose ghosts
+    were filled on the old tree: a new array of ``nblocks`` blocks (ids are
+    stable; new ids start at zero), then the actions of
+    ``srmg.tree.applydelta`` in its order."""
+    out = np.zeros(u.shape[:-1] + (nblocks,), dtype=u.dtype, order="F")
+    out[..., :u.shape[-1]] = u
+    for kind, a, b, sub in actions:
+        if kind == "restrict" and restrict:
+            restrict_block(out[..., a - 1], sub, out[..., b - 1], M, G)
+        elif kind == "interp" and interp:
+            interp_block(out[..., a - 1], out[..., b - 1], sub, M, G)
+    return out
+
+
 class _Op(C.Structure):
     _fields_ = [("geom", C.c_int), ("helm", C.c_int), ("k2", C.c_double),
                 ("bc", C.c_int8 * 27)]

@@ -291,6 +291,75 @@ __global__ __launch_bounds__(WG) void k_interp_add(T *__restrict__ u, const T *_
     u[o] = f + u[o];
 }
 
+// applydelta!'s field transfer (src/refinement.jl:339-370; interp! and
+// restrict! of src/streamer.jl:191-221) from the old block layout (uo, its
+// coarse-fine slabs ibo and neighbour table nbro) into the new one (un).
+// Blocks that keep their (level, coordinate): moved to their new position;
+// cp[k] = (old block, new block).
+template <int D, int M, typename T>
+__global__ __launch_bounds__(WG) void k_regrid_copy(T *__restrict__ un, const T *__restrict__ uo,
+                                                    const int2 *__restrict__ cp, int n) {
+    using LY = Lay<D, M>;
+    const size_t gt = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = (int)(gt / LY::V);
+    if (k >= n) return;
+    const int q = (int)(gt - (size_t)k * LY::V);
+    const int2 p = cp[k];
+    un[(size_t)p.y * LY::V + q] = uo[(size_t)p.x * LY::V + q];
+}
+
+// New blocks: interp! (src/interp.jl:23-60, func = _first) from the parent's
+// old values and its depth-1 ghosts, in k_interp_add's order of terms;
+// ip[k] = (new block, old parent, subblock code).
+template <int D, int M, typename T>
+__global__ __launch_bounds__(WG) void k_regrid_interp(T *__restrict__ un, const T *__restrict__ uo,
+                                                      const T *__restrict__ ibo, const int *__restrict__ nbro,
+                                                      const int4 *__restrict__ ip, int n, LevelArgs<T> a) {
+    using LY = Lay<D, M>;
+    const size_t gt = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = (int)(gt / LY::V);
+    if (k >= n) return;
+    const int q = (int)(gt - (size_t)k * LY::V);
+    int x0, x1, x2;
+    LY::decode(q, x0, x1, x2);
+    const int4 it = ip[k];
+    const int pb = it.y, sb = it.z;
+    const int J0 = (sb & 1) * LY::MH + (x0 >> 1);
+    const int J1 = ((sb >> 1) & 1) * LY::MH + (x1 >> 1);
+    const int J2 = (D == 3) ? ((sb >> 2) & 1) * LY::MH + (x2 >> 1) : 0;
+    T f = w0<D, T>() * uo[(size_t)pb * LY::V + LY::sidx(J0, J1, J2)];
+    f += w1<T>() * nbv<D, M, T>(uo, ibo, nbro, pb, J0, J1, J2, 0, 2 * (x0 & 1) - 1, a.sg);
+    f += w1<T>() * nbv<D, M, T>(uo, ibo, nbro, pb, J0, J1, J2, 1, 2 * (x1 & 1) - 1, a.sg);
+    if (D == 3) f += w1<T>() * nbv<D, M, T>(uo, ibo, nbro, pb, J0, J1, J2, 2, 2 * (x2 & 1) - 1, a.sg);
+    un[(size_t)it.x * LY::V + q] = f;
+}
+
+// Removed blocks: restrict! (src/restrict.jl:5-26) into the parent's
+// subblock, in k_restrict's order; rp[k] = (old block, new parent, subblock
+// code).
+template <int D, int M, typename T>
+__global__ __launch_bounds__(WG) void k_regrid_restrict(T *__restrict__ un, const T *__restrict__ uo,
+                                                        const int4 *__restrict__ rp, int n) {
+    using LY = Lay<D, M>;
+    constexpr int NS = LY::V / LY::NC;
+    const size_t gt = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = (int)(gt / NS);
+    if (k >= n) return;
+    const int j = (int)(gt - (size_t)k * NS);
+    const int j0 = j % LY::MH, j1 = (j / LY::MH) % LY::MH, j2 = (D == 3) ? j / (LY::MH * LY::MH) : 0;
+    const int4 it = rp[k];
+    T f = T(0);
+#pragma unroll
+    for (int c = 0; c < LY::NC; c++) {
+        const int i0 = c & 1, i1 = (c >> 1) & 1, i2 = (c >> 2) & 1;
+        f += uo[(size_t)it.x * LY::V + LY::sidx(2 * j0 + i0, 2 * j1 + i1, D == 3 ? 2 * j2 + i2 : 0)];
+    }
+    const int sb = it.z;
+    const int X0 = (sb & 1) * LY::MH + j0, X1 = ((sb >> 1) & 1) * LY::MH + j1;
+    const int X2 = (D == 3) ? ((sb >> 2) & 1) * LY::MH + j2 : 0;
+    un[(size_t)it.y * LY::V + LY::sidx(X0, X1, X2)] = f / T(LY::NC);
+}
+
 // fill_ghost_interp! (src/connectivity.jl:251-258, src/blockfield.jl:416-434)
 // for the depth-1 ghost layer: slot k holds the ghost slab of one coarse-fine
 // face. The coarse layer adjacent to the face feeds the fine ghost layer next

@@ -723,6 +723,25 @@ struct Ops {
         }
     }
 
+    // applydelta!'s transfer of one field from the old layout (od, oib, onbr)
+    // into the new one: kept blocks (cp), new blocks interpolated from their
+    // old parents (ip), removed blocks restricted into their parents (rp),
+    // restrictions after the copies
+    void regrid_transfer(FieldBuf &f, const DevBuf &od, const DevBuf &oib, const DevBuf &onbr, const DevBuf &cp,
+                         int ncp, const DevBuf &ip, int nip, const DevBuf &rp, int nrp, const LevelArgs<T> &a) {
+        T *un = f.data.as<T>();
+        const T *uo = od.as<T>();
+        if (ncp)
+            hipLaunchKernelGGL((k_regrid_copy<D, M, T>), dim3(nwg((int64_t)ncp * LY::V)), dim3(WG), 0, c->stream, un,
+                               uo, cp.as<int2>(), ncp);
+        if (nip)
+            hipLaunchKernelGGL((k_regrid_interp<D, M, T>), dim3(nwg((int64_t)nip * LY::V)), dim3(WG), 0, c->stream,
+                               un, uo, oib.as<T>(), onbr.as<int>(), ip.as<int4>(), nip, a);
+        if (nrp)
+            hipLaunchKernelGGL((k_regrid_restrict<D, M, T>), dim3(nwg((int64_t)nrp * (LY::V / LY::NC))), dim3(WG), 0,
+                               c->stream, un, uo, rp.as<int4>(), nrp);
+    }
+
     void gs_pass(FieldBuf &u, FieldBuf &b, const LevelArgs<T> &a, int l, int pass) {
         if (!a.nblk) return;
         hipLaunchKernelGGL((k_gs_pass<D, M, T>), dim3(nwg((int64_t)a.nblk * LY::H)), dim3(WG), 0, c->stream,
@@ -1850,6 +1869,162 @@ int srmg_set_forest(srmg_ctx *c, int ntrees, const int *nlevels, const int64_t *
                                      boundary[t], boundary_off[t], refboundary[t], refboundary_off[t], child[t],
                                      child_off[t]});
         set_forest(c, tr);
+    });
+}
+
+// applydelta!'s field transfer (src/refinement.jl:339-370 with interp! and
+// restrict! of src/streamer.jl:191-221) around set_forest: the blocks of the
+// new trees are matched to the old ones by (tree, level, coordinate), so the
+// caller's block ids may be reused or renumbered freely.
+static void regrid_fields(srmg_ctx *c, const std::vector<TreeRecords> &tr, const int *ids, const int *modes,
+                          int nf, const int8_t *bc) {
+    check_tree(c);
+    const int D = c->D;
+    if (nf < 0 || (nf > 0 && (!ids || !modes))) fail("bad field list");
+    if (tr.size() != c->trees.size()) fail("a regrid keeps the number of trees");
+    bool interp = false;
+    for (int k = 0; k < nf; k++) {
+        field(c, ids[k]);
+        if (modes[k] & ~(SRMG_REGRID_INTERP | SRMG_REGRID_RESTRICT)) fail("bad regrid mode");
+        for (int j = 0; j < k; j++)
+            if (ids[j] == ids[k]) fail("a field is listed twice");
+        if (modes[k] & SRMG_REGRID_INTERP) interp = true;
+    }
+    if (interp && !bc) fail("interpolating new blocks needs the boundary conditions");
+    srmg_op op{};
+    if (bc) std::memcpy(op.bc, bc, 27);
+    // the old values, with the coarse-fine ghosts of every level where new
+    // blocks are interpolated (fill_ghost! before applydelta!, src/streamer.jl:439-445)
+    for (int k = 0; k < nf; k++) {
+        FieldBuf &f = field(c, ids[k]);
+        dispatch(c, f.dtype, [&](auto &o) {
+            using T = std::remove_reference_t<decltype(*o.X(f, 0))>;
+            o.settle(f);
+            if (modes[k] & SRMG_REGRID_INTERP)
+                for (int l = 1; l < c->nlevels; l++) o.fill_interp(f, l, level_args<T>(c, l - 1, 1.0, &op, 1.0));
+        });
+    }
+    const int nt = (int)c->trees.size(), nlo = c->nlevels, nbo = c->nb;
+    auto pack = [&](const int *cd) {
+        uint64_t k = 0;
+        for (int d = 0; d < D; d++) k |= (uint64_t)(cd[d] & 0x1fffff) << (21 * d);
+        return k;
+    };
+    using Maps = std::vector<std::unordered_map<uint64_t, int>>;
+    // block of (tree t, level l, coordinate) in a map set, or -1
+    auto find = [&](const Maps &m, int nl, int t, int l, const int *cd) {
+        if (l < 0 || l >= nl) return -1;
+        auto &mm = m[(size_t)t * nl + l];
+        auto it = mm.find(pack(cd));
+        return it == mm.end() ? -1 : it->second;
+    };
+    auto parent_of = [&](const int *cd, int *pc) {
+        int sb = 0;
+        for (int d = 0; d < D; d++) {
+            pc[d] = (cd[d] - 1) / 2 + 1;
+            sb |= ((cd[d] - 1) & 1) << d;
+        }
+        return sb;
+    };
+    Maps om((size_t)nt * nlo);
+    for (int b = 0; b < nbo; b++) om[(size_t)c->h_tree[b] * nlo + c->h_level[b]][pack(&c->h_coord[(size_t)b * D])] = b;
+    // the new trees' blocks from their records, checked before anything changes
+    int nln = 0;
+    for (auto &R : tr) nln = std::max(nln, R.nlevels);
+    Maps nkeys((size_t)nt * std::max(nln, 1));
+    for (int t = 0; t < nt; t++)
+        for (int l = 0; l < tr[t].nlevels; l++)
+            for (int64_t k = tr[t].pairs_off[l]; k < tr[t].pairs_off[l + 1]; k++) {
+                int cd[3] = {1, 1, 1};
+                for (int d = 0; d < D; d++) cd[d] = (int)tr[t].pairs[k * (D + 1) + d];
+                nkeys[(size_t)t * nln + l][pack(cd)] = 0;
+                if (find(om, nlo, t, l, cd) >= 0) continue;
+                int pc[3] = {1, 1, 1};
+                parent_of(cd, pc);
+                if (find(om, nlo, t, l - 1, pc) < 0)
+                    fail("a new block's parent is not in the old tree (a regrid refines by one level)");
+            }
+    for (int b = 0; b < nbo; b++) {
+        const int t = c->h_tree[b], l = c->h_level[b];
+        const int *cd = &c->h_coord[(size_t)b * D];
+        if (find(nkeys, nln, t, l, cd) >= 0) continue;
+        int pc[3] = {1, 1, 1};
+        parent_of(cd, pc);
+        if (find(nkeys, nln, t, l - 1, pc) < 0 || find(om, nlo, t, l - 1, pc) < 0)
+            fail("a removed block's parent is not kept (a regrid coarsens by one level)");
+    }
+    const std::vector<int> ocoord = c->h_coord, olevel = c->h_level, otree = c->h_tree;
+    DevBuf onbr = std::move(c->d_nbr);
+    std::vector<DevBuf> odata(nf), oib(nf);
+    for (int k = 0; k < nf; k++) {
+        FieldBuf &f = field(c, ids[k]);
+        odata[k] = std::move(f.data);
+        oib[k] = std::move(f.ib);
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    set_forest(c, tr);  // new tables; every field reallocated and zeroed
+    Maps nm((size_t)nt * nln);
+    for (int b = 0; b < c->nb; b++) nm[(size_t)c->h_tree[b] * nln + c->h_level[b]][pack(&c->h_coord[(size_t)b * D])] = b;
+    // transfer lists in device block order
+    std::vector<int2> cp;
+    std::vector<int4> ip, rp;
+    for (int b = 0; b < c->nb; b++) {
+        const int t = c->h_tree[b], l = c->h_level[b];
+        const int *cd = &c->h_coord[(size_t)b * D];
+        const int ob = find(om, nlo, t, l, cd);
+        if (ob >= 0) {
+            cp.push_back(make_int2(ob, b));
+            continue;
+        }
+        int pc[3] = {1, 1, 1};
+        const int sb = parent_of(cd, pc);
+        const int opb = find(om, nlo, t, l - 1, pc);
+        if (opb < 0) fail("internal: new block without an old parent");
+        ip.push_back(make_int4(b, opb, sb, 0));
+    }
+    for (int b = 0; b < nbo; b++) {
+        const int t = otree[b], l = olevel[b];
+        const int *cd = &ocoord[(size_t)b * D];
+        if (find(nm, nln, t, l, cd) >= 0) continue;
+        int pc[3] = {1, 1, 1};
+        const int sb = parent_of(cd, pc);
+        const int npb = find(nm, nln, t, l - 1, pc);
+        if (npb < 0) fail("internal: removed block without a kept parent");
+        rp.push_back(make_int4(b, npb, sb, 0));
+    }
+    DevBuf dcp, dip, drp;
+    auto up = [&](DevBuf &d, const void *src, size_t bytes) {
+        d.alloc(bytes);
+        if (bytes) HIPCHK(hipMemcpyAsync(d.p, src, bytes, hipMemcpyHostToDevice, c->stream));
+    };
+    up(dcp, cp.data(), cp.size() * sizeof(int2));
+    up(dip, ip.data(), ip.size() * sizeof(int4));
+    up(drp, rp.data(), rp.size() * sizeof(int4));
+    for (int k = 0; k < nf; k++) {
+        FieldBuf &f = field(c, ids[k]);
+        dispatch(c, f.dtype, [&](auto &o) {
+            using T = std::remove_reference_t<decltype(*o.X(f, 0))>;
+            o.regrid_transfer(f, odata[k], oib[k], onbr, dcp, (int)cp.size(), dip,
+                              (modes[k] & SRMG_REGRID_INTERP) ? (int)ip.size() : 0, drp,
+                              (modes[k] & SRMG_REGRID_RESTRICT) ? (int)rp.size() : 0,
+                              level_args<T>(c, 0, 1.0, &op, 1.0));
+        });
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));  // before the old buffers are released
+}
+
+int srmg_regrid(srmg_ctx *c, int nlevels, int64_t nblocks, const int64_t *pairs, const int64_t *pairs_off,
+                const int64_t *neighbor, const int64_t *neighbor_off, const int64_t *boundary,
+                const int64_t *boundary_off, const int64_t *refboundary, const int64_t *refboundary_off,
+                const int64_t *child, const int64_t *child_off, const int *fields, const int *modes, int nfields,
+                const int8_t *bc) {
+    return guarded([&] {
+        if (!c) fail("null context");
+        regrid_fields(c,
+                      {TreeRecords{nlevels, nblocks, pairs, pairs_off, neighbor, neighbor_off, boundary, boundary_off,
+                                   refboundary, refboundary_off, child, child_off}},
+                      fields, modes, nfields, bc);
     });
 }
 

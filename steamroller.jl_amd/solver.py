@@ -21,6 +21,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import time
 from dataclasses import dataclass
 from typing import Optional
 
@@ -35,6 +36,7 @@ F64, F32 = 0, 1
 KERNEL_GS_PASS, KERNEL_RESIDUAL, KERNEL_GS_SWEEP, KERNEL_STREAM, KERNEL_RESIDUAL_GATHER = 0, 1, 2, 3, 4
 SWEEP_SHAPES = 9  # tile shapes of the fused smoother (Context.set_sweep_config)
 COARSE_GS, COARSE_EXACT = 0, 1
+REGRID_INTERP, REGRID_RESTRICT = 1, 2  # srmg_regrid modes
 GUESS_NONE, GUESS_LEARNED, GUESS_COARSE, GUESS_ZERO = 0, 1, 2, 3
 _DT = {F64: np.float64, F32: np.float32}
 
@@ -123,6 +125,7 @@ def lib():
         "srmg_create": [C.POINTER(P), i32, i32],
         "srmg_destroy": [P],
         "srmg_set_tree": [P, i32, i64] + [P] * 10,
+        "srmg_regrid": [P, i32, i64] + [P] * 10 + [P, P, i32, P],
         "srmg_field_create": [P, i32, C.POINTER(i32)],
         "srmg_field_destroy": [P, i32],
         "srmg_upload": [P, i32, P, i32, i32],
@@ -207,7 +210,7 @@ def axis_decomp(n: int, radial: bool, alow: float, ahigh: float):
 class Context:
     """One device context bound to a tree (``srmg_create`` + ``srmg_set_tree``)."""
 
-    def __init__(self, tree, conn, M: int, forest=None):
+    def __init__(self, tree, conn, M: int, forest=None, nblocks: int | None = None):
         self.D, self.M = tree.D, M
         self.tree, self.conn = tree, conn
         h = C.c_void_p()
@@ -215,7 +218,7 @@ class Context:
         self.h = h
         if forest is None:
             self.ntrees = 1
-            self.regrid(tree, conn)
+            self.regrid(tree, conn, nblocks=nblocks)
         else:
             self.set_forest(forest)
 
@@ -254,8 +257,16 @@ class Context:
         self.trees = [t for t, _ in trees_conns]
         self.nblocks = int(nb_a.sum())
 
-    def regrid(self, tree, conn):
+    def regrid(self, tree, conn, fields=(), bc=None, nblocks: int | None = None):
+        """Install a new tree (``srmg_set_tree``). ``nblocks`` is the number of
+        host block ids (the host fields' length; default: the tree's block
+        count). Without ``fields`` every device field is reallocated at zero.
+        ``fields`` = [(DeviceField, mode), ...] carries those fields across on
+        the device (``srmg_regrid``: applydelta!'s interp!/restrict!, mode a
+        combination of REGRID_INTERP and REGRID_RESTRICT); ``bc`` is needed
+        when a field is interpolated."""
         D = tree.D
+        nblocks = tree.nblocks() if nblocks is None else int(nblocks)
         pairs, poff = flatten_pairs(tree)
         arrs = [pairs, poff]
         for lists, w in ((conn.neighbor, 2 + D), (conn.boundary, 2 + 2 * D),
@@ -265,10 +276,21 @@ class Context:
                 a = np.zeros((1, w), np.int64)
             arrs += [a, o]
         self._tables = arrs
-        _check(lib().srmg_set_tree(self.h, len(tree), tree.nblocks(), *[_p(a) for a in arrs]))
+        fields = list(fields)
+        t0 = time.perf_counter()
+        if fields:
+            ids = np.array([f.id for f, _ in fields], np.int32)
+            modes = np.array([int(m) for _, m in fields], np.int32)
+            b = None if bc is None else _bc_arr(bc)
+            _check(lib().srmg_regrid(self.h, len(tree), nblocks, *[_p(a) for a in arrs], _p(ids), _p(modes),
+                                     len(fields), None if b is None else _p(b)))
+        else:
+            _check(lib().srmg_set_tree(self.h, len(tree), nblocks, *[_p(a) for a in arrs]))
+        # wall time of the library call alone (both return after the device work)
+        self.last_regrid_ms = 1e3 * (time.perf_counter() - t0)
         self.tree, self.conn = tree, conn
         self.ntrees = 1
-        self.nblocks = tree.nblocks()
+        self.nblocks = nblocks
 
     def level_info(self, level: int):
         """(blocks, leaf blocks, coarse-fine ghost slabs) of a 1-based level."""

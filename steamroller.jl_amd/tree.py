@@ -12,6 +12,7 @@ Reference interfaces mirrored here:
   * ``StarStencil`` iteration order                         -- src/stencil.jl:16-33
   * ``connectivity!`` and the four edge records              -- src/connectivity.jl:6-42, 106-149
   * proper-nesting closure used when refining               -- src/refinement.jl:286-331
+  * the tree edit of ``applydelta!``                         -- src/refinement.jl:339-370
 
 Coordinates and block ids are 1-based, exactly as in the reference, so the
 int64 records produced here are bit-for-bit the isbits structs a Julia caller
@@ -212,6 +213,55 @@ def refined_tree(D: int, rootsize: Sequence[int], levels: int,
     return tree
 
 
+# RefDelta (src/refinement.jl:207)
+REMOVE, KEEP, REFINE = -1, 0, 1
+
+
+def applydelta(tree: Tree, delta: Dict[int, int], freeblocks: List[int], nblocks: int, minlevel: int = 1,
+               maxlevel: int | None = None) -> Tuple[int, List[Tuple[str, int, int, Coord]]]:
+    """The tree edit of applydelta! (src/refinement.jl:339-370), in place.
+
+    Blocks are visited as ``@blocks order=serial`` does (finest level first,
+    each level in its ``pairs`` order); level-1 blocks and blocks that are not
+    leaves are skipped. A leaf marked REMOVE above ``minlevel`` is deleted and
+    its id pushed on ``freeblocks``; a leaf marked REFINE below ``maxlevel``
+    gets its 2^D children (``subblocks`` order), each taking an id popped from
+    ``freeblocks`` or, when none is free, the next new id (``newblock!``).
+    ``delta`` maps block ids to REMOVE / KEEP / REFINE (missing ids: KEEP).
+
+    Returns the new number of block ids (the host fields' length) and the
+    field actions in the reference's order: ``("restrict", parent, blk, sub)``
+    for a removed block and ``("interp", new, blk, sub)`` for a new child, sub
+    being the 1-based ``subcoord``. The device regrid (``srmg_regrid``) needs
+    only the edited tree; the actions drive the host restatement in tests.
+    """
+    maxlevel = len(tree) if maxlevel is None else min(maxlevel, len(tree))
+    actions: List[Tuple[str, int, int, Coord]] = []
+    for lvl in range(len(tree), 0, -1):
+        layer = tree[lvl - 1]
+        for c, blk in list(layer.pairs):
+            if lvl <= 1 or not tree.isleaf(lvl, c):
+                continue
+            d = delta.get(blk, KEEP)
+            if d == REMOVE and lvl > minlevel:
+                pblk = tree[lvl - 2].get(parentcoord(c))
+                actions.append(("restrict", pblk, blk, subcoord(c)))
+                del layer.index[c]
+                del layer.coord[blk]
+                freeblocks.append(blk)
+            elif d == REFINE and lvl < maxlevel:
+                for ch in subblocks(c):
+                    if freeblocks:
+                        nblk = freeblocks.pop()
+                    else:
+                        nblocks += 1
+                        nblk = nblocks
+                    tree[lvl][ch] = nblk
+                    actions.append(("interp", nblk, blk, subcoord(ch)))
+    tree.sync()
+    return nblocks, actions
+
+
 class Connectivity:
     """Per-level relation lists (src/connectivity.jl:51-71).
 
@@ -271,13 +321,13 @@ def connectivity(tree: Tree) -> Connectivity:
 def flatten_pairs(tree: Tree) -> Tuple[np.ndarray, np.ndarray]:
     """All levels' ``pairs`` as (coord[D], blk) int64 records plus level offsets."""
     D = tree.D
-    recs, off = [], [0]
+    parts, off = [np.zeros((0, D + 1), np.int64)], [0]
     for layer in tree:
-        for c, blk in layer.pairs:
-            recs.append((*c, blk))
-        off.append(len(recs))
-    arr = np.array(recs, np.int64).reshape(-1, D + 1)
-    return np.ascontiguousarray(arr), np.array(off, np.int64)
+        n = len(layer.pairs)
+        flat = itertools.chain.from_iterable(c + (blk,) for c, blk in layer.pairs)
+        parts.append(np.fromiter(flat, np.int64, count=n * (D + 1)).reshape(n, D + 1))
+        off.append(off[-1] + n)
+    return np.ascontiguousarray(np.concatenate(parts)), np.array(off, np.int64)
 
 
 def flatten_records(lists: List[np.ndarray], width: int) -> Tuple[np.ndarray, np.ndarray]:
