@@ -165,12 +165,31 @@ static void alloc_field(srmg_ctx *c, FieldBuf &f) {
 // regrid: device tables from the reference's records
 // ---------------------------------------------------------------------------
 
+// bit i of x to bit 3i (i < 21) and to bit 2i (i < 32)
+static inline uint64_t spread3(uint64_t x) {
+    x &= 0x1fffff;
+    x = (x | x << 32) & 0x1f00000000ffffull;
+    x = (x | x << 16) & 0x1f0000ff0000ffull;
+    x = (x | x << 8) & 0x100f00f00f00f00full;
+    x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+    x = (x | x << 2) & 0x1249249249249249ull;
+    return x;
+}
+static inline uint64_t spread2(uint64_t x) {
+    x &= 0xffffffffull;
+    x = (x | x << 16) & 0x0000ffff0000ffffull;
+    x = (x | x << 8) & 0x00ff00ff00ff00ffull;
+    x = (x | x << 4) & 0x0f0f0f0f0f0f0f0full;
+    x = (x | x << 2) & 0x3333333333333333ull;
+    x = (x | x << 1) & 0x5555555555555555ull;
+    return x;
+}
+
+// Morton code of a 1-based block coordinate: bit b of (c[d] - 1) at bit b D + d
+// (21 bits per dimension in 3D)
 static uint64_t morton(const int64_t *c, int D) {
-    uint64_t code = 0;
-    for (int bit = 0; bit < 21; bit++)
-        for (int d = 0; d < D; d++)
-            code |= (uint64_t)(((c[d] - 1) >> bit) & 1) << (bit * D + d);
-    return code;
+    if (D == 3) return spread3(c[0] - 1) | spread3(c[1] - 1) << 1 | spread3(c[2] - 1) << 2;
+    return spread2((uint64_t)(c[0] - 1) & 0x1fffff) | spread2((uint64_t)(c[1] - 1) & 0x1fffff) << 1;
 }
 
 // One tree's records as the reference lays them out (src/connectivity.jl:6-42),
@@ -1904,56 +1923,73 @@ static void regrid_fields(srmg_ctx *c, const std::vector<TreeRecords> &tr, const
                 for (int l = 1; l < c->nlevels; l++) o.fill_interp(f, l, level_args<T>(c, l - 1, 1.0, &op, 1.0));
         });
     }
-    const int nt = (int)c->trees.size(), nlo = c->nlevels, nbo = c->nb;
-    auto pack = [&](const int *cd) {
-        uint64_t k = 0;
-        for (int d = 0; d < D; d++) k |= (uint64_t)(cd[d] & 0x1fffff) << (21 * d);
-        return k;
-    };
-    using Maps = std::vector<std::unordered_map<uint64_t, int>>;
-    // block of (tree t, level l, coordinate) in a map set, or -1
-    auto find = [&](const Maps &m, int nl, int t, int l, const int *cd) {
-        if (l < 0 || l >= nl) return -1;
-        auto &mm = m[(size_t)t * nl + l];
-        auto it = mm.find(pack(cd));
-        return it == mm.end() ? -1 : it->second;
-    };
-    auto parent_of = [&](const int *cd, int *pc) {
-        int sb = 0;
-        for (int d = 0; d < D; d++) {
-            pc[d] = (cd[d] - 1) / 2 + 1;
-            sb |= ((cd[d] - 1) & 1) << d;
-        }
-        return sb;
-    };
-    Maps om((size_t)nt * nlo);
-    for (int b = 0; b < nbo; b++) om[(size_t)c->h_tree[b] * nlo + c->h_level[b]][pack(&c->h_coord[(size_t)b * D])] = b;
-    // the new trees' blocks from their records, checked before anything changes
+    // Blocks are matched through sorted Morton codes per (tree, level): the
+    // old blocks' device ranges are in that order already, the new ones are
+    // sorted here as set_forest sorts them (their rank in that order is their
+    // offset in the new device range), and a parent's code is its child's
+    // shifted right by D, so every lookup is a merge of sorted lists.
+    const int nt = (int)c->trees.size(), nlo = c->nlevels;
     int nln = 0;
     for (auto &R : tr) nln = std::max(nln, R.nlevels);
-    Maps nkeys((size_t)nt * std::max(nln, 1));
-    for (int t = 0; t < nt; t++)
-        for (int l = 0; l < tr[t].nlevels; l++)
-            for (int64_t k = tr[t].pairs_off[l]; k < tr[t].pairs_off[l + 1]; k++) {
-                int cd[3] = {1, 1, 1};
-                for (int d = 0; d < D; d++) cd[d] = (int)tr[t].pairs[k * (D + 1) + d];
-                nkeys[(size_t)t * nln + l][pack(cd)] = 0;
-                if (find(om, nlo, t, l, cd) >= 0) continue;
-                int pc[3] = {1, 1, 1};
-                parent_of(cd, pc);
-                if (find(om, nlo, t, l - 1, pc) < 0)
-                    fail("a new block's parent is not in the old tree (a regrid refines by one level)");
+    using Key = std::pair<uint64_t, int>;  // (Morton code, old device block | rank in the new level)
+    std::vector<std::vector<Key>> ol((size_t)nt * nlo), nl((size_t)nt * nln);
+    for (int t = 0; t < nt; t++) {
+        const TreeInfo &T = c->trees[t];
+        for (int l = 0; l < nlo; l++) {
+            auto &v = ol[(size_t)t * nlo + l];
+            for (int b = T.blk0[l]; b < T.blk0[l] + T.nblk[l]; b++) {
+                int64_t cd[3] = {1, 1, 1};
+                for (int d = 0; d < D; d++) cd[d] = c->h_coord[(size_t)b * D + d];
+                v.push_back({morton(cd, D), b});
             }
-    for (int b = 0; b < nbo; b++) {
-        const int t = c->h_tree[b], l = c->h_level[b];
-        const int *cd = &c->h_coord[(size_t)b * D];
-        if (find(nkeys, nln, t, l, cd) >= 0) continue;
-        int pc[3] = {1, 1, 1};
-        parent_of(cd, pc);
-        if (find(nkeys, nln, t, l - 1, pc) < 0 || find(om, nlo, t, l - 1, pc) < 0)
-            fail("a removed block's parent is not kept (a regrid coarsens by one level)");
+        }
+        for (int l = 0; l < tr[t].nlevels; l++) {
+            auto &v = nl[(size_t)t * nln + l];
+            for (int64_t k = tr[t].pairs_off[l]; k < tr[t].pairs_off[l + 1]; k++)
+                v.push_back({morton(tr[t].pairs + k * (D + 1), D), 0});
+            std::sort(v.begin(), v.end());
+            for (size_t i = 0; i < v.size(); i++) v[i].second = (int)i;
+        }
     }
-    const std::vector<int> ocoord = c->h_coord, olevel = c->h_level, otree = c->h_tree;
+    const std::vector<Key> none;
+    auto lst = [&](const std::vector<std::vector<Key>> &L, int n, int t, int l) -> const std::vector<Key> & {
+        return (l < 0 || l >= n) ? none : L[(size_t)t * n + l];
+    };
+    // the entry of a sorted list with this code, for codes queried in
+    // increasing order (cursor i only moves forward), or -1
+    auto seek = [](const std::vector<Key> &v, size_t &i, uint64_t code) {
+        while (i < v.size() && v[i].first < code) i++;
+        return (i < v.size() && v[i].first == code) ? v[i].second : -1;
+    };
+    struct NewItem { int t, l, rank, src, sb; };
+    std::vector<NewItem> cpn, ipn, rpn;  // kept (src = old block), new (src = old parent), removed (rank of the new parent)
+    for (int t = 0; t < nt; t++)
+        for (int l = 0; l < std::max(nlo, nln); l++) {
+            const auto &A = lst(ol, nlo, t, l), &B = lst(nl, nln, t, l);
+            const auto &Ap = lst(ol, nlo, t, l - 1), &Bp = lst(nl, nln, t, l - 1);
+            size_t ia = 0, ib = 0, ca = 0, cb = 0, cpo = 0;
+            while (ia < A.size() || ib < B.size()) {
+                if (ib == B.size() || (ia < A.size() && A[ia].first < B[ib].first)) {
+                    // removed: the parent must be kept
+                    const uint64_t pc = A[ia].first >> D;
+                    const int np = seek(Bp, cb, pc), op = seek(Ap, cpo, pc);
+                    if (np < 0 || op < 0) fail("a removed block's parent is not kept (a regrid coarsens by one level)");
+                    rpn.push_back({t, l - 1, np, A[ia].second, (int)(A[ia].first & ((1u << D) - 1))});
+                    ia++;
+                } else if (ia == A.size() || B[ib].first < A[ia].first) {
+                    // new: the parent must have existed
+                    const int op = seek(Ap, ca, B[ib].first >> D);
+                    if (op < 0) fail("a new block's parent is not in the old tree (a regrid refines by one level)");
+                    ipn.push_back({t, l, B[ib].second, op, (int)(B[ib].first & ((1u << D) - 1))});
+                    ib++;
+                } else {
+                    cpn.push_back({t, l, B[ib].second, A[ia].second, 0});
+                    ia++;
+                    ib++;
+                }
+            }
+        }
+    const std::vector<int> ocoord = c->h_coord;
     DevBuf onbr = std::move(c->d_nbr);
     std::vector<DevBuf> odata(nf), oib(nf);
     for (int k = 0; k < nf; k++) {
@@ -1963,35 +1999,22 @@ static void regrid_fields(srmg_ctx *c, const std::vector<TreeRecords> &tr, const
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     set_forest(c, tr);  // new tables; every field reallocated and zeroed
-    Maps nm((size_t)nt * nln);
-    for (int b = 0; b < c->nb; b++) nm[(size_t)c->h_tree[b] * nln + c->h_level[b]][pack(&c->h_coord[(size_t)b * D])] = b;
-    // transfer lists in device block order
+    auto dev = [&](const NewItem &it) {
+        const int b = c->trees[it.t].blk0[it.l] + it.rank;
+        if (it.rank >= c->trees[it.t].nblk[it.l]) fail("internal: regrid rank outside the level");
+        return b;
+    };
     std::vector<int2> cp;
     std::vector<int4> ip, rp;
-    for (int b = 0; b < c->nb; b++) {
-        const int t = c->h_tree[b], l = c->h_level[b];
-        const int *cd = &c->h_coord[(size_t)b * D];
-        const int ob = find(om, nlo, t, l, cd);
-        if (ob >= 0) {
-            cp.push_back(make_int2(ob, b));
-            continue;
-        }
-        int pc[3] = {1, 1, 1};
-        const int sb = parent_of(cd, pc);
-        const int opb = find(om, nlo, t, l - 1, pc);
-        if (opb < 0) fail("internal: new block without an old parent");
-        ip.push_back(make_int4(b, opb, sb, 0));
+    cp.reserve(cpn.size());
+    for (auto &it : cpn) {
+        const int b = dev(it);
+        for (int d = 0; d < D; d++)
+            if (c->h_coord[(size_t)b * D + d] != ocoord[(size_t)it.src * D + d]) fail("internal: regrid block mismatch");
+        cp.push_back(make_int2(it.src, b));
     }
-    for (int b = 0; b < nbo; b++) {
-        const int t = otree[b], l = olevel[b];
-        const int *cd = &ocoord[(size_t)b * D];
-        if (find(nm, nln, t, l, cd) >= 0) continue;
-        int pc[3] = {1, 1, 1};
-        const int sb = parent_of(cd, pc);
-        const int npb = find(nm, nln, t, l - 1, pc);
-        if (npb < 0) fail("internal: removed block without a kept parent");
-        rp.push_back(make_int4(b, npb, sb, 0));
-    }
+    for (auto &it : ipn) ip.push_back(make_int4(dev(it), it.src, it.sb, 0));
+    for (auto &it : rpn) rp.push_back(make_int4(it.src, dev(it), it.sb, 0));
     DevBuf dcp, dip, drp;
     auto up = [&](DevBuf &d, const void *src, size_t bytes) {
         d.alloc(bytes);
