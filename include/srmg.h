@@ -96,6 +96,27 @@ int srmg_set_forest(srmg_ctx *ctx, int ntrees, const int *nlevels, const int64_t
 /* Number of trees of the context. */
 int srmg_tree_count(srmg_ctx *ctx, int *ntrees);
 
+/* connectivity!(conn, tree, stencil) (src/connectivity.jl:106-149) on the
+ * GPU: the four record lists of a tree from its blocks alone, in the
+ * reference's order (level by level, blocks in `pairs` order, offsets in
+ * stencil order), with the record layouts of srmg_set_tree. stencil:
+ * SRMG_STAR_STENCIL (StarStencil{D}, the order-2 operators) or
+ * SRMG_BOX_STENCIL (BoxStencil{D}, diagonals included). rootsize: the level-1
+ * domain in blocks (level l spans rootsize * 2^(l-1)). The caller provides
+ * each list with a capacity in records (at most nblocks * (number of stencil
+ * offsets) for the first three, the blocks above level 1 for child) and its
+ * nlevels+1 offsets. Fails on a position holding two blocks, a neighbour
+ * position without a parent block (proper nesting broken) or a block without
+ * a parent, as the reference asserts. Uses the current HIP device and a
+ * stream of its own; returns when the records are on the host. */
+enum { SRMG_STAR_STENCIL = 0, SRMG_BOX_STENCIL = 1 };
+int srmg_connectivity(int D, int nlevels, const int64_t *rootsize,
+                      const int64_t *pairs, const int64_t *pairs_off, int stencil,
+                      int64_t *neighbor, int64_t *neighbor_off, int64_t neighbor_cap,
+                      int64_t *boundary, int64_t *boundary_off, int64_t boundary_cap,
+                      int64_t *refboundary, int64_t *refboundary_off, int64_t refboundary_cap,
+                      int64_t *child, int64_t *child_off, int64_t child_cap);
+
 /* Regrid that carries fields across on the device: the field part of
  * applydelta!(tree, refdelta, freeblocks, fields, minlevel, maxlevel)
  * (src/refinement.jl:339-370, with interp! and restrict! of

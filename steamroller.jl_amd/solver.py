@@ -126,6 +126,7 @@ def lib():
         "srmg_destroy": [P],
         "srmg_set_tree": [P, i32, i64] + [P] * 10,
         "srmg_regrid": [P, i32, i64] + [P] * 10 + [P, P, i32, P],
+        "srmg_connectivity": [i32, i32, P, P, P, i32] + [P, P, i64] * 4,
         "srmg_field_create": [P, i32, C.POINTER(i32)],
         "srmg_field_destroy": [P, i32],
         "srmg_upload": [P, i32, P, i32, i32],
@@ -203,6 +204,37 @@ def axis_decomp(n: int, radial: bool, alow: float, ahigh: float):
     Q = np.zeros((n, n))
     _check(lib().srmg_axis_decomp(n, int(radial), alow, ahigh, _p(lam), _p(P), _p(Q)))
     return lam, P, Q
+
+
+# ---- connectivity! on the device ----
+
+STAR_STENCIL, BOX_STENCIL = 0, 1
+
+
+def connectivity_device(tree, stencil: int = STAR_STENCIL):
+    """connectivity!(conn, tree, stencil) (src/connectivity.jl:106-149) computed
+    by ``srmg_connectivity`` on the GPU; returns the same
+    :class:`srmg.tree.Connectivity` as :func:`srmg.tree.connectivity`."""
+    from .tree import Connectivity
+    D, L = tree.D, len(tree)
+    pairs, poff = flatten_pairs(tree)
+    n = int(poff[-1])
+    ns = 2 * D if stencil == STAR_STENCIL else 3 ** D - 1
+    cap = max(n * ns, 1)
+    nchild = max(n - int(poff[1]), 1) if L > 0 else 1
+    root = np.array(tree.rootsize, np.int64)
+    nb, bd, rb = (np.zeros((cap, w), np.int64) for w in (2 + D, 2 + 2 * D, 2 + 2 * D))
+    ch = np.zeros((nchild, 2 + D), np.int64)
+    offs = [np.zeros(L + 1, np.int64) for _ in range(4)]
+    _check(lib().srmg_connectivity(D, L, _p(root), _p(pairs), _p(poff), int(stencil),
+                                   _p(nb), _p(offs[0]), cap, _p(bd), _p(offs[1]), cap,
+                                   _p(rb), _p(offs[2]), cap, _p(ch), _p(offs[3]), nchild))
+    conn = Connectivity(D, L)
+    for lists, arr, o in ((conn.neighbor, nb, offs[0]), (conn.boundary, bd, offs[1]),
+                          (conn.refboundary, rb, offs[2]), (conn.child, ch, offs[3])):
+        for l in range(L):
+            lists[l] = arr[o[l]:o[l + 1]].copy()
+    return conn
 
 
 # ---- device objects ----

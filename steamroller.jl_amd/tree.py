@@ -9,7 +9,7 @@ Reference interfaces mirrored here:
   * ``BlockLayer{D}`` / ``sync!`` / ``get`` / ``hasblock``  -- src/blocklayer.jl:9-115
   * ``Tree(D, domain, levels)`` / ``populate!`` / ``isleaf`` -- src/tree.jl:13-44, 105-115
   * ``parentcoord`` / ``subcoord`` / ``subblocks``         -- src/mesh.jl:40-61
-  * ``StarStencil`` iteration order                         -- src/stencil.jl:16-33
+  * ``StarStencil`` / ``BoxStencil`` iteration order        -- src/stencil.jl:16-56
   * ``connectivity!`` and the four edge records              -- src/connectivity.jl:6-42, 106-149
   * proper-nesting closure used when refining               -- src/refinement.jl:286-331
   * the tree edit of ``applydelta!``                         -- src/refinement.jl:339-370
@@ -55,6 +55,15 @@ def star_stencil(D: int) -> List[Coord]:
             s = [0] * D
             s[d] = v
             out.append(tuple(s))
+    return out
+
+
+def box_stencil(D: int) -> List[Coord]:
+    """src/stencil.jl:36-56: every offset in -1..1 (first index fastest) but zero."""
+    out = []
+    for o in column_major([range(-1, 2)] * D):
+        if any(o):
+            out.append(o)
     return out
 
 
@@ -281,11 +290,12 @@ class Connectivity:
         self.child = [np.zeros((0, 2 + D), np.int64) for _ in range(levels)]
 
 
-def connectivity(tree: Tree) -> Connectivity:
-    """src/connectivity.jl:106-149 with the StarStencil (order-2 operators)."""
+def connectivity(tree: Tree, stencil: Sequence[Coord] | None = None) -> Connectivity:
+    """src/connectivity.jl:106-149; ``stencil`` defaults to the StarStencil
+    (order-2 operators); :func:`box_stencil` gives the BoxStencil."""
     D = tree.D
     conn = Connectivity(D, len(tree))
-    stencil = star_stencil(D)
+    stencil = star_stencil(D) if stencil is None else list(stencil)
     for li, layer in enumerate(tree):
         bnd, nbr, ref, chd = [], [], [], []
         for c, blk in layer.pairs:
