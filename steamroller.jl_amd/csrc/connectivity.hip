@@ -25,7 +25,7 @@
 #include "common.h"
 
 namespace srmg {
-namespace {
+namespace conn {
 
 constexpr uint64_t EMPTY = ~0ull;
 
@@ -193,10 +193,11 @@ __global__ void k_conn_scan(const uint64_t *keys, const int64_t *vals, const int
     }
 }
 
-}  // namespace
+}  // namespace conn
 }  // namespace srmg
 
 using namespace srmg;
+using namespace srmg::conn;
 
 int srmg_connectivity(int D, int nlevels, const int64_t *rootsize, const int64_t *pairs, const int64_t *pairs_off,
                       int stencil, int64_t *neighbor, int64_t *neighbor_off, int64_t neighbor_cap, int64_t *boundary,
@@ -288,14 +289,14 @@ int srmg_connectivity(int D, int nlevels, const int64_t *rootsize, const int64_t
         HIPCHK(hipcub::DeviceScan::ExclusiveSum(dtmp.p, tmp_bytes, cn, on, (int)(n + 1), st));
         HIPCHK(hipcub::DeviceScan::ExclusiveSum(dtmp.p, tmp_bytes, cr, orr, (int)(n + 1), st));
         // per-level offsets: the scans at the levels' first blocks
-        std::vector<int> lofs(3 * (nlevels + 1));
-        for (int l = 0; l <= nlevels; l++)
-            for (int k = 0; k < 3; k++)
-                HIPCHK(hipMemcpyAsync(&lofs[3 * l + k], (const int *)dofs.p + (size_t)k * (n + 1) + pairs_off[l], 4,
-                                      hipMemcpyDeviceToHost, st));
+        std::vector<int> scans((size_t)3 * (n + 1));
+        HIPCHK(hipMemcpyAsync(scans.data(), dofs.p, scans.size() * 4, hipMemcpyDeviceToHost, st));
         int err = 0;
         HIPCHK(hipMemcpyAsync(&err, derr.p, 4, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
+        std::vector<int> lofs(3 * (nlevels + 1));
+        for (int l = 0; l <= nlevels; l++)
+            for (int k = 0; k < 3; k++) lofs[3 * l + k] = scans[(size_t)k * (n + 1) + pairs_off[l]];
         if (err & 1) fail("two blocks at the same position of a level");
         if (err & 2) fail("proper nesting broken: a neighbour position has no parent block");
         if (err & 4) fail("malformed tree: a block above level 1 has no parent");
