@@ -432,6 +432,15 @@ def fill_ghost(u: DeviceField, l: int, conn=None, bc=None):
     _check(lib().srmg_fill_ghost(u.ctx.h, u.id, l, _p(b)))
 
 
+def extrapolate_const(D: int) -> np.ndarray:
+    """``ExtrapolateConst()`` (src/boundary.jl:18-21, 49) as a boundary table:
+    its getbc returns the mirror cell's value, i.e. every entry +1. The
+    reference fills |E|'s ghosts with it after ``electric_field!``
+    (src/streamer.jl:353-355): ``fill_ghost_all(eabs, extrapolate_const(D))``,
+    then ``eabs.download(host, bc=extrapolate_const(D))``."""
+    return np.ones((3,) * D, np.int8)
+
+
 def fill_ghost_all(u: DeviceField, bc):
     b = _bc_arr(bc)
     _check(lib().srmg_fill_ghost_all(u.ctx.h, u.id, _p(b)))

@@ -442,6 +442,49 @@ def test_electric_field_matches_oracle(name):
     np.testing.assert_array_equal(_valid(got, p), fabs[(slice(p.b.G, p.b.G + p.M),) * p.D])
 
 
+def _face_ghosts(a, D, M, G):
+    """the depth-1 ghost layer of every face (corners excluded), per block"""
+    out = []
+    for d in range(D):
+        for g in (G - 1, G + M):
+            idx = tuple(g if e == d else slice(G, G + M) for e in range(D))
+            out.append(a[idx + (slice(None),)])
+    return out
+
+
+@pytest.mark.parametrize("name", ["cfg1", "cyl1-m16", "cart3-m8"])
+def test_eabs_ghosts_with_extrapolate_const_match_oracle(name):
+    # fill_ghost!(eabs, l, conn, ExtrapolateConst()) for every level after
+    # electric_field! (src/streamer.jl:353-355): copy, mirror with +1 and
+    # coarse-fine interpolation, then the host copy with its ghosts
+    p = CASES[name]()
+    print(f"case {name}", flush=True)
+    geom, disc = solver.geometry_from_code(p.geom), _disc(p)
+    ctx, (u, b, r, u1) = _device(p)
+    b.upload(p.b)
+    solver.fill_ghost_all(u, p.bc)
+    for _ in range(2):
+        solver.vcycle(u, b, r, u1, p.s, p.tree, p.conn, geom, p.bc, disc, ndown=2, ntop=4, nup=2)
+    e0 = [0.0, -2.0e6, 1.0e5][:p.D]
+    eabs = ctx.field(F64)
+    solver.electric_field(u, eabs, p.h, e0, p.bc, G=p.b.G, with_faces=False)
+    xc = solver.extrapolate_const(p.D)
+    solver.fill_ghost_all(eabs, xc)
+    got = p.b.like()
+    eabs.download(got, bc=xc)
+    host = p.b.like()
+    u.download(host)
+    o = Oracle.for_problem(p)
+    o.fill_ghost_all(host)
+    _, fabs = o.electric_field(host, p.h, e0)
+    ox = Oracle(p.tree, p.conn, p.M, p.b.G, p.geom, xc, p.helm, p.k2)
+    ox.fill_ghost_all(fabs)
+    G, M = p.b.G, p.M
+    np.testing.assert_array_equal(_valid(got, p), fabs[(slice(G, G + M),) * p.D])
+    for k, (x, y) in enumerate(zip(_face_ghosts(got.u, p.D, M, G), _face_ghosts(fabs, p.D, M, G))):
+        np.testing.assert_array_equal(x, y, err_msg=f"face {k}")
+
+
 @pytest.mark.parametrize("name", ["cfg1", "cart3-m8", "cart3-helm"])
 def test_zero_start_equals_a_solve_from_zeroed_u(name):
     # SRMG_GUESS_ZERO takes resnorm(0) and the first FP64 defect from b alone;
