@@ -310,6 +310,8 @@ def regrid_step(log, p, ctx, u, b, e):
     from srmg.tree import REFINE, REMOVE, applydelta, connectivity, parentcoord, star_stencil, subblocks
     tree, D = p.tree, p.D
     L = len(tree)
+    if L < 3:
+        return None
     t0 = time.time()
     delta = {}
     fin, lay = tree[L - 1], tree[L - 2]

@@ -42,21 +42,23 @@ inline int guarded(F &&f) {
 
 struct DevBuf {
     void *p = nullptr;
-    size_t bytes = 0;
+    size_t bytes = 0;  // size in use
+    size_t cap = 0;    // size allocated
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
-    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) {
+    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes), cap(o.cap) {
         o.p = nullptr;
-        o.bytes = 0;
+        o.bytes = o.cap = 0;
     }
     DevBuf &operator=(DevBuf &&o) noexcept {
         if (this != &o) {
             release();
             p = o.p;
             bytes = o.bytes;
+            cap = o.cap;
             o.p = nullptr;
-            o.bytes = 0;
+            o.bytes = o.cap = 0;
         }
         return *this;
     }
@@ -64,13 +66,20 @@ struct DevBuf {
     void release() {
         if (p) (void)hipFree(p);
         p = nullptr;
-        bytes = 0;
+        bytes = cap = 0;
     }
+    // n bytes; the allocation is kept when it holds n and wastes at most a
+    // quarter of n (regrids change sizes by a few percent); contents are not
+    // preserved either way
     void alloc(size_t n) {
-        release();
         if (n == 0) n = 16;
+        if (p && cap >= n && cap - n <= n / 4) {
+            bytes = n;
+            return;
+        }
+        release();
         HIPCHK(hipMalloc(&p, n));
-        bytes = n;
+        bytes = cap = n;
     }
     template <class T> T *as() const { return static_cast<T *>(p); }
 };

@@ -153,8 +153,7 @@ static FieldBuf &field(srmg_ctx *c, int id) {
 }
 
 static void alloc_field(srmg_ctx *c, FieldBuf &f) {
-    f.alt.release();
-    f.cur.assign(c->nlevels, 0);
+    f.cur.assign(c->nlevels, 0);  // alt (ping-pong) holds nothing until the next sweep
     f.data.alloc((size_t)c->nb * c->V * f.word());
     f.ib.alloc((size_t)std::max(c->nslot, 1) * c->F * f.word());
     HIPCHK(hipMemsetAsync(f.data.p, 0, f.data.bytes, c->stream));
@@ -818,7 +817,7 @@ struct Ops {
     // divide, use shape 0)
     void sweep(FieldBuf &u, FieldBuf &b, const LevelArgs<T> &a, int l, int cfg = -1) {
         if (!a.nblk) return;
-        if (!u.alt.p) u.alt.alloc(u.data.bytes);
+        if (!u.alt.p || u.alt.bytes != u.data.bytes) u.alt.alloc(u.data.bytes);
         T *in = X(u, l);
         T *out = u.cur[l] ? u.data.as<T>() : u.alt.as<T>();
         if (cfg < 0) cfg = c->sweep_cfg[(std::is_same<T, double>::value ? 0 : c->nlevels) + l];
