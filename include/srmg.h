@@ -40,7 +40,9 @@ enum { SRMG_F64 = 0, SRMG_F32 = 1 };
 enum { SRMG_CARTESIAN = 0, SRMG_CYL1 = 1, SRMG_CYL2 = 2 };
 /* discretization: LaplacianDiscretization{D,2}, HelmholtzDiscretization{D,2,T}
  * (src/laplacian.jl:14, src/helmholtz.jl:9-11) */
-enum { SRMG_LAPLACIAN = 0, SRMG_HELMHOLTZ = 1 };
+/* SRMG_LAPLACIAN4: the compact 4th-order LaplacianDiscretization{D,4}; only
+ * srmg_rhs accepts it (its smoother, FourColors, is not built). */
+enum { SRMG_LAPLACIAN = 0, SRMG_HELMHOLTZ = 1, SRMG_LAPLACIAN4 = 2 };
 /* coarsest-level treatment: the reference's ntop Gauss-Seidel sweeps
  * (src/multigrid.jl:89), or an exact FP64 solve of the level-1 problem
  * (an extension with no reference counterpart, needed when level 1 is large). */
@@ -116,6 +118,17 @@ int srmg_connectivity(int D, int nlevels, const int64_t *rootsize,
                       int64_t *boundary, int64_t *boundary_off, int64_t boundary_cap,
                       int64_t *refboundary, int64_t *refboundary_off, int64_t refboundary_cap,
                       int64_t *child, int64_t *child_off, int64_t child_cap);
+
+/* restrict_full!(u, conn) (src/restrict.jl:43-47): every level's blocks
+ * restricted into their parents, finest level first. */
+int srmg_restrict_full(srmg_ctx *ctx, int field);
+
+/* The right-hand side of derivs!'s Poisson block (src/streamer.jl:313-318):
+ * for every level, fill_ghost!(b, l) with op->bc, then rhs_level!(b1, b, l)
+ * (src/multigrid.jl:363-388): b1 = b for the order-2 operators, the compact
+ * 4th-order R b for SRMG_LAPLACIAN4 (Cartesian 2D/3D, cylindrical 2D). b1's
+ * ghosts are derived when read, as for every field. */
+int srmg_rhs(srmg_ctx *ctx, int b1, int b, const srmg_op *op);
 
 /* Regrid that carries fields across on the device: the field part of
  * applydelta!(tree, refdelta, freeblocks, fields, minlevel, maxlevel)

@@ -68,6 +68,7 @@ def lib():
             "ora_freebc_setouter": [P, P, P, i32, i32, dbl, P],
             "ora_freebc": [P, P, P, dbl, dbl, i32, i32, P, P, i64, P],
             "ora_fill_ghost_free": [P, P, P, i32, i32, i32, P],
+            "ora_rhs_level": [P, P, P, i32, P, i32],
         }.items():
             getattr(_LIB, name).argtypes = args
             getattr(_LIB, name).restype = None
@@ -210,6 +211,23 @@ class Oracle:
     def residual_level(self, r, u, b, s, level):
         lib().ora_residual_level(self.h, _ptr(self._a(r)), _ptr(self._a(u)), _ptr(self._a(b)),
                                  s, level, self._opp())
+
+    def rhs_level(self, b1, b, level, order=2):
+        """rhs_level!(b1, b, tree[level], lpl, geom) for a discretization of
+        order 2 or 4 (b's ghosts of this level filled)."""
+        lib().ora_rhs_level(self.h, _ptr(self._a(b1)), _ptr(self._a(b)), level, self._opp(), int(order))
+
+    def rhs(self, b1, b, order=2):
+        """derivs!'s loop (src/streamer.jl:314-318): fill_ghost!(b, l), then
+        rhs_level!(b1, b, l), level by level."""
+        for l in range(1, self.nlevels + 1):
+            self.fill_ghost(b, l)
+            self.rhs_level(b1, b, l, order)
+
+    def restrict_full(self, u):
+        """restrict_full!(u, conn) (src/restrict.jl:43-47): finest level first."""
+        for l in range(self.nlevels, 1, -1):
+            self.restrict_level(u, l)
 
     def restrict_level(self, u, level):
         lib().ora_restrict_level(self.h, _ptr(self._a(u)), level)

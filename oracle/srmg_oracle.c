@@ -129,10 +129,18 @@ static inline double centre_coef(const ora_tree *t, const ora_op *op, double s) 
  * with a[S1] != 0. For the 5-point 2D literal this is the offset order
  * (0,-1), (-1,0), (0,0), (+1,0), (0,+1); for the 7-point 3D literal
  * (0,0,-1), (0,-1,0), (-1,0,0), (0,0,0), (+1,0,0), (0,+1,0), (0,0,+1). */
+static double stencil_sum(const ora_tree *t, const ora_op *op, const double *u,
+                          int64_t blk, const int *I, const int64_t *J, double cc);
+
 static double applystencil(const ora_tree *t, const ora_op *op, const double *u,
                            int64_t blk, const int *I, const int64_t *J, double s) {
+    return stencil_sum(t, op, u, blk, I, J, centre_coef(t, op, s));
+}
+
+/* the star-stencil sum with centre weight cc, in the @stencil order above */
+static double stencil_sum(const ora_tree *t, const ora_op *op, const double *u,
+                          int64_t blk, const int *I, const int64_t *J, double cc) {
     const int D = t->D;
-    double cc = centre_coef(t, op, s);
     /* per-dimension (minus, plus) weights */
     double wm[3] = {1.0, 1.0, 1.0}, wp[3] = {1.0, 1.0, 1.0};
     if (op->geom == 1 || op->geom == 2) {
@@ -369,6 +377,37 @@ void ora_residual_level(const ora_tree *t, double *r, const double *u, const dou
             double Lu = applystencil(t, op, u, blk, I, J, s);
             int64_t o = OFF(t, I, blk);
             r[o] = -(b[o] + Lu / s);
+        }
+    }
+}
+
+/* rhs! / rhs_level! (src/multigrid.jl:363-388) over the blocks of `level`.
+ * order 2 (Laplacian or Helmholtz): b1 = b (src/laplacian.jl:88-92, 121-125,
+ * 296-300; src/helmholtz.jl:27-32). order 4: the compact R stencil
+ * (src/laplacian.jl:142-149 2D Cartesian, 167-175 / 187-195 cylindrical,
+ * 322-337 3D): the @stencil literal's nonzero terms in column-major order --
+ * the order-2 stencil's order with centre weight 8 (2D) or 6 (3D) and the
+ * radial weights on the radial neighbours -- then divided by 12. b's ghosts
+ * of this level must be filled. */
+void ora_rhs_level(const ora_tree *t, double *b1, const double *b, int level, const ora_op *op, int order) {
+    const int D = t->D, M = t->M, G = t->G;
+    int64_t lo = t->pairs_off[level - 1], hi = t->pairs_off[level];
+    for (int64_t k = lo; k < hi; k++) {
+        int64_t blk = t->pairs[k * (D + 1) + D];
+        int I[3] = {1, 1, 1};
+        int n2 = (D >= 2) ? M : 1, n3 = (D >= 3) ? M : 1;
+        for (int a3 = 0; a3 < n3; a3++)
+        for (int a2 = 0; a2 < n2; a2++)
+        for (int a1 = 0; a1 < M; a1++) {
+            I[0] = G + 1 + a1; I[1] = G + 1 + a2; I[2] = G + 1 + a3;
+            int64_t o = OFF(t, I, blk);
+            if (order == 4) {
+                int64_t J[3];
+                global_index(t, blk, I, J);
+                b1[o] = stencil_sum(t, op, b, blk, I, J, (double)(12 - 2 * D)) / 12.0;
+            } else {
+                b1[o] = b[o];
+            }
         }
     }
 }

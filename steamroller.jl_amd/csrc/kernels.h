@@ -174,6 +174,31 @@ __global__ __launch_bounds__(WG) void k_residual(T *__restrict__ r, const T *__r
     r[o] = -(b[o] + Lu / bp.s);
 }
 
+// rhs! (src/multigrid.jl:363-388) of the compact 4th-order Laplacian
+// (LaplacianDiscretization{D,4}, src/laplacian.jl:142-149, 167-175, 187-195,
+// 322-337): b1 = (1/12) [0 1 0; 1 8 1; 0 1 0] b in 2D (the radial weights of
+// the cylindrical forms on the radial neighbours), the 7-point [1 .. 6 .. 1]
+// in 3D; the @stencil order of terms is the order-2 stencil's with centre
+// weight 12 - 2D, then one division by 12.
+template <int D, int M, typename T>
+__global__ __launch_bounds__(WG) void k_rhs4(T *__restrict__ b1, const T *__restrict__ b, const T *__restrict__ ib,
+                                             const int *__restrict__ nbr, const int *__restrict__ coord,
+                                             LevelArgs<T> a) {
+    using LY = Lay<D, M>;
+    int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    int lb = gt / LY::V;
+    if (lb >= a.nblk) return;
+    int blk = a.blk0 + lb;
+    int q = gt - lb * LY::V;
+    int c0, c1, c2;
+    LY::decode(q, c0, c1, c2);
+    size_t o = (size_t)blk * LY::V + q;
+    T wm, wp;
+    radial_weights<D, M, T>(a, coord, blk, c0, c1, wm, wp);
+    T v = stencil<D, M, T>(b, ib, nbr, blk, c0, c1, c2, b[o], a, T(12 - 2 * D), wm, wp);
+    b1[o] = v / T(12);
+}
+
 // restrict! over the Child edges of a level (src/restrict.jl:5-37): each
 // coarse cell of the parent subblock = mean of its 2^D children, summed with
 // the first index fastest. xf holds the fine level, xc the coarse one.

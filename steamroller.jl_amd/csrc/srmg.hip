@@ -925,6 +925,18 @@ struct Ops {
                            c->stream, X(x, l - 1), X(x, l), c->d_parent.as<int>(), c->d_sub.as<int>(), L.blk0,
                            L.nblk);
     }
+    // rhs_level! (src/multigrid.jl:378-388) of level l: b1 = R b on every block
+    void rhs_level(FieldBuf &b1, FieldBuf &b, int l, const LevelArgs<T> &a, bool order4) {
+        const LevelInfo &L = c->lv[l];
+        if (!L.nblk) return;
+        const size_t n = (size_t)L.nblk * LY::V, o = (size_t)L.blk0 * LY::V;
+        if (order4)
+            hipLaunchKernelGGL((k_rhs4<D, M, T>), dim3(nwg((int64_t)n)), dim3(WG), 0, c->stream, X(b1, l), X(b, l),
+                               I(b), nbr(), coord(), a);
+        else
+            hipLaunchKernelGGL((k_copy<T>), dim3(nwg((int64_t)n)), dim3(WG), 0, c->stream, X(b1, l) + o,
+                               X(b, l) + o, n);
+    }
     void postrestrict(FieldBuf &b, FieldBuf &u, FieldBuf &r, int l, const LevelArgs<T> &ac) {
         const LevelInfo &L = c->lv[l];
         if (!L.nblk) return;
@@ -2047,6 +2059,44 @@ int srmg_regrid(srmg_ctx *c, int nlevels, int64_t nblocks, const int64_t *pairs,
                       {TreeRecords{nlevels, nblocks, pairs, pairs_off, neighbor, neighbor_off, boundary, boundary_off,
                                    refboundary, refboundary_off, child, child_off}},
                       fields, modes, nfields, bc);
+    });
+}
+
+int srmg_restrict_full(srmg_ctx *c, int id) {
+    return guarded([&] {
+        FieldBuf &f = field(c, id);
+        check_tree(c);
+        dispatch(c, f.dtype, [&](auto &o) {
+            o.settle(f);
+            for (int l = c->nlevels - 1; l >= 1; l--) o.restrict_(f, l);
+        });
+        HIPCHK(hipGetLastError());
+    });
+}
+
+int srmg_rhs(srmg_ctx *c, int b1, int b, const srmg_op *op) {
+    return guarded([&] {
+        if (!op) fail("null operator");
+        check_tree(c);
+        if (b1 == b) fail("rhs needs two distinct fields");
+        const int t = same_type(c, {b1, b});
+        if (op->disc != SRMG_LAPLACIAN && op->disc != SRMG_HELMHOLTZ && op->disc != SRMG_LAPLACIAN4)
+            fail("unknown discretization");
+        if (op->disc == SRMG_LAPLACIAN4 && op->geometry != SRMG_CARTESIAN && c->D != 2)
+            fail("the order-4 cylindrical forms are 2D");
+        FieldBuf &fb1 = field(c, b1), &fb = field(c, b);
+        dispatch(c, t, [&](auto &o) {
+            using T = std::remove_reference_t<decltype(*o.X(fb, 0))>;
+            o.settle(fb);
+            o.settle(fb1);
+            // fill_ghost!(b, l) then rhs_level!(b1, b, l), level by level
+            // (src/streamer.jl:314-318)
+            for (int l = 0; l < c->nlevels; l++) {
+                if (l > 0) o.fill_interp(fb, l, level_args<T>(c, l - 1, 1.0, op, 1.0));
+                o.rhs_level(fb1, fb, l, level_args<T>(c, l, 1.0, op, 1.0), op->disc == SRMG_LAPLACIAN4);
+            }
+        });
+        HIPCHK(hipGetLastError());
     });
 }
 

@@ -66,8 +66,11 @@ class CylindricalGeometry:
 class LaplacianDiscretization:
     D: int
     P: int = 2
-    code: int = 0
     k2: float = 0.0
+
+    @property
+    def code(self) -> int:
+        return 2 if self.P == 4 else 0  # SRMG_LAPLACIAN4 / SRMG_LAPLACIAN
 
 
 @dataclass(frozen=True)
@@ -94,8 +97,8 @@ class _SolveOpts(C.Structure):
                 ("guess_smooth", C.c_int32)]
 
 
-def make_op(geometry, bc, disc, coarse: int = COARSE_GS) -> _Op:
-    if getattr(disc, "P", 2) != 2:
+def make_op(geometry, bc, disc, coarse: int = COARSE_GS, any_order: bool = False) -> _Op:
+    if getattr(disc, "P", 2) != 2 and not any_order:
         raise SrmgError("only order-2 discretizations are implemented on the device")
     op = _Op()
     op.geometry = int(geometry.code)
@@ -127,6 +130,8 @@ def lib():
         "srmg_set_tree": [P, i32, i64] + [P] * 10,
         "srmg_regrid": [P, i32, i64] + [P] * 10 + [P, P, i32, P],
         "srmg_connectivity": [i32, i32, P, P, P, i32] + [P, P, i64] * 4,
+        "srmg_restrict_full": [P, i32],
+        "srmg_rhs": [P, i32, i32, P],
         "srmg_field_create": [P, i32, C.POINTER(i32)],
         "srmg_field_destroy": [P, i32],
         "srmg_upload": [P, i32, P, i32, i32],
@@ -432,6 +437,19 @@ def fill_ghost(u: DeviceField, l: int, conn=None, bc=None):
     _check(lib().srmg_fill_ghost(u.ctx.h, u.id, l, _p(b)))
 
 
+def restrict_full(u: DeviceField):
+    """restrict_full!(u, conn) (src/restrict.jl:43-47) on the device."""
+    _check(lib().srmg_restrict_full(u.ctx.h, u.id))
+
+
+def rhs(b1: DeviceField, b: DeviceField, geometry, bc, lpldisc):
+    """derivs!'s right-hand side loop (src/streamer.jl:314-318): for every
+    level, fill_ghost!(b, l) and rhs_level!(b1, b, l) (src/multigrid.jl:378-388);
+    b1 = b for order 2, the compact R b for LaplacianDiscretization(D, 4)."""
+    op = make_op(geometry, bc, lpldisc, any_order=True)
+    _check(lib().srmg_rhs(b.ctx.h, b1.id, b.id, C.byref(op)))
+
+
 def extrapolate_const(D: int) -> np.ndarray:
     """``ExtrapolateConst()`` (src/boundary.jl:18-21, 49) as a boundary table:
     its getbc returns the mirror cell's value, i.e. every entry +1. The
@@ -463,12 +481,17 @@ def fmg(u, b, r, u1, s, tree=None, conn=None, geometry=None, bc=None, lpldisc=No
 
 
 def poisson_step(u, q1, r, u1, s, geometry, pbc, lpl, *, poisson_iter=1, poisson_fmg=False, nup=2, ndown=2,
-                 ntop=4, freebc=None, coarse=COARSE_GS):
-    """The Poisson block of derivs! (src/streamer.jl:320-333), with the charge
-    already restricted into q1: setfreebc! when a free boundary is given
-    (``srmg.freebc.FreeBC``), then ``poisson_iter`` times the ghost pre-fill of u
-    on every level and one fmg! (``poisson_fmg``) or vcycle!, warm-started from
-    the previous u. (nup, ndown, ntop) is the reference's poisson_level_iter."""
+                 ntop=4, freebc=None, coarse=COARSE_GS, q=None):
+    """The Poisson block of derivs! (src/streamer.jl:309-333). With the net
+    charge ``q`` given: restrict_full!(q), then per level fill_ghost!(q) and
+    q1 = rhs(q) (313-318); without it q1 already holds the right-hand side.
+    Then setfreebc! when a free boundary is given (``srmg.freebc.FreeBC``), and
+    ``poisson_iter`` times the ghost pre-fill of u on every level and one fmg!
+    (``poisson_fmg``) or vcycle!, warm-started from the previous u.
+    (nup, ndown, ntop) is the reference's poisson_level_iter."""
+    if q is not None:
+        restrict_full(q)
+        rhs(q1, q, geometry, pbc, lpl)
     if freebc is not None:
         freebc.setfreebc(q1, r, u1, s, pbc, lpl, poisson_iter=poisson_iter, nup=nup, ndown=ndown, ntop=ntop,
                          poisson_fmg=poisson_fmg)

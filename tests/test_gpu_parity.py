@@ -442,6 +442,61 @@ def test_electric_field_matches_oracle(name):
     np.testing.assert_array_equal(_valid(got, p), fabs[(slice(p.b.G, p.b.G + p.M),) * p.D])
 
 
+@pytest.mark.parametrize("name", ["cfg1", "cyl1-m16", "cyl2-m8", "cart3-m8", "cart3-m16"])
+@pytest.mark.parametrize("order", [2, 4])
+def test_restrict_full_and_rhs_match_oracle(name, order):
+    # derivs!'s right-hand side (src/streamer.jl:313-318): restrict_full!(q),
+    # then per level fill_ghost!(q, l) and rhs_level!(q1, q, l)
+    p = CASES[name]()
+    if p.helm and order == 4:
+        pytest.skip("order 4 is a Laplacian")
+    print(f"case {name} order {order}", flush=True)
+    geom = solver.geometry_from_code(p.geom)
+    disc = solver.LaplacianDiscretization(p.D, order) if not p.helm else _disc(p)
+    ctx, (q, q1, _, _) = _device(p)
+    q.upload(p.b)
+    solver.restrict_full(q)
+    solver.rhs(q1, q, geom, p.bc, disc)
+    gq, g1 = p.b.like(), p.b.like()
+    q.download(gq)
+    q1.download(g1)
+    o = Oracle.for_problem(p)
+    hq, h1 = p.b.copy(), p.b.like()
+    o.restrict_full(hq)
+    o.rhs(h1, hq, order=order)
+    np.testing.assert_array_equal(_valid(gq, p), _valid(hq, p))
+    np.testing.assert_array_equal(_valid(g1, p), _valid(h1, p))
+    if order == 4:
+        assert not np.array_equal(_valid(g1, p), _valid(gq, p))
+
+
+@pytest.mark.parametrize("name", ["cfg1", "cyl2-m8", "cart3-m8"])
+def test_poisson_step_from_the_charge_matches_oracle(name):
+    # derivs!'s Poisson block (src/streamer.jl:309-333) from the net charge q:
+    # restrict_full!, the right-hand side loop, then poisson_iter warm-started
+    # V-cycles; two steps, q rebuilt before each as netcharge! does
+    p = CASES[name]()
+    print(f"case {name}", flush=True)
+    geom, disc = solver.geometry_from_code(p.geom), _disc(p)
+    ctx, (u, q1, r, u1) = _device(p)
+    q = ctx.field(F64)
+    for _ in range(2):
+        q.upload(p.b)
+        solver.poisson_step(u, q1, r, u1, p.s, geom, p.bc, disc, poisson_iter=2, q=q)
+    got = p.b.like()
+    u.download(got)
+    o = Oracle.for_problem(p)
+    uo, ro, u1o = p.b.like(), p.b.like(), p.b.like()
+    for _ in range(2):
+        qo, q1o = p.b.copy(), p.b.like()
+        o.restrict_full(qo)
+        o.rhs(q1o, qo, order=2)
+        for _ in range(2):
+            o.fill_ghost_all(uo)
+            o.vcycle(uo, q1o, ro, u1o, p.s, ndown=2, ntop=4, nup=2)
+    np.testing.assert_array_equal(_valid(got, p), _valid(uo, p))
+
+
 def _face_ghosts(a, D, M, G):
     """the depth-1 ghost layer of every face (corners excluded), per block"""
     out = []
