@@ -142,7 +142,9 @@ int srmg_rhs(srmg_ctx *ctx, int b1, int b, const srmg_op *op);
  * with bc), otherwise it starts at zero; with SRMG_REGRID_RESTRICT a removed
  * block is restrict!'ed into its parent's subblock. A new block's parent must
  * be in the old tree and a removed block's parent must be kept (one level per
- * regrid, as applydelta! does); this is checked before anything changes.
+ * regrid, as applydelta! does); this and the block coordinates are checked
+ * before anything changes (records that srmg_set_tree would refuse leave the
+ * context without a tree, as a failing srmg_set_tree does).
  * Fields not listed are reallocated as in srmg_set_tree. Single-tree
  * contexts. bc may be NULL when no field is interpolated. */
 enum { SRMG_REGRID_INTERP = 1, SRMG_REGRID_RESTRICT = 2 };

@@ -1956,8 +1956,13 @@ static void regrid_fields(srmg_ctx *c, const std::vector<TreeRecords> &tr, const
         }
         for (int l = 0; l < tr[t].nlevels; l++) {
             auto &v = nl[(size_t)t * nln + l];
-            for (int64_t k = tr[t].pairs_off[l]; k < tr[t].pairs_off[l + 1]; k++)
-                v.push_back({morton(tr[t].pairs + k * (D + 1), D), 0});
+            for (int64_t k = tr[t].pairs_off[l]; k < tr[t].pairs_off[l + 1]; k++) {
+                const int64_t *pc = tr[t].pairs + k * (D + 1);
+                for (int d = 0; d < D; d++)
+                    if (pc[d] < 1 || pc[d] > ((int64_t)T.ext1[d] << l))
+                        fail("block coordinate outside the level domain");
+                v.push_back({morton(pc, D), 0});
+            }
             std::sort(v.begin(), v.end());
             for (size_t i = 0; i < v.size(); i++) v[i].second = (int)i;
         }
