@@ -40,8 +40,12 @@ enum { SRMG_F64 = 0, SRMG_F32 = 1 };
 enum { SRMG_CARTESIAN = 0, SRMG_CYL1 = 1, SRMG_CYL2 = 2 };
 /* discretization: LaplacianDiscretization{D,2}, HelmholtzDiscretization{D,2,T}
  * (src/laplacian.jl:14, src/helmholtz.jl:9-11) */
-/* SRMG_LAPLACIAN4: the compact 4th-order LaplacianDiscretization{D,4}; only
- * srmg_rhs accepts it (its smoother, FourColors, is not built). */
+/* SRMG_LAPLACIAN4: the compact 4th-order LaplacianDiscretization{D,4}.
+ * srmg_rhs applies its R (Cartesian and cylindrical); srmg_vcycle runs its
+ * V-cycle with the FourColors smoother (src/multigrid.jl:206-232, 268-295) for
+ * Cartesian geometry, FP64 fields, the reference's level-1 sweeps and trees
+ * whose levels are complete (no coarse-fine faces: their corner and edge
+ * ghosts are not built); the other solve entry points refuse it. */
 enum { SRMG_LAPLACIAN = 0, SRMG_HELMHOLTZ = 1, SRMG_LAPLACIAN4 = 2 };
 /* coarsest-level treatment: the reference's ntop Gauss-Seidel sweeps
  * (src/multigrid.jl:89), or an exact FP64 solve of the level-1 problem

@@ -69,6 +69,7 @@ def lib():
             "ora_freebc": [P, P, P, dbl, dbl, i32, i32, P, P, i64, P],
             "ora_fill_ghost_free": [P, P, P, i32, i32, i32, P],
             "ora_rhs_level": [P, P, P, i32, P, i32],
+            "ora_gs4_level": [P, P, P, C.c_double, C.c_double, i32, P, i32, i32],
         }.items():
             getattr(_LIB, name).argtypes = args
             getattr(_LIB, name).restype = None
@@ -136,7 +137,7 @@ def applydelta_fields(u: np.ndarray, actions, nblocks: int, M: int, G: int, inte
 
 class _Op(C.Structure):
     _fields_ = [("geom", C.c_int), ("helm", C.c_int), ("k2", C.c_double),
-                ("bc", C.c_int8 * 27)]
+                ("bc", C.c_int8 * 27), ("order", C.c_int)]
 
 
 def _ptr(a: np.ndarray):
@@ -148,7 +149,7 @@ class Oracle:
     ``srmg.tree``; fields are ``ScalarBlockField`` objects (or their ``.u``)."""
 
     def __init__(self, tree, conn, M: int, G: int, geom: int, bc, helm: bool = False,
-                 k2: float = 0.0):
+                 k2: float = 0.0, order: int = 2):
         from_tree = _flatten(tree, conn)
         self.D, self.M, self.G = tree.D, M, G
         self.nlevels = len(tree)
@@ -161,6 +162,7 @@ class Oracle:
         self.op.geom = int(geom)
         self.op.helm = int(bool(helm))
         self.op.k2 = float(k2)
+        self.op.order = int(order)  # 4: LaplacianDiscretization{D,4}, Cartesian (box-stencil records)
         bc = np.asarray(bc, np.int8).ravel()
         for i in range(len(bc)):
             self.op.bc[i] = int(bc[i])
@@ -211,6 +213,11 @@ class Oracle:
     def residual_level(self, r, u, b, s, level):
         lib().ora_residual_level(self.h, _ptr(self._a(r)), _ptr(self._a(u)), _ptr(self._a(b)),
                                  s, level, self._opp())
+
+    def gs4_level(self, u, b, omega, s, level, P1, P2):
+        """one FourColors{P1,P2} pass of gauss_seidel! over a level (order 4)."""
+        lib().ora_gs4_level(self.h, _ptr(self._a(u)), _ptr(self._a(b)), float(omega), float(s), level,
+                            self._opp(), int(P1), int(P2))
 
     def rhs_level(self, b1, b, level, order=2):
         """rhs_level!(b1, b, tree[level], lpl, geom) for a discretization of

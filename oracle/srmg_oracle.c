@@ -53,6 +53,7 @@ typedef struct {
     int helm;          /* 0 LaplacianDiscretization{D,2}; 1 HelmholtzDiscretization{D,2} */
     double k2;
     int8_t bc[27];     /* GeneralHomogeneousBoundaryConditions{D}.a, column-major */
+    int order;         /* 4: LaplacianDiscretization{D,4} (Cartesian); otherwise order 2 */
 } ora_op;
 
 static int64_t *dup64(const int64_t *src, int64_t n) {
@@ -132,10 +133,44 @@ static inline double centre_coef(const ora_tree *t, const ora_op *op, double s) 
 static double stencil_sum(const ora_tree *t, const ora_op *op, const double *u,
                           int64_t blk, const int *I, const int64_t *J, double cc);
 
+static double stencil4(const ora_tree *t, const double *u, int64_t blk, const int *I);
+
 static double applystencil(const ora_tree *t, const ora_op *op, const double *u,
                            int64_t blk, const int *I, const int64_t *J, double s) {
+    if (op->order == 4) return stencil4(t, u, blk, I);
     return stencil_sum(t, op, u, blk, I, J, centre_coef(t, op, s));
 }
+
+/* applystencil for LaplacianDiscretization{D,4} x CartesianGeometry
+ * (src/laplacian.jl:133-141, 305-320): the @stencil literal's nonzero terms in
+ * column-major order (first offset fastest), then one division by 6. 2D:
+ * [1 4 1; 4 -20 4; 1 4 1]; 3D: [0 1 0; 1 2 1; 0 1 0] in the outer slices and
+ * [1 2 1; 2 -24 2; 1 2 1] in the middle one. */
+static double stencil4(const ora_tree *t, const double *u, int64_t blk, const int *I) {
+    const int D = t->D;
+    double v = 0.0;
+    int K[3] = {I[0], I[1], I[2]};
+    const int nk = D == 3 ? 1 : 0;
+    for (int k = -nk; k <= nk; k++)
+        for (int j = -1; j <= 1; j++)
+            for (int i = -1; i <= 1; i++) {
+                const int n = (i != 0) + (j != 0) + (k != 0);
+                double a;
+                if (D == 2) a = n == 0 ? -20.0 : (n == 1 ? 4.0 : 1.0);
+                else if (k == 0) a = n == 0 ? -24.0 : (n == 1 ? 2.0 : 1.0);
+                else a = n == 1 ? 2.0 : (n == 2 ? 1.0 : 0.0);
+                if (a == 0.0) continue;
+                K[0] = I[0] + i;
+                K[1] = I[1] + j;
+                if (D == 3) K[2] = I[2] + k;
+                v += a * u[OFF(t, K, blk)];
+            }
+    return v / 6.0;
+}
+
+/* diagelm of LaplacianDiscretization{D,4} x CartesianGeometry
+ * (src/laplacian.jl:151-152, 339-340) */
+static inline double diag4(int D) { return D == 2 ? -20.0 / 6.0 : -4.0; }
 
 /* the star-stencil sum with centre weight cc, in the @stencil order above */
 static double stencil_sum(const ora_tree *t, const ora_op *op, const double *u,
@@ -345,9 +380,50 @@ void ora_gs_level(const ora_tree *t, double *u, const double *b, double omega, d
     }
 }
 
-/* gauss_seidel_iter! for order-2 discretizations (src/multigrid.jl:250-265) */
+/* gauss_seidel!(..., ::FourColors{P1,P2}) (src/multigrid.jl:206-232) over a
+ * level: I1 runs over every other cell in the first two dimensions (from the
+ * first valid one), every cell in the third; in 3D the two parities shift with
+ * I1[3] (1-based with ghosts), so the colours depend on G as in the reference.
+ * u -= omega (Lu + s b) / c, with no fused multiply-add. */
+void ora_gs4_level(const ora_tree *t, double *u, const double *b, double omega, double s,
+                   int level, const ora_op *op, int P1, int P2) {
+    const int D = t->D, M = t->M, G = t->G;
+    const double c = diag4(D);
+    (void)op;
+    int64_t lo = t->pairs_off[level - 1], hi = t->pairs_off[level];
+    for (int64_t k = lo; k < hi; k++) {
+        int64_t blk = t->pairs[k * (D + 1) + D];
+        int n3 = (D >= 3) ? M : 1;
+        for (int a3 = 0; a3 < n3; a3++)
+        for (int i1 = G + 1; i1 <= G + M; i1 += 2)
+        for (int i0 = G + 1; i0 <= G + M; i0 += 2) {
+            int z = G + 1 + a3;
+            int p1 = P1, p2 = P2;
+            if (D == 3) {
+                p1 = (P1 + z) % 2;
+                p2 = (P2 + z) % 2;
+            }
+            int I[3] = {i0 + p1, i1 + p2, z};
+            int64_t o = OFF(t, I, blk);
+            double Lu = stencil4(t, u, blk, I);
+            u[o] -= omega * (Lu + s * b[o]) / c;
+        }
+    }
+}
+
+/* gauss_seidel_iter! (src/multigrid.jl:250-295): order 2 red-black, order 4
+ * the four colours, each colour followed by the copy and boundary ghost fills */
 void ora_gs_iter(const ora_tree *t, double *u, const double *b, double omega, double s,
                  int n, int level, const ora_op *op) {
+    if (op->order == 4) {
+        for (int it = 0; it < n; it++)
+            for (int c = 0; c < 4; c++) {
+                ora_gs4_level(t, u, b, omega, s, level, op, c >> 1, c & 1);
+                ora_fill_ghost_copy(t, u, level);
+                ora_fill_ghost_bnd(t, u, level, op->bc);
+            }
+        return;
+    }
     for (int it = 0; it < n; it++) {
         ora_gs_level(t, u, b, omega, s, level, op, 0);
         ora_fill_ghost_copy(t, u, level);

@@ -174,6 +174,155 @@ __global__ __launch_bounds__(WG) void k_residual(T *__restrict__ r, const T *__r
     r[o] = -(b[o] + Lu / bp.s);
 }
 
+// ---- LaplacianDiscretization{D,4} x CartesianGeometry on complete levels ----
+//
+// box[blk * 3^D + k] for the direction k = sum (dir_d + 1) 3^d: the
+// same-level neighbour block, or -(1 + mask) where the direction leaves the
+// domain in the dimensions of mask. bc27 is the 3^D boundary table.
+
+// depth-1 value at cell c of block blk (c may lie one cell outside the block
+// in any dimensions): the neighbour's copy (fill_ghost_copy!, with diagonal
+// Neighbor records), or at the domain boundary fill_ghost_bnd!'s a[bnd] times
+// the cell mirrored in the dimensions that leave the domain, which in the
+// other dimensions is again a neighbour's copy (src/connectivity.jl:178-231)
+template <int D, int M, typename T>
+__device__ __forceinline__ T boxv(const T *__restrict__ x, const int *__restrict__ box, int blk, int c0, int c1,
+                                  int c2, const T *bc27) {
+    using LY = Lay<D, M>;
+    constexpr int NB3 = D == 3 ? 27 : 9;
+    int c[3] = {c0, c1, c2};
+    int dir[3] = {0, 0, 0};
+    int k = 0, m = 1;
+    bool out = false;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        dir[d] = c[d] < 0 ? -1 : (c[d] >= M ? 1 : 0);
+        k += (dir[d] + 1) * m;
+        m *= 3;
+        out |= dir[d] != 0;
+    }
+    if (!out) return x[(size_t)blk * LY::V + LY::sidx(c[0], c[1], c[2])];
+    int e = box[blk * NB3 + k];
+    T sign = T(1);
+    if (e < 0) {
+        const int mask = -1 - e;
+        int bi = 0, mb = 1;
+        k = 0;
+        m = 1;
+        out = false;
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            const bool bd = (mask >> d) & 1;
+            bi += ((bd ? dir[d] : 0) + 1) * mb;
+            mb *= 3;
+            if (bd) {
+                c[d] = dir[d] < 0 ? -1 - c[d] : 2 * M - 1 - c[d];
+                dir[d] = 0;
+            }
+            k += (dir[d] + 1) * m;
+            m *= 3;
+            out |= dir[d] != 0;
+        }
+        sign = bc27[bi];
+        if (!out) return sign * x[(size_t)blk * LY::V + LY::sidx(c[0], c[1], c[2])];
+        e = box[blk * NB3 + k];
+        if (e < 0) return T(0);  // not on complete levels (the host checks)
+    }
+#pragma unroll
+    for (int d = 0; d < D; d++) c[d] -= dir[d] * M;
+    return sign * x[(size_t)e * LY::V + LY::sidx(c[0], c[1], c[2])];
+}
+
+template <typename T>
+struct Bc27 {
+    T v[27];
+};
+
+// applystencil (src/laplacian.jl:133-141, 305-320): the literal's nonzero
+// terms in column-major order, then one division by 6
+template <int D, int M, typename T>
+__device__ __forceinline__ T stencil4(const T *__restrict__ x, const int *__restrict__ box, int blk, int c0, int c1,
+                                      int c2, const T *bc27) {
+    T v = T(0);
+    constexpr int NK = D == 3 ? 1 : 0;
+#pragma unroll
+    for (int k = -NK; k <= NK; k++)
+#pragma unroll
+        for (int j = -1; j <= 1; j++)
+#pragma unroll
+            for (int i = -1; i <= 1; i++) {
+                const int n = (i != 0) + (j != 0) + (k != 0);
+                const int a = D == 2 ? (n == 0 ? -20 : (n == 1 ? 4 : 1))
+                                     : (k == 0 ? (n == 0 ? -24 : (n == 1 ? 2 : 1)) : (n == 1 ? 2 : (n == 2 ? 1 : 0)));
+                if (a == 0) continue;
+                v += T(a) * boxv<D, M, T>(x, box, blk, c0 + i, c1 + j, c2 + k, bc27);
+            }
+    return v / T(6);
+}
+
+// gauss_seidel!(..., ::FourColors{P1,P2}) (src/multigrid.jl:206-232) over a
+// level: colour cells (x = P1, y = P2 mod 2 from the first valid cell; in 3D
+// both shifted by the 1-based third index with G = 2 ghost layers, the
+// reference's depth), u -= omega (Lu + s b) / diagelm; cells of one colour do
+// not read each other
+template <int D, int M, typename T>
+__global__ __launch_bounds__(WG) void k_gs4_pass(T *__restrict__ u, const T *__restrict__ b,
+                                                 const int *__restrict__ box, int blk0, int nblk, T s, T omega,
+                                                 Bc27<T> bc, int P1, int P2) {
+    using LY = Lay<D, M>;
+    constexpr int H2 = M / 2, NCC = D == 3 ? H2 * H2 * M : H2 * H2;
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lb = gt / NCC;
+    if (lb >= nblk) return;
+    const int q = gt - lb * NCC;
+    const int blk = blk0 + lb;
+    const int i0 = q % H2, i1 = (q / H2) % H2, z = D == 3 ? q / (H2 * H2) : 0;
+    const int p1 = D == 3 ? (P1 + z + 3) & 1 : P1, p2 = D == 3 ? (P2 + z + 3) & 1 : P2;
+    const int x0 = 2 * i0 + p1, x1 = 2 * i1 + p2;
+    const size_t o = (size_t)blk * LY::V + LY::sidx(x0, x1, z);
+    const T Lu = stencil4<D, M, T>(u, box, blk, x0, x1, z, bc.v);
+    const T c = D == 2 ? T(-20.0 / 6.0) : T(-4);
+    u[o] = u[o] - omega * (Lu + s * b[o]) / c;
+}
+
+// residual! (src/multigrid.jl:309-323) with the order-4 operator
+template <int D, int M, typename T>
+__global__ __launch_bounds__(WG) void k_resid4(T *__restrict__ r, const T *__restrict__ u, const T *__restrict__ b,
+                                               const int *__restrict__ box, int blk0, int nblk, T s, Bc27<T> bc) {
+    using LY = Lay<D, M>;
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lb = gt / LY::V;
+    if (lb >= nblk) return;
+    const int blk = blk0 + lb, q = gt - lb * LY::V;
+    int c0, c1, c2;
+    LY::decode(q, c0, c1, c2);
+    const size_t o = (size_t)blk * LY::V + q;
+    const T Lu = stencil4<D, M, T>(u, box, blk, c0, c1, c2, bc.v);
+    r[o] = -(b[o] + Lu / s);
+}
+
+// residual_postrestrict_level! (src/multigrid.jl:112-119, 331-344) with the
+// order-4 operator on the coarse cells covered by the fine blocks
+template <int D, int M, typename T>
+__global__ __launch_bounds__(WG) void k_postrestrict4(T *__restrict__ bout, const T *__restrict__ u,
+                                                      const T *__restrict__ r, const int *__restrict__ box,
+                                                      const int *__restrict__ parent, const int *__restrict__ sub,
+                                                      int fblk0, int nfine, T s, Bc27<T> bc) {
+    using LY = Lay<D, M>;
+    constexpr int NS = LY::V / LY::NC;
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lb = gt / NS;
+    if (lb >= nfine) return;
+    const int fb = fblk0 + lb, j = gt - lb * NS;
+    const int j0 = j % LY::MH, j1 = (j / LY::MH) % LY::MH, j2 = (D == 3) ? j / (LY::MH * LY::MH) : 0;
+    const int pb = parent[fb], sb = sub[fb];
+    const int X0 = (sb & 1) * LY::MH + j0, X1 = ((sb >> 1) & 1) * LY::MH + j1;
+    const int X2 = (D == 3) ? ((sb >> 2) & 1) * LY::MH + j2 : 0;
+    const size_t o = (size_t)pb * LY::V + LY::sidx(X0, X1, X2);
+    const T Lu = stencil4<D, M, T>(u, box, pb, X0, X1, X2, bc.v);
+    bout[o] = -(r[o] + Lu / s);
+}
+
 // rhs! (src/multigrid.jl:363-388) of the compact 4th-order Laplacian
 // (LaplacianDiscretization{D,4}, src/laplacian.jl:142-149, 167-175, 187-195,
 // 322-337): b1 = (1/12) [0 1 0; 1 8 1; 0 1 0] b in 2D (the radial weights of

@@ -132,7 +132,10 @@ struct srmg_ctx {
     // kind, fields, scales, operators, options, tile shapes, kernel switches);
     // cleared whenever a buffer they point to may move or change its role
     std::unordered_map<std::string, std::unique_ptr<Graph>> graphs;
-    uint64_t graph_gen = 0;  // bumped when coarse-solve matrices are rebuilt (older graphs point to freed ones)
+    uint64_t graph_gen = 0;
+    // the order-4 operator's box neighbour table (built on first use; see box_table)
+    DevBuf d_box;
+    bool box_ok = false;  // bumped when coarse-solve matrices are rebuilt (older graphs point to freed ones)
 };
 
 static void alloc_field(srmg_ctx *c, FieldBuf &f);
@@ -201,6 +204,7 @@ struct TreeRecords {
 
 static void set_forest(srmg_ctx *c, const std::vector<TreeRecords> &tr) {
     c->graphs.clear();
+    c->box_ok = false;
     const int D = c->D, M = c->M, ND = 2 * D;
     const int nt = (int)tr.size();
     if (nt < 1) fail("a forest needs at least one tree");
@@ -936,6 +940,49 @@ struct Ops {
         else
             hipLaunchKernelGGL((k_copy<T>), dim3(nwg((int64_t)n)), dim3(WG), 0, c->stream, X(b1, l) + o,
                                X(b, l) + o, n);
+    }
+    // ---- LaplacianDiscretization{D,4}: vcycle! (src/multigrid.jl:60-106) with
+    // the four-colour smoother on complete levels, step by step ----
+    void gs4_iter(FieldBuf &u, FieldBuf &b, double sl, double omega, int n, int l, const Bc27<T> &bc) {
+        const LevelInfo &L = c->lv[l];
+        if (!L.nblk) return;
+        constexpr int NCC = D == 3 ? (M / 2) * (M / 2) * M : (M / 2) * (M / 2);
+        for (int it = 0; it < n; it++)
+            for (int col = 0; col < 4; col++)
+                hipLaunchKernelGGL((k_gs4_pass<D, M, T>), dim3(nwg((int64_t)L.nblk * NCC)), dim3(WG), 0, c->stream,
+                                   X(u, l), X(b, l), c->d_box.as<int>(), L.blk0, L.nblk, (T)sl, (T)omega, bc,
+                                   col >> 1, col & 1);
+    }
+    void vcycle4(FieldBuf &u, FieldBuf &b, FieldBuf &r, FieldBuf &u1, double s, const srmg_op *op, int lmax,
+                 int ndown, int ntop, int nup, double omega) {
+        Bc27<T> bc{};
+        for (int k = 0; k < 27; k++) bc.v[k] = (T)op->bc[k];
+        if (lmax == 0) lmax = c->nlevels;
+        for (FieldBuf *f : {&u, &b, &r, &u1}) settle(*f);
+        const LevelArgs<T> ac0 = level_args<T>(c, 0, 1.0, op, 1.0);  // mirror signs for the face reads of interp!
+        for (int l = lmax - 1; l >= 1; l--) {
+            const double sl = s / (double)((int64_t)1 << (2 * l));
+            const LevelInfo &L = c->lv[l];
+            gs4_iter(u, b, sl, omega, ndown, l, bc);
+            restrict_(u, l);
+            copy(u1, u, l - 1);
+            if (L.nblk)
+                hipLaunchKernelGGL((k_resid4<D, M, T>), dim3(nwg((int64_t)L.nblk * LY::V)), dim3(WG), 0, c->stream,
+                                   X(r, l), X(u, l), X(b, l), c->d_box.as<int>(), L.blk0, L.nblk, (T)sl, bc);
+            restrict_(r, l);
+            if (L.nblk)
+                hipLaunchKernelGGL((k_postrestrict4<D, M, T>), dim3(nwg((int64_t)L.nblk * (LY::V / LY::NC))),
+                                   dim3(WG), 0, c->stream, X(b, l - 1), X(u, l - 1), X(r, l - 1), c->d_box.as<int>(),
+                                   c->d_parent.as<int>(), c->d_sub.as<int>(), L.blk0, L.nblk, (T)(4 * sl), bc);
+        }
+        gs4_iter(u, b, s, omega, ntop, 0, bc);
+        for (int l = 1; l < lmax; l++) {
+            const double sl = s / (double)((int64_t)1 << (2 * l));
+            diff(u1, u, u1, l - 1);
+            LevelArgs<T> ac = ac0;
+            interp_add(u, u1, l, ac);
+            gs4_iter(u, b, sl, omega, nup, l, bc);
+        }
     }
     void postrestrict(FieldBuf &b, FieldBuf &u, FieldBuf &r, int l, const LevelArgs<T> &ac) {
         const LevelInfo &L = c->lv[l];
@@ -2247,11 +2294,76 @@ static void do_vcycle(srmg_ctx *c, int u, int b, int r, int u1, int lmax, int nd
     HIPCHK(hipGetLastError());
 }
 
+// The box neighbour table of the order-4 operator (k_gs4_pass): every level
+// must be complete (no coarse-fine faces, whose corner and edge ghosts are not
+// built), single tree.
+static void box_table(srmg_ctx *c) {
+    if (c->box_ok) return;
+    const int D = c->D, NB3 = D == 3 ? 27 : 9;
+    if (c->trees.size() != 1) fail("the order-4 operator runs on single-tree contexts");
+    std::vector<int> box((size_t)c->nb * NB3, 0);
+    for (int l = 0; l < c->nlevels; l++) {
+        const LevelInfo &L = c->lv[l];
+        int64_t ext[3] = {1, 1, 1}, n = 1;
+        for (int d = 0; d < D; d++) {
+            ext[d] = (int64_t)c->trees[0].ext1[d] << l;
+            n *= ext[d];
+        }
+        if (L.nblk != n) fail("the order-4 operator needs complete levels (no coarse-fine faces)");
+        std::vector<int> at((size_t)n, -1);
+        auto lin = [&](const int64_t *q) {
+            int64_t k = 0, m = 1;
+            for (int d = 0; d < D; d++) {
+                k += (q[d] - 1) * m;
+                m *= ext[d];
+            }
+            return k;
+        };
+        for (int bk = L.blk0; bk < L.blk0 + L.nblk; bk++) {
+            int64_t q[3] = {1, 1, 1};
+            for (int d = 0; d < D; d++) q[d] = c->h_coord[(size_t)bk * D + d];
+            at[(size_t)lin(q)] = bk;
+        }
+        for (int bk = L.blk0; bk < L.blk0 + L.nblk; bk++)
+            for (int k = 0; k < NB3; k++) {
+                int64_t q[3] = {1, 1, 1};
+                int mask = 0;
+                for (int d = 0; d < D; d++) {
+                    const int dir = (k / (d == 0 ? 1 : (d == 1 ? 3 : 9))) % 3 - 1;
+                    q[d] = c->h_coord[(size_t)bk * D + d] + dir;
+                    if (q[d] < 1 || q[d] > ext[d]) mask |= 1 << d;
+                }
+                box[(size_t)bk * NB3 + k] = mask ? -(1 + mask) : at[(size_t)lin(q)];
+            }
+    }
+    c->d_box.alloc(box.size() * sizeof(int));
+    HIPCHK(hipMemcpy(c->d_box.p, box.data(), box.size() * sizeof(int), hipMemcpyHostToDevice));
+    c->box_ok = true;
+}
+
+static void do_vcycle4(srmg_ctx *c, int u, int b, int r, int u1, double s, const srmg_op *op, int lmax, int ndown,
+                       int ntop, int nup, double omega) {
+    if (op->geometry != SRMG_CARTESIAN) fail("the order-4 operator is built for Cartesian geometry");
+    if (op->coarse != SRMG_COARSE_GS) fail("the order-4 operator uses the reference's level-1 sweeps");
+    if (lmax < 0 || lmax > c->nlevels) fail("lmax out of range");
+    const int t = same_type(c, {u, b, r, u1});
+    if (t != SRMG_F64) fail("the order-4 operator runs in FP64");
+    box_table(c);
+    dispatch(c, t, [&](auto &o) {
+        o.vcycle4(field(c, u), field(c, b), field(c, r), field(c, u1), s, op, lmax, ndown, ntop, nup, omega);
+    });
+    HIPCHK(hipGetLastError());
+}
+
 int srmg_vcycle(srmg_ctx *c, int u, int b, int r, int u1, double s, const srmg_op *op, int lmax, int ndown,
                 int ntop, int nup, double omega) {
     return guarded([&] {
         check_tree(c);
         if (!op) fail("null op");
+        if (op->disc == SRMG_LAPLACIAN4) {
+            do_vcycle4(c, u, b, r, u1, s, op, lmax, ndown, ntop, nup, omega);
+            return;
+        }
         prepare(c, &s, op, omega, true);
         do_vcycle(c, u, b, r, u1, lmax, ndown, ntop, nup, omega);
     });

@@ -1,0 +1,58 @@
+"""Order-4 V-cycles on the device (LaplacianDiscretization{D,4}, Cartesian,
+complete levels: the four-colour smoother, the 9- and 19-point operator with
+its corner and edge ghosts) against the oracle, bit for bit; the right-hand
+side R q first; trees with coarse-fine faces are refused."""
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")  # noqa: F841  (torch's HIP runtime first)
+
+from test_order4 import complete_problem, order4_oracle
+from srmg import configs, solver
+from srmg.solver import F64
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "2d-m16": lambda: complete_problem(2, (2, 3), 16, levels=3, bcd=((1, -1), (-1, 1))),
+    "2d-m8": lambda: complete_problem(2, (3, 2), 8, levels=2, bcd=((-1, -1), (1, 1))),
+    "3d-m8": lambda: complete_problem(3, (2, 2, 3), 8, levels=2, bcd=((1, 1), (-1, -1), (-1, 1))),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_order4_vcycles_match_oracle(name):
+    p = CASES[name]()
+    print(f"case {name}: {p.tree.nblocks()} blocks", flush=True)
+    geom, disc = solver.CartesianGeometry(), solver.LaplacianDiscretization(p.D, 4)
+    ctx = solver.Context(p.tree, p.conn, p.M)
+    u, q, q1, r, u1 = (ctx.field(F64) for _ in range(5))
+    q.upload(p.b)
+    solver.rhs(q1, q, geom, p.bc, disc)
+    solver.fill_ghost_all(u, p.bc)
+    for _ in range(3):
+        solver.vcycle(u, q1, r, u1, p.s, p.tree, p.conn, geom, p.bc, disc, ndown=2, ntop=4, nup=2)
+    got, gb = p.b.like(), p.b.like()
+    u.download(got)
+    q1.download(gb)
+    o = order4_oracle(p)
+    uo, qo, q1o, ro, u1o = p.b.like(), p.b.copy(), p.b.like(), p.b.like(), p.b.like()
+    o.rhs(q1o, qo, order=4)
+    for _ in range(3):
+        o.fill_ghost_all(uo)
+        o.vcycle(uo, q1o, ro, u1o, p.s, ndown=2, ntop=4, nup=2)
+    G, M = p.b.G, p.M
+    vs = (slice(G, G + M),) * p.D
+    np.testing.assert_array_equal(got.u[vs], uo.u[vs])
+    np.testing.assert_array_equal(gb.u[vs], q1o.u[vs])  # the FAS right-hand side too
+    assert np.max(np.abs(uo.u[vs])) > 0
+
+
+def test_order4_refuses_coarse_fine_faces():
+    p = configs.make_cfg1(0)
+    ctx = solver.Context(p.tree, p.conn, p.M)
+    u, b, r, u1 = (ctx.field(F64) for _ in range(4))
+    with pytest.raises(solver.SrmgError, match="complete levels"):
+        solver.vcycle(u, b, r, u1, p.s, p.tree, p.conn, solver.CartesianGeometry(), p.bc,
+                      solver.LaplacianDiscretization(2, 4))
