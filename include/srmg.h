@@ -45,7 +45,8 @@ enum { SRMG_CARTESIAN = 0, SRMG_CYL1 = 1, SRMG_CYL2 = 2 };
  * V-cycle with the FourColors smoother (src/multigrid.jl:206-232, 268-295) for
  * Cartesian geometry, FP64 fields, the reference's level-1 sweeps and trees
  * whose levels are complete (no coarse-fine faces: their corner and edge
- * ghosts are not built); the other solve entry points refuse it. */
+ * ghosts are not built), and srmg_resnorm its residual norm (r receives the
+ * residual, as resnorm! overwrites r); the other solve entry points refuse it. */
 enum { SRMG_LAPLACIAN = 0, SRMG_HELMHOLTZ = 1, SRMG_LAPLACIAN4 = 2 };
 /* coarsest-level treatment: the reference's ntop Gauss-Seidel sweeps
  * (src/multigrid.jl:89), or an exact FP64 solve of the level-1 problem

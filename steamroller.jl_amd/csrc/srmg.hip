@@ -984,6 +984,22 @@ struct Ops {
             gs4_iter(u, b, sl, omega, nup, l, bc);
         }
     }
+    // resnorm! (src/multigrid.jl:391-417) with the order-4 operator: the
+    // residual of every level into r (the reference overwrites r too), then
+    // the weighted leaf sum of its squares in bnorm_partials' fixed order
+    void resnorm4(FieldBuf &r, FieldBuf &u, FieldBuf &b, double s, const srmg_op *op) {
+        Bc27<T> bc{};
+        for (int k = 0; k < 27; k++) bc.v[k] = (T)op->bc[k];
+        for (FieldBuf *f : {&u, &b, &r}) settle(*f);
+        for (int l = 0; l < c->nlevels; l++) {
+            const LevelInfo &L = c->lv[l];
+            if (!L.nblk) continue;
+            const double sl = s / (double)((int64_t)1 << (2 * l));
+            hipLaunchKernelGGL((k_resid4<D, M, T>), dim3(nwg((int64_t)L.nblk * LY::V)), dim3(WG), 0, c->stream,
+                               X(r, l), X(u, l), X(b, l), c->d_box.as<int>(), L.blk0, L.nblk, (T)sl, bc);
+        }
+        bnorm_partials(r);
+    }
     void postrestrict(FieldBuf &b, FieldBuf &u, FieldBuf &r, int l, const LevelArgs<T> &ac) {
         const LevelInfo &L = c->lv[l];
         if (!L.nblk) return;
@@ -2404,8 +2420,15 @@ int srmg_resnorm(srmg_ctx *c, int r, int b, int u, double s, const srmg_op *op, 
     return guarded([&] {
         check_tree(c);
         if (!op || !out) fail("null argument");
-        (void)r;
         if (c->trees.size() != 1) fail("several trees: use srmg_resnorm_batched");
+        if (op->disc == SRMG_LAPLACIAN4) {
+            if (op->geometry != SRMG_CARTESIAN) fail("the order-4 operator is built for Cartesian geometry");
+            if (same_type(c, {r, b, u}) != SRMG_F64) fail("the order-4 operator runs in FP64");
+            box_table(c);
+            dispatch(c, SRMG_F64, [&](auto &o) { o.resnorm4(field(c, r), field(c, u), field(c, b), s, op); });
+            out[0] = read_norms(c)[0];
+            return;
+        }
         prepare(c, &s, op, c->pomega, true);
         do_resnorm(c, b, u, out);
     });

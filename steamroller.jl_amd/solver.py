@@ -509,8 +509,9 @@ def poisson_step(u, q1, r, u1, s, geometry, pbc, lpl, *, poisson_iter=1, poisson
 
 def resnorm(r, b, u, s, tree=None, conn=None, geometry=None, bc=None, lpldisc=None, *,
             coarse=COARSE_GS) -> float:
-    """resnorm! (src/multigrid.jl:391-417)."""
-    op = make_op(geometry, bc, lpldisc, coarse)
+    """resnorm! (src/multigrid.jl:391-417); with LaplacianDiscretization(D, 4)
+    r receives the residual."""
+    op = make_op(geometry, bc, lpldisc, coarse, any_order=True)
     out = C.c_double()
     _check(lib().srmg_resnorm(u.ctx.h, r.id, b.id, u.id, float(s), C.byref(op), C.byref(out)))
     return out.value

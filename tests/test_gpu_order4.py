@@ -56,3 +56,32 @@ def test_order4_refuses_coarse_fine_faces():
     with pytest.raises(solver.SrmgError, match="complete levels"):
         solver.vcycle(u, b, r, u1, p.s, p.tree, p.conn, solver.CartesianGeometry(), p.bc,
                       solver.LaplacianDiscretization(2, 4))
+
+
+@pytest.mark.parametrize("name", ["2d-m16", "3d-m8"])
+def test_order4_resnorm_matches_oracle(name):
+    # resnorm! with the order-4 operator after each V-cycle, against the oracle
+    # to 1e-12 (the leaf sum runs in another fixed order); with the
+    # reference's few level-1 sweeps the norm falls slowly on these trees
+    p = CASES[name]()
+    geom, disc = solver.CartesianGeometry(), solver.LaplacianDiscretization(p.D, 4)
+    ctx = solver.Context(p.tree, p.conn, p.M)
+    u, q, q1, r, u1 = (ctx.field(F64) for _ in range(5))
+    q.upload(p.b)
+    solver.rhs(q1, q, geom, p.bc, disc)
+    o = order4_oracle(p)
+    uo, qo, q1o, ro, u1o = p.b.like(), p.b.copy(), p.b.like(), p.b.like(), p.b.like()
+    o.rhs(q1o, qo, order=4)
+    hist = []
+    for k in range(4):
+        if k:
+            solver.vcycle(u, q1, r, u1, p.s, p.tree, p.conn, geom, p.bc, disc, ndown=2, ntop=4, nup=2)
+            o.fill_ghost_all(uo)
+            o.vcycle(uo, q1o, ro, u1o, p.s, ndown=2, ntop=4, nup=2)
+        got = solver.resnorm(r, q1, u, p.s, geometry=geom, bc=p.bc, lpldisc=disc)
+        o.fill_ghost_all(uo)
+        want = o.resnorm(ro, q1o, uo, p.s)
+        print(f"{name} cycle {k}: {got:.6e} (oracle {want:.6e})", flush=True)
+        assert abs(got - want) <= 1e-12 * want
+        hist.append(got)
+    assert hist[-1] < hist[0]
