@@ -43,7 +43,7 @@ enum { SRMG_CARTESIAN = 0, SRMG_CYL1 = 1, SRMG_CYL2 = 2 };
 /* SRMG_LAPLACIAN4: the compact 4th-order LaplacianDiscretization{D,4}.
  * srmg_rhs applies its R (Cartesian and cylindrical); srmg_vcycle runs its
  * V-cycle with the FourColors smoother (src/multigrid.jl:206-232, 268-295) for
- * Cartesian geometry, FP64 fields, the reference's level-1 sweeps and trees
+ * Cartesian and cylindrical geometry, FP64 fields, the reference's level-1 sweeps and trees
  * whose levels are complete (no coarse-fine faces: their corner and edge
  * ghosts are not built), and srmg_resnorm its residual norm (r receives the
  * residual, as resnorm! overwrites r); the other solve entry points refuse it. */

@@ -134,9 +134,11 @@ static double stencil_sum(const ora_tree *t, const ora_op *op, const double *u,
                           int64_t blk, const int *I, const int64_t *J, double cc);
 
 static double stencil4(const ora_tree *t, const double *u, int64_t blk, const int *I);
+static double stencil4cyl(const ora_tree *t, int N, const double *u, int64_t blk, const int *I, int64_t i);
 
 static double applystencil(const ora_tree *t, const ora_op *op, const double *u,
                            int64_t blk, const int *I, const int64_t *J, double s) {
+    if (op->order == 4 && op->geom) return stencil4cyl(t, op->geom - 1, u, blk, I, J[op->geom - 1]);
     if (op->order == 4) return stencil4(t, u, blk, I);
     return stencil_sum(t, op, u, blk, I, J, centre_coef(t, op, s));
 }
@@ -171,6 +173,47 @@ static double stencil4(const ora_tree *t, const double *u, int64_t blk, const in
 /* diagelm of LaplacianDiscretization{D,4} x CartesianGeometry
  * (src/laplacian.jl:151-152, 339-340) */
 static inline double diag4(int D) { return D == 2 ? -20.0 / 6.0 : -4.0; }
+
+/* LaplacianDiscretization{2,4} x CylindricalGeometry{N+1} (src/laplacian.jl:156-200)
+ * at 1-based global radial index i: the 3x3 literal, entry (a, b) at offset
+ * (a - 1, b - 1) with a the first dimension, visited column-major; the
+ * rational coefficients are integer ratios rounded once, the sum divided by 6 */
+static void cyl4_coefs(int N, int64_t i, double w[3][3]) {
+    const double lo = (double)(2 * (i - 1)) / (double)(2 * i - 1), hi = (double)(2 * i) / (double)(2 * i - 1);
+    const int64_t q = 2 * i - 1;
+    const double pm = (double)(4 * (-2 + 9 * i - 16 * i * i + 8 * i * i * i)) / (double)(q * q * q);
+    const double pp = (double)(4 * (1 + i - 8 * i * i + 8 * i * i * i)) / (double)(q * q * q);
+    const double cc = (double)(-16 * (1 - 5 * i + 5 * i * i)) / (double)((1 - 2 * i) * (1 - 2 * i));
+    /* w[a][b]: a = offset + 1 in the first dimension, b in the second */
+    if (N == 0) {  /* radial first dimension: rows are radial */
+        w[0][0] = lo; w[0][1] = pm; w[0][2] = lo;
+        w[1][0] = 4;  w[1][1] = cc; w[1][2] = 4;
+        w[2][0] = hi; w[2][1] = pp; w[2][2] = hi;
+    } else {       /* radial second dimension: columns are radial */
+        w[0][0] = lo; w[0][1] = 4;  w[0][2] = hi;
+        w[1][0] = pm; w[1][1] = cc; w[1][2] = pp;
+        w[2][0] = lo; w[2][1] = 4;  w[2][2] = hi;
+    }
+}
+
+static double stencil4cyl(const ora_tree *t, int N, const double *u, int64_t blk, const int *I, int64_t i) {
+    double w[3][3];
+    cyl4_coefs(N, i, w);
+    double v = 0.0;
+    int K[3] = {I[0], I[1], I[2]};
+    for (int b = 0; b < 3; b++)
+        for (int a = 0; a < 3; a++) {
+            K[0] = I[0] + a - 1;
+            K[1] = I[1] + b - 1;
+            v += w[a][b] * u[OFF(t, K, blk)];
+        }
+    return v / 6.0;
+}
+
+/* its diagelm: -16 (1 - 5i + 5i^2) / (1 - 2i)^2 / 6 (src/laplacian.jl:197-201) */
+static inline double diag4cyl(int64_t i) {
+    return (double)(-16 * (1 - 5 * i + 5 * i * i)) / (double)((1 - 2 * i) * (1 - 2 * i)) / 6.0;
+}
 
 /* the star-stencil sum with centre weight cc, in the @stencil order above */
 static double stencil_sum(const ora_tree *t, const ora_op *op, const double *u,
@@ -388,8 +431,7 @@ void ora_gs_level(const ora_tree *t, double *u, const double *b, double omega, d
 void ora_gs4_level(const ora_tree *t, double *u, const double *b, double omega, double s,
                    int level, const ora_op *op, int P1, int P2) {
     const int D = t->D, M = t->M, G = t->G;
-    const double c = diag4(D);
-    (void)op;
+    const double c0 = diag4(D);
     int64_t lo = t->pairs_off[level - 1], hi = t->pairs_off[level];
     for (int64_t k = lo; k < hi; k++) {
         int64_t blk = t->pairs[k * (D + 1) + D];
@@ -405,7 +447,10 @@ void ora_gs4_level(const ora_tree *t, double *u, const double *b, double omega, 
             }
             int I[3] = {i0 + p1, i1 + p2, z};
             int64_t o = OFF(t, I, blk);
-            double Lu = stencil4(t, u, blk, I);
+            int64_t J[3];
+            global_index(t, blk, I, J);
+            double Lu = applystencil(t, op, u, blk, I, J, s);
+            double c = op->geom ? diag4cyl(J[op->geom - 1]) : c0;
             u[o] -= omega * (Lu + s * b[o]) / c;
         }
     }

@@ -260,6 +260,58 @@ __device__ __forceinline__ T stencil4(const T *__restrict__ x, const int *__rest
     return v / T(6);
 }
 
+// LaplacianDiscretization{2,4} x CylindricalGeometry{N+1} (src/laplacian.jl:
+// 156-201) at 1-based global radial index i: the 3x3 literal's entries
+// (integer ratios rounded once), visited column-major, divided by 6; and its
+// diagelm
+template <int M, typename T>
+__device__ __forceinline__ T stencil4cyl(const T *__restrict__ x, const int *__restrict__ box, int blk, int c0,
+                                         int c1, const T *bc27, int N, long long i) {
+    const long long q = 2 * i - 1;
+    const T lo = (T)(2 * (i - 1)) / (T)q, hi = (T)(2 * i) / (T)q;
+    const T pm = (T)(4 * (-2 + 9 * i - 16 * i * i + 8 * i * i * i)) / (T)(q * q * q);
+    const T pp = (T)(4 * (1 + i - 8 * i * i + 8 * i * i * i)) / (T)(q * q * q);
+    const T cc = (T)(-16 * (1 - 5 * i + 5 * i * i)) / (T)((1 - 2 * i) * (1 - 2 * i));
+    T w[3][3];
+    if (N == 0) {
+        w[0][0] = lo; w[0][1] = pm; w[0][2] = lo;
+        w[1][0] = T(4); w[1][1] = cc; w[1][2] = T(4);
+        w[2][0] = hi; w[2][1] = pp; w[2][2] = hi;
+    } else {
+        w[0][0] = lo; w[0][1] = T(4); w[0][2] = hi;
+        w[1][0] = pm; w[1][1] = cc; w[1][2] = pp;
+        w[2][0] = lo; w[2][1] = T(4); w[2][2] = hi;
+    }
+    T v = T(0);
+#pragma unroll
+    for (int b = 0; b < 3; b++)
+#pragma unroll
+        for (int a = 0; a < 3; a++) v += w[a][b] * boxv<2, M, T>(x, box, blk, c0 + a - 1, c1 + b - 1, 0, bc27);
+    return v / T(6);
+}
+
+template <typename T>
+__device__ __forceinline__ T diag4cyl(long long i) {
+    return (T)(-16 * (1 - 5 * i + 5 * i * i)) / (T)((1 - 2 * i) * (1 - 2 * i)) / T(6);
+}
+
+// the order-4 operator at cell (c0, c1, c2) of blk: Cartesian (rd < 0) or
+// cylindrical with radial dimension rd (global radial index from coord)
+template <int D, int M, typename T>
+__device__ __forceinline__ T apply4(const T *__restrict__ x, const int *__restrict__ box,
+                                    const int *__restrict__ coord, int blk, int c0, int c1, int c2, const T *bc27,
+                                    int rd, T &diag) {
+    if constexpr (D == 2) {
+        if (rd >= 0) {
+            const long long i = (long long)(rd == 0 ? c0 : c1) + 1 + (long long)M * (coord[blk * D + rd] - 1);
+            diag = diag4cyl<T>(i);
+            return stencil4cyl<M, T>(x, box, blk, c0, c1, bc27, rd, i);
+        }
+    }
+    diag = D == 2 ? T(-20.0 / 6.0) : T(-4);
+    return stencil4<D, M, T>(x, box, blk, c0, c1, c2, bc27);
+}
+
 // gauss_seidel!(..., ::FourColors{P1,P2}) (src/multigrid.jl:206-232) over a
 // level: colour cells (x = P1, y = P2 mod 2 from the first valid cell; in 3D
 // both shifted by the 1-based third index with G = 2 ghost layers, the
@@ -267,8 +319,8 @@ __device__ __forceinline__ T stencil4(const T *__restrict__ x, const int *__rest
 // not read each other
 template <int D, int M, typename T>
 __global__ __launch_bounds__(WG) void k_gs4_pass(T *__restrict__ u, const T *__restrict__ b,
-                                                 const int *__restrict__ box, int blk0, int nblk, T s, T omega,
-                                                 Bc27<T> bc, int P1, int P2) {
+                                                 const int *__restrict__ box, const int *__restrict__ coord, int rd,
+                                                 int blk0, int nblk, T s, T omega, Bc27<T> bc, int P1, int P2) {
     using LY = Lay<D, M>;
     constexpr int H2 = M / 2, NCC = D == 3 ? H2 * H2 * M : H2 * H2;
     const int gt = blockIdx.x * blockDim.x + threadIdx.x;
@@ -280,15 +332,16 @@ __global__ __launch_bounds__(WG) void k_gs4_pass(T *__restrict__ u, const T *__r
     const int p1 = D == 3 ? (P1 + z + 3) & 1 : P1, p2 = D == 3 ? (P2 + z + 3) & 1 : P2;
     const int x0 = 2 * i0 + p1, x1 = 2 * i1 + p2;
     const size_t o = (size_t)blk * LY::V + LY::sidx(x0, x1, z);
-    const T Lu = stencil4<D, M, T>(u, box, blk, x0, x1, z, bc.v);
-    const T c = D == 2 ? T(-20.0 / 6.0) : T(-4);
+    T c;
+    const T Lu = apply4<D, M, T>(u, box, coord, blk, x0, x1, z, bc.v, rd, c);
     u[o] = u[o] - omega * (Lu + s * b[o]) / c;
 }
 
 // residual! (src/multigrid.jl:309-323) with the order-4 operator
 template <int D, int M, typename T>
 __global__ __launch_bounds__(WG) void k_resid4(T *__restrict__ r, const T *__restrict__ u, const T *__restrict__ b,
-                                               const int *__restrict__ box, int blk0, int nblk, T s, Bc27<T> bc) {
+                                               const int *__restrict__ box, const int *__restrict__ coord, int rd,
+                                               int blk0, int nblk, T s, Bc27<T> bc) {
     using LY = Lay<D, M>;
     const int gt = blockIdx.x * blockDim.x + threadIdx.x;
     const int lb = gt / LY::V;
@@ -297,7 +350,8 @@ __global__ __launch_bounds__(WG) void k_resid4(T *__restrict__ r, const T *__res
     int c0, c1, c2;
     LY::decode(q, c0, c1, c2);
     const size_t o = (size_t)blk * LY::V + q;
-    const T Lu = stencil4<D, M, T>(u, box, blk, c0, c1, c2, bc.v);
+    T c;
+    const T Lu = apply4<D, M, T>(u, box, coord, blk, c0, c1, c2, bc.v, rd, c);
     r[o] = -(b[o] + Lu / s);
 }
 
@@ -306,6 +360,7 @@ __global__ __launch_bounds__(WG) void k_resid4(T *__restrict__ r, const T *__res
 template <int D, int M, typename T>
 __global__ __launch_bounds__(WG) void k_postrestrict4(T *__restrict__ bout, const T *__restrict__ u,
                                                       const T *__restrict__ r, const int *__restrict__ box,
+                                                      const int *__restrict__ coord, int rd,
                                                       const int *__restrict__ parent, const int *__restrict__ sub,
                                                       int fblk0, int nfine, T s, Bc27<T> bc) {
     using LY = Lay<D, M>;
@@ -319,7 +374,8 @@ __global__ __launch_bounds__(WG) void k_postrestrict4(T *__restrict__ bout, cons
     const int X0 = (sb & 1) * LY::MH + j0, X1 = ((sb >> 1) & 1) * LY::MH + j1;
     const int X2 = (D == 3) ? ((sb >> 2) & 1) * LY::MH + j2 : 0;
     const size_t o = (size_t)pb * LY::V + LY::sidx(X0, X1, X2);
-    const T Lu = stencil4<D, M, T>(u, box, pb, X0, X1, X2, bc.v);
+    T c;
+    const T Lu = apply4<D, M, T>(u, box, coord, pb, X0, X1, X2, bc.v, rd, c);
     bout[o] = -(r[o] + Lu / s);
 }
 

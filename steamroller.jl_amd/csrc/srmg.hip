@@ -943,6 +943,7 @@ struct Ops {
     }
     // ---- LaplacianDiscretization{D,4}: vcycle! (src/multigrid.jl:60-106) with
     // the four-colour smoother on complete levels, step by step ----
+    int rd4 = -1;  // radial dimension of the order-4 operator (set by vcycle4 / resnorm4)
     void gs4_iter(FieldBuf &u, FieldBuf &b, double sl, double omega, int n, int l, const Bc27<T> &bc) {
         const LevelInfo &L = c->lv[l];
         if (!L.nblk) return;
@@ -950,13 +951,14 @@ struct Ops {
         for (int it = 0; it < n; it++)
             for (int col = 0; col < 4; col++)
                 hipLaunchKernelGGL((k_gs4_pass<D, M, T>), dim3(nwg((int64_t)L.nblk * NCC)), dim3(WG), 0, c->stream,
-                                   X(u, l), X(b, l), c->d_box.as<int>(), L.blk0, L.nblk, (T)sl, (T)omega, bc,
-                                   col >> 1, col & 1);
+                                   X(u, l), X(b, l), c->d_box.as<int>(), coord(), rd4, L.blk0, L.nblk, (T)sl,
+                                   (T)omega, bc, col >> 1, col & 1);
     }
     void vcycle4(FieldBuf &u, FieldBuf &b, FieldBuf &r, FieldBuf &u1, double s, const srmg_op *op, int lmax,
                  int ndown, int ntop, int nup, double omega) {
         Bc27<T> bc{};
         for (int k = 0; k < 27; k++) bc.v[k] = (T)op->bc[k];
+        rd4 = op->geometry == SRMG_CYL1 ? 0 : (op->geometry == SRMG_CYL2 ? 1 : -1);
         if (lmax == 0) lmax = c->nlevels;
         for (FieldBuf *f : {&u, &b, &r, &u1}) settle(*f);
         const LevelArgs<T> ac0 = level_args<T>(c, 0, 1.0, op, 1.0);  // mirror signs for the face reads of interp!
@@ -968,12 +970,14 @@ struct Ops {
             copy(u1, u, l - 1);
             if (L.nblk)
                 hipLaunchKernelGGL((k_resid4<D, M, T>), dim3(nwg((int64_t)L.nblk * LY::V)), dim3(WG), 0, c->stream,
-                                   X(r, l), X(u, l), X(b, l), c->d_box.as<int>(), L.blk0, L.nblk, (T)sl, bc);
+                                   X(r, l), X(u, l), X(b, l), c->d_box.as<int>(), coord(), rd4, L.blk0, L.nblk,
+                                   (T)sl, bc);
             restrict_(r, l);
             if (L.nblk)
                 hipLaunchKernelGGL((k_postrestrict4<D, M, T>), dim3(nwg((int64_t)L.nblk * (LY::V / LY::NC))),
                                    dim3(WG), 0, c->stream, X(b, l - 1), X(u, l - 1), X(r, l - 1), c->d_box.as<int>(),
-                                   c->d_parent.as<int>(), c->d_sub.as<int>(), L.blk0, L.nblk, (T)(4 * sl), bc);
+                                   coord(), rd4, c->d_parent.as<int>(), c->d_sub.as<int>(), L.blk0, L.nblk,
+                                   (T)(4 * sl), bc);
         }
         gs4_iter(u, b, s, omega, ntop, 0, bc);
         for (int l = 1; l < lmax; l++) {
@@ -990,13 +994,15 @@ struct Ops {
     void resnorm4(FieldBuf &r, FieldBuf &u, FieldBuf &b, double s, const srmg_op *op) {
         Bc27<T> bc{};
         for (int k = 0; k < 27; k++) bc.v[k] = (T)op->bc[k];
+        rd4 = op->geometry == SRMG_CYL1 ? 0 : (op->geometry == SRMG_CYL2 ? 1 : -1);
         for (FieldBuf *f : {&u, &b, &r}) settle(*f);
         for (int l = 0; l < c->nlevels; l++) {
             const LevelInfo &L = c->lv[l];
             if (!L.nblk) continue;
             const double sl = s / (double)((int64_t)1 << (2 * l));
             hipLaunchKernelGGL((k_resid4<D, M, T>), dim3(nwg((int64_t)L.nblk * LY::V)), dim3(WG), 0, c->stream,
-                               X(r, l), X(u, l), X(b, l), c->d_box.as<int>(), L.blk0, L.nblk, (T)sl, bc);
+                               X(r, l), X(u, l), X(b, l), c->d_box.as<int>(), coord(), rd4, L.blk0, L.nblk,
+                               (T)sl, bc);
         }
         bnorm_partials(r);
     }
@@ -2357,9 +2363,14 @@ static void box_table(srmg_ctx *c) {
     c->box_ok = true;
 }
 
+static void check_order4_geometry(srmg_ctx *c, const srmg_op *op) {
+    if (op->geometry < SRMG_CARTESIAN || op->geometry > SRMG_CYL2) fail("unknown geometry");
+    if (c->D == 3 && op->geometry != SRMG_CARTESIAN) fail("cylindrical geometry is two-dimensional");
+}
+
 static void do_vcycle4(srmg_ctx *c, int u, int b, int r, int u1, double s, const srmg_op *op, int lmax, int ndown,
                        int ntop, int nup, double omega) {
-    if (op->geometry != SRMG_CARTESIAN) fail("the order-4 operator is built for Cartesian geometry");
+    check_order4_geometry(c, op);
     if (op->coarse != SRMG_COARSE_GS) fail("the order-4 operator uses the reference's level-1 sweeps");
     if (lmax < 0 || lmax > c->nlevels) fail("lmax out of range");
     const int t = same_type(c, {u, b, r, u1});
@@ -2422,7 +2433,7 @@ int srmg_resnorm(srmg_ctx *c, int r, int b, int u, double s, const srmg_op *op, 
         if (!op || !out) fail("null argument");
         if (c->trees.size() != 1) fail("several trees: use srmg_resnorm_batched");
         if (op->disc == SRMG_LAPLACIAN4) {
-            if (op->geometry != SRMG_CARTESIAN) fail("the order-4 operator is built for Cartesian geometry");
+            check_order4_geometry(c, op);
             if (same_type(c, {r, b, u}) != SRMG_F64) fail("the order-4 operator runs in FP64");
             box_table(c);
             dispatch(c, SRMG_F64, [&](auto &o) { o.resnorm4(field(c, r), field(c, u), field(c, b), s, op); });

@@ -467,8 +467,8 @@ def fill_ghost_all(u: DeviceField, bc):
 def vcycle(u, b, r, u1, s, tree=None, conn=None, geometry=None, bc=None, lpldisc=None, *,
            lmax=0, ndown=3, ntop=5, nup=3, omega=1.0, coarse=COARSE_GS):
     """vcycle! (src/multigrid.jl:60-106) on device-resident fields; with
-    LaplacianDiscretization(D, 4) the order-4 form (Cartesian, FP64, complete
-    levels: FourColors smoothing, src/multigrid.jl:268-295)."""
+    LaplacianDiscretization(D, 4) the order-4 form (Cartesian or cylindrical,
+    FP64, complete levels: FourColors smoothing, src/multigrid.jl:268-295)."""
     op = make_op(geometry, bc, lpldisc, coarse, any_order=True)
     _check(lib().srmg_vcycle(u.ctx.h, u.id, b.id, r.id, u1.id, float(s), C.byref(op),
                              int(lmax), int(ndown), int(ntop), int(nup), float(omega)))

@@ -19,13 +19,22 @@ CASES = {
     "2d-m8": lambda: complete_problem(2, (3, 2), 8, levels=2, bcd=((-1, -1), (1, 1))),
     "3d-m8": lambda: complete_problem(3, (2, 2, 3), 8, levels=2, bcd=((1, 1), (-1, -1), (-1, 1))),
 }
+GEOM = {"2d-m16": 0, "2d-m8": 0, "3d-m8": 0, "cyl1-m16": 1, "cyl2-m8": 2}
+CASES["cyl1-m16"] = lambda: complete_problem(2, (2, 3), 16, levels=2, bcd=((1, -1), (-1, -1)))
+CASES["cyl2-m8"] = lambda: complete_problem(2, (3, 2), 8, levels=3, bcd=((-1, -1), (1, -1)))
+
+
+def _oracle(p, name):
+    from oracle import Oracle
+    from srmg.tree import box_stencil, connectivity
+    return Oracle(p.tree, connectivity(p.tree, box_stencil(p.D)), p.M, p.b.G, GEOM[name], p.bc, order=4)
 
 
 @pytest.mark.parametrize("name", list(CASES))
 def test_order4_vcycles_match_oracle(name):
     p = CASES[name]()
     print(f"case {name}: {p.tree.nblocks()} blocks", flush=True)
-    geom, disc = solver.CartesianGeometry(), solver.LaplacianDiscretization(p.D, 4)
+    geom, disc = solver.geometry_from_code(GEOM[name]), solver.LaplacianDiscretization(p.D, 4)
     ctx = solver.Context(p.tree, p.conn, p.M)
     u, q, q1, r, u1 = (ctx.field(F64) for _ in range(5))
     q.upload(p.b)
@@ -36,7 +45,7 @@ def test_order4_vcycles_match_oracle(name):
     got, gb = p.b.like(), p.b.like()
     u.download(got)
     q1.download(gb)
-    o = order4_oracle(p)
+    o = _oracle(p, name)
     uo, qo, q1o, ro, u1o = p.b.like(), p.b.copy(), p.b.like(), p.b.like(), p.b.like()
     o.rhs(q1o, qo, order=4)
     for _ in range(3):

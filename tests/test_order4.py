@@ -123,3 +123,32 @@ def test_order4_vcycles_converge_faster_than_order2(order):
         assert 3.5 < ratio < 4.5
     else:
         assert 10 < ratio < 17
+
+
+@pytest.mark.parametrize("geom", [1, 2])
+def test_cylindrical_order4_operator_is_exact_on_low_polynomials(geom):
+    # LaplacianDiscretization{2,4} x CylindricalGeometry{geom}: L4 u = h^2 lap u
+    # (lap in axisymmetric cylindrical coordinates, lap lap u = 0) for r^2, z^2,
+    # r^2 z and z^3, with r the coordinate of dimension `geom`
+    p = complete_problem(2, (2, 3), 16)
+    o = Oracle(p.tree, connectivity(p.tree, box_stencil(2)), p.M, p.b.G, geom, p.bc, order=4)
+    cases = [(lambda r, z: r * r, lambda r, z: 4.0 + 0 * r), (lambda r, z: z * z, lambda r, z: 2.0 + 0 * r),
+             (lambda r, z: r * r * z, lambda r, z: 4.0 * z), (lambda r, z: z ** 3, lambda r, z: 6.0 * z)]
+    G, M = p.b.G, p.M
+    vs = (slice(G, G + M),) * 2
+    for f, lap in cases:
+        u = p.b.like()
+        for level in (1, 2):
+            for _, blk in p.tree[level - 1].pairs:
+                X, h = _centres(p, blk, level)
+                r, z = (X[0], X[1]) if geom == 1 else (X[1], X[0])
+                u.u[..., blk - 1] = f(r, z)
+        zero, rr = p.b.like(), p.b.like()
+        for level in (1, 2):
+            o.residual_level(rr, u, zero, 1.0, level)
+        for level in (1, 2):
+            for _, blk in p.tree[level - 1].pairs:
+                X, h = _centres(p, blk, level)
+                r, z = (X[0], X[1]) if geom == 1 else (X[1], X[0])
+                np.testing.assert_allclose(-rr.u[vs + (blk - 1,)], (h * h * lap(r, z))[vs], rtol=0,
+                                           atol=1e-14 * (np.max(np.abs(u.u)) + 1))
