@@ -150,8 +150,8 @@ int srmg_rhs(srmg_ctx *ctx, int b1, int b, const srmg_op *op);
  * regrid, as applydelta! does); this and the block coordinates are checked
  * before anything changes (records that srmg_set_tree would refuse leave the
  * context without a tree, as a failing srmg_set_tree does).
- * Fields not listed are reallocated as in srmg_set_tree. Single-tree
- * contexts. bc may be NULL when no field is interpolated. */
+ * Fields not listed are reallocated as in srmg_set_tree. bc may be NULL
+ * when no field is interpolated. */
 enum { SRMG_REGRID_INTERP = 1, SRMG_REGRID_RESTRICT = 2 };
 int srmg_regrid(srmg_ctx *ctx, int nlevels, int64_t nblocks,
                 const int64_t *pairs, const int64_t *pairs_off,
@@ -160,6 +160,16 @@ int srmg_regrid(srmg_ctx *ctx, int nlevels, int64_t nblocks,
                 const int64_t *refboundary, const int64_t *refboundary_off,
                 const int64_t *child, const int64_t *child_off,
                 const int *fields, const int *modes, int nfields, const int8_t *bc);
+/* srmg_regrid for a forest (srmg_set_forest's arguments): tree t of the new
+ * forest replaces tree t of the context, each matched by (level, coordinate);
+ * the number of trees stays. */
+int srmg_regrid_forest(srmg_ctx *ctx, int ntrees, const int *nlevels, const int64_t *nblocks,
+                       const int64_t *const *pairs, const int64_t *const *pairs_off,
+                       const int64_t *const *neighbor, const int64_t *const *neighbor_off,
+                       const int64_t *const *boundary, const int64_t *const *boundary_off,
+                       const int64_t *const *refboundary, const int64_t *const *refboundary_off,
+                       const int64_t *const *child, const int64_t *const *child_off,
+                       const int *fields, const int *modes, int nfields, const int8_t *bc);
 
 /* Device-resident block field of the given element type; returns its id. */
 int srmg_field_create(srmg_ctx *ctx, int dtype, int *id);

@@ -2122,6 +2122,25 @@ static void regrid_fields(srmg_ctx *c, const std::vector<TreeRecords> &tr, const
     HIPCHK(hipStreamSynchronize(c->stream));  // before the old buffers are released
 }
 
+int srmg_regrid_forest(srmg_ctx *c, int ntrees, const int *nlevels, const int64_t *nblocks,
+                       const int64_t *const *pairs, const int64_t *const *pairs_off,
+                       const int64_t *const *neighbor, const int64_t *const *neighbor_off,
+                       const int64_t *const *boundary, const int64_t *const *boundary_off,
+                       const int64_t *const *refboundary, const int64_t *const *refboundary_off,
+                       const int64_t *const *child, const int64_t *const *child_off, const int *fields,
+                       const int *modes, int nfields, const int8_t *bc) {
+    return guarded([&] {
+        if (!c) fail("null context");
+        if (ntrees < 1) fail("ntrees must be >= 1");
+        std::vector<TreeRecords> tr;
+        for (int t = 0; t < ntrees; t++)
+            tr.push_back(TreeRecords{nlevels[t], nblocks[t], pairs[t], pairs_off[t], neighbor[t], neighbor_off[t],
+                                     boundary[t], boundary_off[t], refboundary[t], refboundary_off[t], child[t],
+                                     child_off[t]});
+        regrid_fields(c, tr, fields, modes, nfields, bc);
+    });
+}
+
 int srmg_regrid(srmg_ctx *c, int nlevels, int64_t nblocks, const int64_t *pairs, const int64_t *pairs_off,
                 const int64_t *neighbor, const int64_t *neighbor_off, const int64_t *boundary,
                 const int64_t *boundary_off, const int64_t *refboundary, const int64_t *refboundary_off,

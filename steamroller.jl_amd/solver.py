@@ -129,6 +129,7 @@ def lib():
         "srmg_destroy": [P],
         "srmg_set_tree": [P, i32, i64] + [P] * 10,
         "srmg_regrid": [P, i32, i64] + [P] * 10 + [P, P, i32, P],
+        "srmg_regrid_forest": [P, i32, P, P] + [P] * 10 + [P, P, i32, P],
         "srmg_connectivity": [i32, i32, P, P, P, i32] + [P, P, i64] * 4,
         "srmg_restrict_full": [P, i32],
         "srmg_rhs": [P, i32, i32, P],
@@ -267,11 +268,15 @@ class Context:
         t0, c0 = trees_conns[0]
         return cls(t0, c0, M, forest=list(trees_conns))
 
-    def set_forest(self, trees_conns):
+    def set_forest(self, trees_conns, fields=(), bc=None, nblocks=None):
+        """Install a forest (``srmg_set_forest``); with ``fields`` the trees
+        replace the context's trees one for one and those fields are carried
+        across (``srmg_regrid_forest``, modes as in :meth:`regrid`).
+        ``nblocks``: host block ids per tree (default: the trees' counts)."""
         D = trees_conns[0][0].D
         keep, per = [], [[] for _ in range(10)]
         nl, nbk = [], []
-        for tree, conn in trees_conns:
+        for ti, (tree, conn) in enumerate(trees_conns):
             pairs, poff = flatten_pairs(tree)
             arrs = [pairs, poff]
             for lists, w in ((conn.neighbor, 2 + D), (conn.boundary, 2 + 2 * D),
@@ -284,12 +289,20 @@ class Context:
             for k in range(10):
                 per[k].append(arrs[k].ctypes.data)
             nl.append(len(tree))
-            nbk.append(tree.nblocks())
+            nbk.append(tree.nblocks() if nblocks is None else int(nblocks[ti]))
         ptrs = [(C.c_void_p * len(v))(*v) for v in per]
         nl_a = np.array(nl, np.int32)
         nb_a = np.array(nbk, np.int64)
         self._tables = (keep, ptrs, nl_a, nb_a)
-        _check(lib().srmg_set_forest(self.h, len(trees_conns), _p(nl_a), _p(nb_a), *ptrs))
+        fields = list(fields)
+        if fields:
+            ids = np.array([f.id for f, _ in fields], np.int32)
+            modes = np.array([int(m) for _, m in fields], np.int32)
+            b = None if bc is None else _bc_arr(bc)
+            _check(lib().srmg_regrid_forest(self.h, len(trees_conns), _p(nl_a), _p(nb_a), *ptrs, _p(ids),
+                                            _p(modes), len(fields), None if b is None else _p(b)))
+        else:
+            _check(lib().srmg_set_forest(self.h, len(trees_conns), _p(nl_a), _p(nb_a), *ptrs))
         self.ntrees = len(trees_conns)
         self.trees = [t for t, _ in trees_conns]
         self.nblocks = int(nb_a.sum())

@@ -149,3 +149,32 @@ def test_regrid_refuses_more_than_one_level_and_changes_nothing():
     vs = _vs(p)
     for blk in _blocks(small):
         np.testing.assert_array_equal(back.u[vs + (blk - 1,)], host.u[vs + (blk - 1,)])
+
+
+def test_forest_regrid_equals_single_tree_regrids():
+    # srmg_regrid_forest: each tree of a forest regridded with its own delta
+    # gives, tree by tree, the bits of regridding it alone
+    probs = [rc.problem(2, seed=s) for s in (1, 2)] + [rc.problem(3, seed=3)]
+    probs = [p for p in probs if p.D == 2]  # one dimension per context
+    edits = [rc.edited(p.tree, p.tree.nblocks()) for p in probs]
+    hosts = [rc.random_field(p, seed=10 + k) for k, p in enumerate(probs)]
+    ctx = solver.Context.forest([(p.tree, p.conn) for p in probs], probs[0].M)
+    u = ctx.field(F64)
+    u.upload(solver.concat_fields(hosts))
+    both = REGRID_INTERP | REGRID_RESTRICT
+    ctx.set_forest([(t2, c2) for t2, c2, _, _ in edits], fields=[(u, both)], bc=probs[0].bc,
+                   nblocks=[nb2 for _, _, nb2, _ in edits])
+    nbs = [nb2 for _, _, nb2, _ in edits]
+    got = solver.concat_fields([_host(p, nb) for p, nb in zip(probs, nbs)])
+    u.download(got)
+    parts = solver.split_field(got, nbs)
+    vs = _vs(probs[0])
+    for k, (p, (t2, c2, nb2, _)) in enumerate(zip(probs, edits)):
+        c1 = solver.Context(p.tree, p.conn, p.M)
+        v = c1.field(F64)
+        v.upload(hosts[k])
+        c1.regrid(t2, c2, fields=[(v, both)], bc=p.bc, nblocks=nb2)
+        ref = _host(p, nb2)
+        v.download(ref)
+        for blk in _blocks(t2):
+            np.testing.assert_array_equal(parts[k][vs + (blk - 1,)], ref.u[vs + (blk - 1,)])
