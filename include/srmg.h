@@ -43,10 +43,11 @@ enum { SRMG_CARTESIAN = 0, SRMG_CYL1 = 1, SRMG_CYL2 = 2 };
 /* SRMG_LAPLACIAN4: the compact 4th-order LaplacianDiscretization{D,4}.
  * srmg_rhs applies its R (Cartesian and cylindrical); srmg_vcycle runs its
  * V-cycle with the FourColors smoother (src/multigrid.jl:206-232, 268-295) for
- * Cartesian and cylindrical geometry, FP64 fields, the reference's level-1 sweeps and trees
- * whose levels are complete (no coarse-fine faces: their corner and edge
- * ghosts are not built), and srmg_resnorm its residual norm (r receives the
- * residual, as resnorm! overwrites r); the other solve entry points refuse it. */
+ * Cartesian and cylindrical geometry, FP64 fields, the reference's level-1
+ * sweeps and single trees that are properly nested for the BoxStencil (its
+ * ghosts: srmg_fill_ghost4), and srmg_resnorm its residual norm (r receives
+ * the residual, as resnorm! overwrites r); the other solve entry points
+ * refuse it. */
 enum { SRMG_LAPLACIAN = 0, SRMG_HELMHOLTZ = 1, SRMG_LAPLACIAN4 = 2 };
 /* coarsest-level treatment: the reference's ntop Gauss-Seidel sweeps
  * (src/multigrid.jl:89), or an exact FP64 solve of the level-1 problem
@@ -123,6 +124,14 @@ int srmg_connectivity(int D, int nlevels, const int64_t *rootsize,
                       int64_t *boundary, int64_t *boundary_off, int64_t boundary_cap,
                       int64_t *refboundary, int64_t *refboundary_off, int64_t refboundary_cap,
                       int64_t *child, int64_t *child_off, int64_t child_cap);
+
+/* fill_ghost!(u, l, conn, bc) for every level, ascending, for the order-4
+ * operator: its depth-1 ghosts, corners and edges included, are kept
+ * explicitly (copies from same-level neighbours, then boundary mirrors, then
+ * coarse-fine interpolations, as the reference fills them with BoxStencil
+ * records). Call it where the reference pre-fills u before the cycles
+ * (src/streamer.jl:323-325). FP64 fields, single-tree contexts. */
+int srmg_fill_ghost4(srmg_ctx *ctx, int field, const int8_t *bc);
 
 /* restrict_full!(u, conn) (src/restrict.jl:43-47): every level's blocks
  * restricted into their parents, finest level first. */

@@ -174,75 +174,188 @@ __global__ __launch_bounds__(WG) void k_residual(T *__restrict__ r, const T *__r
     r[o] = -(b[o] + Lu / bp.s);
 }
 
-// ---- LaplacianDiscretization{D,4} x CartesianGeometry on complete levels ----
-//
-// box[blk * 3^D + k] for the direction k = sum (dir_d + 1) 3^d: the
-// same-level neighbour block, or -(1 + mask) where the direction leaves the
-// domain in the dimensions of mask. bc27 is the 3^D boundary table.
-
-// depth-1 value at cell c of block blk (c may lie one cell outside the block
-// in any dimensions): the neighbour's copy (fill_ghost_copy!, with diagonal
-// Neighbor records), or at the domain boundary fill_ghost_bnd!'s a[bnd] times
-// the cell mirrored in the dimensions that leave the domain, which in the
-// other dimensions is again a neighbour's copy (src/connectivity.jl:178-231)
-template <int D, int M, typename T>
-__device__ __forceinline__ T boxv(const T *__restrict__ x, const int *__restrict__ box, int blk, int c0, int c1,
-                                  int c2, const T *bc27) {
-    using LY = Lay<D, M>;
-    constexpr int NB3 = D == 3 ? 27 : 9;
-    int c[3] = {c0, c1, c2};
-    int dir[3] = {0, 0, 0};
-    int k = 0, m = 1;
-    bool out = false;
-#pragma unroll
-    for (int d = 0; d < D; d++) {
-        dir[d] = c[d] < 0 ? -1 : (c[d] >= M ? 1 : 0);
-        k += (dir[d] + 1) * m;
-        m *= 3;
-        out |= dir[d] != 0;
-    }
-    if (!out) return x[(size_t)blk * LY::V + LY::sidx(c[0], c[1], c[2])];
-    int e = box[blk * NB3 + k];
-    T sign = T(1);
-    if (e < 0) {
-        const int mask = -1 - e;
-        int bi = 0, mb = 1;
-        k = 0;
-        m = 1;
-        out = false;
-#pragma unroll
-        for (int d = 0; d < D; d++) {
-            const bool bd = (mask >> d) & 1;
-            bi += ((bd ? dir[d] : 0) + 1) * mb;
-            mb *= 3;
-            if (bd) {
-                c[d] = dir[d] < 0 ? -1 - c[d] : 2 * M - 1 - c[d];
-                dir[d] = 0;
-            }
-            k += (dir[d] + 1) * m;
-            m *= 3;
-            out |= dir[d] != 0;
-        }
-        sign = bc27[bi];
-        if (!out) return sign * x[(size_t)blk * LY::V + LY::sidx(c[0], c[1], c[2])];
-        e = box[blk * NB3 + k];
-        if (e < 0) return T(0);  // not on complete levels (the host checks)
-    }
-#pragma unroll
-    for (int d = 0; d < D; d++) c[d] -= dir[d] * M;
-    return sign * x[(size_t)e * LY::V + LY::sidx(c[0], c[1], c[2])];
-}
+// Linear interpolation weights (src/interp.jl:7-11)
+template <int D, typename T> __device__ __forceinline__ T w0() { return D == 2 ? T(0.5) : T(0.25); }
+template <typename T> __device__ __forceinline__ T w1() { return T(0.25); }
 
 template <typename T>
 struct Bc27 {
     T v[27];
 };
 
+// ---- LaplacianDiscretization{D,4}: the 9- and 19-point operators ----
+//
+// Their stencils reach corner (2D) and edge (3D) ghosts, so the order-4 path
+// keeps every block's depth-1 ghosts explicitly: a frame of (M + 2)^D values
+// per block (the interior unused), filled as fill_ghost! fills the reference's
+// ghost layers (src/connectivity.jl:155-258, with BoxStencil records): the
+// copies from same-level neighbours, then the boundary mirrors (which read
+// the frame where the mirrored cell is itself a ghost), then the coarse-fine
+// interpolations. box[blk * 3^D + k] for the direction k = sum (dir_d + 1) 3^d
+// is the same-level neighbour, -(1 + mask) where the direction leaves the
+// domain in the dimensions of mask, or -16 - j for coarse-fine slot j
+// (coarse block, subblock code).
+template <int D, int M>
+struct Frame {
+    static constexpr int S = M + 2;
+    static constexpr int P = D == 3 ? S * S * S : S * S;
+    __host__ __device__ static constexpr int at(int c0, int c1, int c2) {
+        return D == 3 ? ((c2 + 1) * S + (c1 + 1)) * S + (c0 + 1) : (c1 + 1) * S + (c0 + 1);
+    }
+};
+
+// the value at cell c of block blk, c at most one cell outside the block
+template <int D, int M, typename T>
+__device__ __forceinline__ T boxv(const T *__restrict__ x, const T *__restrict__ g, int blk, int c0, int c1,
+                                  int c2) {
+    using LY = Lay<D, M>;
+    const bool in = c0 >= 0 && c0 < M && c1 >= 0 && c1 < M && (D == 2 || (c2 >= 0 && c2 < M));
+    return in ? x[(size_t)blk * LY::V + LY::sidx(c0, c1, c2)]
+              : g[(size_t)blk * Frame<D, M>::P + Frame<D, M>::at(c0, c1, D == 3 ? c2 : 0)];
+}
+
+// the direction of a frame cell (-1, 0, 1 per dimension), its index k, and
+// whether it is a ghost
+template <int D, int M>
+__device__ __forceinline__ bool frame_cell(int q, int *c, int *dir, int &k) {
+    constexpr int S = M + 2;
+    c[0] = q % S - 1;
+    c[1] = (q / S) % S - 1;
+    c[2] = D == 3 ? q / (S * S) - 1 : 0;
+    bool ghost = false;
+    k = 0;
+    int m = 1;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        dir[d] = 0;
+        if (d < D) {
+            dir[d] = c[d] < 0 ? -1 : (c[d] >= M ? 1 : 0);
+            ghost |= dir[d] != 0;
+            k += (dir[d] + 1) * m;
+            m *= 3;
+        }
+    }
+    return ghost;
+}
+
+// fill_ghost_copy!: ghosts facing a same-level neighbour
+template <int D, int M, typename T>
+__global__ __launch_bounds__(WG) void k_frame_copy(T *__restrict__ g, const T *__restrict__ x,
+                                                   const int *__restrict__ box, int blk0, int nblk) {
+    using LY = Lay<D, M>;
+    constexpr int P = Frame<D, M>::P, NB3 = D == 3 ? 27 : 9;
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lb = gt / P;
+    if (lb >= nblk) return;
+    const int q = gt - lb * P, blk = blk0 + lb;
+    int c[3], dir[3], k;
+    if (!frame_cell<D, M>(q, c, dir, k)) return;
+    const int e = box[blk * NB3 + k];
+    if (e < 0) return;
+#pragma unroll
+    for (int d = 0; d < D; d++) c[d] -= dir[d] * M;
+    g[(size_t)blk * P + q] = x[(size_t)e * LY::V + LY::sidx(c[0], c[1], c[2])];
+}
+
+// fill_ghost_bnd!: ghosts beyond the domain = a[bnd] times the cell mirrored
+// in the dimensions that leave the domain (a ghost of another direction, read
+// from the frame, when other dimensions stay outside)
+template <int D, int M, typename T>
+__global__ __launch_bounds__(WG) void k_frame_bnd(T *__restrict__ g, const T *__restrict__ x,
+                                                  const int *__restrict__ box, int blk0, int nblk, Bc27<T> bc) {
+    constexpr int P = Frame<D, M>::P, NB3 = D == 3 ? 27 : 9;
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lb = gt / P;
+    if (lb >= nblk) return;
+    const int q = gt - lb * P, blk = blk0 + lb;
+    int c[3], dir[3], k;
+    if (!frame_cell<D, M>(q, c, dir, k)) return;
+    const int e = box[blk * NB3 + k];
+    if (e >= 0 || e < -8) return;
+    const int mask = -1 - e;
+    int bi = 0, m = 1;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        const bool bd = (mask >> d) & 1;
+        bi += ((bd ? dir[d] : 0) + 1) * m;
+        m *= 3;
+        if (bd) c[d] = dir[d] < 0 ? -1 - c[d] : 2 * M - 1 - c[d];
+    }
+    g[(size_t)blk * P + q] = bc.v[bi] * boxv<D, M, T>(x, g, blk, c[0], c[1], c[2]);
+}
+
+// fill_ghost_interp! (src/connectivity.jl:251-258, src/interp.jl:23-60): a
+// ghost facing a coarse block is the linear interpolation of the coarse cell
+// under it with its axis neighbours (coarse ghosts from the coarse frame),
+// the offsets towards this block in the dimensions of the direction
+template <int D, int M, typename T>
+__global__ __launch_bounds__(WG) void k_frame_interp(T *__restrict__ g, const T *__restrict__ x,
+                                                     const int *__restrict__ box, const int2 *__restrict__ slots,
+                                                     int blk0, int nblk) {
+    using LY = Lay<D, M>;
+    constexpr int P = Frame<D, M>::P, NB3 = D == 3 ? 27 : 9;
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lb = gt / P;
+    if (lb >= nblk) return;
+    const int q = gt - lb * P, blk = blk0 + lb;
+    int c[3], dir[3], k;
+    if (!frame_cell<D, M>(q, c, dir, k)) return;
+    const int e = box[blk * NB3 + k];
+    if (e > -16) return;
+    const int2 sl = slots[-16 - e];
+    const int pb = sl.x, sb = sl.y;
+    int J[3] = {0, 0, 0}, i[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        const int sbd = (sb >> d) & 1;
+        if (dir[d] != 0) {
+            J[d] = sbd * LY::MH + (dir[d] < 0 ? LY::MH - 1 : 0);
+            i[d] = dir[d] < 0 ? 1 : 0;
+        } else {
+            J[d] = sbd * LY::MH + (c[d] >> 1);
+            i[d] = c[d] & 1;
+        }
+    }
+    T f = w0<D, T>() * x[(size_t)pb * LY::V + LY::sidx(J[0], J[1], J[2])];
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+        int K[3] = {J[0], J[1], J[2]};
+        K[j] += 2 * i[j] - 1;
+        f += w1<T>() * boxv<D, M, T>(x, g, pb, K[0], K[1], K[2]);
+    }
+    g[(size_t)blk * P + q] = f;
+}
+
+// interp_add_level! (src/interp.jl:102-115) reading the coarse ghosts from
+// the frames
+template <int D, int M, typename T>
+__global__ __launch_bounds__(WG) void k_interp_add4(T *__restrict__ u, const T *__restrict__ u1,
+                                                    const T *__restrict__ g1, const int *__restrict__ parent,
+                                                    const int *__restrict__ sub, int blk0, int nblk) {
+    using LY = Lay<D, M>;
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lb = gt / LY::V;
+    if (lb >= nblk) return;
+    const int fb = blk0 + lb, q = gt - lb * LY::V;
+    int x0, x1, x2;
+    LY::decode(q, x0, x1, x2);
+    const int pb = parent[fb], sb = sub[fb];
+    const int J0 = (sb & 1) * LY::MH + (x0 >> 1);
+    const int J1 = ((sb >> 1) & 1) * LY::MH + (x1 >> 1);
+    const int J2 = (D == 3) ? ((sb >> 2) & 1) * LY::MH + (x2 >> 1) : 0;
+    T f = w0<D, T>() * u1[(size_t)pb * LY::V + LY::sidx(J0, J1, J2)];
+    f += w1<T>() * boxv<D, M, T>(u1, g1, pb, J0 + 2 * (x0 & 1) - 1, J1, J2);
+    f += w1<T>() * boxv<D, M, T>(u1, g1, pb, J0, J1 + 2 * (x1 & 1) - 1, J2);
+    if (D == 3) f += w1<T>() * boxv<D, M, T>(u1, g1, pb, J0, J1, J2 + 2 * (x2 & 1) - 1);
+    const size_t o = (size_t)fb * LY::V + q;
+    u[o] = f + u[o];
+}
+
 // applystencil (src/laplacian.jl:133-141, 305-320): the literal's nonzero
 // terms in column-major order, then one division by 6
 template <int D, int M, typename T>
-__device__ __forceinline__ T stencil4(const T *__restrict__ x, const int *__restrict__ box, int blk, int c0, int c1,
-                                      int c2, const T *bc27) {
+__device__ __forceinline__ T stencil4(const T *__restrict__ x, const T *__restrict__ g, int blk, int c0, int c1,
+                                      int c2) {
     T v = T(0);
     constexpr int NK = D == 3 ? 1 : 0;
 #pragma unroll
@@ -255,7 +368,7 @@ __device__ __forceinline__ T stencil4(const T *__restrict__ x, const int *__rest
                 const int a = D == 2 ? (n == 0 ? -20 : (n == 1 ? 4 : 1))
                                      : (k == 0 ? (n == 0 ? -24 : (n == 1 ? 2 : 1)) : (n == 1 ? 2 : (n == 2 ? 1 : 0)));
                 if (a == 0) continue;
-                v += T(a) * boxv<D, M, T>(x, box, blk, c0 + i, c1 + j, c2 + k, bc27);
+                v += T(a) * boxv<D, M, T>(x, g, blk, c0 + i, c1 + j, c2 + k);
             }
     return v / T(6);
 }
@@ -265,8 +378,8 @@ __device__ __forceinline__ T stencil4(const T *__restrict__ x, const int *__rest
 // (integer ratios rounded once), visited column-major, divided by 6; and its
 // diagelm
 template <int M, typename T>
-__device__ __forceinline__ T stencil4cyl(const T *__restrict__ x, const int *__restrict__ box, int blk, int c0,
-                                         int c1, const T *bc27, int N, long long i) {
+__device__ __forceinline__ T stencil4cyl(const T *__restrict__ x, const T *__restrict__ g, int blk, int c0,
+                                         int c1, int N, long long i) {
     const long long q = 2 * i - 1;
     const T lo = (T)(2 * (i - 1)) / (T)q, hi = (T)(2 * i) / (T)q;
     const T pm = (T)(4 * (-2 + 9 * i - 16 * i * i + 8 * i * i * i)) / (T)(q * q * q);
@@ -286,7 +399,7 @@ __device__ __forceinline__ T stencil4cyl(const T *__restrict__ x, const int *__r
 #pragma unroll
     for (int b = 0; b < 3; b++)
 #pragma unroll
-        for (int a = 0; a < 3; a++) v += w[a][b] * boxv<2, M, T>(x, box, blk, c0 + a - 1, c1 + b - 1, 0, bc27);
+        for (int a = 0; a < 3; a++) v += w[a][b] * boxv<2, M, T>(x, g, blk, c0 + a - 1, c1 + b - 1, 0);
     return v / T(6);
 }
 
@@ -298,18 +411,18 @@ __device__ __forceinline__ T diag4cyl(long long i) {
 // the order-4 operator at cell (c0, c1, c2) of blk: Cartesian (rd < 0) or
 // cylindrical with radial dimension rd (global radial index from coord)
 template <int D, int M, typename T>
-__device__ __forceinline__ T apply4(const T *__restrict__ x, const int *__restrict__ box,
-                                    const int *__restrict__ coord, int blk, int c0, int c1, int c2, const T *bc27,
-                                    int rd, T &diag) {
+__device__ __forceinline__ T apply4(const T *__restrict__ x, const T *__restrict__ g,
+                                    const int *__restrict__ coord, int blk, int c0, int c1, int c2, int rd,
+                                    T &diag) {
     if constexpr (D == 2) {
         if (rd >= 0) {
             const long long i = (long long)(rd == 0 ? c0 : c1) + 1 + (long long)M * (coord[blk * D + rd] - 1);
             diag = diag4cyl<T>(i);
-            return stencil4cyl<M, T>(x, box, blk, c0, c1, bc27, rd, i);
+            return stencil4cyl<M, T>(x, g, blk, c0, c1, rd, i);
         }
     }
     diag = D == 2 ? T(-20.0 / 6.0) : T(-4);
-    return stencil4<D, M, T>(x, box, blk, c0, c1, c2, bc27);
+    return stencil4<D, M, T>(x, g, blk, c0, c1, c2);
 }
 
 // gauss_seidel!(..., ::FourColors{P1,P2}) (src/multigrid.jl:206-232) over a
@@ -319,8 +432,8 @@ __device__ __forceinline__ T apply4(const T *__restrict__ x, const int *__restri
 // not read each other
 template <int D, int M, typename T>
 __global__ __launch_bounds__(WG) void k_gs4_pass(T *__restrict__ u, const T *__restrict__ b,
-                                                 const int *__restrict__ box, const int *__restrict__ coord, int rd,
-                                                 int blk0, int nblk, T s, T omega, Bc27<T> bc, int P1, int P2) {
+                                                 const T *__restrict__ g, const int *__restrict__ coord, int rd,
+                                                 int blk0, int nblk, T s, T omega, int P1, int P2) {
     using LY = Lay<D, M>;
     constexpr int H2 = M / 2, NCC = D == 3 ? H2 * H2 * M : H2 * H2;
     const int gt = blockIdx.x * blockDim.x + threadIdx.x;
@@ -333,15 +446,15 @@ __global__ __launch_bounds__(WG) void k_gs4_pass(T *__restrict__ u, const T *__r
     const int x0 = 2 * i0 + p1, x1 = 2 * i1 + p2;
     const size_t o = (size_t)blk * LY::V + LY::sidx(x0, x1, z);
     T c;
-    const T Lu = apply4<D, M, T>(u, box, coord, blk, x0, x1, z, bc.v, rd, c);
+    const T Lu = apply4<D, M, T>(u, g, coord, blk, x0, x1, z, rd, c);
     u[o] = u[o] - omega * (Lu + s * b[o]) / c;
 }
 
 // residual! (src/multigrid.jl:309-323) with the order-4 operator
 template <int D, int M, typename T>
 __global__ __launch_bounds__(WG) void k_resid4(T *__restrict__ r, const T *__restrict__ u, const T *__restrict__ b,
-                                               const int *__restrict__ box, const int *__restrict__ coord, int rd,
-                                               int blk0, int nblk, T s, Bc27<T> bc) {
+                                               const T *__restrict__ g, const int *__restrict__ coord, int rd,
+                                               int blk0, int nblk, T s) {
     using LY = Lay<D, M>;
     const int gt = blockIdx.x * blockDim.x + threadIdx.x;
     const int lb = gt / LY::V;
@@ -351,7 +464,7 @@ __global__ __launch_bounds__(WG) void k_resid4(T *__restrict__ r, const T *__res
     LY::decode(q, c0, c1, c2);
     const size_t o = (size_t)blk * LY::V + q;
     T c;
-    const T Lu = apply4<D, M, T>(u, box, coord, blk, c0, c1, c2, bc.v, rd, c);
+    const T Lu = apply4<D, M, T>(u, g, coord, blk, c0, c1, c2, rd, c);
     r[o] = -(b[o] + Lu / s);
 }
 
@@ -359,10 +472,10 @@ __global__ __launch_bounds__(WG) void k_resid4(T *__restrict__ r, const T *__res
 // order-4 operator on the coarse cells covered by the fine blocks
 template <int D, int M, typename T>
 __global__ __launch_bounds__(WG) void k_postrestrict4(T *__restrict__ bout, const T *__restrict__ u,
-                                                      const T *__restrict__ r, const int *__restrict__ box,
+                                                      const T *__restrict__ r, const T *__restrict__ g,
                                                       const int *__restrict__ coord, int rd,
                                                       const int *__restrict__ parent, const int *__restrict__ sub,
-                                                      int fblk0, int nfine, T s, Bc27<T> bc) {
+                                                      int fblk0, int nfine, T s) {
     using LY = Lay<D, M>;
     constexpr int NS = LY::V / LY::NC;
     const int gt = blockIdx.x * blockDim.x + threadIdx.x;
@@ -375,7 +488,7 @@ __global__ __launch_bounds__(WG) void k_postrestrict4(T *__restrict__ bout, cons
     const int X2 = (D == 3) ? ((sb >> 2) & 1) * LY::MH + j2 : 0;
     const size_t o = (size_t)pb * LY::V + LY::sidx(X0, X1, X2);
     T c;
-    const T Lu = apply4<D, M, T>(u, box, coord, pb, X0, X1, X2, bc.v, rd, c);
+    const T Lu = apply4<D, M, T>(u, g, coord, pb, X0, X1, X2, rd, c);
     bout[o] = -(r[o] + Lu / s);
 }
 
@@ -489,9 +602,6 @@ __global__ __launch_bounds__(WG) void k_fill(T *__restrict__ dst, T v, size_t n)
     if (i < n) dst[i] = v;
 }
 
-// Linear interpolation weights (src/interp.jl:7-11)
-template <int D, typename T> __device__ __forceinline__ T w0() { return D == 2 ? T(0.5) : T(0.25); }
-template <typename T> __device__ __forceinline__ T w1() { return T(0.25); }
 
 // interp_add_level!(u, u1, child) (src/interp.jl:102-141, 185-193): every
 // fine cell += linear interpolation of the parent's correction u1.

@@ -41,6 +41,7 @@ struct FieldBuf {
     DevBuf data;  // nb * V values
     DevBuf alt;   // second buffer for ping-pong smoothing (made on first sweep)
     DevBuf ib;    // nslot * F values (coarse-fine ghost slabs)
+    DevBuf frame; // order 4: every block's depth-1 ghost frame (made on first use)
     std::vector<uint8_t> cur;  // per level: 0 -> values in data, 1 -> in alt
     size_t word() const { return dtype == SRMG_F64 ? 8 : 4; }
     bool settled() const {
@@ -133,8 +134,9 @@ struct srmg_ctx {
     // cleared whenever a buffer they point to may move or change its role
     std::unordered_map<std::string, std::unique_ptr<Graph>> graphs;
     uint64_t graph_gen = 0;
-    // the order-4 operator's box neighbour table (built on first use; see box_table)
-    DevBuf d_box;
+    // the order-4 operator's box neighbour table and coarse-fine slots (built
+    // on first use; see box_table)
+    DevBuf d_box, d_box_slots;
     bool box_ok = false;  // bumped when coarse-solve matrices are rebuilt (older graphs point to freed ones)
 };
 
@@ -156,6 +158,7 @@ static FieldBuf &field(srmg_ctx *c, int id) {
 }
 
 static void alloc_field(srmg_ctx *c, FieldBuf &f) {
+    f.frame.release();
     f.cur.assign(c->nlevels, 0);  // alt (ping-pong) holds nothing until the next sweep
     f.data.alloc((size_t)c->nb * c->V * f.word());
     f.ib.alloc((size_t)std::max(c->nslot, 1) * c->F * f.word());
@@ -944,15 +947,56 @@ struct Ops {
     // ---- LaplacianDiscretization{D,4}: vcycle! (src/multigrid.jl:60-106) with
     // the four-colour smoother on complete levels, step by step ----
     int rd4 = -1;  // radial dimension of the order-4 operator (set by vcycle4 / resnorm4)
+    static constexpr int FP = Frame<D, M>::P;
+    // the field's ghost frames (zero when first made)
+    T *frame(FieldBuf &f) {
+        if (!f.frame.p) {
+            f.frame.alloc((size_t)c->nb * FP * sizeof(T));
+            const size_t n = (size_t)c->nb * FP;
+            hipLaunchKernelGGL(k_fill<T>, dim3(nwg(n)), dim3(WG), 0, c->stream, f.frame.as<T>(), (T)0, n);
+        }
+        return f.frame.as<T>();
+    }
+    // fill_ghost!(x, l) (src/connectivity.jl:155-159): copy, boundary,
+    // coarse-fine interpolation; without the last step the refresh of
+    // gauss_seidel_iter! (copy and boundary)
+    void frame_fill(FieldBuf &x, int l, const Bc27<T> &bc, bool interp = true) {
+        const LevelInfo &L = c->lv[l];
+        if (!L.nblk) return;
+        T *g = frame(x);
+        const unsigned grid = nwg((int64_t)L.nblk * FP);
+        hipLaunchKernelGGL((k_frame_copy<D, M, T>), dim3(grid), dim3(WG), 0, c->stream, g, X(x, l),
+                           c->d_box.as<int>(), L.blk0, L.nblk);
+        hipLaunchKernelGGL((k_frame_bnd<D, M, T>), dim3(grid), dim3(WG), 0, c->stream, g, X(x, l),
+                           c->d_box.as<int>(), L.blk0, L.nblk, bc);
+        if (interp && l > 0)
+            hipLaunchKernelGGL((k_frame_interp<D, M, T>), dim3(grid), dim3(WG), 0, c->stream, g, X(x, l - 1),
+                               c->d_box.as<int>(), c->d_box_slots.as<int2>(), L.blk0, L.nblk);
+    }
+    // copyto! / diffto! include the ghosts (src/blockfield.jl:557-576)
+    void frame_copy(FieldBuf &dst, FieldBuf &src, int l) {
+        const LevelInfo &L = c->lv[l];
+        const size_t n = (size_t)L.nblk * FP, o = (size_t)L.blk0 * FP;
+        if (n) hipLaunchKernelGGL((k_copy<T>), dim3(nwg(n)), dim3(WG), 0, c->stream, frame(dst) + o, frame(src) + o, n);
+    }
+    void frame_diff(FieldBuf &dst, FieldBuf &a1, FieldBuf &a2, int l) {
+        const LevelInfo &L = c->lv[l];
+        const size_t n = (size_t)L.nblk * FP, o = (size_t)L.blk0 * FP;
+        if (n)
+            hipLaunchKernelGGL((k_diff<T>), dim3(nwg(n)), dim3(WG), 0, c->stream, frame(dst) + o, frame(a1) + o,
+                               frame(a2) + o, n);
+    }
     void gs4_iter(FieldBuf &u, FieldBuf &b, double sl, double omega, int n, int l, const Bc27<T> &bc) {
         const LevelInfo &L = c->lv[l];
         if (!L.nblk) return;
         constexpr int NCC = D == 3 ? (M / 2) * (M / 2) * M : (M / 2) * (M / 2);
         for (int it = 0; it < n; it++)
-            for (int col = 0; col < 4; col++)
+            for (int col = 0; col < 4; col++) {
                 hipLaunchKernelGGL((k_gs4_pass<D, M, T>), dim3(nwg((int64_t)L.nblk * NCC)), dim3(WG), 0, c->stream,
-                                   X(u, l), X(b, l), c->d_box.as<int>(), coord(), rd4, L.blk0, L.nblk, (T)sl,
-                                   (T)omega, bc, col >> 1, col & 1);
+                                   X(u, l), X(b, l), frame(u), coord(), rd4, L.blk0, L.nblk, (T)sl, (T)omega,
+                                   col >> 1, col & 1);
+                frame_fill(u, l, bc, false);
+            }
     }
     void vcycle4(FieldBuf &u, FieldBuf &b, FieldBuf &r, FieldBuf &u1, double s, const srmg_op *op, int lmax,
                  int ndown, int ntop, int nup, double omega) {
@@ -961,39 +1005,44 @@ struct Ops {
         rd4 = op->geometry == SRMG_CYL1 ? 0 : (op->geometry == SRMG_CYL2 ? 1 : -1);
         if (lmax == 0) lmax = c->nlevels;
         for (FieldBuf *f : {&u, &b, &r, &u1}) settle(*f);
-        const LevelArgs<T> ac0 = level_args<T>(c, 0, 1.0, op, 1.0);  // mirror signs for the face reads of interp!
         for (int l = lmax - 1; l >= 1; l--) {
             const double sl = s / (double)((int64_t)1 << (2 * l));
             const LevelInfo &L = c->lv[l];
             gs4_iter(u, b, sl, omega, ndown, l, bc);
             restrict_(u, l);
+            frame_fill(u, l - 1, bc);
             copy(u1, u, l - 1);
+            frame_copy(u1, u, l - 1);
             if (L.nblk)
                 hipLaunchKernelGGL((k_resid4<D, M, T>), dim3(nwg((int64_t)L.nblk * LY::V)), dim3(WG), 0, c->stream,
-                                   X(r, l), X(u, l), X(b, l), c->d_box.as<int>(), coord(), rd4, L.blk0, L.nblk,
-                                   (T)sl, bc);
+                                   X(r, l), X(u, l), X(b, l), frame(u), coord(), rd4, L.blk0, L.nblk, (T)sl);
             restrict_(r, l);
             if (L.nblk)
                 hipLaunchKernelGGL((k_postrestrict4<D, M, T>), dim3(nwg((int64_t)L.nblk * (LY::V / LY::NC))),
-                                   dim3(WG), 0, c->stream, X(b, l - 1), X(u, l - 1), X(r, l - 1), c->d_box.as<int>(),
+                                   dim3(WG), 0, c->stream, X(b, l - 1), X(u, l - 1), X(r, l - 1), frame(u),
                                    coord(), rd4, c->d_parent.as<int>(), c->d_sub.as<int>(), L.blk0, L.nblk,
-                                   (T)(4 * sl), bc);
+                                   (T)(4 * sl));
         }
         gs4_iter(u, b, s, omega, ntop, 0, bc);
         for (int l = 1; l < lmax; l++) {
             const double sl = s / (double)((int64_t)1 << (2 * l));
+            const LevelInfo &L = c->lv[l];
             diff(u1, u, u1, l - 1);
-            LevelArgs<T> ac = ac0;
-            interp_add(u, u1, l, ac);
+            frame_diff(u1, u, u1, l - 1);
+            frame_fill(u1, l - 1, bc);
+            if (L.nblk)
+                hipLaunchKernelGGL((k_interp_add4<D, M, T>), dim3(nwg((int64_t)L.nblk * LY::V)), dim3(WG), 0,
+                                   c->stream, X(u, l), X(u1, l - 1), frame(u1), c->d_parent.as<int>(),
+                                   c->d_sub.as<int>(), L.blk0, L.nblk);
+            frame_fill(u, l, bc);
             gs4_iter(u, b, sl, omega, nup, l, bc);
+            frame_fill(u, l, bc);
         }
     }
     // resnorm! (src/multigrid.jl:391-417) with the order-4 operator: the
     // residual of every level into r (the reference overwrites r too), then
     // the weighted leaf sum of its squares in bnorm_partials' fixed order
     void resnorm4(FieldBuf &r, FieldBuf &u, FieldBuf &b, double s, const srmg_op *op) {
-        Bc27<T> bc{};
-        for (int k = 0; k < 27; k++) bc.v[k] = (T)op->bc[k];
         rd4 = op->geometry == SRMG_CYL1 ? 0 : (op->geometry == SRMG_CYL2 ? 1 : -1);
         for (FieldBuf *f : {&u, &b, &r}) settle(*f);
         for (int l = 0; l < c->nlevels; l++) {
@@ -1001,8 +1050,7 @@ struct Ops {
             if (!L.nblk) continue;
             const double sl = s / (double)((int64_t)1 << (2 * l));
             hipLaunchKernelGGL((k_resid4<D, M, T>), dim3(nwg((int64_t)L.nblk * LY::V)), dim3(WG), 0, c->stream,
-                               X(r, l), X(u, l), X(b, l), c->d_box.as<int>(), coord(), rd4, L.blk0, L.nblk,
-                               (T)sl, bc);
+                               X(r, l), X(u, l), X(b, l), frame(u), coord(), rd4, L.blk0, L.nblk, (T)sl);
         }
         bnorm_partials(r);
     }
@@ -2335,50 +2383,60 @@ static void do_vcycle(srmg_ctx *c, int u, int b, int r, int u1, int lmax, int nd
     HIPCHK(hipGetLastError());
 }
 
-// The box neighbour table of the order-4 operator (k_gs4_pass): every level
-// must be complete (no coarse-fine faces, whose corner and edge ghosts are not
-// built), single tree.
+// The box neighbour table of the order-4 operator: for every block and each
+// of the 3^D - 1 directions, the same-level neighbour, the mask of the
+// dimensions in which the direction leaves the domain, or a coarse-fine slot
+// (the coarse block under the position and its subblock code), as
+// connectivity! with the BoxStencil classifies them. Single tree.
 static void box_table(srmg_ctx *c) {
     if (c->box_ok) return;
     const int D = c->D, NB3 = D == 3 ? 27 : 9;
     if (c->trees.size() != 1) fail("the order-4 operator runs on single-tree contexts");
+    std::vector<std::unordered_map<uint64_t, int>> at(c->nlevels);
+    for (int bk = 0; bk < c->nb; bk++) {
+        int64_t q[3] = {1, 1, 1};
+        for (int d = 0; d < D; d++) q[d] = c->h_coord[(size_t)bk * D + d];
+        at[c->h_level[bk]][morton(q, D)] = bk;
+    }
     std::vector<int> box((size_t)c->nb * NB3, 0);
-    for (int l = 0; l < c->nlevels; l++) {
-        const LevelInfo &L = c->lv[l];
-        int64_t ext[3] = {1, 1, 1}, n = 1;
-        for (int d = 0; d < D; d++) {
-            ext[d] = (int64_t)c->trees[0].ext1[d] << l;
-            n *= ext[d];
-        }
-        if (L.nblk != n) fail("the order-4 operator needs complete levels (no coarse-fine faces)");
-        std::vector<int> at((size_t)n, -1);
-        auto lin = [&](const int64_t *q) {
-            int64_t k = 0, m = 1;
-            for (int d = 0; d < D; d++) {
-                k += (q[d] - 1) * m;
-                m *= ext[d];
-            }
-            return k;
-        };
-        for (int bk = L.blk0; bk < L.blk0 + L.nblk; bk++) {
+    std::vector<int2> slots;
+    for (int bk = 0; bk < c->nb; bk++) {
+        const int l = c->h_level[bk];
+        for (int k = 0; k < NB3; k++) {
             int64_t q[3] = {1, 1, 1};
-            for (int d = 0; d < D; d++) q[d] = c->h_coord[(size_t)bk * D + d];
-            at[(size_t)lin(q)] = bk;
-        }
-        for (int bk = L.blk0; bk < L.blk0 + L.nblk; bk++)
-            for (int k = 0; k < NB3; k++) {
-                int64_t q[3] = {1, 1, 1};
-                int mask = 0;
-                for (int d = 0; d < D; d++) {
-                    const int dir = (k / (d == 0 ? 1 : (d == 1 ? 3 : 9))) % 3 - 1;
-                    q[d] = c->h_coord[(size_t)bk * D + d] + dir;
-                    if (q[d] < 1 || q[d] > ext[d]) mask |= 1 << d;
-                }
-                box[(size_t)bk * NB3 + k] = mask ? -(1 + mask) : at[(size_t)lin(q)];
+            int mask = 0;
+            for (int d = 0; d < D; d++) {
+                const int dir = (k / (d == 0 ? 1 : (d == 1 ? 3 : 9))) % 3 - 1;
+                q[d] = c->h_coord[(size_t)bk * D + d] + dir;
+                if (q[d] < 1 || q[d] > ((int64_t)c->trees[0].ext1[d] << l)) mask |= 1 << d;
             }
+            int &e = box[(size_t)bk * NB3 + k];
+            if (mask) {
+                e = -(1 + mask);
+                continue;
+            }
+            auto it = at[l].find(morton(q, D));
+            if (it != at[l].end()) {
+                e = it->second;
+                continue;
+            }
+            int64_t pc[3] = {1, 1, 1};
+            int sb = 0;
+            for (int d = 0; d < D; d++) {
+                pc[d] = (q[d] - 1) / 2 + 1;
+                sb |= (int)((q[d] - 1) & 1) << d;
+            }
+            auto ip = l > 0 ? at[l - 1].find(morton(pc, D)) : at[0].end();
+            if (l == 0 || ip == at[l - 1].end()) fail("proper nesting broken for the box stencil");
+            e = -16 - (int)slots.size();
+            slots.push_back(make_int2(ip->second, sb));
+        }
     }
     c->d_box.alloc(box.size() * sizeof(int));
     HIPCHK(hipMemcpy(c->d_box.p, box.data(), box.size() * sizeof(int), hipMemcpyHostToDevice));
+    c->d_box_slots.alloc(std::max<size_t>(slots.size(), 1) * sizeof(int2));
+    if (!slots.empty())
+        HIPCHK(hipMemcpy(c->d_box_slots.p, slots.data(), slots.size() * sizeof(int2), hipMemcpyHostToDevice));
     c->box_ok = true;
 }
 
@@ -2399,6 +2457,24 @@ static void do_vcycle4(srmg_ctx *c, int u, int b, int r, int u1, double s, const
         o.vcycle4(field(c, u), field(c, b), field(c, r), field(c, u1), s, op, lmax, ndown, ntop, nup, omega);
     });
     HIPCHK(hipGetLastError());
+}
+
+int srmg_fill_ghost4(srmg_ctx *c, int id, const int8_t *bc) {
+    return guarded([&] {
+        FieldBuf &f = field(c, id);
+        check_tree(c);
+        if (!bc) fail("null boundary table");
+        if (f.dtype != SRMG_F64) fail("the order-4 operator runs in FP64");
+        box_table(c);
+        dispatch(c, SRMG_F64, [&](auto &o) {
+            using T = std::remove_reference_t<decltype(*o.X(f, 0))>;
+            Bc27<T> b27{};
+            for (int k = 0; k < 27; k++) b27.v[k] = (T)bc[k];
+            o.settle(f);
+            for (int l = 0; l < c->nlevels; l++) o.frame_fill(f, l, b27);
+        });
+        HIPCHK(hipGetLastError());
+    });
 }
 
 int srmg_vcycle(srmg_ctx *c, int u, int b, int r, int u1, double s, const srmg_op *op, int lmax, int ndown,

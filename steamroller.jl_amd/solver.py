@@ -132,6 +132,7 @@ def lib():
         "srmg_regrid_forest": [P, i32, P, P] + [P] * 10 + [P, P, i32, P],
         "srmg_connectivity": [i32, i32, P, P, P, i32] + [P, P, i64] * 4,
         "srmg_restrict_full": [P, i32],
+        "srmg_fill_ghost4": [P, i32, P],
         "srmg_rhs": [P, i32, i32, P],
         "srmg_field_create": [P, i32, C.POINTER(i32)],
         "srmg_field_destroy": [P, i32],
@@ -472,8 +473,14 @@ def extrapolate_const(D: int) -> np.ndarray:
     return np.ones((3,) * D, np.int8)
 
 
-def fill_ghost_all(u: DeviceField, bc):
+def fill_ghost_all(u: DeviceField, bc, lpldisc=None):
+    """fill_ghost!(u, l, conn, bc) for every level (src/streamer.jl:323-325);
+    with LaplacianDiscretization(D, 4) the order-4 ghosts (corners and edges
+    included, ``srmg_fill_ghost4``)."""
     b = _bc_arr(bc)
+    if lpldisc is not None and getattr(lpldisc, "P", 2) == 4:
+        _check(lib().srmg_fill_ghost4(u.ctx.h, u.id, _p(b)))
+        return
     _check(lib().srmg_fill_ghost_all(u.ctx.h, u.id, _p(b)))
 
 
@@ -511,7 +518,7 @@ def poisson_step(u, q1, r, u1, s, geometry, pbc, lpl, *, poisson_iter=1, poisson
         freebc.setfreebc(q1, r, u1, s, pbc, lpl, poisson_iter=poisson_iter, nup=nup, ndown=ndown, ntop=ntop,
                          poisson_fmg=poisson_fmg)
     for _ in range(int(poisson_iter)):
-        fill_ghost_all(u, pbc)
+        fill_ghost_all(u, pbc, lpl)
         if poisson_fmg:
             fmg(u, q1, r, u1, s, geometry=geometry, bc=pbc, lpldisc=lpl, nup=nup, ndown=ndown, ntop=ntop,
                 coarse=coarse)

@@ -39,8 +39,8 @@ def test_order4_vcycles_match_oracle(name):
     u, q, q1, r, u1 = (ctx.field(F64) for _ in range(5))
     q.upload(p.b)
     solver.rhs(q1, q, geom, p.bc, disc)
-    solver.fill_ghost_all(u, p.bc)
     for _ in range(3):
+        solver.fill_ghost_all(u, p.bc, disc)
         solver.vcycle(u, q1, r, u1, p.s, p.tree, p.conn, geom, p.bc, disc, ndown=2, ntop=4, nup=2)
     got, gb = p.b.like(), p.b.like()
     u.download(got)
@@ -58,11 +58,51 @@ def test_order4_vcycles_match_oracle(name):
     assert np.max(np.abs(uo.u[vs])) > 0
 
 
-def test_order4_refuses_coarse_fine_faces():
-    p = configs.make_cfg1(0)
+AMR = {
+    "amr2-cyl1": lambda: (__import__("regrid_cases").problem(2), 1),
+    "amr3": lambda: (__import__("regrid_cases").problem(3), 0),
+    "cfg1-cart": lambda: (configs.make_cfg1(0), 0),
+}
+
+
+@pytest.mark.parametrize("name", list(AMR))
+def test_order4_vcycles_on_amr_trees_match_oracle(name):
+    # coarse-fine faces, edges and corners: the explicit ghost frames filled in
+    # the reference's order (copy, boundary, interpolation), warm restarts
+    p, gcode = AMR[name]()
+    print(f"case {name}: {p.tree.nblocks()} blocks, levels {[len(l.pairs) for l in p.tree]}", flush=True)
+    geom, disc = solver.geometry_from_code(gcode), solver.LaplacianDiscretization(p.D, 4)
+    ctx = solver.Context(p.tree, p.conn, p.M)
+    u, q, q1, r, u1 = (ctx.field(F64) for _ in range(5))
+    q.upload(p.b)
+    solver.rhs(q1, q, geom, p.bc, disc)
+    for _ in range(3):
+        solver.fill_ghost_all(u, p.bc, disc)
+        solver.vcycle(u, q1, r, u1, p.s, p.tree, p.conn, geom, p.bc, disc, ndown=2, ntop=4, nup=2)
+    got, gb = p.b.like(), p.b.like()
+    u.download(got)
+    q1.download(gb)
+    from oracle import Oracle
+    from srmg.tree import box_stencil, connectivity
+    o = Oracle(p.tree, connectivity(p.tree, box_stencil(p.D)), p.M, p.b.G, gcode, p.bc, order=4)
+    uo, qo, q1o, ro, u1o = p.b.like(), p.b.copy(), p.b.like(), p.b.like(), p.b.like()
+    o.rhs(q1o, qo, order=4)
+    for _ in range(3):
+        o.fill_ghost_all(uo)
+        o.vcycle(uo, q1o, ro, u1o, p.s, ndown=2, ntop=4, nup=2)
+    G, M = p.b.G, p.M
+    vs = (slice(G, G + M),) * p.D
+    np.testing.assert_array_equal(got.u[vs], uo.u[vs])
+    np.testing.assert_array_equal(gb.u[vs], q1o.u[vs])
+
+
+def test_order4_refuses_trees_not_nested_for_the_box_stencil():
+    def crit(l, c):
+        return (c[:, 0] >= 2) & (c[:, 0] <= 4) & (c[:, 1] >= 2) & (c[:, 1] <= 5 * l)
+    p = configs.uniform_problem(2, (4, 4), 16, 3, 0, ((1, -1), (-1, -1)), refine=crit)
     ctx = solver.Context(p.tree, p.conn, p.M)
     u, b, r, u1 = (ctx.field(F64) for _ in range(4))
-    with pytest.raises(solver.SrmgError, match="complete levels"):
+    with pytest.raises(solver.SrmgError, match="nesting"):
         solver.vcycle(u, b, r, u1, p.s, p.tree, p.conn, solver.CartesianGeometry(), p.bc,
                       solver.LaplacianDiscretization(2, 4))
 
