@@ -134,3 +134,31 @@ def test_order4_resnorm_matches_oracle(name):
         assert abs(got - want) <= 1e-12 * want
         hist.append(got)
     assert hist[-1] < hist[0]
+
+
+def test_order4_poisson_step_matches_oracle():
+    # derivs!'s Poisson block with the order-4 discretization on an AMR
+    # cylindrical tree: restrict_full!, R, then warm-started cycles; two steps
+    p, gcode = AMR["amr2-cyl1"]()
+    geom, disc = solver.geometry_from_code(gcode), solver.LaplacianDiscretization(2, 4)
+    ctx = solver.Context(p.tree, p.conn, p.M)
+    u, q, q1, r, u1 = (ctx.field(F64) for _ in range(5))
+    for _ in range(2):
+        q.upload(p.b)
+        solver.poisson_step(u, q1, r, u1, p.s, geom, p.bc, disc, poisson_iter=2, q=q)
+    got = p.b.like()
+    u.download(got)
+    from oracle import Oracle
+    from srmg.tree import box_stencil, connectivity
+    o = Oracle(p.tree, connectivity(p.tree, box_stencil(2)), p.M, p.b.G, gcode, p.bc, order=4)
+    uo, ro, u1o = p.b.like(), p.b.like(), p.b.like()
+    for _ in range(2):
+        qo, q1o = p.b.copy(), p.b.like()
+        o.restrict_full(qo)
+        o.rhs(q1o, qo, order=4)
+        for _ in range(2):
+            o.fill_ghost_all(uo)
+            o.vcycle(uo, q1o, ro, u1o, p.s, ndown=2, ntop=4, nup=2)
+    G, M = p.b.G, p.M
+    vs = (slice(G, G + M),) * 2
+    np.testing.assert_array_equal(got.u[vs], uo.u[vs])
