@@ -1039,6 +1039,48 @@ struct Ops {
             frame_fill(u, l, bc);
         }
     }
+    // fmg! (src/multigrid.jl:17-51) with the order-4 operator
+    void fmg4(FieldBuf &u, FieldBuf &b, FieldBuf &r, FieldBuf &u1, double s, const srmg_op *op, int ndown, int ntop,
+              int nup, double omega) {
+        Bc27<T> bc{};
+        for (int k = 0; k < 27; k++) bc.v[k] = (T)op->bc[k];
+        rd4 = op->geometry == SRMG_CYL1 ? 0 : (op->geometry == SRMG_CYL2 ? 1 : -1);
+        int lmax = 0;
+        for (int l = 0; l < c->nlevels; l++)
+            if (c->lv[l].nblk) lmax = l + 1;
+        for (FieldBuf *f : {&u, &b, &r, &u1}) settle(*f);
+        for (int l = lmax - 1; l >= 1; l--) {
+            const double sl = s / (double)((int64_t)1 << (2 * l));
+            const LevelInfo &L = c->lv[l];
+            restrict_(u, l);
+            frame_fill(u, l - 1, bc);
+            if (L.nblk)
+                hipLaunchKernelGGL((k_resid4<D, M, T>), dim3(nwg((int64_t)L.nblk * LY::V)), dim3(WG), 0, c->stream,
+                                   X(r, l), X(u, l), X(b, l), frame(u), coord(), rd4, L.blk0, L.nblk, (T)sl);
+            restrict_(r, l);
+            if (L.nblk)
+                hipLaunchKernelGGL((k_postrestrict4<D, M, T>), dim3(nwg((int64_t)L.nblk * (LY::V / LY::NC))),
+                                   dim3(WG), 0, c->stream, X(b, l - 1), X(u, l - 1), X(r, l - 1), frame(u),
+                                   coord(), rd4, c->d_parent.as<int>(), c->d_sub.as<int>(), L.blk0, L.nblk,
+                                   (T)(4 * sl));
+        }
+        for (int l = 0; l < lmax; l++) {
+            copy(u1, u, l);
+            frame_copy(u1, u, l);
+            if (l > 0) {
+                const LevelInfo &L = c->lv[l];
+                diff(u1, u, u1, l - 1);
+                frame_diff(u1, u, u1, l - 1);
+                frame_fill(u1, l - 1, bc);
+                if (L.nblk)
+                    hipLaunchKernelGGL((k_interp_add4<D, M, T>), dim3(nwg((int64_t)L.nblk * LY::V)), dim3(WG), 0,
+                                       c->stream, X(u, l), X(u1, l - 1), frame(u1), c->d_parent.as<int>(),
+                                       c->d_sub.as<int>(), L.blk0, L.nblk);
+            }
+            frame_fill(u, l, bc);
+            vcycle4(u, b, r, u1, s, op, l + 1, ndown, ntop, nup, omega);
+        }
+    }
     // resnorm! (src/multigrid.jl:391-417) with the order-4 operator: the
     // residual of every level into r (the reference overwrites r too), then
     // the weighted leaf sum of its squares in bnorm_partials' fixed order
@@ -2506,6 +2548,17 @@ int srmg_fmg(srmg_ctx *c, int u, int b, int r, int u1, double s, const srmg_op *
     return guarded([&] {
         check_tree(c);
         if (!op) fail("null op");
+        if (op->disc == SRMG_LAPLACIAN4) {
+            check_order4_geometry(c, op);
+            if (op->coarse != SRMG_COARSE_GS) fail("the order-4 operator uses the reference's level-1 sweeps");
+            if (same_type(c, {u, b, r, u1}) != SRMG_F64) fail("the order-4 operator runs in FP64");
+            box_table(c);
+            dispatch(c, SRMG_F64, [&](auto &o) {
+                o.fmg4(field(c, u), field(c, b), field(c, r), field(c, u1), s, op, ndown, ntop, nup, omega);
+            });
+            HIPCHK(hipGetLastError());
+            return;
+        }
         prepare(c, &s, op, omega, true);
         int t = same_type(c, {u, b, r, u1});
         dispatch(c, t, [&](auto &o) {

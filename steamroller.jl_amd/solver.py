@@ -496,8 +496,9 @@ def vcycle(u, b, r, u1, s, tree=None, conn=None, geometry=None, bc=None, lpldisc
 
 def fmg(u, b, r, u1, s, tree=None, conn=None, geometry=None, bc=None, lpldisc=None, *,
         lmax=0, ndown=3, ntop=5, nup=3, omega=1.25, coarse=COARSE_GS):
-    """fmg! (src/multigrid.jl:17-51) on device-resident fields."""
-    op = make_op(geometry, bc, lpldisc, coarse)
+    """fmg! (src/multigrid.jl:17-51) on device-resident fields (order 4 as
+    :func:`vcycle`)."""
+    op = make_op(geometry, bc, lpldisc, coarse, any_order=True)
     _check(lib().srmg_fmg(u.ctx.h, u.id, b.id, r.id, u1.id, float(s), C.byref(op),
                           int(ndown), int(ntop), int(nup), float(omega)))
 

@@ -162,3 +162,31 @@ def test_order4_poisson_step_matches_oracle():
     G, M = p.b.G, p.M
     vs = (slice(G, G + M),) * 2
     np.testing.assert_array_equal(got.u[vs], uo.u[vs])
+
+
+@pytest.mark.parametrize("name", ["amr2-cyl1", "amr3"])
+def test_order4_fmg_matches_oracle(name):
+    # fmg! with the order-4 operator (omega = 1.25, the reference's default),
+    # twice from the previous result
+    p, gcode = AMR[name]()
+    geom, disc = solver.geometry_from_code(gcode), solver.LaplacianDiscretization(p.D, 4)
+    ctx = solver.Context(p.tree, p.conn, p.M)
+    u, q, q1, r, u1 = (ctx.field(F64) for _ in range(5))
+    q.upload(p.b)
+    solver.rhs(q1, q, geom, p.bc, disc)
+    for _ in range(2):
+        solver.fill_ghost_all(u, p.bc, disc)
+        solver.fmg(u, q1, r, u1, p.s, geometry=geom, bc=p.bc, lpldisc=disc, ndown=2, ntop=4, nup=2)
+    got = p.b.like()
+    u.download(got)
+    from oracle import Oracle
+    from srmg.tree import box_stencil, connectivity
+    o = Oracle(p.tree, connectivity(p.tree, box_stencil(p.D)), p.M, p.b.G, gcode, p.bc, order=4)
+    uo, qo, q1o, ro, u1o = p.b.like(), p.b.copy(), p.b.like(), p.b.like(), p.b.like()
+    o.rhs(q1o, qo, order=4)
+    for _ in range(2):
+        o.fill_ghost_all(uo)
+        o.fmg(uo, q1o, ro, u1o, p.s, ndown=2, ntop=4, nup=2)
+    G, M = p.b.G, p.M
+    vs = (slice(G, G + M),) * p.D
+    np.testing.assert_array_equal(got.u[vs], uo.u[vs])
