@@ -45,6 +45,7 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched photoionization measurement")
     ap.add_argument("--no-regrid", action="store_true", help="skip the regrid measurement")
+    ap.add_argument("--no-order4", action="store_true", help="skip the order-4 measurement")
     return ap.parse_args()
 
 
@@ -164,6 +165,7 @@ def main():
         log(f"[bench] cpu: {cpu}")
 
     batched = None if args.no_batched else batched_cfg3(log)
+    order4 = None if args.no_order4 else order4_cfg1(log)
     # last: it changes the context's tree
     regrid = None if args.no_regrid else regrid_step(log, p, ctx, u, b, e)
     traffic = committed_traffic(kname["float"], args.config)
@@ -213,6 +215,7 @@ def main():
         "cpu_baseline": cpu,
         "batched_photoionization": batched,
         "regrid": regrid,
+        "order4_cfg1": order4,
     }
     print(json.dumps(out), flush=True)
 
@@ -290,6 +293,42 @@ def batched_cfg3(log, ntrees=64):
         log(f"[bench] batched {key}: {res[key]}")
     res["workload"] = (f"cfg3: 3 photoionization Helmholtz terms on the cfg1 tree + {ntrees} random trees, "
                        f"{len(probs)} problems, {leaf} leaf cells, to 1e-8 each")
+    return res
+
+
+def order4_cfg1(log, cycles=10):
+    """V-cycles (2, 4, 2) of the order-2 and the compact order-4 Poisson
+    operator on the cfg1 tree (2D cylindrical AMR, FP64), from u = 0 with the
+    same charge: milliseconds per cycle and the relative residual reached
+    (resnorm! of each operator). The order-4 path runs step by step (no
+    fusion or graphs) with its explicit corner and edge ghosts."""
+    from srmg import configs, solver
+    from srmg.solver import F64
+    p = configs.make_cfg1(0)
+    geom = solver.geometry_from_code(p.geom)
+    res = {}
+    for order in (2, 4):
+        disc = solver.LaplacianDiscretization(p.D, order)
+        ctx = solver.Context(p.tree, p.conn, p.M)
+        u, q, q1, r, u1 = (ctx.field(F64) for _ in range(5))
+        q.upload(p.b)
+        solver.rhs(q1, q, geom, p.bc, disc)
+        solver.fill_ghost_all(u, p.bc, disc)
+        r0 = solver.resnorm(r, q1, u, p.s, geometry=geom, bc=p.bc, lpldisc=disc)
+        solver.vcycle(u, q1, r, u1, p.s, p.tree, p.conn, geom, p.bc, disc, ndown=2, ntop=4, nup=2)  # warm-up
+        ctx.synchronize()
+        t = time.perf_counter()
+        for _ in range(cycles):
+            solver.fill_ghost_all(u, p.bc, disc)
+            solver.vcycle(u, q1, r, u1, p.s, p.tree, p.conn, geom, p.bc, disc, ndown=2, ntop=4, nup=2)
+        ctx.synchronize()
+        ms = 1e3 * (time.perf_counter() - t) / cycles
+        rn = solver.resnorm(r, q1, u, p.s, geometry=geom, bc=p.bc, lpldisc=disc)
+        res[f"order{order}"] = {"ms_per_cycle": round(ms, 3), "rel_residual_after": float(f"{rn / r0:.4g}"),
+                                "cycles": cycles + 1}
+    res["workload"] = (f"cfg1 tree ({p.tree.nblocks()} blocks of {p.M}^2, levels {[len(l) for l in p.tree]}), "
+                       "cylindrical, FP64, V(2,4,2) with the reference's level-1 sweeps")
+    log(f"[bench] order 4: {res}")
     return res
 
 
