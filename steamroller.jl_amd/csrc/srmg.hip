@@ -2495,8 +2495,34 @@ static void do_vcycle4(srmg_ctx *c, int u, int b, int r, int u1, double s, const
     const int t = same_type(c, {u, b, r, u1});
     if (t != SRMG_F64) fail("the order-4 operator runs in FP64");
     box_table(c);
+    FieldBuf &fu = field(c, u), &fb = field(c, b), &fr = field(c, r), &fu1 = field(c, u1);
+    // the cycle is launch-bound on small trees: it is captured once per
+    // configuration and replayed (frames made and fields settled first, so
+    // the captured launches allocate nothing and copy no ping-pong buffers)
     dispatch(c, t, [&](auto &o) {
-        o.vcycle4(field(c, u), field(c, b), field(c, r), field(c, u1), s, op, lmax, ndown, ntop, nup, omega);
+        for (FieldBuf *f : {&fu, &fb, &fr, &fu1}) o.settle(*f);
+        o.frame(fu);
+        o.frame(fu1);
+    });
+    std::string k = "order4";
+    auto add = [&](const auto &v) { k.append(reinterpret_cast<const char *>(&v), sizeof(v)); };
+    for (FieldBuf *f : {&fu, &fb, &fr, &fu1}) {
+        add(f->data.p);
+        add(f->frame.p);
+    }
+    add(s);
+    add(op->geometry);
+    k.append(reinterpret_cast<const char *>(op->bc), 27);
+    add(lmax);
+    add(ndown);
+    add(ntop);
+    add(nup);
+    add(omega);
+    auto &slot = c->graphs[k];
+    if (!slot) slot = std::make_unique<Graph>();
+    slot->gen = c->graph_gen;
+    run_maybe_graph(c, *slot, true, [&] {
+        dispatch(c, t, [&](auto &o) { o.vcycle4(fu, fb, fr, fu1, s, op, lmax, ndown, ntop, nup, omega); });
     });
     HIPCHK(hipGetLastError());
 }
