@@ -2633,12 +2633,60 @@ int srmg_synchronize(srmg_ctx *c) {
     return guarded([&] { HIPCHK(hipStreamSynchronize(c->stream)); });
 }
 
+// srmg_solve with the order-4 operator: the ghost pre-fill of u, then
+// V-cycles (captured) and resnorm! until the relative residual is <= tol;
+// resnorm(0) from b alone (the residual of u = 0 is -b)
+static void solve4(srmg_ctx *c, int ui, int bi, int ri, int u1i, double s, const srmg_op *op,
+                   const srmg_solve_opts *o, int *cycles, double *hist) {
+    check_order4_geometry(c, op);
+    if (op->coarse != SRMG_COARSE_GS) fail("the order-4 operator uses the reference's level-1 sweeps");
+    if (o->guess != SRMG_GUESS_NONE && o->guess != SRMG_GUESS_ZERO)
+        fail("the order-4 solve starts from u or from zero");
+    if (same_type(c, {ui, bi, ri, u1i}) != SRMG_F64) fail("the order-4 operator runs in FP64");
+    box_table(c);
+    FieldBuf &u = field(c, ui), &b = field(c, bi), &r = field(c, ri);
+    std::vector<double> r0;
+    bool done = false;
+    dispatch(c, SRMG_F64, [&](auto &ops) {
+        using T = std::remove_reference_t<decltype(*ops.X(u, 0))>;
+        ops.settle(b);
+        ops.bnorm_partials(b);
+        r0 = safe_r0(read_norms(c));
+        if (o->guess == SRMG_GUESS_ZERO) {
+            ops.zero_all(u);
+            const size_t n = (size_t)c->nb * ops.FP;
+            hipLaunchKernelGGL(k_fill<T>, dim3(nwg(n)), dim3(WG), 0, c->stream, ops.frame(u), (T)0, n);
+            done = record(r0, r0, o->tol, 0, o->max_cycles, hist);
+        } else {
+            Bc27<T> b27{};
+            for (int k = 0; k < 27; k++) b27.v[k] = (T)op->bc[k];
+            ops.settle(u);
+            for (int l = 0; l < c->nlevels; l++) ops.frame_fill(u, l, b27);
+            ops.resnorm4(r, u, b, s, op);
+            done = record(read_norms(c), r0, o->tol, 0, o->max_cycles, hist);
+        }
+    });
+    int k = 0;
+    while (k < o->max_cycles && !done) {
+        do_vcycle4(c, ui, bi, ri, u1i, s, op, 0, o->ndown, o->ntop, o->nup, o->omega);
+        dispatch(c, SRMG_F64, [&](auto &ops) { ops.resnorm4(r, u, b, s, op); });
+        k++;
+        done = record(read_norms(c), r0, o->tol, k, o->max_cycles, hist);
+    }
+    if (cycles) *cycles = k;
+}
+
 int srmg_solve(srmg_ctx *c, int u, int b, int r, int u1, double s, const srmg_op *op,
                const srmg_solve_opts *opts, int *cycles, double *hist) {
     return guarded([&] {
         check_tree(c);
         if (!op) fail("null op");
         check_opts(opts);
+        if (op->disc == SRMG_LAPLACIAN4) {
+            if (c->trees.size() != 1) fail("the order-4 operator runs on single-tree contexts");
+            solve4(c, u, b, r, u1, s, op, opts, cycles, hist);
+            return;
+        }
         prepare(c, &s, op, opts->omega, true);
         solve_plain(c, u, b, r, u1, opts, cycles, hist);
     });

@@ -554,8 +554,9 @@ def _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph, inner_cycles=1, g
 def solve(u, b, r, u1, s, geometry, bc, lpldisc, *, tol=1e-8, max_cycles=50, ndown=2, ntop=4,
           nup=2, omega=1.0, coarse=COARSE_EXACT, use_graph=True, guess=GUESS_NONE, guess_smooth=GUESS_SMOOTH):
     """Pre-fill ghosts, then V-cycles until resnorm(u) <= tol * resnorm(0).
-    Returns (cycles, relative residual history)."""
-    op = make_op(geometry, bc, lpldisc, coarse)
+    Returns (cycles, relative residual history). LaplacianDiscretization(D, 4)
+    needs coarse=COARSE_GS (the reference's level-1 sweeps)."""
+    op = make_op(geometry, bc, lpldisc, coarse, any_order=True)
     o = _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph, 1, guess, guess_smooth)
     hist = np.zeros(max_cycles + 1)
     n = C.c_int()

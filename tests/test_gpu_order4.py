@@ -190,3 +190,37 @@ def test_order4_fmg_matches_oracle(name):
     G, M = p.b.G, p.M
     vs = (slice(G, G + M),) * p.D
     np.testing.assert_array_equal(got.u[vs], uo.u[vs])
+
+
+@pytest.mark.parametrize("name", ["amr2-cyl1", "amr3"])
+def test_order4_solve_equals_its_cycles(name):
+    # srmg_solve with the order-4 operator: the pre-fill, then captured cycles
+    # and resnorm!; the same bits as the same calls one by one, and its history
+    p, gcode = AMR[name]()
+    geom, disc = solver.geometry_from_code(gcode), solver.LaplacianDiscretization(p.D, 4)
+    out = []
+    for mode in ("solve", "calls"):
+        ctx = solver.Context(p.tree, p.conn, p.M)
+        u, q, q1, r, u1 = (ctx.field(F64) for _ in range(5))
+        q.upload(p.b)
+        solver.rhs(q1, q, geom, p.bc, disc)
+        if mode == "solve":
+            n, hist = solver.solve(u, q1, r, u1, p.s, geom, p.bc, disc, tol=1e-30, max_cycles=3,
+                                   coarse=solver.COARSE_GS, ndown=2, ntop=4, nup=2)
+            assert n == 3
+        else:
+            zero = ctx.field(F64)
+            solver.fill_ghost_all(zero, p.bc, disc)
+            r0 = solver.resnorm(r, q1, zero, p.s, geometry=geom, bc=p.bc, lpldisc=disc)
+            solver.fill_ghost_all(u, p.bc, disc)
+            hist = [solver.resnorm(r, q1, u, p.s, geometry=geom, bc=p.bc, lpldisc=disc) / r0]
+            for _ in range(3):
+                solver.vcycle(u, q1, r, u1, p.s, p.tree, p.conn, geom, p.bc, disc, ndown=2, ntop=4, nup=2)
+                hist.append(solver.resnorm(r, q1, u, p.s, geometry=geom, bc=p.bc, lpldisc=disc) / r0)
+        host = p.b.like()
+        u.download(host)
+        out.append((host.u.copy(), np.asarray(hist)))
+    G, M = p.b.G, p.M
+    vs = (slice(G, G + M),) * p.D
+    np.testing.assert_array_equal(out[0][0][vs], out[1][0][vs])
+    np.testing.assert_array_equal(out[0][1], out[1][1])
