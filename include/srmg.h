@@ -45,8 +45,9 @@ enum { SRMG_CARTESIAN = 0, SRMG_CYL1 = 1, SRMG_CYL2 = 2 };
  * V-cycle with the FourColors smoother (src/multigrid.jl:206-232, 268-295) for
  * Cartesian and cylindrical geometry, FP64 fields, the reference's level-1
  * sweeps and single trees that are properly nested for the BoxStencil (its
- * ghosts: srmg_fill_ghost4), srmg_fmg its FMG, srmg_solve its solve to a
- * tolerance (guess NONE or ZERO), and srmg_resnorm its residual norm (r receives
+ * ghosts: srmg_fill_ghost4), srmg_fmg its FMG, srmg_solve and
+ * srmg_solve_mixed its solves to a tolerance (guess NONE or ZERO; the mixed
+ * form with FP32 order-4 correction cycles), and srmg_resnorm its residual norm (r receives
  * the residual, as resnorm! overwrites r); the other solve entry points
  * refuse it. */
 enum { SRMG_LAPLACIAN = 0, SRMG_HELMHOLTZ = 1, SRMG_LAPLACIAN4 = 2 };

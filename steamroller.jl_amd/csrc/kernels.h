@@ -425,6 +425,26 @@ __device__ __forceinline__ T apply4(const T *__restrict__ x, const T *__restrict
     return stencil4<D, M, T>(x, g, blk, c0, c1, c2);
 }
 
+// the FP32 right-hand side of the order-4 correction equation of a mixed
+// precision step: be = b + L4 u / s in FP64, rounded once
+template <int D, int M>
+__global__ __launch_bounds__(WG) void k_defect4(float *__restrict__ be, const double *__restrict__ u,
+                                                const double *__restrict__ b, const double *__restrict__ g,
+                                                const int *__restrict__ coord, int rd, int blk0, int nblk,
+                                                double s) {
+    using LY = Lay<D, M>;
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lb = gt / LY::V;
+    if (lb >= nblk) return;
+    const int blk = blk0 + lb, q = gt - lb * LY::V;
+    int c0, c1, c2;
+    LY::decode(q, c0, c1, c2);
+    const size_t o = (size_t)blk * LY::V + q;
+    double c;
+    const double Lu = apply4<D, M, double>(u, g, coord, blk, c0, c1, c2, rd, c);
+    be[o] = (float)(b[o] + Lu / s);
+}
+
 // gauss_seidel!(..., ::FourColors{P1,P2}) (src/multigrid.jl:206-232) over a
 // level: colour cells (x = P1, y = P2 mod 2 from the first valid cell; in 3D
 // both shifted by the 1-based third index with G = 2 ghost layers, the

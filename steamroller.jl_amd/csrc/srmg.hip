@@ -1982,6 +1982,73 @@ static void solve_mixed_dm(srmg_ctx *c, FieldBuf &u, FieldBuf &b, FieldBuf &e, F
     if (cycles) *cycles = k;
 }
 
+// Mixed precision with the order-4 operator: each outer step forms the FP32
+// right-hand side be = b + L4 u / s on every level, runs inner_cycles FP32
+// order-4 V-cycles on e from e = 0, adds e to u on every level and restricts u
+// finest level first, so covered cells hold their children's mean; then the
+// FP64 residual norm. No fused kernels: the steps of the order-4 path.
+template <int D, int M>
+static void solve_mixed4_dm(srmg_ctx *c, FieldBuf &u, FieldBuf &b, FieldBuf &e, FieldBuf &be, FieldBuf &re,
+                            FieldBuf &ue, double s, const srmg_op *op, const srmg_solve_opts *o, int *cycles,
+                            double *hist) {
+    using LY = Lay<D, M>;
+    Ops<D, M, double> o64(c);
+    Ops<D, M, float> o32(c);
+    const int rd = op->geometry == SRMG_CYL1 ? 0 : (op->geometry == SRMG_CYL2 ? 1 : -1);
+    Bc27<double> b64{};
+    Bc27<float> b32{};
+    for (int k = 0; k < 27; k++) {
+        b64.v[k] = (double)op->bc[k];
+        b32.v[k] = (float)op->bc[k];
+    }
+    for (FieldBuf *f : {&u, &b}) o64.settle(*f);
+    for (FieldBuf *f : {&e, &be, &re, &ue}) o32.settle(*f);
+    FieldBuf r;  // the FP64 residual of resnorm! (scratch of this solve)
+    r.dtype = SRMG_F64;
+    alloc_field(c, r);
+    o64.bnorm_partials(b);
+    const std::vector<double> r0 = safe_r0(read_norms(c));
+    bool done;
+    if (o->guess == SRMG_GUESS_ZERO) {
+        o64.zero_all(u);
+        const size_t n = (size_t)c->nb * o64.FP;
+        hipLaunchKernelGGL(k_fill<double>, dim3(nwg(n)), dim3(WG), 0, c->stream, o64.frame(u), 0.0, n);
+        done = record(r0, r0, o->tol, 0, o->max_cycles, hist);
+    } else {
+        for (int l = 0; l < c->nlevels; l++) o64.frame_fill(u, l, b64);
+        o64.resnorm4(r, u, b, s, op);
+        done = record(read_norms(c), r0, o->tol, 0, o->max_cycles, hist);
+    }
+    const size_t ncell = (size_t)c->nb * LY::V, nfr = (size_t)c->nb * o32.FP;
+    int k = 0;
+    while (k < o->max_cycles && !done) {
+        for (int l = 0; l < c->nlevels; l++) {
+            const LevelInfo &L = c->lv[l];
+            if (!L.nblk) continue;
+            const double sl = s / (double)((int64_t)1 << (2 * l));
+            hipLaunchKernelGGL((k_defect4<D, M>), dim3(nwg((int64_t)L.nblk * LY::V)), dim3(WG), 0, c->stream,
+                               be.data.as<float>(), u.data.as<double>(), b.data.as<double>(), o64.frame(u),
+                               o64.coord(), rd, L.blk0, L.nblk, sl);
+        }
+        o32.zero_all(e);
+        hipLaunchKernelGGL(k_fill<float>, dim3(nwg(nfr)), dim3(WG), 0, c->stream, o32.frame(e), 0.f, nfr);
+        for (int it = 0; it < o->inner_cycles; it++) {
+            if (it > 0)
+                for (int l = 0; l < c->nlevels; l++) o32.frame_fill(e, l, b32);
+            o32.vcycle4(e, be, re, ue, s, op, 0, o->ndown, o->ntop, o->nup, o->omega);
+        }
+        hipLaunchKernelGGL(k_add_correction, dim3(nwg(ncell)), dim3(WG), 0, c->stream, u.data.as<double>(),
+                           e.data.as<float>(), ncell);
+        for (int l = c->nlevels - 1; l >= 1; l--) o64.restrict_(u, l);
+        for (int l = 0; l < c->nlevels; l++) o64.frame_fill(u, l, b64);
+        o64.resnorm4(r, u, b, s, op);
+        k++;
+        done = record(read_norms(c), r0, o->tol, k, o->max_cycles, hist);
+    }
+    if (cycles) *cycles = k;
+    HIPCHK(hipStreamSynchronize(c->stream));  // before the scratch residual is released
+}
+
 static void solve_mixed(srmg_ctx *c, int ui, int bi, int ei, int bei, int rei, int uei, const srmg_solve_opts *o,
                         int *cycles, double *hist) {
     check_opts(o);
@@ -2709,6 +2776,29 @@ int srmg_solve_mixed(srmg_ctx *c, int u, int b, int e, int be, int re, int ue, d
         check_tree(c);
         if (!op) fail("null op");
         check_opts(opts);
+        if (op->disc == SRMG_LAPLACIAN4) {
+            check_order4_geometry(c, op);
+            if (c->trees.size() != 1) fail("the order-4 operator runs on single-tree contexts");
+            if (op->coarse != SRMG_COARSE_GS) fail("the order-4 operator uses the reference's level-1 sweeps");
+            if (opts->guess != SRMG_GUESS_NONE && opts->guess != SRMG_GUESS_ZERO)
+                fail("the order-4 solve starts from u or from zero");
+            if (same_type(c, {u, b}) != SRMG_F64) fail("u and b must be FP64");
+            if (same_type(c, {e, be, re, ue}) != SRMG_F32) fail("correction fields must be FP32");
+            box_table(c);
+            FieldBuf &fu = field(c, u), &fb = field(c, b), &fe = field(c, e), &fbe = field(c, be),
+                     &fre = field(c, re), &fue = field(c, ue);
+#define SRMG_M4CASE(DD, MM)                                                                            \
+    if (c->D == DD && c->M == MM) {                                                                    \
+        solve_mixed4_dm<DD, MM>(c, fu, fb, fe, fbe, fre, fue, s, op, opts, cycles, hist);              \
+        return;                                                                                        \
+    }
+            SRMG_M4CASE(2, 16)
+            SRMG_M4CASE(2, 8)
+            SRMG_M4CASE(3, 8)
+            SRMG_M4CASE(3, 16)
+#undef SRMG_M4CASE
+            fail("unsupported (D, M)");
+        }
         prepare(c, &s, op, opts->omega, true);
         solve_mixed(c, u, b, e, be, re, ue, opts, cycles, hist);
     });

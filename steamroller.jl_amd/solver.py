@@ -569,8 +569,9 @@ def solve_mixed(u, b, e, be, re, ue, s, geometry, bc, lpldisc, *, tol=1e-8, max_
                 ndown=2, ntop=4, nup=2, omega=1.0, coarse=COARSE_EXACT, use_graph=True, inner_cycles=1,
                 guess=GUESS_NONE, guess_smooth=GUESS_SMOOTH):
     """FP64 defect correction around FP32 V-cycles (u, b FP64; e, be, re, ue FP32).
-    Returns (outer steps, relative residual history per outer step)."""
-    op = make_op(geometry, bc, lpldisc, coarse)
+    Returns (outer steps, relative residual history per outer step).
+    LaplacianDiscretization(D, 4) needs coarse=COARSE_GS."""
+    op = make_op(geometry, bc, lpldisc, coarse, any_order=True)
     o = _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph, inner_cycles, guess, guess_smooth)
     hist = np.zeros(max_cycles + 1)
     n = C.c_int()

@@ -224,3 +224,34 @@ def test_order4_solve_equals_its_cycles(name):
     vs = (slice(G, G + M),) * p.D
     np.testing.assert_array_equal(out[0][0][vs], out[1][0][vs])
     np.testing.assert_array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("name", ["amr2-cyl1", "amr3"])
+def test_order4_mixed_precision_reaches_the_fp64_solution(name):
+    # FP64 defect correction around FP32 order-4 V-cycles: to the same
+    # tolerance as the FP64 order-4 solve, the potentials agree to 1e-6
+    p, gcode = AMR[name]()
+    geom, disc = solver.geometry_from_code(gcode), solver.LaplacianDiscretization(p.D, 4)
+    sols = []
+    for mixed in (False, True):
+        ctx = solver.Context(p.tree, p.conn, p.M)
+        u, q, q1, r, u1 = (ctx.field(F64) for _ in range(5))
+        q.upload(p.b)
+        solver.rhs(q1, q, geom, p.bc, disc)
+        # the reference's level-1 sweeps converge slowly on the 48 x 64 cylindrical level 1
+        tol, ntop = (1e-7, 100) if p.D == 2 else (1e-9, 30)
+        kw = dict(tol=tol, max_cycles=400, coarse=solver.COARSE_GS, ndown=2, ntop=ntop, nup=2)
+        if mixed:
+            e, be, re, ue = (ctx.field(solver.F32) for _ in range(4))
+            n, hist = solver.solve_mixed(u, q1, e, be, re, ue, p.s, geom, p.bc, disc, inner_cycles=2, **kw)
+        else:
+            n, hist = solver.solve(u, q1, r, u1, p.s, geom, p.bc, disc, **kw)
+        print(f"{name} mixed={mixed}: {n} cycles, final {hist[-1]:.3e}", flush=True)
+        assert hist[-1] <= tol
+        host = p.b.like()
+        u.download(host)
+        G, M = p.b.G, p.M
+        sols.append(host.u[(slice(G, G + M),) * p.D].copy())
+    err = np.max(np.abs(sols[1] - sols[0])) / np.max(np.abs(sols[0]))
+    print(f"{name}: potential rel diff {err:.2e}")
+    assert err <= (1e-5 if p.D == 2 else 1e-6)
