@@ -238,10 +238,16 @@ __device__ __forceinline__ bool frame_cell(int q, int *c, int *dir, int &k) {
     return ghost;
 }
 
-// fill_ghost_copy!: ghosts facing a same-level neighbour
+// fill_ghost_copy! then fill_ghost_bnd! in one pass. A ghost facing a
+// same-level neighbour is that neighbour's cell. A ghost beyond the domain is
+// a[bnd] times the cell mirrored in the dimensions that leave the domain; when
+// other dimensions stay outside, the mirrored cell is a ghost of a direction
+// inside the domain: a same-level neighbour's cell (read from the neighbour,
+// the value the copy writes) or a coarse-fine ghost (read from the frame, as
+// the interpolation that refills it comes after both fills).
 template <int D, int M, typename T>
-__global__ __launch_bounds__(WG) void k_frame_copy(T *__restrict__ g, const T *__restrict__ x,
-                                                   const int *__restrict__ box, int blk0, int nblk) {
+__global__ __launch_bounds__(WG) void k_frame_copybnd(T *__restrict__ g, const T *__restrict__ x,
+                                                      const int *__restrict__ box, int blk0, int nblk, Bc27<T> bc) {
     using LY = Lay<D, M>;
     constexpr int P = Frame<D, M>::P, NB3 = D == 3 ? 27 : 9;
     const int gt = blockIdx.x * blockDim.x + threadIdx.x;
@@ -251,37 +257,36 @@ __global__ __launch_bounds__(WG) void k_frame_copy(T *__restrict__ g, const T *_
     int c[3], dir[3], k;
     if (!frame_cell<D, M>(q, c, dir, k)) return;
     const int e = box[blk * NB3 + k];
-    if (e < 0) return;
+    if (e >= 0) {
 #pragma unroll
-    for (int d = 0; d < D; d++) c[d] -= dir[d] * M;
-    g[(size_t)blk * P + q] = x[(size_t)e * LY::V + LY::sidx(c[0], c[1], c[2])];
-}
-
-// fill_ghost_bnd!: ghosts beyond the domain = a[bnd] times the cell mirrored
-// in the dimensions that leave the domain (a ghost of another direction, read
-// from the frame, when other dimensions stay outside)
-template <int D, int M, typename T>
-__global__ __launch_bounds__(WG) void k_frame_bnd(T *__restrict__ g, const T *__restrict__ x,
-                                                  const int *__restrict__ box, int blk0, int nblk, Bc27<T> bc) {
-    constexpr int P = Frame<D, M>::P, NB3 = D == 3 ? 27 : 9;
-    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
-    const int lb = gt / P;
-    if (lb >= nblk) return;
-    const int q = gt - lb * P, blk = blk0 + lb;
-    int c[3], dir[3], k;
-    if (!frame_cell<D, M>(q, c, dir, k)) return;
-    const int e = box[blk * NB3 + k];
-    if (e >= 0 || e < -8) return;
+        for (int d = 0; d < D; d++) c[d] -= dir[d] * M;
+        g[(size_t)blk * P + q] = x[(size_t)e * LY::V + LY::sidx(c[0], c[1], c[2])];
+        return;
+    }
+    if (e < -8) return;
     const int mask = -1 - e;
-    int bi = 0, m = 1;
+    int bi = 0, m = 1, k2 = 0;
 #pragma unroll
     for (int d = 0; d < D; d++) {
         const bool bd = (mask >> d) & 1;
         bi += ((bd ? dir[d] : 0) + 1) * m;
-        m *= 3;
         if (bd) c[d] = dir[d] < 0 ? -1 - c[d] : 2 * M - 1 - c[d];
+        k2 += ((bd ? 0 : dir[d]) + 1) * m;
+        m *= 3;
     }
-    g[(size_t)blk * P + q] = bc.v[bi] * boxv<D, M, T>(x, g, blk, c[0], c[1], c[2]);
+    T v;
+    const int e2 = box[blk * NB3 + k2];
+    if (k2 == NB3 / 2) {
+        v = x[(size_t)blk * LY::V + LY::sidx(c[0], c[1], c[2])];
+    } else if (e2 >= 0) {
+#pragma unroll
+        for (int d = 0; d < D; d++)
+            if (!((mask >> d) & 1)) c[d] -= dir[d] * M;
+        v = x[(size_t)e2 * LY::V + LY::sidx(c[0], c[1], c[2])];
+    } else {
+        v = g[(size_t)blk * P + Frame<D, M>::at(c[0], c[1], D == 3 ? c[2] : 0)];
+    }
+    g[(size_t)blk * P + q] = bc.v[bi] * v;
 }
 
 // fill_ghost_interp! (src/connectivity.jl:251-258, src/interp.jl:23-60): a

@@ -965,9 +965,7 @@ struct Ops {
         if (!L.nblk) return;
         T *g = frame(x);
         const unsigned grid = nwg((int64_t)L.nblk * FP);
-        hipLaunchKernelGGL((k_frame_copy<D, M, T>), dim3(grid), dim3(WG), 0, c->stream, g, X(x, l),
-                           c->d_box.as<int>(), L.blk0, L.nblk);
-        hipLaunchKernelGGL((k_frame_bnd<D, M, T>), dim3(grid), dim3(WG), 0, c->stream, g, X(x, l),
+        hipLaunchKernelGGL((k_frame_copybnd<D, M, T>), dim3(grid), dim3(WG), 0, c->stream, g, X(x, l),
                            c->d_box.as<int>(), L.blk0, L.nblk, bc);
         if (interp && l > 0)
             hipLaunchKernelGGL((k_frame_interp<D, M, T>), dim3(grid), dim3(WG), 0, c->stream, g, X(x, l - 1),
