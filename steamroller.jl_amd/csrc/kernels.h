@@ -2792,8 +2792,8 @@ __global__ __launch_bounds__(256) void k_add_restrict(double *__restrict__ u, fl
     constexpr int NB = 1 << D, V = LY::V, MH = LY::MH;
     const int tid = threadIdx.x;
     if (blockIdx.x >= ngroups) {
-        for (int i = (blockIdx.x - ngroups) * 256 + tid; i < (blockIdx.x - ngroups + 1) * 256 && i < nleafc * V;
-             i += 256) {
+        const int c0 = (int)(blockIdx.x - ngroups) * 256;
+        for (int i = c0 + tid; i < c0 + 256 && i < nleafc * V; i += 256) {
             const size_t o = (size_t)leaves[i / V] * V + (size_t)(i % V);
             u[o] += (double)e[o];
             e[o] = 0.0f;

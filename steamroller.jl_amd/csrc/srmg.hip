@@ -24,9 +24,6 @@
 
 using namespace srmg;
 
-// work groups of one level's residual-norm launch (fixed, so the summation
-// order does not depend on the level size)
-constexpr unsigned RESNORM_WGS = 4096;
 // first-stage work groups of the fixed-order sum of partials
 constexpr int SUM_CHUNKS = 256;
 // per-tree totals start here in d_sum
