@@ -238,34 +238,37 @@ __device__ __forceinline__ bool frame_cell(int q, int *c, int *dir, int &k) {
     return ghost;
 }
 
-// fill_ghost_copy! then fill_ghost_bnd! in one pass. A ghost facing a
-// same-level neighbour is that neighbour's cell. A ghost beyond the domain is
-// a[bnd] times the cell mirrored in the dimensions that leave the domain; when
-// other dimensions stay outside, the mirrored cell is a ghost of a direction
-// inside the domain: a same-level neighbour's cell (read from the neighbour,
-// the value the copy writes) or a coarse-fine ghost (read from the frame, as
-// the interpolation that refills it comes after both fills).
+// The value fill_ghost_copy! then fill_ghost_bnd! give ghost cell c of blk (c
+// at most one cell outside the block, modified). A ghost facing a same-level
+// neighbour is that neighbour's cell. A ghost beyond the domain is a[bnd]
+// times the cell mirrored in the dimensions that leave the domain; when other
+// dimensions stay outside, the mirrored cell is a ghost of a direction inside
+// the domain: a same-level neighbour's cell (read from the neighbour, the value
+// the copy writes) or a coarse-fine ghost (read from the frame, as the
+// interpolation that refills it comes after both fills). Coarse-fine ghosts
+// themselves are read from the frame.
 template <int D, int M, typename T>
-__global__ __launch_bounds__(WG) void k_frame_copybnd(T *__restrict__ g, const T *__restrict__ x,
-                                                      const int *__restrict__ box, int blk0, int nblk, Bc27<T> bc) {
+__device__ __forceinline__ T ghost_value(const T *__restrict__ x, const T *__restrict__ g,
+                                         const int *__restrict__ box, const Bc27<T> &bc, int blk, int *c) {
     using LY = Lay<D, M>;
     constexpr int P = Frame<D, M>::P, NB3 = D == 3 ? 27 : 9;
-    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
-    const int lb = gt / P;
-    if (lb >= nblk) return;
-    const int q = gt - lb * P, blk = blk0 + lb;
-    int c[3], dir[3], k;
-    if (!frame_cell<D, M>(q, c, dir, k)) return;
+    int dir[3] = {0, 0, 0}, k = 0, m = 1;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        dir[d] = c[d] < 0 ? -1 : (c[d] >= M ? 1 : 0);
+        k += (dir[d] + 1) * m;
+        m *= 3;
+    }
     const int e = box[blk * NB3 + k];
     if (e >= 0) {
 #pragma unroll
         for (int d = 0; d < D; d++) c[d] -= dir[d] * M;
-        g[(size_t)blk * P + q] = x[(size_t)e * LY::V + LY::sidx(c[0], c[1], c[2])];
-        return;
+        return x[(size_t)e * LY::V + LY::sidx(c[0], c[1], c[2])];
     }
-    if (e < -8) return;
+    if (e < -8) return g[(size_t)blk * P + Frame<D, M>::at(c[0], c[1], D == 3 ? c[2] : 0)];
     const int mask = -1 - e;
-    int bi = 0, m = 1, k2 = 0;
+    int bi = 0, k2 = 0;
+    m = 1;
 #pragma unroll
     for (int d = 0; d < D; d++) {
         const bool bd = (mask >> d) & 1;
@@ -286,8 +289,50 @@ __global__ __launch_bounds__(WG) void k_frame_copybnd(T *__restrict__ g, const T
     } else {
         v = g[(size_t)blk * P + Frame<D, M>::at(c[0], c[1], D == 3 ? c[2] : 0)];
     }
-    g[(size_t)blk * P + q] = bc.v[bi] * v;
+    return bc.v[bi] * v;
 }
+
+// fill_ghost_copy! then fill_ghost_bnd! in one pass (ghost_value); one pass
+// needs no ordering between blocks and gives the bits of the two in order
+template <int D, int M, typename T>
+__global__ __launch_bounds__(WG) void k_frame_copybnd(T *__restrict__ g, const T *__restrict__ x,
+                                                      const int *__restrict__ box, int blk0, int nblk, Bc27<T> bc) {
+    constexpr int P = Frame<D, M>::P, NB3 = D == 3 ? 27 : 9;
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lb = gt / P;
+    if (lb >= nblk) return;
+    const int q = gt - lb * P, blk = blk0 + lb;
+    int c[3], dir[3], k;
+    if (!frame_cell<D, M>(q, c, dir, k)) return;
+    if (box[blk * NB3 + k] < -8) return;  // coarse-fine: k_frame_interp
+    g[(size_t)blk * P + q] = ghost_value<D, M, T>(x, g, box, bc, blk, c);
+}
+
+// reads of cells at most one cell outside a block for the order-4 stencils:
+// from the frame as last filled (FrameRead), or live (LiveRead: the values a
+// copy-then-boundary fill of the frame would give now)
+template <int D, int M, typename T>
+struct FrameRead {
+    const T *__restrict__ x;
+    const T *__restrict__ g;
+    __device__ __forceinline__ T operator()(int blk, int c0, int c1, int c2) const {
+        return boxv<D, M, T>(x, g, blk, c0, c1, c2);
+    }
+};
+template <int D, int M, typename T>
+struct LiveRead {
+    const T *__restrict__ x;
+    const T *__restrict__ g;
+    const int *__restrict__ box;
+    const Bc27<T> &bc;
+    __device__ __forceinline__ T operator()(int blk, int c0, int c1, int c2) const {
+        using LY = Lay<D, M>;
+        if (c0 >= 0 && c0 < M && c1 >= 0 && c1 < M && (D == 2 || (c2 >= 0 && c2 < M)))
+            return x[(size_t)blk * LY::V + LY::sidx(c0, c1, c2)];
+        int c[3] = {c0, c1, D == 3 ? c2 : 0};
+        return ghost_value<D, M, T>(x, g, box, bc, blk, c);
+    }
+};
 
 // fill_ghost_interp! (src/connectivity.jl:251-258, src/interp.jl:23-60): a
 // ghost facing a coarse block is the linear interpolation of the coarse cell
@@ -358,9 +403,8 @@ __global__ __launch_bounds__(WG) void k_interp_add4(T *__restrict__ u, const T *
 
 // applystencil (src/laplacian.jl:133-141, 305-320): the literal's nonzero
 // terms in column-major order, then one division by 6
-template <int D, int M, typename T>
-__device__ __forceinline__ T stencil4(const T *__restrict__ x, const T *__restrict__ g, int blk, int c0, int c1,
-                                      int c2) {
+template <int D, int M, typename T, typename A>
+__device__ __forceinline__ T stencil4(const A &at, int blk, int c0, int c1, int c2) {
     T v = T(0);
     constexpr int NK = D == 3 ? 1 : 0;
 #pragma unroll
@@ -373,7 +417,7 @@ __device__ __forceinline__ T stencil4(const T *__restrict__ x, const T *__restri
                 const int a = D == 2 ? (n == 0 ? -20 : (n == 1 ? 4 : 1))
                                      : (k == 0 ? (n == 0 ? -24 : (n == 1 ? 2 : 1)) : (n == 1 ? 2 : (n == 2 ? 1 : 0)));
                 if (a == 0) continue;
-                v += T(a) * boxv<D, M, T>(x, g, blk, c0 + i, c1 + j, c2 + k);
+                v += T(a) * at(blk, c0 + i, c1 + j, c2 + k);
             }
     return v / T(6);
 }
@@ -382,9 +426,8 @@ __device__ __forceinline__ T stencil4(const T *__restrict__ x, const T *__restri
 // 156-201) at 1-based global radial index i: the 3x3 literal's entries
 // (integer ratios rounded once), visited column-major, divided by 6; and its
 // diagelm
-template <int M, typename T>
-__device__ __forceinline__ T stencil4cyl(const T *__restrict__ x, const T *__restrict__ g, int blk, int c0,
-                                         int c1, int N, long long i) {
+template <int M, typename T, typename A>
+__device__ __forceinline__ T stencil4cyl(const A &at, int blk, int c0, int c1, int N, long long i) {
     const long long q = 2 * i - 1;
     const T lo = (T)(2 * (i - 1)) / (T)q, hi = (T)(2 * i) / (T)q;
     const T pm = (T)(4 * (-2 + 9 * i - 16 * i * i + 8 * i * i * i)) / (T)(q * q * q);
@@ -404,7 +447,7 @@ __device__ __forceinline__ T stencil4cyl(const T *__restrict__ x, const T *__res
 #pragma unroll
     for (int b = 0; b < 3; b++)
 #pragma unroll
-        for (int a = 0; a < 3; a++) v += w[a][b] * boxv<2, M, T>(x, g, blk, c0 + a - 1, c1 + b - 1, 0);
+        for (int a = 0; a < 3; a++) v += w[a][b] * at(blk, c0 + a - 1, c1 + b - 1, 0);
     return v / T(6);
 }
 
@@ -415,19 +458,24 @@ __device__ __forceinline__ T diag4cyl(long long i) {
 
 // the order-4 operator at cell (c0, c1, c2) of blk: Cartesian (rd < 0) or
 // cylindrical with radial dimension rd (global radial index from coord)
-template <int D, int M, typename T>
-__device__ __forceinline__ T apply4(const T *__restrict__ x, const T *__restrict__ g,
-                                    const int *__restrict__ coord, int blk, int c0, int c1, int c2, int rd,
-                                    T &diag) {
+template <int D, int M, typename T, typename A>
+__device__ __forceinline__ T apply4a(const A &at, const int *__restrict__ coord, int blk, int c0, int c1, int c2,
+                                     int rd, T &diag) {
     if constexpr (D == 2) {
         if (rd >= 0) {
             const long long i = (long long)(rd == 0 ? c0 : c1) + 1 + (long long)M * (coord[blk * D + rd] - 1);
             diag = diag4cyl<T>(i);
-            return stencil4cyl<M, T>(x, g, blk, c0, c1, rd, i);
+            return stencil4cyl<M, T>(at, blk, c0, c1, rd, i);
         }
     }
     diag = D == 2 ? T(-20.0 / 6.0) : T(-4);
-    return stencil4<D, M, T>(x, g, blk, c0, c1, c2);
+    return stencil4<D, M, T>(at, blk, c0, c1, c2);
+}
+template <int D, int M, typename T>
+__device__ __forceinline__ T apply4(const T *__restrict__ x, const T *__restrict__ g,
+                                    const int *__restrict__ coord, int blk, int c0, int c1, int c2, int rd,
+                                    T &diag) {
+    return apply4a<D, M, T>(FrameRead<D, M, T>{x, g}, coord, blk, c0, c1, c2, rd, diag);
 }
 
 // the FP32 right-hand side of the order-4 correction equation of a mixed
@@ -455,10 +503,15 @@ __global__ __launch_bounds__(WG) void k_defect4(float *__restrict__ be, const do
 // both shifted by the 1-based third index with G = 2 ghost layers, the
 // reference's depth), u -= omega (Lu + s b) / diagelm; cells of one colour do
 // not read each other
-template <int D, int M, typename T>
+// LIVE: ghosts read live (LiveRead) instead of from the frame, which equals a
+// pass that follows a copy-then-boundary fill: the cells one pass reads across
+// a block face are of other colours, or, for a boundary mirror, the cell being
+// updated itself (read before its write), so none changes during the pass
+template <int D, int M, typename T, bool LIVE>
 __global__ __launch_bounds__(WG) void k_gs4_pass(T *__restrict__ u, const T *__restrict__ b,
                                                  const T *__restrict__ g, const int *__restrict__ coord, int rd,
-                                                 int blk0, int nblk, T s, T omega, int P1, int P2) {
+                                                 int blk0, int nblk, T s, T omega, int P1, int P2,
+                                                 const int *__restrict__ box, Bc27<T> bc) {
     using LY = Lay<D, M>;
     constexpr int H2 = M / 2, NCC = D == 3 ? H2 * H2 * M : H2 * H2;
     const int gt = blockIdx.x * blockDim.x + threadIdx.x;
@@ -470,8 +523,11 @@ __global__ __launch_bounds__(WG) void k_gs4_pass(T *__restrict__ u, const T *__r
     const int p1 = D == 3 ? (P1 + z + 3) & 1 : P1, p2 = D == 3 ? (P2 + z + 3) & 1 : P2;
     const int x0 = 2 * i0 + p1, x1 = 2 * i1 + p2;
     const size_t o = (size_t)blk * LY::V + LY::sidx(x0, x1, z);
-    T c;
-    const T Lu = apply4<D, M, T>(u, g, coord, blk, x0, x1, z, rd, c);
+    T c, Lu;
+    if constexpr (LIVE)
+        Lu = apply4a<D, M, T>(LiveRead<D, M, T>{u, g, box, bc}, coord, blk, x0, x1, z, rd, c);
+    else
+        Lu = apply4a<D, M, T>(FrameRead<D, M, T>{u, g}, coord, blk, x0, x1, z, rd, c);
     u[o] = u[o] - omega * (Lu + s * b[o]) / c;
 }
 

@@ -985,13 +985,22 @@ struct Ops {
         const LevelInfo &L = c->lv[l];
         if (!L.nblk) return;
         constexpr int NCC = D == 3 ? (M / 2) * (M / 2) * M : (M / 2) * (M / 2);
+        // the reference refills the copy and boundary ghosts after every
+        // colour pass; the passes after the first read those values live
+        // (k_gs4_pass<LIVE>), so only the last refill is made
         for (int it = 0; it < n; it++)
             for (int col = 0; col < 4; col++) {
-                hipLaunchKernelGGL((k_gs4_pass<D, M, T>), dim3(nwg((int64_t)L.nblk * NCC)), dim3(WG), 0, c->stream,
-                                   X(u, l), X(b, l), frame(u), coord(), rd4, L.blk0, L.nblk, (T)sl, (T)omega,
-                                   col >> 1, col & 1);
-                frame_fill(u, l, bc, false);
+                const dim3 grid(nwg((int64_t)L.nblk * NCC));
+                if (it == 0 && col == 0)
+                    hipLaunchKernelGGL((k_gs4_pass<D, M, T, false>), grid, dim3(WG), 0, c->stream, X(u, l), X(b, l),
+                                       frame(u), coord(), rd4, L.blk0, L.nblk, (T)sl, (T)omega, col >> 1, col & 1,
+                                       c->d_box.as<int>(), bc);
+                else
+                    hipLaunchKernelGGL((k_gs4_pass<D, M, T, true>), grid, dim3(WG), 0, c->stream, X(u, l), X(b, l),
+                                       frame(u), coord(), rd4, L.blk0, L.nblk, (T)sl, (T)omega, col >> 1, col & 1,
+                                       c->d_box.as<int>(), bc);
             }
+        if (n > 0) frame_fill(u, l, bc, false);
     }
     void vcycle4(FieldBuf &u, FieldBuf &b, FieldBuf &r, FieldBuf &u1, double s, const srmg_op *op, int lmax,
                  int ndown, int ntop, int nup, double omega) {
