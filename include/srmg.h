@@ -82,7 +82,8 @@ int srmg_destroy(srmg_ctx *ctx);
  *   child       (fine, coarse, subblock[D])          Child{D}, src/connectivity.jl:36-42
  * Block ids are the reference's 1-based ids, at most nblocks. Level 1 must be
  * complete (populate!, src/tree.jl:31-44). Existing fields are reallocated
- * (contents lost): upload them again after a regrid. */
+ * (contents lost): upload them again after a regrid. Records are checked
+ * before the context changes: a refused call leaves the old tree and fields. */
 int srmg_set_tree(srmg_ctx *ctx, int nlevels, int64_t nblocks,
                   const int64_t *pairs, const int64_t *pairs_off,
                   const int64_t *neighbor, const int64_t *neighbor_off,
@@ -158,9 +159,9 @@ int srmg_rhs(srmg_ctx *ctx, int b1, int b, const srmg_op *op);
  * with bc), otherwise it starts at zero; with SRMG_REGRID_RESTRICT a removed
  * block is restrict!'ed into its parent's subblock. A new block's parent must
  * be in the old tree and a removed block's parent must be kept (one level per
- * regrid, as applydelta! does); this and the block coordinates are checked
- * before anything changes (records that srmg_set_tree would refuse leave the
- * context without a tree, as a failing srmg_set_tree does).
+ * regrid, as applydelta! does); this, the block coordinates and every check
+ * of srmg_set_tree are made before anything changes: a refused regrid leaves
+ * the context with its old tree and field values.
  * Fields not listed are reallocated as in srmg_set_tree. bc may be NULL
  * when no field is interpolated. */
 enum { SRMG_REGRID_INTERP = 1, SRMG_REGRID_RESTRICT = 2 };

@@ -60,6 +60,14 @@ def lib():
         _LIB.ora_resnorm.restype = dbl
         _LIB.ora_coarse_solve.argtypes = [P, P, P, dbl, P]
         _LIB.ora_coarse_solve.restype = i32
+        _LIB.ora_set_threads.argtypes = [i32]
+        _LIB.ora_set_threads.restype = None
+        _LIB.ora_max_threads.argtypes = []
+        _LIB.ora_max_threads.restype = i32
+        _LIB.ora_axis_decomp.argtypes = [i32, i32, dbl, dbl, P, P, P]
+        _LIB.ora_axis_decomp.restype = i32
+        _LIB.ora_axis_decomp_jacobi.argtypes = [i32, i32, dbl, dbl, P, P, P]
+        _LIB.ora_axis_decomp_jacobi.restype = None
         _LIB.ora_electric_field.argtypes = [P, P, P, P, dbl, P]
         _LIB.ora_electric_field.restype = None
         for name, args in {
@@ -74,6 +82,31 @@ def lib():
             getattr(_LIB, name).argtypes = args
             getattr(_LIB, name).restype = None
     return _LIB
+
+
+def set_threads(n: int) -> None:
+    """OpenMP threads of the oracle's level loops."""
+    lib().ora_set_threads(int(n))
+
+
+def max_threads() -> int:
+    return int(lib().ora_max_threads())
+
+
+def axis_decomp(n: int, radial: bool, alow: float, ahigh: float, method: str = "ql"):
+    """The level-1 1D operator of one axis as T = P diag(lam) Q: the implicit
+    QL eigensolver of the exact coarse solve (method "ql") or the cyclic
+    Jacobi cross-check ("jacobi")."""
+    lam = np.zeros(n)
+    P = np.zeros((n, n))
+    Q = np.zeros((n, n))
+    if method == "ql":
+        if lib().ora_axis_decomp(int(n), int(bool(radial)), float(alow), float(ahigh), _ptr(lam), _ptr(P), _ptr(Q)):
+            raise RuntimeError("QL iteration did not converge")
+    else:
+        lib().ora_axis_decomp_jacobi(int(n), int(bool(radial)), float(alow), float(ahigh), _ptr(lam), _ptr(P),
+                                     _ptr(Q))
+    return lam, P, Q
 
 
 def dst_matrix(n: int, kind: int) -> np.ndarray:

@@ -26,6 +26,7 @@
  * (src/laplacian.jl:25-49: column-major over the 3^D literal, zeros skipped).
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -55,6 +56,10 @@ typedef struct {
     int8_t bc[27];     /* GeneralHomogeneousBoundaryConditions{D}.a, column-major */
     int order;         /* 4: LaplacianDiscretization{D,4} (Cartesian); otherwise order 2 */
 } ora_op;
+
+/* OpenMP threads of the level loops (the CPU baseline times 1 and all) */
+void ora_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int ora_max_threads(void) { return omp_get_max_threads(); }
 
 static int64_t *dup64(const int64_t *src, int64_t n) {
     int64_t *p = (int64_t *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int64_t));
@@ -643,13 +648,78 @@ void ora_interp_add_level(const ora_tree *t, double *u, const double *u1, int le
 /* Level 1 is the complete root grid with homogeneous mirror BCs, so its    */
 /* operator is a Kronecker sum of 1D tridiagonal operators. Each 1D operator */
 /* T = W^-1 S (W = diag(2i-1) on the radial axis, identity otherwise, S      */
-/* symmetric) is diagonalised with the cyclic Jacobi method on              */
-/* A = W^-1/2 S W^-1/2 = Y L Y^T, so T = P L Q, P = W^-1/2 Y, Q = Y^T W^1/2. */
+/* symmetric) is diagonalised through A = W^-1/2 S W^-1/2 = Y L Y^T, so     */
+/* T = P L Q with P = W^-1/2 Y and Q = Y^T W^1/2.                            */
+/*                                                                          */
+/* The eigensolver is the implicit QL method with Wilkinson shifts on the   */
+/* tridiagonal A (the textbook tql2 iteration: deflate at the first         */
+/* negligible off-diagonal, shift from the trailing 2x2 block, chase the    */
+/* bulge with Givens rotations from the bottom up, accumulating them into  */
+/* the eigenvector matrix). It is the method the HIP library uses, so both */
+/* sides hold the same matrices and exact-coarse V-cycles agree bit for    */
+/* bit. It is pinned by the cyclic Jacobi method below (an independent     */
+/* algorithm, kept for that purpose), by the dense eigenvalues and by      */
+/* scipy's spsolve of the assembled level-1 operator (tests/test_oracle.py).*/
 /* ---------------------------------------------------------------------- */
 
+/* d[n]: diagonal (eigenvalues on return); e[n]: e[i] couples i and i+1
+ * (destroyed); z: n x n row-major, eigenvectors in its columns on return.
+ * Returns 0, or -1 if an eigenvalue needs more than 200 iterations. */
+static int ql_tridiag(int n, double *d, double *e, double *z) {
+    const double eps = 2.220446049250313e-16;
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) z[i * n + j] = (i == j) ? 1.0 : 0.0;
+    if (n > 0) e[n - 1] = 0.0;
+    for (int l = 0; l < n; l++) {
+        int iters = 0;
+        for (;;) {
+            /* first m >= l whose coupling to m+1 is negligible */
+            int m;
+            for (m = l; m < n - 1; m++)
+                if (fabs(e[m]) <= eps * (fabs(d[m]) + fabs(d[m + 1]))) break;
+            if (m == l) break;             /* d[l] has converged */
+            if (++iters > 200) return -1;
+            /* Wilkinson shift from the 2x2 block at l */
+            double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+            double r = hypot(g, 1.0);
+            g = d[m] - d[l] + e[l] / (g + copysign(r, g));
+            double sn = 1.0, cs = 1.0, p = 0.0;
+            int i;
+            for (i = m - 1; i >= l; i--) {
+                double f = sn * e[i], bb = cs * e[i];
+                r = hypot(f, g);
+                e[i + 1] = r;
+                if (r == 0.0) {           /* exact split: restart at l */
+                    d[i + 1] -= p;
+                    e[m] = 0.0;
+                    break;
+                }
+                sn = f / r;
+                cs = g / r;
+                g = d[i + 1] - p;
+                r = (d[i] - g) * sn + 2.0 * cs * bb;
+                p = sn * r;
+                d[i + 1] = g + p;
+                g = cs * r - bb;
+                for (int k = 0; k < n; k++) {  /* rotate columns i, i+1 */
+                    double zi = z[k * n + i], zi1 = z[k * n + i + 1];
+                    z[k * n + i + 1] = sn * zi + cs * zi1;
+                    z[k * n + i] = cs * zi - sn * zi1;
+                }
+            }
+            if (i >= l) continue;
+            d[l] -= p;
+            e[l] = g;
+            e[m] = 0.0;
+        }
+    }
+    return 0;
+}
+
+/* cyclic Jacobi: A (n x n, row-major) symmetric, destroyed; eigenvalues on its
+ * diagonal; V receives the eigenvectors as columns. Only the cross-check
+ * ora_axis_decomp_jacobi uses it. */
 static void jacobi_eig(int n, double *A, double *V) {
-    /* A (n x n, row-major) symmetric, destroyed; eigenvalues on its diagonal;
-     * V receives the eigenvectors as columns. */
     for (int i = 0; i < n; i++)
         for (int j = 0; j < n; j++) V[i * n + j] = (i == j) ? 1.0 : 0.0;
     for (int sweep = 0; sweep < 100; sweep++) {
@@ -688,40 +758,70 @@ static void jacobi_eig(int n, double *A, double *V) {
     }
 }
 
-/* 1D operator of dimension d at level 1 with n cells; returns lam[n], P, Q (n x n row-major) */
-static void axis_decomp(int n, int radial, double alow, double ahigh,
-                        double *lam, double *P, double *Q) {
-    double *A = (double *)calloc((size_t)n * n, sizeof(double));
-    double *Y = (double *)calloc((size_t)n * n, sizeof(double));
-    double *w = (double *)calloc((size_t)n, sizeof(double));
+/* the 1D level-1 operator of one axis: the diagonal diag[i] of T (mirror
+ * ghost folded in at the ends; the radial axis weight (2i-2)/(2i-1) is 0 at
+ * i = 1), the weights w[i] of W, and the symmetric couplings
+ * off[i] = S[i][i+1] / sqrt(w[i] w[i+1]) of A */
+static void axis_operator(int n, int radial, double alow, double ahigh, double *diag, double *w, double *off) {
     for (int i = 0; i < n; i++) {
         int ii = i + 1;
         w[i] = radial ? (double)(2 * ii - 1) : 1.0;
-        double diag = -2.0;
+        double dg = -2.0;
         if (!radial) {
-            if (i == 0) diag += alow;
-            if (i == n - 1) diag += ahigh;
-        } else {
-            /* axis weight (2i-2)/(2i-1) is 0 at i = 1 */
-            if (i == n - 1) diag += ((double)(2 * ii) / (double)(2 * ii - 1)) * ahigh;
+            if (i == 0) dg += alow;
+            if (i == n - 1) dg += ahigh;
+        } else if (i == n - 1) {
+            dg += ((double)(2 * ii) / (double)(2 * ii - 1)) * ahigh;
         }
-        A[i * n + i] = diag * w[i];           /* S_ii */
+        diag[i] = dg;
+        off[i] = 0.0;
         if (i + 1 < n) {
             double sij = radial ? (double)(2 * ii) : 1.0;
-            A[i * n + i + 1] = sij;
-            A[(i + 1) * n + i] = sij;
+            off[i] = sij / sqrt(w[i] * (radial ? (double)(2 * ii + 1) : 1.0));
         }
     }
-    for (int i = 0; i < n; i++)
-        for (int j = 0; j < n; j++) A[i * n + j] /= sqrt(w[i] * w[j]);
+}
+
+/* 1D operator of dimension d at level 1 with n cells; returns lam[n], P, Q
+ * (n x n row-major) with T = P diag(lam) Q. Returns 0, or -1 if QL fails. */
+int ora_axis_decomp(int n, int radial, double alow, double ahigh, double *lam, double *P, double *Q) {
+    double *w = (double *)calloc((size_t)n, sizeof(double));
+    double *e = (double *)calloc((size_t)n, sizeof(double));
+    double *Y = (double *)calloc((size_t)n * n, sizeof(double));
+    axis_operator(n, radial, alow, ahigh, lam, w, e);
+    int rc = ql_tridiag(n, lam, e, Y);
+    for (int i = 0; i < n; i++) {
+        double sw = sqrt(w[i]);
+        for (int k = 0; k < n; k++) {
+            P[(size_t)i * n + k] = Y[(size_t)i * n + k] / sw;
+            Q[(size_t)k * n + i] = Y[(size_t)i * n + k] * sw;
+        }
+    }
+    free(w); free(e); free(Y);
+    return rc;
+}
+
+/* The same decomposition by the cyclic Jacobi method on the dense A
+ * (cross-check of ora_axis_decomp; eigenvalues in Jacobi's order). */
+void ora_axis_decomp_jacobi(int n, int radial, double alow, double ahigh, double *lam, double *P, double *Q) {
+    double *w = (double *)calloc((size_t)n, sizeof(double));
+    double *dg = (double *)calloc((size_t)n, sizeof(double));
+    double *e = (double *)calloc((size_t)n, sizeof(double));
+    double *A = (double *)calloc((size_t)n * n, sizeof(double));
+    double *Y = (double *)calloc((size_t)n * n, sizeof(double));
+    axis_operator(n, radial, alow, ahigh, dg, w, e);
+    for (int i = 0; i < n; i++) {
+        A[(size_t)i * n + i] = dg[i];
+        if (i + 1 < n) A[(size_t)i * n + i + 1] = A[(size_t)(i + 1) * n + i] = e[i];
+    }
     jacobi_eig(n, A, Y);
-    for (int k = 0; k < n; k++) lam[k] = A[k * n + k];
+    for (int k = 0; k < n; k++) lam[k] = A[(size_t)k * n + k];
     for (int i = 0; i < n; i++)
         for (int k = 0; k < n; k++) {
-            P[i * n + k] = Y[i * n + k] / sqrt(w[i]);
-            Q[k * n + i] = Y[i * n + k] * sqrt(w[i]);
+            P[(size_t)i * n + k] = Y[(size_t)i * n + k] / sqrt(w[i]);
+            Q[(size_t)k * n + i] = Y[(size_t)i * n + k] * sqrt(w[i]);
         }
-    free(A); free(Y); free(w);
+    free(w); free(dg); free(e); free(A); free(Y);
 }
 
 /* apply the n x n matrix Mx along axis d of a column-major grid with sizes N[] */
@@ -773,7 +873,11 @@ int ora_coarse_solve(ora_tree *t, double *u, const double *b, double s, const or
             t->ck_lam[d] = (double *)malloc(sizeof(double) * N[d]);
             t->ck_P[d] = (double *)malloc(sizeof(double) * N[d] * N[d]);
             t->ck_Q[d] = (double *)malloc(sizeof(double) * N[d] * N[d]);
-            axis_decomp(N[d], d == radialdim, alow, ahigh, t->ck_lam[d], t->ck_P[d], t->ck_Q[d]);
+            if (ora_axis_decomp(N[d], d == radialdim, alow, ahigh, t->ck_lam[d], t->ck_P[d], t->ck_Q[d])) {
+                t->ck_valid = 0;
+                free(x); free(tmp);
+                return -1;
+            }
         }
         lam[d] = t->ck_lam[d]; P[d] = t->ck_P[d]; Q[d] = t->ck_Q[d];
     }

@@ -298,10 +298,11 @@ template <int D, int M, typename T>
 __global__ __launch_bounds__(WG) void k_frame_copybnd(T *__restrict__ g, const T *__restrict__ x,
                                                       const int *__restrict__ box, int blk0, int nblk, Bc27<T> bc) {
     constexpr int P = Frame<D, M>::P, NB3 = D == 3 ? 27 : 9;
-    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
-    const int lb = gt / P;
+    // 64-bit: a level's nblk * P frame cells can pass 2^31 before its nblk * V cells do
+    const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t lb = gt / P;
     if (lb >= nblk) return;
-    const int q = gt - lb * P, blk = blk0 + lb;
+    const int q = (int)(gt - lb * P), blk = blk0 + (int)lb;
     int c[3], dir[3], k;
     if (!frame_cell<D, M>(q, c, dir, k)) return;
     if (box[blk * NB3 + k] < -8) return;  // coarse-fine: k_frame_interp
@@ -344,10 +345,11 @@ __global__ __launch_bounds__(WG) void k_frame_interp(T *__restrict__ g, const T 
                                                      int blk0, int nblk) {
     using LY = Lay<D, M>;
     constexpr int P = Frame<D, M>::P, NB3 = D == 3 ? 27 : 9;
-    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
-    const int lb = gt / P;
+    // 64-bit: a level's nblk * P frame cells can pass 2^31 before its nblk * V cells do
+    const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t lb = gt / P;
     if (lb >= nblk) return;
-    const int q = gt - lb * P, blk = blk0 + lb;
+    const int q = (int)(gt - lb * P), blk = blk0 + (int)lb;
     int c[3], dir[3], k;
     if (!frame_cell<D, M>(q, c, dir, k)) return;
     const int e = box[blk * NB3 + k];

@@ -202,9 +202,10 @@ struct TreeRecords {
     const int64_t *pairs, *pairs_off, *nbr, *nbr_off, *bnd, *bnd_off, *ref, *ref_off, *chd, *chd_off;
 };
 
-static void set_forest(srmg_ctx *c, const std::vector<TreeRecords> &tr) {
-    c->graphs.clear();
-    c->box_ok = false;
+// The host tables of a forest, built and checked into `c`, a staging context
+// that holds only D, M and V: nothing of the caller's context changes here,
+// so records that fail a check leave the context with its old tree and fields.
+static void build_tables(srmg_ctx *c, const std::vector<TreeRecords> &tr) {
     const int D = c->D, M = c->M, ND = 2 * D;
     const int nt = (int)tr.size();
     if (nt < 1) fail("a forest needs at least one tree");
@@ -442,6 +443,66 @@ static void set_forest(srmg_ctx *c, const std::vector<TreeRecords> &tr) {
             fail("internal: parent table entry out of range");
     }
     if ((int64_t)nb * c->V >= ((int64_t)1 << 31)) fail("field too large for 32-bit cell indices");
+}
+
+// Install tables that build_tables accepted (moved out of `t`) and make the
+// device side: tables, scratch, and every field reallocated. A device error
+// here (out of memory) leaves the context without a tree (later calls refuse).
+static void commit_tables(srmg_ctx *c, srmg_ctx &t);
+static void upload_tables(srmg_ctx *c);
+
+static void set_forest(srmg_ctx *c, const std::vector<TreeRecords> &tr) {
+    srmg_ctx t;
+    t.D = c->D;
+    t.M = c->M;
+    t.V = c->V;
+    t.F = c->F;
+    build_tables(&t, tr);
+    commit_tables(c, t);
+}
+
+static void commit_tables(srmg_ctx *c, srmg_ctx &t) {
+    c->graphs.clear();
+    c->box_ok = false;
+    c->nlevels = t.nlevels;
+    c->nhost = t.nhost;
+    c->nb = t.nb;
+    c->nslot = t.nslot;
+    c->lv = std::move(t.lv);
+    c->trees = std::move(t.trees);
+    c->groups = std::move(t.groups);
+    c->h_nbr = std::move(t.h_nbr);
+    c->h_coord = std::move(t.h_coord);
+    c->h_parent = std::move(t.h_parent);
+    c->h_sub = std::move(t.h_sub);
+    c->h_level = std::move(t.h_level);
+    c->h_leaves = std::move(t.h_leaves);
+    c->h_tree = std::move(t.h_tree);
+    c->host2dev = std::move(t.host2dev);
+    c->dev2host = std::move(t.dev2host);
+    c->h_slots = std::move(t.h_slots);
+    try {
+        upload_tables(c);
+    } catch (...) {
+        c->nb = 0;
+        c->nlevels = 0;
+        for (auto &f : c->fields)
+            if (f) {
+                f->data.release();
+                f->alt.release();
+                f->ib.release();
+                f->frame.release();
+            }
+        throw;
+    }
+}
+
+static void upload_tables(srmg_ctx *c) {
+    const int D = c->D, M = c->M, ND = 2 * D, nb = c->nb, nlevels = c->nlevels, nt = (int)c->trees.size();
+    // blocks with children
+    std::vector<char> has_child(nb, 0);
+    for (int b = 0; b < nb; b++)
+        if (c->h_parent[b] >= 0) has_child[c->h_parent[b]] = 1;
     // radial weight tables (2J-2)/(2J-1), 2J/(2J-1) for J = 0..n (entry 0 unused)
     std::vector<double> w64;
     for (int l = 0; l < nlevels; l++)
@@ -1838,7 +1899,8 @@ static std::vector<double> read_norms(srmg_ctx *c) {
 
 static void check_opts(const srmg_solve_opts *o) {
     if (!o) fail("null options");
-    if (o->max_cycles < 0 || o->ndown < 0 || o->nup < 0 || o->ntop < 0) fail("bad solve options");
+    if (o->max_cycles < 0 || o->ndown < 0 || o->nup < 0 || o->ntop < 0 || o->inner_cycles < 0)
+        fail("bad solve options");
     if (o->guess < SRMG_GUESS_NONE || o->guess > SRMG_GUESS_ZERO) fail("bad first-iterate mode");
     if (o->guess_smooth < 0) fail("bad first-iterate smoothing count");
 }
@@ -2036,7 +2098,7 @@ static void solve_mixed4_dm(srmg_ctx *c, FieldBuf &u, FieldBuf &b, FieldBuf &e, 
         }
         o32.zero_all(e);
         hipLaunchKernelGGL(k_fill<float>, dim3(nwg(nfr)), dim3(WG), 0, c->stream, o32.frame(e), 0.f, nfr);
-        for (int it = 0; it < o->inner_cycles; it++) {
+        for (int it = 0; it < std::max(1, (int)o->inner_cycles); it++) {
             if (it > 0)
                 for (int l = 0; l < c->nlevels; l++) o32.frame_fill(e, l, b32);
             o32.vcycle4(e, be, re, ue, s, op, 0, o->ndown, o->ntop, o->nup, o->omega);
@@ -2235,6 +2297,14 @@ static void regrid_fields(srmg_ctx *c, const std::vector<TreeRecords> &tr, const
                 }
             }
         }
+    // the new tables, checked before anything of the context changes: records
+    // that fail leave the old tree and fields in place
+    srmg_ctx staged;
+    staged.D = c->D;
+    staged.M = c->M;
+    staged.V = c->V;
+    staged.F = c->F;
+    build_tables(&staged, tr);
     const std::vector<int> ocoord = c->h_coord;
     DevBuf onbr = std::move(c->d_nbr);
     std::vector<DevBuf> odata(nf), oib(nf);
@@ -2244,7 +2314,7 @@ static void regrid_fields(srmg_ctx *c, const std::vector<TreeRecords> &tr, const
         oib[k] = std::move(f.ib);
     }
     HIPCHK(hipStreamSynchronize(c->stream));
-    set_forest(c, tr);  // new tables; every field reallocated and zeroed
+    commit_tables(c, staged);  // new tables; every field reallocated and zeroed
     auto dev = [&](const NewItem &it) {
         const int b = c->trees[it.t].blk0[it.l] + it.rank;
         if (it.rank >= c->trees[it.t].nblk[it.l]) fail("internal: regrid rank outside the level");
@@ -2381,6 +2451,8 @@ int srmg_zero(srmg_ctx *c, int id) {
         check_tree(c);
         HIPCHK(hipMemsetAsync(f.data.p, 0, f.data.bytes, c->stream));
         HIPCHK(hipMemsetAsync(f.ib.p, 0, f.ib.bytes, c->stream));
+        // the order-4 ghost frame too (fill!(u, 0) zeroes the ghosts)
+        if (f.frame.p) HIPCHK(hipMemsetAsync(f.frame.p, 0, f.frame.bytes, c->stream));
         std::fill(f.cur.begin(), f.cur.end(), 0);
     });
 }

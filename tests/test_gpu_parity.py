@@ -1,11 +1,12 @@
 """GPU parity: the HIP library through its C ABI against the CPU oracle.
 
-With the reference's coarsest-level treatment (ntop Gauss-Seidel sweeps) the
-FP64 device path performs the same floating-point operations in the same order
-as the oracle, so potentials and FAS right-hand sides must agree bit for bit.
-With the exact coarsest-level solve the two sides diagonalise the level-1
-operator with different eigensolvers; there the bar is 1e-10 relative in the
-potential and 1e-12 (relative to resnorm(0)) in the residual history.
+The FP64 device path performs the same floating-point operations in the same
+order as the oracle, so potentials and FAS right-hand sides must agree bit for
+bit, with the reference's coarsest-level treatment (ntop Gauss-Seidel sweeps)
+and with the exact coarsest-level solve alike (both sides diagonalise the
+level-1 operator with the same implicit QL method and apply the transforms as
+k-ordered sums). Residual norms are summed in another fixed order on the
+device: there the bar is 1e-12 relative (histories: 1e-12 of resnorm(0)).
 """
 
 import numpy as np
@@ -197,7 +198,21 @@ def test_exact_coarse_solve_history(name):
     gu = p.b.like()
     u.download(gu)
     ref = _valid(ou, p)
-    assert np.max(np.abs(_valid(gu, p) - ref)) <= 1e-10 * np.max(np.abs(ref))
+    err = np.max(np.abs(_valid(gu, p) - ref)) / np.max(np.abs(ref))
+    print(f"{name}: {n} cycles, potential max rel diff vs oracle {err:.3e}", flush=True)
+    np.testing.assert_array_equal(_valid(gu, p), ref)
+
+
+@pytest.mark.parametrize("name", ["cfg1", "cyl1-m16", "cyl2-m8", "cart2-helm", "cart3-m8", "cart3-helm"])
+def test_vcycle_and_fmg_bitwise_exact_coarse(name):
+    p = CASES[name]()
+    print(f"case {name}", flush=True)
+    (gu, gb), (ou, ob), _ = _run_cycles(p, 3, coarse=solver.COARSE_EXACT)
+    np.testing.assert_array_equal(_valid(gu, p), _valid(ou, p))
+    np.testing.assert_array_equal(_valid(gb, p), _valid(ob, p))
+    (gu, gb), (ou, ob), _ = _run_cycles(p, 1, fmg=True, omega=1.25, coarse=solver.COARSE_EXACT)
+    np.testing.assert_array_equal(_valid(gu, p), _valid(ou, p))
+    np.testing.assert_array_equal(_valid(gb, p), _valid(ob, p))
 
 
 @pytest.mark.parametrize("name", ["cfg1", "cyl2-m8", "cart3-m8", "cfg4-small"])

@@ -1,0 +1,167 @@
+"""Refused calls leave a usable context, and option edge cases.
+
+* srmg_set_tree and srmg_regrid check every record before anything of the
+  context changes: a refused call leaves the old tree and field values, and
+  the next solve equals the same solve on a context that never saw the bad
+  records.
+* inner_cycles = 0 means one FP32 cycle per FP64 correction on the order-4
+  path too (zero-initialised options from a C or Julia caller).
+* srmg_zero also clears the order-4 ghost frames (fill!(u, 0) zeroes ghosts).
+"""
+
+import copy
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")  # noqa: F841  (torch's HIP runtime first)
+
+import regrid_cases as rc
+from srmg import solver
+from srmg.blockfield import ScalarBlockField
+from srmg.solver import F32, F64, REGRID_INTERP, REGRID_RESTRICT
+
+pytestmark = pytest.mark.gpu
+
+
+def _vs(p):
+    return (slice(p.b.G, p.b.G + p.M),) * p.D
+
+
+def _blocks(tree):
+    return [b for layer in tree for _, b in layer.pairs]
+
+
+def _bad_conns(p):
+    """Connectivities that srmg_set_tree must refuse, each with the message."""
+    out = []
+    c = copy.deepcopy(p.conn)
+    lvl = next(l for l in range(len(c.neighbor)) if len(c.neighbor[l]))
+    c.neighbor[lvl] = c.neighbor[lvl].copy()
+    c.neighbor[lvl][0, 1] = 10 ** 6
+    out.append((c, "unknown block"))
+    c = copy.deepcopy(p.conn)
+    lvl = next(l for l in range(len(c.boundary)) if len(c.boundary[l]))
+    c.boundary[lvl] = c.boundary[lvl][1:].copy()
+    out.append((c, "has no neighbour, boundary or refinement record"))
+    c = copy.deepcopy(p.conn)
+    lvl = next(l for l in range(len(c.child)) if len(c.child[l]))
+    c.child[lvl] = c.child[lvl][1:].copy()
+    out.append((c, "block without Child record"))
+    return out
+
+
+def _solve(ctx, u, b, r, u1, p, geom, disc):
+    n, h = solver.solve(u, b, r, u1, p.s, geom, p.bc, disc, tol=1e-9, max_cycles=8)
+    host = p.b.like()
+    u.download(host)
+    return n, np.asarray(h).copy(), host
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_refused_set_tree_keeps_the_old_tree_and_fields(D):
+    p = rc.problem(D)
+    geom = solver.geometry_from_code(1 if D == 2 else 0)
+    disc = solver.LaplacianDiscretization(D, 2)
+    ctx = solver.Context(p.tree, p.conn, p.M)
+    u, b, r, u1 = (ctx.field(F64) for _ in range(4))
+    b.upload(p.b)
+    seed = rc.random_field(p)
+    u.upload(seed)
+    for conn, msg in _bad_conns(p):
+        print(f"D={D}: set_tree with a bad record ({msg})", flush=True)
+        with pytest.raises(solver.SrmgError, match=msg):
+            ctx.regrid(p.tree, conn)
+    back = p.b.like()
+    u.download(back)
+    vs = _vs(p)
+    for blk in _blocks(p.tree):
+        np.testing.assert_array_equal(back.u[vs + (blk - 1,)], seed.u[vs + (blk - 1,)])
+    got = _solve(ctx, u, b, r, u1, p, geom, disc)
+    ctx2 = solver.Context(p.tree, p.conn, p.M)
+    v, b2, r2, v1 = (ctx2.field(F64) for _ in range(4))
+    b2.upload(p.b)
+    v.upload(seed)
+    ref = _solve(ctx2, v, b2, r2, v1, p, geom, disc)
+    assert got[0] == ref[0]
+    np.testing.assert_array_equal(got[1], ref[1])
+    np.testing.assert_array_equal(got[2].u[vs], ref[2].u[vs])
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_refused_regrid_keeps_the_old_tree_and_fields(D):
+    p = rc.problem(D)
+    ctx = solver.Context(p.tree, p.conn, p.M)
+    u, w = ctx.field(F64), ctx.field(F32)
+    seed = rc.random_field(p)
+    u.upload(seed)
+    w.upload(seed)
+    t2, c2, nb2, _ = rc.edited(p.tree, p.tree.nblocks())
+    q = copy.copy(p)
+    q.conn = c2
+    for conn, msg in _bad_conns(q):
+        print(f"D={D}: regrid with a bad record ({msg})", flush=True)
+        with pytest.raises(solver.SrmgError, match=msg):
+            ctx.regrid(t2, conn, fields=[(u, REGRID_INTERP | REGRID_RESTRICT), (w, REGRID_INTERP)], bc=p.bc,
+                       nblocks=nb2)
+    vs = _vs(p)
+    for f, tol in ((u, 0.0), (w, 1e-6)):
+        back = p.b.like()
+        f.download(back)
+        for blk in _blocks(p.tree):
+            np.testing.assert_allclose(back.u[vs + (blk - 1,)], seed.u[vs + (blk - 1,)], rtol=tol, atol=0)
+    # the good records still regrid the context, as they regrid a fresh one
+    ctx.regrid(t2, c2, fields=[(u, REGRID_INTERP | REGRID_RESTRICT)], bc=p.bc, nblocks=nb2)
+    ctx2 = solver.Context(p.tree, p.conn, p.M)
+    v = ctx2.field(F64)
+    v.upload(seed)
+    ctx2.regrid(t2, c2, fields=[(v, REGRID_INTERP | REGRID_RESTRICT)], bc=p.bc, nblocks=nb2)
+    a, e = ScalarBlockField(p.D, p.M, p.b.G, nb2), ScalarBlockField(p.D, p.M, p.b.G, nb2)
+    u.download(a)
+    v.download(e)
+    for blk in _blocks(t2):
+        np.testing.assert_array_equal(a.u[vs + (blk - 1,)], e.u[vs + (blk - 1,)])
+
+
+def test_order4_mixed_with_zero_inner_cycles_runs_one():
+    p = rc.problem(2)
+    geom, disc = solver.geometry_from_code(1), solver.LaplacianDiscretization(2, 4)
+    res = []
+    for inner in (0, 1):
+        ctx = solver.Context(p.tree, p.conn, p.M)
+        u, q, q1 = (ctx.field(F64) for _ in range(3))
+        e, be, re, ue = (ctx.field(F32) for _ in range(4))
+        q.upload(p.b)
+        solver.rhs(q1, q, geom, p.bc, disc)
+        n, hist = solver.solve_mixed(u, q1, e, be, re, ue, p.s, geom, p.bc, disc, inner_cycles=inner, tol=1e-30,
+                                     max_cycles=4, coarse=solver.COARSE_GS, ndown=2, ntop=20, nup=2)
+        print(f"inner_cycles={inner}: history {hist}", flush=True)
+        host = p.b.like()
+        u.download(host)
+        res.append((np.asarray(hist).copy(), host.u[_vs(p)].copy()))
+    assert res[1][0][-1] < 0.5 * res[1][0][0]
+    np.testing.assert_array_equal(res[0][0], res[1][0])
+    np.testing.assert_array_equal(res[0][1], res[1][1])
+
+
+def test_zero_clears_the_order4_ghost_frames():
+    # a V-cycle straight after zero() (no ghost pre-fill) must see the zero
+    # ghosts of a fresh field, not those of the previous solve
+    p = rc.problem(3)
+    geom, disc = solver.geometry_from_code(0), solver.LaplacianDiscretization(3, 4)
+    outs = []
+    for reuse in (True, False):
+        ctx = solver.Context(p.tree, p.conn, p.M)
+        u, q, q1, r, u1 = (ctx.field(F64) for _ in range(5))
+        q.upload(p.b)
+        solver.rhs(q1, q, geom, p.bc, disc)
+        if reuse:
+            solver.solve(u, q1, r, u1, p.s, geom, p.bc, disc, tol=1e-30, max_cycles=2, coarse=solver.COARSE_GS)
+            u.zero()
+            q.upload(p.b)
+            solver.rhs(q1, q, geom, p.bc, disc)
+        solver.vcycle(u, q1, r, u1, p.s, p.tree, p.conn, geom, p.bc, disc, ndown=2, ntop=4, nup=2)
+        host = p.b.like()
+        u.download(host)
+        outs.append(host.u[_vs(p)].copy())
+    np.testing.assert_array_equal(outs[0], outs[1])

@@ -373,3 +373,40 @@ def test_electric_field_of_a_linear_potential(D, M, rootsize):
     assert nface > 0 and ncell > 0
     # faces outside the valid range are not written
     assert np.all(f[(slice(0, G),) + (slice(None),) * D] == 0)
+
+
+AXES = [(8, False, -1, -1), (16, False, 1, -1), (64, True, 1, -1), (128, False, -1, 1),
+        (32, True, 1, 1), (64, False, 1, 1), (256, True, 1, -1), (512, False, -1, -1)]
+
+
+@pytest.mark.parametrize("n,radial,alow,ahigh", AXES)
+def test_ql_axis_decomposition_pinned_by_jacobi_and_dense_eigenvalues(n, radial, alow, ahigh):
+    """The exact coarse solve's implicit QL eigensolver against two independent
+    routes: the cyclic Jacobi method and numpy's dense eigenvalues of the
+    assembled finite-volume operator; both factorizations reconstruct T."""
+    from assemble import axis_matrix
+    from oracle import axis_decomp
+    T = axis_matrix(n, radial, alow, ahigh).toarray()
+    lam, P, Q = axis_decomp(n, radial, alow, ahigh, "ql")
+    lj, Pj, Qj = axis_decomp(n, radial, alow, ahigh, "jacobi") if n <= 128 else (None, None, None)
+    scale = np.max(np.abs(T))
+    np.testing.assert_allclose(P @ np.diag(lam) @ Q, T, rtol=0, atol=1e-12 * scale)
+    np.testing.assert_allclose(P @ Q, np.eye(n), rtol=0, atol=1e-12)
+    np.testing.assert_allclose(np.sort(lam), np.sort(np.linalg.eigvals(T).real), rtol=0, atol=1e-12 * scale)
+    if lj is not None:
+        np.testing.assert_allclose(np.sort(lam), np.sort(lj), rtol=0, atol=1e-13 * scale)
+        np.testing.assert_allclose(Pj @ np.diag(lj) @ Qj, T, rtol=0, atol=1e-12 * scale)
+
+
+@pytest.mark.parametrize("n,radial,alow,ahigh", AXES)
+def test_library_axis_decomposition_equals_the_oracle_bitwise(n, radial, alow, ahigh):
+    """The library's host eigensolver (srmg_axis_decomp, which the exact coarse
+    solve uploads) and the oracle's produce the same bits, so exact-coarse
+    V-cycles can agree bit for bit on the GPU (no device work here)."""
+    from oracle import axis_decomp
+    from srmg import solver
+    lam, P, Q = solver.axis_decomp(n, radial, alow, ahigh)
+    ol, oP, oQ = axis_decomp(n, radial, alow, ahigh, "ql")
+    np.testing.assert_array_equal(lam, ol)
+    np.testing.assert_array_equal(P, oP)
+    np.testing.assert_array_equal(Q, oQ)
