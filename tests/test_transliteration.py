@@ -33,10 +33,12 @@ def cartesian(ranges):
 
 
 class Ref:
-    def __init__(self, p):
+    denom = 1  # the @stencil denominator of the order-2 operators
+
+    def __init__(self, p, conn=None):
         self.p = p
         self.D, self.M, self.G = p.D, p.M, p.b.G
-        self.tree, self.conn = p.tree, p.conn
+        self.tree, self.conn = p.tree, (p.conn if conn is None else conn)
         self.bc = p.bc.reshape((3,) * self.D, order="F")
 
     # -- fields: Julia-layout arrays indexed with 1-based tuples ------------
@@ -76,7 +78,7 @@ class Ref:
                 continue
             I1 = tuple(I[d] + S1[d] - 2 for d in range(self.D))
             v += c * self.get(u, I1, blk)
-        return v / 1
+        return v / self.denom
 
     def diagelm(self, s):
         c = -4 if self.D == 2 else -6
@@ -286,6 +288,66 @@ class Ref:
             self.vcycle(u, b, r, u1, s, lmax=l, ndown=ndown, ntop=ntop, nup=nup, omega=omega)
 
 
+class Ref4(Ref):
+    """LaplacianDiscretization{D,4}: the compact stencil literals of
+    src/laplacian.jl:133-201 (2D Cartesian and CylindricalGeometry{1,2}) and
+    305-340 (3D Cartesian) with their denominator 6, diagelm, the FourColors
+    gauss_seidel! of src/multigrid.jl:206-232 (u -= ω (Lu + s b) / c, no
+    muladd) and the four-pass gauss_seidel_iter! of src/multigrid.jl:268-295.
+    Ghost fills run on the BoxStencil records."""
+    denom = 6
+
+    def literal(self, J, s):
+        D, p = self.D, self.p
+        if D == 3:
+            a1 = [[0, 1, 0], [1, 2, 1], [0, 1, 0]]
+            a2 = [[1, 2, 1], [2, -24, 2], [1, 2, 1]]
+            return [[[(a1, a2, a1)[k][i][j] for k in range(3)] for j in range(3)] for i in range(3)]
+        if p.geom == 0:
+            return [[1, 4, 1], [4, -20, 4], [1, 4, 1]]
+        i = J[0] if p.geom == 1 else J[1]
+        # Int / Int in Julia: the exact ratio rounded once (Python's true division)
+        lo = 2 * (i - 1) / (2 * i - 1)
+        hi = 2 * i / (2 * i - 1)
+        pm = 4 * (-2 + 9 * i - 16 * i ** 2 + 8 * i ** 3) / (2 * i - 1) ** 3
+        cc = -16 * (1 - 5 * i + 5 * i ** 2) / (1 - 2 * i) ** 2
+        pp = 4 * (1 + i - 8 * i ** 2 + 8 * i ** 3) / (2 * i - 1) ** 3
+        if p.geom == 1:
+            return [[lo, pm, lo], [4, cc, 4], [hi, pp, hi]]
+        return [[lo, 4, hi], [pm, cc, pp], [lo, 4, hi]]
+
+    def diagelm4(self, J):
+        if self.D == 3:
+            return -4
+        if self.p.geom == 0:
+            return -20 / 6
+        i = J[0] if self.p.geom == 1 else J[1]
+        return -16 * (1 - 5 * i + 5 * i ** 2) / (1 - 2 * i) ** 2 / 6
+
+    def gauss_seidel4(self, u, b, omega, s, blkpos, blk, P1, P2):
+        G, M, D = self.G, self.M, self.D
+        ranges = [list(range(G + 1, G + M + 1, 2))] * 2 + [list(range(G + 1, G + M + 1))] * (D - 2)
+        for t in itertools.product(*reversed(ranges)):
+            I1 = tuple(reversed(t))
+            p1, p2 = P1, P2
+            if D == 3:
+                p1 = (p1 + I1[2]) % 2
+                p2 = (p2 + I1[2]) % 2
+            I = (I1[0] + p1, I1[1] + p2) + I1[2:]
+            J = self.J(I, blkpos)
+            c = self.diagelm4(J)
+            Lu = self.applystencil(u, blk, I, J, s)
+            self.set(u, I, blk, self.get(u, I, blk) - omega * (Lu + s * self.get(b, I, blk)) / c)
+
+    def gauss_seidel_iter(self, u, b, omega, s, n, l):
+        for _ in range(n):
+            for P1, P2 in ((0, 0), (0, 1), (1, 0), (1, 1)):
+                for c, blk in self.tree[l - 1].pairs:
+                    self.gauss_seidel4(u, b, omega, s, c, blk, P1, P2)
+                self.fill_ghost_copy(u, l)
+                self.fill_ghost_bnd(u, l)
+
+
 def _tiny(D, geom, bcd, helm=False, k2=0.0):
     if D == 2:
         rootsize, M = (2, 3), 4
@@ -357,3 +419,103 @@ def test_fmg_matches_transliteration_bitwise(case):
             o.fmg(u, b, r, u1, p.s, ndown=1, ntop=2, nup=1)
         out.append(u)
     np.testing.assert_array_equal(out[0].u, out[1].u)
+
+
+def _tiny4(D, geom, bcd):
+    """Tiny trees properly nested for the BoxStencil: 2D three levels (a 2x2
+    patch of level-1 blocks refined, then one interior level-2 block), 3D two
+    levels (one refined block)."""
+    if D == 2:
+        rootsize, M, levels = (4, 4), 4, 3
+
+        def crit(l, c):
+            if l == 1:
+                return np.all((c >= 2) & (c <= 3), axis=1)
+            return np.all(c == np.array([4, 4]), axis=1)
+    else:
+        rootsize, M, levels = (2, 2, 2), 4, 2
+
+        def crit(l, c):
+            return np.all(c == np.array([1, 2, 1]), axis=1)
+    return configs.uniform_problem(D, rootsize, M, levels, geom, bcd, seed=5, refine=crit)
+
+
+CASES4 = [
+    (2, 0, ((1, 1), (-1, -1))),
+    (2, 1, ((1, -1), (-1, -1))),
+    (2, 2, ((-1, 1), (1, -1))),
+    (3, 0, ((1, 1), (1, -1), (-1, -1))),
+]
+
+
+@pytest.mark.parametrize("case", CASES4)
+def test_order4_vcycle_matches_transliteration_bitwise(case):
+    """The four colours in the reference's order with the 3D z-shift of their
+    parities, the compact stencils and the BoxStencil ghost fills: two
+    V-cycles of the transliteration and of the oracle, bit for bit."""
+    from srmg.tree import box_stencil, connectivity
+    p = _tiny4(*case)
+    conn = connectivity(p.tree, box_stencil(p.D))
+    ref = Ref4(p, conn)
+    o = Oracle(p.tree, conn, p.M, p.b.G, p.geom, p.bc, order=4)
+    fields = []
+    for impl in ("ref", "oracle"):
+        u, r, u1, b = p.b.like(), p.b.like(), p.b.like(), p.b.copy()
+        rng = np.random.default_rng(8)
+        for layer in p.tree:
+            for _, blk in layer.pairs:
+                u.valid(blk)[...] = rng.standard_normal((p.M,) * p.D) * 1e-3
+        for l in range(1, len(p.tree) + 1):
+            (ref.fill_ghost if impl == "ref" else o.fill_ghost)(u, l)
+        for _ in range(2):
+            if impl == "ref":
+                ref.vcycle(u, b, r, u1, p.s, ndown=2, ntop=2, nup=1, omega=1.0)
+            else:
+                o.vcycle(u, b, r, u1, p.s, ndown=2, ntop=2, nup=1, omega=1.0)
+        fields.append((u, b))
+    (ru, rb), (ou, ob) = fields
+    np.testing.assert_array_equal(ru.u, ou.u)
+    np.testing.assert_array_equal(rb.u, ob.u)
+
+
+def test_order4_colour_order_and_z_shift_are_observable():
+    """The pin above can see the colour order and the 3D parity shift: with the
+    passes reversed, or without the z shift, the transliteration's 3D result
+    differs from the oracle's."""
+    from srmg.tree import box_stencil, connectivity
+    p = _tiny4(3, 0, ((1, 1), (1, -1), (-1, -1)))
+    conn = connectivity(p.tree, box_stencil(p.D))
+    o = Oracle(p.tree, conn, p.M, p.b.G, p.geom, p.bc, order=4)
+
+    class Reversed(Ref4):
+        def gauss_seidel_iter(self, u, b, omega, s, n, l):
+            for _ in range(n):
+                for P1, P2 in ((1, 1), (1, 0), (0, 1), (0, 0)):
+                    for c, blk in self.tree[l - 1].pairs:
+                        self.gauss_seidel4(u, b, omega, s, c, blk, P1, P2)
+                    self.fill_ghost_copy(u, l)
+                    self.fill_ghost_bnd(u, l)
+
+    class NoShift(Ref4):
+        def gauss_seidel4(self, u, b, omega, s, blkpos, blk, P1, P2):
+            G, M = self.G, self.M
+            for z in range(G + 1, G + M + 1):
+                for i1 in range(G + 1, G + M + 1, 2):
+                    for i0 in range(G + 1, G + M + 1, 2):
+                        I = (i0 + P1, i1 + P2, z)  # parities without the z term
+                        J = self.J(I, blkpos)
+                        Lu = self.applystencil(u, blk, I, J, s)
+                        self.set(u, I, blk, self.get(u, I, blk)
+                                 - omega * (Lu + s * self.get(b, I, blk)) / self.diagelm4(J))
+
+    def run(impl):
+        u, r, u1, b = p.b.like(), p.b.like(), p.b.like(), p.b.copy()
+        for l in range(1, len(p.tree) + 1):
+            o.fill_ghost(u, l)
+        impl.vcycle(u, b, r, u1, p.s, ndown=1, ntop=1, nup=1, omega=1.0)
+        return u.u
+
+    want = run(o)
+    np.testing.assert_array_equal(run(Ref4(p, conn)), want)
+    assert not np.array_equal(run(Reversed(p, conn)), want)
+    assert not np.array_equal(run(NoShift(p, conn)), want)
