@@ -8,15 +8,16 @@
 // record. Here one thread handles one block: the block positions go into a
 // hash table keyed by (level, Morton code of the coordinate), a first pass
 // counts each block's records, exclusive scans give every block its offsets
-// in the four lists, and a second pass writes the records. The lists come out
+// in the four lists (three launches: per-chunk totals, a scan of the totals,
+// the chunks rescanned from their offsets), and a second pass writes the
+// records. The lists come out
 // in the reference's order: level by level, blocks in `pairs` order, offsets
 // in stencil order. Block positions may not repeat within a level; a missing
 // parent (proper nesting broken) is an error, as the reference asserts.
 
 #include <hip/hip_runtime.h>
 
-#include <hipcub/device/device_scan.hpp>
-
+#include <algorithm>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -28,6 +29,97 @@ namespace srmg {
 namespace conn {
 
 constexpr uint64_t EMPTY = ~0ull;
+
+// ---- exclusive prefix sums of K int arrays of n entries (array k at x + k n) ----
+
+constexpr int SCAN_WG = 256, SCAN_ITEMS = 16, SCAN_CHUNK = SCAN_WG * SCAN_ITEMS;
+
+// exclusive prefix of v over the work group (threads in order) and the total
+__device__ __forceinline__ int wg_exclusive(int v, int *wave_tot, int &total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int inc = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += t;
+    }
+    if (lane == 63) wave_tot[w] = inc;
+    __syncthreads();
+    int before = 0;
+    total = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_WG / 64; k++) {
+        const int t = wave_tot[k];
+        if (k < w) before += t;
+        total += t;
+    }
+    __syncthreads();
+    return before + inc - v;
+}
+
+// per chunk of SCAN_CHUNK entries: its total (part[k nchunk + chunk])
+__global__ __launch_bounds__(SCAN_WG) void k_scan_totals(const int *__restrict__ x, int64_t n, int nchunk,
+                                                         int *__restrict__ part) {
+    __shared__ int wt[SCAN_WG / 64];
+    const int k = blockIdx.y, c = blockIdx.x;
+    const int *xa = x + (size_t)k * n;
+    const int64_t i0 = (int64_t)c * SCAN_CHUNK + (int64_t)threadIdx.x * SCAN_ITEMS;
+    int v = 0;
+    for (int j = 0; j < SCAN_ITEMS; j++)
+        if (i0 + j < n) v += xa[i0 + j];
+    int total;
+    (void)wg_exclusive(v, wt, total);
+    if (threadIdx.x == 0) part[(size_t)k * nchunk + c] = total;
+}
+
+// exclusive scan of each array's chunk totals, in place (one work group per array)
+__global__ __launch_bounds__(SCAN_WG) void k_scan_chunk_offsets(int *__restrict__ part, int nchunk) {
+    __shared__ int wt[SCAN_WG / 64];
+    int *pa = part + (size_t)blockIdx.x * nchunk;
+    int carry = 0;
+    for (int base = 0; base < nchunk; base += SCAN_WG) {
+        const int i = base + (int)threadIdx.x;
+        const int v = i < nchunk ? pa[i] : 0;
+        int total;
+        const int ex = wg_exclusive(v, wt, total);
+        if (i < nchunk) pa[i] = carry + ex;
+        carry += total;
+    }
+}
+
+// each chunk rescanned from its offset into y
+__global__ __launch_bounds__(SCAN_WG) void k_scan_apply(const int *__restrict__ x, int64_t n, int nchunk,
+                                                        const int *__restrict__ part, int *__restrict__ y) {
+    __shared__ int wt[SCAN_WG / 64];
+    const int k = blockIdx.y, c = blockIdx.x;
+    const int *xa = x + (size_t)k * n;
+    int *ya = y + (size_t)k * n;
+    const int64_t i0 = (int64_t)c * SCAN_CHUNK + (int64_t)threadIdx.x * SCAN_ITEMS;
+    int loc[SCAN_ITEMS];
+    int v = 0;
+    for (int j = 0; j < SCAN_ITEMS; j++) {
+        loc[j] = i0 + j < n ? xa[i0 + j] : 0;
+        v += loc[j];
+    }
+    int total;
+    int run = part[(size_t)k * nchunk + c] + wg_exclusive(v, wt, total);
+    for (int j = 0; j < SCAN_ITEMS; j++) {
+        if (i0 + j < n) ya[i0 + j] = run;
+        run += loc[j];
+    }
+}
+
+// y = exclusive prefix sums of the K arrays of n entries in x; part holds
+// K * scan_chunks(n) ints
+static int scan_chunks(int64_t n) { return (int)std::max<int64_t>(1, (n + SCAN_CHUNK - 1) / SCAN_CHUNK); }
+
+static void exclusive_scan(const int *x, int *y, int64_t n, int K, int *part, hipStream_t st) {
+    const int nchunk = scan_chunks(n);
+    hipLaunchKernelGGL(k_scan_totals, dim3((unsigned)nchunk, (unsigned)K), dim3(SCAN_WG), 0, st, x, n, nchunk, part);
+    hipLaunchKernelGGL(k_scan_chunk_offsets, dim3((unsigned)K), dim3(SCAN_WG), 0, st, part, nchunk);
+    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nchunk, (unsigned)K), dim3(SCAN_WG), 0, st, x, n, nchunk, part,
+                       y);
+}
 
 struct ConnArgs {
     int D, nlevels, ns;
@@ -281,13 +373,9 @@ int srmg_connectivity(int D, int nlevels, const int64_t *rootsize, const int64_t
                                dvals.as<int64_t>(), dpairs.as<int64_t>(), doff.as<int64_t>(), n, a, cb, cn, cr,
                                nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, derr.as<int>());
         }
-        // exclusive scans over n + 1 entries: entry n is the total
-        size_t tmp_bytes = 0;
-        HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, cb, ob, (int)(n + 1), st));
-        dtmp.alloc(std::max<size_t>(tmp_bytes, 16));
-        HIPCHK(hipcub::DeviceScan::ExclusiveSum(dtmp.p, tmp_bytes, cb, ob, (int)(n + 1), st));
-        HIPCHK(hipcub::DeviceScan::ExclusiveSum(dtmp.p, tmp_bytes, cn, on, (int)(n + 1), st));
-        HIPCHK(hipcub::DeviceScan::ExclusiveSum(dtmp.p, tmp_bytes, cr, orr, (int)(n + 1), st));
+        // exclusive scans of the three count arrays over n + 1 entries: entry n is the total
+        dtmp.alloc((size_t)3 * scan_chunks(n + 1) * 4);
+        exclusive_scan(cb, ob, n + 1, 3, dtmp.as<int>(), st);
         // per-level offsets: the scans at the levels' first blocks
         std::vector<int> scans((size_t)3 * (n + 1));
         HIPCHK(hipMemcpyAsync(scans.data(), dofs.p, scans.size() * 4, hipMemcpyDeviceToHost, st));
