@@ -341,6 +341,18 @@ int srmg_guess_fallbacks(srmg_ctx *ctx, int *n);
 int srmg_grid_solve(srmg_ctx *ctx, int D, const int *n, double s, const srmg_op *op, const double *f, double *u,
                     int ngrid);
 
+/* srmg_grid_solve without any device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (serving
+ * and training-data loops that forbid hipMalloc). The transform operands and
+ * the intermediate grids live in a file-scope device array of 16 MiB, one per
+ * GPU, held until the process ends; the call fails (nothing launched) when
+ * the grids need more: sum over axes of (2 n_d^2 + n_d) + ngrid (1 + 2 cells)
+ * doubles, each piece rounded up to 256 bytes. Same results as
+ * srmg_grid_solve. Every call on one GPU shares that array: calls that
+ * overlap on one GPU (other threads, other contexts) race. */
+int srmg_grid_solve_noalloc(srmg_ctx *ctx, int D, const int *n, double s, const srmg_op *op, const double *f,
+                            double *u, int ngrid);
+
 /* U-Net of 3^D convolutions on a D-dimensional grid n[0] x n[1] (x n[2]):
  * `levels` resolution levels with widths[l] channels (powers of two in 8..64;
  * extents divisible by 2^(levels-1)), BF16 matrix-core kernels with FP32

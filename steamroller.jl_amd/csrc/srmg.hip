@@ -11,6 +11,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <type_traits>
 #include <string>
 #include <unordered_map>
@@ -131,6 +132,8 @@ struct srmg_ctx {
     // cleared whenever a buffer they point to may move or change its role
     std::unordered_map<std::string, std::unique_ptr<Graph>> graphs;
     uint64_t graph_gen = 0;
+    // this device's copy of the file-scope scratch array (srmg_grid_solve_noalloc)
+    void *scratch = nullptr;
     // the order-4 operator's box neighbour table and coarse-fine slots (built
     // on first use; see box_table)
     DevBuf d_box, d_box_slots;
@@ -709,9 +712,9 @@ static inline unsigned nwg(int64_t threads) { return (unsigned)((threads + WG - 
 
 static std::vector<double> read_norms(srmg_ctx *c);
 
-// n x n matrix of an axis transform on the device: A (row-major), then its
-// transpose (the coalesced operand of k_axis_gemm_c)
-static void upload_axis_matrix(DevBuf &buf, const std::vector<double> &A, int n) {
+// n x n matrix of an axis transform: A (row-major), then its transpose (the
+// coalesced operand of k_axis_gemm_c)
+static std::vector<double> axis_matrix_both(const std::vector<double> &A, int n) {
     const size_t nn = (size_t)n * n;
     std::vector<double> both(2 * nn);
     for (int i = 0; i < n; i++)
@@ -719,6 +722,13 @@ static void upload_axis_matrix(DevBuf &buf, const std::vector<double> &A, int n)
             both[(size_t)i * n + k] = A[(size_t)i * n + k];
             both[nn + (size_t)k * n + i] = A[(size_t)i * n + k];
         }
+    return both;
+}
+
+// the same on the device
+static void upload_axis_matrix(DevBuf &buf, const std::vector<double> &A, int n) {
+    const size_t nn = (size_t)n * n;
+    std::vector<double> both = axis_matrix_both(A, n);
     buf.alloc(2 * nn * sizeof(double));
     HIPCHK(hipMemcpy(buf.p, both.data(), 2 * nn * sizeof(double), hipMemcpyHostToDevice));
 }
@@ -2136,6 +2146,34 @@ static void solve_mixed(srmg_ctx *c, int ui, int bi, int ei, int bei, int rei, i
 // C ABI
 // ---------------------------------------------------------------------------
 
+// ---------------------------------------------------------------------------
+// device scratch of the entry point that allocates nothing
+// ---------------------------------------------------------------------------
+
+// srmg_grid_solve_noalloc keeps its transform operands and intermediate grids
+// here instead of in hipMalloc'd buffers. Every GPU holds its own copy until
+// the process ends; calls on one GPU share it, so calls that overlap on one
+// GPU race.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)16 << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+// The device address of the current device's scratch array, looked up once
+// per device (host code uses this address, never the array's name).
+static void *scratch_address() {
+    static std::mutex mu;
+    static std::unordered_map<int, void *> addr;
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = addr.find(dev);
+    if (it != addr.end()) return it->second;
+    void *p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) fail("hipGetSymbolAddress returned no address for the scratch array");
+    addr[dev] = p;
+    return p;
+}
+
 extern "C" {
 
 const char *srmg_last_error(void) { return g_err.c_str(); }
@@ -2150,6 +2188,7 @@ int srmg_create(srmg_ctx **out, int D, int M) {
         c->V = (D == 2) ? M * M : M * M * M;
         c->F = (D == 2) ? M : M * M;
         HIPCHK(hipStreamCreate(&c->stream));
+        c->scratch = scratch_address();
         *out = c.release();
     });
 }
@@ -3136,6 +3175,101 @@ static void electric_field_impl(srmg_ctx *c, int ui, int eabsi, double h, const 
     }
 }
 
+// ---------------------------------------------------------------------------
+// exact solves on stacked uniform grids (srmg_grid_solve and its variant
+// without device allocations)
+// ---------------------------------------------------------------------------
+
+struct GridSolvePlan {
+    int dims;      // D
+    int N[3];      // extents (N[2] = 1 in 2D)
+    size_t total;  // cells per grid
+    int rdim;      // radial dimension, or -1
+};
+
+// device buffers of one grid solve: per axis the eigenvalues and the two
+// transform matrices (each as written by axis_matrix_both), the per-grid
+// Helmholtz shifts and two ping-pong grids
+struct GridSolveBufs {
+    double *lam[3], *P[3], *Q[3];
+    double *shift, *g1, *g2;
+};
+
+static GridSolvePlan grid_solve_plan(srmg_ctx *c, int D, const int *n, const srmg_op *op, const double *f,
+                                     const double *u, int ngrid) {
+    if (!c || !n || !op || !f || !u) fail("null argument");
+    if (D != 2 && D != 3) fail("D must be 2 or 3");
+    if (ngrid < 1) fail("ngrid must be >= 1");
+    GridSolvePlan g{D, {n[0], n[1], D == 3 ? n[2] : 1}, 0, -1};
+    for (int d = 0; d < D; d++)
+        if (g.N[d] < 1) fail("bad grid extents");
+    if (op->geometry != SRMG_CARTESIAN && D != 2) fail("cylindrical geometry is two-dimensional");
+    g.rdim = (op->geometry == SRMG_CYL1) ? 0 : (op->geometry == SRMG_CYL2 ? 1 : -1);
+    g.total = (size_t)g.N[0] * g.N[1] * g.N[2];
+    return g;
+}
+
+// Separable exact solve of ngrid stacked grids: the operands are computed on
+// the host and copied into bufs on the context's stream, then forward
+// transforms along every axis, the eigenvalue scaling and the inverse
+// transforms, the last one into u. Returns when u is complete.
+static void grid_solve_run(srmg_ctx *c, const GridSolvePlan &g, double s, const srmg_op *op, const double *f,
+                           double *u, int ngrid, const GridSolveBufs &bufs) {
+    const int *N = g.N;
+    const size_t total = g.total;
+    // host operands stay alive until the stream is synchronized below
+    std::vector<std::vector<double>> hold;
+    hold.reserve(3 * 3 + 1);
+    const int dims = g.dims;
+    for (int d = 0; d < dims; d++) {
+        int il = 0, ih = 0, m = 1;
+        for (int e = 0; e < dims; e++) {
+            il += ((e == d ? -1 : 0) + 1) * m;
+            ih += ((e == d ? 1 : 0) + 1) * m;
+            m *= 3;
+        }
+        const int nd = N[d];
+        std::vector<double> hl(nd), hP((size_t)nd * nd), hQ((size_t)nd * nd);
+        axis_decomp(nd, d == g.rdim, (double)op->bc[il], (double)op->bc[ih], hl.data(), hP.data(), hQ.data());
+        hold.push_back(std::move(hl));
+        HIPCHK(hipMemcpyAsync(bufs.lam[d], hold.back().data(), nd * sizeof(double), hipMemcpyHostToDevice,
+                              c->stream));
+        hold.push_back(axis_matrix_both(hP, nd));
+        HIPCHK(hipMemcpyAsync(bufs.P[d], hold.back().data(), hold.back().size() * sizeof(double),
+                              hipMemcpyHostToDevice, c->stream));
+        hold.push_back(axis_matrix_both(hQ, nd));
+        HIPCHK(hipMemcpyAsync(bufs.Q[d], hold.back().data(), hold.back().size() * sizeof(double),
+                              hipMemcpyHostToDevice, c->stream));
+    }
+    hold.push_back(std::vector<double>(ngrid, op->disc == SRMG_HELMHOLTZ ? -s * op->k2 : 0.0));
+    HIPCHK(hipMemcpyAsync(bufs.shift, hold.back().data(), ngrid * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    const double *in = f;
+    double *a = bufs.g1, *b2 = bufs.g2;
+    auto axis = [&](const double *A, int d, int stride, const double *x, double *y) {
+        launch_axis_gemm(c->stream, A, N[d], stride, (int)(ngrid * total / N[d]), x, y, c->axis_gemm);
+    };
+    int stride = 1;
+    for (int d = 0; d < dims; d++) {
+        axis(bufs.Q[d], d, stride, in, a);
+        in = a;
+        std::swap(a, b2);
+        stride *= N[d];
+    }
+    double *cur = const_cast<double *>(in);
+    hipLaunchKernelGGL(k_eig_scale, dim3(nwg(total * ngrid)), dim3(WG), 0, c->stream, cur, bufs.lam[0], bufs.lam[1],
+                       dims == 3 ? bufs.lam[2] : bufs.lam[1], N[0], N[1], dims, bufs.shift, total, ngrid);
+    stride = 1;
+    for (int d = 0; d < dims; d++) {
+        double *dst = (d == dims - 1) ? u : a;
+        axis(bufs.P[d], d, stride, cur, dst);
+        cur = dst;
+        std::swap(a, b2);
+        stride *= N[d];
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+}
+
 extern "C" {
 
 int srmg_set_guess(srmg_ctx *c, srmg_unet *net) {
@@ -3155,65 +3289,53 @@ int srmg_guess_fallbacks(srmg_ctx *c, int *n) {
 int srmg_grid_solve(srmg_ctx *c, int D, const int *n, double s, const srmg_op *op, const double *f, double *u,
                     int ngrid) {
     return guarded([&] {
-        if (!c || !n || !op || !f || !u) fail("null argument");
-        if (D != 2 && D != 3) fail("D must be 2 or 3");
-        if (ngrid < 1) fail("ngrid must be >= 1");
-        int N[3] = {n[0], n[1], D == 3 ? n[2] : 1};
-        for (int d = 0; d < D; d++)
-            if (N[d] < 1) fail("bad grid extents");
-        if (op->geometry != SRMG_CARTESIAN && D != 2) fail("cylindrical geometry is two-dimensional");
-        const int rdim = (op->geometry == SRMG_CYL1) ? 0 : (op->geometry == SRMG_CYL2 ? 1 : -1);
-        DevBuf P[3], Q[3], lam[3];
+        GridSolvePlan g = grid_solve_plan(c, D, n, op, f, u, ngrid);
+        DevBuf P[3], Q[3], lam[3], shift, g1, g2;
+        GridSolveBufs bufs{};
         for (int d = 0; d < D; d++) {
-            int il = 0, ih = 0, m = 1;
-            for (int e = 0; e < D; e++) {
-                il += ((e == d ? -1 : 0) + 1) * m;
-                ih += ((e == d ? 1 : 0) + 1) * m;
-                m *= 3;
-            }
-            const int nd = N[d];
-            std::vector<double> hl(nd), hP((size_t)nd * nd), hQ((size_t)nd * nd);
-            axis_decomp(nd, d == rdim, (double)op->bc[il], (double)op->bc[ih], hl.data(), hP.data(), hQ.data());
-            lam[d].alloc(nd * sizeof(double));
-            HIPCHK(hipMemcpy(lam[d].p, hl.data(), nd * sizeof(double), hipMemcpyHostToDevice));
-            upload_axis_matrix(P[d], hP, nd);
-            upload_axis_matrix(Q[d], hQ, nd);
+            lam[d].alloc(g.N[d] * sizeof(double));
+            P[d].alloc(2 * (size_t)g.N[d] * g.N[d] * sizeof(double));
+            Q[d].alloc(2 * (size_t)g.N[d] * g.N[d] * sizeof(double));
+            bufs.lam[d] = lam[d].as<double>();
+            bufs.P[d] = P[d].as<double>();
+            bufs.Q[d] = Q[d].as<double>();
         }
-        const size_t total = (size_t)N[0] * N[1] * N[2];
-        std::vector<double> hs(ngrid, op->disc == SRMG_HELMHOLTZ ? -s * op->k2 : 0.0);
-        DevBuf shift, g1, g2;
         shift.alloc(ngrid * sizeof(double));
-        g1.alloc(total * ngrid * sizeof(double));
-        g2.alloc(total * ngrid * sizeof(double));
-        HIPCHK(hipMemcpy(shift.p, hs.data(), ngrid * sizeof(double), hipMemcpyHostToDevice));
-        HIPCHK(hipStreamSynchronize(c->stream));
-        const double *in = f;
-        double *a = g1.as<double>(), *b2 = g2.as<double>();
-        auto axis = [&](const DevBuf &A, int d, int stride, const double *x, double *y) {
-            launch_axis_gemm(c->stream, A.as<double>(), N[d], stride, (int)(ngrid * total / N[d]), x, y,
-                             c->axis_gemm);
+        g1.alloc(g.total * ngrid * sizeof(double));
+        g2.alloc(g.total * ngrid * sizeof(double));
+        bufs.shift = shift.as<double>();
+        bufs.g1 = g1.as<double>();
+        bufs.g2 = g2.as<double>();
+        grid_solve_run(c, g, s, op, f, u, ngrid, bufs);
+    });
+}
+
+int srmg_grid_solve_noalloc(srmg_ctx *c, int D, const int *n, double s, const srmg_op *op, const double *f,
+                            double *u, int ngrid) {
+    return guarded([&] {
+        GridSolvePlan g = grid_solve_plan(c, D, n, op, f, u, ngrid);
+        if (!c->scratch) fail("the context has no device scratch address");
+        // carve the scratch array: every piece starts on a 256-byte boundary
+        char *base = static_cast<char *>(c->scratch);
+        size_t off = 0;
+        auto take = [&](size_t bytes) {
+            double *p = reinterpret_cast<double *>(base + off);
+            off += (bytes + 255) & ~(size_t)255;
+            return p;
         };
-        int stride = 1;
+        GridSolveBufs bufs{};
         for (int d = 0; d < D; d++) {
-            axis(Q[d], d, stride, in, a);
-            in = a;
-            std::swap(a, b2);
-            stride *= N[d];
+            bufs.lam[d] = take(g.N[d] * sizeof(double));
+            bufs.P[d] = take(2 * (size_t)g.N[d] * g.N[d] * sizeof(double));
+            bufs.Q[d] = take(2 * (size_t)g.N[d] * g.N[d] * sizeof(double));
         }
-        double *cur = const_cast<double *>(in);
-        hipLaunchKernelGGL(k_eig_scale, dim3(nwg(total * ngrid)), dim3(WG), 0, c->stream, cur, lam[0].as<double>(),
-                           lam[1].as<double>(), D == 3 ? lam[2].as<double>() : lam[1].as<double>(), N[0], N[1], D,
-                           shift.as<double>(), total, ngrid);
-        stride = 1;
-        for (int d = 0; d < D; d++) {
-            double *dst = (d == D - 1) ? u : a;
-            axis(P[d], d, stride, cur, dst);
-            cur = dst;
-            std::swap(a, b2);
-            stride *= N[d];
-        }
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipStreamSynchronize(c->stream));
+        bufs.shift = take(ngrid * sizeof(double));
+        bufs.g1 = take(g.total * ngrid * sizeof(double));
+        bufs.g2 = take(g.total * ngrid * sizeof(double));
+        if (off > NOALLOC_SCRATCH_BYTES)
+            fail("srmg_grid_solve_noalloc: the grids need " + std::to_string(off) + " bytes of scratch, above the " +
+                 std::to_string(NOALLOC_SCRATCH_BYTES) + "-byte maximum (use srmg_grid_solve)");
+        grid_solve_run(c, g, s, op, f, u, ngrid, bufs);
     });
 }
 

@@ -171,6 +171,7 @@ def lib():
         "srmg_set_guess": [P, P],
         "srmg_guess_fallbacks": [P, C.POINTER(i32)],
         "srmg_grid_solve": [P, i32, P, dbl, P, P, P, i32],
+        "srmg_grid_solve_noalloc": [P, i32, P, dbl, P, P, P, i32],
         "srmg_unet_create": [C.POINTER(P), i32, P, i32, P, C.c_uint64],
         "srmg_unet_destroy": [P],
         "srmg_unet_param_count": [P, C.POINTER(i64)],
@@ -673,13 +674,15 @@ def _ptr(x) -> int:
     return int(x.data_ptr()) if hasattr(x, "data_ptr") else int(x)
 
 
-def grid_solve(ctx: Context, D: int, n, s: float, geometry, bc, disc, f, u, ngrid: int):
+def grid_solve(ctx: Context, D: int, n, s: float, geometry, bc, disc, f, u, ngrid: int, noalloc: bool = False):
     """Exact FP64 solve of L_1 u = f on ngrid stacked uniform level-1 grids
-    (device buffers, x fastest)."""
+    (device buffers, x fastest). noalloc: srmg_grid_solve_noalloc, which
+    allocates no device memory (its grids are limited by the library's fixed
+    scratch array)."""
     op = make_op(geometry, bc, disc, COARSE_EXACT)
     nn = (C.c_int * 3)(*(list(n) + [1] * (3 - len(n))))
-    _check(lib().srmg_grid_solve(ctx.h, int(D), nn, float(s), C.byref(op), C.c_void_p(_ptr(f)),
-                                 C.c_void_p(_ptr(u)), int(ngrid)))
+    fn = lib().srmg_grid_solve_noalloc if noalloc else lib().srmg_grid_solve
+    _check(fn(ctx.h, int(D), nn, float(s), C.byref(op), C.c_void_p(_ptr(f)), C.c_void_p(_ptr(u)), int(ngrid)))
 
 
 def set_guess(ctx: Context, net: Optional["UNet"]):
