@@ -8,7 +8,8 @@ right-hand side already resident in HBM. Two forms are timed: plain FP64
 V-cycles, and FP64 defect correction around FP32 V-cycles (FP64 residual,
 update and coarsest-level solve) with 1, 2 or 3 FP32 cycles per correction,
 from u = 0 and from the learned level-1 first iterate (the package's trained
-network). The headline is the fastest FP32 form. The smoother kernel's average
+network), each also with fmg! (the reference's full multigrid) as the first
+cycle. The headline is the fastest FP32 form. The smoother kernel's average
 launch time is measured with HIP events on the solver's stream for the
 roofline figure; its HBM traffic per launch comes from the committed counter
 summary when that summary is for the same kernel. The CPU baseline is the
@@ -104,9 +105,12 @@ def main():
 
     # every form starts from u = 0 (GUESS_ZERO: the initial norm and defect from b alone)
     Z = solver.GUESS_ZERO
-    specs = {"fp64": (False, 1, Z), "mixed": (True, 1, Z), "mixed2": (True, 2, Z), "mixed3": (True, 3, Z)}
+    specs = {"fp64": (False, 1, Z, False), "fp64_fmg": (False, 1, Z, True), "mixed": (True, 1, Z, False),
+             "mixed2": (True, 2, Z, False), "mixed3": (True, 3, Z, False), "mixed2_fmg": (True, 2, Z, True),
+             "mixed3_fmg": (True, 3, Z, True)}
     if net is not None:
-        specs["mixed2_learned"] = (True, 2, solver.GUESS_LEARNED)
+        specs["mixed2_learned"] = (True, 2, solver.GUESS_LEARNED, False)
+        specs["mixed3_fmg_learned"] = (True, 3, solver.GUESS_LEARNED, True)
     G, M = p.b.G, p.M
 
     def valid(f):
@@ -116,17 +120,18 @@ def main():
         return float(np.max(np.abs(a - ref)) / np.max(np.abs(ref)))
 
     forms, potentials, histories = {}, {}, {}
-    for name, (mixed, inner, gmode) in specs.items():
+    for name, (mixed, inner, gmode, fmg) in specs.items():
         times, cycles = [], []
         for it in range(args.warmup + args.steps):
             reset()
             ctx.synchronize()
             ts = time.perf_counter()
             if not mixed:
-                n, hist = solver.solve(u, b, r, u1, p.s, geom, p.bc, disc, tol=1e-8, max_cycles=60, guess=gmode)
+                n, hist = solver.solve(u, b, r, u1, p.s, geom, p.bc, disc, tol=1e-8, max_cycles=60, guess=gmode,
+                                       first_fmg=fmg)
             else:
                 n, hist = solver.solve_mixed(u, b, e, be, re, ue, p.s, geom, p.bc, disc, tol=1e-8,
-                                             max_cycles=60, inner_cycles=inner, guess=gmode)
+                                             max_cycles=60, inner_cycles=inner, guess=gmode, first_fmg=fmg)
             ctx.synchronize()
             dt = time.perf_counter() - ts
             if it >= args.warmup:
@@ -136,7 +141,8 @@ def main():
                 raise RuntimeError(f"{name} did not converge: {hist}")
         forms[name] = {"ms": 1e3 * float(np.mean(times)), "cycles": int(cycles[-1]),
                        "mcells_s": n_leaf / float(np.mean(times)) / 1e6, "final_rel_residual": float(hist[-1]),
-                       "first_rel_residual": float(hist[0])}
+                       "first_rel_residual": float(hist[0]),
+                       "history": [float(f"{x:.3g}") for x in hist]}
         if gmode == solver.GUESS_LEARNED:
             forms[name]["guess_fallbacks"] = solver.guess_fallbacks(ctx)
         log(f"[bench] {name}: {forms[name]}")
@@ -145,10 +151,10 @@ def main():
         potentials[name] = valid(got).copy()
         histories[name] = np.asarray(hist).copy()
         if name != "fp64":
-            # potential error of the mixed form against the FP64 GPU solution
+            # potential error of the form against the plain FP64 GPU solution
             forms[name]["potential_rel_err_vs_fp64"] = rel_err(potentials[name], potentials["fp64"])
     del got
-    best = min((k for k in forms if k != "fp64"), key=lambda k: forms[k]["ms"])
+    best = min((k for k in forms if not k.startswith("fp64")), key=lambda k: forms[k]["ms"])
 
     # roofline: the smoother pass on the finest level, FP64 and FP32
     L = len(p.tree)
@@ -163,7 +169,12 @@ def main():
         roof[name] = {"ms": ms, "gbs": algo / (ms * 1e-3) / 1e9, "bytes": algo}
     ms = ctx.time_kernel(solver.KERNEL_STREAM, L, u, b, r, p.s, geom, p.bc, disc, reps=40)
     stream_gbs = 3 * 8 * cells / (ms * 1e-3) / 1e9
-    log(f"[bench] smoother: {roof}; stream r=u+b on the same level: {stream_gbs:.0f} GB/s")
+    stream16 = {}
+    for name, (uu, bb, rr, w) in {"fp64": (u, b, r, 8), "fp32": (e, be, re, 4)}.items():
+        ms = ctx.time_kernel(solver.KERNEL_STREAM16, L, uu, bb, rr, p.s, geom, p.bc, disc, reps=40)
+        stream16[name] = round(3 * w * cells / (ms * 1e-3) / 1e9, 1)
+    log(f"[bench] smoother: {roof}; stream r=u+b on the same level: {stream_gbs:.0f} GB/s; "
+        f"16-byte nontemporal stream: {stream16} GB/s")
 
     cpu = None
     if not args.no_cpu:
@@ -211,8 +222,11 @@ def main():
                                   "from the oracle's FP64 solve of the same problem to 1e-8 (the CPU baseline run)",
                       headline_form=best,
                       speedup_vs_fp64=round(forms["fp64"]["ms"] / forms[best]["ms"], 3),
-                      note="mixedK: FP64 defect correction around K FP32 V-cycles; _learned: first iterate "
-                           "from the network's level-1 prediction (level-1 smoothing, zero fallback)"),
+                      speedup_vs_fp64_fmg=round(forms["fp64_fmg"]["ms"] / forms[best]["ms"], 3),
+                      note="fp64: plain FP64 V-cycles (the reference's solver); mixedK: FP64 defect correction "
+                           "around K FP32 V-cycles; _fmg: the first cycle (first correction) is fmg!; "
+                           "_learned: first iterate from the network's level-1 prediction (level-1 smoothing, "
+                           "zero fallback)"),
         "roofline": {
             "bound": "hbm",
             "kernel": f"{kname['float']} (level {L}, {nblk} blocks)",
@@ -226,6 +240,7 @@ def main():
                             "ms": round(roof["fp64"]["ms"], 5)},
             "ms_per_launch": round(roof["fp32"]["ms"], 5),
             "measured_stream_gbs": round(stream_gbs, 1),
+            "measured_stream16_gbs": stream16,
             "tile_shapes": {"fp64": tuned64.tolist(), "fp32": tuned32.tolist()},
             "algorithmic_bytes_per_launch": roof["fp32"]["bytes"],
         },
@@ -425,10 +440,11 @@ def regrid_step(log, p, ctx, u, b, e):
 
 def sweep_kernel_name(D, M, t, cfg, grouped):
     """Name of the sweep kernel a tile shape launches (as the profiler lists it)."""
-    if cfg in (7, 8) and grouped and (D, M) == (3, 8):
-        return f"k_gs_sweep_col{'z' if cfg == 8 else ''}<{D}, {M}, {t}, -1>"
+    if cfg in (7, 8, 9, 10, 11) and grouped and (D, M) == (3, 8):
+        var = {8: ", 0", 9: ", 1", 10: ", 2", 11: ", 3"}.get(cfg, "")
+        return f"k_gs_sweep_col{'' if cfg == 7 else 'z'}<{D}, {M}, {t}, -1{var}>"
     if cfg >= 4 and grouped and ((D, M) in ((3, 8), (2, 16))):
-        nt = {4: 256, 5: 512, 6: 1024, 7: 256, 8: 512}[cfg]
+        nt = {4: 256, 5: 512, 6: 1024, 7: 256, 8: 512, 9: 512, 10: 512, 11: 512}[cfg]
         if ((1 << D) * M ** D // 2) % nt == 0:
             return f"k_gs_sweep_group<{D}, {M}, {t}, {nt}, -1>"
     small = (M ** D // 2) <= 256

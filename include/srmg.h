@@ -248,6 +248,10 @@ typedef struct {
     int32_t inner_cycles;      /* mixed precision: FP32 V-cycles per FP64 correction (0 means 1) */
     int32_t guess;             /* first iterate: SRMG_GUESS_* */
     int32_t guess_smooth;      /* Gauss-Seidel sweeps on level 1 after a learned first iterate */
+    int32_t first_fmg;         /* 1: the first cycle is fmg! (src/multigrid.jl:17-51) from the first
+                                  iterate, the later ones vcycle!; in the mixed form the first
+                                  correction runs fmg! on e before its inner_cycles V-cycles.
+                                  Order-2 operators. */
 } srmg_solve_opts;
 
 /* First iterate of a solve. NONE: u as given. ZERO: u = 0 whatever u holds
@@ -383,13 +387,14 @@ int srmg_unet_synchronize(srmg_unet *net);
 /* ---- measurement ---- */
 
 enum { SRMG_KERNEL_GS_PASS = 0, SRMG_KERNEL_RESIDUAL = 1, SRMG_KERNEL_GS_SWEEP = 2, SRMG_KERNEL_STREAM = 3,
-       SRMG_KERNEL_RESIDUAL_GATHER = 4 };
+       SRMG_KERNEL_RESIDUAL_GATHER = 4, SRMG_KERNEL_STREAM16 = 5 };
 
 /* Average duration of one launch of a level kernel, timed with HIP events on
  * the context's stream over `reps` back-to-back launches: one colour of the
  * Gauss-Seidel sweep (alternating), the residual (LDS-tiled, or the per-cell
  * gather form), one fused red-black sweep, or the stream kernel r = u + b
- * over the level (bandwidth calibration).
+ * over the level (bandwidth calibration; STREAM16: 16-byte loads and
+ * nontemporal 16-byte stores per lane).
  * Modifies u (smoothers) or r (residual, stream). */
 int srmg_time_kernel(srmg_ctx *ctx, int kind, int level, int u, int b, int r, double s,
                      const srmg_op *op, int reps, double *ms_per_launch);
@@ -401,11 +406,13 @@ int srmg_time_kernel(srmg_ctx *ctx, int kind, int level, int u, int b, int r, do
  * size; 4, 5, 6 (3D M=8 and 2D M=16) one sibling group of 2^D blocks as one
  * tile on 256 / 512 / 1024 threads; 7 (3D M=8) one sibling group on 256
  * threads, one thread per column of the group; 8 (3D M=8) the same on 512
- * threads, two per column (levels without complete sibling groups, and thread
- * counts that do not divide a group's cells per colour, run 0; 7 and 8 run 4
- * and 5 where the column forms do not apply). */
+ * threads, two per column; 9, 10, 11 (3D M=8) shape 8 with nontemporal
+ * stores of the result, nontemporal loads of the right-hand side, or both
+ * (levels without complete sibling groups, and thread counts that do not
+ * divide a group's cells per colour, run 0; 7 to 11 run 4 and 5 where the
+ * column forms do not apply). */
 int srmg_autotune(srmg_ctx *ctx, int u, int b, double s, const srmg_op *op, int reps, int *chosen);
-/* Force the tile shape (0..8) of the fused sweep for one element type and level (1-based).
+/* Force the tile shape (0..11) of the fused sweep for one element type and level (1-based).
  * Until autotuned or forced, 3D M=8 contexts run shape 8 and the others shape 0. */
 int srmg_set_sweep_config(srmg_ctx *ctx, int dtype, int level, int cfg);
 /* 1 when the level's blocks form complete sibling groups (tile shapes 4..8 apply). */

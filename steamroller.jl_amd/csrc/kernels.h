@@ -292,21 +292,29 @@ __device__ __forceinline__ T ghost_value(const T *__restrict__ x, const T *__res
     return bc.v[bi] * v;
 }
 
+// frame cell q of block blk: same-level copy or boundary mirror (coarse-fine
+// frame cells are k_frame_interp's)
+template <int D, int M, typename T>
+__device__ __forceinline__ void frame_copybnd_cell(T *__restrict__ g, const T *__restrict__ x,
+                                                   const int *__restrict__ box, int blk, int q, const Bc27<T> &bc) {
+    constexpr int P = Frame<D, M>::P, NB3 = D == 3 ? 27 : 9;
+    int c[3], dir[3], k;
+    if (!frame_cell<D, M>(q, c, dir, k)) return;
+    if (box[blk * NB3 + k] < -8) return;  // coarse-fine: k_frame_interp
+    g[(size_t)blk * P + q] = ghost_value<D, M, T>(x, g, box, bc, blk, c);
+}
+
 // fill_ghost_copy! then fill_ghost_bnd! in one pass (ghost_value); one pass
 // needs no ordering between blocks and gives the bits of the two in order
 template <int D, int M, typename T>
 __global__ __launch_bounds__(WG) void k_frame_copybnd(T *__restrict__ g, const T *__restrict__ x,
                                                       const int *__restrict__ box, int blk0, int nblk, Bc27<T> bc) {
-    constexpr int P = Frame<D, M>::P, NB3 = D == 3 ? 27 : 9;
+    constexpr int P = Frame<D, M>::P;
     // 64-bit: a level's nblk * P frame cells can pass 2^31 before its nblk * V cells do
     const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t lb = gt / P;
     if (lb >= nblk) return;
-    const int q = (int)(gt - lb * P), blk = blk0 + (int)lb;
-    int c[3], dir[3], k;
-    if (!frame_cell<D, M>(q, c, dir, k)) return;
-    if (box[blk * NB3 + k] < -8) return;  // coarse-fine: k_frame_interp
-    g[(size_t)blk * P + q] = ghost_value<D, M, T>(x, g, box, bc, blk, c);
+    frame_copybnd_cell<D, M, T>(g, x, box, blk0 + (int)lb, (int)(gt - lb * P), bc);
 }
 
 // reads of cells at most one cell outside a block for the order-4 stencils:
@@ -509,18 +517,13 @@ __global__ __launch_bounds__(WG) void k_defect4(float *__restrict__ be, const do
 // pass that follows a copy-then-boundary fill: the cells one pass reads across
 // a block face are of other colours, or, for a boundary mirror, the cell being
 // updated itself (read before its write), so none changes during the pass
+// colour cell q of block blk in the FourColors{P1,P2} pass
 template <int D, int M, typename T, bool LIVE>
-__global__ __launch_bounds__(WG) void k_gs4_pass(T *__restrict__ u, const T *__restrict__ b,
-                                                 const T *__restrict__ g, const int *__restrict__ coord, int rd,
-                                                 int blk0, int nblk, T s, T omega, int P1, int P2,
-                                                 const int *__restrict__ box, Bc27<T> bc) {
+__device__ __forceinline__ void gs4_cell(T *__restrict__ u, const T *__restrict__ b, const T *__restrict__ g,
+                                         const int *__restrict__ coord, int rd, int blk, int q, T s, T omega, int P1,
+                                         int P2, const int *__restrict__ box, const Bc27<T> &bc) {
     using LY = Lay<D, M>;
-    constexpr int H2 = M / 2, NCC = D == 3 ? H2 * H2 * M : H2 * H2;
-    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
-    const int lb = gt / NCC;
-    if (lb >= nblk) return;
-    const int q = gt - lb * NCC;
-    const int blk = blk0 + lb;
+    constexpr int H2 = M / 2;
     const int i0 = q % H2, i1 = (q / H2) % H2, z = D == 3 ? q / (H2 * H2) : 0;
     const int p1 = D == 3 ? (P1 + z + 3) & 1 : P1, p2 = D == 3 ? (P2 + z + 3) & 1 : P2;
     const int x0 = 2 * i0 + p1, x1 = 2 * i1 + p2;
@@ -532,6 +535,19 @@ __global__ __launch_bounds__(WG) void k_gs4_pass(T *__restrict__ u, const T *__r
         Lu = apply4a<D, M, T>(FrameRead<D, M, T>{u, g}, coord, blk, x0, x1, z, rd, c);
     u[o] = u[o] - omega * (Lu + s * b[o]) / c;
 }
+
+template <int D, int M, typename T, bool LIVE>
+__global__ __launch_bounds__(WG) void k_gs4_pass(T *__restrict__ u, const T *__restrict__ b,
+                                                 const T *__restrict__ g, const int *__restrict__ coord, int rd,
+                                                 int blk0, int nblk, T s, T omega, int P1, int P2,
+                                                 const int *__restrict__ box, Bc27<T> bc) {
+    constexpr int H2 = M / 2, NCC = D == 3 ? H2 * H2 * M : H2 * H2;
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lb = gt / NCC;
+    if (lb >= nblk) return;
+    gs4_cell<D, M, T, LIVE>(u, b, g, coord, rd, blk0 + lb, gt - lb * NCC, s, omega, P1, P2, box, bc);
+}
+
 
 // residual! (src/multigrid.jl:309-323) with the order-4 operator
 template <int D, int M, typename T>
@@ -677,6 +693,20 @@ __global__ __launch_bounds__(WG) void k_triad(T *__restrict__ r, const T *__rest
                                               size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) r[i] = a[i] + b[i];
+}
+
+// the same with 16-byte loads and nontemporal 16-byte stores per lane (n a
+// multiple of 16 / sizeof(T)): the strongest plain stream form, the
+// bandwidth ceiling of the calibration
+template <typename T>
+__global__ __launch_bounds__(WG) void k_triad16(T *__restrict__ r, const T *__restrict__ a, const T *__restrict__ b,
+                                                size_t n) {
+    constexpr int K = 16 / sizeof(T);
+    typedef T V __attribute__((ext_vector_type(K)));
+    const size_t i = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * K;
+    if (i >= n) return;
+    const V x = *reinterpret_cast<const V *>(a + i), y = *reinterpret_cast<const V *>(b + i);
+    __builtin_nontemporal_store(x + y, reinterpret_cast<V *>(r + i));
 }
 
 template <typename T>
@@ -1637,7 +1667,10 @@ struct ColSweepZ {
                                                             sm.tile + TL::origin(k), coord, a, sm.prm[k], rd, rb, rg);
     }
 
-    template <int P>
+    // VAR bit 0: the column goes out with nontemporal stores; bit 1: the
+    // block's own right-hand side is read with nontemporal loads (both keep
+    // L2 for the neighbours' face cells, which other work groups re-read)
+    template <int P, int VAR>
     __device__ static __forceinline__ void run(Smem &sm, T *__restrict__ uo, const T *__restrict__ ui,
                                                const T *__restrict__ b, const T *__restrict__ ib,
                                                const int *__restrict__ coord, const LevelArgs<T> &a, int blk0,
@@ -1651,7 +1684,8 @@ struct ColSweepZ {
         static_for<M>([&](auto zc) {
             constexpr int z = decltype(zc)::value;
             uc[z] = ui[g0 + CS::template zoff<P, z>()];
-            bc[z] = b[g0 + CS::template zoff<P, z>()];
+            if constexpr (VAR & 2) bc[z] = __builtin_nontemporal_load(&b[g0 + CS::template zoff<P, z>()]);
+            else bc[z] = b[g0 + CS::template zoff<P, z>()];
         });
         if (tid < NB * HT::N) sm.tab[tid] = tv;
         if (tid < NB) sm.prm[tid] = bpl;
@@ -1733,12 +1767,13 @@ struct ColSweepZ {
         });
         static_for<M>([&](auto zc) {
             constexpr int z = decltype(zc)::value;
-            uo[g0 + CS::template zoff<P, z>()] = uc[z];
+            if constexpr (VAR & 1) __builtin_nontemporal_store(uc[z], &uo[g0 + CS::template zoff<P, z>()]);
+            else uo[g0 + CS::template zoff<P, z>()] = uc[z];
         });
     }
 };
 
-template <int D, int M, typename T, int RD>
+template <int D, int M, typename T, int RD, int VAR = 0>
 __global__ __launch_bounds__(512) void k_gs_sweep_colz(T *__restrict__ uo, const T *__restrict__ ui,
                                                        const T *__restrict__ b, const T *__restrict__ ib,
                                                        const int *__restrict__ htab, const int *__restrict__ coord,
@@ -1753,8 +1788,8 @@ __global__ __launch_bounds__(512) void k_gs_sweep_colz(T *__restrict__ uo, const
     if (tid < CZ::NB * HT::N) tv = htab[(size_t)blk0 * HT::N + tid];
     BP<T> bpl{};
     if (tid < CZ::NB) bpl = bparams(a, blk0 + tid);
-    if (w & 1) CZ::template run<1>(sm, uo, ui, b, ib, coord, a, blk0, w, lane, tv, bpl);
-    else CZ::template run<0>(sm, uo, ui, b, ib, coord, a, blk0, w, lane, tv, bpl);
+    if (w & 1) CZ::template run<1, VAR>(sm, uo, ui, b, ib, coord, a, blk0, w, lane, tv, bpl);
+    else CZ::template run<0, VAR>(sm, uo, ui, b, ib, coord, a, blk0, w, lane, tv, bpl);
 }
 
 // ---- tiled residual kernels ----
@@ -2750,28 +2785,42 @@ __global__ __launch_bounds__(256) void k_axis_gemm_small(const double *__restric
 }
 
 // k_axis_gemm for n <= 64 with one wave per work group and no LDS; each
-// output is the k-ordered sum of products of k_axis_gemm (same bits).
+// output is the k-ordered sum of products of k_axis_gemm (same bits). Every
+// operand is loaded before the first product, so a launch waits for one
+// memory round trip instead of one per few k; the operand that is the same
+// for every output of the wave is loaded across the lanes (lane k holds entry
+// k) and read back lane by lane.
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+
 // Axis with stride 1 (a column is n contiguous values): lane i computes row i
-// of RB consecutive columns. Row k of At (= column k of A) is a coalesced load;
-// X[col][k] is the same for every lane (a scalar load).
+// of RB consecutive columns. Row k of At (= column k of A) is a coalesced
+// load; lane k holds X[col][k].
 template <int RB>
 __global__ __launch_bounds__(64) void k_axis_gemm_c(const double *__restrict__ At, int n, int ncol,
                                                     const double *__restrict__ in, double *__restrict__ out) {
     const int i = threadIdx.x;
     const int c0 = blockIdx.x * RB;
-    if (i >= n) return;
-    double acc[RB];
+    double av[64], xv[RB], acc[RB];
 #pragma unroll
-    for (int r = 0; r < RB; r++) acc[r] = 0.0;
-#pragma unroll 8
-    for (int k = 0; k < n; k++) {
-        const double a = At[(size_t)k * n + i];
+    for (int k = 0; k < 64; k++) av[k] = (k < n && i < n) ? At[(size_t)k * n + i] : 0.0;
 #pragma unroll
-        for (int r = 0; r < RB; r++) {
-            const int col = min(c0 + r, ncol - 1);
-            acc[r] += a * in[(size_t)col * n + k];
+    for (int r = 0; r < RB; r++) {
+        const int col = min(c0 + r, ncol - 1);
+        xv[r] = i < n ? in[(size_t)col * n + i] : 0.0;
+        acc[r] = 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 64; k++) {
+        if (k < n) {
+#pragma unroll
+            for (int r = 0; r < RB; r++) acc[r] += av[k] * readlane_f64(xv[r], k);
         }
     }
+    if (i >= n) return;
 #pragma unroll
     for (int r = 0; r < RB; r++)
         if (c0 + r < ncol) out[(size_t)(c0 + r) * n + i] = acc[r];
@@ -2779,25 +2828,29 @@ __global__ __launch_bounds__(64) void k_axis_gemm_c(const double *__restrict__ A
 
 // Axis with a stride (inner) that is a multiple of 64: lane j is one of 64
 // consecutive columns (the same outer index), and the work group computes RB
-// rows of them. X[o][k][j] is a coalesced load; A[i][k] is the same for every
-// lane (a scalar load).
+// rows of them. X[o][k][j] is a coalesced load; lane k holds A[i][k].
 template <int RB>
 __global__ __launch_bounds__(64) void k_axis_gemm_r(const double *__restrict__ A, int n, int inner,
                                                     const double *__restrict__ in, double *__restrict__ out) {
-    const int col = blockIdx.x * 64 + threadIdx.x;
+    const int lane = threadIdx.x;
+    const int col = blockIdx.x * 64 + lane;
     const int i0 = blockIdx.y * RB;
     const size_t o = (size_t)(col / inner), jj = (size_t)(col % inner);
     const double *__restrict__ x = in + o * (size_t)n * inner + jj;
-    double acc[RB];
+    double xv[64], av[RB], acc[RB];
 #pragma unroll
-    for (int r = 0; r < RB; r++) acc[r] = 0.0;
-#pragma unroll 8
-    for (int k = 0; k < n; k++) {
-        const double xv = x[(size_t)k * inner];
+    for (int k = 0; k < 64; k++) xv[k] = k < n ? x[(size_t)k * inner] : 0.0;
 #pragma unroll
-        for (int r = 0; r < RB; r++) {
-            const int i = min(i0 + r, n - 1);
-            acc[r] += A[(size_t)i * n + k] * xv;
+    for (int r = 0; r < RB; r++) {
+        const int i = min(i0 + r, n - 1);
+        av[r] = lane < n ? A[(size_t)i * n + lane] : 0.0;
+        acc[r] = 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 64; k++) {
+        if (k < n) {
+#pragma unroll
+            for (int r = 0; r < RB; r++) acc[r] += readlane_f64(av[r], k) * xv[k];
         }
     }
     double *__restrict__ y = out + o * (size_t)n * inner + jj;

@@ -86,7 +86,8 @@ struct CoarseGroup {
 struct Graph {
     hipGraph_t g = nullptr;
     hipGraphExec_t x = nullptr;
-    uint64_t gen = 0;  // srmg_ctx::graph_gen when captured
+    uint64_t gen = 0;     // srmg_ctx::graph_gen when captured
+    bool ran = false;     // the step has run eagerly once (lazily built state exists)
     ~Graph() {
         if (x) (void)hipGraphExecDestroy(x);
         if (g) (void)hipGraphDestroy(g);
@@ -848,7 +849,7 @@ struct Ops {
     static constexpr int BPW0 = SMALL ? 256 / LY::H : 1;
     // sibling-group tiles (k_gs_sweep_group) where a group's tile fits in LDS
     static constexpr bool GROUP_OK = (D == 3 && M == 8) || (D == 2 && M == 16);
-    static constexpr int NCFG = GROUP_OK ? 9 : 4;
+    static constexpr int NCFG = GROUP_OK ? 12 : 4;
     bool grouped(int l) const { return c->group_tiles && c->lv[l].grouped; }
     template <int K> static constexpr int cfg_bpw() { return SMALL ? BPW0 * (K == 0 ? 1 : (K == 3 ? 4 : 2)) : 1; }
     template <int K> static constexpr int cfg_nt() {
@@ -864,7 +865,7 @@ struct Ops {
                                I(u), c->d_halo.as<int>(), coord(), a);
         });
     }
-    template <int NTG, int COL = 0>
+    template <int NTG, int COL = 0, int VAR = 0>
     void sweep_group_launch(T *out, T *in, FieldBuf &u, FieldBuf &b, const LevelArgs<T> &a, int l) {
         if constexpr (GROUP_OK && (((1 << D) * LY::H) % NTG == 0)) {
             unsigned g = (unsigned)(a.nblk >> D);
@@ -876,8 +877,8 @@ struct Ops {
                     hipLaunchKernelGGL((k_gs_sweep_col<D, M, T, RD>), dim3(g), dim3(256), 0, c->stream, out, in,
                                        X(b, l), I(u), c->d_halo.as<int>(), coord(), a);
                 else if constexpr (COL == 2 && D == 3 && M == 8 && RD < 0)
-                    hipLaunchKernelGGL((k_gs_sweep_colz<D, M, T, RD>), dim3(g), dim3(512), 0, c->stream, out, in,
-                                       X(b, l), I(u), c->d_halo.as<int>(), coord(), a);
+                    hipLaunchKernelGGL((k_gs_sweep_colz<D, M, T, RD, VAR>), dim3(g), dim3(512), 0, c->stream, out,
+                                       in, X(b, l), I(u), c->d_halo.as<int>(), coord(), a);
                 else
                     hipLaunchKernelGGL((k_gs_sweep_group<D, M, T, NTG, RD>), dim3(g), dim3(NTG), 0, c->stream, out,
                                        in, X(b, l), I(u), c->d_halo.as<int>(), coord(), a);
@@ -888,8 +889,9 @@ struct Ops {
     }
     // tile shapes 0..3: k_gs_sweep (cfg_bpw / cfg_nt); 4, 5, 6: sibling groups
     // on 256 / 512 / 1024 threads; 7: sibling groups, one thread per column;
-    // 8: sibling groups, two threads per column (levels that are not grouped, and group sizes the thread count does not
-    // divide, use shape 0)
+    // 8: sibling groups, two threads per column; 9, 10, 11: shape 8 with nontemporal
+    // stores, nontemporal right-hand-side loads, both (levels that are not grouped, and group sizes the thread
+    // count does not divide, use shape 0)
     void sweep(FieldBuf &u, FieldBuf &b, const LevelArgs<T> &a, int l, int cfg = -1) {
         if (!a.nblk) return;
         if (!u.alt.p || u.alt.bytes != u.data.bytes) u.alt.alloc(u.data.bytes);
@@ -906,6 +908,9 @@ struct Ops {
             case 6: sweep_group_launch<1024>(out, in, u, b, a, l); break;
             case 7: sweep_group_launch<256, 1>(out, in, u, b, a, l); break;
             case 8: sweep_group_launch<512, 2>(out, in, u, b, a, l); break;
+            case 9: sweep_group_launch<512, 2, 1>(out, in, u, b, a, l); break;
+            case 10: sweep_group_launch<512, 2, 2>(out, in, u, b, a, l); break;
+            case 11: sweep_group_launch<512, 2, 3>(out, in, u, b, a, l); break;
             default: sweep_launch<0>(out, in, u, b, a, l); break;
         }
         u.cur[l] ^= 1;
@@ -1913,6 +1918,7 @@ static void check_opts(const srmg_solve_opts *o) {
         fail("bad solve options");
     if (o->guess < SRMG_GUESS_NONE || o->guess > SRMG_GUESS_ZERO) fail("bad first-iterate mode");
     if (o->guess_smooth < 0) fail("bad first-iterate smoothing count");
+    if (o->first_fmg != 0 && o->first_fmg != 1) fail("first_fmg must be 0 or 1");
 }
 
 // relative residuals; true when every tree is at or below tol
@@ -1961,14 +1967,20 @@ static void solve_plain(srmg_ctx *c, int ui, int bi, int ri, int u1i, const srmg
         }
         int k = 0;
         Graph &gr = solve_graph(c, 0, {&u, &b, &r, &u1}, o);
+        Graph &gf = solve_graph(c, 2, {&u, &b, &r, &u1}, o);
         while (k < o->max_cycles && !done) {
             // a first cycle without a cached graph runs eagerly, so that lazily
             // built state (coarse-solve matrices, ping-pong buffers) exists
-            // before a graph is captured
-            run_maybe_graph(c, gr, o->use_graph && (k > 0 || gr.x), [&] {
-                ops.vcycle(u, b, r, u1, s, op, 0, o->ndown, o->ntop, o->nup, o->omega);
+            // before a graph is captured; the fmg! step (first_fmg: the first
+            // cycle) is captured on its second use
+            const bool f = o->first_fmg && k == 0;
+            Graph &g = f ? gf : gr;
+            run_maybe_graph(c, g, o->use_graph && (f ? (g.ran || g.x) : (k > 0 || g.x)), [&] {
+                if (f) ops.fmg(u, b, r, u1, s, op, o->ndown, o->ntop, o->nup, o->omega);
+                else ops.vcycle(u, b, r, u1, s, op, 0, o->ndown, o->ntop, o->nup, o->omega);
                 ops.resnorm_partials(u, b, s, op);
             });
+            g.ran = true;
             k++;
             done = record(read_norms(c), r0, o->tol, k, o->max_cycles, hist);
         }
@@ -2006,11 +2018,14 @@ static void solve_mixed_dm(srmg_ctx *c, FieldBuf &u, FieldBuf &b, FieldBuf &e, F
     const size_t ncell = (size_t)c->nb * LY::V, nib = e.ib.bytes / sizeof(float);
     if (fused)
         hipLaunchKernelGGL(k_fill<float>, dim3(nwg(ncell)), dim3(WG), 0, c->stream, e.data.as<float>(), 0.0f, ncell);
-    auto correct = [&] {
+    // one outer step; first: fmg! on the correction equation before its
+    // V-cycles (first_fmg)
+    auto correct = [&](bool first) {
         if (!fused)
             hipLaunchKernelGGL(k_fill<float>, dim3(nwg(ncell)), dim3(WG), 0, c->stream, e.data.as<float>(), 0.0f,
                                ncell);
         hipLaunchKernelGGL(k_fill<float>, dim3(nwg(nib)), dim3(WG), 0, c->stream, e.ib.as<float>(), 0.0f, nib);
+        if (first) o32.fmg(e, be, re, ue, s, op, o->ndown, o->ntop, o->nup, o->omega);
         for (int k = 0; k < std::max(1, (int)o->inner_cycles); k++)
             o32.vcycle(e, be, re, ue, s, op, 0, o->ndown, o->ntop, o->nup, o->omega);
         if constexpr (Ops<D, M, double>::GROUP_OK) {
@@ -2050,8 +2065,12 @@ static void solve_mixed_dm(srmg_ctx *c, FieldBuf &u, FieldBuf &b, FieldBuf &e, F
     }
     int k = 0;
     Graph &gr = solve_graph(c, 1, {&u, &b, &e, &be, &re, &ue}, o);
+    Graph &gf = solve_graph(c, 3, {&u, &b, &e, &be, &re, &ue}, o);
     while (k < o->max_cycles && !done) {
-        run_maybe_graph(c, gr, o->use_graph && (k > 0 || gr.x), correct);
+        const bool f = o->first_fmg && k == 0;
+        Graph &g = f ? gf : gr;
+        run_maybe_graph(c, g, o->use_graph && (f ? (g.ran || g.x) : (k > 0 || g.x)), [&] { correct(f); });
+        g.ran = true;
         k++;
         done = record(read_norms(c), r0, o->tol, k, o->max_cycles, hist);
     }
@@ -2824,6 +2843,7 @@ static void solve4(srmg_ctx *c, int ui, int bi, int ri, int u1i, double s, const
     if (op->coarse != SRMG_COARSE_GS) fail("the order-4 operator uses the reference's level-1 sweeps");
     if (o->guess != SRMG_GUESS_NONE && o->guess != SRMG_GUESS_ZERO)
         fail("the order-4 solve starts from u or from zero");
+    if (o->first_fmg) fail("first_fmg is for the order-2 operators");
     if (same_type(c, {ui, bi, ri, u1i}) != SRMG_F64) fail("the order-4 operator runs in FP64");
     box_table(c);
     FieldBuf &u = field(c, ui), &b = field(c, bi), &r = field(c, ri);
@@ -2897,6 +2917,7 @@ int srmg_solve_mixed(srmg_ctx *c, int u, int b, int e, int be, int re, int ue, d
             if (op->coarse != SRMG_COARSE_GS) fail("the order-4 operator uses the reference's level-1 sweeps");
             if (opts->guess != SRMG_GUESS_NONE && opts->guess != SRMG_GUESS_ZERO)
                 fail("the order-4 solve starts from u or from zero");
+            if (opts->first_fmg) fail("first_fmg is for the order-2 operators");
             if (same_type(c, {u, b}) != SRMG_F64) fail("u and b must be FP64");
             if (same_type(c, {e, be, re, ue}) != SRMG_F32) fail("correction fields must be FP32");
             box_table(c);
@@ -2965,6 +2986,12 @@ int srmg_time_kernel(srmg_ctx *c, int kind, int level, int u, int b, int r, doub
                     size_t n = (size_t)c->lv[l].nblk * c->V, off = (size_t)c->lv[l].blk0 * c->V;
                     hipLaunchKernelGGL((k_triad<T>), dim3(nwg(n)), dim3(WG), 0, c->stream, o.X(field(c, r), l) + off,
                                        o.X(field(c, u), l) + off, o.X(field(c, b), l) + off, n);
+                } else if (kind == SRMG_KERNEL_STREAM16) {
+                    size_t n = (size_t)c->lv[l].nblk * c->V, off = (size_t)c->lv[l].blk0 * c->V;
+                    const size_t K = 16 / sizeof(T);
+                    hipLaunchKernelGGL((k_triad16<T>), dim3(nwg((int64_t)(n / K))), dim3(WG), 0, c->stream,
+                                       o.X(field(c, r), l) + off, o.X(field(c, u), l) + off,
+                                       o.X(field(c, b), l) + off, n);
                 } else fail("unknown kernel kind");
             };
             launch(0);
@@ -3034,7 +3061,7 @@ int srmg_set_sweep_config(srmg_ctx *c, int dtype, int level, int cfg) {
         check_tree(c);
         if (level < 1 || level > c->nlevels) fail("level out of range");
         if (dtype != SRMG_F64 && dtype != SRMG_F32) fail("bad dtype");
-        if (cfg < 0 || cfg > 8) fail("sweep configuration must be 0..8");
+        if (cfg < 0 || cfg > 11) fail("sweep configuration must be 0..11");
         c->sweep_cfg[(dtype == SRMG_F64 ? 0 : c->nlevels) + level - 1] = cfg;
     });
 }

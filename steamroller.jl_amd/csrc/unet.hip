@@ -43,6 +43,8 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int TPB = 256;
+// convolutions over fewer voxels split their K loop over the waves of a work group
+constexpr int64_t SPLITK_NVOX = 65536;
 constexpr int MAXLEV = 6;
 constexpr int FEAT = 8;  // input channels
 constexpr int LOSS_WGS = 1024;
@@ -85,14 +87,21 @@ __device__ inline uint4 pack8(const float *o) {
 // mirrored, transposed weights; `mask` then applies the ReLU derivative of
 // the layer that produced the gradient's target (out *= mask > 0).
 // OUT 0: BF16 output with cout_s stored channels; OUT 1: FP32, one channel.
-template <int D, int R, int OUT>
+// SK > 1 (grids with few voxels, where a launch of one wave per 64 voxels
+// leaves the chip idle and each wave walks the whole K loop): the SK waves of
+// a work group share one group of 64 voxels and take every SK-th 32-wide K
+// step; waves 1..SK-1 hand their accumulators to wave 0 through LDS, which
+// adds them in wave order before the epilogue.
+template <int D, int R, int OUT, int SK = 1>
 __global__ __launch_bounds__(TPB) void k_conv(const uint16_t *__restrict__ in, int cin_log2,
                                               const uint16_t *__restrict__ w, int kp, int ktot,
                                               const float *__restrict__ bias, int nbias, int relu,
                                               const uint16_t *__restrict__ mask, void *__restrict__ out, int cout_s,
                                               int n0, int n1, int n2, int64_t nvox) {
+    static_assert(SK == 1 || SK == TPB / 64, "split K over the waves of a work group");
     const int lane = threadIdx.x & 63, col = lane & 15, g = lane >> 4;
-    const int64_t vbase = ((int64_t)blockIdx.x * (TPB / 64) + (threadIdx.x >> 6)) * 64;
+    const int wv = threadIdx.x >> 6;
+    const int64_t vbase = (SK > 1 ? (int64_t)blockIdx.x : (int64_t)blockIdx.x * (TPB / 64) + wv) * 64;
     const int64_t V = (int64_t)n0 * n1 * n2;
     int px[4], py[4], pz[4];
     int64_t pn[4];
@@ -113,7 +122,7 @@ __global__ __launch_bounds__(TPB) void k_conv(const uint16_t *__restrict__ in, i
 #pragma unroll
         for (int q = 0; q < 4; q++) acc[r][q] = f32x4{0.f, 0.f, 0.f, 0.f};
     const int cin = 1 << cin_log2;
-    for (int kc = 0; kc < kp; kc += 32) {
+    for (int kc = SK > 1 ? 32 * wv : 0; kc < kp; kc += 32 * SK) {
         const int kk = kc + 8 * g;
         bf16x8 a[R];
 #pragma unroll
@@ -135,6 +144,23 @@ __global__ __launch_bounds__(TPB) void k_conv(const uint16_t *__restrict__ in, i
         for (int r = 0; r < R; r++)
 #pragma unroll
             for (int q = 0; q < 4; q++) acc[r][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[r], b[q], acc[r][q], 0, 0, 0);
+    }
+    if constexpr (SK > 1) {
+        __shared__ f32x4 red[SK - 1][R][4][64];
+        if (wv > 0) {
+#pragma unroll
+            for (int r = 0; r < R; r++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) red[wv - 1][r][q][lane] = acc[r][q];
+        }
+        __syncthreads();
+        if (wv > 0) return;
+#pragma unroll
+        for (int k = 0; k < SK - 1; k++)
+#pragma unroll
+            for (int r = 0; r < R; r++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) acc[r][q] += red[k][r][q][lane];
     }
     // lane holds D[4 g + j][col]: output channels 16 r + 4 g + j of voxel col
 #pragma unroll
@@ -358,19 +384,38 @@ __global__ __launch_bounds__(TPB) void k_pool_bwd(const uint16_t *__restrict__ d
 // ---------------------------------------------------------------------------
 
 // scale[n] = max |f[n]| (1 for an all-zero grid); one work group per grid
-template <typename TF>
-__global__ __launch_bounds__(TPB) void k_absmax(const TF *__restrict__ f, int64_t V, double *__restrict__ scale) {
+// maximum over the work group (all threads get it)
+__device__ __forceinline__ double wg_max(double m) {
     __shared__ double red[TPB];
-    const TF *p = f + (int64_t)blockIdx.x * V;
-    double m = 0.0;
-    for (int64_t i = threadIdx.x; i < V; i += TPB) m = fmax(m, fabs((double)p[i]));
     red[threadIdx.x] = m;
     __syncthreads();
     for (int s = TPB / 2; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) scale[blockIdx.x] = red[0] > 0.0 ? red[0] : 1.0;
+    return red[0];
+}
+
+// max |f| per grid in two stages: ABSMAX_PARTS work groups per grid reduce
+// contiguous chunks, then one work group per grid reduces the chunk maxima
+// (a maximum does not depend on the order: the same value as one pass)
+constexpr int ABSMAX_PARTS = 64;
+template <typename TF>
+__global__ __launch_bounds__(TPB) void k_absmax_part(const TF *__restrict__ f, int64_t V,
+                                                     double *__restrict__ part) {
+    const int64_t chunk = (V + ABSMAX_PARTS - 1) / ABSMAX_PARTS;
+    const TF *p = f + (int64_t)blockIdx.y * V;
+    const int64_t i0 = (int64_t)blockIdx.x * chunk, i1 = min(V, i0 + chunk);
+    double m = 0.0;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += TPB) m = fmax(m, fabs((double)p[i]));
+    m = wg_max(m);
+    if (threadIdx.x == 0) part[(int64_t)blockIdx.y * ABSMAX_PARTS + blockIdx.x] = m;
+}
+
+__global__ __launch_bounds__(TPB) void k_absmax(const double *__restrict__ part, double *__restrict__ scale) {
+    const double m = wg_max(threadIdx.x < ABSMAX_PARTS ? part[(int64_t)blockIdx.x * ABSMAX_PARTS + threadIdx.x]
+                                                       : 0.0);
+    if (threadIdx.x == 0) scale[blockIdx.x] = m > 0.0 ? m : 1.0;
 }
 
 template <int D, typename TF>
@@ -515,7 +560,7 @@ struct srmg_unet {
     int cap = 0;
     Tensor X;
     std::vector<Tensor> A, E, P, C, S, Dd;
-    DevBuf Y, gY, scale, lossp, loss, partial;
+    DevBuf Y, gY, scale, absmax_part, lossp, loss, partial;
     bool packed = false;
 };
 
@@ -552,6 +597,7 @@ void ensure_batch(srmg_unet *n, int B) {
     n->Y.alloc((size_t)nv(0) * sizeof(float));
     n->gY.alloc((size_t)nv(0) * 8 * 2);
     n->scale.alloc((size_t)B * sizeof(double));
+    n->absmax_part.alloc((size_t)B * ABSMAX_PARTS * sizeof(double));
     n->lossp.alloc(LOSS_WGS * sizeof(double));
     n->loss.alloc(sizeof(double));
     n->cap = B;
@@ -572,9 +618,18 @@ template <int D, int OUT>
 void conv_launch_r(int R, unsigned grid, hipStream_t st, const uint16_t *in, int cin_log2, const uint16_t *w, int kp,
                    int ktot, const float *bias, int nbias, int relu, const uint16_t *mask, void *out, int cout_s,
                    const Ext &e, int64_t nvox) {
+    // few voxels: split K over the waves of a work group (one group of 64 voxels per work group)
+    const bool split = nvox < SPLITK_NVOX;
+    const unsigned gsk = (unsigned)((nvox + 63) / 64);
 #define SRMG_CONV(RR)                                                                                              \
-    hipLaunchKernelGGL((k_conv<D, RR, OUT>), dim3(grid), dim3(TPB), 0, st, in, cin_log2, w, kp, ktot, bias, nbias, \
-                       relu, mask, out, cout_s, e.n0, e.n1, e.n2, nvox)
+    do {                                                                                                           \
+        if (split)                                                                                                 \
+            hipLaunchKernelGGL((k_conv<D, RR, OUT, TPB / 64>), dim3(gsk), dim3(TPB), 0, st, in, cin_log2, w, kp,   \
+                               ktot, bias, nbias, relu, mask, out, cout_s, e.n0, e.n1, e.n2, nvox);                \
+        else                                                                                                       \
+            hipLaunchKernelGGL((k_conv<D, RR, OUT>), dim3(grid), dim3(TPB), 0, st, in, cin_log2, w, kp, ktot,      \
+                               bias, nbias, relu, mask, out, cout_s, e.n0, e.n1, e.n2, nvox);                      \
+    } while (0)
     if (R == 1)
         SRMG_CONV(1);
     else if (R == 2)
@@ -697,7 +752,9 @@ void backward(srmg_unet *n, hipStream_t st, int B) {
 
 template <typename TF>
 void features(srmg_unet *n, hipStream_t st, const TF *f, int B) {
-    hipLaunchKernelGGL((k_absmax<TF>), dim3(B), dim3(TPB), 0, st, f, n->V[0], n->scale.as<double>());
+    hipLaunchKernelGGL((k_absmax_part<TF>), dim3(ABSMAX_PARTS, B), dim3(TPB), 0, st, f, n->V[0],
+                       n->absmax_part.as<double>());
+    hipLaunchKernelGGL(k_absmax, dim3(B), dim3(TPB), 0, st, n->absmax_part.as<double>(), n->scale.as<double>());
     const int64_t nvox = (int64_t)B * n->V[0];
     if (n->D == 3)
         hipLaunchKernelGGL((k_features<3, TF>), dim3(nblocks(nvox)), dim3(TPB), 0, st, f, n->scale.as<double>(),

@@ -50,6 +50,22 @@ def cfg4_grid_problem() -> GridProblem:
                        boundaryconditions(((1, 1), (1, 1), (-1, -1))), h * h * ELEMENTARY_CHARGE / EPSILON_0)
 
 
+def cfg2_grid_problem() -> GridProblem:
+    """Level 1 of the 2D cylindrical streamer workload: 128 x 256 cells on
+    r in [0, 1 cm], z in [0, 2 cm] (CylindricalGeometry{1}, r first), BC
+    ((1,-1),(-1,-1)): Neumann on the axis, Dirichlet at the outer radius and
+    the electrodes."""
+    from .configs import ELEMENTARY_CHARGE, EPSILON_0
+    from .tree import boundaryconditions
+    h = 1e-2 / 128
+    return GridProblem(2, (128, 256), (1e-2, 2e-2), solver.CylindricalGeometry(1),
+                       boundaryconditions(((1, -1), (-1, -1))), h * h * ELEMENTARY_CHARGE / EPSILON_0)
+
+
+def grid_problem(config: str) -> GridProblem:
+    return {"cfg2": cfg2_grid_problem, "cfg4": cfg4_grid_problem}[config]()
+
+
 def synth_charges(gp: GridProblem, seeds: Sequence[int], device="cuda"):
     """Charge densities [len(seeds), V] (float64, x fastest) on the device;
     sample k is fixed by seeds[k]."""
@@ -62,19 +78,28 @@ def synth_charges(gp: GridProblem, seeds: Sequence[int], device="cuda"):
     X = [mesh[D - 1 - d].reshape(-1) for d in range(D)]
     out = torch.zeros((len(seeds), X[0].numel()), dtype=torch.float64, device=device)
     Lv = np.array(gp.L[:D])
+    # cylindrical grids: half of the charges sit on the axis (r < 0.05 L_r;
+    # their filaments run along z), as streamer channels and heads do
+    rdim = {1: 0, 2: 1}.get(int(gp.geometry.code), -1)
     for k, seed in enumerate(seeds):
         rng = np.random.default_rng(int(seed))
         acc = torch.zeros_like(X[0])
         for _ in range(int(rng.integers(1, 13))):
             amp = 10 ** rng.uniform(18, 21) * (-1 if rng.uniform() < 0.3 else 1)
+            axial = rdim >= 0 and rng.uniform() < 0.5
             if rng.uniform() < 0.5:
                 c = rng.uniform(0.1, 0.9, D) * Lv
+                if axial:
+                    c[rdim] = rng.uniform(0.0, 0.05) * Lv[rdim]
                 w = rng.uniform(0.2e-3, 1.0e-3)
                 r2 = sum((X[d] - c[d]) ** 2 for d in range(D))
                 acc += amp * torch.exp(-r2 / (w * w))
             else:
                 a = rng.uniform(0.1, 0.9, D) * Lv
                 dirn = rng.normal(size=D)
+                if axial:
+                    a[rdim] = rng.uniform(0.0, 0.05) * Lv[rdim]
+                    dirn[rdim] = 0.1 * dirn[rdim]
                 dirn /= np.linalg.norm(dirn)
                 b = np.clip(a + dirn * rng.uniform(1e-3, 5e-3), 0.02 * Lv, 0.98 * Lv)
                 ab = b - a
@@ -180,5 +205,11 @@ def load(path: str) -> solver.UNet:
     return net
 
 
+def weights_path(config: str) -> str:
+    """The committed network of a workload's level-1 grid (cfg2: 2D 128 x 256
+    cylindrical, cfg4: 3D 64^3 Cartesian)."""
+    return os.path.join(WEIGHTS_DIR, f"unet_{config}_level1.npz")
+
+
 def cfg4_weights_path() -> str:
-    return os.path.join(WEIGHTS_DIR, "unet_cfg4_level1.npz")
+    return weights_path("cfg4")

@@ -33,8 +33,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libsrmg.so")
 
 F64, F32 = 0, 1
-KERNEL_GS_PASS, KERNEL_RESIDUAL, KERNEL_GS_SWEEP, KERNEL_STREAM, KERNEL_RESIDUAL_GATHER = 0, 1, 2, 3, 4
-SWEEP_SHAPES = 9  # tile shapes of the fused smoother (Context.set_sweep_config)
+KERNEL_GS_PASS, KERNEL_RESIDUAL, KERNEL_GS_SWEEP, KERNEL_STREAM, KERNEL_RESIDUAL_GATHER, KERNEL_STREAM16 = 0, 1, 2, 3, 4, 5
+SWEEP_SHAPES = 12  # tile shapes of the fused smoother (Context.set_sweep_config)
 COARSE_GS, COARSE_EXACT = 0, 1
 REGRID_INTERP, REGRID_RESTRICT = 1, 2  # srmg_regrid modes
 GUESS_NONE, GUESS_LEARNED, GUESS_COARSE, GUESS_ZERO = 0, 1, 2, 3
@@ -94,7 +94,7 @@ class _SolveOpts(C.Structure):
     _fields_ = [("ndown", C.c_int32), ("ntop", C.c_int32), ("nup", C.c_int32),
                 ("omega", C.c_double), ("tol", C.c_double), ("max_cycles", C.c_int32),
                 ("use_graph", C.c_int32), ("inner_cycles", C.c_int32), ("guess", C.c_int32),
-                ("guess_smooth", C.c_int32)]
+                ("guess_smooth", C.c_int32), ("first_fmg", C.c_int32)]
 
 
 def make_op(geometry, bc, disc, coarse: int = COARSE_GS, any_order: bool = False) -> _Op:
@@ -542,23 +542,27 @@ def resnorm(r, b, u, s, tree=None, conn=None, geometry=None, bc=None, lpldisc=No
 GUESS_SMOOTH = 4  # level-1 sweeps after a learned first iterate (default)
 
 
-def _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph, inner_cycles=1, guess=0, guess_smooth=GUESS_SMOOTH):
+def _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph, inner_cycles=1, guess=0, guess_smooth=GUESS_SMOOTH,
+          first_fmg=False):
     o = _SolveOpts()
     o.ndown, o.ntop, o.nup = int(ndown), int(ntop), int(nup)
     o.omega, o.tol, o.max_cycles, o.use_graph = float(omega), float(tol), int(max_cycles), int(use_graph)
     o.inner_cycles = int(inner_cycles)
     o.guess = int(guess)
     o.guess_smooth = int(guess_smooth)
+    o.first_fmg = int(bool(first_fmg))
     return o
 
 
 def solve(u, b, r, u1, s, geometry, bc, lpldisc, *, tol=1e-8, max_cycles=50, ndown=2, ntop=4,
-          nup=2, omega=1.0, coarse=COARSE_EXACT, use_graph=True, guess=GUESS_NONE, guess_smooth=GUESS_SMOOTH):
-    """Pre-fill ghosts, then V-cycles until resnorm(u) <= tol * resnorm(0).
-    Returns (cycles, relative residual history). LaplacianDiscretization(D, 4)
-    needs coarse=COARSE_GS (the reference's level-1 sweeps)."""
+          nup=2, omega=1.0, coarse=COARSE_EXACT, use_graph=True, guess=GUESS_NONE, guess_smooth=GUESS_SMOOTH,
+          first_fmg=False):
+    """Pre-fill ghosts, then V-cycles until resnorm(u) <= tol * resnorm(0)
+    (first_fmg: the first cycle is fmg!). Returns (cycles, relative residual
+    history). LaplacianDiscretization(D, 4) needs coarse=COARSE_GS (the
+    reference's level-1 sweeps)."""
     op = make_op(geometry, bc, lpldisc, coarse, any_order=True)
-    o = _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph, 1, guess, guess_smooth)
+    o = _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph, 1, guess, guess_smooth, first_fmg)
     hist = np.zeros(max_cycles + 1)
     n = C.c_int()
     _check(lib().srmg_solve(u.ctx.h, u.id, b.id, r.id, u1.id, float(s), C.byref(op), C.byref(o),
@@ -568,12 +572,14 @@ def solve(u, b, r, u1, s, geometry, bc, lpldisc, *, tol=1e-8, max_cycles=50, ndo
 
 def solve_mixed(u, b, e, be, re, ue, s, geometry, bc, lpldisc, *, tol=1e-8, max_cycles=50,
                 ndown=2, ntop=4, nup=2, omega=1.0, coarse=COARSE_EXACT, use_graph=True, inner_cycles=1,
-                guess=GUESS_NONE, guess_smooth=GUESS_SMOOTH):
-    """FP64 defect correction around FP32 V-cycles (u, b FP64; e, be, re, ue FP32).
-    Returns (outer steps, relative residual history per outer step).
+                guess=GUESS_NONE, guess_smooth=GUESS_SMOOTH,
+          first_fmg=False):
+    """FP64 defect correction around FP32 V-cycles (u, b FP64; e, be, re, ue FP32;
+    first_fmg: the first correction runs fmg! on e before its V-cycles). Returns (outer
+    steps, relative residual history per outer step).
     LaplacianDiscretization(D, 4) needs coarse=COARSE_GS."""
     op = make_op(geometry, bc, lpldisc, coarse, any_order=True)
-    o = _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph, inner_cycles, guess, guess_smooth)
+    o = _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph, inner_cycles, guess, guess_smooth, first_fmg)
     hist = np.zeros(max_cycles + 1)
     n = C.c_int()
     _check(lib().srmg_solve_mixed(u.ctx.h, u.id, b.id, e.id, be.id, re.id, ue.id, float(s),
@@ -623,11 +629,12 @@ def resnorm_batched(r, b, u, s, geometry, bc, discs, *, coarse=COARSE_GS):
 
 
 def solve_batched(u, b, r, u1, s, geometry, bc, discs, *, tol=1e-8, max_cycles=50, ndown=2, ntop=4, nup=2,
-                  omega=1.0, coarse=COARSE_EXACT, use_graph=True, guess=GUESS_NONE, guess_smooth=GUESS_SMOOTH):
+                  omega=1.0, coarse=COARSE_EXACT, use_graph=True, guess=GUESS_NONE, guess_smooth=GUESS_SMOOTH,
+          first_fmg=False):
     """Solve every tree of a forest to tol; returns (cycles, history[ntrees, cycles + 1])."""
     sv = np.ascontiguousarray(np.asarray(s, np.float64))
     ops = _ops(geometry, bc, discs, coarse)
-    o = _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph, 1, guess, guess_smooth)
+    o = _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph, 1, guess, guess_smooth, first_fmg)
     hist = np.zeros((len(discs), max_cycles + 1))
     n = C.c_int()
     _check(lib().srmg_solve_batched(u.ctx.h, u.id, b.id, r.id, u1.id, _p(sv), ops, C.byref(o), C.byref(n),
@@ -637,11 +644,12 @@ def solve_batched(u, b, r, u1, s, geometry, bc, discs, *, tol=1e-8, max_cycles=5
 
 def solve_mixed_batched(u, b, e, be, re, ue, s, geometry, bc, discs, *, tol=1e-8, max_cycles=50, ndown=2,
                         ntop=4, nup=2, omega=1.0, coarse=COARSE_EXACT, use_graph=True, inner_cycles=1,
-                        guess=GUESS_NONE, guess_smooth=GUESS_SMOOTH):
+                        guess=GUESS_NONE, guess_smooth=GUESS_SMOOTH,
+          first_fmg=False):
     """FP64 defect correction around FP32 V-cycles on every tree of a forest."""
     sv = np.ascontiguousarray(np.asarray(s, np.float64))
     ops = _ops(geometry, bc, discs, coarse)
-    o = _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph, inner_cycles, guess, guess_smooth)
+    o = _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph, inner_cycles, guess, guess_smooth, first_fmg)
     hist = np.zeros((len(discs), max_cycles + 1))
     n = C.c_int()
     _check(lib().srmg_solve_mixed_batched(u.ctx.h, u.id, b.id, e.id, be.id, re.id, ue.id, _p(sv), ops,

@@ -1,11 +1,13 @@
-"""Train the learned first iterate for the 3D Cartesian workload's level-1 grid
-(64^3, BC ((1,1),(1,1),(-1,-1))) and evaluate it.
+"""Train the learned first iterate for a workload's level-1 grid and evaluate
+it: cfg4, the 3D Cartesian workload (64^3, BC ((1,1),(1,1),(-1,-1))), or cfg2,
+the 2D cylindrical streamer workload (128 x 256, BC ((1,-1),(-1,-1))).
 
-    python steamroller.jl_amd/train_guess.py --samples 20000 --epochs 2 --out weights.npz
+    python steamroller.jl_amd/train_guess.py --config cfg4 --samples 20000 --epochs 4 --out weights.npz
+    python steamroller.jl_amd/train_guess.py --config cfg2 --samples 20000 --epochs 16 --out weights.npz
 
 Prints the dataset and training progress, the held-out prediction error, and
-cycles to 1e-8 on a 3-level tree of the workload with the zero, exact-coarse
-and learned first iterates.
+cycles to 1e-8 on a tree of the workload (cfg4: a 3-level variant; cfg2: the
+full tree) with the zero, exact-coarse and learned first iterates.
 """
 
 import argparse
@@ -19,12 +21,15 @@ sys.path.insert(0, os.path.dirname(HERE))
 
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--config", default="cfg4", choices=["cfg2", "cfg4"])
     ap.add_argument("--samples", type=int, default=20000)
     ap.add_argument("--eval-samples", type=int, default=256)
     ap.add_argument("--epochs", type=int, default=2)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--lr", type=float, default=1e-3)
-    ap.add_argument("--widths", type=str, default="8,16,32,64")
+    ap.add_argument("--widths", type=str, default="",
+                    help="channels per level (default: cfg4 8,16,32,64; cfg2 8,16,32,64,64, five levels so that "
+                         "the 16x8 coarsest grid sees the whole 128x256 level)")
     ap.add_argument("--out", type=str, default="")
     ap.add_argument("--init", type=str, default="", help="start from these weights")
     ap.add_argument("--no-solve-eval", action="store_true")
@@ -38,20 +43,21 @@ def main():
     from srmg import configs, guess, solver
 
     log = lambda m: print(m, flush=True)  # noqa: E731
-    gp = guess.cfg4_grid_problem()
-    small = configs.uniform_problem(3, (1, 1, 1), 8, 1, 0, ((1, 1), (1, 1), (-1, -1)))
+    gp = guess.grid_problem(a.config)
+    # any context serves the grid solves (they take their own operator)
+    small = configs.uniform_problem(gp.D, (1,) * gp.D, 8, 1, 0, tuple((1, 1) for _ in range(gp.D)))
     ctx = solver.Context(small.tree, small.conn, small.M)
     t0 = time.time()
     F, U = (None, None) if a.eval_only else guess.make_dataset(ctx, gp, a.samples, seed0=0, log=log)
     Fe, Ue = guess.make_dataset(ctx, gp, a.eval_samples, seed0=10_000_000, log=log)
     log(f"[dataset] {0 if a.eval_only else a.samples} + {a.eval_samples} samples in {time.time() - t0:.1f} s")
-    widths = tuple(int(x) for x in a.widths.split(","))
-    net = solver.UNet(3, gp.n, len(widths), widths, seed=0)
+    widths = tuple(int(x) for x in (a.widths or {"cfg2": "8,16,32,64,64", "cfg4": "8,16,32,64"}[a.config]).split(","))
+    net = solver.UNet(gp.D, gp.n, len(widths), widths, seed=0)
     if a.init:
         net = guess.load(a.init)
     else:
         net.output_scale = guess.output_scale(F, U)
-        net.set_problem(gp.geometry, gp.bc, solver.LaplacianDiscretization(3, 2))
+        net.set_problem(gp.geometry, gp.bc, solver.LaplacianDiscretization(gp.D, 2))
     log(f"[net] {net.nparams} parameters, output scale {net.output_scale:.4e}")
     e0 = guess.evaluate(net, Fe, Ue)
     log(f"[eval] before training: median relative error {np.median(e0):.4f}")
@@ -74,7 +80,8 @@ def main():
     t2 = time.time()
     for _ in range(20):
         net.forward(Fe[:1], fb, 1)
-    log(f"[infer] one 64^3 grid: {1e3 * (time.time() - t2) / 20:.3f} ms per forward (host-timed, synchronous)")
+    log(f"[infer] one {'x'.join(map(str, gp.n))} grid: {1e3 * (time.time() - t2) / 20:.3f} ms per forward "
+        "(host-timed, synchronous)")
     if a.out and not a.eval_only:
         guess.save(net, gp, a.out, {"train_samples": a.samples, "epochs": a.epochs,
                                     "eval_median_rel_err": float(np.median(e1)),
@@ -82,9 +89,9 @@ def main():
         log(f"[save] {a.out}")
     if a.no_solve_eval:
         return
-    p = configs.make_cfg4(0, scale=0.35, levels=3)
-    geom = solver.CartesianGeometry()
-    disc = solver.LaplacianDiscretization(3, 2)
+    p = configs.make_cfg4(0, scale=0.35, levels=3) if a.config == "cfg4" else configs.make_cfg2(0)
+    geom = solver.geometry_from_code(p.geom)
+    disc = solver.LaplacianDiscretization(p.D, 2)
     for coarse, maxc in ((solver.COARSE_EXACT, 30), (solver.COARSE_GS, 200)):
         for mode, nsm in ((solver.GUESS_NONE, 0), (solver.GUESS_COARSE, 0), (solver.GUESS_LEARNED, 0),
                           (solver.GUESS_LEARNED, 4), (solver.GUESS_LEARNED, 16)):

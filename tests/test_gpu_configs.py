@@ -2,7 +2,8 @@
 
 * cfg4: the 3D Cartesian AMR Poisson problem (8^3 blocks, 5 levels, about 62k
   blocks, 28M leaf cells). The FP64 solve to 1e-8 (exact level-1 solve) gives
-  the oracle's potential bit for bit and its residual history to 1e-12; every
+  the oracle's potential bit for bit and its residual history to 1e-12, also
+  with fmg! as the first cycle (src/multigrid.jl:17-51); every
   mixed-precision form that the benchmark times stays within 1e-7 of the
   oracle's FP64 potential (src/multigrid.jl:60-106 restated by the oracle).
 * cfg2: the 2D cylindrical streamer-like tree (6 levels, 7.7k blocks, 1.5M
@@ -56,9 +57,10 @@ def _rel(a, ref):
 
 
 @functools.lru_cache(maxsize=None)
-def _oracle_solve(name):
-    """The oracle's FP64 solve from u = 0: exact level-1 solve, V(2, 4, 2),
-    stop at resnorm(u) <= 1e-8 resnorm(0). Returns (cycles, history, u)."""
+def _oracle_solve(name, fmg=False):
+    """The oracle's FP64 solve from u = 0: exact level-1 solve, V(2, 4, 2)
+    (fmg: the first cycle is fmg! with the same options), stop at
+    resnorm(u) <= 1e-8 resnorm(0). Returns (cycles, history, u)."""
     p = _cfg(name)
     t = time.time()
     o = Oracle.for_problem(p)
@@ -67,10 +69,13 @@ def _oracle_solve(name):
     o.fill_ghost_all(u)
     hist = [o.resnorm(r, b, u, p.s) / r0]
     while hist[-1] > TOL and len(hist) < 40:
-        o.vcycle(u, b, r, u1, p.s, ndown=2, ntop=4, nup=2, coarse_exact=True)
+        if fmg and len(hist) == 1:
+            o.fmg(u, b, r, u1, p.s, ndown=2, ntop=4, nup=2, omega=1.0, coarse_exact=True)
+        else:
+            o.vcycle(u, b, r, u1, p.s, ndown=2, ntop=4, nup=2, coarse_exact=True)
         hist.append(o.resnorm(r, b, u, p.s) / r0)
-    print(f"{name}: oracle {len(hist) - 1} cycles to {hist[-1]:.3e} in {time.time() - t:.1f}s "
-          f"({os.environ.get('OMP_NUM_THREADS', 'all')} threads)", flush=True)
+    print(f"{name}: oracle{' (fmg first)' if fmg else ''} {len(hist) - 1} cycles to {hist[-1]:.3e} in "
+          f"{time.time() - t:.1f}s ({os.environ.get('OMP_NUM_THREADS', 'all')} threads)", flush=True)
     return len(hist) - 1, np.asarray(hist), _valid(u, p).copy()
 
 
@@ -80,20 +85,21 @@ def _gpu_forms(p, forms):
     ctx = solver.Context(p.tree, p.conn, p.M)
     u, b, r, u1 = (ctx.field(F64) for _ in range(4))
     e, be, re, ue = (ctx.field(F32) for _ in range(4))
-    if "mixed2_learned" in forms and p.D == 3:
+    if any(f.endswith("learned") for f in forms) and p.D == 3:
         from srmg import guess
         solver.set_guess(ctx, guess.load(guess.cfg4_weights_path()))
     out = {}
     for form in forms:
         b.upload(p.b)
         u.zero()
-        if form == "fp64":
-            n, h = solver.solve(u, b, r, u1, p.s, geom, p.bc, disc, tol=TOL, max_cycles=40)
+        fmg = "_fmg" in form
+        if form.startswith("fp64"):
+            n, h = solver.solve(u, b, r, u1, p.s, geom, p.bc, disc, tol=TOL, max_cycles=40, first_fmg=fmg)
         else:
             inner = int(form[5]) if len(form) > 5 and form[5].isdigit() else 1
             g = solver.GUESS_LEARNED if form.endswith("learned") else solver.GUESS_ZERO
             n, h = solver.solve_mixed(u, b, e, be, re, ue, p.s, geom, p.bc, disc, tol=TOL, max_cycles=40,
-                                      inner_cycles=inner, guess=g)
+                                      inner_cycles=inner, guess=g, first_fmg=fmg)
         host = p.b.like()
         u.download(host)
         out[form] = (n, np.asarray(h).copy(), _valid(host, p).copy())
@@ -109,24 +115,26 @@ def _check_against_oracle(name, forms):
         err = _rel(ug, u_o)
         print(f"{name} {form}: potential max rel diff vs the oracle's FP64 solve {err:.3e}", flush=True)
         assert h[-1] <= TOL
-        if form == "fp64":
-            assert n == n_o
-            np.testing.assert_allclose(h, h_o, rtol=0, atol=1e-12)
-            np.testing.assert_array_equal(ug, u_o)
+        if form.startswith("fp64"):
+            # the same cycle sequence in the oracle: bit for bit
+            n_f, h_f, u_f = _oracle_solve(name, fmg="_fmg" in form)
+            assert n == n_f
+            np.testing.assert_allclose(h, h_f, rtol=0, atol=1e-12)
+            np.testing.assert_array_equal(ug, u_f)
         else:
             assert err <= 1e-7
 
 
 def test_cfg4_full_size_solves_against_oracle():
     from srmg import guess
-    forms = ["fp64", "mixed", "mixed2", "mixed3"]
+    forms = ["fp64", "fp64_fmg", "mixed", "mixed2", "mixed3", "mixed3_fmg"]
     if os.path.exists(guess.cfg4_weights_path()):
-        forms.append("mixed2_learned")
+        forms += ["mixed2_learned", "mixed3_fmg_learned"]
     _check_against_oracle("cfg4", forms)
 
 
 def test_cfg2_full_size_solves_against_oracle():
-    _check_against_oracle("cfg2", ["fp64", "mixed2"])
+    _check_against_oracle("cfg2", ["fp64", "fp64_fmg", "mixed2", "mixed3_fmg"])
 
 
 @pytest.mark.parametrize("coarse", [solver.COARSE_GS, solver.COARSE_EXACT])

@@ -260,3 +260,60 @@ def test_learned_first_iterate_with_the_exact_level1_solve_converges_to_the_same
     G, M = p.b.G, p.M
     v = (slice(G, G + M),) * 3
     assert np.max(np.abs(u1[v] - u0[v])) / np.max(np.abs(u0[v])) < 1e-6
+
+
+# ---- the 2D network of the cylindrical streamer workload (cfg2's level 1) ----
+
+def _trained2d():
+    from srmg import guess
+    path = guess.weights_path("cfg2")
+    if not os.path.exists(path):
+        pytest.skip("no trained 2D weights in the package")
+    return guess.load(path)
+
+
+def test_trained_2d_network_predicts_held_out_level1_solutions():
+    from srmg import guess
+    net = _trained2d()
+    gp = guess.cfg2_grid_problem()
+    assert net.D == 2 and tuple(net.n) == gp.n
+    small = configs.uniform_problem(2, (1, 1), 8, 1, 0, ((1, 1), (1, 1)))
+    ctx = solver.Context(small.tree, small.conn, small.M)
+    F, U = guess.make_dataset(ctx, gp, 64, seed0=10_000_000, log=lambda m: None)
+    err = guess.evaluate(net, F, U)
+    print(f"2D held-out relative error: median {np.median(err):.3f}, 90th percentile {np.percentile(err, 90):.3f}")
+    assert np.median(err) < 0.2
+
+
+@pytest.mark.parametrize("coarse", [solver.COARSE_EXACT, solver.COARSE_GS])
+def test_2d_learned_first_iterate_on_the_streamer_tree(coarse):
+    # the full cfg2 tree: the learned start is accepted (residual below the
+    # zero start's) and converges to the zero start's solution with the exact
+    # level-1 solve; with the reference's level-1 sweeps on the 128 x 256
+    # level 1 its first cycles stay below the zero start's
+    net = _trained2d()
+    p = configs.make_cfg2(0)
+    geom, disc = solver.geometry_from_code(p.geom), solver.LaplacianDiscretization(2, 2)
+    tol, maxc = (1e-8, 40) if coarse == solver.COARSE_EXACT else (1e-30, 10)
+    res = {}
+    for mode in (solver.GUESS_NONE, solver.GUESS_LEARNED):
+        ctx = solver.Context(p.tree, p.conn, p.M)
+        solver.set_guess(ctx, net)
+        u, b, r, u1 = (ctx.field(F64) for _ in range(4))
+        b.upload(p.b)
+        n, h = solver.solve(u, b, r, u1, p.s, geom, p.bc, disc, tol=tol, max_cycles=maxc, coarse=coarse,
+                            guess=mode)
+        out = p.b.like()
+        u.download(out)
+        res[mode] = (n, np.asarray(h).copy(), out.u.copy(), solver.guess_fallbacks(ctx))
+    (n0, h0, u0, _), (n1, h1, u1, fb) = res[solver.GUESS_NONE], res[solver.GUESS_LEARNED]
+    print(f"coarse={coarse}: zero start {n0} cycles {np.array2string(h0, precision=3)}; learned start {n1} "
+          f"cycles {np.array2string(h1, precision=3)}, fallbacks {fb}", flush=True)
+    assert fb == 0 and h1[0] < 1.0
+    if coarse == solver.COARSE_EXACT:
+        assert h1[-1] <= 1e-8 and n1 <= n0
+        G, M = p.b.G, p.M
+        v = (slice(G, G + M),) * 2
+        assert np.max(np.abs(u1[v] - u0[v])) / np.max(np.abs(u0[v])) < 1e-6
+    else:
+        assert np.all(h1[1:4] < h0[1:4])
