@@ -2785,42 +2785,28 @@ __global__ __launch_bounds__(256) void k_axis_gemm_small(const double *__restric
 }
 
 // k_axis_gemm for n <= 64 with one wave per work group and no LDS; each
-// output is the k-ordered sum of products of k_axis_gemm (same bits). Every
-// operand is loaded before the first product, so a launch waits for one
-// memory round trip instead of one per few k; the operand that is the same
-// for every output of the wave is loaded across the lanes (lane k holds entry
-// k) and read back lane by lane.
-__device__ __forceinline__ double readlane_f64(double v, int lane) {
-    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
-    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
-    return __hiloint2double(hi, lo);
-}
-
+// output is the k-ordered sum of products of k_axis_gemm (same bits).
 // Axis with stride 1 (a column is n contiguous values): lane i computes row i
-// of RB consecutive columns. Row k of At (= column k of A) is a coalesced
-// load; lane k holds X[col][k].
+// of RB consecutive columns. Row k of At (= column k of A) is a coalesced load;
+// X[col][k] is the same for every lane (a scalar load).
 template <int RB>
 __global__ __launch_bounds__(64) void k_axis_gemm_c(const double *__restrict__ At, int n, int ncol,
                                                     const double *__restrict__ in, double *__restrict__ out) {
     const int i = threadIdx.x;
     const int c0 = blockIdx.x * RB;
-    double av[64], xv[RB], acc[RB];
+    if (i >= n) return;
+    double acc[RB];
 #pragma unroll
-    for (int k = 0; k < 64; k++) av[k] = (k < n && i < n) ? At[(size_t)k * n + i] : 0.0;
+    for (int r = 0; r < RB; r++) acc[r] = 0.0;
+#pragma unroll 8
+    for (int k = 0; k < n; k++) {
+        const double a = At[(size_t)k * n + i];
 #pragma unroll
-    for (int r = 0; r < RB; r++) {
-        const int col = min(c0 + r, ncol - 1);
-        xv[r] = i < n ? in[(size_t)col * n + i] : 0.0;
-        acc[r] = 0.0;
-    }
-#pragma unroll
-    for (int k = 0; k < 64; k++) {
-        if (k < n) {
-#pragma unroll
-            for (int r = 0; r < RB; r++) acc[r] += av[k] * readlane_f64(xv[r], k);
+        for (int r = 0; r < RB; r++) {
+            const int col = min(c0 + r, ncol - 1);
+            acc[r] += a * in[(size_t)col * n + k];
         }
     }
-    if (i >= n) return;
 #pragma unroll
     for (int r = 0; r < RB; r++)
         if (c0 + r < ncol) out[(size_t)(c0 + r) * n + i] = acc[r];
@@ -2828,29 +2814,25 @@ __global__ __launch_bounds__(64) void k_axis_gemm_c(const double *__restrict__ A
 
 // Axis with a stride (inner) that is a multiple of 64: lane j is one of 64
 // consecutive columns (the same outer index), and the work group computes RB
-// rows of them. X[o][k][j] is a coalesced load; lane k holds A[i][k].
+// rows of them. X[o][k][j] is a coalesced load; A[i][k] is the same for every
+// lane (a scalar load).
 template <int RB>
 __global__ __launch_bounds__(64) void k_axis_gemm_r(const double *__restrict__ A, int n, int inner,
                                                     const double *__restrict__ in, double *__restrict__ out) {
-    const int lane = threadIdx.x;
-    const int col = blockIdx.x * 64 + lane;
+    const int col = blockIdx.x * 64 + threadIdx.x;
     const int i0 = blockIdx.y * RB;
     const size_t o = (size_t)(col / inner), jj = (size_t)(col % inner);
     const double *__restrict__ x = in + o * (size_t)n * inner + jj;
-    double xv[64], av[RB], acc[RB];
+    double acc[RB];
 #pragma unroll
-    for (int k = 0; k < 64; k++) xv[k] = k < n ? x[(size_t)k * inner] : 0.0;
+    for (int r = 0; r < RB; r++) acc[r] = 0.0;
+#pragma unroll 8
+    for (int k = 0; k < n; k++) {
+        const double xv = x[(size_t)k * inner];
 #pragma unroll
-    for (int r = 0; r < RB; r++) {
-        const int i = min(i0 + r, n - 1);
-        av[r] = lane < n ? A[(size_t)i * n + lane] : 0.0;
-        acc[r] = 0.0;
-    }
-#pragma unroll
-    for (int k = 0; k < 64; k++) {
-        if (k < n) {
-#pragma unroll
-            for (int r = 0; r < RB; r++) acc[r] += readlane_f64(av[r], k) * xv[k];
+        for (int r = 0; r < RB; r++) {
+            const int i = min(i0 + r, n - 1);
+            acc[r] += A[(size_t)i * n + k] * xv;
         }
     }
     double *__restrict__ y = out + o * (size_t)n * inner + jj;

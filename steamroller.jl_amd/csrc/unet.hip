@@ -228,7 +228,43 @@ __global__ __launch_bounds__(TPB) void k_wgrad(const uint16_t *__restrict__ dz, 
     f32x4 acc[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int64_t vs = v0; vs < v1; vs += 32) {
+    // rows of a multiple of 8 voxels: each lane's 8 consecutive voxels lie in
+    // one row (chunks start at multiples of 32), so their coordinates are
+    // found once per step
+    const bool rows8 = (n0 & 7) == 0;
+    for (int64_t vs = v0; vs < v1 && rows8; vs += 32) {
+        const int64_t vb = vs + 8 * g;
+        uint32_t av[8], bv[4][8];
+        const bool okv = vb < v1;
+        const int64_t nn = okv ? vb / V : 0, r = okv ? vb - nn * V : 0;
+        const int rr = (int)r, px0 = rr % n0, py = (rr / n0) % n1, pz = rr / (n0 * n1);
+#pragma unroll
+        for (int i = 0; i < 8; i++) av[i] = (okv && co < cg) ? dz[(vb + i) * cg + co] : 0u;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int yy = py + kdy[j], zz = pz + kdz[j];
+            const bool rowok = okv && kmode[j] == 1 && yy >= 0 && yy < n1 && zz >= 0 && zz < n2;
+            const int64_t base = (nn * V + (int64_t)n0 * (yy + (int64_t)n1 * zz)) * cin + kci[j];
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const int xx = px0 + i + kdx[j];
+                uint32_t val = 0u;
+                if (okv && kmode[j] == 2) val = 0x3f80u;  // 1.0
+                else if (rowok && xx >= 0 && xx < n0) val = x[base + (int64_t)xx * cin];
+                bv[j][i] = val;
+            }
+        }
+        const uint4 aq = make_uint4(av[0] | (av[1] << 16), av[2] | (av[3] << 16), av[4] | (av[5] << 16),
+                                    av[6] | (av[7] << 16));
+        const bf16x8 a = __builtin_bit_cast(bf16x8, aq);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint4 bq = make_uint4(bv[j][0] | (bv[j][1] << 16), bv[j][2] | (bv[j][3] << 16),
+                                        bv[j][4] | (bv[j][5] << 16), bv[j][6] | (bv[j][7] << 16));
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, __builtin_bit_cast(bf16x8, bq), acc[j], 0, 0, 0);
+        }
+    }
+    for (int64_t vs = v0; vs < v1 && !rows8; vs += 32) {
         uint32_t av[8], bv[4][8];
 #pragma unroll
         for (int i = 0; i < 8; i++) {
@@ -277,6 +313,8 @@ __global__ __launch_bounds__(TPB) void k_wgrad_reduce(const float *__restrict__ 
     const int i = blockIdx.x * TPB + threadIdx.x;
     if (i >= cout * kx) return;
     float s = 0.f;
+    // loads of 16 chunks in flight, the sum in chunk order
+#pragma unroll 16
     for (int c = 0; c < nchunk; c++) s += partial[(int64_t)c * cout * kx + i];
     const int co = i / kx, k = i - co * kx;
     if (k < ktot)
