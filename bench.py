@@ -48,7 +48,23 @@ def parse():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched photoionization measurement")
     ap.add_argument("--no-regrid", action="store_true", help="skip the regrid measurement")
     ap.add_argument("--no-order4", action="store_true", help="skip the order-4 measurement")
+    ap.add_argument("--forms", default="", help="comma-separated solve forms to time (fp64 and fp64_fmg always)")
+    ap.add_argument("--markers", action="store_true",
+                    help="roctx markers 'begin FORM I' / 'end FORM I' around every solve (for rocprofv3 --marker-trace)")
     return ap.parse_args()
+
+
+def roctx_marker():
+    """roctxMarkA of the profiler SDK's marker library, or None where it is absent."""
+    import ctypes
+    for name in ("librocprofiler-sdk-roctx.so.1", "librocprofiler-sdk-roctx.so", "libroctx64.so.4"):
+        try:
+            f = ctypes.CDLL(name).roctxMarkA
+        except (OSError, AttributeError):
+            continue
+        f.argtypes = [ctypes.c_char_p]
+        return lambda msg: f(msg.encode())
+    return None
 
 
 def log(*a):
@@ -105,12 +121,15 @@ def main():
 
     # every form starts from u = 0 (GUESS_ZERO: the initial norm and defect from b alone)
     Z = solver.GUESS_ZERO
-    specs = {"fp64": (False, 1, Z, False), "fp64_fmg": (False, 1, Z, True), "mixed": (True, 1, Z, False),
-             "mixed2": (True, 2, Z, False), "mixed3": (True, 3, Z, False), "mixed2_fmg": (True, 2, Z, True),
-             "mixed3_fmg": (True, 3, Z, True)}
+    # (mixed, inner cycles, first iterate, fmg! first, inner cycles of the first correction)
+    specs = {"fp64": (False, 1, Z, False, 0), "fp64_fmg": (False, 1, Z, True, 0), "mixed": (True, 1, Z, False, 0),
+             "mixed2": (True, 2, Z, False, 0), "mixed3": (True, 3, Z, False, 0),
+             "mixed2_then3": (True, 3, Z, False, 2), "mixed2_fmg": (True, 2, Z, True, 0),
+             "mixed3_fmg": (True, 3, Z, True, 0)}
     if net is not None:
-        specs["mixed2_learned"] = (True, 2, solver.GUESS_LEARNED, False)
-        specs["mixed3_fmg_learned"] = (True, 3, solver.GUESS_LEARNED, True)
+        specs["mixed2_learned"] = (True, 2, solver.GUESS_LEARNED, False, 0)
+        specs["mixed2_then3_learned"] = (True, 3, solver.GUESS_LEARNED, False, 2)
+        specs["mixed3_fmg_learned"] = (True, 3, solver.GUESS_LEARNED, True, 0)
     G, M = p.b.G, p.M
 
     def valid(f):
@@ -119,21 +138,36 @@ def main():
     def rel_err(a, ref):
         return float(np.max(np.abs(a - ref)) / np.max(np.abs(ref)))
 
+    if args.forms:
+        keep = set(args.forms.split(",")) | {"fp64", "fp64_fmg"}
+        unknown = keep - set(specs)
+        if unknown:
+            raise SystemExit(f"unknown forms {sorted(unknown)}; known: {sorted(specs)}")
+        specs = {k: v for k, v in specs.items() if k in keep}
+    mark = roctx_marker() if args.markers else None
+    if args.markers and mark is None:
+        raise SystemExit("--markers: no roctx library found")
+
     forms, potentials, histories = {}, {}, {}
-    for name, (mixed, inner, gmode, fmg) in specs.items():
+    for name, (mixed, inner, gmode, fmg, inner_first) in specs.items():
         times, cycles = [], []
         for it in range(args.warmup + args.steps):
             reset()
             ctx.synchronize()
+            if mark:
+                mark(f"begin {name} {it}")
             ts = time.perf_counter()
             if not mixed:
                 n, hist = solver.solve(u, b, r, u1, p.s, geom, p.bc, disc, tol=1e-8, max_cycles=60, guess=gmode,
                                        first_fmg=fmg)
             else:
                 n, hist = solver.solve_mixed(u, b, e, be, re, ue, p.s, geom, p.bc, disc, tol=1e-8,
-                                             max_cycles=60, inner_cycles=inner, guess=gmode, first_fmg=fmg)
+                                             max_cycles=60, inner_cycles=inner, guess=gmode, first_fmg=fmg,
+                                             inner_first=inner_first)
             ctx.synchronize()
             dt = time.perf_counter() - ts
+            if mark:
+                mark(f"end {name} {it}")
             if it >= args.warmup:
                 times.append(dt)
                 cycles.append(n)

@@ -252,6 +252,9 @@ typedef struct {
                                   iterate, the later ones vcycle!; in the mixed form the first
                                   correction runs fmg! on e before its inner_cycles V-cycles.
                                   Order-2 operators. */
+    int32_t inner_first;       /* mixed precision: FP32 V-cycles of the first correction (0 means
+                                  inner_cycles); the first correction starts from the largest
+                                  defect, where the FP32 floor stops progress sooner */
 } srmg_solve_opts;
 
 /* First iterate of a solve. NONE: u as given. ZERO: u = 0 whatever u holds
@@ -279,7 +282,8 @@ int srmg_solve_batched(srmg_ctx *ctx, int u, int b, int r, int u1, const double 
                        const srmg_solve_opts *opts, int *cycles, double *hist);
 
 /* Mixed precision: u, b FP64; e, be, re, ue FP32 scratch fields. Each outer
- * step forms the FP64 residual, runs inner_cycles FP32 V-cycles on the
+ * step forms the FP64 residual, runs inner_cycles FP32 V-cycles (inner_first in
+ * the first step, when set) on the
  * correction equation from e = 0, adds e to u and restricts u so covered
  * cells equal the mean of their children; the coarsest-level solve is FP64.
  * cycles counts outer steps; hist has one entry per outer step. */

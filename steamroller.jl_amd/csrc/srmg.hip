@@ -1890,6 +1890,8 @@ static Graph &solve_graph(srmg_ctx *c, int kind, std::initializer_list<const Fie
     add(o->nup);
     add(o->omega);
     add(o->inner_cycles);
+    add(o->inner_first);
+    add(o->first_fmg);
     for (int v : c->sweep_cfg) add(v);
     add(c->group_tiles);
     add(c->axis_gemm);
@@ -1914,7 +1916,8 @@ static std::vector<double> read_norms(srmg_ctx *c) {
 
 static void check_opts(const srmg_solve_opts *o) {
     if (!o) fail("null options");
-    if (o->max_cycles < 0 || o->ndown < 0 || o->nup < 0 || o->ntop < 0 || o->inner_cycles < 0)
+    if (o->max_cycles < 0 || o->ndown < 0 || o->nup < 0 || o->ntop < 0 || o->inner_cycles < 0 ||
+        o->inner_first < 0)
         fail("bad solve options");
     if (o->guess < SRMG_GUESS_NONE || o->guess > SRMG_GUESS_ZERO) fail("bad first-iterate mode");
     if (o->guess_smooth < 0) fail("bad first-iterate smoothing count");
@@ -2018,15 +2021,18 @@ static void solve_mixed_dm(srmg_ctx *c, FieldBuf &u, FieldBuf &b, FieldBuf &e, F
     const size_t ncell = (size_t)c->nb * LY::V, nib = e.ib.bytes / sizeof(float);
     if (fused)
         hipLaunchKernelGGL(k_fill<float>, dim3(nwg(ncell)), dim3(WG), 0, c->stream, e.data.as<float>(), 0.0f, ncell);
-    // one outer step; first: fmg! on the correction equation before its
-    // V-cycles (first_fmg)
+    // one outer step; first: the first step, which runs fmg! on the
+    // correction equation before its V-cycles (first_fmg) and inner_first
+    // V-cycles when that is set
+    const int n_inner = std::max(1, (int)o->inner_cycles), n_first = o->inner_first > 0 ? o->inner_first : n_inner;
+    const bool first_differs = o->first_fmg || n_first != n_inner;
     auto correct = [&](bool first) {
         if (!fused)
             hipLaunchKernelGGL(k_fill<float>, dim3(nwg(ncell)), dim3(WG), 0, c->stream, e.data.as<float>(), 0.0f,
                                ncell);
         hipLaunchKernelGGL(k_fill<float>, dim3(nwg(nib)), dim3(WG), 0, c->stream, e.ib.as<float>(), 0.0f, nib);
-        if (first) o32.fmg(e, be, re, ue, s, op, o->ndown, o->ntop, o->nup, o->omega);
-        for (int k = 0; k < std::max(1, (int)o->inner_cycles); k++)
+        if (first && o->first_fmg) o32.fmg(e, be, re, ue, s, op, o->ndown, o->ntop, o->nup, o->omega);
+        for (int k = 0; k < (first ? n_first : n_inner); k++)
             o32.vcycle(e, be, re, ue, s, op, 0, o->ndown, o->ntop, o->nup, o->omega);
         if constexpr (Ops<D, M, double>::GROUP_OK) {
             if (fused) {
@@ -2067,7 +2073,7 @@ static void solve_mixed_dm(srmg_ctx *c, FieldBuf &u, FieldBuf &b, FieldBuf &e, F
     Graph &gr = solve_graph(c, 1, {&u, &b, &e, &be, &re, &ue}, o);
     Graph &gf = solve_graph(c, 3, {&u, &b, &e, &be, &re, &ue}, o);
     while (k < o->max_cycles && !done) {
-        const bool f = o->first_fmg && k == 0;
+        const bool f = first_differs && k == 0;
         Graph &g = f ? gf : gr;
         run_maybe_graph(c, g, o->use_graph && (f ? (g.ran || g.x) : (k > 0 || g.x)), [&] { correct(f); });
         g.ran = true;
@@ -2127,7 +2133,8 @@ static void solve_mixed4_dm(srmg_ctx *c, FieldBuf &u, FieldBuf &b, FieldBuf &e, 
         }
         o32.zero_all(e);
         hipLaunchKernelGGL(k_fill<float>, dim3(nwg(nfr)), dim3(WG), 0, c->stream, o32.frame(e), 0.f, nfr);
-        for (int it = 0; it < std::max(1, (int)o->inner_cycles); it++) {
+        const int n_inner = std::max(1, (int)o->inner_cycles);
+        for (int it = 0; it < (k == 0 && o->inner_first > 0 ? o->inner_first : n_inner); it++) {
             if (it > 0)
                 for (int l = 0; l < c->nlevels; l++) o32.frame_fill(e, l, b32);
             o32.vcycle4(e, be, re, ue, s, op, 0, o->ndown, o->ntop, o->nup, o->omega);

@@ -94,7 +94,7 @@ class _SolveOpts(C.Structure):
     _fields_ = [("ndown", C.c_int32), ("ntop", C.c_int32), ("nup", C.c_int32),
                 ("omega", C.c_double), ("tol", C.c_double), ("max_cycles", C.c_int32),
                 ("use_graph", C.c_int32), ("inner_cycles", C.c_int32), ("guess", C.c_int32),
-                ("guess_smooth", C.c_int32), ("first_fmg", C.c_int32)]
+                ("guess_smooth", C.c_int32), ("first_fmg", C.c_int32), ("inner_first", C.c_int32)]
 
 
 def make_op(geometry, bc, disc, coarse: int = COARSE_GS, any_order: bool = False) -> _Op:
@@ -543,7 +543,7 @@ GUESS_SMOOTH = 4  # level-1 sweeps after a learned first iterate (default)
 
 
 def _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph, inner_cycles=1, guess=0, guess_smooth=GUESS_SMOOTH,
-          first_fmg=False):
+          first_fmg=False, inner_first=0):
     o = _SolveOpts()
     o.ndown, o.ntop, o.nup = int(ndown), int(ntop), int(nup)
     o.omega, o.tol, o.max_cycles, o.use_graph = float(omega), float(tol), int(max_cycles), int(use_graph)
@@ -551,6 +551,7 @@ def _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph, inner_cycles=1, g
     o.guess = int(guess)
     o.guess_smooth = int(guess_smooth)
     o.first_fmg = int(bool(first_fmg))
+    o.inner_first = int(inner_first)
     return o
 
 
@@ -572,14 +573,15 @@ def solve(u, b, r, u1, s, geometry, bc, lpldisc, *, tol=1e-8, max_cycles=50, ndo
 
 def solve_mixed(u, b, e, be, re, ue, s, geometry, bc, lpldisc, *, tol=1e-8, max_cycles=50,
                 ndown=2, ntop=4, nup=2, omega=1.0, coarse=COARSE_EXACT, use_graph=True, inner_cycles=1,
-                guess=GUESS_NONE, guess_smooth=GUESS_SMOOTH,
-          first_fmg=False):
+                guess=GUESS_NONE, guess_smooth=GUESS_SMOOTH, first_fmg=False, inner_first=0):
     """FP64 defect correction around FP32 V-cycles (u, b FP64; e, be, re, ue FP32;
-    first_fmg: the first correction runs fmg! on e before its V-cycles). Returns (outer
-    steps, relative residual history per outer step).
+    first_fmg: the first correction runs fmg! on e before its V-cycles;
+    inner_first: V-cycles of the first correction, 0 for inner_cycles).
+    Returns (outer steps, relative residual history per outer step).
     LaplacianDiscretization(D, 4) needs coarse=COARSE_GS."""
     op = make_op(geometry, bc, lpldisc, coarse, any_order=True)
-    o = _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph, inner_cycles, guess, guess_smooth, first_fmg)
+    o = _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph, inner_cycles, guess, guess_smooth, first_fmg,
+              inner_first)
     hist = np.zeros(max_cycles + 1)
     n = C.c_int()
     _check(lib().srmg_solve_mixed(u.ctx.h, u.id, b.id, e.id, be.id, re.id, ue.id, float(s),
@@ -644,12 +646,12 @@ def solve_batched(u, b, r, u1, s, geometry, bc, discs, *, tol=1e-8, max_cycles=5
 
 def solve_mixed_batched(u, b, e, be, re, ue, s, geometry, bc, discs, *, tol=1e-8, max_cycles=50, ndown=2,
                         ntop=4, nup=2, omega=1.0, coarse=COARSE_EXACT, use_graph=True, inner_cycles=1,
-                        guess=GUESS_NONE, guess_smooth=GUESS_SMOOTH,
-          first_fmg=False):
+                        guess=GUESS_NONE, guess_smooth=GUESS_SMOOTH, first_fmg=False, inner_first=0):
     """FP64 defect correction around FP32 V-cycles on every tree of a forest."""
     sv = np.ascontiguousarray(np.asarray(s, np.float64))
     ops = _ops(geometry, bc, discs, coarse)
-    o = _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph, inner_cycles, guess, guess_smooth, first_fmg)
+    o = _opts(ndown, ntop, nup, omega, tol, max_cycles, use_graph, inner_cycles, guess, guess_smooth, first_fmg,
+              inner_first)
     hist = np.zeros((len(discs), max_cycles + 1))
     n = C.c_int()
     _check(lib().srmg_solve_mixed_batched(u.ctx.h, u.id, b.id, e.id, be.id, re.id, ue.id, _p(sv), ops,

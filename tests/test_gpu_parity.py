@@ -372,6 +372,47 @@ def test_mixed_precision_matches_fp64(name):
     assert err <= 1e-7
 
 
+@pytest.mark.parametrize("name", ["cfg1", "cart3-m8"])
+def test_mixed_first_correction_cycle_count(name):
+    """inner_first sets the FP32 V-cycles of the first correction only: the
+    first outer step of (inner_first=a, inner_cycles=b) is that of
+    inner_cycles=a, bit for bit, and inner_first=b is inner_cycles=b; the
+    captured graphs of both steps replay to the same bits."""
+    p = CASES[name]()
+    print(f"case {name}", flush=True)
+    geom, disc = solver.geometry_from_code(p.geom), _disc(p)
+    ctx = solver.Context(p.tree, p.conn, p.M)
+    v, b = ctx.field(F64), ctx.field(F64)
+    e, be, re, ue = (ctx.field(F32) for _ in range(4))
+    b.upload(p.b)
+
+    def run(inner, first=0):
+        n, h = solver.solve_mixed(v, b, e, be, re, ue, p.s, geom, p.bc, disc, tol=1e-8, max_cycles=12,
+                                  inner_cycles=inner, inner_first=first, guess=solver.GUESS_ZERO)
+        host = p.b.like()
+        v.download(host)
+        return n, h, _valid(host, p).copy()
+
+    n2, h2, _ = run(2)
+    n3, h3, u3 = run(3)
+    n23, h23, u23 = run(3, 2)
+    n32, h32, _ = run(2, 3)
+    n33, h33, u33 = run(3, 3)
+    print(f"{name}: [2] {h2}, [3] {h3}, [2,3] {h23}, [3,2] {h32}", flush=True)
+    assert h23[1] == h2[1] and h32[1] == h3[1]
+    assert h23[-1] <= 1e-8 and h32[-1] <= 1e-8
+    assert n33 == n3
+    np.testing.assert_array_equal(h33, h3)
+    np.testing.assert_array_equal(u33, u3)
+    # second solve of the same schedule: both graphs replayed
+    m23, g23, w23 = run(3, 2)
+    assert m23 == n23
+    np.testing.assert_array_equal(g23, h23)
+    np.testing.assert_array_equal(w23, u23)
+    err = np.max(np.abs(u23 - u3)) / np.max(np.abs(u3))
+    assert err <= 1e-7, err
+
+
 @pytest.mark.parametrize("name", ["cfg1", "cyl2-m8", "cart3-m8", "cart3-m16"])
 @pytest.mark.parametrize("dtype", [F64, F32])
 def test_every_sweep_tile_shape_gives_identical_results(name, dtype):

@@ -96,3 +96,27 @@ def test_missing_extension_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(solver, "_lib", None)
     with pytest.raises(solver.SrmgError):
         solver.lib()
+
+
+@pytest.mark.parametrize("cname,pyname", [("srmg_op", "_Op"), ("srmg_solve_opts", "_SolveOpts")])
+def test_ctypes_records_match_the_c_header_layout(cname, pyname, tmp_path):
+    """Size and field offsets of the ctypes mirrors equal what a C compiler
+    lays out for include/srmg.h."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("no C compiler")
+    py = getattr(solver, pyname)
+    fields = [f[0] for f in py._fields_]
+    src = tmp_path / "layout.c"
+    body = "".join(f'    printf("{f} %zu\\n", offsetof({cname}, {f}));\n' for f in fields)
+    src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "srmg.h"\nint main(void) {\n'
+                   f'    printf("sizeof %zu\\n", sizeof({cname}));\n{body}    return 0;\n}}\n')
+    exe = tmp_path / "layout"
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    subprocess.run(["gcc", "-std=c99", "-I", inc, str(src), "-o", str(exe)], check=True)
+    out = dict(line.split() for line in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                       text=True).stdout.splitlines())
+    assert int(out["sizeof"]) == ctypes.sizeof(py)
+    for f in fields:
+        assert int(out[f]) == getattr(py, f).offset, f
