@@ -424,9 +424,13 @@ int srmg_level_grouped(srmg_ctx *ctx, int level, int *grouped);
 /* Use (default) or bypass the sibling-group forms of the sweep, residual and
  * prolongation kernels on grouped levels; both give the same bits. */
 int srmg_set_group_tiles(srmg_ctx *ctx, int enable);
-/* Kernels of the level-1 axis transforms (exact coarsest solve and
- * srmg_grid_solve): 1 (default) one wave per work group without LDS where the
- * axis length is at most 64, 0 the LDS-tiled kernels; both give the same bits. */
+/* Kernels of the level-1 solve (exact coarsest solve and srmg_grid_solve):
+ * 2 (default) three launches for 3D grids whose extents are multiples of 8 and
+ * at most 64 (plane and column tiles in LDS: forward x and y, forward z with
+ * the eigenvalue division and inverse z, inverse y and x), else as 1; 1 one
+ * launch per axis transform, one wave per work group without LDS where the
+ * axis length is at most 64; 0 the LDS-tiled axis kernels. All give the same
+ * bits. */
 int srmg_set_axis_gemm(srmg_ctx *ctx, int variant);
 
 /* Blocks, leaf blocks and coarse-fine ghost slabs of a level (1-based). */

@@ -908,7 +908,8 @@ int ora_coarse_solve(ora_tree *t, double *u, const double *b, double s, const or
         if (fabs(ev) < 1e-13) { singular = 1; ev = 1.0; }
         x[gi] /= ev;
     }
-    for (int d = 0; d < D; d++) apply_axis(P[d], d, N, D, x, tmp);
+    /* inverse transforms in the reverse axis order (the last forward axis first) */
+    for (int d = D - 1; d >= 0; d--) apply_axis(P[d], d, N, D, x, tmp);
     for (int64_t k = t->pairs_off[0]; k < t->pairs_off[1]; k++) {
         const int64_t *p = t->pairs + k * (D + 1);
         int64_t blk = p[D];

@@ -2859,6 +2859,258 @@ __global__ __launch_bounds__(WG) void k_eig_scale(double *__restrict__ g, const 
     g[gi] /= ev;
 }
 
+// ---- level-1 exact solve of 3D grids in three launches ----
+//
+// For stacked 3D grids [g][z][y][x] (x fastest) whose extents are multiples of
+// 8 and at most 64, the transforms of k_axis_gemm (each output one sum over k
+// in increasing order, from 0.0, of A[i][k] x[k], no fused multiply-add) in
+// the order forward x, y, z; the eigenvalue division of k_eig_scale; inverse
+// z, y, x. Each launch keeps a plane or a set of columns in LDS, so the nine
+// launches of the axis-by-axis form (gather, three transforms, scale, three
+// transforms, scatter) become three with the same bits:
+//   k_l1_fwd_xy  plane (g, z), IC output columns: gather, forward x, forward y
+//   k_l1_z       CC columns (y, x) of grid g: forward z, divide, inverse z
+//   k_l1_inv_yx  plane (g, z), JC output rows: inverse y, inverse x, scatter
+// BLK: the input (output) is the level-1 block field, gathered through btab
+// (btab[g * nper + (bz * nb1 + by) * nb0 + bx] is the block at block position
+// (bx, by, bz) of grid g) and scaled by cs[g]; otherwise a plain grid.
+
+constexpr int L1T = 256;  // threads per work group of the three kernels (four waves)
+constexpr int L1S = 65;   // LDS row stride of a staged plane (doubles)
+
+// out[r] = sum over k < K of a(r, k) b(r, k), in increasing k from 0.0 (no
+// fused multiply-add), for the R outputs of a thread at once
+template <int R, class Fa, class Fb>
+__device__ __forceinline__ void l1_dots(double (&out)[R], int K, Fa fa, Fb fb) {
+#pragma unroll
+    for (int r = 0; r < R; r++) out[r] = 0.0;
+#pragma unroll 8
+    for (int k = 0; k < K; k++) {
+#pragma unroll
+        for (int r = 0; r < R; r++) out[r] += fa(r, k) * fb(r, k);
+    }
+}
+
+// rows of an n x m matrix (m <= 64) staged row by row: wave w takes rows w,
+// w + 4, ..., lane = column; every load is issued before the first store
+template <class Ld, class St>
+__device__ __forceinline__ void l1_rows(int n, int m, Ld ld, St st) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double v[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int row = w + 4 * r;
+        v[r] = (row < n && lane < m) ? ld(row, lane) : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int row = w + 4 * r;
+        if (row < n && lane < m) st(row, lane, v[r]);
+    }
+}
+
+template <int M, typename T, bool BLK, int IC>
+__global__ __launch_bounds__(L1T) void k_l1_fwd_xy(double *__restrict__ out, const T *__restrict__ src,
+                                                   const double *__restrict__ gin, const int *__restrict__ btab,
+                                                   const double *__restrict__ cs, const double *__restrict__ A0t,
+                                                   const double *__restrict__ A1t, int N0, int N1, int N2) {
+    using LY = Lay<3, M>;
+    constexpr int RO = 64 * IC / L1T;  // outputs per thread
+    constexpr int KS = L1T / IC;       // rows of a0 loaded per pass
+    __shared__ double xs[64 * L1S];    // the plane X[y][x]
+    __shared__ double a1[64 * 64];     // a1[k][j] = A1[j][k]
+    __shared__ double a0[64 * IC];     // a0[k][il] = A0[i0 + il][k]
+    __shared__ double y1[64 * IC];     // Y1[y][il]
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int z = blockIdx.x % N2, g = blockIdx.x / N2, i0 = blockIdx.y * IC;
+    const int il = tid % IC, yb = tid / IC;
+    double a0r[RO];
+#pragma unroll
+    for (int r = 0; r < RO; r++) {
+        const int k = yb + KS * r;
+        a0r[r] = k < N0 ? A0t[(size_t)k * N0 + i0 + il] : 0.0;
+    }
+    if constexpr (BLK) {
+        const int nb0 = N0 / M, nb1 = N1 / M, nper = nb0 * nb1 * (N2 / M);
+        const int *tb = btab + (size_t)g * nper + (z / M) * nb0 * nb1;
+        const double c = cs[g];
+        int blk[16];
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int y = w + 4 * r;
+            blk[r] = (y < N1 && lane < N0) ? tb[(y / M) * nb0 + lane / M] : 0;
+        }
+        double v[16];
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int y = w + 4 * r;
+            v[r] = (y < N1 && lane < N0)
+                       ? c * (double)src[(size_t)blk[r] * LY::V + LY::sidx(lane % M, y % M, z % M)]
+                       : 0.0;
+        }
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int y = w + 4 * r;
+            if (y < N1 && lane < N0) xs[y * L1S + lane] = v[r];
+        }
+    } else {
+        const double *p = gin + ((size_t)g * N2 + z) * N1 * N0;
+        l1_rows(N1, N0, [&](int y, int x) { return p[(size_t)y * N0 + x]; },
+                [&](int y, int x, double v) { xs[y * L1S + x] = v; });
+    }
+    l1_rows(N1, N1, [&](int k, int j) { return A1t[(size_t)k * N1 + j]; },
+            [&](int k, int j, double v) { a1[k * 64 + j] = v; });
+#pragma unroll
+    for (int r = 0; r < RO; r++) {
+        const int k = yb + KS * r;
+        if (k < N0) a0[k * IC + il] = a0r[r];
+    }
+    __syncthreads();
+    // Y1[y][il] = sum_k A0[i0 + il][k] X[y][k], y = yb + KS r
+    {
+        double acc[RO];
+        l1_dots(acc, N0, [&](int r, int k) { return a0[k * IC + il]; },
+                [&](int r, int k) { return xs[(yb + KS * r) * L1S + k]; });
+#pragma unroll
+        for (int r = 0; r < RO; r++) y1[(yb + KS * r) * IC + il] = acc[r];
+    }
+    __syncthreads();
+    // Y2[j][il] = sum_k A1[j][k] Y1[k][il], j = yb + KS r
+    double acc[RO];
+    l1_dots(acc, N1, [&](int r, int k) { return a1[k * 64 + yb + KS * r]; },
+            [&](int r, int k) { return y1[k * IC + il]; });
+    double *po = out + ((size_t)g * N2 + z) * N1 * N0 + i0 + il;
+#pragma unroll
+    for (int r = 0; r < RO; r++) {
+        const int j = yb + KS * r;
+        if (j < N1) po[(size_t)j * N0] = acc[r];
+    }
+}
+
+// forward z, division by shift[g] + l0[x] + l1[y] + l2[z] (k_eig_scale's sum
+// order), inverse z on CC consecutive columns c = y N0 + x of grid g
+template <int CC>
+__global__ __launch_bounds__(L1T) void k_l1_z(double *__restrict__ out, const double *__restrict__ in,
+                                              const double *__restrict__ A2t, const double *__restrict__ B2t,
+                                              const double *__restrict__ l0, const double *__restrict__ l1,
+                                              const double *__restrict__ l2, const double *__restrict__ shift,
+                                              int N0, int N1, int N2) {
+    constexpr int RO = 64 * CC / L1T;  // outputs per thread
+    constexpr int ZS = L1T / CC;       // z rows per pass
+    __shared__ double am[64 * 64];     // am[k][zo] = A2[zo][k]
+    __shared__ double bm[64 * 64];     // bm[k][zo] = B2[zo][k]
+    __shared__ double ts[64 * CC];     // the columns, Y[z][cc]
+    __shared__ double ws[64 * CC];     // the divided forward transform
+    const int tid = threadIdx.x;
+    const int ncb = N0 * N1 / CC;
+    const int g = blockIdx.x / ncb, c0 = (blockIdx.x - g * ncb) * CC;
+    const size_t plane = (size_t)N0 * N1;
+    const int cc = tid % CC, zb = tid / CC;
+    const double *pin = in + (size_t)g * N2 * plane + c0 + cc;
+    double tv[RO];
+#pragma unroll
+    for (int r = 0; r < RO; r++) {
+        const int zz = zb + ZS * r;
+        tv[r] = zz < N2 ? pin[(size_t)zz * plane] : 0.0;
+    }
+    const int c = c0 + cc, y = c / N0, x = c - y * N0;
+    double ev0 = shift[g];
+    ev0 += l0[x];
+    ev0 += l1[y];
+    double ev[RO];
+#pragma unroll
+    for (int r = 0; r < RO; r++) {
+        const int zo = zb + ZS * r;
+        ev[r] = ev0 + (zo < N2 ? l2[zo] : 0.0);
+    }
+    l1_rows(N2, N2, [&](int k, int zo) { return A2t[(size_t)k * N2 + zo]; },
+            [&](int k, int zo, double v) { am[k * 64 + zo] = v; });
+    l1_rows(N2, N2, [&](int k, int zo) { return B2t[(size_t)k * N2 + zo]; },
+            [&](int k, int zo, double v) { bm[k * 64 + zo] = v; });
+#pragma unroll
+    for (int r = 0; r < RO; r++) {
+        const int zz = zb + ZS * r;
+        if (zz < N2) ts[zz * CC + cc] = tv[r];
+    }
+    __syncthreads();
+    {
+        double acc[RO];
+        l1_dots(acc, N2, [&](int r, int k) { return am[k * 64 + zb + ZS * r]; },
+                [&](int r, int k) { return ts[k * CC + cc]; });
+#pragma unroll
+        for (int r = 0; r < RO; r++) ws[(zb + ZS * r) * CC + cc] = acc[r] / ev[r];
+    }
+    __syncthreads();
+    double acc[RO];
+    l1_dots(acc, N2, [&](int r, int k) { return bm[k * 64 + zb + ZS * r]; },
+            [&](int r, int k) { return ws[k * CC + cc]; });
+    double *po = out + (size_t)g * N2 * plane + c0 + cc;
+#pragma unroll
+    for (int r = 0; r < RO; r++) {
+        const int zo = zb + ZS * r;
+        if (zo < N2) po[(size_t)zo * plane] = acc[r];
+    }
+}
+
+template <int M, typename T, bool BLK, int JC>
+__global__ __launch_bounds__(L1T) void k_l1_inv_yx(T *__restrict__ dst, double *__restrict__ gout,
+                                                   const double *__restrict__ in, const int *__restrict__ btab,
+                                                   const double *__restrict__ B1t, const double *__restrict__ B0t,
+                                                   int N0, int N1, int N2) {
+    using LY = Lay<3, M>;
+    constexpr int RO = JC / 4;       // outputs per thread: rows jl = w + 4 r, column = lane
+    constexpr int KS = L1T / JC;     // rows of b1 loaded per pass
+    __shared__ double vs[64 * L1S];  // the plane V[y][x]
+    __shared__ double b0[64 * 64];   // b0[k][i] = B0[i][k]
+    __shared__ double b1[64 * JC];   // b1[k][jl] = B1[j0 + jl][k]
+    __shared__ double ts[JC * L1S];  // T[jl][x]
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int z = blockIdx.x % N2, g = blockIdx.x / N2, j0 = blockIdx.y * JC;
+    const int jq = tid % JC, kb = tid / JC;
+    double b1r[64 / KS];
+#pragma unroll
+    for (int r = 0; r < 64 / KS; r++) {
+        const int k = kb + KS * r;
+        b1r[r] = k < N1 ? B1t[(size_t)k * N1 + j0 + jq] : 0.0;
+    }
+    const double *pin = in + ((size_t)g * N2 + z) * N1 * N0;
+    l1_rows(N1, N0, [&](int y, int x) { return pin[(size_t)y * N0 + x]; },
+            [&](int y, int x, double v) { vs[y * L1S + x] = v; });
+    l1_rows(N0, N0, [&](int k, int i) { return B0t[(size_t)k * N0 + i]; },
+            [&](int k, int i, double v) { b0[k * 64 + i] = v; });
+#pragma unroll
+    for (int r = 0; r < 64 / KS; r++) {
+        const int k = kb + KS * r;
+        if (k < N1) b1[k * JC + jq] = b1r[r];
+    }
+    __syncthreads();
+    // T[jl][x] = sum_k B1[j0 + jl][k] V[k][x], jl = w + 4 r, x = lane
+    {
+        double acc[RO];
+        l1_dots(acc, N1, [&](int r, int k) { return b1[k * JC + w + 4 * r]; },
+                [&](int r, int k) { return vs[k * L1S + lane]; });
+#pragma unroll
+        for (int r = 0; r < RO; r++) ts[(w + 4 * r) * L1S + lane] = acc[r];
+    }
+    __syncthreads();
+    // U[jl][i] = sum_k B0[i][k] T[jl][k], i = lane
+    double acc[RO];
+    l1_dots(acc, N0, [&](int r, int k) { return b0[k * 64 + lane]; },
+            [&](int r, int k) { return ts[(w + 4 * r) * L1S + k]; });
+    if (lane >= N0) return;
+    const int nb0 = N0 / M, nb1 = N1 / M, nper = nb0 * nb1 * (N2 / M);
+#pragma unroll
+    for (int r = 0; r < RO; r++) {
+        const int j = j0 + w + 4 * r;
+        if constexpr (BLK) {
+            const int blk = btab[(size_t)g * nper + ((z / M) * nb1 + j / M) * nb0 + lane / M];
+            dst[(size_t)blk * LY::V + LY::sidx(lane % M, j % M, z % M)] = (T)acc[r];
+        } else {
+            gout[(((size_t)g * N2 + z) * N1 + j) * N0 + lane] = acc[r];
+        }
+    }
+}
+
 // ---- mixed precision helpers ----
 
 // u += e over n cells (FP64 += FP32)

@@ -80,6 +80,7 @@ struct CoarseGroup {
     int8_t bc[27] = {0};
     DevBuf P[3], Q[3], lam[3];
     DevBuf cs, shift;           // per tree: -s (gather factor) and the Helmholtz shift -s k2
+    DevBuf btab;                // per tree, the level-1 block at block position (bz nb1 + by) nb0 + bx
 };
 
 // a captured solve step: the graph and its executable
@@ -117,7 +118,7 @@ struct srmg_ctx {
     std::unique_ptr<FieldBuf> zero[2];  // all-zero scratch fields (F64, F32), made on demand
     srmg_unet *guess_net = nullptr;     // network of the learned first iterate (not owned)
     bool group_tiles = true;            // sibling-group kernels on grouped levels
-    int axis_gemm = 1;                  // level-1 transform kernels (launch_axis_gemm)
+    int axis_gemm = 2;                  // level-1 transform kernels (launch_axis_gemm, launch_l1_fused)
     int guess_fallbacks = 0;            // trees of the last solve that started from zero instead
     hipStream_t stream = nullptr;
     int maxpartials = 0;
@@ -578,9 +579,20 @@ static void upload_tables(srmg_ctx *c) {
     for (auto &G : c->groups) ng = std::max(ng, (size_t)G.ntree * G.N[0] * G.N[1] * (D == 3 ? G.N[2] : 1));
     c->d_grid.alloc(ng * sizeof(double));
     c->d_grid2.alloc(ng * sizeof(double));
+    std::vector<std::vector<int>> btabs;
+    btabs.reserve(c->groups.size());
     for (auto &G : c->groups) {
         G.cs.alloc(G.ntree * sizeof(double));
         G.shift.alloc(G.ntree * sizeof(double));
+        std::vector<int> tab((size_t)G.ntree * G.nper, 0);
+        for (int lb = 0; lb < G.ntree * G.nper; lb++) {
+            const int blk = G.l1_blk0 + lb;
+            int pos = 0;
+            for (int d = D - 1; d >= 0; d--) pos = pos * (G.N[d] / M) + c->h_coord[(size_t)blk * D + d] - 1;
+            tab[(size_t)(lb / G.nper) * G.nper + pos] = blk;
+        }
+        btabs.push_back(std::move(tab));
+        up(G.btab, btabs.back().data(), btabs.back().size() * sizeof(int));
     }
     // until autotuned or set: the column sweep on two threads per column for
     // 8^3 blocks (levels without complete sibling groups run shape 0)
@@ -741,13 +753,13 @@ static void upload_axis_matrix(DevBuf &buf, const std::vector<double> &A, int n)
 // bits.
 static void launch_axis_gemm(hipStream_t st, const double *A, int n, int inner, int ncol, const double *in,
                              double *out, int variant) {
-    if (n <= 64 && variant == 1 && inner == 1) {
+    if (n <= 64 && variant >= 1 && inner == 1) {
         constexpr int RB = 2;
         hipLaunchKernelGGL(k_axis_gemm_c<RB>, dim3((unsigned)((ncol + RB - 1) / RB)), dim3(64), 0, st,
                            A + (size_t)n * n, n, ncol, in, out);
         return;
     }
-    if (n <= 64 && variant == 1 && inner % 64 == 0) {
+    if (n <= 64 && variant >= 1 && inner % 64 == 0) {
         constexpr int RB = 2;
         hipLaunchKernelGGL(k_axis_gemm_r<RB>, dim3((unsigned)(ncol / 64), (unsigned)((n + RB - 1) / RB)), dim3(64),
                            0, st, A, n, inner, in, out);
@@ -770,6 +782,35 @@ static void launch_axis_gemm(hipStream_t st, const double *A, int n, int inner, 
     else
         hipLaunchKernelGGL(k_axis_gemm<16>, dim3((unsigned)((ncol + 15) / 16), gy), dim3(256), 0, st, A, n, inner,
                            ncol, in, out);
+}
+
+// the three-launch level-1 solve (k_l1_fwd_xy, k_l1_z, k_l1_inv_yx) applies
+// to 3D grids whose extents are multiples of 8 and at most 64
+static bool l1_fused_ok(int D, const int *N) {
+    if (D != 3) return false;
+    for (int d = 0; d < 3; d++)
+        if (N[d] > 64 || N[d] % 8) return false;
+    return true;
+}
+
+// Q[d], P[d]: forward and inverse matrices of axis d as upload_axis_matrix
+// lays them out (A, then its transpose); w1, w2: two grids of scratch. BLK:
+// from the block field src (through btab, times cs) to the block field dst;
+// otherwise from the grid gin to the grid gout.
+template <int M, typename T, bool BLK>
+static void launch_l1_fused(hipStream_t st, int ngrid, const int *N, const T *src, const double *gin,
+                            const int *btab, const double *cs, const double *const *Q, const double *const *P,
+                            const double *const *lam, const double *shift, double *w1, double *w2, T *dst,
+                            double *gout) {
+    constexpr int IC = 8, CC = 16, JC = 8;
+    const int N0 = N[0], N1 = N[1], N2 = N[2];
+    const size_t n00 = (size_t)N0 * N0, n11 = (size_t)N1 * N1, n22 = (size_t)N2 * N2;
+    hipLaunchKernelGGL((k_l1_fwd_xy<M, T, BLK, IC>), dim3((unsigned)(ngrid * N2), (unsigned)(N0 / IC)), dim3(L1T), 0,
+                       st, w1, src, gin, btab, cs, Q[0] + n00, Q[1] + n11, N0, N1, N2);
+    hipLaunchKernelGGL((k_l1_z<CC>), dim3((unsigned)(ngrid * (N0 * N1 / CC))), dim3(L1T), 0, st, w2, w1, Q[2] + n22,
+                       P[2] + n22, lam[0], lam[1], lam[2], shift, N0, N1, N2);
+    hipLaunchKernelGGL((k_l1_inv_yx<M, T, BLK, JC>), dim3((unsigned)(ngrid * N2), (unsigned)(N1 / JC)), dim3(L1T), 0,
+                       st, dst, gout, w2, btab, P[1] + n11, P[0] + n00, N0, N1, N2);
 }
 
 template <int D, int M, typename T>
@@ -1648,6 +1689,16 @@ void Ops<D, M, T>::coarse_exact(FieldBuf &u, FieldBuf &b, double s, const srmg_o
         const size_t total = (size_t)N[0] * N[1] * (D == 3 ? N[2] : 1);
         const int nblk = G.ntree * G.nper;
         double *g = c->d_grid.as<double>(), *g2 = c->d_grid2.as<double>();
+        if (D == 3 && c->axis_gemm == 2 && l1_fused_ok(D, N)) {
+            const double *Qp[3] = {G.Q[0].as<double>(), G.Q[1].as<double>(), G.Q[2].as<double>()};
+            const double *Pp[3] = {G.P[0].as<double>(), G.P[1].as<double>(), G.P[2].as<double>()};
+            const double *Lp[3] = {G.lam[0].as<double>(), G.lam[1].as<double>(), G.lam[2].as<double>()};
+            if constexpr (D == 3)
+                launch_l1_fused<M, T, true>(c->stream, G.ntree, N, X(b, 0), nullptr, G.btab.as<int>(),
+                                            G.cs.as<double>(), Qp, Pp, Lp, G.shift.as<double>(), g, g2, X(u, 0),
+                                            nullptr);
+            continue;
+        }
         hipLaunchKernelGGL((k_grid_gather<D, M, T>), dim3(nwg((int64_t)nblk * LY::V)), dim3(WG), 0, c->stream, g,
                            X(b, 0), coord(), G.l1_blk0, nblk, N[0], N[1], G.nper, total, G.cs.as<double>());
         int stride = 1;
@@ -1660,11 +1711,12 @@ void Ops<D, M, T>::coarse_exact(FieldBuf &u, FieldBuf &b, double s, const srmg_o
                            G.lam[0].as<double>(), G.lam[1].as<double>(),
                            D == 3 ? G.lam[2].as<double>() : G.lam[1].as<double>(), N[0], N[1], D,
                            G.shift.as<double>(), total, G.ntree);
-        stride = 1;
-        for (int d = 0; d < D; d++) {
+        // inverse transforms in the reverse axis order (the last forward axis first)
+        for (int d = D - 1; d >= 0; d--) {
+            stride = 1;
+            for (int e = 0; e < d; e++) stride *= N[e];
             axis(G.P[d].as<double>(), N[d], stride, (int)(G.ntree * total / N[d]), g, g2);
             std::swap(g, g2);
-            stride *= N[d];
         }
         hipLaunchKernelGGL((k_grid_scatter<D, M, T>), dim3(nwg((int64_t)nblk * LY::V)), dim3(WG), 0, c->stream,
                            X(u, 0), g, coord(), G.l1_blk0, nblk, N[0], N[1], G.nper, total);
@@ -3083,7 +3135,7 @@ int srmg_set_group_tiles(srmg_ctx *c, int enable) {
 int srmg_set_axis_gemm(srmg_ctx *c, int variant) {
     return guarded([&] {
         if (!c) fail("null context");
-        if (variant < 0 || variant > 1) fail("axis transform variant must be 0 or 1");
+        if (variant < 0 || variant > 2) fail("axis transform variant must be 0, 1 or 2");
         c->axis_gemm = variant;
     });
 }
@@ -3277,6 +3329,13 @@ static void grid_solve_run(srmg_ctx *c, const GridSolvePlan &g, double s, const 
     }
     hold.push_back(std::vector<double>(ngrid, op->disc == SRMG_HELMHOLTZ ? -s * op->k2 : 0.0));
     HIPCHK(hipMemcpyAsync(bufs.shift, hold.back().data(), ngrid * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (c->axis_gemm == 2 && l1_fused_ok(dims, N)) {
+        launch_l1_fused<8, double, false>(c->stream, ngrid, N, nullptr, f, nullptr, nullptr, bufs.Q, bufs.P,
+                                          bufs.lam, bufs.shift, bufs.g1, bufs.g2, nullptr, u);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return;
+    }
     const double *in = f;
     double *a = bufs.g1, *b2 = bufs.g2;
     auto axis = [&](const double *A, int d, int stride, const double *x, double *y) {
@@ -3292,13 +3351,14 @@ static void grid_solve_run(srmg_ctx *c, const GridSolvePlan &g, double s, const 
     double *cur = const_cast<double *>(in);
     hipLaunchKernelGGL(k_eig_scale, dim3(nwg(total * ngrid)), dim3(WG), 0, c->stream, cur, bufs.lam[0], bufs.lam[1],
                        dims == 3 ? bufs.lam[2] : bufs.lam[1], N[0], N[1], dims, bufs.shift, total, ngrid);
-    stride = 1;
-    for (int d = 0; d < dims; d++) {
-        double *dst = (d == dims - 1) ? u : a;
+    // inverse transforms in the reverse axis order (the last forward axis first)
+    for (int d = dims - 1; d >= 0; d--) {
+        stride = 1;
+        for (int e = 0; e < d; e++) stride *= N[e];
+        double *dst = (d == 0) ? u : a;
         axis(bufs.P[d], d, stride, cur, dst);
         cur = dst;
         std::swap(a, b2);
-        stride *= N[d];
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));

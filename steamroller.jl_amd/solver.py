@@ -374,8 +374,10 @@ class Context:
         _check(lib().srmg_set_group_tiles(self.h, int(bool(enable))))
 
     def set_axis_gemm(self, variant: int):
-        """Kernels of the level-1 axis transforms: 1 (default) one wave per
-        work group, 0 the LDS-tiled ones (the same bits)."""
+        """Kernels of the level-1 solve: 2 (default) three launches for 3D
+        grids with extents that are multiples of 8 and at most 64 (else as 1),
+        1 one launch per axis transform with one wave per work group, 0 the
+        LDS-tiled axis kernels (all give the same bits)."""
         _check(lib().srmg_set_axis_gemm(self.h, int(variant)))
 
     def set_sweep_config(self, dtype: int, level: int, cfg: int):

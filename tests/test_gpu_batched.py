@@ -204,6 +204,9 @@ def test_forest_3d_column_kernels_equal_single_solves(mixed, shape):
     got = _download(u, probs)
     for k, p in enumerate(probs):
         c1 = solver.Context(p.tree, p.conn, p.M)
+        # the forest's three-launch level-1 solve of its stacked grids against
+        # the axis-by-axis transforms of one grid
+        c1.set_axis_gemm(1)
         for l in range(1, len(p.tree) + 1):
             c1.set_sweep_config(F64, l, shape)
             c1.set_sweep_config(F32, l, shape)

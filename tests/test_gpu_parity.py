@@ -217,13 +217,14 @@ def test_vcycle_and_fmg_bitwise_exact_coarse(name):
 
 @pytest.mark.parametrize("name", ["cfg1", "cyl2-m8", "cart3-m8", "cfg4-small"])
 def test_axis_transform_variants_give_identical_solves(name):
-    """The one-wave level-1 transform kernels and the LDS-tiled ones sum the
-    same products in the same order: exact-coarse solves agree bit for bit."""
+    """The three-launch level-1 solve (variant 2, the default), the one-wave
+    transform kernels (1) and the LDS-tiled ones (0) sum the same products in
+    the same order: exact-coarse solves agree bit for bit."""
     p = configs.make_cfg4(0, scale=0.35, levels=3) if name == "cfg4-small" else CASES[name]()
     print(f"case {name}", flush=True)
     geom, disc = solver.geometry_from_code(p.geom), _disc(p)
     res = []
-    for variant in (0, 1):
+    for variant in (0, 1, 2):
         ctx, (u, b, r, u1) = _device(p)
         ctx.set_axis_gemm(variant)
         b.upload(p.b)
@@ -231,11 +232,12 @@ def test_axis_transform_variants_give_identical_solves(name):
         host = p.b.like()
         u.download(host)
         res.append((np.asarray(hist), _valid(host, p).copy()))
-    np.testing.assert_array_equal(res[1][0], res[0][0])
-    np.testing.assert_array_equal(res[1][1], res[0][1])
+    for k in (1, 2):
+        np.testing.assert_array_equal(res[k][0], res[0][0])
+        np.testing.assert_array_equal(res[k][1], res[0][1])
 
 
-@pytest.mark.parametrize("D,n", [(3, (64, 64, 64)), (3, (64, 32, 16)), (2, (64, 128))])
+@pytest.mark.parametrize("D,n", [(3, (64, 64, 64)), (3, (64, 32, 16)), (3, (16, 40, 64)), (2, (64, 128))])
 def test_axis_transform_variants_give_identical_grid_solves(D, n):
     p = CASES["cart3-m8" if D == 3 else "cyl2-m8"]()
     bc = p.bc
@@ -245,7 +247,7 @@ def test_axis_transform_variants_give_identical_grid_solves(D, n):
     g = torch.Generator(device="cpu").manual_seed(7)
     f = torch.randn((2, V), dtype=torch.float64, generator=g).cuda()
     outs = []
-    for variant in (0, 1):
+    for variant in (0, 1, 2):
         ctx = solver.Context(p.tree, p.conn, p.M)
         ctx.set_axis_gemm(variant)
         u = torch.empty_like(f)
@@ -254,6 +256,7 @@ def test_axis_transform_variants_give_identical_grid_solves(D, n):
         torch.cuda.synchronize()
         outs.append(u.cpu().numpy())
     np.testing.assert_array_equal(outs[1], outs[0])
+    np.testing.assert_array_equal(outs[2], outs[0])
 
 
 def test_cached_solve_graphs_replay_exactly():
