@@ -2784,13 +2784,14 @@ __global__ __launch_bounds__(256) void k_axis_gemm_small(const double *__restric
     }
 }
 
-// k_axis_gemm for n <= 64 with one wave per work group and no LDS; each
-// output is the k-ordered sum of products of k_axis_gemm (same bits).
-// Axis with stride 1 (a column is n contiguous values): lane i computes row i
-// of RB consecutive columns. Row k of At (= column k of A) is a coalesced load;
-// X[col][k] is the same for every lane (a scalar load).
+// k_axis_gemm for n <= 256 without LDS; each output is the k-ordered sum of
+// products of k_axis_gemm (same bits).
+// Axis with stride 1 (a column is n contiguous values): thread i computes row
+// i of RB consecutive columns (n threads rounded up to whole waves). Row k of
+// At (= column k of A) is a coalesced load; X[col][k] is the same for every
+// lane (a scalar load).
 template <int RB>
-__global__ __launch_bounds__(64) void k_axis_gemm_c(const double *__restrict__ At, int n, int ncol,
+__global__ __launch_bounds__(256) void k_axis_gemm_c(const double *__restrict__ At, int n, int ncol,
                                                     const double *__restrict__ in, double *__restrict__ out) {
     const int i = threadIdx.x;
     const int c0 = blockIdx.x * RB;

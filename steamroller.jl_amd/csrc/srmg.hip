@@ -746,23 +746,32 @@ static void upload_axis_matrix(DevBuf &buf, const std::vector<double> &A, int n)
     HIPCHK(hipMemcpy(buf.p, both.data(), 2 * nn * sizeof(double), hipMemcpyHostToDevice));
 }
 
-// one axis of the level-1 transforms (A from upload_axis_matrix). n <= 64:
-// one wave per work group and no LDS where the stride allows it (variant 1),
+// one axis of the level-1 transforms (A from upload_axis_matrix). n <= 256:
+// kernels without LDS where the stride allows it (variants 1 and 2),
 // else the LDS-tiled kernels; the column tile of the tiled kernel is narrowed
 // until the launch has at least 512 work groups. Every variant gives the same
 // bits.
 static void launch_axis_gemm(hipStream_t st, const double *A, int n, int inner, int ncol, const double *in,
                              double *out, int variant) {
-    if (n <= 64 && variant >= 1 && inner == 1) {
-        constexpr int RB = 2;
-        hipLaunchKernelGGL(k_axis_gemm_c<RB>, dim3((unsigned)((ncol + RB - 1) / RB)), dim3(64), 0, st,
-                           A + (size_t)n * n, n, ncol, in, out);
+    // two rows or columns per wave, one where that leaves fewer than 1024 waves
+    // (small grids: more waves in flight)
+    if (n <= 256 && variant >= 1 && inner == 1) {
+        const unsigned th = (unsigned)((n + 63) / 64 * 64);
+        if ((int64_t)(ncol / 2) * (th / 64) >= 1024)
+            hipLaunchKernelGGL(k_axis_gemm_c<2>, dim3((unsigned)((ncol + 1) / 2)), dim3(th), 0, st, A + (size_t)n * n, n,
+                               ncol, in, out);
+        else
+            hipLaunchKernelGGL(k_axis_gemm_c<1>, dim3((unsigned)ncol), dim3(th), 0, st, A + (size_t)n * n, n, ncol, in,
+                               out);
         return;
     }
-    if (n <= 64 && variant >= 1 && inner % 64 == 0) {
-        constexpr int RB = 2;
-        hipLaunchKernelGGL(k_axis_gemm_r<RB>, dim3((unsigned)(ncol / 64), (unsigned)((n + RB - 1) / RB)), dim3(64),
-                           0, st, A, n, inner, in, out);
+    if (n <= 256 && variant >= 1 && inner % 64 == 0) {
+        if ((int64_t)(ncol / 64) * ((n + 1) / 2) >= 1024)
+            hipLaunchKernelGGL(k_axis_gemm_r<2>, dim3((unsigned)(ncol / 64), (unsigned)((n + 1) / 2)), dim3(64), 0, st, A,
+                               n, inner, in, out);
+        else
+            hipLaunchKernelGGL(k_axis_gemm_r<1>, dim3((unsigned)(ncol / 64), (unsigned)n), dim3(64), 0, st, A, n, inner,
+                               in, out);
         return;
     }
     if (n <= 64) {

@@ -237,7 +237,8 @@ def test_axis_transform_variants_give_identical_solves(name):
         np.testing.assert_array_equal(res[k][1], res[0][1])
 
 
-@pytest.mark.parametrize("D,n", [(3, (64, 64, 64)), (3, (64, 32, 16)), (3, (16, 40, 64)), (2, (64, 128))])
+@pytest.mark.parametrize("D,n", [(3, (64, 64, 64)), (3, (64, 32, 16)), (3, (16, 40, 64)), (3, (32, 16, 128)),
+                                 (2, (64, 128)), (2, (128, 256)), (2, (96, 200))])
 def test_axis_transform_variants_give_identical_grid_solves(D, n):
     p = CASES["cart3-m8" if D == 3 else "cyl2-m8"]()
     bc = p.bc
