@@ -3128,8 +3128,10 @@ __global__ __launch_bounds__(WG) void k_add_correction(double *__restrict__ u, c
 // group's u (k_restrict's order) and e = 0. Work groups past ngroups do
 // u += e, e = 0 on the leaf blocks leaves[0..nleafc) of the coarser level
 // (used for the coarsest level, which has no parent). The values equal those
-// of u += e everywhere followed by restrict! level by level.
-template <int D, int M>
+// of u += e everywhere followed by restrict! level by level. ZERO: the leaf
+// blocks' u is known to be 0 (the first step of a solve from u = 0) and is
+// not read: u = 0.0 + e.
+template <int D, int M, bool ZERO = false>
 __global__ __launch_bounds__(256) void k_add_restrict(double *__restrict__ u, float *__restrict__ e,
                                                       const int *__restrict__ isleaf, const int *__restrict__ parent,
                                                       int blk0, unsigned ngroups, const int *__restrict__ leaves,
@@ -3141,7 +3143,7 @@ __global__ __launch_bounds__(256) void k_add_restrict(double *__restrict__ u, fl
         const int c0 = (int)(blockIdx.x - ngroups) * 256;
         for (int i = c0 + tid; i < c0 + 256 && i < nleafc * V; i += 256) {
             const size_t o = (size_t)leaves[i / V] * V + (size_t)(i % V);
-            u[o] += (double)e[o];
+            u[o] = (ZERO ? 0.0 : u[o]) + (double)e[o];
             e[o] = 0.0f;
         }
         return;
@@ -3154,10 +3156,12 @@ __global__ __launch_bounds__(256) void k_add_restrict(double *__restrict__ u, fl
     for (int i = tid; i < NB * V; i += 256) {
         const int k = i / V;
         const size_t o = (size_t)g0 * V + i;
-        double v = u[o];
+        double v;
         if (leaf[k]) {
-            v += (double)e[o];
+            v = (ZERO ? 0.0 : u[o]) + (double)e[o];
             u[o] = v;
+        } else {
+            v = u[o];
         }
         e[o] = 0.0f;
         ut[i] = v;

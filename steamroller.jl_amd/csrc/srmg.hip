@@ -1953,6 +1953,7 @@ static Graph &solve_graph(srmg_ctx *c, int kind, std::initializer_list<const Fie
     add(o->inner_cycles);
     add(o->inner_first);
     add(o->first_fmg);
+    add(o->guess);
     for (int v : c->sweep_cfg) add(v);
     add(c->group_tiles);
     add(c->axis_gemm);
@@ -2064,8 +2065,22 @@ static void solve_mixed_dm(srmg_ctx *c, FieldBuf &u, FieldBuf &b, FieldBuf &e, F
     Ops<D, M, float> o32(c);
     std::vector<double> r0;
     const bool zero_start = o->guess == SRMG_GUESS_ZERO;
+    // on trees whose refined levels are all grouped, the add, the restriction
+    // and the zeroing of e for the next step run as one launch per level
+    bool fused = (Ops<D, M, double>::GROUP_OK) && c->group_tiles && c->nlevels > 1;
+    for (int l = 1; l < c->nlevels && fused; l++) fused = c->lv[l].nblk > 0 && c->lv[l].grouped;
+    // from u = 0 with the fused add: u's cells are written by the first
+    // step's add (u = 0 + e on leaves, restrictions above) before anything
+    // reads them, so only its ghost slabs and buffer state are cleared
+    const bool zero_add = zero_start && fused;
     if (zero_start) {
-        o64.zero_all(u);
+        if (zero_add) {
+            const size_t ni = u.ib.bytes / sizeof(double);
+            if (ni) hipLaunchKernelGGL(k_fill<double>, dim3(nwg(ni)), dim3(WG), 0, c->stream, u.ib.as<double>(), 0.0, ni);
+            std::fill(u.cur.begin(), u.cur.end(), 0);
+        } else {
+            o64.zero_all(u);
+        }
         o64.bnorm_partials(b);
         r0 = safe_r0(read_norms(c));
     } else {
@@ -2075,10 +2090,6 @@ static void solve_mixed_dm(srmg_ctx *c, FieldBuf &u, FieldBuf &b, FieldBuf &e, F
         if (o->guess) o64.first_iterate(u, b, o->guess, o->guess_smooth, r0);
     }
     auto defect_and_norm = [&] { o64.defect_and_norm(be.data.as<float>(), u, b, s, op); };
-    // on trees whose refined levels are all grouped, the add, the restriction
-    // and the zeroing of e for the next step run as one launch per level
-    bool fused = (Ops<D, M, double>::GROUP_OK) && c->group_tiles && c->nlevels > 1;
-    for (int l = 1; l < c->nlevels && fused; l++) fused = c->lv[l].nblk > 0 && c->lv[l].grouped;
     const size_t ncell = (size_t)c->nb * LY::V, nib = e.ib.bytes / sizeof(float);
     if (fused)
         hipLaunchKernelGGL(k_fill<float>, dim3(nwg(ncell)), dim3(WG), 0, c->stream, e.data.as<float>(), 0.0f, ncell);
@@ -2086,7 +2097,7 @@ static void solve_mixed_dm(srmg_ctx *c, FieldBuf &u, FieldBuf &b, FieldBuf &e, F
     // correction equation before its V-cycles (first_fmg) and inner_first
     // V-cycles when that is set
     const int n_inner = std::max(1, (int)o->inner_cycles), n_first = o->inner_first > 0 ? o->inner_first : n_inner;
-    const bool first_differs = o->first_fmg || n_first != n_inner;
+    const bool first_differs = o->first_fmg || n_first != n_inner || zero_add;
     auto correct = [&](bool first) {
         if (!fused)
             hipLaunchKernelGGL(k_fill<float>, dim3(nwg(ncell)), dim3(WG), 0, c->stream, e.data.as<float>(), 0.0f,
@@ -2103,9 +2114,14 @@ static void solve_mixed_dm(srmg_ctx *c, FieldBuf &u, FieldBuf &b, FieldBuf &e, F
                 const unsigned ng = (unsigned)(L.nblk >> D);
                 const int nleafc = (l == 1) ? L0.nleaf : 0;
                 const unsigned nx = (unsigned)(((int64_t)nleafc * LY::V + 255) / 256);
-                hipLaunchKernelGGL((k_add_restrict<D, M>), dim3(ng + nx), dim3(256), 0, c->stream,
-                                   u.data.as<double>(), e.data.as<float>(), c->d_isleaf.as<int>(),
-                                   c->d_parent.as<int>(), L.blk0, ng, c->d_leaves.as<int>() + L0.leaf0, nleafc);
+                if (first && zero_add)
+                    hipLaunchKernelGGL((k_add_restrict<D, M, true>), dim3(ng + nx), dim3(256), 0, c->stream,
+                                       u.data.as<double>(), e.data.as<float>(), c->d_isleaf.as<int>(),
+                                       c->d_parent.as<int>(), L.blk0, ng, c->d_leaves.as<int>() + L0.leaf0, nleafc);
+                else
+                    hipLaunchKernelGGL((k_add_restrict<D, M>), dim3(ng + nx), dim3(256), 0, c->stream,
+                                       u.data.as<double>(), e.data.as<float>(), c->d_isleaf.as<int>(),
+                                       c->d_parent.as<int>(), L.blk0, ng, c->d_leaves.as<int>() + L0.leaf0, nleafc);
             }
             }
         }
@@ -2141,6 +2157,7 @@ static void solve_mixed_dm(srmg_ctx *c, FieldBuf &u, FieldBuf &b, FieldBuf &e, F
         k++;
         done = record(read_norms(c), r0, o->tol, k, o->max_cycles, hist);
     }
+    if (k == 0 && zero_add) o64.zero_all(u);  // no step wrote u
     if (cycles) *cycles = k;
 }
 

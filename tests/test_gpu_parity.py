@@ -634,3 +634,23 @@ def test_zero_start_equals_a_solve_from_zeroed_u(name):
     np.testing.assert_array_equal(hm1[1:], hm0[1:])
     np.testing.assert_array_equal(f1[0], f0[0])
     np.testing.assert_array_equal(f1[1], f0[1])
+
+
+@pytest.mark.parametrize("name", ["cfg1", "cart3-m8"])
+def test_mixed_zero_start_without_a_step_leaves_u_zero(name):
+    # the zero start skips clearing u when the first step writes every cell;
+    # a solve that takes no step (tolerance already met) must still return 0
+    p = CASES[name]()
+    geom, disc = solver.geometry_from_code(p.geom), _disc(p)
+    ctx = solver.Context(p.tree, p.conn, p.M)
+    v, b = ctx.field(F64), ctx.field(F64)
+    e, be, re, ue = (ctx.field(F32) for _ in range(4))
+    b.upload(p.b)
+    v.upload(p.b)  # garbage
+    for tol, cyc in ((2.0, 12), (1e-9, 0)):
+        n, h = solver.solve_mixed(v, b, e, be, re, ue, p.s, geom, p.bc, disc, tol=tol, max_cycles=cyc,
+                                  inner_cycles=2, guess=solver.GUESS_ZERO)
+        assert n == 0
+        host = p.b.like()
+        v.download(host)
+        assert not np.any(_valid(host, p)), (tol, cyc)
