@@ -2574,10 +2574,11 @@ __global__ __launch_bounds__(WG) void k_fb_setouter(double *__restrict__ q, cons
 
 // resnorm partials of u = 0 (the residual is -b): the same blocks, visiting
 // order and tree sums as k_resid_tile<RES_NORM>, so the partials are bitwise
-// those of a residual norm of a zero field, without reading one.
+// those of a residual norm of a zero field, without reading one. be (may be
+// null) receives (float) b on the visited blocks.
 template <int D, int M, typename T, int BPW, int NT>
 __global__ __launch_bounds__(NT) void k_bnorm(const T *__restrict__ b, const int *__restrict__ list, int n,
-                                              double wgt, double *__restrict__ partial) {
+                                              double wgt, double *__restrict__ partial, float *__restrict__ be) {
     constexpr int V = Lay<D, M>::V;
     __shared__ int blks[BPW];
     __shared__ double red[NT];
@@ -2590,7 +2591,10 @@ __global__ __launch_bounds__(NT) void k_bnorm(const T *__restrict__ b, const int
     for (int lb = 0; lb < BPW; lb++) acc[lb] = 0.0;
     for (int i = threadIdx.x; i < nloc * V; i += NT) {
         const int lb = i / V, q = i - lb * V;
-        const T rr = -b[(size_t)blks[lb] * V + q];
+        const size_t o = (size_t)blks[lb] * V + q;
+        const T bv = b[o];
+        if (be) be[o] = (float)bv;
+        const T rr = -bv;
 #pragma unroll
         for (int k = 0; k < BPW; k++)
             if (k == lb) acc[k] += (double)rr * (double)rr;
@@ -2605,12 +2609,6 @@ __global__ __launch_bounds__(NT) void k_bnorm(const T *__restrict__ b, const int
         if (threadIdx.x == 0) partial[first + lb] = red[0] * wgt;
         __syncthreads();
     }
-}
-
-// be = (float) b over n values (the FP64 defect of u = 0)
-__global__ __launch_bounds__(WG) void k_to_float(float *__restrict__ out, const double *__restrict__ in, size_t n) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = (float)in[i];
 }
 
 // ---- exact level-1 solve (FP64) ----

@@ -1342,8 +1342,8 @@ struct Ops {
         return np;
     }
     // resnorm partials of u = 0 from b alone (bitwise those of resnorm_partials
-    // on a zero field)
-    int bnorm_partials(FieldBuf &b) {
+    // on a zero field); be (may be null): (float) b on every leaf block
+    int bnorm_partials(FieldBuf &b, float *be = nullptr) {
         int np = 0;
         for (int l = 0; l < c->nlevels; l++) {
             const LevelInfo &L = c->lv[l];
@@ -1351,7 +1351,7 @@ struct Ops {
             double wgt = std::ldexp(1.0, -c->D * l);
             unsigned g = (unsigned)((L.nleaf + RB - 1) / RB);
             hipLaunchKernelGGL((k_bnorm<D, M, T, RB, RNT>), dim3(g), dim3(RNT), 0, c->stream, X(b, l),
-                               c->d_leaves.as<int>() + L.leaf0, L.nleaf, wgt, c->d_partial.as<double>() + np);
+                               c->d_leaves.as<int>() + L.leaf0, L.nleaf, wgt, c->d_partial.as<double>() + np, be);
             np += L.nleaf;
         }
         sum_partials(np, c->d_rng_leaf);
@@ -2081,7 +2081,11 @@ static void solve_mixed_dm(srmg_ctx *c, FieldBuf &u, FieldBuf &b, FieldBuf &e, F
         } else {
             o64.zero_all(u);
         }
-        o64.bnorm_partials(b);
+        // the FP64 defect of u = 0 is b itself (b + L 0 / s rounds to b): its
+        // FP32 copy on the leaf blocks comes with the norm's read of b (on
+        // covered blocks the correction's right-hand side is the FAS one,
+        // written before any use)
+        o64.bnorm_partials(b, be.data.as<float>());
         r0 = safe_r0(read_norms(c));
     } else {
         for (int l = 1; l < c->nlevels; l++) o64.fill_interp(u, l, level_args<double>(c, l - 1, s, op, 1.0));
@@ -2137,10 +2141,6 @@ static void solve_mixed_dm(srmg_ctx *c, FieldBuf &u, FieldBuf &b, FieldBuf &e, F
     };
     bool done;
     if (zero_start) {
-        // the FP64 defect of u = 0 is b itself (b + L 0 / s rounds to b)
-        const size_t n = (size_t)c->nb * LY::V;
-        hipLaunchKernelGGL(k_to_float, dim3(nwg(n)), dim3(WG), 0, c->stream, be.data.as<float>(), b.data.as<double>(),
-                           n);
         done = record(r0, r0, o->tol, 0, o->max_cycles, hist);
     } else {
         defect_and_norm();
