@@ -1,7 +1,8 @@
 """Launch overhead of the batched photoionization solves: the forest of the
 three Helmholtz terms on the cfg1 tree plus 64 uneven random trees (67
 problems), solved to 1e-8 twice per form (FP64; FP32 with FP64 defect
-correction); the second solve of each form replays captured graphs.
+correction, two FP32 V-cycles per correction or two in the first and three
+in the later ones); the second solve of each form replays captured graphs.
 
     python steamroller.jl_amd/forest_bench.py
 
@@ -37,7 +38,7 @@ def main():
     bh = solver.concat_fields([p.b for p in probs])
     print(f"forest: {len(probs)} problems, {sum(p.tree.nblocks() for p in probs)} blocks, "
           f"{sum(p.n_leaf for p in probs)} leaf cells", flush=True)
-    for form in ("fp64", "mixed2"):
+    for form in ("fp64", "mixed2", "mixed2_then3"):
         for rep in range(2):
             b.upload(bh)
             u.zero()
@@ -47,8 +48,10 @@ def main():
                 n, h = solver.solve_batched(u, b, r, u1, ss, geom, bc, discs, tol=1e-8, max_cycles=60,
                                             guess=solver.GUESS_ZERO)
             else:
+                first, inner = (2, 3) if form == "mixed2_then3" else (0, 2)
                 n, h = solver.solve_mixed_batched(u, b, e, be, re, ue, ss, geom, bc, discs, tol=1e-8,
-                                                  max_cycles=60, inner_cycles=2, guess=solver.GUESS_ZERO)
+                                                  max_cycles=60, inner_cycles=inner, inner_first=first,
+                                                  guess=solver.GUESS_ZERO)
             ctx.synchronize()
             ms = 1e3 * (time.perf_counter() - t)
             print(f"{form} solve {rep + 1}: {n} cycles, {ms:.3f} ms, worst final {h[:, -1].max():.3e}", flush=True)
