@@ -291,7 +291,10 @@ def batched_cfg3(log, ntrees=64):
     cfg1 tree plus `ntrees` uneven random trees (seeds 0..ntrees-1, 1e4-1e6
     cells, k^2 cycling over the three terms), solved to 1e-8 in one forest
     (every launch covers every tree) and one by one in their own contexts;
-    FP64 and FP32 with FP64 residual. Each timing is the second of two runs."""
+    FP64 and FP32 with FP64 residual. The forest's sweep tile shapes are
+    autotuned first; the single contexts keep the default shapes (tuned on
+    their few-microsecond launches they came out slower). Each timing is the
+    second of two runs."""
     import numpy as np
     from srmg import configs, solver
     from srmg.solver import F32, F64
@@ -314,6 +317,13 @@ def batched_cfg3(log, ntrees=64):
         u, b, r, u1 = (ctx.field(F64) for _ in range(4))
         e, be, re, ue = (ctx.field(F32) for _ in range(4))
         bh = solver.concat_fields([p.b for p in probs])
+        # sweep tile shapes per level, outside the timed solves
+        if mixed:
+            be.upload(bh)
+            ctx.autotune(e, be, ss[0], geom, bc, discs[0], reps=4)
+        else:
+            b.upload(bh)
+            ctx.autotune(u, b, ss[0], geom, bc, discs[0], reps=4)
 
         def run_forest():
             b.upload(bh)

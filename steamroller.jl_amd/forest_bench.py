@@ -4,12 +4,13 @@ problems), solved to 1e-8 twice per form (FP64; FP32 with FP64 defect
 correction, two FP32 V-cycles per correction or two in the first and three
 in the later ones); the second solve of each form replays captured graphs.
 
-    python steamroller.jl_amd/forest_bench.py
+    python steamroller.jl_amd/forest_bench.py [--autotune]
 
 Prints the host-timed milliseconds of the second solve of each form. Under
 rocprofv3 --kernel-trace the kernel durations of the same solves give the
 device busy time; the difference is what launching costs, i.e. the most a
-single persistent kernel per smoother step could save.
+single persistent kernel per smoother step could save. --autotune picks the
+sweep tile shapes per level (FP64 and FP32) before the solves.
 """
 
 import os
@@ -36,6 +37,12 @@ def main():
     u, b, r, u1 = (ctx.field(F64) for _ in range(4))
     e, be, re, ue = (ctx.field(F32) for _ in range(4))
     bh = solver.concat_fields([p.b for p in probs])
+    if "--autotune" in sys.argv[1:]:
+        b.upload(bh)
+        t64 = ctx.autotune(u, b, ss[0], geom, bc, discs[0], reps=4)
+        be.upload(bh)
+        t32 = ctx.autotune(e, be, ss[0], geom, bc, discs[0], reps=4)
+        print(f"tile shapes per level: fp64 {t64.tolist()} fp32 {t32.tolist()}", flush=True)
     print(f"forest: {len(probs)} problems, {sum(p.tree.nblocks() for p in probs)} blocks, "
           f"{sum(p.n_leaf for p in probs)} leaf cells", flush=True)
     for form in ("fp64", "mixed2", "mixed2_then3"):

@@ -307,6 +307,7 @@ class Context:
             _check(lib().srmg_set_forest(self.h, len(trees_conns), _p(nl_a), _p(nb_a), *ptrs))
         self.ntrees = len(trees_conns)
         self.trees = [t for t, _ in trees_conns]
+        self.nlevels = int(nl_a.max())
         self.nblocks = int(nb_a.sum())
 
     def regrid(self, tree, conn, fields=(), bc=None, nblocks: int | None = None):
@@ -342,6 +343,7 @@ class Context:
         self.last_regrid_ms = 1e3 * (time.perf_counter() - t0)
         self.tree, self.conn = tree, conn
         self.ntrees = 1
+        self.nlevels = len(tree)
         self.nblocks = nblocks
 
     def level_info(self, level: int):
@@ -359,9 +361,10 @@ class Context:
         return ms.value
 
     def autotune(self, u, b, s, geometry, bc, disc, reps=5):
-        """Pick the fastest fused-sweep tile shape per level (times the sweeps on u)."""
+        """Pick the fastest fused-sweep tile shape per level (times the sweeps on u;
+        on a forest every tree takes s and disc for the trial sweeps)."""
         op = make_op(geometry, bc, disc)
-        out = np.zeros(len(self.tree), np.int32)
+        out = np.zeros(self.nlevels, np.int32)
         _check(lib().srmg_autotune(self.h, u.id, b.id, float(s), C.byref(op), int(reps), _p(out)))
         return out
 
