@@ -1890,29 +1890,46 @@ static void from_device_layout(srmg_ctx *c, const std::vector<TD> &data, const s
 // solve drivers
 // ---------------------------------------------------------------------------
 
-// Run `body` (which only enqueues kernels on c->stream) either directly or via
-// a graph captured on its first use.
+// Record the launches of `body` (which only enqueues kernels on c->stream)
+// into gr without running them.
+template <class F>
+static void capture_graph(srmg_ctx *c, Graph &gr, F &&body) {
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    try {
+        body();
+    } catch (...) {
+        hipGraph_t tmp = nullptr;
+        (void)hipStreamEndCapture(c->stream, &tmp);
+        if (tmp) (void)hipGraphDestroy(tmp);
+        throw;
+    }
+    HIPCHK(hipStreamEndCapture(c->stream, &gr.g));
+    HIPCHK(hipGraphInstantiate(&gr.x, gr.g, nullptr, nullptr, 0));
+    gr.gen = c->graph_gen;
+}
+
+// Run `body` either directly or via a graph captured on its first use.
 template <class F>
 static void run_maybe_graph(srmg_ctx *c, Graph &gr, bool use_graph, F &&body) {
     if (!use_graph) {
         body();
         return;
     }
-    if (!gr.x) {
-        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        try {
-            body();
-        } catch (...) {
-            hipGraph_t tmp = nullptr;
-            (void)hipStreamEndCapture(c->stream, &tmp);
-            if (tmp) (void)hipGraphDestroy(tmp);
-            throw;
-        }
-        HIPCHK(hipStreamEndCapture(c->stream, &gr.g));
-        HIPCHK(hipGraphInstantiate(&gr.x, gr.g, nullptr, nullptr, 0));
-        gr.gen = c->graph_gen;
-    }
+    if (!gr.x) capture_graph(c, gr, body);
     HIPCHK(hipGraphLaunch(gr.x, c->stream));
+}
+
+// A solve's first step when it differs from the later ones (fmg! first, a
+// zero start, another FP32 cycle count): run eagerly the first time, so that
+// lazily built state exists, then recorded at once (a step leaves the host
+// state as it found it, so recording it again changes nothing), so that the
+// next solve replays it from its first step.
+template <class F>
+static void run_first_step(srmg_ctx *c, Graph &g, bool use_graph, F &&body) {
+    const bool replay = use_graph && (g.ran || g.x);
+    run_maybe_graph(c, g, replay, body);
+    if (use_graph && !replay && !g.x) capture_graph(c, g, body);
+    g.ran = true;
 }
 
 // The cached graph of a solve step: the key holds every value the step's
@@ -2040,12 +2057,17 @@ static void solve_plain(srmg_ctx *c, int ui, int bi, int ri, int u1i, const srmg
             // cycle) is captured on its second use
             const bool f = o->first_fmg && k == 0;
             Graph &g = f ? gf : gr;
-            run_maybe_graph(c, g, o->use_graph && (f ? (g.ran || g.x) : (k > 0 || g.x)), [&] {
+            auto step = [&] {
                 if (f) ops.fmg(u, b, r, u1, s, op, o->ndown, o->ntop, o->nup, o->omega);
                 else ops.vcycle(u, b, r, u1, s, op, 0, o->ndown, o->ntop, o->nup, o->omega);
                 ops.resnorm_partials(u, b, s, op);
-            });
-            g.ran = true;
+            };
+            if (f) {
+                run_first_step(c, g, o->use_graph, step);
+            } else {
+                run_maybe_graph(c, g, o->use_graph && (k > 0 || g.x), step);
+                g.ran = true;
+            }
             k++;
             done = record(read_norms(c), r0, o->tol, k, o->max_cycles, hist);
         }
@@ -2152,8 +2174,13 @@ static void solve_mixed_dm(srmg_ctx *c, FieldBuf &u, FieldBuf &b, FieldBuf &e, F
     while (k < o->max_cycles && !done) {
         const bool f = first_differs && k == 0;
         Graph &g = f ? gf : gr;
-        run_maybe_graph(c, g, o->use_graph && (f ? (g.ran || g.x) : (k > 0 || g.x)), [&] { correct(f); });
-        g.ran = true;
+        auto step = [&] { correct(f); };
+        if (f) {
+            run_first_step(c, g, o->use_graph, step);
+        } else {
+            run_maybe_graph(c, g, o->use_graph && (k > 0 || g.x), step);
+            g.ran = true;
+        }
         k++;
         done = record(read_norms(c), r0, o->tol, k, o->max_cycles, hist);
     }

@@ -224,3 +224,26 @@ def test_forest_3d_column_kernels_equal_single_solves(mixed, shape):
         ref = p.b.like()
         v.download(ref)
         np.testing.assert_array_equal(_valid(got[k], p), _valid(ref.u, p))
+
+
+def test_forest_autotune_covers_every_level_and_keeps_results():
+    # the forest's first tree is the shallowest: autotune reports one shape per
+    # level of the deepest tree, and tuned shapes give the same bits
+    probs = sorted(_problems(n_random=3), key=lambda p: len(p.tree))
+    assert len(probs[0].tree) < max(len(p.tree) for p in probs)
+    geom = solver.geometry_from_code(probs[0].geom)
+    discs = [_disc(p) for p in probs]
+    outs = []
+    for tune in (False, True):
+        ctx, (u, b, r, u1) = _forest(probs)
+        if tune:
+            shapes = ctx.autotune(u, b, probs[0].s, geom, probs[0].bc, discs[0], reps=2)
+            print(f"tuned shapes {shapes.tolist()}", flush=True)
+            assert len(shapes) == ctx.nlevels == max(len(p.tree) for p in probs)
+            b.upload(solver.concat_fields([p.b for p in probs]))
+        n, hist = solver.solve_batched(u, b, r, u1, [p.s for p in probs], geom, probs[0].bc, discs, tol=1e-30,
+                                       max_cycles=3)
+        outs.append((hist.copy(), [_valid(g, p).copy() for g, p in zip(_download(u, probs), probs)]))
+    np.testing.assert_array_equal(outs[1][0], outs[0][0])
+    for a, c in zip(outs[1][1], outs[0][1]):
+        np.testing.assert_array_equal(a, c)
