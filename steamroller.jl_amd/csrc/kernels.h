@@ -1100,16 +1100,24 @@ struct RecFace {
     }
 
     // operands outside the tile: one layer deeper in the neighbour, its
-    // right-hand side, and its ghosts across its side faces
+    // right-hand side, and its ghosts across its side faces. ZIN: the field
+    // and its ghost slabs are zero (only the right-hand side is read).
+    template <bool ZIN = false>
     __device__ static __forceinline__ void prefetch(int dir, int tq, int e, const int *gs, const T *__restrict__ x,
                                                     const T *__restrict__ b, const T *__restrict__ ib, T &rd, T &rb,
                                                     T *rg) {
         int cB[3];
         cell(dir, tq, cB);
+        rb = b[(size_t)e * LY::V + LY::sidx(cB[0], cB[1], cB[2])];
+        if constexpr (ZIN) {
+            rd = T(0);
+#pragma unroll
+            for (int k = 0; k < NS; k++) rg[k] = T(0);
+            return;
+        }
         int c2[3] = {cB[0], cB[1], cB[2]};
         c2[DIM] = dir > 0 ? 1 : M - 2;
         rd = x[(size_t)e * LY::V + LY::sidx(c2[0], c2[1], c2[2])];
-        rb = b[(size_t)e * LY::V + LY::sidx(cB[0], cB[1], cB[2])];
         // one guarded load per side face: the neighbour's cell (same level)
         // or the frozen slab (coarse-fine); nothing at a domain boundary or
         // off the edge of the face
@@ -1642,7 +1650,7 @@ struct ColSweepZ {
     static constexpr int SY = TL::stride(1), SZ = TL::stride(2);
     using Smem = typename CS::Smem;
 
-    template <int DIM>
+    template <int DIM, bool ZIN = false>
     __device__ static __forceinline__ void rec_pre(Smem &sm, int q, const T *__restrict__ ui, const T *__restrict__ b,
                                                    const T *__restrict__ ib, int &re, T &rd, T &rb, T *rg) {
         constexpr int DA = DIM == 0 ? 1 : 0, DB = DIM == 2 ? 1 : 2;
@@ -1652,8 +1660,8 @@ struct ColSweepZ {
         re = tb[f];
         rd = rb = T(0);
         if (re >= 0)
-            RecFace<D, M, T, DIM, TL>::prefetch(rside ? 1 : -1, rtq, re, tb + HT::ND + f * HT::NS, ui, b, ib, rd, rb,
-                                                rg);
+            RecFace<D, M, T, DIM, TL>::template prefetch<ZIN>(rside ? 1 : -1, rtq, re, tb + HT::ND + f * HT::NS, ui,
+                                                              b, ib, rd, rb, rg);
     }
     template <int DIM>
     __device__ static __forceinline__ void rec_cmp(Smem &sm, int q, const int *__restrict__ coord,
@@ -1669,7 +1677,10 @@ struct ColSweepZ {
 
     // VAR bit 0: the column goes out with nontemporal stores; bit 1: the
     // block's own right-hand side is read with nontemporal loads (both keep
-    // L2 for the neighbours' face cells, which other work groups re-read)
+    // L2 for the neighbours' face cells, which other work groups re-read);
+    // bit 2: the input field and its ghost slabs are zero on this level (the
+    // first sweep of a correction solve): nothing of it is read, the sweep
+    // gives the bits of the same sweep over zeros
     template <int P, int VAR>
     __device__ static __forceinline__ void run(Smem &sm, T *__restrict__ uo, const T *__restrict__ ui,
                                                const T *__restrict__ b, const T *__restrict__ ib,
@@ -1680,10 +1691,11 @@ struct ColSweepZ {
         const int Y = 8 * (wc >> 1) + (lane >> 3), X = 2 * (lane & 7) + ((Y + P) & 1);
         const int x = X & 7, y = Y & 7, kb = ((X >> 3) | ((Y >> 3) << 1)) | (wz << 2);
         const size_t g0 = (size_t)(blk0 + kb) * V + (size_t)((x >> 1) + LY::MH * y);
+        constexpr bool ZIN = (VAR & 4) != 0;
         T uc[M], bc[M];
         static_for<M>([&](auto zc) {
             constexpr int z = decltype(zc)::value;
-            uc[z] = ui[g0 + CS::template zoff<P, z>()];
+            uc[z] = ZIN ? T(0) : ui[g0 + CS::template zoff<P, z>()];
             if constexpr (VAR & 2) bc[z] = __builtin_nontemporal_load(&b[g0 + CS::template zoff<P, z>()]);
             else bc[z] = b[g0 + CS::template zoff<P, z>()];
         });
@@ -1706,8 +1718,12 @@ struct ColSweepZ {
             hF[j] = F;
             hk[j] = k;
             T v = T(0), m = T(1);
-            SRMG_FACE_SWITCH(F, (v = halo_value<D, M, T, FF>(e, blk0 + k, lane, ui, ib, T(1)),
-                                 m = e == NB_BOUNDARY ? a.sg[FF] : T(1)));
+            if constexpr (ZIN) {
+                SRMG_FACE_SWITCH(F, (m = e == NB_BOUNDARY ? a.sg[FF] : T(1)));
+            } else {
+                SRMG_FACE_SWITCH(F, (v = halo_value<D, M, T, FF>(e, blk0 + k, lane, ui, ib, T(1)),
+                                     m = e == NB_BOUNDARY ? a.sg[FF] : T(1)));
+            }
             hv[j] = v;
             hm[j] = m;
         }
@@ -1715,10 +1731,10 @@ struct ColSweepZ {
         int re[2];
         T rd[2], rb[2], rg[2][4];
         if (w < 4) {
-            rec_pre<0>(sm, tid, ui, b, ib, re[0], rd[0], rb[0], rg[0]);
-            rec_pre<2>(sm, tid, ui, b, ib, re[1], rd[1], rb[1], rg[1]);
+            rec_pre<0, ZIN>(sm, tid, ui, b, ib, re[0], rd[0], rb[0], rg[0]);
+            rec_pre<2, ZIN>(sm, tid, ui, b, ib, re[1], rd[1], rb[1], rg[1]);
         } else {
-            rec_pre<1>(sm, tid - 256, ui, b, ib, re[0], rd[0], rb[0], rg[0]);
+            rec_pre<1, ZIN>(sm, tid - 256, ui, b, ib, re[0], rd[0], rb[0], rg[0]);
         }
         // the tile
         const int pc = TL::at(X, Y, 8 * wz);
@@ -3128,8 +3144,10 @@ __global__ __launch_bounds__(WG) void k_add_correction(double *__restrict__ u, c
 // (used for the coarsest level, which has no parent). The values equal those
 // of u += e everywhere followed by restrict! level by level. ZERO: the leaf
 // blocks' u is known to be 0 (the first step of a solve from u = 0) and is
-// not read: u = 0.0 + e.
-template <int D, int M, bool ZERO = false>
+// not read: u = 0.0 + e. KEEPE: e is not cleared on the group's own blocks
+// (the next correction's first sweep there reads none of it; the parents'
+// e is cleared).
+template <int D, int M, bool ZERO = false, bool KEEPE = false>
 __global__ __launch_bounds__(256) void k_add_restrict(double *__restrict__ u, float *__restrict__ e,
                                                       const int *__restrict__ isleaf, const int *__restrict__ parent,
                                                       int blk0, unsigned ngroups, const int *__restrict__ leaves,
@@ -3161,7 +3179,7 @@ __global__ __launch_bounds__(256) void k_add_restrict(double *__restrict__ u, fl
         } else {
             v = u[o];
         }
-        e[o] = 0.0f;
+        if (!KEEPE) e[o] = 0.0f;
         ut[i] = v;
     }
     __syncthreads();

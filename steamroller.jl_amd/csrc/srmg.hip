@@ -942,13 +942,36 @@ struct Ops {
     // 8: sibling groups, two threads per column; 9, 10, 11: shape 8 with nontemporal
     // stores, nontemporal right-hand-side loads, both (levels that are not grouped, and group sizes the thread
     // count does not divide, use shape 0)
-    void sweep(FieldBuf &u, FieldBuf &b, const LevelArgs<T> &a, int l, int cfg = -1) {
+    // the tile shape a sweep of level l runs
+    int sweep_shape(int l, int cfg = -1) const {
+        if (cfg < 0) cfg = c->sweep_cfg[(std::is_same<T, double>::value ? 0 : c->nlevels) + l];
+        if (cfg >= 4 && !(GROUP_OK && c->group_tiles && c->lv[l].grouped)) cfg = 0;
+        return cfg;
+    }
+    // the sweep of level l can take a zero input without reading it (the
+    // two-threads-per-column shapes of 3D 8^3 Cartesian blocks)
+    bool zin_ok(int l, int rdim) const {
+        const int cfg = sweep_shape(l);
+        return D == 3 && M == 8 && rdim < 0 && cfg >= 8 && cfg <= 11;
+    }
+    // zin: u and its ghost slabs are zero on level l (only with zin_ok(l))
+    void sweep(FieldBuf &u, FieldBuf &b, const LevelArgs<T> &a, int l, int cfg = -1, bool zin = false) {
         if (!a.nblk) return;
         if (!u.alt.p || u.alt.bytes != u.data.bytes) u.alt.alloc(u.data.bytes);
         T *in = X(u, l);
         T *out = u.cur[l] ? u.data.as<T>() : u.alt.as<T>();
-        if (cfg < 0) cfg = c->sweep_cfg[(std::is_same<T, double>::value ? 0 : c->nlevels) + l];
-        if (cfg >= 4 && !(GROUP_OK && grouped(l))) cfg = 0;
+        cfg = sweep_shape(l, cfg);
+        if (zin) {
+            if (!zin_ok(l, a.rdim)) fail("internal: zero-input sweep on a tile shape without it");
+            switch (cfg) {
+                case 8: sweep_group_launch<512, 2, 4>(out, in, u, b, a, l); break;
+                case 9: sweep_group_launch<512, 2, 5>(out, in, u, b, a, l); break;
+                case 10: sweep_group_launch<512, 2, 6>(out, in, u, b, a, l); break;
+                default: sweep_group_launch<512, 2, 7>(out, in, u, b, a, l); break;
+            }
+            u.cur[l] ^= 1;
+            return;
+        }
         switch (cfg) {
             case 1: sweep_launch<1>(out, in, u, b, a, l); break;
             case 2: sweep_launch<2>(out, in, u, b, a, l); break;
@@ -966,8 +989,8 @@ struct Ops {
         u.cur[l] ^= 1;
     }
     // gauss_seidel_iter! (src/multigrid.jl:250-265)
-    void smooth(FieldBuf &u, FieldBuf &b, const LevelArgs<T> &a, int l, int n) {
-        for (int i = 0; i < n; i++) sweep(u, b, a, l);
+    void smooth(FieldBuf &u, FieldBuf &b, const LevelArgs<T> &a, int l, int n, bool zin = false) {
+        for (int i = 0; i < n; i++) sweep(u, b, a, l, -1, zin && i == 0);
     }
     // residual kernels from LDS tiles (base tile shape)
     static constexpr int RB = SMALL ? 256 / LY::H : 1;
@@ -1425,13 +1448,15 @@ struct Ops {
     // needs no work: its interpolation ghosts come from level l-2, unchanged
     // since they were last filled. Levels whose sweep count is odd are copied
     // back at the end so that every cycle starts from the same buffers.
+    // zin: u and its ghost slabs are zero on the finest level lmax - 1 (its
+    // first sweep reads none of them; only with zin_ok)
     void vcycle(FieldBuf &u, FieldBuf &b, FieldBuf &r, FieldBuf &u1, double s, const srmg_op *op, int lmax,
-                int ndown, int ntop, int nup, double omega) {
+                int ndown, int ntop, int nup, double omega, bool zin = false) {
         if (lmax == 0) lmax = c->nlevels;
         for (int l = lmax - 1; l >= 1; l--) {
             LevelArgs<T> a = level_args<T>(c, l, s, op, omega);
             LevelArgs<T> ac = level_args<T>(c, l - 1, s, op, omega);
-            smooth(u, b, a, l, ndown);
+            smooth(u, b, a, l, ndown, zin && l == lmax - 1);
             if (!resid_restrict(r, u, &u1, b, a, l)) {
                 residual(r, u, b, a, l);
                 restrict_(u, l);
@@ -2124,6 +2149,12 @@ static void solve_mixed_dm(srmg_ctx *c, FieldBuf &u, FieldBuf &b, FieldBuf &e, F
     // V-cycles when that is set
     const int n_inner = std::max(1, (int)o->inner_cycles), n_first = o->inner_first > 0 ? o->inner_first : n_inner;
     const bool first_differs = o->first_fmg || n_first != n_inner || zero_add;
+    // the first sweep of each correction solve on the finest level starts from
+    // e = 0 there: with the fused add the sweep reads none of e (and the add
+    // leaves e on that level as it is)
+    const int Lf = c->nlevels - 1;
+    const int rdim = op->geometry == SRMG_CYL1 ? 0 : (op->geometry == SRMG_CYL2 ? 1 : -1);
+    const bool zin = fused && o32.zin_ok(Lf, rdim);
     auto correct = [&](bool first) {
         if (!fused)
             hipLaunchKernelGGL(k_fill<float>, dim3(nwg(ncell)), dim3(WG), 0, c->stream, e.data.as<float>(), 0.0f,
@@ -2131,7 +2162,8 @@ static void solve_mixed_dm(srmg_ctx *c, FieldBuf &u, FieldBuf &b, FieldBuf &e, F
         hipLaunchKernelGGL(k_fill<float>, dim3(nwg(nib)), dim3(WG), 0, c->stream, e.ib.as<float>(), 0.0f, nib);
         if (first && o->first_fmg) o32.fmg(e, be, re, ue, s, op, o->ndown, o->ntop, o->nup, o->omega);
         for (int k = 0; k < (first ? n_first : n_inner); k++)
-            o32.vcycle(e, be, re, ue, s, op, 0, o->ndown, o->ntop, o->nup, o->omega);
+            o32.vcycle(e, be, re, ue, s, op, 0, o->ndown, o->ntop, o->nup, o->omega,
+                       zin && k == 0 && !(first && o->first_fmg));
         if constexpr (Ops<D, M, double>::GROUP_OK) {
             if (fused) {
             // u += e, e = 0 and the restriction of u, finest level first
@@ -2140,14 +2172,21 @@ static void solve_mixed_dm(srmg_ctx *c, FieldBuf &u, FieldBuf &b, FieldBuf &e, F
                 const unsigned ng = (unsigned)(L.nblk >> D);
                 const int nleafc = (l == 1) ? L0.nleaf : 0;
                 const unsigned nx = (unsigned)(((int64_t)nleafc * LY::V + 255) / 256);
-                if (first && zero_add)
-                    hipLaunchKernelGGL((k_add_restrict<D, M, true>), dim3(ng + nx), dim3(256), 0, c->stream,
-                                       u.data.as<double>(), e.data.as<float>(), c->d_isleaf.as<int>(),
-                                       c->d_parent.as<int>(), L.blk0, ng, c->d_leaves.as<int>() + L0.leaf0, nleafc);
-                else
-                    hipLaunchKernelGGL((k_add_restrict<D, M>), dim3(ng + nx), dim3(256), 0, c->stream,
-                                       u.data.as<double>(), e.data.as<float>(), c->d_isleaf.as<int>(),
-                                       c->d_parent.as<int>(), L.blk0, ng, c->d_leaves.as<int>() + L0.leaf0, nleafc);
+                // the finest level's e stays for the next correction's
+                // zero-input first sweep
+                auto add = [&](auto zero, auto keep) {
+                    hipLaunchKernelGGL((k_add_restrict<D, M, decltype(zero)::value, decltype(keep)::value>),
+                                       dim3(ng + nx), dim3(256), 0, c->stream, u.data.as<double>(),
+                                       e.data.as<float>(), c->d_isleaf.as<int>(), c->d_parent.as<int>(), L.blk0, ng,
+                                       c->d_leaves.as<int>() + L0.leaf0, nleafc);
+                };
+                using F_ = std::false_type;
+                using T_ = std::true_type;
+                const bool zr = first && zero_add, kp = zin && l == Lf;
+                if (zr && kp) add(T_(), T_());
+                else if (zr) add(T_(), F_());
+                else if (kp) add(F_(), T_());
+                else add(F_(), F_());
             }
             }
         }
