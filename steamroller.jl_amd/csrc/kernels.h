@@ -2856,6 +2856,66 @@ __global__ __launch_bounds__(64) void k_axis_gemm_r(const double *__restrict__ A
         if (i0 + r < n) y[(size_t)(i0 + r) * inner] = acc[r];
 }
 
+// k_axis_gemm for 64 < n <= 256 with the whole k range of a 16 x 16 output
+// tile in LDS: the tile's 16 rows of A and 16 columns of X are loaded at once
+// (every load in flight before the first product), then each thread sums its
+// output over k in increasing order from 0.0 (the bits of k_axis_gemm). One
+// load round trip per launch instead of one per k chunk or per few k.
+constexpr int AXF = 16;  // output rows and columns per work group
+__global__ __launch_bounds__(256) void k_axis_gemm_full(const double *__restrict__ A, int n, int inner, int ncol,
+                                                        const double *__restrict__ in, double *__restrict__ out) {
+    __shared__ double As[AXF * 257];  // As[r][k], row stride n + 1
+    __shared__ double Xs[256 * (AXF + 1)];  // Xs[k][c]
+    const int tid = threadIdx.x;
+    const int i0 = blockIdx.y * AXF, c0 = blockIdx.x * AXF;
+    const int sa = n + 1;
+    double va[16], vx[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const int e = tid + q * 256, r = e / n, k = e - r * n;
+        va[q] = (r < AXF && i0 + r < n) ? A[(size_t)(i0 + r) * n + k] : 0.0;
+    }
+    // X tile: element (k, c) of column c0 + c; consecutive threads take
+    // consecutive k for the stride-1 axis and consecutive columns otherwise
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const int e = tid + q * 256;
+        int k, c;
+        if (inner == 1) { c = e / n; k = e - c * n; }
+        else { k = e / AXF; c = e - k * AXF; }
+        const int col = c0 + c;
+        double v = 0.0;
+        if (k < n && c < AXF && col < ncol) {
+            const size_t o = (size_t)(col / inner), jj = (size_t)(col % inner);
+            v = in[(o * (size_t)n + k) * inner + jj];
+        }
+        vx[q] = v;
+    }
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const int e = tid + q * 256, r = e / n, k = e - r * n;
+        if (r < AXF) As[r * sa + k] = va[q];
+    }
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const int e = tid + q * 256;
+        int k, c;
+        if (inner == 1) { c = e / n; k = e - c * n; }
+        else { k = e / AXF; c = e - k * AXF; }
+        if (k < n && c < AXF) Xs[k * (AXF + 1) + c] = vx[q];
+    }
+    __syncthreads();
+    const int r = tid / AXF, c = tid - r * AXF;
+    const int i = i0 + r, col = c0 + c;
+    double acc = 0.0;
+#pragma unroll 8
+    for (int k = 0; k < n; k++) acc += As[r * sa + k] * Xs[k * (AXF + 1) + c];
+    if (i < n && col < ncol) {
+        const size_t o = (size_t)(col / inner), jj = (size_t)(col % inner);
+        out[(o * (size_t)n + i) * inner + jj] = acc;
+    }
+}
+
 // divide by the eigenvalues of the Kronecker-sum operator; the grid holds
 // `ngrid` stacked level-1 grids of `total` points, grid g with shift[g]
 // (Helmholtz: -s k^2; Laplacian: 0)

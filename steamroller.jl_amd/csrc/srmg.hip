@@ -753,6 +753,11 @@ static void upload_axis_matrix(DevBuf &buf, const std::vector<double> &A, int n)
 // bits.
 static void launch_axis_gemm(hipStream_t st, const double *A, int n, int inner, int ncol, const double *in,
                              double *out, int variant) {
+    if (n > 64 && n <= 256 && variant >= 1) {
+        hipLaunchKernelGGL(k_axis_gemm_full, dim3((unsigned)((ncol + AXF - 1) / AXF), (unsigned)((n + AXF - 1) / AXF)),
+                           dim3(256), 0, st, A, n, inner, ncol, in, out);
+        return;
+    }
     // two rows or columns per wave, one where that leaves fewer than 1024 waves
     // (small grids: more waves in flight)
     if (n <= 256 && variant >= 1 && inner == 1) {
