@@ -1491,8 +1491,12 @@ struct Ops {
         hipLaunchKernelGGL((k_efield<D, M, T>), dim3((unsigned)nblk), dim3(256), 0, c->stream, eabs, ef, u, ib,
                            c->d_halo.as<int>(), args);
     }
-    // first iterate of a solve from the level-1 problem (SRMG_GUESS_*)
-    void first_iterate(FieldBuf &u, FieldBuf &b, int mode, int nsmooth, const std::vector<double> &r0);
+    // first iterate of a solve from the level-1 problem (SRMG_GUESS_*); check:
+    // its residual norm decides here whether a tree falls back to u = 0
+    // (otherwise the caller does that with fall_back from a norm of its own)
+    void first_iterate(FieldBuf &u, FieldBuf &b, int mode, int nsmooth, const std::vector<double> &r0,
+                       bool check = true);
+    int fall_back(FieldBuf &u, const std::vector<double> &rg, const std::vector<double> &r0);
 
     // fmg! (src/multigrid.jl:17-51)
     void fmg(FieldBuf &u, FieldBuf &b, FieldBuf &r, FieldBuf &u1, double s, const srmg_op *op, int ndown, int ntop,
@@ -1536,7 +1540,8 @@ struct Ops {
 // Trees whose residual is then not finite or larger than with u = 0 restart
 // from zero.
 template <int D, int M, typename T>
-void Ops<D, M, T>::first_iterate(FieldBuf &u, FieldBuf &b, int mode, int nsmooth, const std::vector<double> &r0) {
+void Ops<D, M, T>::first_iterate(FieldBuf &u, FieldBuf &b, int mode, int nsmooth, const std::vector<double> &r0,
+                                  bool check) {
     const double s = c->ps[0];
     const srmg_op *op = &c->pops[0];
     const int nt = (int)c->trees.size();
@@ -1555,8 +1560,18 @@ void Ops<D, M, T>::first_iterate(FieldBuf &u, FieldBuf &b, int mode, int nsmooth
     }
     settle(u);
     const size_t nall = (size_t)c->nb * LY::V;
-    hipLaunchKernelGGL((k_copy<T>), dim3(nwg(nall)), dim3(WG), 0, c->stream, u.data.as<T>(), X(b, 0), nall);
-    for (int l = c->nlevels - 1; l >= 1; l--) restrict_(u, l);
+    // b restricted to level 1: b on the coarser levels, the finest level's
+    // restriction read from b itself (levels are contiguous, finest last)
+    const int Lt = c->nlevels - 1;
+    const size_t oL = Lt >= 1 ? (size_t)c->lv[Lt].blk0 * LY::V : nall;
+    hipLaunchKernelGGL((k_copy<T>), dim3(nwg(oL)), dim3(WG), 0, c->stream, u.data.as<T>(), X(b, 0), oL);
+    if (Lt >= 1 && c->lv[Lt].nblk) {
+        const LevelInfo &L = c->lv[Lt];
+        hipLaunchKernelGGL((k_restrict<D, M, T>), dim3(nwg((int64_t)L.nblk * (LY::V / LY::NC))), dim3(WG), 0,
+                           c->stream, X(u, Lt - 1), X(b, Lt), c->d_parent.as<int>(), c->d_sub.as<int>(), L.blk0,
+                           L.nblk);
+    }
+    for (int l = Lt - 1; l >= 1; l--) restrict_(u, l);
     // levels >= 2 and the ghost slabs start at zero
     const size_t o1 = (size_t)c->lv[0].nblk * LY::V, ni = u.ib.bytes / sizeof(T);
     if (nall > o1)
@@ -1589,10 +1604,19 @@ void Ops<D, M, T>::first_iterate(FieldBuf &u, FieldBuf &b, int mode, int nsmooth
     for (int l = 1; l < c->nlevels; l++) interp_fill(u, u, l, level_args<T>(c, l - 1, s, op, 1.0));
     for (int l = c->nlevels - 1; l >= 1; l--) restrict_(u, l);
     for (int l = 1; l < c->nlevels; l++) fill_interp(u, l, level_args<T>(c, l - 1, s, op, 1.0));
+    if (!check) return;
     resnorm_partials(u, b, s, op);
-    std::vector<double> rg = read_norms(c);
+    fall_back(u, read_norms(c), r0);
+}
+
+// u = 0 on every tree whose first-iterate residual rg is not finite or exceeds
+// resnorm(u = 0); returns the number of such trees (also in guess_fallbacks)
+template <int D, int M, typename T>
+int Ops<D, M, T>::fall_back(FieldBuf &u, const std::vector<double> &rg, const std::vector<double> &r0) {
+    const double s = c->ps[0];
+    const srmg_op *op = &c->pops[0];
     c->guess_fallbacks = 0;
-    for (int t = 0; t < nt; t++) {
+    for (int t = 0; t < (int)c->trees.size(); t++) {
         if (std::isfinite(rg[t]) && rg[t] <= r0[t]) continue;
         c->guess_fallbacks++;
         const TreeInfo &ti = c->trees[t];
@@ -1605,6 +1629,7 @@ void Ops<D, M, T>::first_iterate(FieldBuf &u, FieldBuf &b, int mode, int nsmooth
     }
     if (c->guess_fallbacks)
         for (int l = 1; l < c->nlevels; l++) fill_interp(u, l, level_args<T>(c, l - 1, s, op, 1.0));
+    return c->guess_fallbacks;
 }
 
 // ---------------------------------------------------------------------------
@@ -2141,9 +2166,11 @@ static void solve_mixed_dm(srmg_ctx *c, FieldBuf &u, FieldBuf &b, FieldBuf &e, F
         r0 = safe_r0(read_norms(c));
     } else {
         for (int l = 1; l < c->nlevels; l++) o64.fill_interp(u, l, level_args<double>(c, l - 1, s, op, 1.0));
-        o64.resnorm_partials(zero_field(c, SRMG_F64), b, s, op);
+        o64.bnorm_partials(b);  // resnorm(u = 0), from b alone
         r0 = safe_r0(read_norms(c));
-        if (o->guess) o64.first_iterate(u, b, o->guess, o->guess_smooth, r0);
+        // a first iterate is checked against r0 with the norm of the first
+        // FP64 defect below, which the solve needs anyway
+        if (o->guess) o64.first_iterate(u, b, o->guess, o->guess_smooth, r0, false);
     }
     auto defect_and_norm = [&] { o64.defect_and_norm(be.data.as<float>(), u, b, s, op); };
     const size_t ncell = (size_t)c->nb * LY::V, nib = e.ib.bytes / sizeof(float);
@@ -2210,7 +2237,12 @@ static void solve_mixed_dm(srmg_ctx *c, FieldBuf &u, FieldBuf &b, FieldBuf &e, F
         done = record(r0, r0, o->tol, 0, o->max_cycles, hist);
     } else {
         defect_and_norm();
-        done = record(read_norms(c), r0, o->tol, 0, o->max_cycles, hist);
+        std::vector<double> rg = read_norms(c);
+        if (o->guess && o64.fall_back(u, rg, r0)) {
+            defect_and_norm();
+            rg = read_norms(c);
+        }
+        done = record(rg, r0, o->tol, 0, o->max_cycles, hist);
     }
     int k = 0;
     Graph &gr = solve_graph(c, 1, {&u, &b, &e, &be, &re, &ue}, o);
