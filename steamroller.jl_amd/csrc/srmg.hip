@@ -2174,19 +2174,23 @@ static void solve_mixed_dm(srmg_ctx *c, FieldBuf &u, FieldBuf &b, FieldBuf &e, F
     }
     auto defect_and_norm = [&] { o64.defect_and_norm(be.data.as<float>(), u, b, s, op); };
     const size_t ncell = (size_t)c->nb * LY::V, nib = e.ib.bytes / sizeof(float);
-    if (fused)
-        hipLaunchKernelGGL(k_fill<float>, dim3(nwg(ncell)), dim3(WG), 0, c->stream, e.data.as<float>(), 0.0f, ncell);
-    // one outer step; first: the first step, which runs fmg! on the
-    // correction equation before its V-cycles (first_fmg) and inner_first
-    // V-cycles when that is set
-    const int n_inner = std::max(1, (int)o->inner_cycles), n_first = o->inner_first > 0 ? o->inner_first : n_inner;
-    const bool first_differs = o->first_fmg || n_first != n_inner || zero_add;
     // the first sweep of each correction solve on the finest level starts from
     // e = 0 there: with the fused add the sweep reads none of e (and the add
     // leaves e on that level as it is)
     const int Lf = c->nlevels - 1;
     const int rdim = op->geometry == SRMG_CYL1 ? 0 : (op->geometry == SRMG_CYL2 ? 1 : -1);
     const bool zin = fused && o32.zin_ok(Lf, rdim);
+    if (fused) {
+        // e = 0 (the finest level's only where fmg! reads it first)
+        const size_t nclr = (zin && !o->first_fmg) ? (size_t)c->lv[Lf].blk0 * LY::V : ncell;
+        if (nclr)
+            hipLaunchKernelGGL(k_fill<float>, dim3(nwg(nclr)), dim3(WG), 0, c->stream, e.data.as<float>(), 0.0f, nclr);
+    }
+    // one outer step; first: the first step, which runs fmg! on the
+    // correction equation before its V-cycles (first_fmg) and inner_first
+    // V-cycles when that is set
+    const int n_inner = std::max(1, (int)o->inner_cycles), n_first = o->inner_first > 0 ? o->inner_first : n_inner;
+    const bool first_differs = o->first_fmg || n_first != n_inner || zero_add;
     auto correct = [&](bool first) {
         if (!fused)
             hipLaunchKernelGGL(k_fill<float>, dim3(nwg(ncell)), dim3(WG), 0, c->stream, e.data.as<float>(), 0.0f,
