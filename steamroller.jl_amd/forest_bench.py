@@ -4,13 +4,15 @@ problems), solved to 1e-8 twice per form (FP64; FP32 with FP64 defect
 correction, two FP32 V-cycles per correction or two in the first and three
 in the later ones); the second solve of each form replays captured graphs.
 
-    python steamroller.jl_amd/forest_bench.py [--autotune]
+    python steamroller.jl_amd/forest_bench.py [--autotune] [--markers]
 
 Prints the host-timed milliseconds of the second solve of each form. Under
 rocprofv3 --kernel-trace the kernel durations of the same solves give the
 device busy time; the difference is what launching costs, i.e. the most a
 single persistent kernel per smoother step could save. --autotune picks the
-sweep tile shapes per level (FP64 and FP32) before the solves.
+sweep tile shapes per level (FP64 and FP32) before the solves; --markers puts
+roctx marks 'begin FORM I' / 'end FORM I' around every solve (for
+rocprofv3 --marker-trace and trace_summary.py --window).
 """
 
 import os
@@ -45,11 +47,20 @@ def main():
         print(f"tile shapes per level: fp64 {t64.tolist()} fp32 {t32.tolist()}", flush=True)
     print(f"forest: {len(probs)} problems, {sum(p.tree.nblocks() for p in probs)} blocks, "
           f"{sum(p.n_leaf for p in probs)} leaf cells", flush=True)
+    mark = None
+    if "--markers" in sys.argv[1:]:
+        sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+        from bench import roctx_marker
+        mark = roctx_marker()
+        if mark is None:
+            raise SystemExit("--markers: no roctx library found")
     for form in ("fp64", "mixed2", "mixed2_then3"):
         for rep in range(2):
             b.upload(bh)
             u.zero()
             ctx.synchronize()
+            if mark:
+                mark(f"begin {form} {rep}")
             t = time.perf_counter()
             if form == "fp64":
                 n, h = solver.solve_batched(u, b, r, u1, ss, geom, bc, discs, tol=1e-8, max_cycles=60,
@@ -61,6 +72,8 @@ def main():
                                                   guess=solver.GUESS_ZERO)
             ctx.synchronize()
             ms = 1e3 * (time.perf_counter() - t)
+            if mark:
+                mark(f"end {form} {rep}")
             print(f"{form} solve {rep + 1}: {n} cycles, {ms:.3f} ms, worst final {h[:, -1].max():.3e}", flush=True)
 
 
