@@ -432,6 +432,13 @@ int srmg_set_group_tiles(srmg_ctx *ctx, int enable);
  * axis length is at most 64; 0 the LDS-tiled axis kernels. All give the same
  * bits. */
 int srmg_set_axis_gemm(srmg_ctx *ctx, int variant);
+/* Order in which the column sweeps (tile shapes 8..11) of grouped 3D levels
+ * visit the sibling groups: 0 (default) the device block order (Morton); 1
+ * slabs two groups thick in z, row by row, so a group's x and y neighbours
+ * are read close in time (less HBM traffic, not faster on cfg4). Both give
+ * the same bits. No counterpart in the reference (a tuning switch like the
+ * two above). */
+int srmg_set_group_order(srmg_ctx *ctx, int order);
 
 /* Blocks, leaf blocks and coarse-fine ghost slabs of a level (1-based). */
 int srmg_level_info(srmg_ctx *ctx, int level, int64_t *nblocks, int64_t *nleaf, int64_t *nslots);

@@ -31,6 +31,9 @@ struct LevelArgs {
     // pp[3 pix[blk] + 0..2]; nullptr when the context holds one problem
     const T *pp;
     const int *pix;
+    // processing order of the level's sibling groups in the column sweeps:
+    // work group slot i takes group gord[i]; nullptr: group i
+    const int *gord;
 };
 
 template <typename T>
@@ -1797,7 +1800,9 @@ __global__ __launch_bounds__(512) void k_gs_sweep_colz(T *__restrict__ uo, const
     using CZ = ColSweepZ<D, M, T, RD>;
     using HT = Halo<D>;
     __shared__ typename CZ::Smem sm;
-    const int blk0 = a.blk0 + xcd_remap(blockIdx.x, gridDim.x) * CZ::NB;
+    int gi = xcd_remap(blockIdx.x, gridDim.x);
+    if (a.gord) gi = a.gord[gi];
+    const int blk0 = a.blk0 + gi * CZ::NB;
     const int tid = threadIdx.x;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
     int tv = 0;

@@ -162,6 +162,7 @@ def lib():
         "srmg_level_grouped": [P, i32, C.POINTER(i32)],
         "srmg_set_group_tiles": [P, i32],
         "srmg_set_axis_gemm": [P, i32],
+        "srmg_set_group_order": [P, i32],
         "srmg_electric_field": [P, i32, i32, dbl, P, P, P, i32],
         "srmg_freebc_create": [P, C.POINTER(P), i32, i32, P, dbl, P, i64],
         "srmg_freebc_destroy": [P],
@@ -382,6 +383,13 @@ class Context:
         1 one launch per axis transform with one wave per work group, 0 the
         LDS-tiled axis kernels (all give the same bits)."""
         _check(lib().srmg_set_axis_gemm(self.h, int(variant)))
+
+    def set_group_order(self, order: int):
+        """Order in which the column sweeps of grouped 3D levels visit the
+        sibling groups: 0 (default) the device block order; 1 slabs two
+        groups thick in z, row by row (x and y neighbours close in time, less
+        HBM traffic); same bits either way."""
+        _check(lib().srmg_set_group_order(self.h, int(order)))
 
     def set_sweep_config(self, dtype: int, level: int, cfg: int):
         _check(lib().srmg_set_sweep_config(self.h, dtype, level, cfg))

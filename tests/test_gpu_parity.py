@@ -443,6 +443,41 @@ def test_every_sweep_tile_shape_gives_identical_results(name, dtype):
         np.testing.assert_array_equal(o, outs[0])
 
 
+@pytest.mark.parametrize("name", ["cart3-m8", "cfg4-small"])
+def test_group_order_gives_identical_solves(name):
+    """The column sweeps visit sibling groups in slab order or in block order:
+    the same bits (FP64 and mixed solves, every sweep variant of shapes 8-11)."""
+    p = configs.make_cfg4(0, scale=0.35, levels=3) if name == "cfg4-small" else CASES[name]()
+    geom, disc = solver.geometry_from_code(p.geom), _disc(p)
+    for cfg in (8, 9, 10, 11):
+        res = []
+        for order in (0, 1):
+            ctx = solver.Context(p.tree, p.conn, p.M)
+            ctx.set_group_order(order)
+            for l in range(1, len(p.tree) + 1):
+                ctx.set_sweep_config(F64, l, cfg)
+                ctx.set_sweep_config(F32, l, cfg)
+            u, b, r, u1 = (ctx.field(F64) for _ in range(4))
+            b.upload(p.b)
+            n, h = solver.solve(u, b, r, u1, p.s, geom, p.bc, disc, tol=1e-9, max_cycles=12)
+            v, b2 = ctx.field(F64), ctx.field(F64)
+            b2.upload(p.b)
+            e, be, re, ue = (ctx.field(F32) for _ in range(4))
+            nm, hm = solver.solve_mixed(v, b2, e, be, re, ue, p.s, geom, p.bc, disc, tol=1e-9, max_cycles=12,
+                                        inner_cycles=3, inner_first=2)
+            outs = []
+            for f in (u, v):
+                host = p.b.like()
+                f.download(host)
+                outs.append(_valid(host, p).copy())
+            res.append((h, hm, outs))
+        print(f"{name} shape {cfg}: fp64 {len(res[0][0])} cycles, mixed {len(res[0][1])} steps", flush=True)
+        np.testing.assert_array_equal(res[1][0], res[0][0])
+        np.testing.assert_array_equal(res[1][1], res[0][1])
+        np.testing.assert_array_equal(res[1][2][0], res[0][2][0])
+        np.testing.assert_array_equal(res[1][2][1], res[0][2][1])
+
+
 @pytest.mark.parametrize("name", ["cfg1", "cart3-m8", "cfg4-small"])
 def test_sibling_group_kernels_give_identical_solves(name):
     p = configs.make_cfg4(0, scale=0.35, levels=3) if name == "cfg4-small" else CASES[name]()
