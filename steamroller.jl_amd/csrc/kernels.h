@@ -1827,24 +1827,45 @@ __device__ __forceinline__ void load_tile(T *tile, int *tab, int *blks, const T 
     using TL = Tile<D, M>;
     using HT = Halo<D>;
     constexpr int F = LY::F, V = LY::V, ND = 2 * D;
+    // each phase issues all of its loads before the first tile store (a loop
+    // that stores between its loads waits out each load in turn)
+    constexpr int PI = (BPW * V + NT - 1) / NT, PH = (BPW * ND * F + NT - 1) / NT;
     const int tid = threadIdx.x;
     if (tid < BPW) blks[tid] = (tid < nloc) ? (list ? list[first + tid] : first + tid) : 0;
     if (LDSB) lds_barrier(); else __syncthreads();
-    for (int i = tid; i < nloc * V; i += NT) {
-        int lb = i / V, q = i - lb * V;
-        int cx, cy, cz;
-        LY::decode(q, cx, cy, cz);
-        tile[lb * TL::SZ + TL::at(cx, cy, cz)] = x[(size_t)blks[lb] * V + q];
+    T xv[PI];
+#pragma unroll
+    for (int j = 0; j < PI; j++) {
+        const int i = tid + NT * j, lb = min(i / V, BPW - 1);
+        xv[j] = (i < nloc * V) ? x[(size_t)blks[lb] * V + (i - lb * V)] : T(0);
     }
     for (int i = tid; i < nloc * ND; i += NT) {
         int lb = i / ND;
         tab[i] = htab[(size_t)blks[lb] * HT::N + (i - lb * ND)];
     }
+#pragma unroll
+    for (int j = 0; j < PI; j++) {
+        const int i = tid + NT * j, lb = min(i / V, BPW - 1);
+        if (i < nloc * V) {
+            int cx, cy, cz;
+            LY::decode(i - lb * V, cx, cy, cz);
+            tile[lb * TL::SZ + TL::at(cx, cy, cz)] = xv[j];
+        }
+    }
     if (LDSB) lds_barrier(); else __syncthreads();
-    for (int i = tid; i < nloc * ND * F; i += NT) {
-        int lb = i / (ND * F), rem = i - lb * (ND * F);
-        int f = rem / F, t = rem - f * F;
-        halo_switch<D, M, T>(tile + lb * TL::SZ, tab[lb * ND + f], f, blks[lb], t, x, ib, sg);
+    T hv[PH];
+#pragma unroll
+    for (int j = 0; j < PH; j++) {
+        const int i = tid + NT * j, lb = min(i / (ND * F), BPW - 1), rem = i - lb * (ND * F);
+        const int f = min(rem / F, ND - 1), t = rem - f * F;
+        hv[j] = T(0);
+        if (i < nloc * ND * F) hv[j] = halo_value_sw<D, M, T>(f, tab[lb * ND + f], blks[lb], t, x, ib, sg);
+    }
+#pragma unroll
+    for (int j = 0; j < PH; j++) {
+        const int i = tid + NT * j, lb = min(i / (ND * F), BPW - 1), rem = i - lb * (ND * F);
+        const int f = min(rem / F, ND - 1), t = rem - f * F;
+        if (i < nloc * ND * F) tile[lb * TL::SZ + halo_at_sw<D, M, TL>(f, t)] = hv[j];
     }
     if (LDSB) lds_barrier(); else __syncthreads();
 }
@@ -2452,21 +2473,31 @@ __global__ __launch_bounds__(256) void k_postrestrict_group(T *__restrict__ bout
     using LY = Lay<D, M>;
     using TL = Tile<D, M>;
     constexpr int NB = 1 << D, V = LY::V;
+    constexpr int PER = (V + 255) / 256;
     __shared__ T tile[TL::SZ];
     __shared__ int tab[2 * D];
     __shared__ int blks[1];
-    load_tile<D, M, T, 1, 256>(tile, tab, blks, u, ib, htab, parent, fblk0 + (int)blockIdx.x * NB, 1, ac.sg);
-    const int pb = blks[0];
+    // the parent's r first: its loads stay in flight while the tile is staged
+    const int pb = parent[fblk0 + (int)blockIdx.x * NB];
+    T rv[PER];
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+        const int q = threadIdx.x + 256 * j;
+        rv[j] = q < V ? r[(size_t)pb * V + q] : T(0);
+    }
+    load_tile<D, M, T, 1, 256, true>(tile, tab, blks, u, ib, htab, nullptr, pb, 1, ac.sg);
     const BP<T> bp = bparams(ac, pb);
-    for (int q = threadIdx.x; q < V; q += 256) {
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+        const int q = threadIdx.x + 256 * j;
+        if (q >= V) break;
         int x, y, z;
         LY::decode(q, x, y, z);
         const int p = TL::at(x, y, z);
         T wm, wp;
         rweights<D, M, T, RD>(ac, coord, pb, RD == 0 ? x : y, wm, wp);
         T Lu = tstencil<D, M, T, RD>(tile, p, tile[p], bp.cc, wm, wp);
-        const size_t o = (size_t)pb * V + q;
-        bout[o] = -(r[o] + Lu / bp.s);
+        bout[(size_t)pb * V + q] = -(rv[j] + Lu / bp.s);
     }
 }
 
@@ -2607,15 +2638,23 @@ __global__ __launch_bounds__(NT) void k_bnorm(const T *__restrict__ b, const int
     const int nloc = min(BPW, n - first);
     if ((int)threadIdx.x < BPW) blks[threadIdx.x] = ((int)threadIdx.x < nloc) ? list[first + threadIdx.x] : 0;
     __syncthreads();
+    // every load of the thread's cells before the first store and sum
+    constexpr int PER = (BPW * V + NT - 1) / NT;
     double acc[BPW];
 #pragma unroll
     for (int lb = 0; lb < BPW; lb++) acc[lb] = 0.0;
-    for (int i = threadIdx.x; i < nloc * V; i += NT) {
-        const int lb = i / V, q = i - lb * V;
-        const size_t o = (size_t)blks[lb] * V + q;
-        const T bv = b[o];
-        if (be) be[o] = (float)bv;
-        const T rr = -bv;
+    T bv[PER];
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+        const int i = threadIdx.x + NT * j, lb = min(i / V, BPW - 1);
+        bv[j] = (i < nloc * V) ? b[(size_t)blks[lb] * V + (i - lb * V)] : T(0);
+    }
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+        const int i = threadIdx.x + NT * j, lb = min(i / V, BPW - 1);
+        if (i >= nloc * V) break;
+        if (be) be[(size_t)blks[lb] * V + (i - lb * V)] = (float)bv[j];
+        const T rr = -bv[j];
 #pragma unroll
         for (int k = 0; k < BPW; k++)
             if (k == lb) acc[k] += (double)rr * (double)rr;
@@ -3229,20 +3268,32 @@ __global__ __launch_bounds__(256) void k_add_restrict(double *__restrict__ u, fl
         }
         return;
     }
+    // every load of the thread's cells is issued before the first store
+    // (a loop that stores between its loads waits out each load in turn)
+    constexpr int PER = NB * V / 256;
+    static_assert(NB * V % 256 == 0, "group cells must be a multiple of the work-group size");
     __shared__ double ut[NB * V];
     __shared__ int leaf[NB];
     const int g0 = blk0 + (int)blockIdx.x * NB;
     if (tid < NB) leaf[tid] = isleaf[g0 + tid];
     __syncthreads();
-    for (int i = tid; i < NB * V; i += 256) {
-        const int k = i / V;
+    float ev[PER];
+    double uv[PER];
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+        const int i = tid + 256 * j;
         const size_t o = (size_t)g0 * V + i;
-        double v;
-        if (leaf[k]) {
-            v = (ZERO ? 0.0 : u[o]) + (double)e[o];
+        ev[j] = e[o];
+        uv[j] = (ZERO && leaf[i / V]) ? 0.0 : u[o];
+    }
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+        const int i = tid + 256 * j;
+        const size_t o = (size_t)g0 * V + i;
+        double v = uv[j];
+        if (leaf[i / V]) {
+            v = (ZERO ? 0.0 : uv[j]) + (double)ev[j];
             u[o] = v;
-        } else {
-            v = u[o];
         }
         if (!KEEPE) e[o] = 0.0f;
         ut[i] = v;
