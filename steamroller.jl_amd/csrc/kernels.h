@@ -1250,12 +1250,16 @@ __global__ __launch_bounds__(NT) void k_gs_sweep(T *__restrict__ uo, const T *__
     }
     __syncthreads();
 
-    // round trip 2: halo cells into the tile ...
-    for (int i = tid; i < NHALO; i += NT) {
-        int lb = i / (ND * F), rem = i - lb * (ND * F);
-        int f = rem / F, t = rem - f * F;
-        if (lb >= nloc) continue;
-        halo_switch<D, M, T>(tile + lb * TL::SZ, tab[lb * HT::N + f], f, blk0 + lb, t, ui, ib, a.sg);
+    // round trip 2: halo cells (stored into the tile once every load of the
+    // round trip is issued) ...
+    constexpr int PH = (NHALO + NT - 1) / NT;
+    T hv[PH];
+#pragma unroll
+    for (int j = 0; j < PH; j++) {
+        const int i = tid + NT * j, lb = min(i / (ND * F), BPW - 1), rem = i - lb * (ND * F);
+        const int f = min(rem / F, ND - 1), t = rem - f * F;
+        hv[j] = T(0);
+        if (i < NHALO && lb < nloc) hv[j] = halo_value_sw<D, M, T>(f, tab[lb * HT::N + f], blk0 + lb, t, ui, ib, a.sg);
     }
     // ... and the operands of the recomputed neighbour cells
     T rd[RPT], rb[RPT], rg[RPT][NS];
@@ -1277,6 +1281,12 @@ __global__ __launch_bounds__(NT) void k_gs_sweep(T *__restrict__ uo, const T *__
         if (dim == 0) RecFace<D, M, T, 0>::prefetch(dir, tq, e, gs, ui, b, ib, rd[k], rb[k], rg[k]);
         else if (dim == 1) RecFace<D, M, T, 1>::prefetch(dir, tq, e, gs, ui, b, ib, rd[k], rb[k], rg[k]);
         else RecFace<D, M, T, (D == 3 ? 2 : 1)>::prefetch(dir, tq, e, gs, ui, b, ib, rd[k], rb[k], rg[k]);
+    }
+#pragma unroll
+    for (int j = 0; j < PH; j++) {
+        const int i = tid + NT * j, lb = min(i / (ND * F), BPW - 1), rem = i - lb * (ND * F);
+        const int f = min(rem / F, ND - 1), t = rem - f * F;
+        if (i < NHALO && lb < nloc) tile[lb * TL::SZ + halo_at_sw<D, M, TL>(f, t)] = hv[j];
     }
     __syncthreads();
 
@@ -1387,12 +1397,17 @@ __global__ __launch_bounds__(NT) void k_gs_sweep_group(T *__restrict__ uo, const
     }
     __syncthreads();
 
-    // halo of the exterior faces: block lb, dimension d, face side = its bit d
-    for (int i = tid; i < NHALO; i += NT) {
-        const int lb = i / (D * F), rem = i - lb * (D * F);
-        const int d = rem / F, t = rem - d * F;
+    // halo of the exterior faces: block lb, dimension d, face side = its bit
+    // d (stored into the tile once the recompute operands' loads are issued)
+    constexpr int PH = (NHALO + NT - 1) / NT;
+    T hv[PH];
+#pragma unroll
+    for (int j = 0; j < PH; j++) {
+        const int i = tid + NT * j, lb = min(i / (D * F), NB - 1), rem = i - lb * (D * F);
+        const int d = min(rem / F, D - 1), t = rem - d * F;
         const int f = 2 * d + ((lb >> d) & 1);
-        halo_switch<D, M, T, TL>(tile + TL::origin(lb), tab[lb * HT::N + f], f, blk0 + lb, t, ui, ib, a.sg);
+        hv[j] = T(0);
+        if (i < NHALO) hv[j] = halo_value_sw<D, M, T>(f, tab[lb * HT::N + f], blk0 + lb, t, ui, ib, a.sg);
     }
     T rd[RPT], rb[RPT], rg[RPT][NS];
     int rj[RPT];
@@ -1413,6 +1428,13 @@ __global__ __launch_bounds__(NT) void k_gs_sweep_group(T *__restrict__ uo, const
         if (d == 0) RecFace<D, M, T, 0, TL>::prefetch(dir, tq, e, gs, ui, b, ib, rd[k], rb[k], rg[k]);
         else if (d == 1) RecFace<D, M, T, 1, TL>::prefetch(dir, tq, e, gs, ui, b, ib, rd[k], rb[k], rg[k]);
         else RecFace<D, M, T, (D == 3 ? 2 : 1), TL>::prefetch(dir, tq, e, gs, ui, b, ib, rd[k], rb[k], rg[k]);
+    }
+#pragma unroll
+    for (int j = 0; j < PH; j++) {
+        const int i = tid + NT * j, lb = min(i / (D * F), NB - 1), rem = i - lb * (D * F);
+        const int d = min(rem / F, D - 1), t = rem - d * F;
+        const int f = 2 * d + ((lb >> d) & 1);
+        if (i < NHALO) tile[TL::origin(lb) + halo_at_sw<D, M, TL>(f, t)] = hv[j];
     }
     __syncthreads();
 
@@ -1965,11 +1987,22 @@ __global__ __launch_bounds__(NT) void k_resid_tile(TO *__restrict__ out, const T
     const int base = (MODE == RES_NORM) ? first : a.blk0 + first;
     load_tile<D, M, T, BPW, NT>(tile, tab, blks, u, ib, htab, lst, base, nloc, a.sg);
     if ((int)threadIdx.x < nloc) prm[threadIdx.x] = bparams(a, blks[threadIdx.x]);
+    // the right-hand sides of the thread's cells, all in flight at once
+    constexpr int PER = (BPW * V + NT - 1) / NT;
+    T bv[PER];
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+        const int i = threadIdx.x + NT * j, lb = min(i / V, BPW - 1);
+        bv[j] = (i < nloc * V) ? b[(size_t)blks[lb] * V + (i - lb * V)] : T(0);
+    }
     __syncthreads();
     double acc[BPW];
 #pragma unroll
     for (int lb = 0; lb < BPW; lb++) acc[lb] = 0.0;
-    for (int i = threadIdx.x; i < nloc * V; i += NT) {
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+        const int i = threadIdx.x + NT * j;
+        if (i >= nloc * V) break;
         int lb = i / V, q = i - lb * V;
         int blk = blks[lb];
         int x, y, z;
@@ -1980,9 +2013,9 @@ __global__ __launch_bounds__(NT) void k_resid_tile(TO *__restrict__ out, const T
         const BP<T> bp = prm[lb];
         T Lu = tstencil<D, M, T, RD>(tile, p, tile[p], bp.cc, wm, wp);
         size_t o = (size_t)blk * V + q;
-        T rr = -(b[o] + Lu / bp.s);
+        T rr = -(bv[j] + Lu / bp.s);
         if (MODE == RES_WRITE) out[o] = (TO)rr;
-        if (MODE == RES_DEFECT) out[o] = (TO)(b[o] + Lu / bp.s);
+        if (MODE == RES_DEFECT) out[o] = (TO)(bv[j] + Lu / bp.s);
         if (MODE == RES_NORM || (MODE == RES_DEFECT && isleaf[blk])) {
 #pragma unroll
             for (int k = 0; k < BPW; k++)
@@ -2001,6 +2034,61 @@ __global__ __launch_bounds__(NT) void k_resid_tile(TO *__restrict__ out, const T
             __syncthreads();
         }
     }
+}
+
+// Stage sibling group g0 (2^D blocks from device block g0) of u into a
+// Tile<D, M, 2> with the halo of the group's exterior faces, its halo table
+// entries and block parameters; every load of a phase is issued before the
+// phase's first LDS store. Ends with a barrier (which also waits for loads
+// the caller issued before the call).
+template <int D, int M, typename T, int NT>
+__device__ __forceinline__ void stage_group(T *tile, int *tab, BP<T> *prm, const T *__restrict__ u,
+                                            const T *__restrict__ ib, const int *__restrict__ htab,
+                                            const LevelArgs<T> &a, int g0) {
+    using LY = Lay<D, M>;
+    using TL = Tile<D, M, 2>;
+    using HT = Halo<D>;
+    constexpr int NB = 1 << D, V = LY::V, F = LY::F;
+    constexpr int PU = (NB * V + NT - 1) / NT, PH = (NB * D * F + NT - 1) / NT;
+    const int tid = threadIdx.x;
+    T uv[PU];
+#pragma unroll
+    for (int j = 0; j < PU; j++) {
+        const int i = tid + NT * j;
+        uv[j] = (i < NB * V) ? u[(size_t)g0 * V + i] : T(0);
+    }
+    for (int i = tid; i < NB * 2 * D; i += NT) {
+        const int lb = i / (2 * D);
+        tab[i] = htab[(size_t)(g0 + lb) * HT::N + (i - lb * 2 * D)];
+    }
+    if (tid < NB) prm[tid] = bparams(a, g0 + tid);
+#pragma unroll
+    for (int j = 0; j < PU; j++) {
+        const int i = tid + NT * j, lb = min(i / V, NB - 1), q = i - lb * V;
+        if (i < NB * V) {
+            int x, y, z;
+            LY::decode(q, x, y, z);
+            tile[TL::origin(lb) + TL::at(x, y, z)] = uv[j];
+        }
+    }
+    __syncthreads();
+    T hv[PH];
+#pragma unroll
+    for (int j = 0; j < PH; j++) {
+        const int i = tid + NT * j, lb = min(i / (D * F), NB - 1), rem = i - lb * (D * F);
+        const int d = min(rem / F, D - 1), t = rem - d * F;
+        const int f = 2 * d + ((lb >> d) & 1);
+        hv[j] = T(0);
+        if (i < NB * D * F) hv[j] = halo_value_sw<D, M, T>(f, tab[lb * 2 * D + f], g0 + lb, t, u, ib, a.sg);
+    }
+#pragma unroll
+    for (int j = 0; j < PH; j++) {
+        const int i = tid + NT * j, lb = min(i / (D * F), NB - 1), rem = i - lb * (D * F);
+        const int d = min(rem / F, D - 1), t = rem - d * F;
+        const int f = 2 * d + ((lb >> d) & 1);
+        if (i < NB * D * F) tile[TL::origin(lb) + halo_at_sw<D, M, TL>(f, t)] = hv[j];
+    }
+    __syncthreads();
 }
 
 // k_resid_tile (RES_WRITE, RES_DEFECT) on a level of complete sibling groups:
@@ -2025,35 +2113,31 @@ __global__ __launch_bounds__(1024) void k_resid_group(TO *__restrict__ out, cons
     __shared__ int tab[NB * 2 * D];
     __shared__ BP<T> prm[NB];
     __shared__ double red[MODE == RES_DEFECT ? NT : 1];
+    constexpr int NR = NB / TEAMS, QR = (V + RT - 1) / RT;
     const int gl = xcd_remap(blockIdx.x, gridDim.x);
     const int g0 = a.blk0 + gl * NB;
     const int tid = threadIdx.x;
-    for (int i = tid; i < NB * V; i += NT) {
-        const int lb = i / V, q = i - lb * V;
-        int x, y, z;
-        LY::decode(q, x, y, z);
-        tile[TL::origin(lb) + TL::at(x, y, z)] = u[(size_t)(g0 + lb) * V + q];
-    }
-    for (int i = tid; i < NB * 2 * D; i += NT) {
-        const int lb = i / (2 * D);
-        tab[i] = htab[(size_t)(g0 + lb) * HT::N + (i - lb * 2 * D)];
-    }
-    if (tid < NB) prm[tid] = bparams(a, g0 + tid);
-    __syncthreads();
-    for (int i = tid; i < NB * D * F; i += NT) {
-        const int lb = i / (D * F), rem = i - lb * (D * F);
-        const int d = rem / F, t = rem - d * F;
-        const int f = 2 * d + ((lb >> d) & 1);
-        halo_switch<D, M, T, TL>(tile + TL::origin(lb), tab[lb * 2 * D + f], f, g0 + lb, t, u, ib, a.sg);
-    }
-    __syncthreads();
     const int team = tid / RT, t = tid - team * RT;
-    for (int round = 0; round < NB / TEAMS; round++) {
+    // the right-hand sides of the thread's cells, in flight during the staging
+    T bv[NR][QR];
+#pragma unroll
+    for (int round = 0; round < NR; round++)
+#pragma unroll
+        for (int h = 0; h < QR; h++) {
+            const int q = t + RT * h;
+            bv[round][h] = q < V ? b[(size_t)(g0 + team + round * TEAMS) * V + q] : T(0);
+        }
+    stage_group<D, M, T, NT>(tile, tab, prm, u, ib, htab, a, g0);
+#pragma unroll
+    for (int round = 0; round < NR; round++) {
         const int lb = team + round * TEAMS, blk = g0 + lb;
         const BP<T> bp = prm[lb];
         const bool leaf = (MODE == RES_DEFECT) && isleaf[blk];
         double acc = 0.0;
-        for (int q = t; q < V; q += RT) {
+#pragma unroll
+        for (int h = 0; h < QR; h++) {
+            const int q = t + RT * h;
+            if (q >= V) break;
             int x, y, z;
             LY::decode(q, x, y, z);
             const int p = TL::origin(lb) + TL::at(x, y, z);
@@ -2061,10 +2145,10 @@ __global__ __launch_bounds__(1024) void k_resid_group(TO *__restrict__ out, cons
             rweights<D, M, T, RD>(a, coord, blk, RD == 0 ? x : y, wm, wp);
             T Lu = tstencil<D, M, T, RD, TL>(tile, p, tile[p], bp.cc, wm, wp);
             const size_t o = (size_t)blk * V + q;
-            T rr = -(b[o] + Lu / bp.s);
+            T rr = -(bv[round][h] + Lu / bp.s);
             if (MODE == RES_WRITE) out[o] = (TO)rr;
             if (MODE == RES_DEFECT) {
-                out[o] = (TO)(b[o] + Lu / bp.s);
+                out[o] = (TO)(bv[round][h] + Lu / bp.s);
                 if (leaf) acc += (double)rr * (double)rr;
             }
         }
@@ -2117,39 +2201,35 @@ __global__ __launch_bounds__(1024) void k_resid_restrict(const T *__restrict__ u
     __shared__ T rt[NB * V];
     __shared__ int tab[NB * 2 * D];
     __shared__ BP<T> prm[NB];
+    constexpr int NR = NB / TEAMS, QR = (V + RT - 1) / RT;
     const int gl = xcd_remap(blockIdx.x, ngroups);
     const int g0 = a.blk0 + gl * NB;
-    for (int i = tid; i < NB * V; i += NT) {
-        const int lb = i / V, q = i - lb * V;
-        int x, y, z;
-        LY::decode(q, x, y, z);
-        tile[TL::origin(lb) + TL::at(x, y, z)] = u[(size_t)(g0 + lb) * V + q];
-    }
-    for (int i = tid; i < NB * 2 * D; i += NT) {
-        const int lb = i / (2 * D);
-        tab[i] = htab[(size_t)(g0 + lb) * HT::N + (i - lb * 2 * D)];
-    }
-    if (tid < NB) prm[tid] = bparams(a, g0 + tid);
-    __syncthreads();
-    for (int i = tid; i < NB * D * F; i += NT) {
-        const int lb = i / (D * F), rem = i - lb * (D * F);
-        const int d = rem / F, t = rem - d * F;
-        const int f = 2 * d + ((lb >> d) & 1);
-        halo_switch<D, M, T, TL>(tile + TL::origin(lb), tab[lb * 2 * D + f], f, g0 + lb, t, u, ib, a.sg);
-    }
-    __syncthreads();
     const int team = tid / RT, t = tid - team * RT;
-    for (int round = 0; round < NB / TEAMS; round++) {
+    // the right-hand sides of the thread's cells, in flight during the staging
+    T bv[NR][QR];
+#pragma unroll
+    for (int round = 0; round < NR; round++)
+#pragma unroll
+        for (int h = 0; h < QR; h++) {
+            const int q = t + RT * h;
+            bv[round][h] = q < V ? b[(size_t)(g0 + team + round * TEAMS) * V + q] : T(0);
+        }
+    stage_group<D, M, T, NT>(tile, tab, prm, u, ib, htab, a, g0);
+#pragma unroll
+    for (int round = 0; round < NR; round++) {
         const int lb = team + round * TEAMS, blk = g0 + lb;
         const BP<T> bp = prm[lb];
-        for (int q = t; q < V; q += RT) {
+#pragma unroll
+        for (int h = 0; h < QR; h++) {
+            const int q = t + RT * h;
+            if (q >= V) break;
             int x, y, z;
             LY::decode(q, x, y, z);
             const int p = TL::origin(lb) + TL::at(x, y, z);
             T wm, wp;
             rweights<D, M, T, RD>(a, coord, blk, RD == 0 ? x : y, wm, wp);
             T Lu = tstencil<D, M, T, RD, TL>(tile, p, tile[p], bp.cc, wm, wp);
-            rt[lb * V + q] = -(b[(size_t)blk * V + q] + Lu / bp.s);
+            rt[lb * V + q] = -(bv[round][h] + Lu / bp.s);
         }
     }
     __syncthreads();
