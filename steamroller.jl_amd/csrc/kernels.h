@@ -2360,8 +2360,7 @@ struct ColResid {
     using TL = Tile<D, M, 2>;
     static constexpr int NB = 8, V = LY::V, NG = 2 * M, ND = 2 * D;
     struct Smem {
-        T tile[TL::SZ];
-        T rt[NG * NG * NG];  // residual of group cell (X, Y, Z) at X + NG (Y + NG Z)
+        T tile[TL::SZ];  // u with its halo; then the residual at the group's cells
         int tab[NB * ND];
         BP<T> prm[NB];
     };
@@ -2376,33 +2375,48 @@ struct ColResid {
         const typename CT::Col c = CT::template column<P>(g0, w, lane);
         T ux[NG], bx[NG];
         CT::template stage<P>(sm.tile, sm.tab, sm.prm, c, u, b, ib, a, g0, w, lane, tv, bpl, ux, bx);
-        // residual r = -(b + L u / s)
+        // residual r = -(b + L u / s), kept in registers
         const BP<T> bq[2] = {sm.prm[c.kxy], sm.prm[c.kxy | 4]};
+        T rr[NG];
         static_for<NG>([&](auto zc) {
             constexpr int Z = decltype(zc)::value;
             const BP<T> &bp = bq[Z >> 3];
             const T v = CT::template lu<Z>(sm.tile, c, ux, bp.cc);
-            sm.rt[c.X + NG * (c.Y + NG * Z)] = -(bx[Z] + v / bp.s);
+            rr[Z] = -(bx[Z] + v / bp.s);
+        });
+        // the parent's cells, in storage order: coarse cell (X0, X1, X2) sums
+        // group cells (2 X0 + i0, 2 X1 + i1, 2 X2 + i2), q = i0 + 2 i1 + 4 i2;
+        // u's sums first, then the residual takes u's place in the tile
+        constexpr int PC = V / 256;
+        T fu[PC];
+#pragma unroll
+        for (int h = 0; h < PC; h++) {
+            int X0, X1, X2;
+            LY::decode(tid + 256 * h, X0, X1, X2);
+            fu[h] = T(0);
+#pragma unroll
+            for (int q = 0; q < LY::NC; q++)
+                fu[h] += sm.tile[TL::at(2 * X0 + (q & 1), 2 * X1 + ((q >> 1) & 1), 2 * X2 + ((q >> 2) & 1))];
+        }
+        lds_barrier();  // every read of u in the tile is done
+        static_for<NG>([&](auto zc) {
+            constexpr int Z = decltype(zc)::value;
+            sm.tile[TL::at(c.X, c.Y, Z)] = rr[Z];
         });
         lds_barrier();
-        // the parent's cells, in storage order: coarse cell (X0, X1, X2) sums
-        // group cells (2 X0 + i0, 2 X1 + i1, 2 X2 + i2), q = i0 + 2 i1 + 4 i2
         const int pb = parent[g0];
 #pragma unroll
-        for (int h = 0; h < V / 256; h++) {
+        for (int h = 0; h < PC; h++) {
             const int i = tid + 256 * h;
             int X0, X1, X2;
             LY::decode(i, X0, X1, X2);
-            T fr = T(0), fu = T(0);
+            T fr = T(0);
 #pragma unroll
-            for (int q = 0; q < LY::NC; q++) {
-                const int cx = 2 * X0 + (q & 1), cy = 2 * X1 + ((q >> 1) & 1), cz = 2 * X2 + ((q >> 2) & 1);
-                fr += sm.rt[cx + NG * (cy + NG * cz)];
-                fu += sm.tile[TL::at(cx, cy, cz)];
-            }
+            for (int q = 0; q < LY::NC; q++)
+                fr += sm.tile[TL::at(2 * X0 + (q & 1), 2 * X1 + ((q >> 1) & 1), 2 * X2 + ((q >> 2) & 1))];
             const size_t o = (size_t)pb * V + i;
             rc[o] = fr / T(LY::NC);
-            const T uv = fu / T(LY::NC);
+            const T uv = fu[h] / T(LY::NC);
             uc[o] = uv;
             if (u1c) u1c[o] = uv;
         }
@@ -2713,7 +2727,7 @@ __global__ __launch_bounds__(NT) void k_bnorm(const T *__restrict__ b, const int
                                               double wgt, double *__restrict__ partial, float *__restrict__ be) {
     constexpr int V = Lay<D, M>::V;
     __shared__ int blks[BPW];
-    __shared__ double red[NT];
+    __shared__ double red[BPW * NT];
     const int first = xcd_remap(blockIdx.x, gridDim.x) * BPW;
     const int nloc = min(BPW, n - first);
     if ((int)threadIdx.x < BPW) blks[threadIdx.x] = ((int)threadIdx.x < nloc) ? list[first + threadIdx.x] : 0;
@@ -2739,16 +2753,19 @@ __global__ __launch_bounds__(NT) void k_bnorm(const T *__restrict__ b, const int
         for (int k = 0; k < BPW; k++)
             if (k == lb) acc[k] += (double)rr * (double)rr;
     }
-    for (int lb = 0; lb < nloc; lb++) {
-        red[threadIdx.x] = acc[lb];
-        __syncthreads();
-        for (int w = NT / 2; w > 0; w >>= 1) {
-            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-            __syncthreads();
+    // the halving tree of every block at once (the same pairs as one block
+    // at a time; blocks past nloc are summed and dropped)
+#pragma unroll
+    for (int lb = 0; lb < BPW; lb++) red[lb * NT + threadIdx.x] = acc[lb];
+    __syncthreads();
+    for (int w = NT / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+#pragma unroll
+            for (int lb = 0; lb < BPW; lb++) red[lb * NT + threadIdx.x] += red[lb * NT + threadIdx.x + w];
         }
-        if (threadIdx.x == 0) partial[first + lb] = red[0] * wgt;
         __syncthreads();
     }
+    if ((int)threadIdx.x < nloc) partial[first + threadIdx.x] = red[threadIdx.x * NT] * wgt;
 }
 
 // ---- exact level-1 solve (FP64) ----

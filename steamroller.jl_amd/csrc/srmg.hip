@@ -1033,6 +1033,10 @@ struct Ops {
     // residual kernels from LDS tiles (base tile shape)
     static constexpr int RB = SMALL ? 256 / LY::H : 1;
     static constexpr int RNT = SMALL ? 256 : 512;
+    // blocks per work group of k_bnorm: its per-block partials are those of
+    // k_resid_tile<RB, RNT> for any count (each block's cells take the same
+    // thread slots of the same halving tree, up to exact additions of zero)
+    static constexpr int BB = LY::V <= 512 ? 8 : RB;
     void residual(FieldBuf &r, FieldBuf &u, FieldBuf &b, const LevelArgs<T> &a, int l) {
         if (!a.nblk) return;
         if constexpr (GROUP_OK) {
@@ -1410,8 +1414,8 @@ struct Ops {
             const LevelInfo &L = c->lv[l];
             if (!L.nleaf) continue;
             double wgt = std::ldexp(1.0, -c->D * l);
-            unsigned g = (unsigned)((L.nleaf + RB - 1) / RB);
-            hipLaunchKernelGGL((k_bnorm<D, M, T, RB, RNT>), dim3(g), dim3(RNT), 0, c->stream, X(b, l),
+            unsigned g = (unsigned)((L.nleaf + BB - 1) / BB);
+            hipLaunchKernelGGL((k_bnorm<D, M, T, BB, RNT>), dim3(g), dim3(RNT), 0, c->stream, X(b, l),
                                c->d_leaves.as<int>() + L.leaf0, L.nleaf, wgt, c->d_partial.as<double>() + np, be);
             np += L.nleaf;
         }
