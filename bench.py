@@ -100,9 +100,9 @@ def main():
     log(f"[bench] uploaded ({time.time() - t0:.1f}s)")
 
     t1 = time.time()
-    tuned64 = ctx.autotune(u, b, p.s, geom, p.bc, disc, reps=4)
+    tuned64 = ctx.autotune(u, b, p.s, geom, p.bc, disc, reps=8)
     e.zero(); be.upload(p.b)
-    tuned32 = ctx.autotune(e, be, p.s, geom, p.bc, disc, reps=4)
+    tuned32 = ctx.autotune(e, be, p.s, geom, p.bc, disc, reps=8)
     log(f"[bench] autotuned sweep tile shapes per level: fp64 {tuned64.tolist()} fp32 {tuned32.tolist()} "
         f"({time.time() - t1:.1f}s)")
 
@@ -320,10 +320,10 @@ def batched_cfg3(log, ntrees=64):
         # sweep tile shapes per level, outside the timed solves
         if mixed:
             be.upload(bh)
-            ctx.autotune(e, be, ss[0], geom, bc, discs[0], reps=4)
+            ctx.autotune(e, be, ss[0], geom, bc, discs[0], reps=8)
         else:
             b.upload(bh)
-            ctx.autotune(u, b, ss[0], geom, bc, discs[0], reps=4)
+            ctx.autotune(u, b, ss[0], geom, bc, discs[0], reps=8)
 
         def run_forest():
             b.upload(bh)

@@ -394,7 +394,9 @@ enum { SRMG_KERNEL_GS_PASS = 0, SRMG_KERNEL_RESIDUAL = 1, SRMG_KERNEL_GS_SWEEP =
        SRMG_KERNEL_RESIDUAL_GATHER = 4, SRMG_KERNEL_STREAM16 = 5 };
 
 /* Average duration of one launch of a level kernel, timed with HIP events on
- * the context's stream over `reps` back-to-back launches: one colour of the
+ * the context's stream over `reps` back-to-back launches (for the fused sweep
+ * and the stream kernels: captured in a graph and replayed, as solves run
+ * them, the faster of two replays): one colour of the
  * Gauss-Seidel sweep (alternating), the residual (LDS-tiled, or the per-cell
  * gather form), one fused red-black sweep, or the stream kernel r = u + b
  * over the level (bandwidth calibration; STREAM16: 16-byte loads and
@@ -404,7 +406,8 @@ int srmg_time_kernel(srmg_ctx *ctx, int kind, int level, int u, int b, int r, do
                      const srmg_op *op, int reps, double *ms_per_launch);
 
 /* Autotuner: times every compiled tile shape of the fused sweep on every
- * level with `reps` sweeps of u, keeps the fastest per level for later solves
+ * level with `reps` sweeps of u (captured in a graph and replayed, as solves
+ * run them), keeps the fastest per level for later solves
  * of fields of u's element type, and writes the choices to chosen[nlevels] if
  * non-NULL. u is restored. Shapes: 0..3 blocks per work group x work-group
  * size; 4, 5, 6 (3D M=8 and 2D M=16) one sibling group of 2^D blocks as one

@@ -41,9 +41,9 @@ def main():
     bh = solver.concat_fields([p.b for p in probs])
     if "--autotune" in sys.argv[1:]:
         b.upload(bh)
-        t64 = ctx.autotune(u, b, ss[0], geom, bc, discs[0], reps=4)
+        t64 = ctx.autotune(u, b, ss[0], geom, bc, discs[0], reps=8)
         be.upload(bh)
-        t32 = ctx.autotune(e, be, ss[0], geom, bc, discs[0], reps=4)
+        t32 = ctx.autotune(e, be, ss[0], geom, bc, discs[0], reps=8)
         print(f"tile shapes per level: fp64 {t64.tolist()} fp32 {t32.tolist()}", flush=True)
     print(f"forest: {len(probs)} problems, {sum(p.tree.nblocks() for p in probs)} blocks, "
           f"{sum(p.n_leaf for p in probs)} leaf cells", flush=True)
