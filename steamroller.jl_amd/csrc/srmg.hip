@@ -1070,9 +1070,8 @@ struct Ops {
             const int nleafc = u1 ? Lc.nleaf : 0;
             with_rd(a.rdim, [&](auto rd) {
                 constexpr int RD = decltype(rd)::value;
-                // FP32 3D 8^3 Cartesian: the column form (k_resid_restrict_col;
-                // in FP64 its 80 KB of LDS per group make it no faster)
-                if constexpr (D == 3 && M == 8 && RD < 0 && std::is_same<T, float>::value) {
+                // 3D 8^3 Cartesian: the column form (k_resid_restrict_col)
+                if constexpr (D == 3 && M == 8 && RD < 0) {
                     const unsigned ncp = (unsigned)(((int64_t)nleafc * LY::V + 255) / 256);
                     hipLaunchKernelGGL((k_resid_restrict_col<D, M, T, RD>), dim3(ng + ncp), dim3(256), 0, c->stream,
                                        X(u, l), X(b, l), I(u), c->d_halo.as<int>(), c->d_parent.as<int>(), a, ng,
