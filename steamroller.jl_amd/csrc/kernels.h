@@ -865,18 +865,33 @@ __global__ __launch_bounds__(WG) void k_fill_interp(T *__restrict__ ib, const T 
     fill_interp_one<D, M, T>(blockIdx.x * blockDim.x + threadIdx.x, ib, x, nbr, slots, slot0, nslot, a);
 }
 
+template <typename T> struct Vec16;
+template <> struct Vec16<float> { using V = float4; };
+template <> struct Vec16<double> { using V = double2; };
+
 // Two independent steps of the V-cycle ascent in one launch: the first
-// nwg_diff work groups form dst = a1 - a2 over n values (k_diff), the rest
-// refresh nslot coarse-fine ghost slabs (k_fill_interp). The slabs read
-// another level than the difference writes.
+// nwg_diff work groups form dst = a1 - a2 over n values (k_diff; 16 bytes
+// per lane, n a multiple of 16 / sizeof(T), the arrays 16-byte aligned; dst
+// may be a2), the rest refresh nslot coarse-fine ghost slabs
+// (k_fill_interp). The slabs read another level than the difference writes.
 template <int D, int M, typename T>
 __global__ __launch_bounds__(WG) void k_diff_fill(T *dst, const T *a1, const T *a2, size_t n, unsigned nwg_diff,
                                                   T *__restrict__ ib, const T *__restrict__ x,
                                                   const int *__restrict__ nbr, const int4 *__restrict__ slots,
                                                   int slot0, int nslot, LevelArgs<T> a) {
     if (blockIdx.x < nwg_diff) {
-        const size_t i = (size_t)blockIdx.x * WG + threadIdx.x;
-        if (i < n) dst[i] = a1[i] - a2[i];
+        using V = typename Vec16<T>::V;
+        constexpr int K = sizeof(V) / sizeof(T);
+        const size_t i = ((size_t)blockIdx.x * WG + threadIdx.x) * K;
+        if (i < n) {
+            V p = *reinterpret_cast<const V *>(a1 + i);
+            const V q = *reinterpret_cast<const V *>(a2 + i);
+            T *pv = reinterpret_cast<T *>(&p);
+            const T *qv = reinterpret_cast<const T *>(&q);
+#pragma unroll
+            for (int k = 0; k < K; k++) pv[k] = pv[k] - qv[k];
+            *reinterpret_cast<V *>(dst + i) = p;
+        }
     } else {
         fill_interp_one<D, M, T>((blockIdx.x - nwg_diff) * WG + threadIdx.x, ib, x, nbr, slots, slot0, nslot, a);
     }

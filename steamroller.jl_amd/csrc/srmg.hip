@@ -1343,7 +1343,8 @@ struct Ops {
         const size_t n = (size_t)L.nblk * LY::V;
         const int nslot = (acf && l >= 1) ? L.nslot : 0;
         if (!n && !nslot) return;
-        const unsigned nd = nwg(n), nf = nwg((int64_t)nslot * LY::F);
+        static_assert(LY::V % (16 / sizeof(T)) == 0, "k_diff_fill takes 16 bytes per lane");
+        const unsigned nd = nwg((int64_t)(n / (16 / sizeof(T)))), nf = nwg((int64_t)nslot * LY::F);
         const size_t o = (size_t)L.blk0 * LY::V;
         const LevelArgs<T> a = nslot ? *acf : LevelArgs<T>{};
         hipLaunchKernelGGL((k_diff_fill<D, M, T>), dim3(nd + nf), dim3(WG), 0, c->stream, X(u1, l) + o, X(u, l) + o,
