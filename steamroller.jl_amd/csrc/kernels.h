@@ -2274,10 +2274,11 @@ __global__ __launch_bounds__(1024) void k_resid_restrict(const T *__restrict__ u
 // k_resid_restrict for D = 3, M = 8 with the column mapping of k_gs_sweep_col:
 // 256 threads per sibling group, thread = (X, Y) column of the group with its
 // 16 cells and right-hand sides in registers (Z neighbours from registers).
-// The residual goes to an LDS copy of the group, from which (and from the u
-// tile) the parent's cells are restricted in k_restrict's order. Same
-// operations as k_resid_restrict, so the same bits. Work groups past ngroups
-// copy u to u1 on the coarse level's leaf blocks (256 cells each).
+// The residual stays in registers while the parent's u is restricted from
+// the u tile; then it takes u's place in the tile, from which the parent's
+// residual is restricted (both in k_restrict's order). Same operations as
+// k_resid_restrict, so the same bits. Work groups past ngroups copy u to u1
+// on the coarse level's leaf blocks (256 cells each).
 // Column staging shared by the column residual kernels (3D, 8^3, one sibling
 // group of 16^3 cells per 256-thread work group): thread (w, lane) owns the
 // (X, Y) column X + Y = w (mod 2) of rows 8 (w >> 1) .. 8 (w >> 1) + 7 (as in
