@@ -49,6 +49,9 @@ def parse():
     ap.add_argument("--no-regrid", action="store_true", help="skip the regrid measurement")
     ap.add_argument("--no-order4", action="store_true", help="skip the order-4 measurement")
     ap.add_argument("--forms", default="", help="comma-separated solve forms to time (fp64 and fp64_fmg always)")
+    ap.add_argument("--shapes", default="",
+                    help="fixed sweep tile shapes instead of the autotuner's: 'fp64 list;fp32 list', "
+                         "one shape per level, e.g. '0,0,9,8,9;0,0,9,8,9'")
     ap.add_argument("--markers", action="store_true",
                     help="roctx markers 'begin FORM I' / 'end FORM I' around every solve (for rocprofv3 --marker-trace)")
     return ap.parse_args()
@@ -100,11 +103,21 @@ def main():
     log(f"[bench] uploaded ({time.time() - t0:.1f}s)")
 
     t1 = time.time()
-    tuned64 = ctx.autotune(u, b, p.s, geom, p.bc, disc, reps=8)
-    e.zero(); be.upload(p.b)
-    tuned32 = ctx.autotune(e, be, p.s, geom, p.bc, disc, reps=8)
-    log(f"[bench] autotuned sweep tile shapes per level: fp64 {tuned64.tolist()} fp32 {tuned32.tolist()} "
-        f"({time.time() - t1:.1f}s)")
+    if args.shapes:
+        tuned64, tuned32 = (np.array([int(x) for x in part.split(",")], np.int32)
+                            for part in args.shapes.split(";"))
+        if len(tuned64) != len(p.tree) or len(tuned32) != len(p.tree):
+            raise SystemExit(f"--shapes needs {len(p.tree)} shapes per precision")
+        for lvl in range(1, len(p.tree) + 1):
+            ctx.set_sweep_config(F64, lvl, int(tuned64[lvl - 1]))
+            ctx.set_sweep_config(F32, lvl, int(tuned32[lvl - 1]))
+        log(f"[bench] fixed sweep tile shapes per level: fp64 {tuned64.tolist()} fp32 {tuned32.tolist()}")
+    else:
+        tuned64 = ctx.autotune(u, b, p.s, geom, p.bc, disc, reps=8)
+        e.zero(); be.upload(p.b)
+        tuned32 = ctx.autotune(e, be, p.s, geom, p.bc, disc, reps=8)
+        log(f"[bench] autotuned sweep tile shapes per level: fp64 {tuned64.tolist()} fp32 {tuned32.tolist()} "
+            f"({time.time() - t1:.1f}s)")
 
     def reset():
         # restore the right-hand side, start from u = 0 (outside the timed region)
