@@ -1995,7 +1995,7 @@ __global__ __launch_bounds__(NT) void k_resid_tile(TO *__restrict__ out, const T
     __shared__ int tab[BPW * 2 * D];
     __shared__ int blks[BPW];
     __shared__ BP<T> prm[BPW];
-    __shared__ double red[NT];
+    __shared__ double red[MODE == RES_WRITE ? 1 : BPW * NT];
     const int first = xcd_remap(blockIdx.x, gridDim.x) * BPW;
     const int nloc = min(BPW, n - first);
     const int *lst = (MODE == RES_NORM) ? list : nullptr;
@@ -2038,15 +2038,29 @@ __global__ __launch_bounds__(NT) void k_resid_tile(TO *__restrict__ out, const T
         }
     }
     if (MODE != RES_WRITE) {
-        for (int lb = 0; lb < nloc; lb++) {
-            red[threadIdx.x] = acc[lb];
-            __syncthreads();
-            for (int w = NT / 2; w > 0; w >>= 1) {
-                if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-                __syncthreads();
+        // each block's halving tree over the NT threads, all blocks at once:
+        // the levels of 64 and more across waves in LDS, the rest inside one
+        // wave per block by lane shuffles (the same pairs in the same order)
+        static_assert(NT % 64 == 0 && NT >= 128, "whole waves, at least two");
+#pragma unroll
+        for (int lb = 0; lb < BPW; lb++) red[lb * NT + threadIdx.x] = acc[lb];
+        __syncthreads();
+        for (int w = NT / 2; w >= 64; w >>= 1) {
+            if ((int)threadIdx.x < w) {
+#pragma unroll
+                for (int lb = 0; lb < BPW; lb++) red[lb * NT + threadIdx.x] += red[lb * NT + threadIdx.x + w];
             }
-            if (threadIdx.x == 0) partial[first + lb] = red[0] * wgt;
             __syncthreads();
+        }
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        for (int lb = wave; lb < BPW; lb += NT / 64) {
+            double v = red[lb * NT + lane];
+#pragma unroll
+            for (int w = 32; w > 0; w >>= 1) {
+                const double o = __shfl_down(v, w, 64);
+                if (lane < w) v += o;
+            }
+            if (lane == 0 && lb < nloc) partial[first + lb] = v * wgt;
         }
     }
 }
@@ -2168,13 +2182,23 @@ __global__ __launch_bounds__(1024) void k_resid_group(TO *__restrict__ out, cons
             }
         }
         if (MODE == RES_DEFECT) {
+            // the team's halving tree: levels of 64 and more in LDS, the
+            // rest inside the team's first wave by lane shuffles (same pairs)
             red[tid] = acc;
             __syncthreads();
-            for (int w = RT / 2; w > 0; w >>= 1) {
+            for (int w = RT / 2; w >= 64; w >>= 1) {
                 if (t < w) red[tid] += red[tid + w];
                 __syncthreads();
             }
-            if (t == 0) partial[(blk - a.blk0)] = red[tid] * wgt;
+            if (t < 64) {
+                double v = red[tid];
+#pragma unroll
+                for (int w = 32; w > 0; w >>= 1) {
+                    const double o = __shfl_down(v, w, 64);
+                    if (t < w) v += o;
+                }
+                if (t == 0) partial[(blk - a.blk0)] = v * wgt;
+            }
             __syncthreads();
         }
     }
@@ -2499,14 +2523,26 @@ struct ColDefect {
             sm.tile[k * V + tid] = acc;
         }
         lds_barrier();
-        for (int wd = 128; wd > 0; wd >>= 1) {
+        // the tree's levels 128 and 64 across waves in LDS; 32 .. 1 inside
+        // wave w for blocks w and w + 4 by lane shuffles (the same pairs)
+        for (int wd = 128; wd >= 64; wd >>= 1) {
             for (int i = tid; i < NB * wd; i += 256) {
                 const int k = i / wd, t = i - k * wd;
                 sm.tile[k * V + t] += sm.tile[k * V + t + wd];
             }
             lds_barrier();
         }
-        if (tid < NB) partial[g0 + tid - a.blk0] = sm.tile[tid * V] * wgt;
+#pragma unroll
+        for (int j = 0; j < NB / 4; j++) {
+            const int k = w + 4 * j;
+            double v = sm.tile[k * V + lane];
+#pragma unroll
+            for (int wd = 32; wd > 0; wd >>= 1) {
+                const double o = __shfl_down(v, wd, 64);
+                if (lane < wd) v += o;
+            }
+            if (lane == 0) partial[g0 + k - a.blk0] = v * wgt;
+        }
     }
 };
 
@@ -2770,18 +2806,30 @@ __global__ __launch_bounds__(NT) void k_bnorm(const T *__restrict__ b, const int
             if (k == lb) acc[k] += (double)rr * (double)rr;
     }
     // the halving tree of every block at once (the same pairs as one block
-    // at a time; blocks past nloc are summed and dropped)
+    // at a time; blocks past nloc are summed and dropped): the levels of 64
+    // and more across waves in LDS, the rest inside one wave per block by
+    // lane shuffles
+    static_assert(NT % 64 == 0 && NT >= 128, "whole waves, at least two");
 #pragma unroll
     for (int lb = 0; lb < BPW; lb++) red[lb * NT + threadIdx.x] = acc[lb];
     __syncthreads();
-    for (int w = NT / 2; w > 0; w >>= 1) {
+    for (int w = NT / 2; w >= 64; w >>= 1) {
         if ((int)threadIdx.x < w) {
 #pragma unroll
             for (int lb = 0; lb < BPW; lb++) red[lb * NT + threadIdx.x] += red[lb * NT + threadIdx.x + w];
         }
         __syncthreads();
     }
-    if ((int)threadIdx.x < nloc) partial[first + threadIdx.x] = red[threadIdx.x * NT] * wgt;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int lb = wave; lb < BPW; lb += NT / 64) {
+        double v = red[lb * NT + lane];
+#pragma unroll
+        for (int w = 32; w > 0; w >>= 1) {
+            const double o = __shfl_down(v, w, 64);
+            if (lane < w) v += o;
+        }
+        if (lane == 0 && lb < nloc) partial[first + lb] = v * wgt;
+    }
 }
 
 // ---- exact level-1 solve (FP64) ----
