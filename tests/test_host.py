@@ -120,3 +120,16 @@ def test_ctypes_records_match_the_c_header_layout(cname, pyname, tmp_path):
     assert int(out["sizeof"]) == ctypes.sizeof(py)
     for f in fields:
         assert int(out[f]) == getattr(py, f).offset, f
+
+
+@pytest.mark.parametrize("name,arg", [("srmg_set_group_order", 1), ("srmg_set_axis_gemm", 2),
+                                      ("srmg_set_group_tiles", 1)])
+def test_tuning_switches_refuse_a_null_context(name, arg):
+    """The switches report a null context through the status code and
+    srmg_last_error without touching a device."""
+    lib = ctypes.CDLL(solver.LIB_PATH)
+    fn = getattr(lib, name)
+    fn.argtypes, fn.restype = [ctypes.c_void_p, ctypes.c_int], ctypes.c_int
+    lib.srmg_last_error.restype = ctypes.c_char_p
+    assert fn(None, arg) != 0
+    assert lib.srmg_last_error() == b"null context"
