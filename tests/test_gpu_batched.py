@@ -173,14 +173,16 @@ def _problems_3d():
     return out
 
 
-@pytest.mark.parametrize("shape", [7, 8])
+@pytest.mark.parametrize("shape,order", [(7, 0), (8, 0), (8, 1)])
 @pytest.mark.parametrize("mixed", [False, True])
-def test_forest_3d_column_kernels_equal_single_solves(mixed, shape):
-    # the column-mapped sweeps (shapes 7 and 8), residual/restriction, defect
-    # and norm kernels on forests: every tree gets its own bits
+def test_forest_3d_column_kernels_equal_single_solves(mixed, shape, order):
+    # the column-mapped sweeps (shapes 7 and 8; the latter also visiting the
+    # sibling groups in slab order, tree by tree), residual/restriction,
+    # defect and norm kernels on forests: every tree gets its own bits
     probs = _problems_3d()
     geom = solver.CartesianGeometry()
     ctx = solver.Context.forest([(p.tree, p.conn) for p in probs], 8)
+    ctx.set_group_order(order)
     nl = max(len(p.tree) for p in probs)
     grouped = [ctx.level_grouped(l) for l in range(1, nl + 1)]
     print(f"grouped levels {grouped}", flush=True)
