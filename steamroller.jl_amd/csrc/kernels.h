@@ -2121,13 +2121,13 @@ __device__ __forceinline__ void stage_group(T *tile, int *tab, BP<T> *prm, const
 }
 
 // k_resid_tile (RES_WRITE, RES_DEFECT) on a level of complete sibling groups:
-// one group of 2^D blocks per work group of 1024 threads, staged as one
+// one group of 2^D blocks per work group of NT threads, staged as one
 // (2M)^D tile with the halo of the group's exterior faces only. Teams of RT
 // threads take one block at a time and visit its cells in the order of
 // k_resid_tile<.., 1, RT, ..> (cells t, t + RT, ...; partials as the same
 // RT-wide tree sum), so outputs and partials are bitwise the same.
-template <int D, int M, typename T, typename TO, int RT, int MODE, int RD>
-__global__ __launch_bounds__(1024) void k_resid_group(TO *__restrict__ out, const T *__restrict__ u,
+template <int D, int M, typename T, typename TO, int RT, int MODE, int RD, int NT = 1024>
+__global__ __launch_bounds__(NT) void k_resid_group(TO *__restrict__ out, const T *__restrict__ u,
                                                       const T *__restrict__ b, const T *__restrict__ ib,
                                                       const int *__restrict__ htab, const int *__restrict__ coord,
                                                       const int *__restrict__ isleaf, LevelArgs<T> a, double wgt,
@@ -2135,7 +2135,7 @@ __global__ __launch_bounds__(1024) void k_resid_group(TO *__restrict__ out, cons
     using LY = Lay<D, M>;
     using TL = Tile<D, M, 2>;
     using HT = Halo<D>;
-    constexpr int NT = 1024, NB = 1 << D, V = LY::V, F = LY::F;
+    constexpr int NB = 1 << D, V = LY::V, F = LY::F;
     constexpr int TEAMS = NT / RT;
     static_assert(NB % TEAMS == 0 && RT >= 64, "team layout");
     __shared__ T tile[TL::SZ];
@@ -2212,8 +2212,8 @@ __global__ __launch_bounds__(1024) void k_resid_group(TO *__restrict__ out, cons
 // groups past ngroups copy u -> u1 on the coarse level's leaf blocks (lists
 // leaves[0..nleafc)), completing copyto!(u1, u) on that level. Same values
 // and operations as residual + restrict(u) + copy + restrict(r).
-template <int D, int M, typename T, int RT, int RD>
-__global__ __launch_bounds__(1024) void k_resid_restrict(const T *__restrict__ u, const T *__restrict__ b,
+template <int D, int M, typename T, int RT, int RD, int NT = 1024>
+__global__ __launch_bounds__(NT) void k_resid_restrict(const T *__restrict__ u, const T *__restrict__ b,
                                                          const T *__restrict__ ib, const int *__restrict__ htab,
                                                          const int *__restrict__ coord,
                                                          const int *__restrict__ parent, LevelArgs<T> a,
@@ -2223,7 +2223,7 @@ __global__ __launch_bounds__(1024) void k_resid_restrict(const T *__restrict__ u
     using LY = Lay<D, M>;
     using TL = Tile<D, M, 2>;
     using HT = Halo<D>;
-    constexpr int NT = 1024, NB = 1 << D, V = LY::V, F = LY::F, MH = LY::MH;
+    constexpr int NB = 1 << D, V = LY::V, F = LY::F, MH = LY::MH;
     constexpr int TEAMS = NT / RT;
     static_assert(NB % TEAMS == 0 && RT >= 64, "team layout");
     const int tid = threadIdx.x;

@@ -1037,14 +1037,19 @@ struct Ops {
     // k_resid_tile<RB, RNT> for any count (each block's cells take the same
     // thread slots of the same halving tree, up to exact additions of zero)
     static constexpr int BB = LY::V <= 512 ? 8 : RB;
+    // threads per work group of the sibling-group residual kernels: a 2D
+    // group (1,024 cells) on 256 threads, eight groups resident per CU (on
+    // 1,024 threads, one cell each, only two fit and the small work per
+    // group left the launch latency-bound)
+    static constexpr int GNT = D == 2 ? 256 : 1024;
     void residual(FieldBuf &r, FieldBuf &u, FieldBuf &b, const LevelArgs<T> &a, int l) {
         if (!a.nblk) return;
         if constexpr (GROUP_OK) {
             if (grouped(l)) {
                 with_rd(a.rdim, [&](auto rd) {
                     constexpr int RD = decltype(rd)::value;
-                    hipLaunchKernelGGL((k_resid_group<D, M, T, T, RNT, RES_WRITE, RD>), dim3((unsigned)(a.nblk >> D)),
-                                       dim3(1024), 0, c->stream, X(r, l), X(u, l), X(b, l), I(u),
+                    hipLaunchKernelGGL((k_resid_group<D, M, T, T, RNT, RES_WRITE, RD, GNT>),
+                                       dim3((unsigned)(a.nblk >> D)), dim3(GNT), 0, c->stream, X(r, l), X(u, l), X(b, l), I(u),
                                        c->d_halo.as<int>(), coord(), (const int *)nullptr, a, 0.0,
                                        (double *)nullptr);
                 });
@@ -1078,8 +1083,8 @@ struct Ops {
                                        X(r, l - 1), X(u, l - 1), u1 ? X(*u1, l - 1) : (T *)nullptr, X(u, l - 1),
                                        c->d_leaves.as<int>() + Lc.leaf0, nleafc);
                 } else {
-                    const unsigned ncp = (unsigned)(((int64_t)nleafc * LY::V + 1023) / 1024);
-                    hipLaunchKernelGGL((k_resid_restrict<D, M, T, RNT, RD>), dim3(ng + ncp), dim3(1024), 0,
+                    const unsigned ncp = (unsigned)(((int64_t)nleafc * LY::V + GNT - 1) / GNT);
+                    hipLaunchKernelGGL((k_resid_restrict<D, M, T, RNT, RD, GNT>), dim3(ng + ncp), dim3(GNT), 0,
                                        c->stream, X(u, l), X(b, l), I(u), c->d_halo.as<int>(), coord(),
                                        c->d_parent.as<int>(), a, ng, X(r, l - 1), X(u, l - 1),
                                        u1 ? X(*u1, l - 1) : (T *)nullptr, X(u, l - 1),
@@ -1450,8 +1455,8 @@ struct Ops {
                                                0, c->stream, be, X(u, l), X(b, l), I(u), c->d_halo.as<int>(),
                                                c->d_isleaf.as<int>(), a, wgt, c->d_partial.as<double>() + np);
                         else
-                            hipLaunchKernelGGL((k_resid_group<D, M, T, float, RNT, RES_DEFECT, RD>),
-                                               dim3((unsigned)(a.nblk >> D)), dim3(1024), 0, c->stream, be,
+                            hipLaunchKernelGGL((k_resid_group<D, M, T, float, RNT, RES_DEFECT, RD, GNT>),
+                                               dim3((unsigned)(a.nblk >> D)), dim3(GNT), 0, c->stream, be,
                                                X(u, l), X(b, l), I(u), c->d_halo.as<int>(), coord(),
                                                c->d_isleaf.as<int>(), a, wgt, c->d_partial.as<double>() + np);
                         return;
