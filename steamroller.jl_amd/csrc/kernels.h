@@ -897,14 +897,28 @@ __global__ __launch_bounds__(WG) void k_diff_fill(T *dst, const T *a1, const T *
     }
 }
 
+// acc + p[i0] + p[i0 + WG] + ... over i < end, added in that order, with
+// four loads in flight at a time
+__device__ __forceinline__ double strided_sum(const double *__restrict__ p, int i0, int end, double acc) {
+    int i = i0;
+    for (; i + 3 * WG < end; i += 4 * WG) {
+        const double p0 = p[i], p1 = p[i + WG], p2 = p[i + 2 * WG], p3 = p[i + 3 * WG];
+        acc += p0;
+        acc += p1;
+        acc += p2;
+        acc += p3;
+    }
+    for (; i < end; i += WG) acc += p[i];
+    return acc;
+}
+
 // First stage of a fixed-order sum: work group g sums the contiguous chunk
 // [g * chunk, (g + 1) * chunk) of the n partials.
 __global__ __launch_bounds__(WG) void k_sum_chunks(const double *__restrict__ partial, int n, int chunk,
                                                    double *__restrict__ out) {
     __shared__ double red[WG];
     int lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
-    double acc = 0.0;
-    for (int i = lo + threadIdx.x; i < hi; i += WG) acc += partial[i];
+    const double acc = strided_sum(partial, lo + (int)threadIdx.x, hi, 0.0);
     red[threadIdx.x] = acc;
     __syncthreads();
     for (int w = WG / 2; w > 0; w >>= 1) {
@@ -922,8 +936,8 @@ __global__ __launch_bounds__(WG) void k_sum_ranges(const double *__restrict__ pa
     __shared__ double red[WG];
     double acc = 0.0;
     for (int k = 0; k < nr; k++) {
-        int2 r = rng[blockIdx.x * nr + k];
-        for (int i = r.x + threadIdx.x; i < r.x + r.y; i += WG) acc += partial[i];
+        const int2 r = rng[blockIdx.x * nr + k];
+        acc = strided_sum(partial, r.x + (int)threadIdx.x, r.x + r.y, acc);
     }
     red[threadIdx.x] = acc;
     __syncthreads();
