@@ -2134,7 +2134,9 @@ __device__ __forceinline__ void stage_group(T *tile, int *tab, BP<T> *prm, const
     __syncthreads();
 }
 
-// k_resid_tile (RES_WRITE, RES_DEFECT) on a level of complete sibling groups:
+// k_resid_tile (RES_WRITE, RES_DEFECT; RES_DEFECT with out == nullptr forms
+// only the partials, on a level of leaves in block order those of
+// k_resid_tile<RES_NORM>) on a level of complete sibling groups:
 // one group of 2^D blocks per work group of NT threads, staged as one
 // (2M)^D tile with the halo of the group's exterior faces only. Teams of RT
 // threads take one block at a time and visit its cells in the order of
@@ -2191,7 +2193,7 @@ __global__ __launch_bounds__(NT) void k_resid_group(TO *__restrict__ out, const 
             T rr = -(bv[round][h] + Lu / bp.s);
             if (MODE == RES_WRITE) out[o] = (TO)rr;
             if (MODE == RES_DEFECT) {
-                out[o] = (TO)(bv[round][h] + Lu / bp.s);
+                if (out != nullptr) out[o] = (TO)(bv[round][h] + Lu / bp.s);
                 if (leaf) acc += (double)rr * (double)rr;
             }
         }

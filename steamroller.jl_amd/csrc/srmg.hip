@@ -1401,6 +1401,17 @@ struct Ops {
                         return;
                     }
                 }
+                // grouped 2D level of leaves in block order: the sibling-group
+                // form, partials only (the same per-block sums)
+                if constexpr (GROUP_OK && D == 2) {
+                    if (grouped(l) && L.all_leaves) {
+                        hipLaunchKernelGGL((k_resid_group<D, M, T, T, RNT, RES_DEFECT, RD, GNT>),
+                                           dim3((unsigned)(L.nblk >> D)), dim3(GNT), 0, c->stream, (T *)nullptr,
+                                           X(u, l), X(b, l), I(u), c->d_halo.as<int>(), coord(),
+                                           c->d_isleaf.as<int>(), a, wgt, c->d_partial.as<double>() + np);
+                        return;
+                    }
+                }
                 hipLaunchKernelGGL((k_resid_tile<D, M, T, T, RB, RNT, RES_NORM, RD>), dim3(g), dim3(RNT), 0,
                                    c->stream, (T *)nullptr, X(u, l), X(b, l), I(u), c->d_halo.as<int>(), coord(),
                                    c->d_leaves.as<int>() + L.leaf0, (const int *)nullptr, L.nleaf, a, wgt,
