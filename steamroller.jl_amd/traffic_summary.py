@@ -3,10 +3,14 @@ WRITE_SIZE; KiB per dispatch) into HBM bytes per launch, calibrated by the
 stream kernel r = u + b of the same run (known bytes: reads 2 w N, writes w N).
 
     python steamroller.jl_amd/traffic_summary.py FETCH.csv WRITE.csv N_CELLS OUT.json [FETCH2.csv WRITE2.csv ...]
+    python steamroller.jl_amd/traffic_summary.py --all FETCH.csv WRITE.csv N_CELLS OUT.json
 
 Writes OUT.json (per kernel: raw counters, calibration factors, traffic bytes)
 and prints a text table. Further (FETCH, WRITE) pairs of runs of other tile
 shapes add their sweep kernels, each pair calibrated by its own stream kernel.
+--all reports every kernel of the first pair by name and grid size (work
+groups), calibrated the same way, as HBM bytes read and written per launch
+(for a solve profiled under the counters: bench.py runs the stream kernel).
 """
 
 import csv
@@ -25,12 +29,37 @@ def per_kernel(path, counter):
     return {k: sum(v) / len(v) for k, v in vals.items()}
 
 
+def per_launch_shape(path, counter):
+    """Average counter per dispatch by (kernel, work groups)."""
+    vals = defaultdict(list)
+    for row in csv.DictReader(open(path)):
+        if row["Counter_Name"] != counter:
+            continue
+        name = row["Kernel_Name"].split("(")[0].replace("void ", "").replace("srmg::", "")
+        wg = int(row["Grid_Size"]) // max(1, int(row["Workgroup_Size"]))
+        vals[(name, wg)].append(float(row["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in vals.items()}
+
+
 def main():
-    ncells, out = int(sys.argv[3]), sys.argv[4]
-    pairs = [(sys.argv[1], sys.argv[2])] + list(zip(sys.argv[5::2], sys.argv[6::2]))
+    args = sys.argv[1:]
+    every = "--all" in args
+    if every:
+        args.remove("--all")
+    ncells, out = int(args[2]), args[3]
+    pairs = [(args[0], args[1])] + list(zip(args[4::2], args[5::2]))
     res = {"calibration": None, "kernels": {}}
     for fetch_csv, write_csv in pairs:
         add_pair(res, fetch_csv, write_csv, ncells)
+    if every:
+        cal = res["calibration"]
+        fetch, write = per_launch_shape(args[0], "FETCH_SIZE"), per_launch_shape(args[1], "WRITE_SIZE")
+        rows = []
+        for k in sorted(set(fetch) & set(write), key=lambda k: -(fetch[k] + write[k])):
+            rd, wr = fetch[k] * 1024 * cal["fetch_factor"], write[k] * 1024 * cal["write_factor"]
+            rows.append({"kernel": k[0], "work_groups": k[1], "read_bytes": rd, "write_bytes": wr})
+            print(f"{k[0]:50s} {k[1]:7d} groups  read {rd / 1e6:9.2f} MB  write {wr / 1e6:9.2f} MB")
+        res["all_kernels"] = rows
     with open(out, "w") as f:
         json.dump(res, f, indent=1)
 

@@ -77,3 +77,30 @@ def test_trace_summary_window_sums_the_dispatches_between_the_marks(tmp_path):
     # rows by total time: k_b (3 us, 4 work groups), then k_a<1> (1 us, 2)
     assert lines[3] == "| k_b | 4 | 1 | 3.0 | 3.0 | 75.0 |"
     assert lines[4] == "| k_a<1> | 2 | 1 | 1.0 | 1.0 | 25.0 |"
+
+
+def test_traffic_summary_all_reports_every_kernel_by_grid_size(tmp_path):
+    n = 1 << 16
+    rows = [("k_triad<double>", 8 * n / 1024, 4096), ("k_add<3>", 100.0, 256 * 5498), ("k_add<3>", 300.0, 256 * 5498),
+            ("k_add<3>", 10.0, 256 * 1568)]
+    for name, counter, scale in (("f.csv", "FETCH_SIZE", 1.0), ("w.csv", "WRITE_SIZE", 0.5)):
+        with open(tmp_path / name, "w", newline="") as f:
+            w = csv.DictWriter(f, fieldnames=["Counter_Name", "Kernel_Name", "Counter_Value", "Grid_Size",
+                                              "Workgroup_Size"])
+            w.writeheader()
+            for k, v, g in rows:
+                val = v if k.startswith("k_triad") and counter == "FETCH_SIZE" else v * scale
+                if k.startswith("k_triad") and counter == "WRITE_SIZE":
+                    val = 8 * n / 1024
+                w.writerow({"Counter_Name": counter, "Kernel_Name": f"void srmg::{k}(double*)",
+                            "Counter_Value": val, "Grid_Size": g, "Workgroup_Size": 256})
+    out = tmp_path / "a.json"
+    r = subprocess.run([sys.executable, os.path.join(TOOLS, "traffic_summary.py"), "--all", str(tmp_path / "f.csv"),
+                        str(tmp_path / "w.csv"), str(n), str(out)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    d = json.load(open(out))
+    got = {(k["kernel"], k["work_groups"]): k for k in d["all_kernels"]}
+    # two dispatches on 5498 groups average; reads scaled by 2 (half reported)
+    assert got[("k_add<3>", 5498)]["read_bytes"] == 200.0 * 1024 * 2
+    assert got[("k_add<3>", 5498)]["write_bytes"] == 100.0 * 1024
+    assert got[("k_add<3>", 1568)]["read_bytes"] == 10.0 * 1024 * 2
